@@ -1,0 +1,118 @@
+// C entry points of the oracle for ctypes (tests/, smoke, bench cpu_baseline).
+// Test infrastructure only.
+#include <chrono>
+#include <cstring>
+#include <exception>
+#include <string>
+
+#include "oracle.hpp"
+
+using namespace oracle;
+
+namespace {
+thread_local std::string g_err;
+int fail(const std::exception& e) { g_err = e.what(); return -1; }
+}  // namespace
+
+extern "C" {
+
+const char* oracle_last_error() { return g_err.c_str(); }
+
+int oracle_validation(int center, const int* bc6, int kernel, int N, int lda, const char* kdir, double* err) {
+    try {
+        validation_case(center, bc6, kernel, N, lda, kdir ? kdir : ".", err);
+        return 0;
+    } catch (const std::exception& e) { return fail(e); }
+}
+
+// bc: [3][2][lda] ints; rhs/field: dense, dim 0 fastest, components slowest
+int oracle_solve(const int* nglob, const int* bc, int lda, const double* h, const double* L, int center, int odiff, int kernel,
+                 double alpha, const char* kdir, int type, const double* rhs, double* field) {
+    try {
+        Solver s(nglob, bc, lda, h, L, center == 0, (DiffType)odiff);
+        s.kernel_dir = kdir ? kdir : ".";
+        s.set_green((GreenType)kernel);
+        s.set_alpha(alpha);
+        s.setup();
+        s.solve(field, rhs, (SolverType)type);
+        return 0;
+    } catch (const std::exception& e) { return fail(e); }
+}
+
+// repeated solves on one set-up solver (for the CPU baseline timing); returns seconds of the last `reps` solves
+int oracle_solve_timed(const int* nglob, const int* bc, int lda, const double* h, const double* L, int center, int odiff,
+                       int kernel, double alpha, const char* kdir, int type, const double* rhs, double* field, int warm,
+                       int reps, double* seconds) {
+    try {
+        Solver s(nglob, bc, lda, h, L, center == 0, (DiffType)odiff);
+        s.kernel_dir = kdir ? kdir : ".";
+        s.set_green((GreenType)kernel);
+        s.set_alpha(alpha);
+        s.setup();
+        for (int i = 0; i < warm; ++i) s.solve(field, rhs, (SolverType)type);
+        const auto t0 = std::chrono::steady_clock::now();
+        for (int i = 0; i < reps; ++i) s.solve(field, rhs, (SolverType)type);
+        *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        return 0;
+    } catch (const std::exception& e) { return fail(e); }
+}
+
+// spectral-space data after the forward transforms (for stage-wise comparisons)
+int oracle_spectral_size(const int* nglob, const int* bc, int lda, const double* h, const double* L, int center, int* n_out,
+                         int* nf) {
+    try {
+        Solver s(nglob, bc, lda, h, L, center == 0, NOD);
+        Topo t = s.hat[s.ndim - 1];
+        if (s.fwd[s.ndim - 1].r2c) t.to_complex(s.fwd[s.ndim - 1].dim);
+        for (int d = 0; d < 3; ++d) n_out[d] = t.n[d];
+        *nf = t.nf;
+        return 0;
+    } catch (const std::exception& e) { return fail(e); }
+}
+
+int oracle_green(const int* nglob, const int* bc, const double* h, const double* L, int center, int kernel, double alpha,
+                 const char* kdir, double* green /* sized by oracle_spectral_size with lda = 1 */) {
+    try {
+        Solver s(nglob, bc, 1, h, L, center == 0, NOD);
+        s.kernel_dir = kdir ? kdir : ".";
+        s.set_green((GreenType)kernel);
+        s.set_alpha(alpha);
+        s.setup();
+        std::memcpy(green, s.green.data(), s.green.size() * sizeof(double));
+        return 0;
+    } catch (const std::exception& e) { return fail(e); }
+}
+
+int oracle_plan_info(const int* nglob, const int* bc, int lda, const double* h, const double* L, int center, double* normfact,
+                     double* volfact, double* kfact, double* koffset, double* symstart, int* order) {
+    try {
+        Solver s(nglob, bc, lda, h, L, center == 0, NOD);
+        *normfact = s.normfact;
+        *volfact = s.volfact;
+        s.spectral_info(kfact, koffset, symstart);
+        for (int i = 0; i < 3; ++i) order[i] = s.fwd[i].dim;
+        return s.ndim;
+    } catch (const std::exception& e) { return fail(e); }
+}
+
+int oracle_c2c(int n, int sign, const double* in, double* out) {
+    try { fft_c2c(n, sign, (const cplx*)in, (cplx*)out); return 0; } catch (const std::exception& e) { return fail(e); }
+}
+int oracle_r2c(int n, const double* in, double* out) {
+    try { fft_r2c(n, in, (cplx*)out); return 0; } catch (const std::exception& e) { return fail(e); }
+}
+int oracle_c2r(int n, const double* in, double* out) {
+    try { fft_c2r(n, (const cplx*)in, out); return 0; } catch (const std::exception& e) { return fail(e); }
+}
+int oracle_r2r(int n, int kind, const double* in, double* out) {
+    try { fft_r2r(n, (R2R)kind, in, out); return 0; } catch (const std::exception& e) { return fail(e); }
+}
+double oracle_lgf1d(int order, int n, double c) {
+    try { return lgf1d(order, n, c); } catch (const std::exception& e) { fail(e); return 0.0 / 0.0; }
+}
+double oracle_lgf_symbol(int order, double kh) { return lgf_symbol(order, kh); }
+double oracle_bessel_k0(double x) { return bessel_k0(x); }
+double oracle_bessel_k1(double x) { return bessel_k1(x); }
+double oracle_expint_e1(double x) { return expint_e1(x); }
+
+}  // extern "C"

@@ -1,0 +1,331 @@
+// Oracle Green's functions.
+//   drivers: src/green_functions.cpp:27-167 (3 unbounded), 192-391 (2), 407-538 (1), 560-708 (0)
+//   kernels: src/green_kernels.hpp (HEJ_2..10, CHAT_2, LGF_2..8)
+//   tables:  src/green_functions.hpp:30-92 (.ker files, 32^3 / 32^2 raw float64)
+// Special functions come from libstdc++ (std::cyl_bessel_k, std::expint), an
+// implementation independent of both the reference and the product.
+// HEJ_0 and the MEHR kernels are not restated here (parity unpinned for them).
+// Test infrastructure only.
+#include <cmath>
+#include <cstdio>
+#include <map>
+#include <mutex>
+#include <stdexcept>
+
+#include "oracle.hpp"
+
+namespace oracle {
+
+double lgf3_expansion(const std::string& kdir, int order, double n1, double n2, double n3);
+double lgf2_expansion(const std::string& kdir, int order, double n1, double n2);
+
+namespace {
+
+const double PI = M_PI;
+const double EULER = 0.57721566490153286061;
+
+double e1(double x) { return -std::expint(-x); }
+
+int lgf_order(GreenType t) {
+    switch (t) {
+        case LGF_2: return 2;
+        case LGF_4: return 4;
+        case LGF_6: return 6;
+        case LGF_8: return 8;
+        default: return 0;
+    }
+}
+
+std::mutex g_tab_mtx;
+std::map<std::string, std::vector<double>> g_tabs;
+
+const std::vector<double>& lgf_table(const std::string& kdir, int order, int dim) {
+    char name[512];
+    std::snprintf(name, sizeof(name), "%s/LGF_%d_%dd_32.ker", kdir.c_str(), order, dim);
+    std::lock_guard<std::mutex> lk(g_tab_mtx);
+    auto it = g_tabs.find(name);
+    if (it != g_tabs.end()) return it->second;
+    const size_t n = (dim == 3) ? 32 * 32 * 32 : 32 * 32;
+    std::vector<double> v(n);
+    FILE* f = std::fopen(name, "rb");
+    if (!f) throw std::runtime_error(std::string("unable to read ") + name);
+    if (std::fread(v.data(), sizeof(double), n, f) != n) { std::fclose(f); throw std::runtime_error("short .ker file"); }
+    std::fclose(f);
+    return g_tabs[name] = v;
+}
+
+// symmetrised index of green_functions (Topology.hpp:415-430): mirrored indices are negative
+inline int symid(int i, double ss) {
+    if (ss == 0.0 || i <= ss) return i;
+    return -std::max((int)std::fabs(2.0 * ss - i), 1);
+}
+
+inline double& gat(const Topo& t, double* g, int i0, int i1, int i2) {
+    return g[((size_t)i0 + (size_t)t.n[0] * ((size_t)i1 + (size_t)t.n[1] * i2)) * t.nf];
+}
+
+// polynomial parts of the HEJ kernels, green_kernels.hpp:26-54, 340-415, 461-566, 681-717
+double hej3_poly(GreenType t, double rho) {
+    const double r2 = rho * rho;
+    switch (t) {
+        case HEJ_2: return 0.0;
+        case HEJ_4: return rho;
+        case HEJ_6: return rho * (7.0 / 4.0 - r2 / 4.0);
+        case HEJ_8: return rho * (19.0 / 8.0 - (2.0 / 3.0) * r2 + r2 * r2 / 24.0);
+        case HEJ_10: return rho * (187.0 / 64.0 - (233.0 / 192.0) * r2 + (29.0 / 192.0) * r2 * r2 - r2 * r2 * r2 / 192.0);
+        default: throw std::runtime_error("not a HEJ kernel");
+    }
+}
+double hej3_g0(GreenType t, double eps) {
+    const double base = -M_SQRT2 / (eps * std::sqrt(PI * PI * PI));
+    switch (t) {
+        case HEJ_2: return base / 4.0;
+        case HEJ_4: return base * 3.0 / 8.0;
+        case HEJ_6: return base * 15.0 / 32.0;
+        case HEJ_8: return base * 35.0 / 64.0;
+        case HEJ_10: return base * 315.0 / 512.0;
+        default: throw std::runtime_error("not a HEJ kernel");
+    }
+}
+// k=0 (2 unbounded) exponential coefficient Q(rho^2) and the r=0 constant
+double hej2_q(GreenType t, double r2) {
+    switch (t) {
+        case HEJ_2: return 0.0;
+        case HEJ_4: return 0.5;
+        case HEJ_6: return 0.75 - r2 / 8.0;
+        case HEJ_8: return 11.0 / 12.0 - 7.0 * r2 / 24.0 + r2 * r2 / 48.0;
+        case HEJ_10: return 25.0 / 24.0 - 23.0 * r2 / 48.0 + 13.0 * r2 * r2 / 192.0 - r2 * r2 * r2 / 384.0;
+        default: throw std::runtime_error("not a HEJ kernel");
+    }
+}
+double hej2_q0(GreenType t) { return hej2_q(t, 0.0); }
+
+double hej_spectral(GreenType t, double ksq, double sig) {
+    const double s2 = ksq * sig * sig, h = 0.5 * s2;
+    double p = 1.0, term = 1.0;
+    int jmax = 0;
+    switch (t) {
+        case HEJ_2: jmax = 0; break;
+        case HEJ_4: jmax = 1; break;
+        case HEJ_6: jmax = 2; break;
+        case HEJ_8: jmax = 3; break;
+        case HEJ_10: jmax = 4; break;
+        default: throw std::runtime_error("not a HEJ kernel");
+    }
+    for (int j = 1; j <= jmax; ++j) { term *= h / j; p += term; }
+    return -p * std::exp(-h) / ksq;
+}
+
+double lgf_spectral(int order, const double k[3], double h) {
+    double s = 0;
+    for (int d = 0; d < 3; ++d) s += lgf_symbol(order, k[d] * h);
+    return -h * h / s;
+}
+
+}  // namespace
+
+void green_fill_3unb(const Topo& t, const double hfact[3], const double symstart[3], double* g, GreenType type,
+                     double length, const std::string& kdir) {
+    for (int d = 0; d < 3; ++d)
+        if (hfact[d] == 0.0) throw std::runtime_error("hfact cannot be 0");
+    const int ord = lgf_order(type);
+    const std::vector<double>* tab = nullptr;
+    if (ord) {
+        if (!(hfact[0] == hfact[1] && hfact[1] == hfact[2])) throw std::runtime_error("LGF needs an isotropic grid");
+        tab = &lgf_table(kdir, ord, 3);
+    } else if (!(type == CHAT_2 || (type >= HEJ_2 && type <= HEJ_10))) {
+        throw std::runtime_error("kernel not restated in the oracle");
+    }
+    for (int i2 = 0; i2 < t.n[2]; ++i2)
+        for (int i1 = 0; i1 < t.n[1]; ++i1)
+            for (int i0 = 0; i0 < t.n[0]; ++i0) {
+                const int is[3] = {symid(i0, symstart[0]), symid(i1, symstart[1]), symid(i2, symstart[2])};
+                const double x0 = is[0] * hfact[0], x1 = is[1] * hfact[1], x2 = is[2] * hfact[2];
+                const double r = std::sqrt(x0 * x0 + x1 * x1 + x2 * x2);
+                double v;
+                if (ord) {
+                    const int ax0 = t.axis, ax1 = (ax0 + 1) % 3, ax2 = (ax0 + 2) % 3;
+                    const int a = std::abs(is[ax0]), b = std::abs(is[ax1]), c = std::abs(is[ax2]);
+                    double gv;
+                    if (a < 32 && b < 32 && c < 32) gv = (*tab)[a + 32 * b + 32 * 32 * c];
+                    else gv = lgf3_expansion(kdir, ord, a, b, c);
+                    v = -gv / hfact[0];
+                } else if (type == CHAT_2) {
+                    v = -1.0 / (4.0 * PI * r);
+                } else {
+                    const double rho = r / length;
+                    v = -1.0 / (4.0 * PI * r) *
+                        (std::erf(rho / M_SQRT2) + hej3_poly(type, rho) * std::exp(-0.5 * rho * rho) / std::sqrt(2.0 * PI));
+                }
+                gat(t, g, i0, i1, i2) = v;
+            }
+    if (!ord) {
+        double g0;
+        if (type == CHAT_2) {
+            const double vol = hfact[0] * hfact[1] * hfact[2];
+            g0 = -0.5 * std::pow(1.5 / (2.0 * PI) * vol, 2.0 / 3.0) / vol;
+        } else {
+            g0 = hej3_g0(type, length);
+        }
+        g[0] = g0;
+    }
+}
+
+void green_fill_2unb(const Topo& t, const double hfact[3], const double kfact[3], const double koffset[3],
+                     const double symstart[3], double* g, GreenType type, double length, const std::string& kdir) {
+    const bool iso = (hfact[0] == 0.0 && hfact[1] == hfact[2]) || (hfact[1] == 0.0 && hfact[0] == hfact[2]) ||
+                     (hfact[2] == 0.0 && hfact[0] == hfact[1]);
+    const int ord = lgf_order(type);
+    const std::vector<double>* tab = nullptr;
+    if (ord) {
+        if (!iso) throw std::runtime_error("LGF needs an isotropic grid");
+        tab = &lgf_table(kdir, ord, 2);
+    } else if (!(type == CHAT_2 || (type >= HEJ_2 && type <= HEJ_10))) {
+        throw std::runtime_error("kernel not restated in the oracle");
+    }
+    const double sh = hfact[0] + hfact[1] + hfact[2];
+    const double sk = kfact[0] + kfact[1] + kfact[2];
+    const double req = std::sqrt(hfact[0] * hfact[1] + hfact[1] * hfact[2] + hfact[2] * hfact[0]) / std::sqrt(PI);
+    // the 2-D table lookup takes the first two nonzero |indices| in the Green topology's axis order
+    // (green_kernels.hpp:134-151)
+    for (int i2 = 0; i2 < t.n[2]; ++i2)
+        for (int i1 = 0; i1 < t.n[1]; ++i1)
+            for (int i0 = 0; i0 < t.n[0]; ++i0) {
+                const int is[3] = {symid(i0, symstart[0]), symid(i1, symstart[1]), symid(i2, symstart[2])};
+                const double k = (is[0] + koffset[0]) * kfact[0] + (is[1] + koffset[1]) * kfact[1] + (is[2] + koffset[2]) * kfact[2];
+                const double x0 = is[0] * hfact[0], x1 = is[1] * hfact[1], x2 = is[2] * hfact[2];
+                const double r = std::sqrt(x0 * x0 + x1 * x1 + x2 * x2);
+                double v;
+                const bool r0 = r <= sh * 0.2;
+                const bool k0 = !r0 && k <= sk * 0.2;
+                if (ord) {
+                    if (r0 || k0) {
+                        const int ax0 = t.axis, ax1 = (ax0 + 1) % 3, ax2 = (ax0 + 2) % 3;
+                        const int a[3] = {std::abs(is[ax0]), std::abs(is[ax1]), std::abs(is[ax2])};
+                        const int j1 = (a[0] == 0) ? a[1] : a[0];
+                        const int j2 = (a[0] == 0) ? a[2] : (a[1] == 0 ? a[2] : a[1]);
+                        double gv;
+                        if (j1 < 32 && j2 < 32) gv = (*tab)[j1 + 32 * j2];
+                        else gv = lgf2_expansion(kdir, ord, j1, j2);
+                        v = -gv;
+                    } else {
+                        v = 0.0;
+                    }
+                } else if (type == CHAT_2) {
+                    if (r0) {
+                        const double kr = k * req;
+                        v = -(1.0 - kr * std::cyl_bessel_k(1.0, kr)) / (PI * kr * kr);
+                    } else if (k0) {
+                        v = std::log(r) / (2.0 * PI);
+                    } else {
+                        v = -std::cyl_bessel_k(0.0, std::fabs(k) * r) / (2.0 * PI);
+                    }
+                } else {
+                    if (r0) {
+                        v = -(0.5 * EULER - std::log(M_SQRT2 * length) + hej2_q0(type)) / (2.0 * PI);
+                    } else if (k0) {
+                        const double rho2 = (r / length) * (r / length);
+                        v = (std::log(r) - hej2_q(type, rho2) * std::exp(-0.5 * rho2) + 0.5 * e1(0.5 * rho2)) / (2.0 * PI);
+                    } else {
+                        v = 0.0;
+                    }
+                }
+                gat(t, g, i0, i1, i2) = v;
+            }
+    if (type == CHAT_2) g[0] = 0.25 / (2.0 * PI) * (PI - 6.0 + 2.0 * std::log(0.5 * PI * req * req));
+}
+
+void green_fill_1unb(const Topo& t, const double hfact[3], const double kfact[3], const double koffset[3],
+                     const double symstart[3], double* g, GreenType type, double length) {
+    const int ord = lgf_order(type);
+    if (!ord && !(type == CHAT_2 || (type >= HEJ_2 && type <= HEJ_10)))
+        throw std::runtime_error("kernel not restated in the oracle");
+    const double sk = kfact[0] + kfact[1] + kfact[2];
+    const double h = hfact[0] + hfact[1] + hfact[2];
+    for (int i2 = 0; i2 < t.n[2]; ++i2)
+        for (int i1 = 0; i1 < t.n[1]; ++i1)
+            for (int i0 = 0; i0 < t.n[0]; ++i0) {
+                const int is[3] = {symid(i0, symstart[0]), symid(i1, symstart[1]), symid(i2, symstart[2])};
+                const double kk[3] = {(is[0] + koffset[0]) * kfact[0], (is[1] + koffset[1]) * kfact[1],
+                                      (is[2] + koffset[2]) * kfact[2]};
+                const double k = std::sqrt(kk[0] * kk[0] + kk[1] * kk[1] + kk[2] * kk[2]);
+                const double x0 = is[0] * hfact[0], x1 = is[1] * hfact[1], x2 = is[2] * hfact[2];
+                const double r = std::sqrt(x0 * x0 + x1 * x1 + x2 * x2);
+                const bool k0 = k <= sk * 0.2;
+                double v;
+                if (ord) {
+                    double c = 0;
+                    for (int d = 0; d < 3; ++d) c += lgf_symbol(ord, kk[d] * h);
+                    const int n = (int)std::round(std::fabs(r) / h);
+                    v = -h * lgf1d(ord, n, c);
+                } else if (type == CHAT_2) {
+                    v = k0 ? 0.5 * std::fabs(r) : -0.5 * std::exp(-k * r) / k;
+                } else {
+                    const double sig = length, rho = r / sig, rho2 = rho * rho;
+                    const double c = 1.0 / (M_SQRT2 * std::sqrt(PI));
+                    if (k0) {
+                        const double er = 0.5 * r * std::erf(r / (M_SQRT2 * sig));
+                        const double ex = std::exp(-0.5 * rho2);
+                        double tail;
+                        switch (type) {
+                            case HEJ_2: tail = (1.0 - ex); break;
+                            case HEJ_4: tail = 0.5 * (1.0 - ex); break;
+                            case HEJ_6: tail = 0.125 * (3.0 - ex * (rho2 + 3.0)); break;
+                            case HEJ_8: tail = 5.0 / 16.0 - ex * (-rho2 * rho2 / 48.0 + 0.25 * rho2 + 5.0 / 16.0); break;
+                            default:
+                                tail = 35.0 / 128.0 - ex * (rho2 * rho2 * rho2 / 384.0 - 23.0 / 384.0 * rho2 * rho2 +
+                                                            47.0 / 128.0 * rho2 + 35.0 / 128.0);
+                        }
+                        v = er - tail * sig * c;
+                    } else {
+                        const double s = k * sig, s2 = s * s;
+                        const double sub = (s * rho > 100.0)
+                                               ? 0.0
+                                               : ((1.0 - std::erf((s - rho) / M_SQRT2)) * std::exp(-s * rho) +
+                                                  (1.0 - std::erf((s + rho) / M_SQRT2)) * std::exp(s * rho));
+                        double poly;
+                        switch (type) {
+                            case HEJ_2: poly = 0.0; break;
+                            case HEJ_4: poly = 0.25; break;
+                            case HEJ_6: poly = 5.0 / 16.0 + (s2 - rho2) / 16.0; break;
+                            case HEJ_8:
+                                poly = 11.0 / 32.0 + s2 / 12.0 - 0.125 * rho2 - s2 * rho2 / 48.0 + (s2 * s2 + rho2 * rho2) / 96.0;
+                                break;
+                            default:
+                                poly = 93.0 / 256.0 + 73.0 / 768.0 * s2 - 47.0 / 256.0 * rho2 - 17.0 / 384.0 * s2 * rho2 +
+                                       11.0 / 768.0 * s2 * s2 + 23.0 / 768.0 * rho2 * rho2 +
+                                       (s2 * rho2 * rho2 - s2 * s2 * rho2) / 256.0 + (s2 * s2 * s2 - rho2 * rho2 * rho2) / 768.0;
+                        }
+                        v = -0.25 * sig / s * sub - sig * M_SQRT2 / std::sqrt(PI) * poly * std::exp(-0.5 * (s2 + rho2));
+                    }
+                }
+                gat(t, g, i0, i1, i2) = v;
+            }
+}
+
+void green_fill_0unb(const Topo& t, double hgrid, const double kfact[3], const double koffset[3], const double symstart[3],
+                     double* g, GreenType type, double length, const int* istart_custom) {
+    if (kfact[0] == 0.0 || kfact[1] == 0.0) throw std::runtime_error("dk cannot be 0");
+    const int ord = lgf_order(type);
+    if (!ord && !(type == CHAT_2 || type == HEJ_0 || (type >= HEJ_2 && type <= HEJ_10)))
+        throw std::runtime_error("kernel not restated in the oracle");
+    int lo[3] = {0, 0, 0};
+    if (istart_custom)
+        for (int d = 0; d < 3; ++d) lo[d] = std::min(std::max(istart_custom[d], 0), t.n[d]);
+    for (int i2 = lo[2]; i2 < t.n[2]; ++i2)
+        for (int i1 = lo[1]; i1 < t.n[1]; ++i1)
+            for (int i0 = lo[0]; i0 < t.n[0]; ++i0) {
+                const int il[3] = {symid(i0, symstart[0]), symid(i1, symstart[1]), symid(i2, symstart[2])};
+                const double k[3] = {(il[0] + koffset[0]) * kfact[0], (il[1] + koffset[1]) * kfact[1],
+                                     (il[2] + koffset[2]) * kfact[2]};
+                const double ksq = k[0] * k[0] + k[1] * k[1] + k[2] * k[2];
+                double v;
+                if (ord) v = lgf_spectral(ord, k, hgrid);
+                else if (type == CHAT_2 || type == HEJ_0) v = -1.0 / ksq;
+                else v = hej_spectral(type, ksq, length);
+                gat(t, g, i0, i1, i2) = v;
+            }
+    if (koffset[0] + koffset[1] + koffset[2] < 0.2) g[0] = 0.0;
+}
+
+}  // namespace oracle

@@ -1,0 +1,129 @@
+// Oracle restatement of the reference's validation harness, the producer of
+// samples/validation/data_ref (samples/validation/src/validation_3d.cpp:28-486
+// with MANUFACTURED_SOLUTION, validation_3d.hpp:64-165, main.cpp:70-135).
+// Test infrastructure only.
+#include <cmath>
+#include <stdexcept>
+#include <vector>
+
+#include "oracle.hpp"
+
+namespace oracle {
+
+namespace {
+
+struct Manu {
+    double freq = 1, sign[2] = {0, 0}, center = 0.5, sigma = 0.5;
+};
+const double C_BUMP = 10.0;
+
+double bump(double y) { return std::fabs(y) >= 1.0 ? 0.0 : std::exp(C_BUMP * (1.0 - 1.0 / (1.0 - y * y))); }
+double bump_d2(double ysq, double sl) {
+    if (std::fabs(ysq) >= 1.0) return 0.0;
+    return std::exp(C_BUMP * (1.0 - 1.0 / (1.0 - ysq))) * (2.0 * C_BUMP * (2.0 * (C_BUMP - 1.0) * ysq + 3.0 * ysq * ysq - 1.0)) /
+           std::pow(ysq - 1.0, 4) / (sl * sl);
+}
+
+enum FKind { ODDODD, EVENEVEN, ODDEVEN, EVENODD, UNBK, CST };
+
+double fsol(FKind k, double x, double L, const Manu& p) {
+    switch (k) {
+        case ODDODD: return std::sin((2.0 * M_PI / L * p.freq) * x);
+        case EVENEVEN: return std::cos((M_PI / L * p.freq) * x);
+        case ODDEVEN: return std::sin((M_PI / L * (p.freq + 0.5)) * x);
+        case EVENODD: return std::cos((M_PI / L * (p.freq + 0.5)) * x);
+        case UNBK: {
+            const double sl = p.sigma * L;
+            return bump((x - p.center * L) / sl) + p.sign[0] * bump((x + p.center * L) / sl) +
+                   p.sign[1] * bump((x - (2.0 - p.center) * L) / sl);
+        }
+        default: return 1.0;
+    }
+}
+double frhs(FKind k, double x, double L, const Manu& p) {
+    switch (k) {
+        case ODDODD: { const double a = 2.0 * M_PI / L * p.freq; return -a * a * std::sin(a * x); }
+        case EVENEVEN: { const double a = M_PI / L * p.freq; return -a * a * std::cos(a * x); }
+        case ODDEVEN: { const double a = M_PI / L * (p.freq + 0.5); return -a * a * std::sin(a * x); }
+        case EVENODD: { const double a = M_PI / L * (p.freq + 0.5); return -a * a * std::cos(a * x); }
+        case UNBK: {
+            const double sl = p.sigma * L;
+            const double y0 = (x - p.center * L) / sl, y1 = (x + p.center * L) / sl, y2 = (x - (2.0 - p.center) * L) / sl;
+            return bump_d2(y0 * y0, sl) + p.sign[0] * bump_d2(y1 * y1, sl) + p.sign[1] * bump_d2(y2 * y2, sl);
+        }
+        default: return 0.0;
+    }
+}
+
+}  // namespace
+
+void validation_case(int center, const int bc6[6], int kernel, int N, int lda, const std::string& kdir, double* err) {
+    const bool cell = (center == 1);
+    const int fact = cell ? 0 : 1;
+    const bool twod = (bc6[4] == NONE);
+    const int res = N - fact;
+    int nglob[3] = {res + fact, res + fact, twod ? 1 : res + fact};
+    const double L[3] = {1.0, 1.0, 1.0};
+    double h[3] = {L[0] / (nglob[0] - fact), L[1] / (nglob[1] - fact), nglob[2] == 1 ? 1.0 : L[2] / (nglob[2] - fact)};
+    std::vector<int> bc(6 * lda);
+    for (int f = 0; f < 6; ++f)
+        for (int c = 0; c < lda; ++c) bc[f * lda + c] = bc6[f];
+    Solver s(nglob, bc.data(), lda, h, L, !cell, NOD);
+    s.kernel_dir = kdir;
+    s.set_green((GreenType)kernel);
+    s.setup();
+
+    const size_t per = (size_t)nglob[0] * nglob[1] * nglob[2];
+    std::vector<double> rhs(per * lda, 0.0), sol(per * lda, 1.0), field(per * lda, 0.0);
+    for (int lia = 0; lia < lda; ++lia) {
+        Manu p[3];
+        p[0].freq = 1; p[1].freq = 2; p[2].freq = 4;
+        FKind fk[3];
+        for (int d = 0; d < 3; ++d) {
+            const int l = bc6[2 * d], r = bc6[2 * d + 1];
+            if ((l == PER && r == PER) || (l == ODD && r == ODD)) fk[d] = ODDODD;
+            else if (l == EVEN && r == EVEN) fk[d] = EVENEVEN;
+            else if (l == ODD && r == EVEN) fk[d] = ODDEVEN;
+            else if (l == EVEN && r == ODD) fk[d] = EVENODD;
+            else if (l == UNB) {
+                if (r == ODD) { p[d].center = 0.7; p[d].sign[1] = -1.0; }
+                else if (r == EVEN) { p[d].center = 0.7; p[d].sign[1] = 1.0; }
+                fk[d] = UNBK;
+            } else if (r == UNB) {
+                if (l == ODD) { p[d].center = 0.3; p[d].sign[0] = -1.0; }
+                else if (l == EVEN) { p[d].center = 0.3; p[d].sign[0] = 1.0; }
+                fk[d] = UNBK;
+            } else {
+                fk[d] = CST;
+            }
+        }
+        const double shift = cell ? 0.5 : 0.0;
+        for (int i2 = 0; i2 < nglob[2]; ++i2)
+            for (int i1 = 0; i1 < nglob[1]; ++i1)
+                for (int i0 = 0; i0 < nglob[0]; ++i0) {
+                    const size_t id = lia * per + (size_t)i0 + (size_t)nglob[0] * ((size_t)i1 + (size_t)nglob[1] * i2);
+                    const double x[3] = {(i0 + shift) * h[0], (i1 + shift) * h[1], (i2 + shift) * h[2]};
+                    for (int d = 0; d < 3; ++d) {
+                        const int d2 = (d + 1) % 3, d3 = (d + 2) % 3;
+                        sol[id] *= fsol(fk[d], x[d], L[d], p[d]);
+                        rhs[id] += frhs(fk[d], x[d], L[d], p[d]) * fsol(fk[d2], x[d2], L[d2], p[d2]) * fsol(fk[d3], x[d3], L[d3], p[d3]);
+                    }
+                }
+    }
+    s.solve(field.data(), rhs.data(), STD);
+    double vol = 1.0;
+    for (int d = 0; d < 3; ++d)
+        if (bc6[2 * d] != NONE && bc6[2 * d + 1] != NONE) vol *= h[d];
+    for (int lia = 0; lia < lda; ++lia) {
+        double e2 = 0, ei = 0;
+        for (size_t i = 0; i < per; ++i) {
+            const double e = sol[lia * per + i] - field[lia * per + i];
+            ei = std::max(ei, std::fabs(e));
+            e2 += e * e * vol;
+        }
+        err[2 * lia] = std::sqrt(e2);
+        err[2 * lia + 1] = ei;
+    }
+}
+
+}  // namespace oracle
