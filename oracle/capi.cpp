@@ -25,6 +25,13 @@ int oracle_validation(int center, const int* bc6, int kernel, int N, int lda, co
     } catch (const std::exception& e) { return fail(e); }
 }
 
+int oracle_vtube(int vdir, int symx, int symy, int N, int kernel, int order, double sigma, const char* kdir, double* err) {
+    try {
+        vtube_case(vdir, symx, symy, N, kernel, order, sigma, kdir ? kdir : ".", err);
+        return 0;
+    } catch (const std::exception& e) { return fail(e); }
+}
+
 // bc: [3][2][lda] ints; rhs/field: dense, dim 0 fastest, components slowest
 int oracle_solve(const int* nglob, const int* bc, int lda, const double* h, const double* L, int center, int odiff, int kernel,
                  double alpha, const char* kdir, int type, const double* rhs, double* field) {

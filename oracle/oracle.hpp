@@ -114,5 +114,7 @@ struct Solver {
 
 // validation harness of samples/validation (manufactured solutions)
 void validation_case(int center, const int bc6[6], int kernel, int N, int lda, const std::string& kdir, double* err /*[2*lda]*/);
+// vortex-tube ROT harness of samples/solve_vtube
+void vtube_case(int vdir, int symx, int symy, int N, int kernel, int order, double sigma, const std::string& kdir, double* err /*[6]*/);
 
 }  // namespace oracle

@@ -1,6 +1,8 @@
-// Oracle restatement of the reference's validation harness, the producer of
+// Oracle restatement of the reference's validation harnesses, the producers of
 // samples/validation/data_ref (samples/validation/src/validation_3d.cpp:28-486
-// with MANUFACTURED_SOLUTION, validation_3d.hpp:64-165, main.cpp:70-135).
+// with MANUFACTURED_SOLUTION, validation_3d.hpp:64-165, main.cpp:70-135) and
+// samples/solve_vtube/data_ref (solve_vtube/src/vtube.cpp: vortex tubes with
+// image tubes, ROT solve; main_vtube.cpp:110-200 for the boundary conditions).
 // Test infrastructure only.
 #include <cmath>
 #include <stdexcept>
@@ -123,6 +125,71 @@ void validation_case(int center, const int bc6[6], int kernel, int N, int lda, c
         }
         err[2 * lia] = std::sqrt(e2);
         err[2 * lia + 1] = ei;
+    }
+}
+
+
+// vortex-tube Biot-Savart case: tube along vdir, image tubes by symx/symy in {-1,0,1},
+// Gaussian core sigma, cell-centred N^3 on [0,1]^3, CHAT_2 kernel unless given
+void vtube_case(int vdir, int symx, int symy, int N, int kernel, int order, double sigma, const std::string& kdir, double* err) {
+    const int lda = 3;
+    const int nglob[3] = {N, N, N};
+    const double L[3] = {1, 1, 1};
+    const double h[3] = {1.0 / N, 1.0 / N, 1.0 / N};
+    const int dir2 = vdir, dir0 = (vdir + 1) % 3, dir1 = (vdir + 2) % 3;
+    int bc[3][2][3];
+    double xc = 0.5, yc = 0.5, xs = 0.0, ys = 0.0;
+    // main_vtube.cpp:117-190: odd/odd/even along the tube, images by symmetry, unbounded otherwise
+    for (int c = 0; c < 3; ++c) {
+        bc[dir2][0][c] = bc[dir2][1][c] = (c == dir2) ? EVEN : ODD;
+        bc[dir0][0][c] = bc[dir0][1][c] = UNB;
+        bc[dir1][0][c] = bc[dir1][1][c] = UNB;
+    }
+    if (symy == 1) { bc[dir1][0][dir0] = EVEN; bc[dir1][0][dir1] = ODD; bc[dir1][0][dir2] = EVEN; }
+    if (symy == -1) { bc[dir1][0][dir0] = ODD; bc[dir1][0][dir1] = EVEN; bc[dir1][0][dir2] = ODD; }
+    if (symx == 1) { bc[dir0][0][dir0] = ODD; bc[dir0][0][dir1] = EVEN; bc[dir0][0][dir2] = EVEN; }
+    if (symx == -1) { bc[dir0][0][dir0] = EVEN; bc[dir0][0][dir1] = ODD; bc[dir0][0][dir2] = ODD; }
+    if (symx) { xc = 0.25; xs = symx; }
+    if (symy) { yc = 0.25; ys = symy; }
+    std::vector<int> flat(6 * lda);
+    for (int d = 0; d < 3; ++d)
+        for (int s = 0; s < 2; ++s)
+            for (int c = 0; c < 3; ++c) flat[(d * 2 + s) * lda + c] = bc[d][s][c];
+    Solver s(nglob, flat.data(), lda, h, L, false, (DiffType)order);
+    s.kernel_dir = kdir;
+    s.set_green((GreenType)kernel);
+    s.setup();
+    const size_t per = (size_t)N * N * N;
+    std::vector<double> rhs(3 * per, 0.0), sol(3 * per, 0.0), field(3 * per, 0.0);
+    const double eps = 100 * 2.220446049250313e-16;
+    for (int i2 = 0; i2 < N; ++i2)
+        for (int i1 = 0; i1 < N; ++i1)
+            for (int i0 = 0; i0 < N; ++i0) {
+                const size_t id = (size_t)i0 + (size_t)N * ((size_t)i1 + (size_t)N * i2);
+                const double pos[3] = {(i0 + 0.5) * h[0], (i1 + 0.5) * h[1], (i2 + 0.5) * h[2]};
+                const double sx[4] = {-1, +1, -1, +1}, sy[4] = {-1, -1, +1, +1}, w[4] = {1.0, xs, ys, xs * ys};
+                for (int t = 0; t < 4; ++t) {
+                    const double x = pos[dir0] + sx[t] * xc * L[dir0];
+                    const double y = pos[dir1] + sy[t] * yc * L[dir1];
+                    const double th = std::atan2(y, x), r = std::sqrt(x * x + y * y), rho = r / sigma;
+                    const double vort = 1.0 / (2.0 * M_PI * sigma * sigma) * std::exp(-rho * rho * 0.5);
+                    const double vel = (r < eps) ? 0.0 : 1.0 / (2.0 * M_PI * r) * (1.0 - std::exp(-rho * rho * 0.5));
+                    rhs[dir2 * per + id] += -vort * w[t];
+                    sol[dir0 * per + id] += -std::sin(th) * vel * w[t];
+                    sol[dir1 * per + id] += +std::cos(th) * vel * w[t];
+                }
+            }
+    s.solve(field.data(), rhs.data(), ROT);
+    const double vol = h[0] * h[1] * h[2];
+    for (int c = 0; c < 3; ++c) {
+        double e2 = 0, ei = 0;
+        for (size_t i = 0; i < per; ++i) {
+            const double e = sol[c * per + i] - field[c * per + i];
+            ei = std::max(ei, std::fabs(e));
+            e2 += e * e * vol;
+        }
+        err[2 * c] = std::sqrt(e2);
+        err[2 * c + 1] = ei;
     }
 }
 
