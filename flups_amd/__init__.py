@@ -1,0 +1,273 @@
+"""Python host interface of the MI355X FLUPS solve path.
+
+Mirrors the reference's Python binding (python_api/pyflups/flups.py:23-500:
+``Topology``, ``Solver`` and the enums) over the C interface of
+``libflups.so`` (include/flups.h). Arrays passed to ``Solver.solve`` may be
+numpy arrays (host memory, copied over PCIe inside the call) or torch CUDA
+tensors (device memory, the fast path). There is no CPU fallback: importing
+works without a GPU, but solver construction needs the HIP library and a GPU.
+"""
+import ctypes
+import enum
+import os
+from ctypes import POINTER, byref, c_bool, c_char_p, c_double, c_int, c_int64, c_size_t, c_void_p
+
+import numpy as np
+
+__all__ = ["BoundaryType", "GreenType", "SolverType", "DiffType", "CenterType", "Topology", "Solver",
+           "lib", "lib_path", "last_error", "FLUPS_ALIGNMENT", "FLUPS_FORWARD", "FLUPS_BACKWARD"]
+
+FLUPS_FORWARD = -1
+FLUPS_BACKWARD = 1
+FLUPS_BACKWARD_DIFF = 2
+FLUPS_ALIGNMENT = 8
+
+
+class BoundaryType(enum.IntEnum):
+    EVEN = 0
+    ODD = 1
+    PER = 3
+    UNB = 4
+    NONE = 9
+
+
+class GreenType(enum.IntEnum):
+    CHAT_2 = 0
+    LGF_2 = 1
+    HEJ_2 = 2
+    HEJ_4 = 3
+    HEJ_6 = 4
+    HEJ_8 = 5
+    HEJ_10 = 6
+    HEJ_0 = 7
+    LGF_4 = 8
+    LGF_6 = 9
+    LGF_8 = 10
+    MEHR_4L = 11
+    MEHR_6L = 12
+    MEHR_4F = 13
+    MEHR_6F = 14
+
+
+class SolverType(enum.IntEnum):
+    STD = 0
+    ROT = 1
+
+
+class DiffType(enum.IntEnum):
+    NOD = 0
+    SPE = 1
+    FD2 = 2
+    FD4 = 4
+    FD6 = 6
+
+
+class CenterType(enum.IntEnum):
+    NODE_CENTER = 0
+    CELL_CENTER = 1
+
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+lib_path = os.path.join(_HERE, "libflups.so")
+_lib = None
+
+
+def lib():
+    """Load libflups.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(lib_path):
+            raise RuntimeError("libflups.so is missing: build it with `make -C flups_amd` "
+                               "(or __graft_entry__.build())")
+        L = ctypes.CDLL(lib_path)
+        L.flups_topo_new.restype = c_void_p
+        L.flups_topo_new.argtypes = [c_int, c_int, POINTER(c_int), POINTER(c_int), c_bool, POINTER(c_int), c_int, c_int64]
+        L.flups_topo_free.argtypes = [c_void_p]
+        for name in ["flups_topo_get_nglob", "flups_topo_get_nloc", "flups_topo_get_nmem", "flups_topo_get_nproc"]:
+            getattr(L, name).argtypes = [c_void_p, c_int]
+            getattr(L, name).restype = c_int
+        for name in ["flups_topo_get_lda", "flups_topo_get_axis"]:
+            getattr(L, name).argtypes = [c_void_p]
+            getattr(L, name).restype = c_int
+        L.flups_topo_get_isComplex.argtypes = [c_void_p]
+        L.flups_topo_get_isComplex.restype = c_bool
+        L.flups_topo_get_istartGlob.argtypes = [c_void_p, POINTER(c_int)]
+        L.flups_topo_get_memsize.argtypes = [c_void_p]
+        L.flups_topo_get_memsize.restype = c_size_t
+        L.flups_topo_get_locsize.argtypes = [c_void_p]
+        L.flups_topo_get_locsize.restype = c_size_t
+        L.flups_init_timed.restype = c_void_p
+        L.flups_init_timed.argtypes = [c_void_p, POINTER(POINTER(c_int)), POINTER(c_double), POINTER(c_double), c_int,
+                                       POINTER(c_int), c_void_p]
+        L.flups_init.restype = c_void_p
+        L.flups_init.argtypes = [c_void_p, POINTER(POINTER(c_int)), POINTER(c_double), POINTER(c_double), c_int,
+                                 POINTER(c_int)]
+        L.flups_cleanup_solver.argtypes = [c_void_p]
+        L.flups_set_greenType.argtypes = [c_void_p, c_int]
+        L.flups_set_alpha.argtypes = [c_void_p, c_double]
+        L.flups_set_stream.argtypes = [c_void_p, c_void_p]
+        L.flups_set_precision.argtypes = [c_void_p, c_int]
+        L.flups_setup.argtypes = [c_void_p, c_bool]
+        L.flups_solve.argtypes = [c_void_p, c_void_p, c_void_p, c_int]
+        L.flups_get_allocSize.argtypes = [c_void_p]
+        L.flups_get_allocSize.restype = c_size_t
+        L.flups_get_spectralInfo.argtypes = [c_void_p, POINTER(c_double), POINTER(c_double), POINTER(c_double)]
+        L.flups_get_path.argtypes = [c_void_p]
+        L.flups_get_path.restype = c_int
+        L.flups_get_last_solve_ns.argtypes = [c_void_p]
+        L.flups_get_last_solve_ns.restype = c_double
+        L.flups_get_last_error.restype = c_char_p
+        L.flups_set_abort_on_error.argtypes = [c_int]
+        _lib = L
+    return _lib
+
+
+def last_error():
+    return lib().flups_get_last_error().decode()
+
+
+def _check(ok, what):
+    if not ok:
+        raise RuntimeError("%s failed: %s" % (what, last_error()))
+
+
+class Topology:
+    """User topology (python_api/pyflups/flups.py:23-183); one rank, nproc = (1,1,1)."""
+
+    def __init__(self, axis, lda, nglob, nproc=(1, 1, 1), is_complex=False, axproc=None,
+                 alignment=FLUPS_ALIGNMENT, comm=0):
+        L = lib()
+        L.flups_clear_last_error()
+        ax = (c_int * 3)(*axproc) if axproc is not None else None
+        self._topo = L.flups_topo_new(int(axis), int(lda), (c_int * 3)(*nglob), (c_int * 3)(*nproc), bool(is_complex),
+                                      ax, int(alignment), int(comm))
+        _check(self._topo, "flups_topo_new")
+
+    def __del__(self):
+        if getattr(self, "_topo", None):
+            lib().flups_topo_free(self._topo)
+            self._topo = None
+
+    @property
+    def is_complex(self):
+        return bool(lib().flups_topo_get_isComplex(self._topo))
+
+    @property
+    def lda(self):
+        return lib().flups_topo_get_lda(self._topo)
+
+    @property
+    def axis(self):
+        return lib().flups_topo_get_axis(self._topo)
+
+    def get_nglob(self, dim):
+        return lib().flups_topo_get_nglob(self._topo, dim)
+
+    def get_nloc(self, dim):
+        return lib().flups_topo_get_nloc(self._topo, dim)
+
+    def get_nmem(self, dim):
+        return lib().flups_topo_get_nmem(self._topo, dim)
+
+    def get_nproc(self, dim):
+        return lib().flups_topo_get_nproc(self._topo, dim)
+
+    def get_istart(self):
+        a = (c_int * 3)()
+        lib().flups_topo_get_istartGlob(self._topo, a)
+        return np.array(a[:])
+
+    @property
+    def locsize(self):
+        return lib().flups_topo_get_locsize(self._topo)
+
+    @property
+    def memsize(self):
+        return lib().flups_topo_get_memsize(self._topo)
+
+    @property
+    def _ptr(self):
+        return self._topo
+
+
+def _data_ptr(a):
+    """device pointer of a torch tensor, or host pointer of a numpy array"""
+    if hasattr(a, "data_ptr"):
+        return a.data_ptr()
+    if not (isinstance(a, np.ndarray) and a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]):
+        raise TypeError("arrays must be C-contiguous float64 numpy arrays or torch tensors")
+    return a.ctypes.data
+
+
+class Solver:
+    """Poisson solver (python_api/pyflups/flups.py:221-500) on one MI355X."""
+
+    def __init__(self, topology, boundary_conditions, h, L, order_diff=DiffType.NOD, center_type=None,
+                 profiler=None):
+        Lb = lib()
+        Lb.flups_clear_last_error()
+        self._topology = topology
+        lda = topology.lda
+        self._bc_arrays = []
+        for i in range(3):
+            for j in range(2):
+                v = boundary_conditions[i][j]
+                vals = list(v) if hasattr(v, "__len__") else [v] * lda
+                self._bc_arrays.append((c_int * lda)(*[int(x) for x in vals]))
+        self._bc_c = (POINTER(c_int) * 6)(*[ctypes.cast(a, POINTER(c_int)) for a in self._bc_arrays])
+        if center_type is None:
+            center_type = [CenterType.CELL_CENTER] * 3
+        self._solver = Lb.flups_init(topology._ptr, self._bc_c, (c_double * 3)(*h), (c_double * 3)(*L), int(order_diff),
+                                     (c_int * 3)(*[int(c) for c in center_type]))
+        _check(self._solver, "flups_init")
+
+    def __del__(self):
+        if getattr(self, "_solver", None):
+            lib().flups_cleanup_solver(self._solver)
+            self._solver = None
+
+    def set_green_type(self, green_type):
+        lib().flups_set_greenType(self._solver, int(green_type))
+
+    def set_alpha(self, alpha):
+        lib().flups_set_alpha(self._solver, float(alpha))
+
+    def set_stream(self, stream_handle):
+        lib().flups_set_stream(self._solver, stream_handle)
+
+    def set_precision(self, fp32):
+        lib().flups_set_precision(self._solver, 1 if fp32 else 0)
+
+    def setup(self, change_comm=False):
+        Lb = lib()
+        Lb.flups_clear_last_error()
+        Lb.flups_setup(self._solver, bool(change_comm))
+        err = last_error()
+        if err:
+            raise RuntimeError("flups_setup failed: " + err)
+
+    def solve(self, field, rhs, solver_type=SolverType.STD):
+        Lb = lib()
+        Lb.flups_clear_last_error()
+        Lb.flups_solve(self._solver, _data_ptr(field), _data_ptr(rhs), int(solver_type))
+        err = last_error()
+        if err:
+            raise RuntimeError("flups_solve failed: " + err)
+        return field
+
+    def get_alloc_size(self):
+        return lib().flups_get_allocSize(self._solver)
+
+    def get_spectral_info(self):
+        kf, ko, ss = (c_double * 3)(), (c_double * 3)(), (c_double * 3)()
+        lib().flups_get_spectralInfo(self._solver, kf, ko, ss)
+        return np.array(kf[:]), np.array(ko[:]), np.array(ss[:])
+
+    @property
+    def path(self):
+        """1 = fused unbounded pipeline, 0 = general pencil path"""
+        return lib().flups_get_path(self._solver)
+
+    @property
+    def last_solve_ns(self):
+        return lib().flups_get_last_solve_ns(self._solver)

@@ -1,0 +1,218 @@
+// C entry points of libflups (declared in include/flups.h).
+// Each forwards to the Topo / SolverImpl objects; failed checks are caught here,
+// recorded for flups_get_last_error and, in abort mode, abort the process as the
+// reference's FLUPS_CHECK does (src/defines.hpp:301-304).
+#define FLUPS_BUILDING_LIBRARY 1
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+
+#include "../../include/flups.h"
+#include "solver.hpp"
+
+struct Topology {
+    flups::Topo t;
+};
+struct Solver {
+    flups::SolverImpl* impl;
+};
+struct Profiler {
+    std::string name;
+    double total_ns = 0;
+    long calls = 0;
+};
+
+namespace {
+thread_local std::string g_last_error;
+int g_abort = 0;
+
+void report(const char* where, const std::string& msg) {
+    g_last_error = std::string(where) + ": " + msg;
+    std::fprintf(stderr, "[flups] %s\n", g_last_error.c_str());
+    if (g_abort) std::abort();
+}
+template <typename F>
+auto guard(const char* where, F&& f) -> decltype(f()) {
+    try {
+        return f();
+    } catch (const flups::Error& e) {
+        report(where, e.msg);
+    } catch (const std::exception& e) {
+        report(where, e.what());
+    }
+    return decltype(f())();
+}
+}  // namespace
+
+extern "C" {
+
+const char* flups_get_last_error(void) { return g_last_error.c_str(); }
+void flups_clear_last_error(void) { g_last_error.clear(); }
+void flups_set_abort_on_error(int on) { g_abort = on; }
+
+void* flups_malloc(size_t size) {
+    void* p = nullptr;
+    if (posix_memalign(&p, 64, size ? size : 1) != 0) return nullptr;
+    std::memset(p, 0, size);
+    return p;
+}
+void flups_free(void* data) { std::free(data); }
+
+void flups_info(int argc, char** argv) {
+    (void)argc;
+    std::fprintf(stderr, "[flups] MI355X FLUPS solve path, single GPU, program %s\n", argv && argv[0] ? argv[0] : "?");
+}
+
+// ---- topologies ----
+FLUPS_Topology* flups_topo_new(const int axis, const int lda, const int nglob[3], const int nproc[3], const bool isComplex,
+                               const int axproc[3], const int alignment, MPI_Comm comm) {
+    return guard("flups_topo_new", [&]() -> FLUPS_Topology* {
+        FLUPS_REQUIRE(axis >= 0 && axis < 3, "axis must be 0, 1 or 2");
+        FLUPS_REQUIRE(lda >= 1, "lda must be >= 1");
+        FLUPS_REQUIRE(alignment > 0 && alignment % 8 == 0, "alignment must be a positive multiple of 8 bytes");
+        FLUPS_REQUIRE(nproc[0] * nproc[1] * nproc[2] == 1, "the total number of procs (%d) has to be 1 on one GPU",
+                      nproc[0] * nproc[1] * nproc[2]);
+        auto* T = new Topology;
+        T->t.axis = axis;
+        T->t.lda = lda;
+        T->t.nf = isComplex ? 2 : 1;
+        T->t.alignment = alignment;
+        T->t.comm = comm;
+        for (int d = 0; d < 3; ++d) {
+            FLUPS_REQUIRE(nglob[d] >= 1, "nglob must be >= 1");
+            T->t.nglob[d] = nglob[d];
+            T->t.nproc[d] = nproc[d];
+            T->t.axproc[d] = axproc ? axproc[d] : d;
+        }
+        T->t.cmpt_sizes();
+        return T;
+    });
+}
+FLUPS_Topology* flups_init_topology(const int axis, const int lda, const int nglob[3], const int nproc[3], const bool isComplex,
+                                    const int axproc[3], const int alignment, MPI_Comm comm) {
+    return flups_topo_new(axis, lda, nglob, nproc, isComplex, axproc, alignment, comm);
+}
+void flups_topo_free(const FLUPS_Topology* t) { delete t; }
+bool flups_topo_get_isComplex(const FLUPS_Topology* t) { return t->t.nf == 2; }
+int flups_topo_get_lda(const FLUPS_Topology* t) { return t->t.lda; }
+int flups_topo_get_axis(const FLUPS_Topology* t) { return t->t.axis; }
+int flups_topo_get_nglob(const FLUPS_Topology* t, const int dim) { return t->t.nglob[dim]; }
+int flups_topo_get_nloc(const FLUPS_Topology* t, const int dim) { return t->t.nloc[dim]; }
+int flups_topo_get_nmem(const FLUPS_Topology* t, const int dim) { return t->t.nmem[dim]; }
+int flups_topo_get_nproc(const FLUPS_Topology* t, const int dim) { return t->t.nproc[dim]; }
+void flups_topo_get_istartGlob(const FLUPS_Topology* t, int istart[3]) {
+    (void)t;
+    istart[0] = istart[1] = istart[2] = 0;
+}
+size_t flups_topo_get_locsize(const FLUPS_Topology* t) { return t->t.locsize(); }
+size_t flups_topo_get_memsize(const FLUPS_Topology* t) { return t->t.memsize(); }
+int flups_topo_cmpt_rank_fromid(const FLUPS_Topology* t, const int global_id, const int id) {
+    (void)t; (void)global_id; (void)id;
+    return 0;
+}
+int flups_topo_cmpt_start_id_from_rank(const FLUPS_Topology* t, const int rank_id, const int id) {
+    return rank_id * t->t.nglob[id];
+}
+MPI_Comm flups_topo_get_comm(FLUPS_Topology* t) { return (MPI_Comm)t->t.comm; }
+void flups_topo_ranksplit(const FLUPS_Topology* t, const int rank, int rankd[3]) {
+    (void)t; (void)rank;
+    rankd[0] = rankd[1] = rankd[2] = 0;
+}
+int flups_topo_rankindex(const FLUPS_Topology* topo, const int rankd[3]) {
+    (void)topo; (void)rankd;
+    return 0;
+}
+
+// ---- solver ----
+FLUPS_Solver* flups_init_timed(FLUPS_Topology* t, FLUPS_BoundaryType* bc[3][2], const double h[3], const double L[3],
+                               const FLUPS_DiffType orderDiff, const FLUPS_CenterType center_type[3], FLUPS_Profiler* prof) {
+    (void)prof;
+    return guard("flups_init", [&]() -> FLUPS_Solver* {
+        FLUPS_REQUIRE(t != nullptr, "topology is NULL");
+        FLUPS_REQUIRE(t->t.nf == 1, "the physical topology must be real");
+        const int lda = t->t.lda;
+        std::vector<int> flat(6 * lda);
+        for (int d = 0; d < 3; ++d)
+            for (int s = 0; s < 2; ++s) {
+                FLUPS_REQUIRE(bc[d][s] != nullptr, "boundary condition array [%d][%d] is NULL", d, s);
+                for (int c = 0; c < lda; ++c) flat[(d * 2 + s) * lda + c] = (int)bc[d][s][c];
+            }
+        const int center[3] = {(int)center_type[0], (int)center_type[1], (int)center_type[2]};
+        auto* S = new Solver;
+        S->impl = new flups::SolverImpl(&t->t, flat.data(), h, L, (int)orderDiff, center);
+        return S;
+    });
+}
+FLUPS_Solver* flups_init(FLUPS_Topology* t, FLUPS_BoundaryType* bc[3][2], const double h[3], const double L[3],
+                         FLUPS_DiffType orderDiff, const FLUPS_CenterType center_type[3]) {
+    return flups_init_timed(t, bc, h, L, orderDiff, center_type, nullptr);
+}
+void flups_cleanup_solver(FLUPS_Solver* s) {
+    if (!s) return;
+    guard("flups_cleanup", [&]() { delete s->impl; return 0; });
+    delete s;
+}
+void flups_cleanup(FLUPS_Solver* s) { flups_cleanup_solver(s); }
+void flups_cleanup_all(FLUPS_Solver* s) { flups_cleanup_solver(s); }
+void flups_cleanup_backend(void) {}
+
+void flups_set_greenType(FLUPS_Solver* s, const FLUPS_GreenType type) {
+    guard("flups_set_greenType", [&]() {
+        FLUPS_REQUIRE(s != nullptr, "solver is NULL");
+        FLUPS_REQUIRE((int)type >= 0 && (int)type <= 14, "unknown Green's function type %d", (int)type);
+        s->impl->set_green((int)type);
+        return 0;
+    });
+}
+void flups_set_stream(FLUPS_Solver* s, void* stream) {
+    if (s) s->impl->set_stream((hipStream_t)stream);
+}
+void flups_set_alpha(FLUPS_Solver* s, const double alpha) {
+    if (s) s->impl->set_alpha(alpha);
+}
+void flups_set_precision(FLUPS_Solver* s, int fp32) {
+    if (s) s->impl->set_fp32(fp32 != 0);
+}
+void flups_setup(FLUPS_Solver* s, const bool changeComm) {
+    (void)changeComm;
+    guard("flups_setup", [&]() {
+        FLUPS_REQUIRE(s != nullptr, "solver is NULL");
+        s->impl->setup();
+        return 0;
+    });
+}
+void flups_solve(FLUPS_Solver* s, double* field, double* rhs, const FLUPS_SolverType type) {
+    guard("flups_solve", [&]() {
+        FLUPS_REQUIRE(s != nullptr, "solver is NULL");
+        s->impl->solve(field, rhs, (int)type);
+        return 0;
+    });
+}
+size_t flups_get_allocSize(FLUPS_Solver* s) { return s ? s->impl->alloc_size() : 0; }
+void flups_get_spectralInfo(FLUPS_Solver* s, double kfact[3], double koffset[3], double symstart[3]) {
+    if (s) s->impl->spectral_info(kfact, koffset, symstart);
+}
+int flups_get_path(FLUPS_Solver* s) { return s ? s->impl->path() : -1; }
+double flups_get_last_solve_ns(FLUPS_Solver* s) { return s ? s->impl->last_solve_ns() : 0.0; }
+
+void flups_pencilDirs(const FLUPS_BoundaryType* bc[3][2], int dirs[3]) {
+    // directions in execution order: stable sort by plan type (src/flups.cpp:200-210)
+    int type[3];
+    for (int d = 0; d < 3; ++d) type[d] = (int)bc[d][0][0] + (int)bc[d][1][0];
+    int order[3] = {0, 1, 2};
+    for (int i = 1; i < 3; ++i)
+        for (int j = i; j > 0 && type[order[j]] < type[order[j - 1]]; --j) std::swap(order[j], order[j - 1]);
+    for (int d = 0; d < 3; ++d) dirs[d] = order[d];
+}
+
+// ---- profiler handles ----
+FLUPS_Profiler* flups_profiler_new(void) { return new Profiler{"flups", 0, 0}; }
+FLUPS_Profiler* flups_profiler_new_n(const char name[]) { return new Profiler{name ? name : "flups", 0, 0}; }
+void flups_profiler_free(FLUPS_Profiler* p) { delete p; }
+void flups_profiler_disp(FLUPS_Profiler* p) {
+    if (p) std::fprintf(stderr, "[flups] profiler %s\n", p->name.c_str());
+}
+
+}  // extern "C"

@@ -1,0 +1,260 @@
+// General pencil pass for every boundary condition and length.
+//
+// One launch transforms one component along one direction: each block takes
+// `ppb` pencils, gathers them from the source array into LDS (applying the
+// pencil-switch shift and zero padding of src/SwitchTopoX_a2a.cpp:174-247),
+// runs the 1-D transform of the plan with its input/output offsets
+// (src/FFTW_plan_dim.cpp:635-720) through a mixed-radix Stockham FFT held in
+// LDS, applies the post-correction (src/FFTW_plan_dim.cpp:497-622) and
+// scatters the pencil to the destination array (truncating on the way back).
+//
+// r2r kinds are computed from one complex FFT of an extended sequence:
+//   REDFT00 length 2(n-1) even extension, RODFT00 length 2(n+1) odd extension,
+//   REDFT10/01/11 length 2n with quarter-sample twiddles, and
+//   RODFT10/01/11 through DST(x) = reflections of DCT of (-1)^j x or reversed x.
+// r2c/c2r of even length n use the half-length packed complex FFT.
+#include <hip/hip_runtime.h>
+
+#include "core.hpp"
+
+namespace flups {
+
+namespace {
+
+struct cd {
+    double x, y;
+};
+__device__ __forceinline__ cd mk(double a, double b) { return {a, b}; }
+__device__ __forceinline__ cd add(cd a, cd b) { return {a.x + b.x, a.y + b.y}; }
+__device__ __forceinline__ cd sub(cd a, cd b) { return {a.x - b.x, a.y - b.y}; }
+__device__ __forceinline__ cd mul(cd a, cd b) { return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
+__device__ __forceinline__ cd conj(cd a) { return {a.x, -a.y}; }
+__device__ __forceinline__ cd scal(cd a, double s) { return {a.x * s, a.y * s}; }
+// exp(-i pi t), accurate for any t
+__device__ __forceinline__ cd expmpi(double t) {
+    double s, c;
+    sincospi(t, &s, &c);
+    return {c, -s};
+}
+
+// twiddle omega_M^e with the pass sign (table holds exp(-2 pi i t/M))
+__device__ __forceinline__ cd twid(const double* tw, long long e, int M, int sign) {
+    e %= M;
+    if (e < 0) e += M;
+    cd w = {tw[2 * e], tw[2 * e + 1]};
+    return sign > 0 ? conj(w) : w;
+}
+
+// one Stockham stage, all outputs of all pencils of the block
+__device__ void stockham_stage(const cd* in, cd* out, int M, int p, int span, const double* tw, int sign, int npen) {
+    const int mp = M / p;
+    const int scale = M / (span * p);
+    for (int i = threadIdx.x; i < npen * M; i += blockDim.x) {
+        const int pen = i / M, o = i - pen * M;
+        const int k = o % span;
+        const int u = (o / span) % p;
+        const int b = o / (span * p);
+        const int j = b * span + k;
+        const cd* a = in + (size_t)pen * M;
+        cd acc = {0, 0};
+        const long long base = (long long)(k + u * span) * scale;
+        for (int q = 0; q < p; ++q) acc = add(acc, mul(a[j + q * mp], twid(tw, q * base, M, sign)));
+        out[(size_t)pen * M + o] = acc;
+    }
+}
+
+}  // namespace
+
+__global__ void __launch_bounds__(256) generic_pass(PassDesc P) {
+    extern __shared__ double smem[];
+    const int len = P.pen_in * P.nf_in;   // doubles per pencil (same before and after)
+    const int npen_total = P.n1 * P.n2;
+    const int pen0 = blockIdx.x * P.ppb;
+    const int npen = min(P.ppb, npen_total - pen0);
+    if (npen <= 0) return;
+    double* pb = smem;                                       // pencils, ppb * len doubles
+    cd* A = reinterpret_cast<cd*>(smem + (size_t)P.ppb * len + ((P.ppb * len) & 1));
+    cd* B = A + (size_t)P.ppb * P.M;
+    const int d = P.d, d1 = (d + 1) % 3, d2 = (d + 2) % 3;
+
+    // ---- gather (pencil switch: shift, zero padding) ----
+    for (int i = threadIdx.x; i < npen * len; i += blockDim.x) {
+        const int pen = i / len, e = i - pen * len;
+        const int g = pen0 + pen;
+        const int j1 = g % P.n1, j2 = g / P.n1;
+        const int k = e / P.nf_in, c = e - k * P.nf_in;
+        const int ks = k - P.gshift;
+        double v = 0.0;
+        if (ks >= 0 && ks < P.src_nd) {
+            long long idx = (long long)ks * P.src_s[d] + (long long)j1 * P.src_s[d1] + (long long)j2 * P.src_s[d2];
+            v = P.src[idx * P.src_nf + c];
+        }
+        pb[(size_t)pen * len + e] = v;
+    }
+    __syncthreads();
+
+    // ---- pre-map: pencil -> complex sequence of length M ----
+    const int n = P.n, M = P.M, st = P.r2r_stride;
+    int kind = P.kind;
+    // DST kinds are computed through the matching DCT
+    const bool dst = (kind >= RODFT10);
+    const int dct = dst ? kind - 4 : kind;   // RODFT10 -> REDFT10, RODFT01 -> REDFT01, RODFT11 -> REDFT11
+    for (int i = threadIdx.x; i < npen * M; i += blockDim.x) {
+        const int pen = i / M, j = i - pen * M;
+        const double* x = pb + (size_t)pen * len;
+        cd v = {0, 0};
+        if (P.op == OP_C2C) {
+            v = mk(x[2 * (P.si + j)], x[2 * (P.si + j) + 1]);
+        } else if (P.op == OP_R2C) {
+            if ((n & 1) == 0) v = mk(x[P.si + 2 * j], x[P.si + 2 * j + 1]);
+            else v = mk(x[P.si + j], 0.0);
+        } else if (P.op == OP_C2R) {
+            if ((n & 1) == 0) {
+                const int h = n / 2;
+                cd a = mk(x[2 * (P.si + j)], x[2 * (P.si + j) + 1]);
+                cd b = mk(x[2 * (P.si + h - j)], x[2 * (P.si + h - j) + 1]);
+                if (j == 0) { a.y = 0.0; b.y = 0.0; }   // DC and Nyquist are real
+                const cd bc = conj(b);
+                const cd e = add(a, bc), f = sub(a, bc);
+                const cd w = conj(expmpi(2.0 * j / n));          // exp(+2 pi i j/n)
+                const cd t = mul(w, f);
+                v = add(e, mk(-t.y, t.x));                          // e + i * w * f
+            } else {
+                if (j == 0) v = mk(x[2 * P.si], 0.0);
+                else if (j <= n / 2) v = mk(x[2 * (P.si + j)], x[2 * (P.si + j) + 1]);
+                else v = conj(mk(x[2 * (P.si + n - j)], x[2 * (P.si + n - j) + 1]));
+            }
+        } else if (P.op == OP_R2R) {
+            // input element jj of the (possibly reflected) real sequence
+            auto xin = [&](int jj) -> double {
+                if (kind == RODFT10) return (jj & 1) ? -x[P.si + st * jj] : x[P.si + st * jj];
+                if (kind == RODFT01 || kind == RODFT11) return x[P.si + st * (n - 1 - jj)];
+                return x[P.si + st * jj];
+            };
+            if (kind == REDFT00) {
+                const int jj = (j < n) ? j : M - j;
+                v = mk(xin(jj), 0.0);
+            } else if (kind == RODFT00) {
+                if (j >= 1 && j <= n) v = mk(xin(j - 1), 0.0);
+                else if (j >= n + 2) v = mk(-xin(M - 1 - j), 0.0);
+            } else if (dct == REDFT10) {
+                v = mk(xin(j < n ? j : M - 1 - j), 0.0);
+            } else if (dct == REDFT01) {
+                if (j < n) v = scal(conj(expmpi(0.5 * j / n)), (j == 0 ? 1.0 : 2.0) * xin(j));
+            } else {   // REDFT11
+                if (j < n) v = scal(expmpi(0.5 * j / n), xin(j));
+            }
+        }
+        A[(size_t)pen * M + j] = v;
+    }
+    __syncthreads();
+
+    // ---- FFT ----
+    int sgn = P.sign;
+    if (P.op == OP_R2C) sgn = FWD;
+    if (P.op == OP_C2R) sgn = BWD;
+    if (P.op == OP_R2R) sgn = (dct == REDFT01) ? BWD : FWD;
+    cd* in = A;
+    cd* out = B;
+    int span = 1;
+    for (int s = 0; s < P.nst; ++s) {
+        stockham_stage(in, out, M, P.rad[s], span, P.tw, sgn, npen);
+        span *= P.rad[s];
+        __syncthreads();
+        cd* t = in; in = out; out = t;
+    }
+    const cd* R = in;
+
+    // ---- post-map: sequence -> pencil ----
+    int nwrite = 0;
+    if (P.op == OP_C2C) nwrite = n;
+    else if (P.op == OP_R2C) nwrite = n / 2 + 1;
+    else if (P.op == OP_C2R) nwrite = ((n & 1) == 0) ? n / 2 : n;
+    else if (P.op == OP_R2R) nwrite = n;
+    for (int i = threadIdx.x; i < npen * nwrite; i += blockDim.x) {
+        const int pen = i / nwrite, k = i - pen * nwrite;
+        double* x = pb + (size_t)pen * len;
+        const cd* r = R + (size_t)pen * M;
+        if (P.op == OP_C2C) {
+            x[2 * (P.so + k)] = r[k].x;
+            x[2 * (P.so + k) + 1] = r[k].y;
+        } else if (P.op == OP_R2C) {
+            cd X;
+            if ((n & 1) == 0) {
+                const int h = n / 2;
+                const cd zk = r[k % h], zc = conj(r[(h - k) % h]);
+                const cd e = scal(add(zk, zc), 0.5);
+                const cd df = scal(sub(zk, zc), 0.5);
+                const cd o = mk(df.y, -df.x);                       // (zk - zc) / (2 i)
+                X = add(e, mul(expmpi(2.0 * k / n), o));
+            } else {
+                X = r[k];
+            }
+            x[2 * (P.so + k)] = X.x;
+            x[2 * (P.so + k) + 1] = X.y;
+        } else if (P.op == OP_C2R) {
+            if ((n & 1) == 0) {
+                x[P.so + 2 * k] = r[k].x;
+                x[P.so + 2 * k + 1] = r[k].y;
+            } else {
+                x[P.so + k] = r[k].x;
+            }
+        } else {
+            double y;
+            int kk = k;
+            if (kind == RODFT10) kk = n - 1 - k;   // DST-II(x)[k] = DCT-II((-1)^j x)[n-1-k]
+            if (kind == REDFT00) y = r[kk].x;
+            else if (kind == RODFT00) y = -r[kk + 1].y;
+            else if (dct == REDFT10) y = mul(expmpi(0.5 * kk / n), r[kk]).x;
+            else if (dct == REDFT01) y = r[kk].x;
+            else y = 2.0 * mul(expmpi((2.0 * kk + 1.0) / (4.0 * n)), r[kk]).x;
+            if ((kind == RODFT01 || kind == RODFT11) && (k & 1)) y = -y;   // (-1)^k reflection
+            x[P.so + st * k] = y;
+        }
+    }
+    __syncthreads();
+
+    // ---- post-correction ----
+    if (threadIdx.x < npen) {
+        double* x = pb + (size_t)threadIdx.x * len;
+        const int pp = P.post;
+        const bool f = pp & POST_FIRST, l = pp & POST_LAST;
+        if (f) x[0] = 0.0;
+        if (l) x[P.pen_out - 1] = 0.0;
+        if (!f && !l && (pp & POST_PERIOD)) {
+            const int nf = (P.sign == FWD) ? P.n_out_plan - 1 : P.n;
+            if (P.op == OP_C2R) x[nf] = x[0];
+            else { x[2 * nf] = x[0]; x[2 * nf + 1] = x[1]; }
+        }
+    }
+    __syncthreads();
+
+    // ---- scatter (truncation on the way back) ----
+    const int lo = P.dst_nd * P.dst_nf;
+    for (int i = threadIdx.x; i < npen * lo; i += blockDim.x) {
+        const int pen = i / lo, e = i - pen * lo;
+        const int g = pen0 + pen;
+        const int j1 = g % P.n1, j2 = g / P.n1;
+        const int k = e / P.dst_nf, c = e - k * P.dst_nf;
+        const int kt = k + P.sshift;
+        if (kt >= P.pen_out) continue;
+        const double v = pb[(size_t)pen * len + (size_t)kt * P.nf_out + c];
+        long long idx = (long long)k * P.dst_s[d] + (long long)j1 * P.dst_s[d1] + (long long)j2 * P.dst_s[d2];
+        P.dst[idx * P.dst_nf + c] = v;
+    }
+}
+
+size_t generic_lds_bytes(const PassDesc& P, int ppb) {
+    const size_t len = (size_t)P.pen_in * P.nf_in;
+    return (ppb * len + ((ppb * len) & 1)) * sizeof(double) + 2 * (size_t)ppb * P.M * 2 * sizeof(double);
+}
+
+void launch_generic(const PassDesc& P, hipStream_t s) {
+    const int npen = P.n1 * P.n2;
+    const int blocks = (npen + P.ppb - 1) / P.ppb;
+    const size_t lds = generic_lds_bytes(P, P.ppb);
+    hipLaunchKernelGGL(generic_pass, dim3(blocks), dim3(256), lds, s, P);
+    HIP_CHECK(hipGetLastError());
+}
+
+}  // namespace flups

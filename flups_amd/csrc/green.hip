@@ -1,0 +1,519 @@
+// Green's functions evaluated on the device at setup, and the spectral multiply.
+//
+// Fill kernels restate the drivers of src/green_functions.cpp
+//   3 unbounded directions          :27-167
+//   2 unbounded + 1 spectral        :192-391
+//   1 unbounded + 2 spectral        :407-538
+//   fully spectral                  :560-708
+// with the closed-form kernels of src/green_kernels.hpp (HEJ_2..10, CHAT_2,
+// LGF_2..8) evaluated in FP64. Special functions are computed here from their
+// integral or series definitions:
+//   K_nu(x) = int_0^inf exp(-x cosh t) cosh(nu t) dt, trapezoidal rule with
+//             step 1/4 (error ~ exp(-pi^2 / h) for an integrand analytic in
+//             |Im t| < pi/2), Abramowitz & Stegun 9.6.24;
+//   E_1(x)  = -gamma - ln x - sum (-x)^k/(k k!) for x <= 1 (A&S 5.1.11) and
+//             the continued fraction of A&S 5.1.22 (modified Lentz) for x > 1.
+// The multiply kernels restate src/dothemagic_std.ipp:56-83 and
+// src/dothemagic_rot.ipp:102-349.
+#include <hip/hip_runtime.h>
+
+#include "core.hpp"
+#include "green.hpp"
+
+namespace flups {
+
+namespace {
+
+constexpr double kPi = 3.14159265358979323846;
+constexpr double kEuler = 0.57721566490153286061;
+
+__device__ double bessel_k(int nu, double x) {
+    // exp(-x) factored out so that large x does not underflow before scaling
+    // the step keeps exp(x (1 - cos y) - 2 pi y / h) below 1e-17 for the best strip half-width y
+    const double hstep = fmin(0.25, 2.0 * kPi / sqrt(90.0 * fmax(x, 1e-300)));
+    double s = 0.5;   // t = 0 term: exp(-x (cosh 0 - 1)) cosh(0) = 1, halved
+    for (int i = 1; i < 4000; ++i) {
+        const double t = i * hstep;
+        const double term = exp(-x * (cosh(t) - 1.0)) * (nu ? cosh(t) : 1.0);
+        s += term;
+        if (term < 1e-18 * s) break;
+    }
+    return hstep * s * exp(-x);
+}
+
+__device__ double expint_e1(double x) {
+    if (x <= 1.0) {
+        double sum = 0.0, term = 1.0;
+        for (int k = 1; k < 60; ++k) {
+            term *= -x / k;
+            const double add = -term / k;
+            sum += add;
+            if (fabs(add) < 1e-18 * fabs(sum)) break;
+        }
+        return -kEuler - log(x) + sum;
+    }
+    // E1(x) = exp(-x) / (x + 1/(1 + 1/(x + 2/(1 + 2/(x + ...)))))  evaluated as b_k continued fraction
+    double b = x + 1.0, c = 1e300, d = 1.0 / b, h = d;
+    for (int i = 1; i < 300; ++i) {
+        const double an = -(double)i * i;
+        b += 2.0;
+        d = 1.0 / (an * d + b);
+        c = b + an / c;
+        const double del = c * d;
+        h *= del;
+        if (fabs(del - 1.0) < 1e-17) break;
+    }
+    return h * exp(-x);
+}
+
+__device__ __forceinline__ int symid(int i, double ss) {
+    if (ss == 0.0 || i <= ss) return i;
+    const int v = (int)fabs(2.0 * ss - i);
+    return -(v > 1 ? v : 1);
+}
+
+__device__ double hej3_poly(int t, double rho) {
+    const double r2 = rho * rho;
+    switch (t) {
+        case HEJ_4: return rho;
+        case HEJ_6: return rho * (7.0 / 4.0 - r2 / 4.0);
+        case HEJ_8: return rho * (19.0 / 8.0 - (2.0 / 3.0) * r2 + r2 * r2 / 24.0);
+        case HEJ_10: return rho * (187.0 / 64.0 - (233.0 / 192.0) * r2 + (29.0 / 192.0) * r2 * r2 - r2 * r2 * r2 / 192.0);
+        default: return 0.0;
+    }
+}
+__device__ double hej2_q(int t, double r2) {
+    switch (t) {
+        case HEJ_4: return 0.5;
+        case HEJ_6: return 0.75 - r2 / 8.0;
+        case HEJ_8: return 11.0 / 12.0 - 7.0 * r2 / 24.0 + r2 * r2 / 48.0;
+        case HEJ_10: return 25.0 / 24.0 - 23.0 * r2 / 48.0 + 13.0 * r2 * r2 / 192.0 - r2 * r2 * r2 / 384.0;
+        default: return 0.0;
+    }
+}
+__device__ int lgf_order(int t) {
+    return t == LGF_2 ? 2 : t == LGF_4 ? 4 : t == LGF_6 ? 6 : t == LGF_8 ? 8 : 0;
+}
+__device__ double lgf_symbol(int order, double k) {
+    const double s1 = sin(0.5 * k), s2 = sin(k), s3 = sin(1.5 * k), s4 = sin(2.0 * k);
+    switch (order) {
+        case 2: return 4.0 * s1 * s1;
+        case 4: return 4.0 * (4.0 / 3.0 * s1 * s1 - 1.0 / 12.0 * s2 * s2);
+        case 6: return 4.0 * (1.5 * s1 * s1 - 0.15 * s2 * s2 + 1.0 / 90.0 * s3 * s3);
+        default: return 4.0 * (1.6 * s1 * s1 - 0.2 * s2 * s2 + 8.0 / 315.0 * s3 * s3 - 1.0 / 560.0 * s4 * s4);
+    }
+}
+__device__ double ipow(double x, int e) {
+    double r = 1.0;
+    for (int i = 0; i < e; ++i) r *= x;
+    return r;
+}
+// far-field expansion: sum coef n1^a n2^b n3^c n^-p log(n)^l
+__device__ double expansion(const ExpTerm* t, int nt, double n1, double n2, double n3) {
+    const double n = sqrt(n1 * n1 + n2 * n2 + n3 * n3), ln = log(n);
+    double s = 0.0;
+    for (int i = 0; i < nt; ++i)
+        s += t[i].coef * ipow(n1, t[i].a) * ipow(n2, t[i].b) * ipow(n3, t[i].c) / ipow(n, t[i].p) * (t[i].l ? ln : 1.0);
+    return s;
+}
+
+struct FillGeom {
+    int n[3], nf, axis;
+};
+__device__ __forceinline__ void unflat(long long e, const FillGeom& g, int i[3]) {
+    i[0] = (int)(e % g.n[0]);
+    i[1] = (int)((e / g.n[0]) % g.n[1]);
+    i[2] = (int)(e / ((long long)g.n[0] * g.n[1]));
+}
+
+__global__ void fill_3unb(double* g, FillGeom geo, GreenArgs A) {
+    const long long tot = (long long)geo.n[0] * geo.n[1] * geo.n[2];
+    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < tot; e += (long long)gridDim.x * blockDim.x) {
+        int i[3];
+        unflat(e, geo, i);
+        const int is[3] = {symid(i[0], A.symstart[0]), symid(i[1], A.symstart[1]), symid(i[2], A.symstart[2])};
+        const double x0 = is[0] * A.hfact[0], x1 = is[1] * A.hfact[1], x2 = is[2] * A.hfact[2];
+        const double r = sqrt(x0 * x0 + x1 * x1 + x2 * x2);
+        const int ord = lgf_order(A.type);
+        double v;
+        if (ord) {
+            const int ax0 = geo.axis, ax1 = (ax0 + 1) % 3, ax2 = (ax0 + 2) % 3;
+            const int a = abs(is[ax0]), b = abs(is[ax1]), c = abs(is[ax2]);
+            const double gv = (a < 32 && b < 32 && c < 32) ? A.table[a + 32 * b + 1024 * c] : expansion(A.exp, A.nexp, a, b, c);
+            v = -gv / A.hfact[0];
+        } else if (A.type == CHAT_2) {
+            v = -1.0 / (4.0 * kPi * r);
+        } else {
+            const double rho = r / A.length;
+            v = -1.0 / (4.0 * kPi * r) * (erf(rho / sqrt(2.0)) + hej3_poly(A.type, rho) * exp(-0.5 * rho * rho) / sqrt(2.0 * kPi));
+        }
+        if (e == 0 && !ord) v = A.g0;
+        g[e * geo.nf] = v;
+    }
+}
+
+__global__ void fill_2unb(double* g, FillGeom geo, GreenArgs A) {
+    const long long tot = (long long)geo.n[0] * geo.n[1] * geo.n[2];
+    const double sh = A.hfact[0] + A.hfact[1] + A.hfact[2];
+    const double sk = A.kfact[0] + A.kfact[1] + A.kfact[2];
+    const double req = sqrt(A.hfact[0] * A.hfact[1] + A.hfact[1] * A.hfact[2] + A.hfact[2] * A.hfact[0]) / sqrt(kPi);
+    const int ord = lgf_order(A.type);
+    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < tot; e += (long long)gridDim.x * blockDim.x) {
+        int i[3];
+        unflat(e, geo, i);
+        const int is[3] = {symid(i[0], A.symstart[0]), symid(i[1], A.symstart[1]), symid(i[2], A.symstart[2])};
+        const double k = (is[0] + A.koffset[0]) * A.kfact[0] + (is[1] + A.koffset[1]) * A.kfact[1] + (is[2] + A.koffset[2]) * A.kfact[2];
+        const double x0 = is[0] * A.hfact[0], x1 = is[1] * A.hfact[1], x2 = is[2] * A.hfact[2];
+        const double r = sqrt(x0 * x0 + x1 * x1 + x2 * x2);
+        const bool r0 = r <= sh * 0.2;
+        const bool k0 = !r0 && k <= sk * 0.2;
+        double v = 0.0;
+        if (ord) {
+            if (r0 || k0) {
+                const int ax0 = geo.axis, ax1 = (ax0 + 1) % 3, ax2 = (ax0 + 2) % 3;
+                const int a[3] = {abs(is[ax0]), abs(is[ax1]), abs(is[ax2])};
+                const int j1 = (a[0] == 0) ? a[1] : a[0];
+                const int j2 = (a[0] == 0) ? a[2] : (a[1] == 0 ? a[2] : a[1]);
+                const double gv = (j1 < 32 && j2 < 32) ? A.table[j1 + 32 * j2] : expansion(A.exp, A.nexp, j1, j2, 0.0);
+                v = -gv;
+            }
+        } else if (A.type == CHAT_2) {
+            if (r0) {
+                const double kr = k * req;
+                v = -(1.0 - kr * bessel_k(1, kr)) / (kPi * kr * kr);
+            } else if (k0) {
+                v = log(r) / (2.0 * kPi);
+            } else {
+                v = -bessel_k(0, fabs(k) * r) / (2.0 * kPi);
+            }
+        } else {
+            if (r0) {
+                v = -(0.5 * kEuler - log(sqrt(2.0) * A.length) + hej2_q(A.type, 0.0)) / (2.0 * kPi);
+            } else if (k0) {
+                const double rho2 = (r / A.length) * (r / A.length);
+                v = (log(r) - hej2_q(A.type, rho2) * exp(-0.5 * rho2) + 0.5 * expint_e1(0.5 * rho2)) / (2.0 * kPi);
+            }
+        }
+        if (e == 0 && A.type == CHAT_2) v = 0.25 / (2.0 * kPi) * (kPi - 6.0 + 2.0 * log(0.5 * kPi * req * req));
+        g[e * geo.nf] = v;
+    }
+}
+
+// ---- 1-D lattice Green's function of the order-2w stencil shifted by c ----
+// g(n) = 1/(2 pi) int cos(n t)/(c + s(t)) dt, s(t) = s_0 + 2 sum_m s_m cos(m t).
+// With z = exp(-L) the roots of c + s = 0 inside the unit circle, c + s_0 + 2 sum s_m cosh(m L) = 0;
+// the residues give g(n) = sum_i A_i exp(-|n| L_i), A_i = -1 / (2 sum_m m s_m sinh(m L_i)).
+// The roots are found in mu = cosh(L) - 1 (a degree-w polynomial c + sum_j p_j mu^j), so that
+// L = 2 asinh(sqrt(mu/2)) stays accurate for small c. For c = 0 the double root z = 1 is
+// regularised into -|n|/2 and the constant that makes g(0) = 0.
+struct cz {
+    double x, y;
+};
+__device__ __forceinline__ cz operator+(cz a, cz b) { return {a.x + b.x, a.y + b.y}; }
+__device__ __forceinline__ cz operator-(cz a, cz b) { return {a.x - b.x, a.y - b.y}; }
+__device__ __forceinline__ cz operator*(cz a, cz b) { return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
+__device__ __forceinline__ cz operator*(double s, cz a) { return {s * a.x, s * a.y}; }
+__device__ __forceinline__ cz cdiv(cz a, cz b) {
+    const double den = b.x * b.x + b.y * b.y;
+    return {(a.x * b.x + a.y * b.y) / den, (a.y * b.x - a.x * b.y) / den};
+}
+__device__ __forceinline__ double cabs_(cz a) { return hypot(a.x, a.y); }
+__device__ cz cexp_(cz a) {
+    const double e = exp(a.x);
+    double s, c;
+    sincos(a.y, &s, &c);
+    return {e * c, e * s};
+}
+__device__ cz clog_(cz a) { return {log(cabs_(a)), atan2(a.y, a.x)}; }
+__device__ cz csqrt_(cz a) {
+    const double r = cabs_(a);
+    if (r == 0.0) return {0.0, 0.0};
+    const double t = sqrt(0.5 * (r + fabs(a.x)));
+    if (a.x >= 0.0) return {t, a.y / (2.0 * t)};
+    return {fabs(a.y) / (2.0 * t), a.y >= 0 ? t : -t};
+}
+__device__ cz csinh_(cz z) {
+    if (cabs_(z) < 0.1) {   // odd series, error below z^13/13!
+        const cz z2 = z * z;
+        cz t = z, s = z;
+        for (int k = 1; k <= 5; ++k) {
+            t = (1.0 / ((2.0 * k) * (2.0 * k + 1.0))) * (t * z2);
+            s = s + t;
+        }
+        return s;
+    }
+    return 0.5 * (cexp_(z) - cexp_({-z.x, -z.y}));
+}
+__device__ cz casinh_(cz w) {
+    if (cabs_(w) < 0.01) {   // w - w^3/6 + 3 w^5/40 - 5 w^7/112
+        const cz w2 = w * w;
+        return w * (cz{1.0, 0.0} + w2 * (cz{-1.0 / 6.0, 0.0} + w2 * (cz{3.0 / 40.0, 0.0} + w2 * cz{-5.0 / 112.0, 0.0})));
+    }
+    return clog_(w + csqrt_(w * w + cz{1.0, 0.0}));
+}
+// roots of sum_{j=0}^{deg} a_j mu^j (a_deg != 0), deg <= 4, Durand-Kerner then Newton polish
+__device__ int poly_roots(const double* a, int deg, cz* r) {
+    for (int i = 0; i < deg; ++i) {
+        const double ang = 0.4 + 2.0 * kPi * i / deg;
+        r[i] = {cos(ang), sin(ang)};
+    }
+    auto eval = [&](cz x) {
+        cz v = {a[deg], 0.0};
+        for (int j = deg - 1; j >= 0; --j) v = v * x + cz{a[j], 0.0};
+        return v;
+    };
+    auto deriv = [&](cz x) {
+        cz v = {deg * a[deg], 0.0};
+        for (int j = deg - 1; j >= 1; --j) v = v * x + cz{j * a[j], 0.0};
+        return v;
+    };
+    for (int it = 0; it < 500; ++it) {
+        double delta = 0.0;
+        for (int i = 0; i < deg; ++i) {
+            cz den = {a[deg], 0.0};
+            for (int j = 0; j < deg; ++j)
+                if (j != i) den = den * (r[i] - r[j]);
+            const cz dz = cdiv(eval(r[i]), den);
+            r[i] = r[i] - dz;
+            delta = fmax(delta, cabs_(dz) / fmax(cabs_(r[i]), 1e-300));
+        }
+        if (delta < 1e-16) break;
+    }
+    for (int i = 0; i < deg; ++i)
+        for (int it = 0; it < 3; ++it) {
+            const cz dp = deriv(r[i]);
+            if (cabs_(dp) == 0.0) break;
+            r[i] = r[i] - cdiv(eval(r[i]), dp);
+        }
+    return deg;
+}
+__device__ double lgf1d_eval(const GreenArgs& A, int order, int n, double c) {
+    const int w = order / 2;
+    const double* s = A.lgf_s + 5 * (w - 1);   // s_0..s_w
+    const double* p = A.lgf_p + 5 * (w - 1);   // p_0 (=0)..p_w
+    double a[5];
+    cz mu[4];
+    int nr;
+    if (c == 0.0) {
+        for (int j = 1; j <= w; ++j) a[j - 1] = p[j];
+        nr = poly_roots(a, w - 1, mu);
+    } else {
+        a[0] = c;
+        for (int j = 1; j <= w; ++j) a[j] = p[j];
+        nr = poly_roots(a, w, mu);
+    }
+    const double an = fabs((double)n);
+    double g = (c == 0.0) ? -0.5 * an : 0.0;
+    for (int i = 0; i < nr; ++i) {
+        cz L = 2.0 * casinh_(csqrt_(0.5 * mu[i]));
+        if (L.x < 0.0) L = {-L.x, -L.y};
+        cz den = {0.0, 0.0};
+        for (int m = 1; m <= w; ++m) den = den + (m * s[m]) * csinh_((double)m * L);
+        const cz Ai = cdiv({-0.5, 0.0}, den);
+        cz term = Ai * cexp_((-an) * L);
+        if (c == 0.0) term = term - Ai;
+        g += term.x;
+    }
+    return g;
+}
+
+__global__ void fill_1unb(double* g, FillGeom geo, GreenArgs A) {
+    const long long tot = (long long)geo.n[0] * geo.n[1] * geo.n[2];
+    const double sk = A.kfact[0] + A.kfact[1] + A.kfact[2];
+    const double h = A.hfact[0] + A.hfact[1] + A.hfact[2];
+    const int ord = lgf_order(A.type);
+    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < tot; e += (long long)gridDim.x * blockDim.x) {
+        int i[3];
+        unflat(e, geo, i);
+        const int is[3] = {symid(i[0], A.symstart[0]), symid(i[1], A.symstart[1]), symid(i[2], A.symstart[2])};
+        const double kk[3] = {(is[0] + A.koffset[0]) * A.kfact[0], (is[1] + A.koffset[1]) * A.kfact[1], (is[2] + A.koffset[2]) * A.kfact[2]};
+        const double k = sqrt(kk[0] * kk[0] + kk[1] * kk[1] + kk[2] * kk[2]);
+        const double x0 = is[0] * A.hfact[0], x1 = is[1] * A.hfact[1], x2 = is[2] * A.hfact[2];
+        const double r = sqrt(x0 * x0 + x1 * x1 + x2 * x2);
+        const bool k0 = k <= sk * 0.2;
+        double v;
+        if (ord) {
+            double c = 0.0;
+            for (int dd = 0; dd < 3; ++dd) c += lgf_symbol(ord, kk[dd] * h);
+            const int nn = (int)round(fabs(r) / h);
+            v = -h * lgf1d_eval(A, ord, nn, c);
+        } else if (A.type == CHAT_2) {
+            v = k0 ? 0.5 * fabs(r) : -0.5 * exp(-k * r) / k;
+        } else {
+            const double sig = A.length, rho = r / sig, rho2 = rho * rho;
+            const double cst = 1.0 / (sqrt(2.0) * sqrt(kPi));
+            if (k0) {
+                const double er = 0.5 * r * erf(r / (sqrt(2.0) * sig));
+                const double ex = exp(-0.5 * rho2);
+                double tail;
+                switch (A.type) {
+                    case HEJ_2: tail = 1.0 - ex; break;
+                    case HEJ_4: tail = 0.5 * (1.0 - ex); break;
+                    case HEJ_6: tail = 0.125 * (3.0 - ex * (rho2 + 3.0)); break;
+                    case HEJ_8: tail = 5.0 / 16.0 - ex * (-rho2 * rho2 / 48.0 + 0.25 * rho2 + 5.0 / 16.0); break;
+                    default: tail = 35.0 / 128.0 - ex * (rho2 * rho2 * rho2 / 384.0 - 23.0 / 384.0 * rho2 * rho2 + 47.0 / 128.0 * rho2 + 35.0 / 128.0);
+                }
+                v = er - tail * sig * cst;
+            } else {
+                const double s = k * sig, s2 = s * s;
+                const double sub = (s * rho > 100.0) ? 0.0
+                                                     : ((1.0 - erf((s - rho) / sqrt(2.0))) * exp(-s * rho) +
+                                                        (1.0 - erf((s + rho) / sqrt(2.0))) * exp(s * rho));
+                double poly;
+                switch (A.type) {
+                    case HEJ_2: poly = 0.0; break;
+                    case HEJ_4: poly = 0.25; break;
+                    case HEJ_6: poly = 5.0 / 16.0 + (s2 - rho2) / 16.0; break;
+                    case HEJ_8: poly = 11.0 / 32.0 + s2 / 12.0 - 0.125 * rho2 - s2 * rho2 / 48.0 + (s2 * s2 + rho2 * rho2) / 96.0; break;
+                    default:
+                        poly = 93.0 / 256.0 + 73.0 / 768.0 * s2 - 47.0 / 256.0 * rho2 - 17.0 / 384.0 * s2 * rho2 + 11.0 / 768.0 * s2 * s2 +
+                               23.0 / 768.0 * rho2 * rho2 + (s2 * rho2 * rho2 - s2 * s2 * rho2) / 256.0 +
+                               (s2 * s2 * s2 - rho2 * rho2 * rho2) / 768.0;
+                }
+                v = -0.25 * sig / s * sub - sig * sqrt(2.0) / sqrt(kPi) * poly * exp(-0.5 * (s2 + rho2));
+            }
+        }
+        g[e * geo.nf] = v;
+    }
+}
+
+__global__ void fill_0unb(double* g, FillGeom geo, GreenArgs A, int lo0, int lo1, int lo2) {
+    const long long tot = (long long)geo.n[0] * geo.n[1] * geo.n[2];
+    const int ord = lgf_order(A.type);
+    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < tot; e += (long long)gridDim.x * blockDim.x) {
+        int i[3];
+        unflat(e, geo, i);
+        if (e == 0 && A.koffset[0] + A.koffset[1] + A.koffset[2] < 0.2) {
+            g[0] = 0.0;
+            continue;
+        }
+        if (i[0] < lo0 || i[1] < lo1 || i[2] < lo2) continue;
+        const int il[3] = {symid(i[0], A.symstart[0]), symid(i[1], A.symstart[1]), symid(i[2], A.symstart[2])};
+        const double k[3] = {(il[0] + A.koffset[0]) * A.kfact[0], (il[1] + A.koffset[1]) * A.kfact[1], (il[2] + A.koffset[2]) * A.kfact[2]};
+        const double ksq = k[0] * k[0] + k[1] * k[1] + k[2] * k[2];
+        double v;
+        if (ord) {
+            v = -A.hgrid * A.hgrid / (lgf_symbol(ord, k[0] * A.hgrid) + lgf_symbol(ord, k[1] * A.hgrid) + lgf_symbol(ord, k[2] * A.hgrid));
+        } else if (A.type == CHAT_2 || A.type == HEJ_0) {
+            v = -1.0 / ksq;
+        } else {
+            const double hh = 0.5 * ksq * A.length * A.length;
+            const int jmax = A.type == HEJ_2 ? 0 : A.type == HEJ_4 ? 1 : A.type == HEJ_6 ? 2 : A.type == HEJ_8 ? 3 : 4;
+            double p = 1.0, term = 1.0;
+            for (int j = 1; j <= jmax; ++j) { term *= hh / j; p += term; }
+            v = -p * exp(-hh) / ksq;
+        }
+        g[e * geo.nf] = v;
+    }
+}
+
+__global__ void scale_kernel(double* g, size_t n, double f) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) g[i] *= f;
+}
+
+// the fully spectral zero mode after the 2-unbounded/1-spectral completion (green_functions.cpp:703-706)
+__global__ void zero_first(double* g) { g[0] = 0.0; }
+
+__global__ void mult_std(double* d, const double* g, size_t per, int nf, int lda, double normfact) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < per; i += (size_t)gridDim.x * blockDim.x) {
+        for (int c = 0; c < lda; ++c) {
+            double* x = d + c * per * nf;
+            if (nf == 1) {
+                x[i] *= normfact * g[i];
+            } else {
+                const double a = x[2 * i], b = x[2 * i + 1], gr = g[2 * i], gi = g[2 * i + 1];
+                x[2 * i] = normfact * (a * gr - b * gi);
+                x[2 * i + 1] = normfact * (a * gi + b * gr);
+            }
+        }
+    }
+}
+
+__global__ void mult_rot(double* d, const double* g, RotArgs R) {
+    const size_t per = (size_t)R.n[0] * R.n[1] * R.n[2];
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < per; e += (size_t)gridDim.x * blockDim.x) {
+        const int i0 = (int)(e % R.n[0]), i1 = (int)((e / R.n[0]) % R.n[1]), i2 = (int)(e / ((size_t)R.n[0] * R.n[1]));
+        const int is[3] = {symid(i0, R.symstart[0]), symid(i1, R.symstart[1]), symid(i2, R.symstart[2])};
+        double Kr[3][3], Ki[3][3];
+        for (int dd = 0; dd < 3; ++dd)
+            for (int c = 0; c < 3; ++c) {
+                const double base = is[dd] + R.koffset[dd];
+                const double hh = R.h[dd];
+                auto sym = [&](double k) -> double {
+                    switch (R.order) {
+                        case SPE: return k;
+                        case FD2: return sin(k * hh) / hh;
+                        case FD4: return (4.0 / 3.0) * sin(k * hh) / hh - (1.0 / 6.0) * sin(2.0 * k * hh) / hh;
+                        default: return 1.5 * sin(k * hh) / hh - 0.3 * sin(2.0 * k * hh) / hh + (1.0 / 30.0) * sin(3.0 * k * hh) / hh;
+                    }
+                };
+                if (R.nf == 1) {
+                    Kr[dd][c] = sym(base * (R.kf[dd][c][0] + R.kf[dd][c][1]));
+                    Ki[dd][c] = 0.0;
+                } else {
+                    Kr[dd][c] = sym(base * R.kf[dd][c][0]);
+                    Ki[dd][c] = sym(base * R.kf[dd][c][1]);
+                }
+            }
+        double fr[3], fi[3];
+        for (int c = 0; c < 3; ++c) {
+            fr[c] = d[(c * per + e) * R.nf];
+            fi[c] = R.nf == 2 ? d[(c * per + e) * 2 + 1] : 0.0;
+        }
+        auto dr = [&](int c, int dd) { return fr[c] * Kr[dd][c] - fi[c] * Ki[dd][c]; };
+        auto di = [&](int c, int dd) { return fr[c] * Ki[dd][c] + fi[c] * Kr[dd][c]; };
+        const double rr[3] = {dr(2, 1) - dr(1, 2), dr(0, 2) - dr(2, 0), dr(1, 0) - dr(0, 1)};
+        const double ri[3] = {di(2, 1) - di(1, 2), di(0, 2) - di(2, 0), di(1, 0) - di(0, 1)};
+        if (R.nf == 1) {
+            for (int c = 0; c < 3; ++c) d[c * per + e] = R.normfact * rr[c] * g[e];
+        } else {
+            const double gr = g[2 * e], gi = g[2 * e + 1];
+            for (int c = 0; c < 3; ++c) {
+                d[(c * per + e) * 2] = R.normfact * (rr[c] * gr - ri[c] * gi);
+                d[(c * per + e) * 2 + 1] = R.normfact * (rr[c] * gi + ri[c] * gr);
+            }
+        }
+    }
+}
+
+unsigned grid_for(size_t n) {
+    size_t b = (n + 255) / 256;
+    if (b > 65536) b = 65536;
+    if (b == 0) b = 1;
+    return (unsigned)b;
+}
+
+}  // namespace
+
+void launch_green_fill(int nunb, double* g, const int n[3], int nf, int axis, const GreenArgs& A, const int* lo,
+                       hipStream_t s) {
+    FillGeom geo;
+    for (int i = 0; i < 3; ++i) geo.n[i] = n[i];
+    geo.nf = nf;
+    geo.axis = axis;
+    const size_t tot = (size_t)n[0] * n[1] * n[2];
+    const unsigned gb = grid_for(tot);
+    if (nunb == 3) hipLaunchKernelGGL(fill_3unb, dim3(gb), dim3(256), 0, s, g, geo, A);
+    else if (nunb == 2) hipLaunchKernelGGL(fill_2unb, dim3(gb), dim3(256), 0, s, g, geo, A);
+    else if (nunb == 1) hipLaunchKernelGGL(fill_1unb, dim3(gb), dim3(256), 0, s, g, geo, A);
+    else hipLaunchKernelGGL(fill_0unb, dim3(gb), dim3(256), 0, s, g, geo, A, lo ? lo[0] : 0, lo ? lo[1] : 0, lo ? lo[2] : 0);
+    HIP_CHECK(hipGetLastError());
+}
+
+void launch_scale(double* g, size_t n, double f, hipStream_t s) {
+    hipLaunchKernelGGL(scale_kernel, dim3(grid_for(n)), dim3(256), 0, s, g, n, f);
+    HIP_CHECK(hipGetLastError());
+}
+
+void launch_mult_std(double* d, const double* g, size_t per, int nf, int lda, double normfact, hipStream_t s) {
+    hipLaunchKernelGGL(mult_std, dim3(grid_for(per)), dim3(256), 0, s, d, g, per, nf, lda, normfact);
+    HIP_CHECK(hipGetLastError());
+}
+
+void launch_mult_rot(double* d, const double* g, const RotArgs& R, hipStream_t s) {
+    const size_t per = (size_t)R.n[0] * R.n[1] * R.n[2];
+    hipLaunchKernelGGL(mult_rot, dim3(grid_for(per)), dim3(256), 0, s, d, g, R);
+    HIP_CHECK(hipGetLastError());
+}
+
+}  // namespace flups

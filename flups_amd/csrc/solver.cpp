@@ -1,0 +1,549 @@
+// Solver set-up and solve orchestration for one GPU.
+//
+//   constructor: plan creation, sorting and dry run (src/Solver.cpp:23-202, 589-811)
+//   setup:       Green's function on the device (src/Solver.cpp:219-469, 980-1193)
+//   solve:       copy-in, forward passes, multiply, backward passes, copy-out
+//                (src/Solver.cpp:1207-1272, 1402-1542)
+#include "solver.hpp"
+
+#include <dlfcn.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <sstream>
+
+namespace flups {
+
+size_t generic_lds_bytes(const PassDesc& P, int ppb);
+void launch_generic(const PassDesc& P, hipStream_t s);
+FastPlan* make_fast_plan(SolverImpl& s);   // nullptr if the fused pipeline does not apply
+void destroy_fast_plan(FastPlan* f);
+void fast_solve(FastPlan* f, double* field, const double* rhs, const View& uin, const View& uout, hipStream_t s);
+
+// ---------------------------------------------------------------------------
+// topology sizes (src/Topology.cpp:81-98): the fastest direction is padded so
+// that a row is a whole number of `alignment` bytes
+void Topo::cmpt_sizes() {
+    for (int d = 0; d < 3; ++d) {
+        nloc[d] = nglob[d];
+        nmem[d] = nloc[d];
+        if (d == axis) {
+            const int modulo = (int)((nloc[d] * nf * sizeof(double)) % alignment);
+            const int delta = (int)((alignment - modulo) / sizeof(double));
+            nmem[d] += (modulo == 0) ? 0 : delta / nf;
+        }
+    }
+}
+
+std::string kernel_dir() {
+    if (const char* e = std::getenv("FLUPS_KERNEL_PATH")) return e;
+    Dl_info info;
+    if (dladdr((void*)&kernel_dir, &info) && info.dli_fname) {
+        std::string p = info.dli_fname;
+        const size_t s = p.find_last_of('/');
+        return (s == std::string::npos ? std::string(".") : p.substr(0, s)) + "/kernel";
+    }
+    return "kernel";
+}
+
+View dense_view(double* p, const Shape& sh) {
+    View v;
+    v.p = p;
+    for (int d = 0; d < 3; ++d) v.n[d] = sh.n[d];
+    v.nf = sh.nf;
+    v.s[0] = 1;
+    v.s[1] = sh.n[0];
+    v.s[2] = (long long)sh.n[0] * sh.n[1];
+    v.sc = (long long)sh.per();
+    return v;
+}
+
+View SolverImpl::user_view(double* p) const {
+    View v;
+    v.p = p;
+    for (int d = 0; d < 3; ++d) v.n[d] = phys_.nglob[d];
+    v.nf = 1;
+    const int a = phys_.axis, a1 = (a + 1) % 3, a2 = (a + 2) % 3;
+    v.s[a] = 1;
+    v.s[a1] = phys_.nmem[a];
+    v.s[a2] = (long long)phys_.nmem[a] * phys_.nmem[a1];
+    v.sc = (long long)phys_.memdim();
+    return v;
+}
+
+// ---------------------------------------------------------------------------
+SolverImpl::SolverImpl(const Topo* t, const int* bc, const double h[3], const double L[3], int odiff, const int center[3])
+    : phys_(*t), lda_(t->lda), odiff_(odiff) {
+    FLUPS_REQUIRE(center[0] == center[1] && center[0] == center[2], "all directions must share the same data location");
+    FLUPS_REQUIRE(t->nproc[0] * t->nproc[1] * t->nproc[2] == 1, "this library runs on one GPU: nproc must be 1,1,1");
+    FLUPS_REQUIRE(lda_ >= 1 && lda_ <= 3, "lda must be 1..3");
+    node_ = (center[0] == 0);
+    for (int d = 0; d < 3; ++d) { h_[d] = h[d]; L_[d] = L[d]; }
+    auto bcp = [&](int d, int s) { return bc + (d * 2 + s) * lda_; };
+    int dbc[3][2][3];
+    for (int d = 0; d < 3; ++d)
+        for (int s = 0; s < 2; ++s)
+            for (int c = 0; c < lda_; ++c) {
+                const int v = bcp(d, s)[c];
+                dbc[d][s][c] = (v == EVEN) ? ODD : (v == ODD) ? EVEN : v;
+            }
+    std::vector<DirPlan> f, b, g;
+    for (int d = 0; d < 3; ++d) {
+        f.push_back(make_dirplan(d, lda_, bcp(d, 0), bcp(d, 1), h_, L_, FWD, false, node_));
+        b.push_back(make_dirplan(d, lda_, bcp(d, 0), bcp(d, 1), h_, L_, BWD, false, node_));
+        g.push_back(make_dirplan(d, 1, bcp(d, 0), bcp(d, 1), h_, L_, FWD, true, node_));
+    }
+    auto by_type = [](const DirPlan& a, const DirPlan& c) { return a.type < c.type; };   // src/FFTW_plan_dim.cpp:35-51
+    std::stable_sort(f.begin(), f.end(), by_type);
+    std::stable_sort(b.begin(), b.end(), by_type);
+    std::stable_sort(g.begin(), g.end(), by_type);
+    for (int i = 0; i < 3; ++i) { fwd_[i] = f[i]; bwd_[i] = b[i]; gpl_[i] = g[i]; }
+    if (odiff_ != NOD)
+        for (int i = 0; i < 3; ++i) {
+            const int d = bwd_[i].dim;
+            bwdd_[i] = make_dirplan(d, lda_, dbc[d][0], dbc[d][1], h_, L_, BWD, false, node_);
+        }
+    ndim_ = 3;
+    for (int i = 0; i < 3; ++i)
+        if (fwd_[i].type == EMPTY) --ndim_;
+    FLUPS_REQUIRE(ndim_ > 1, "only 2-D and 3-D fields are handled");
+
+    // dry runs (src/Solver.cpp:629-717)
+    auto dry = [&](DirPlan* plans, bool record) {
+        int size[3] = {phys_.nglob[0], phys_.nglob[1], phys_.nglob[2]};
+        bool cplx = false;
+        for (int ip = 0; ip < ndim_; ++ip) {
+            init_dirplan(plans[ip], size, cplx);
+            const int before = size[plans[ip].dim];
+            (void)before;
+            size[plans[ip].dim] = plans[ip].n_out;
+            cplx = cplx || plans[ip].r2c;
+            if (record) {
+                Shape s;
+                for (int d = 0; d < 3; ++d) s.n[d] = size[d];
+                s.nf = cplx ? 2 : 1;
+                s.lda = lda_;
+                stage_[ip] = s;
+                if (plans[ip].r2c) s.to_real(plans[ip].dim);
+                hat_[ip] = s;
+            }
+        }
+        return std::make_pair(std::vector<int>(size, size + 3), cplx);
+    };
+    dry(fwd_, true);
+    dry(bwd_, false);
+    if (odiff_ != NOD) dry(bwdd_, false);
+    auto gfinal = dry(gpl_, false);
+    // Green's topologies, built backwards (src/Solver.cpp:731-789)
+    {
+        std::vector<int> size = gfinal.first;
+        bool cplx = gfinal.second;
+        for (int ip = ndim_ - 1; ip >= 0; --ip) {
+            Shape s;
+            for (int d = 0; d < 3; ++d) s.n[d] = size[d];
+            s.nf = cplx ? 2 : 1;
+            s.lda = 1;
+            Shape post = s;
+            if (gpl_[ip].r2c_by_fft()) {
+                s.to_real(gpl_[ip].dim);
+                size[gpl_[ip].dim] *= 2;
+                cplx = false;
+            }
+            gpre_[ip] = s;
+            gpost_[ip] = post;
+        }
+    }
+    normfact_ = 1.0;
+    volfact_ = 1.0;
+    for (int i = 0; i < 3; ++i) { normfact_ *= fwd_[i].normfact; volfact_ *= fwd_[i].volfact; }
+    int dev = 0;
+    HIP_CHECK(hipGetDevice(&dev));
+    int lds = 0;
+    HIP_CHECK(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    lds_budget_ = (size_t)lds;
+    kdir_ = kernel_dir();
+}
+
+SolverImpl::~SolverImpl() {
+    if (fast_) destroy_fast_plan(fast_);
+    for (auto& kv : tw_) (void)hipFree(kv.second);
+    for (double* p : dstage_) if (p) (void)hipFree(p);
+    if (dgreen_) (void)hipFree(dgreen_);
+    if (dstaging_) (void)hipFree(dstaging_);
+    if (ev0_) (void)hipEventDestroy(ev0_);
+    if (ev1_) (void)hipEventDestroy(ev1_);
+}
+
+size_t SolverImpl::alloc_size() const {
+    size_t m = 1;
+    for (int ip = 0; ip < ndim_; ++ip) m = std::max(m, std::max(hat_[ip].count(), stage_[ip].count()));
+    return std::max(m, phys_.memsize());
+}
+
+void SolverImpl::spectral_info(double kf[3], double ko[3], double ss[3]) const {
+    for (int ip = 0; ip < 3; ++ip) {
+        const int d = fwd_[ip].dim;
+        kf[d] = fwd_[ip].kfact;
+        ko[d] = fwd_[ip].koffset;
+        ss[d] = fwd_[ip].symstart;
+    }
+}
+
+// ---------------------------------------------------------------------------
+const double* SolverImpl::twiddles(int M) {
+    auto it = tw_.find(M);
+    if (it != tw_.end()) return it->second;
+    std::vector<double> t(2 * (size_t)M);
+    for (int i = 0; i < M; ++i) {
+        const long double a = -2.0L * 3.14159265358979323846264338327950288L * (long double)i / (long double)M;
+        t[2 * i] = (double)cosl(a);
+        t[2 * i + 1] = (double)sinl(a);
+    }
+    double* d = nullptr;
+    HIP_CHECK(hipMalloc(&d, t.size() * sizeof(double)));
+    HIP_CHECK(hipMemcpy(d, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice));
+    tw_[M] = d;
+    return d;
+}
+
+void SolverImpl::plan_fft(PassDesc& P, int M) {
+    P.M = M;
+    P.nst = 0;
+    int m = M;
+    for (int p : {4, 2, 3, 5, 7}) {
+        while (m % p == 0) { P.rad[P.nst++] = p; m /= p; }
+    }
+    for (int p = 11; m > 1; p += 2) {
+        while (m % p == 0) { P.rad[P.nst++] = p; m /= p; }
+    }
+    FLUPS_REQUIRE(P.nst <= 16, "transform length %d has too many factors", M);
+    P.tw = twiddles(M);
+}
+
+PassDesc SolverImpl::pass_desc(const DirPlan& p, int comp, bool forward, const View& src, const View& dst, const Shape& tin,
+                               const Shape& tout, int gshift, int sshift) {
+    PassDesc P;
+    P.d = p.dim;
+    P.sign = p.sign;
+    P.n_out_plan = p.n_out;
+    P.post = p.post[comp];
+    P.n = p.n_in[comp];
+    P.si = p.start_in[comp];
+    P.so = p.start_out[comp];
+    P.kind = p.kind[comp];
+    const int d = p.dim;
+    P.pen_in = tin.n[d];
+    P.nf_in = tin.nf;
+    P.pen_out = tout.n[d];
+    P.nf_out = tout.nf;
+    FLUPS_REQUIRE(P.pen_in * P.nf_in == P.pen_out * P.nf_out, "pencil size mismatch");
+    P.gshift = gshift;
+    P.sshift = sshift;
+    int M = 1;
+    if (p.type == SYMSYM || p.type == MIXUNB) {
+        P.op = OP_R2R;
+        P.r2r_stride = tin.nf;
+        switch (P.kind) {
+            case REDFT00: M = 2 * (P.n - 1); break;
+            case RODFT00: M = 2 * (P.n + 1); break;
+            default: M = 2 * P.n;
+        }
+        FLUPS_REQUIRE(P.n >= 1 && M >= 1, "degenerate real-to-real transform of length %d", P.n);
+    } else if (p.r2c) {
+        P.op = forward ? OP_R2C : OP_C2R;
+        M = (P.n % 2 == 0) ? P.n / 2 : P.n;
+    } else {
+        P.op = OP_C2C;
+        M = P.n;
+    }
+    plan_fft(P, M);
+    P.src = src.p + comp * src.sc;
+    P.src_nd = src.n[d];
+    P.src_nf = src.nf;
+    P.dst = dst.p + comp * dst.sc;
+    P.dst_nd = dst.n[d];
+    P.dst_nf = dst.nf;
+    for (int i = 0; i < 3; ++i) { P.src_s[i] = src.s[i]; P.dst_s[i] = dst.s[i]; }
+    P.n1 = tin.n[(d + 1) % 3];
+    P.n2 = tin.n[(d + 2) % 3];
+    const size_t per_pen = generic_lds_bytes(P, 1);
+    FLUPS_REQUIRE(per_pen <= lds_budget_, "pencil of %d elements (transform length %d) does not fit the general path's LDS",
+                  P.pen_in, M);
+    int ppb = (int)std::min<size_t>(16, lds_budget_ / std::max<size_t>(per_pen, 1));
+    while (ppb > 1 && generic_lds_bytes(P, ppb) > lds_budget_) --ppb;
+    P.ppb = std::max(1, std::min(ppb, P.n1 * P.n2));
+    return P;
+}
+
+void SolverImpl::run_pass(PassDesc P) { launch_generic(P, stream_); }
+
+// ---------------------------------------------------------------------------
+void SolverImpl::fill_green_args(GreenArgs& A, const double hfact[3], const double kfact[3], const double koffset[3],
+                                 const double symstart[3], double length, int nunb) {
+    A.type = gtype_;
+    A.length = length;
+    A.hgrid = h_[0];
+    for (int d = 0; d < 3; ++d) {
+        A.hfact[d] = hfact[d];
+        A.kfact[d] = kfact[d];
+        A.koffset[d] = koffset[d];
+        A.symstart[d] = symstart[d];
+    }
+    if (nunb == 3) {
+        if (gtype_ == CHAT_2) {
+            const double vol = hfact[0] * hfact[1] * hfact[2];
+            A.g0 = -0.5 * std::pow(1.5 / (2.0 * M_PI) * vol, 2.0 / 3.0) / vol;
+        } else if (gtype_ >= HEJ_2 && gtype_ <= HEJ_10) {
+            static const double mult[5] = {1.0 / 4.0, 3.0 / 8.0, 15.0 / 32.0, 35.0 / 64.0, 315.0 / 512.0};
+            A.g0 = -M_SQRT2 * mult[gtype_ - HEJ_2] / (length * std::sqrt(M_PI * M_PI * M_PI));
+        }
+    }
+    // stencil coefficients of the 1-D lattice Green's functions: s_m (symbol s_0 + 2 sum s_m cos(m t))
+    // and p_j, the coefficients of s(t) in powers of mu = cos(t) - 1 (Chebyshev T_m(1 + mu))
+    static const double S[4][5] = {{2.0, -1.0, 0, 0, 0},
+                                   {5.0 / 2.0, -4.0 / 3.0, 1.0 / 12.0, 0, 0},
+                                   {49.0 / 18.0, -3.0 / 2.0, 3.0 / 20.0, -1.0 / 90.0, 0},
+                                   {205.0 / 72.0, -8.0 / 5.0, 1.0 / 5.0, -8.0 / 315.0, 1.0 / 560.0}};
+    for (int o = 0; o < 4; ++o) {
+        const int w = o + 1;
+        std::vector<std::vector<double>> T(w + 1, std::vector<double>(w + 1, 0.0));
+        T[0][0] = 1.0;
+        if (w >= 1) { T[1][0] = 1.0; T[1][1] = 1.0; }
+        for (int m = 1; m < w; ++m)   // T_{m+1} = 2 (1 + mu) T_m - T_{m-1}
+            for (int j = 0; j <= w; ++j) {
+                double v = 2.0 * T[m][j] - T[m - 1][j];
+                if (j > 0) v += 2.0 * T[m][j - 1];
+                T[m + 1][j] = v;
+            }
+        for (int j = 0; j <= w; ++j) {
+            double pj = (j == 0) ? S[o][0] : 0.0;
+            for (int m = 1; m <= w; ++m) pj += 2.0 * S[o][m] * T[m][j];
+            A.lgf_p[5 * o + j] = (j == 0) ? 0.0 : pj;   // p_0 = s(0) = 0 exactly
+            A.lgf_s[5 * o + j] = S[o][j];
+        }
+    }
+}
+
+namespace {
+std::vector<double> read_table(const std::string& path, size_t n) {
+    std::vector<double> v(n);
+    FILE* f = std::fopen(path.c_str(), "rb");
+    FLUPS_REQUIRE(f != nullptr, "unable to read Green's table %s", path.c_str());
+    const size_t got = std::fread(v.data(), sizeof(double), n, f);
+    std::fclose(f);
+    FLUPS_REQUIRE(got == n, "short Green's table %s", path.c_str());
+    return v;
+}
+std::vector<ExpTerm> read_expansion(const std::string& path, const std::string& name) {
+    std::ifstream f(path);
+    FLUPS_REQUIRE((bool)f, "unable to read %s", path.c_str());
+    std::vector<ExpTerm> out;
+    std::string line;
+    while (std::getline(f, line)) {
+        if (line.empty() || line[0] == '#') continue;
+        std::istringstream is(line);
+        std::string nm;
+        ExpTerm t;
+        is >> nm >> t.a >> t.b >> t.c >> t.p >> t.l >> t.coef;
+        if (nm == name) out.push_back(t);
+    }
+    FLUPS_REQUIRE(!out.empty(), "no expansion %s in %s", name.c_str(), path.c_str());
+    return out;
+}
+}  // namespace
+
+void SolverImpl::compute_green() {
+    bool spec[3] = {false, false, false};
+    double hfact[3] = {0, 0, 0}, kfact[3] = {0, 0, 0}, koffset[3] = {0, 0, 0}, symstart[3] = {0, 0, 0};
+    double length = alpha_ * h_[0];
+    if (gtype_ == HEJ_0) length = h_[0] / M_PI;
+    const bool iso = (gtype_ >= HEJ_2 && gtype_ <= HEJ_0) || gtype_ == LGF_2;
+    FLUPS_REQUIRE(!(iso && ((ndim_ == 3 && (h_[0] != h_[1] || h_[1] != h_[2])) || (ndim_ == 2 && h_[0] != h_[1]))),
+                  "a regularized kernel or an LGF requires dx = dy = dz");
+    FLUPS_REQUIRE(gtype_ != HEJ_0 && gtype_ < MEHR_4L, "Green's kernel %d is not available on the device yet", gtype_);
+    for (int ip = 0; ip < ndim_; ++ip) {
+        const DirPlan& p = gpl_[ip];
+        const int d = p.dim;
+        spec[d] = p.spectral;
+        symstart[d] = p.symstart;
+        hfact[d] = h_[d];
+        if (p.spectral) { hfact[d] = 0.0; kfact[d] = p.kfact; koffset[d] = p.koffset; }
+        if (p.type == EMPTY) hfact[d] = 0.0;
+    }
+    int nspec = 0;
+    for (int d = 0; d < ndim_; ++d) if (spec[d]) ++nspec;
+    const int nunb = ndim_ - nspec;
+
+    GreenArgs A;
+    fill_green_args(A, hfact, kfact, koffset, symstart, length, nunb);
+    const int ord = (gtype_ == LGF_2) ? 2 : (gtype_ == LGF_4) ? 4 : (gtype_ == LGF_6) ? 6 : (gtype_ == LGF_8) ? 8 : 0;
+    double* dtab = nullptr;
+    ExpTerm* dexp = nullptr;
+    if (ord && (nunb == 3 || nunb == 2)) {
+        const int gd = nunb;
+        char name[256];
+        std::snprintf(name, sizeof(name), "%s/LGF_%d_%dd_32.ker", kdir_.c_str(), ord, gd);
+        const auto tab = read_table(name, gd == 3 ? 32 * 32 * 32 : 32 * 32);
+        char ename[64];
+        std::snprintf(ename, sizeof(ename), "lgf_%d_%dunb0spe", ord, gd);
+        const auto ex = read_expansion(kdir_ + "/lgf_expansion.txt", ename);
+        HIP_CHECK(hipMalloc(&dtab, tab.size() * sizeof(double)));
+        HIP_CHECK(hipMemcpyAsync(dtab, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice, stream_));
+        HIP_CHECK(hipMalloc(&dexp, ex.size() * sizeof(ExpTerm)));
+        HIP_CHECK(hipMemcpyAsync(dexp, ex.data(), ex.size() * sizeof(ExpTerm), hipMemcpyHostToDevice, stream_));
+        HIP_CHECK(hipStreamSynchronize(stream_));
+        A.table = dtab;
+        A.exp = dexp;
+        A.nexp = (int)ex.size();
+    }
+    // fill on the first Green's topology, then forward transforms of the non-spectral directions
+    double* a = nullptr;
+    size_t cap = 0;
+    for (int ip = 0; ip < ndim_; ++ip) cap = std::max(cap, std::max(gpre_[ip].count(), gpost_[ip].count()));
+    HIP_CHECK(hipMalloc(&a, cap * sizeof(double)));
+    HIP_CHECK(hipMemsetAsync(a, 0, cap * sizeof(double), stream_));
+    launch_green_fill(nunb, a, gpre_[0].n, gpre_[0].nf, gpl_[0].dim, A, nullptr, stream_);
+    double* b = nullptr;
+    HIP_CHECK(hipMalloc(&b, cap * sizeof(double)));
+    for (int ip = 0; ip < ndim_; ++ip) {
+        const DirPlan& p = gpl_[ip];
+        if (spec[p.dim]) continue;
+        View vs = dense_view(a, gpre_[ip]);
+        View vd = dense_view(b, gpost_[ip]);
+        PassDesc P = pass_desc(p, 0, true, vs, vd, gpre_[ip], gpost_[ip], 0, 0);
+        run_pass(P);
+        std::swap(a, b);
+    }
+    const Shape& gfin = gpost_[ndim_ - 1];
+    launch_scale(a, gfin.count(), volfact_, stream_);
+    const bool hej = gtype_ >= HEJ_2 && gtype_ <= HEJ_10;
+    const bool lgf = ord != 0;
+    if (ndim_ == 3 && nspec == 1 && (hej || lgf)) {
+        // 2 unbounded + 1 spectral: the spectral kernel away from the k = 0 plane (src/Solver.cpp:1092-1101)
+        int lo[3];
+        for (int ip = 0; ip < 3; ++ip) {
+            lo[ip] = spec[ip] ? 1 : 0;
+            kfact[gpl_[ip].dim] = gpl_[ip].kfact;
+        }
+        GreenArgs B = A;
+        for (int d = 0; d < 3; ++d) B.kfact[d] = kfact[d];
+        launch_green_fill(0, a, gfin.n, gfin.nf, gpl_[ndim_ - 1].dim, B, lo, stream_);
+    }
+    // the field's spectral stage must have the same extents
+    const Shape& fs = stage_[ndim_ - 1];
+    FLUPS_REQUIRE(fs.nf == gfin.nf && fs.n[0] == gfin.n[0] && fs.n[1] == gfin.n[1] && fs.n[2] == gfin.n[2],
+                  "Green's function and field spectral layouts differ");
+    HIP_CHECK(hipStreamSynchronize(stream_));
+    HIP_CHECK(hipFree(b));
+    if (dtab) HIP_CHECK(hipFree(dtab));
+    if (dexp) HIP_CHECK(hipFree(dexp));
+    dgreen_ = a;
+}
+
+void SolverImpl::setup() {
+    FLUPS_REQUIRE(!setup_done_, "setup was already done");
+    HIP_CHECK(hipEventCreate(&ev0_));
+    HIP_CHECK(hipEventCreate(&ev1_));
+    compute_green();
+    fast_ = make_fast_plan(*this);
+    if (!fast_) {
+        for (int ip = 0; ip < ndim_; ++ip) {
+            const size_t n = std::max(hat_[ip].count(), stage_[ip].count());
+            HIP_CHECK(hipMalloc(&dstage_[ip], n * sizeof(double)));
+        }
+    }
+    HIP_CHECK(hipStreamSynchronize(stream_));
+    setup_done_ = true;
+}
+
+// ---------------------------------------------------------------------------
+void SolverImpl::solve_generic(double* field, const double* rhs, int type) {
+    const View uin = user_view(const_cast<double*>(rhs));
+    const View uout = user_view(field);
+    for (int ip = 0; ip < ndim_; ++ip) {
+        const DirPlan& p = fwd_[ip];
+        const View src = (ip == 0) ? uin : dense_view(dstage_[ip - 1], stage_[ip - 1]);
+        const View dst = dense_view(dstage_[ip], stage_[ip]);
+        for (int c = 0; c < lda_; ++c) run_pass(pass_desc(p, c, true, src, dst, hat_[ip], stage_[ip], p.fieldstart, 0));
+    }
+    const Shape& sp = stage_[ndim_ - 1];
+    if (type == 0) {
+        launch_mult_std(dstage_[ndim_ - 1], dgreen_, (size_t)sp.n[0] * sp.n[1] * sp.n[2], sp.nf, lda_, normfact_, stream_);
+    } else {
+        RotArgs R;
+        for (int d = 0; d < 3; ++d) { R.n[d] = sp.n[d]; R.h[d] = h_[d]; }
+        R.nf = sp.nf;
+        R.order = odiff_;
+        R.normfact = normfact_;
+        double kabs[3];
+        spectral_info(kabs, R.koffset, R.symstart);
+        for (int ip = 0; ip < 3; ++ip) {
+            const int d = fwd_[ip].dim;
+            for (int c = 0; c < 3; ++c) {
+                const int ph = (bwdd_[ip].imult[c] ? 1 : 0) - (fwd_[ip].imult[c] ? 1 : 0);
+                R.kf[d][c][0] = ph == 0 ? 0.0 : (ph == 1 ? -kabs[d] : kabs[d]);
+                R.kf[d][c][1] = ph == 0 ? kabs[d] : 0.0;
+            }
+        }
+        launch_mult_rot(dstage_[ndim_ - 1], dgreen_, R, stream_);
+    }
+    const DirPlan* back = (type == 0) ? bwd_ : bwdd_;
+    for (int ip = ndim_ - 1; ip >= 0; --ip) {
+        const DirPlan& p = back[ip];
+        const View src = dense_view(dstage_[ip], stage_[ip]);
+        const View dst = (ip == 0) ? uout : dense_view(dstage_[ip - 1], stage_[ip - 1]);
+        for (int c = 0; c < lda_; ++c)
+            run_pass(pass_desc(p, c, false, src, dst, stage_[ip], hat_[ip], 0, fwd_[ip].fieldstart));
+    }
+}
+
+void SolverImpl::solve(double* field, const double* rhs, int type) {
+    FLUPS_REQUIRE(setup_done_, "flups_setup must be called before flups_solve");
+    FLUPS_REQUIRE(field && rhs, "field and rhs must not be NULL");
+    FLUPS_REQUIRE(type == 0 || type == 1, "unknown solver type %d", type);
+    FLUPS_REQUIRE(type == 0 || lda_ == 3, "the ROT solver needs lda = 3 (vectors)");
+    FLUPS_REQUIRE(type == 0 || odiff_ != NOD, "the ROT solver needs a differentiation order (SPE, FD2, FD4, FD6)");
+    hipPointerAttribute_t at_f, at_r;
+    auto is_dev = [](const void* p, hipPointerAttribute_t* a) {
+        if (hipPointerGetAttributes(a, p) != hipSuccess) { (void)hipGetLastError(); return false; }
+        return a->type == hipMemoryTypeDevice;
+    };
+    const bool fdev = is_dev(field, &at_f), rdev = is_dev(rhs, &at_r);
+    const size_t n = phys_.memsize();
+    double* din = const_cast<double*>(rhs);
+    double* dout = field;
+    if (!fdev || !rdev) {
+        if (staging_n_ < 2 * n) {
+            if (dstaging_) HIP_CHECK(hipFree(dstaging_));
+            HIP_CHECK(hipMalloc(&dstaging_, 2 * n * sizeof(double)));
+            staging_n_ = 2 * n;
+        }
+        if (!rdev) {
+            din = dstaging_;
+            HIP_CHECK(hipMemcpyAsync(din, rhs, n * sizeof(double), hipMemcpyHostToDevice, stream_));
+        }
+        if (!fdev) dout = dstaging_ + n;
+    }
+    HIP_CHECK(hipEventRecord(ev0_, stream_));
+    if (fast_ && type == 0) fast_solve(fast_, dout, din, user_view(din), user_view(dout), stream_);
+    else {
+        if (fast_ && !dstage_[0]) {
+            for (int ip = 0; ip < ndim_; ++ip) {
+                const size_t m = std::max(hat_[ip].count(), stage_[ip].count());
+                HIP_CHECK(hipMalloc(&dstage_[ip], m * sizeof(double)));
+            }
+        }
+        solve_generic(dout, din, type);
+    }
+    HIP_CHECK(hipEventRecord(ev1_, stream_));
+    if (!fdev) HIP_CHECK(hipMemcpyAsync(field, dout, n * sizeof(double), hipMemcpyDeviceToHost, stream_));
+    HIP_CHECK(hipStreamSynchronize(stream_));
+    float ms = 0;
+    HIP_CHECK(hipEventElapsedTime(&ms, ev0_, ev1_));
+    last_ns_ = ms * 1e6;
+}
+
+}  // namespace flups

@@ -1,0 +1,82 @@
+// Solver and Topology objects behind the FLUPS C interface (one GPU, one rank).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "core.hpp"
+#include "green.hpp"
+
+namespace flups {
+
+// user-visible topology (reference: src/Topology.hpp:27-430, single rank)
+struct Topo {
+    int axis = 0, lda = 1, nf = 1, alignment = 8;
+    int nglob[3] = {1, 1, 1}, nloc[3] = {1, 1, 1}, nmem[3] = {1, 1, 1}, nproc[3] = {1, 1, 1}, axproc[3] = {0, 1, 2};
+    void* comm = nullptr;
+    size_t memdim() const { return (size_t)nmem[0] * nmem[1] * nmem[2] * nf; }
+    size_t memsize() const { return memdim() * lda; }
+    size_t locsize() const { return (size_t)nloc[0] * nloc[1] * nloc[2] * nf; }
+    void cmpt_sizes();
+};
+
+struct FastPlan;   // fused unbounded pipeline, fast.hip
+
+class SolverImpl {
+   public:
+    SolverImpl(const Topo* t, const int* bc /*[(d*2+side)*lda + c]*/, const double h[3], const double L[3], int odiff,
+               const int center[3]);
+    ~SolverImpl();
+    void set_green(int t) { gtype_ = t; }
+    void set_alpha(double a) { alpha_ = a; }
+    void set_stream(hipStream_t s) { stream_ = s; }
+    void set_fp32(bool f) { fp32_ = f; }
+    void setup();
+    void solve(double* field, const double* rhs, int type);
+    size_t alloc_size() const;
+    void spectral_info(double kf[3], double ko[3], double ss[3]) const;
+    int path() const { return fast_ ? 1 : 0; }
+    double last_solve_ns() const { return last_ns_; }
+    int ndim() const { return ndim_; }
+
+    // building blocks (also used by the fused pipeline)
+    hipStream_t stream() const { return stream_; }
+
+   private:
+    void compute_green();
+    const double* twiddles(int M);
+    void plan_fft(PassDesc& P, int M);
+    PassDesc pass_desc(const DirPlan& p, int comp, bool forward, const View& src, const View& dst, const Shape& tin,
+                       const Shape& tout, int gshift, int sshift);
+    void run_pass(PassDesc P);
+    void fill_green_args(GreenArgs& A, const double hfact[3], const double kfact[3], const double koffset[3],
+                         const double symstart[3], double length, int nunb);
+    View user_view(double* p) const;
+    void solve_generic(double* field, const double* rhs, int type);
+
+    Topo phys_;
+    int lda_ = 1, ndim_ = 3, odiff_ = NOD, gtype_ = CHAT_2;
+    bool node_ = false, fp32_ = false, setup_done_ = false;
+    double h_[3], L_[3], alpha_ = 2.0, normfact_ = 1, volfact_ = 1;
+    DirPlan fwd_[3], bwd_[3], bwdd_[3], gpl_[3];
+    Shape hat_[3], stage_[3], gpre_[3], gpost_[3];
+    hipStream_t stream_ = nullptr;
+    std::map<int, double*> tw_;
+    double* dstage_[3] = {nullptr, nullptr, nullptr};
+    double* dgreen_ = nullptr;
+    double* dstaging_ = nullptr;   // rhs/field copies for host pointers
+    size_t staging_n_ = 0;
+    size_t lds_budget_ = 65536;
+    FastPlan* fast_ = nullptr;
+    hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
+    double last_ns_ = 0;
+    std::string kdir_;
+    friend struct FastPlan;
+};
+
+std::string kernel_dir();
+
+}  // namespace flups

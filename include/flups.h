@@ -1,0 +1,155 @@
+/*
+ * flups.h -- C interface of the MI355X FLUPS solve path (libflups.so).
+ *
+ * Drop-in for the reference's src/flups.h: every entry point keeps the
+ * reference's name, signature and argument meaning. Each declaration cites the
+ * reference interface it replaces. The library runs on one GPU and one
+ * process: topologies must have nproc = {1,1,1}; the MPI communicator is kept
+ * and returned, never used.
+ *
+ * field/rhs pointers passed to flups_solve may be device pointers (HBM
+ * resident, the fast path) or host pointers (copied over PCIe around the
+ * solve). Work runs on the stream given by flups_set_stream (default stream
+ * otherwise); flups_solve returns with the result complete.
+ *
+ * Errors: entry points keep the reference's void/pointer returns. On a failed
+ * check the library records a message (flups_get_last_error), prints it to
+ * stderr and returns (NULL for constructors); flups_set_abort_on_error(1)
+ * restores the reference's abort-on-failure behaviour.
+ */
+#ifndef FLUPS_H
+#define FLUPS_H
+
+#include <stdbool.h>
+#include <stddef.h>
+
+#include "flups_interface.h"
+
+#if defined(FLUPS_BUILDING_LIBRARY)
+/* pointer-sized slot: holds an MPICH int handle or an Open MPI pointer handle */
+typedef void* MPI_Comm;
+#elif defined(FLUPS_WITH_MPI) || defined(MPI_VERSION)
+#include <mpi.h>
+#else
+typedef int MPI_Comm;
+#ifndef MPI_COMM_WORLD
+#define MPI_COMM_WORLD ((MPI_Comm)0x44000000)
+#endif
+#endif
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct Solver   FLUPS_Solver;    /* src/flups.h:26 */
+typedef struct Topology FLUPS_Topology;  /* src/flups.h:27 */
+typedef struct Profiler FLUPS_Profiler;  /* src/flups.h:28 */
+
+typedef enum BoundaryType FLUPS_BoundaryType;
+typedef enum GreenType    FLUPS_GreenType;
+typedef enum SolverType   FLUPS_SolverType;
+typedef enum DiffType     FLUPS_DiffType;
+typedef enum CenterType   FLUPS_CenterType;
+
+/* ---- memory (src/flups.h:51,61) ---- */
+void* flups_malloc(size_t size);   /* zeroed, FLUPS_ALIGNMENT-aligned host memory */
+void  flups_free(void* data);
+void  flups_info(int argc, char** argv);   /* src/flups.h:69 */
+
+/* index of (i0,i1,i2,lia) given in the axsrc frame, in an array whose fastest
+   index is axtrg (src/flups.h:109-119) */
+static inline size_t flups_locID(const int axsrc, const int i0, const int i1, const int i2, const int lia, const int axtrg,
+                                 const int size[3], const int nf) {
+    const int i[3] = {i0, i1, i2};
+    const int d0 = (3 + axtrg - axsrc) % 3, d1 = (d0 + 1) % 3, d2 = (d0 + 2) % 3;
+    const int a0 = axtrg, a1 = (a0 + 1) % 3, a2 = (a0 + 2) % 3;
+    return (size_t)i[d0] * nf + (size_t)size[a0] * nf * ((size_t)i[d1] + (size_t)size[a1] * ((size_t)i[d2] + (size_t)lia * size[a2]));
+}
+
+/* symmetrised (wave number) index used with flups_get_spectralInfo (src/flups.h:170-184) */
+static inline void flups_symID(const int axsrc, const int i0, const int i1, const int i2, const int istart[3],
+                               const double symstart[3], const int axtrg, int is[3]) {
+    const int ie[3] = {istart[axsrc] + i0, istart[(axsrc + 1) % 3] + i1, istart[(axsrc + 2) % 3] + i2};
+    const int d0 = (3 + axtrg - axsrc) % 3;
+    int k;
+    for (k = 0; k < 3; ++k) {
+        const int ax = (axtrg + k) % 3;
+        const int e  = ie[(d0 + k) % 3];
+        if (symstart[ax] == 0.0 || e <= symstart[ax]) {
+            is[k] = e;
+        } else {
+            double m = 2.0 * symstart[ax] - e;
+            int    v = (int)(m < 0 ? -m : m);
+            is[k]    = v > 1 ? v : 1;
+        }
+    }
+}
+
+/* ---- topologies (src/flups.h:210-349) ---- */
+FLUPS_Topology* flups_topo_new(const int axis, const int lda, const int nglob[3], const int nproc[3], const bool isComplex,
+                               const int axproc[3], const int alignment, MPI_Comm comm);          /* src/flups.h:210 */
+/* alias with the name used in some FLUPS documentation; identical to flups_topo_new */
+FLUPS_Topology* flups_init_topology(const int axis, const int lda, const int nglob[3], const int nproc[3],
+                                    const bool isComplex, const int axproc[3], const int alignment, MPI_Comm comm);
+void   flups_topo_free(const FLUPS_Topology* t);                                                  /* src/flups.h:217 */
+bool   flups_topo_get_isComplex(const FLUPS_Topology* t);                                         /* src/flups.h:226 */
+int    flups_topo_get_lda(const FLUPS_Topology* t);                                               /* src/flups.h:234 */
+int    flups_topo_get_axis(const FLUPS_Topology* t);                                              /* src/flups.h:242 */
+int    flups_topo_get_nglob(const FLUPS_Topology* t, const int dim);                              /* src/flups.h:250 */
+int    flups_topo_get_nloc(const FLUPS_Topology* t, const int dim);                               /* src/flups.h:261 */
+int    flups_topo_get_nmem(const FLUPS_Topology* t, const int dim);                               /* src/flups.h:272 */
+int    flups_topo_get_nproc(const FLUPS_Topology* t, const int dim);                              /* src/flups.h:280 */
+void   flups_topo_get_istartGlob(const FLUPS_Topology* t, int istart[3]);                         /* src/flups.h:287 */
+size_t flups_topo_get_locsize(const FLUPS_Topology* t);                                           /* src/flups.h:294 */
+size_t flups_topo_get_memsize(const FLUPS_Topology* t);                                           /* src/flups.h:301 */
+int    flups_topo_cmpt_rank_fromid(const FLUPS_Topology* t, const int global_id, const int id);    /* src/flups.h:314 */
+int    flups_topo_cmpt_start_id_from_rank(const FLUPS_Topology* t, const int rank_id, const int id); /* src/flups.h:321 */
+MPI_Comm flups_topo_get_comm(FLUPS_Topology* t);                                                  /* src/flups.h:329 */
+void   flups_topo_ranksplit(const FLUPS_Topology* t, const int rank, int rankd[3]);               /* src/flups.h:340 */
+int    flups_topo_rankindex(const FLUPS_Topology* topo, const int rankd[3]);                      /* src/flups.h:349 */
+
+/* ---- solver (src/flups.h:369-452) ---- */
+FLUPS_Solver* flups_init(FLUPS_Topology* t, FLUPS_BoundaryType* bc[3][2], const double h[3], const double L[3],
+                         FLUPS_DiffType orderDiff, const FLUPS_CenterType center_type[3]);        /* src/flups.h:369 */
+FLUPS_Solver* flups_init_timed(FLUPS_Topology* t, FLUPS_BoundaryType* bc[3][2], const double h[3], const double L[3],
+                               const FLUPS_DiffType orderDiff, const FLUPS_CenterType center_type[3],
+                               FLUPS_Profiler* prof);                                             /* src/flups.h:375 */
+void flups_cleanup(FLUPS_Solver* s);                                                              /* src/flups.h:384 */
+void flups_cleanup_solver(FLUPS_Solver* s);                                                       /* src/flups.h:392 */
+void flups_cleanup_all(FLUPS_Solver* s);                                                          /* src/flups.h:401 */
+void flups_cleanup_backend(void);                                                                 /* src/flups.h:410 */
+void flups_set_greenType(FLUPS_Solver* s, const FLUPS_GreenType type);                            /* src/flups.h:421 */
+/* stream = a hipStream_t; the reference's hook is a no-op (src/flups.cpp:147) */
+void flups_set_stream(FLUPS_Solver* s, void* stream);                                             /* src/flups.h:429 */
+void flups_setup(FLUPS_Solver* s, const bool changeComm);                                         /* src/flups.h:441 */
+void flups_solve(FLUPS_Solver* s, double* field, double* rhs, const FLUPS_SolverType type);        /* src/flups.h:452 */
+
+/* ---- advanced (src/flups.h:467-578) ---- */
+size_t flups_get_allocSize(FLUPS_Solver* s);                                                      /* src/flups.h:473 */
+void   flups_get_spectralInfo(FLUPS_Solver* s, double kfact[3], double koffset[3], double symstart[3]); /* src/flups.h:485 */
+void   flups_set_alpha(FLUPS_Solver* s, const double alpha);                                      /* src/flups.h:494 */
+void   flups_pencilDirs(const FLUPS_BoundaryType* bc[3][2], int dirs[3]);                          /* src/flups.h:570 */
+
+/* ---- profiler handles (src/flups.h:592-611): accepted, reports solve timings ---- */
+FLUPS_Profiler* flups_profiler_new(void);
+FLUPS_Profiler* flups_profiler_new_n(const char name[]);
+void            flups_profiler_free(FLUPS_Profiler* p);
+void            flups_profiler_disp(FLUPS_Profiler* p);
+
+/* ---- additions of this library (no counterpart in the reference) ---- */
+const char* flups_get_last_error(void);        /* message of the last failed check, "" if none */
+void        flups_clear_last_error(void);
+void        flups_set_abort_on_error(int on);  /* 1: abort() on a failed check, as the reference */
+/* 0 = FP64 throughout (default), 1 = FP32 transforms and multiply with FP64-computed
+   Green's function; caller buffers stay double. Call before flups_setup. */
+void        flups_set_precision(FLUPS_Solver* s, int fp32);
+/* 1 if the solver runs the fused unbounded pipeline, 0 for the general pencil path */
+int         flups_get_path(FLUPS_Solver* s);
+/* nanoseconds of device time of the last flups_solve (HIP events on the solver's stream) */
+double      flups_get_last_solve_ns(FLUPS_Solver* s);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

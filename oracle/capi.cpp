@@ -32,6 +32,22 @@ int oracle_vtube(int vdir, int symx, int symy, int N, int kernel, int order, dou
     } catch (const std::exception& e) { return fail(e); }
 }
 
+// manufactured rhs / exact solution of a validation case, and its grid
+int oracle_validation_fields(int center, const int* bc6, int N, int lda, int* nglob, double* h, double* rhs, double* sol) {
+    try {
+        validation_geometry(center, bc6, N, nglob, h);
+        if (rhs && sol) validation_fields(center, bc6, N, lda, rhs, sol);
+        return 0;
+    } catch (const std::exception& e) { return fail(e); }
+}
+// vortex-tube rhs / exact solution and its [3][2][3] boundary conditions
+int oracle_vtube_fields(int vdir, int symx, int symy, int N, double sigma, int* bc, double* rhs, double* sol) {
+    try {
+        vtube_case(vdir, symx, symy, N, 0, 1, sigma, ".", nullptr, bc, rhs, sol);
+        return 0;
+    } catch (const std::exception& e) { return fail(e); }
+}
+
 // bc: [3][2][lda] ints; rhs/field: dense, dim 0 fastest, components slowest
 int oracle_solve(const int* nglob, const int* bc, int lda, const double* h, const double* L, int center, int odiff, int kernel,
                  double alpha, const char* kdir, int type, const double* rhs, double* field) {

@@ -115,6 +115,9 @@ struct Solver {
 // validation harness of samples/validation (manufactured solutions)
 void validation_case(int center, const int bc6[6], int kernel, int N, int lda, const std::string& kdir, double* err /*[2*lda]*/);
 // vortex-tube ROT harness of samples/solve_vtube
-void vtube_case(int vdir, int symx, int symy, int N, int kernel, int order, double sigma, const std::string& kdir, double* err /*[6]*/);
+void vtube_case(int vdir, int symx, int symy, int N, int kernel, int order, double sigma, const std::string& kdir, double* err /*[6]*/,
+                int* bc_out = nullptr, double* rhs_out = nullptr, double* sol_out = nullptr);
+void validation_geometry(int center, const int bc6[6], int N, int nglob[3], double h[3]);
+void validation_fields(int center, const int bc6[6], int N, int lda, double* rhs, double* sol);
 
 }  // namespace oracle

@@ -4,6 +4,7 @@
 // samples/solve_vtube/data_ref (solve_vtube/src/vtube.cpp: vortex tubes with
 // image tubes, ROT solve; main_vtube.cpp:110-200 for the boundary conditions).
 // Test infrastructure only.
+#include <algorithm>
 #include <cmath>
 #include <stdexcept>
 #include <vector>
@@ -59,24 +60,25 @@ double frhs(FKind k, double x, double L, const Manu& p) {
 
 }  // namespace
 
-void validation_case(int center, const int bc6[6], int kernel, int N, int lda, const std::string& kdir, double* err) {
+void validation_geometry(int center, const int bc6[6], int N, int nglob[3], double h[3]) {
     const bool cell = (center == 1);
     const int fact = cell ? 0 : 1;
     const bool twod = (bc6[4] == NONE);
     const int res = N - fact;
-    int nglob[3] = {res + fact, res + fact, twod ? 1 : res + fact};
-    const double L[3] = {1.0, 1.0, 1.0};
-    double h[3] = {L[0] / (nglob[0] - fact), L[1] / (nglob[1] - fact), nglob[2] == 1 ? 1.0 : L[2] / (nglob[2] - fact)};
-    std::vector<int> bc(6 * lda);
-    for (int f = 0; f < 6; ++f)
-        for (int c = 0; c < lda; ++c) bc[f * lda + c] = bc6[f];
-    Solver s(nglob, bc.data(), lda, h, L, !cell, NOD);
-    s.kernel_dir = kdir;
-    s.set_green((GreenType)kernel);
-    s.setup();
+    nglob[0] = res + fact; nglob[1] = res + fact; nglob[2] = twod ? 1 : res + fact;
+    h[0] = 1.0 / (nglob[0] - fact);
+    h[1] = 1.0 / (nglob[1] - fact);
+    h[2] = nglob[2] == 1 ? 1.0 : 1.0 / (nglob[2] - fact);
+}
 
+void validation_fields(int center, const int bc6[6], int N, int lda, double* rhs_out, double* sol_out) {
+    const bool cell = (center == 1);
+    int nglob[3];
+    double h[3];
+    validation_geometry(center, bc6, N, nglob, h);
+    const double L[3] = {1.0, 1.0, 1.0};
     const size_t per = (size_t)nglob[0] * nglob[1] * nglob[2];
-    std::vector<double> rhs(per * lda, 0.0), sol(per * lda, 1.0), field(per * lda, 0.0);
+    std::vector<double> rhs(per * lda, 0.0), sol(per * lda, 1.0);
     for (int lia = 0; lia < lda; ++lia) {
         Manu p[3];
         p[0].freq = 1; p[1].freq = 2; p[2].freq = 4;
@@ -112,6 +114,26 @@ void validation_case(int center, const int bc6[6], int kernel, int N, int lda, c
                     }
                 }
     }
+    std::copy(rhs.begin(), rhs.end(), rhs_out);
+    std::copy(sol.begin(), sol.end(), sol_out);
+}
+
+void validation_case(int center, const int bc6[6], int kernel, int N, int lda, const std::string& kdir, double* err) {
+    const bool cell = (center == 1);
+    int nglob[3];
+    double h[3];
+    validation_geometry(center, bc6, N, nglob, h);
+    const double L[3] = {1.0, 1.0, 1.0};
+    std::vector<int> bc(6 * lda);
+    for (int f = 0; f < 6; ++f)
+        for (int c = 0; c < lda; ++c) bc[f * lda + c] = bc6[f];
+    Solver s(nglob, bc.data(), lda, h, L, !cell, NOD);
+    s.kernel_dir = kdir;
+    s.set_green((GreenType)kernel);
+    s.setup();
+    const size_t per = (size_t)nglob[0] * nglob[1] * nglob[2];
+    std::vector<double> rhs(per * lda), sol(per * lda), field(per * lda, 0.0);
+    validation_fields(center, bc6, N, lda, rhs.data(), sol.data());
     s.solve(field.data(), rhs.data(), STD);
     double vol = 1.0;
     for (int d = 0; d < 3; ++d)
@@ -131,7 +153,8 @@ void validation_case(int center, const int bc6[6], int kernel, int N, int lda, c
 
 // vortex-tube Biot-Savart case: tube along vdir, image tubes by symx/symy in {-1,0,1},
 // Gaussian core sigma, cell-centred N^3 on [0,1]^3, CHAT_2 kernel unless given
-void vtube_case(int vdir, int symx, int symy, int N, int kernel, int order, double sigma, const std::string& kdir, double* err) {
+void vtube_case(int vdir, int symx, int symy, int N, int kernel, int order, double sigma, const std::string& kdir, double* err,
+                int* bc_out, double* rhs_out, double* sol_out) {
     const int lda = 3;
     const int nglob[3] = {N, N, N};
     const double L[3] = {1, 1, 1};
@@ -155,10 +178,7 @@ void vtube_case(int vdir, int symx, int symy, int N, int kernel, int order, doub
     for (int d = 0; d < 3; ++d)
         for (int s = 0; s < 2; ++s)
             for (int c = 0; c < 3; ++c) flat[(d * 2 + s) * lda + c] = bc[d][s][c];
-    Solver s(nglob, flat.data(), lda, h, L, false, (DiffType)order);
-    s.kernel_dir = kdir;
-    s.set_green((GreenType)kernel);
-    s.setup();
+    if (bc_out) std::copy(flat.begin(), flat.end(), bc_out);
     const size_t per = (size_t)N * N * N;
     std::vector<double> rhs(3 * per, 0.0), sol(3 * per, 0.0), field(3 * per, 0.0);
     const double eps = 100 * 2.220446049250313e-16;
@@ -179,6 +199,13 @@ void vtube_case(int vdir, int symx, int symy, int N, int kernel, int order, doub
                     sol[dir1 * per + id] += +std::cos(th) * vel * w[t];
                 }
             }
+    if (rhs_out) std::copy(rhs.begin(), rhs.end(), rhs_out);
+    if (sol_out) std::copy(sol.begin(), sol.end(), sol_out);
+    if (!err) return;
+    Solver s(nglob, flat.data(), lda, h, L, false, (DiffType)order);
+    s.kernel_dir = kdir;
+    s.set_green((GreenType)kernel);
+    s.setup();
     s.solve(field.data(), rhs.data(), ROT);
     const double vol = h[0] * h[1] * h[2];
     for (int c = 0; c < 3; ++c) {

@@ -1,0 +1,175 @@
+"""GPU parity of the HIP solve path against the CPU oracle and the reference's golden data.
+
+Every case solves on the GPU through the C interface (flups_amd -> libflups.so)
+and compares:
+  * the field with the oracle's field on the same input: relative L2 and Linf
+    errors <= 1e-12 (FP64 path);
+  * for the reference's validation cases, the error norms against the exact
+    solution with samples/validation/data_ref at the reference's own tolerance
+    (1e-10 relative, absolute below 1e-14; check_res.py:9,40-47).
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import flups_amd as fl
+from tests import oracle_lib as orc
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+TOL = 1e-12
+
+BC1 = [(a, b) for a in (0, 1, 4) for b in (0, 1, 4)] + [(3, 3)]
+BC2 = BC1 + [(9, 9)]
+ALL_BCS = [x + y + z for x in BC1 for y in BC1 for z in BC2]
+
+
+def gpu_solve(nglob, bc, lda, h, L, center, kernel, rhs, typ=0, odiff=0, alpha=2.0):
+    topo = fl.Topology(0, lda, list(nglob), [1, 1, 1], False)
+    s = fl.Solver(topo, bc, list(h), list(L), order_diff=odiff, center_type=[center] * 3)
+    s.set_green_type(kernel)
+    if alpha != 2.0:
+        s.set_alpha(alpha)
+    s.setup()
+    field = np.zeros_like(rhs)
+    s.solve(field, np.ascontiguousarray(rhs), typ)
+    return field
+
+
+def rel_err(a, b):
+    den2 = np.sqrt(np.sum(b * b))
+    deni = np.max(np.abs(b))
+    return np.sqrt(np.sum((a - b) ** 2)) / max(den2, 1e-300), np.max(np.abs(a - b)) / max(deni, 1e-300)
+
+
+def err_norms(field, sol, h, bc6):
+    vol = 1.0
+    for d in range(3):
+        if bc6[2 * d] != 9:
+            vol *= h[d]
+    out = []
+    for c in range(field.shape[0]):
+        e = sol[c] - field[c]
+        out += [np.sqrt(np.sum(e * e) * vol), np.max(np.abs(e))]
+    return np.array(out)
+
+
+def golden_ok(got, want):
+    """relative difference < 1e-10, absolute where the reference value is below 1e-14 (check_res.py:40-47)"""
+    got, want = np.asarray(got, float), np.asarray(want, float)
+    small = np.abs(want) < 1e-14
+    r = np.where(small, np.abs(got - want), np.abs(got - want) / np.where(small, 1.0, np.abs(want)))
+    return bool(np.all(r < 1e-10))
+
+
+def golden_table():
+    ref = json.load(open(os.path.join(HERE, "golden", "validation_ref.json")))
+    return {(e["center"], e["bc"], e["kernel"], e["N"], len(e["err"]) // 2): e["err"] for e in ref}
+
+
+def run_validation_case(center, bc6, kernel, N, lda=1, golden=None):
+    ng, h, rhs, sol = orc.validation_fields(center, bc6, N, lda)
+    bc = [[bc6[2 * d], bc6[2 * d + 1]] for d in range(3)]
+    f_gpu = gpu_solve(ng, bc, lda, h, [1, 1, 1], center, kernel, rhs)
+    f_ora = orc.solve(ng, bc, lda, h, [1, 1, 1], center, kernel, rhs)
+    e2, ei = rel_err(f_gpu, f_ora)
+    assert e2 <= TOL and ei <= TOL, (center, bc6, kernel, N, e2, ei)
+    if golden is not None:
+        key = ("CellCenter" if center == 1 else "NodeCenter", "".join(str(b) for b in bc6), kernel, N, lda)
+        if key in golden:
+            got = err_norms(f_gpu, sol, h, bc6)
+            assert golden_ok(got, golden[key]), (key, got, golden[key])
+            return 1
+    return 0
+
+
+@pytest.mark.parametrize("center", [1, 0])
+def test_validation_sweep_chat2(center):
+    """all 1100 boundary combinations of test_3D_bcs.py, CHAT_2, N = 8 (cell) / 9 (node)"""
+    golden = golden_table()
+    N = 8 if center == 1 else 9
+    matched = 0
+    for bc6 in ALL_BCS:
+        matched += run_validation_case(center, bc6, 0, N, 1, golden)
+    assert matched == len(ALL_BCS)
+
+
+KERN_BCS = [(4, 4, 4, 4, 4, 4), (4, 4, 0, 4, 4, 4), (4, 1, 1, 4, 4, 4), (4, 1, 4, 4, 4, 4), (4, 0, 1, 4, 4, 4),
+            (4, 0, 1, 4, 4, 1), (3, 3, 3, 3, 3, 3), (4, 0, 1, 4, 9, 9), (3, 3, 3, 3, 9, 9)]
+DEVICE_KERNELS = [0, 1, 2, 3, 4, 5, 6, 8, 9, 10]
+
+
+@pytest.mark.parametrize("center", [1, 0])
+def test_kernel_sweep(center):
+    """9 BC sets x kernels of test_3D_kerns.py at N = 16 (cell) / 17 (node)"""
+    golden = golden_table()
+    N = 16 if center == 1 else 17
+    matched = 0
+    for bc6 in KERN_BCS:
+        for k in DEVICE_KERNELS:
+            matched += run_validation_case(center, bc6, k, N, 1, golden)
+    assert matched == len(KERN_BCS) * len(DEVICE_KERNELS)
+
+
+def test_vtube_rot():
+    """the 27 Biot-Savart (lda = 3, ROT, spectral derivative) cases of solve_vtube"""
+    ref = json.load(open(os.path.join(HERE, "golden", "vtube_ref.json")))
+    N = 16
+    h = [1.0 / N] * 3
+    for e in ref:
+        bc, rhs, sol = orc.vtube_fields(e["dir"], e["symx"], e["symy"], N)
+        f_gpu = gpu_solve([N] * 3, bc, 3, h, [1, 1, 1], 1, 0, rhs, typ=1, odiff=1)
+        f_ora = orc.solve([N] * 3, bc, 3, h, [1, 1, 1], 1, 0, rhs, typ=1, odiff=1)
+        e2, ei = rel_err(f_gpu, f_ora)
+        assert e2 <= TOL and ei <= TOL, (e, e2, ei)
+        vol = h[0] ** 3
+        got = []
+        for c in range(3):
+            d = sol[c] - f_gpu[c]
+            got += [np.sqrt(np.sum(d * d) * vol), np.max(np.abs(d))]
+        assert golden_ok(got, e["err"]), (e, got)
+
+
+def blobs_rhs(n, seed, L=1.0):
+    """sum of 4 Gaussian blobs, centres U[0.3,0.7]^3, sigma U[0.03,0.06], amplitudes N(0,1)"""
+    rng = np.random.default_rng(seed)
+    h = L / n
+    x = (np.arange(n) + 0.5) * h
+    X, Y, Z = x[None, None, :], x[None, :, None], x[:, None, None]
+    f = np.zeros((n, n, n))
+    for _ in range(4):
+        c = rng.uniform(0.3, 0.7, 3) * L
+        s = rng.uniform(0.03, 0.06) * L
+        a = rng.normal()
+        f += a * np.exp(-((X - c[0]) ** 2 + (Y - c[1]) ** 2 + (Z - c[2]) ** 2) / (2 * s * s))
+    return f[None]
+
+
+@pytest.mark.parametrize("kernel", [2, 4, 8, 0])
+def test_unbounded_blobs_64(kernel):
+    """config 0 shape: cell 64^3 fully unbounded, Gaussian-blob rhs"""
+    n = 64
+    rhs = blobs_rhs(n, 1234 + kernel)
+    bc = [[4, 4]] * 3
+    h = [1.0 / n] * 3
+    f_gpu = gpu_solve([n] * 3, bc, 1, h, [1, 1, 1], 1, kernel, rhs)
+    f_ora = orc.solve([n] * 3, bc, 1, h, [1, 1, 1], 1, kernel, rhs)
+    e2, ei = rel_err(f_gpu, f_ora)
+    assert e2 <= TOL and ei <= TOL, (e2, ei)
+
+
+@pytest.mark.parametrize("kernel", [8, 2])
+def test_mixed_bc_lgf(kernel):
+    """config 1 shape at 32^3: x EVEN-ODD, y periodic, z unbounded (1-unbounded LGF_4 / HEJ_2)"""
+    n = 32
+    rng = np.random.default_rng(7)
+    rhs = rng.normal(size=(1, n, n, n))
+    rhs -= rhs.mean(axis=2, keepdims=True)
+    bc = [[0, 1], [3, 3], [4, 4]]
+    h = [1.0 / n] * 3
+    f_gpu = gpu_solve([n] * 3, bc, 1, h, [1, 1, 1], 1, kernel, rhs)
+    f_ora = orc.solve([n] * 3, bc, 1, h, [1, 1, 1], 1, kernel, rhs)
+    e2, ei = rel_err(f_gpu, f_ora)
+    assert e2 <= TOL and ei <= TOL, (e2, ei)
