@@ -3,9 +3,11 @@
 Mirrors the reference's Python binding (python_api/pyflups/flups.py:23-500:
 ``Topology``, ``Solver`` and the enums) over the C interface of
 ``libflups.so`` (include/flups.h). Arrays passed to ``Solver.solve`` may be
-numpy arrays (host memory, copied over PCIe inside the call) or torch CUDA
-tensors (device memory, the fast path). There is no CPU fallback: importing
-works without a GPU, but solver construction needs the HIP library and a GPU.
+numpy arrays (host memory, copied over PCIe inside the call), ``DeviceArray``
+buffers allocated by the library in HBM (the fast path), or torch tensors on
+the GPU (import torch before the first solver so that its HIP runtime starts
+first). There is no CPU fallback: importing works without a GPU, but solver
+construction needs the HIP library and a GPU.
 """
 import ctypes
 import enum
@@ -14,7 +16,7 @@ from ctypes import POINTER, byref, c_bool, c_char_p, c_double, c_int, c_int64, c
 
 import numpy as np
 
-__all__ = ["BoundaryType", "GreenType", "SolverType", "DiffType", "CenterType", "Topology", "Solver",
+__all__ = ["BoundaryType", "GreenType", "SolverType", "DiffType", "CenterType", "Topology", "Solver", "DeviceArray",
            "lib", "lib_path", "last_error", "FLUPS_ALIGNMENT", "FLUPS_FORWARD", "FLUPS_BACKWARD"]
 
 FLUPS_FORWARD = -1
@@ -72,10 +74,25 @@ lib_path = os.path.join(_HERE, "libflups.so")
 _lib = None
 
 
+def _init_torch_first():
+    """PyTorch-ROCm bundles its own HIP runtime; in a process that uses both, torch's
+    runtime must claim the device before libflups' does, so when torch is already
+    imported it is initialised first."""
+    import sys
+    torch = sys.modules.get("torch")
+    if torch is None:
+        return
+    try:
+        torch.cuda.is_available()
+    except Exception:
+        pass
+
+
 def lib():
     """Load libflups.so (raises if it has not been built)."""
     global _lib
     if _lib is None:
+        _init_torch_first()
         if not os.path.exists(lib_path):
             raise RuntimeError("libflups.so is missing: build it with `make -C flups_amd` "
                                "(or __graft_entry__.build())")
@@ -118,6 +135,13 @@ def lib():
         L.flups_get_last_solve_ns.restype = c_double
         L.flups_get_last_error.restype = c_char_p
         L.flups_set_abort_on_error.argtypes = [c_int]
+        L.flups_device_malloc.restype = c_void_p
+        L.flups_device_malloc.argtypes = [c_size_t]
+        L.flups_device_free.argtypes = [c_void_p]
+        L.flups_memcpy.argtypes = [c_void_p, c_void_p, c_size_t]
+        L.flups_memcpy.restype = c_int
+        L.flups_pointer_is_device.argtypes = [c_void_p]
+        L.flups_pointer_is_device.restype = c_int
         _lib = L
     return _lib
 
@@ -190,8 +214,38 @@ class Topology:
         return self._topo
 
 
+class DeviceArray:
+    """float64 array in HBM allocated by libflups (shape in C order, dimension 0 of FLUPS last)."""
+
+    def __init__(self, shape):
+        self.shape = tuple(int(s) for s in shape)
+        self.nbytes = int(np.prod(self.shape)) * 8
+        self.ptr = lib().flups_device_malloc(self.nbytes)
+        _check(self.ptr, "flups_device_malloc")
+
+    @classmethod
+    def from_numpy(cls, a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        d = cls(a.shape)
+        _check(lib().flups_memcpy(d.ptr, a.ctypes.data, d.nbytes), "flups_memcpy")
+        return d
+
+    def numpy(self):
+        out = np.empty(self.shape)
+        _check(lib().flups_memcpy(out.ctypes.data, self.ptr, self.nbytes), "flups_memcpy")
+        return out
+
+    def data_ptr(self):
+        return self.ptr
+
+    def __del__(self):
+        if getattr(self, "ptr", None):
+            lib().flups_device_free(self.ptr)
+            self.ptr = None
+
+
 def _data_ptr(a):
-    """device pointer of a torch tensor, or host pointer of a numpy array"""
+    """device pointer (DeviceArray / torch tensor) or host pointer (numpy array)"""
     if hasattr(a, "data_ptr"):
         return a.data_ptr()
     if not (isinstance(a, np.ndarray) and a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]):

@@ -195,6 +195,20 @@ void flups_get_spectralInfo(FLUPS_Solver* s, double kfact[3], double koffset[3],
     if (s) s->impl->spectral_info(kfact, koffset, symstart);
 }
 int flups_get_path(FLUPS_Solver* s) { return s ? s->impl->path() : -1; }
+void flups_profile_enable(FLUPS_Solver* s, int on) {
+    if (s) s->impl->profile(on != 0);
+}
+int flups_profile_get(FLUPS_Solver* s, int idx, char* name, int name_len, double* total_ms, long* calls,
+                      double* bytes_per_call) {
+    if (!s) return 0;
+    const char* nm = "";
+    const int n = s->impl->profile_get(idx, &nm, total_ms, calls, bytes_per_call);
+    if (n && name && name_len > 0) {
+        std::strncpy(name, nm, name_len - 1);
+        name[name_len - 1] = 0;
+    }
+    return n;
+}
 double flups_get_last_solve_ns(FLUPS_Solver* s) { return s ? s->impl->last_solve_ns() : 0.0; }
 
 void flups_pencilDirs(const FLUPS_BoundaryType* bc[3][2], int dirs[3]) {
@@ -205,6 +219,32 @@ void flups_pencilDirs(const FLUPS_BoundaryType* bc[3][2], int dirs[3]) {
     for (int i = 1; i < 3; ++i)
         for (int j = i; j > 0 && type[order[j]] < type[order[j - 1]]; --j) std::swap(order[j], order[j - 1]);
     for (int d = 0; d < 3; ++d) dirs[d] = order[d];
+}
+
+// ---- device memory helpers (HBM-resident buffers for callers without a HIP allocator) ----
+void* flups_device_malloc(size_t bytes) {
+    return guard("flups_device_malloc", [&]() -> void* {
+        void* p = nullptr;
+        HIP_CHECK(hipMalloc(&p, bytes ? bytes : 1));
+        return p;
+    });
+}
+void flups_device_free(void* p) {
+    if (p) (void)hipFree(p);
+}
+int flups_memcpy(void* dst, const void* src, size_t bytes) {
+    return guard("flups_memcpy", [&]() {
+        HIP_CHECK(hipMemcpy(dst, src, bytes, hipMemcpyDefault));
+        return 1;
+    });
+}
+int flups_pointer_is_device(const void* p) {
+    hipPointerAttribute_t a;
+    if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    return a.type == hipMemoryTypeDevice ? 1 : 0;
 }
 
 // ---- profiler handles ----

@@ -1,18 +1,630 @@
-// Fused unbounded pipeline (placeholder until the fused kernels land).
+// Fused pipeline for fully unbounded problems (every direction UNB-UNB).
+//
+// The reference runs, per solve, 3 x (pencil switch + batched 1-D FFT), the
+// multiply, then 3 x (FFT + switch back) (src/Solver.cpp:1207-1272). For the
+// unbounded case the transforms are: r2c of the zero-padded x rows (length
+// M0 = 2 N0 cell / 2 (N0-1) node), c2c along y (M1), c2c along z (M2),
+// multiply by normfact * G, and the inverse chain with truncation. Here that is
+// five kernels, each one HBM read and one HBM write of the field:
+//   x_r2c     user rows (real)        -> S0 (nx = M0/2+1 complex, N1, N2)
+//   col_fft   S0 along y, zero-padded -> S1 (nx, M1, N2)
+//   z_fused   S1 along z: forward FFT, x normfact*G, inverse FFT, keep N2 (in place)
+//   col_fft   S1 along y, inverse, keep N1 -> S0
+//   x_c2r     S0 rows -> user rows (real), keep N0
+// Stage arrays keep x fastest with the row padded to a multiple of 8 complex
+// values, so a tile of 8 consecutive x columns is one aligned 128-byte segment
+// for FP64 (16 columns for FP32). The pencil switches of the reference become
+// these strided tile loads; nothing is transposed in memory.
+//
+// Column FFTs: a team of M/8 threads owns one column, 8 values per thread at
+// rows j + i*M/8. Radix-8 Stockham stages (a final radix-4 or radix-2 stage
+// when M is not a power of 8) exchange through LDS; the first stage reads its
+// inputs straight from HBM and the last stage keeps its outputs in registers,
+// in the same row pattern. The forward z transform therefore ends exactly where
+// the inverse one starts, so the Green's multiply sits in registers between
+// them. The x rows use the half-length packed complex FFT for r2c / c2r.
 #include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <vector>
 
 #include "solver.hpp"
 
 namespace flups {
 
-struct FastPlan {};
-FastPlan* make_fast_plan(SolverImpl& s) {
-    (void)s;
-    return nullptr;
+namespace {
+
+template <typename R>
+struct C2 {
+    R x, y;
+};
+template <typename R>
+__device__ __forceinline__ C2<R> cadd(C2<R> a, C2<R> b) { return {a.x + b.x, a.y + b.y}; }
+template <typename R>
+__device__ __forceinline__ C2<R> csub(C2<R> a, C2<R> b) { return {a.x - b.x, a.y - b.y}; }
+template <typename R>
+__device__ __forceinline__ C2<R> cmul(C2<R> a, C2<R> b) { return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
+template <typename R>
+__device__ __forceinline__ C2<R> cconj(C2<R> a) { return {a.x, -a.y}; }
+// multiply by -i (forward) or +i (inverse)
+template <typename R, bool FWD>
+__device__ __forceinline__ C2<R> rot90(C2<R> a) { return FWD ? C2<R>{a.y, -a.x} : C2<R>{-a.y, a.x}; }
+
+template <typename R, bool FWD>
+__device__ __forceinline__ void dft2(C2<R>& a, C2<R>& b) {
+    const C2<R> t = a;
+    a = cadd(t, b);
+    b = csub(t, b);
 }
-void destroy_fast_plan(FastPlan* f) { delete f; }
+template <typename R, bool FWD>
+__device__ __forceinline__ void dft4(C2<R>& a0, C2<R>& a1, C2<R>& a2, C2<R>& a3) {
+    const C2<R> s0 = cadd(a0, a2), d0 = csub(a0, a2);
+    const C2<R> s1 = cadd(a1, a3), d1 = rot90<R, FWD>(csub(a1, a3));
+    a0 = cadd(s0, s1);
+    a2 = csub(s0, s1);
+    a1 = cadd(d0, d1);
+    a3 = csub(d0, d1);
+}
+template <typename R, bool FWD>
+__device__ __forceinline__ void dft8(C2<R>* v) {
+    const R h = (R)0.70710678118654752440;
+    C2<R> a0 = v[0], a1 = v[1], a2 = v[2], a3 = v[3], a4 = v[4], a5 = v[5], a6 = v[6], a7 = v[7];
+    // radix-2 split into even/odd halves, then 4-point DFTs
+    C2<R> e0 = cadd(a0, a4), o0 = csub(a0, a4);
+    C2<R> e1 = cadd(a1, a5), o1 = csub(a1, a5);
+    C2<R> e2 = cadd(a2, a6), o2 = csub(a2, a6);
+    C2<R> e3 = cadd(a3, a7), o3 = csub(a3, a7);
+    // twiddles w8^k on the odd half
+    o1 = FWD ? C2<R>{h * (o1.x + o1.y), h * (o1.y - o1.x)} : C2<R>{h * (o1.x - o1.y), h * (o1.y + o1.x)};
+    o2 = rot90<R, FWD>(o2);
+    o3 = FWD ? C2<R>{h * (o3.y - o3.x), -h * (o3.x + o3.y)} : C2<R>{-h * (o3.x + o3.y), h * (o3.x - o3.y)};
+    dft4<R, FWD>(e0, e1, e2, e3);
+    dft4<R, FWD>(o0, o1, o2, o3);
+    v[0] = e0; v[2] = e1; v[4] = e2; v[6] = e3;
+    v[1] = o0; v[3] = o1; v[5] = o2; v[7] = o3;
+}
+
+// twiddle exp(-+2 pi i e / M) from the table tw[t] = exp(-2 pi i t / M)
+template <typename R, bool FWD>
+__device__ __forceinline__ C2<R> twl(const C2<R>* tw, int e) {
+    const C2<R> w = tw[e];
+    return FWD ? w : cconj(w);
+}
+
+// Full column FFT of length M held by a team of T = M/8 threads (j = 0..T-1).
+// On entry v[i] = x[j + i*T]; on exit v[i] = X[j + i*T]. lds points at the
+// column (element e at lds[e*ES]); the whole block must call this together.
+template <typename R, int M, bool FWD, int ES>
+__device__ __forceinline__ void column_fft(C2<R>* v, C2<R>* lds, int j, const C2<R>* tw) {
+    constexpr int T = M / 8;
+    int span = 1;
+#pragma unroll
+    for (int st = 0; st < 8; ++st) {
+        // stages of radix 8 while 8*span <= M, then one radix 4 or 2 stage
+        if (span >= M) break;
+        const int P = (span * 8 <= M) ? 8 : (M / span);
+        if (span > 1) {
+            __syncthreads();   // previous stage's writes are visible
+#pragma unroll
+            for (int i = 0; i < 8; ++i) v[i] = lds[(j + i * T) * ES];
+        }
+        if (P == 8) {
+            const int k = j % span;
+            if (span > 1) {
+                const int stride = M / (span * 8);
+#pragma unroll
+                for (int q = 1; q < 8; ++q) v[q] = cmul(v[q], twl<R, FWD>(tw, q * k * stride));
+            }
+            dft8<R, FWD>(v);
+            if (span * 8 < M) {
+                __syncthreads();   // everyone has read this stage's inputs
+                const int base = (j / span) * span * 8 + k;
+#pragma unroll
+                for (int u = 0; u < 8; ++u) lds[(base + u * span) * ES] = v[u];
+            }
+        } else if (P == 4) {
+            // two butterflies b = j + t*T (t = 0, 1), inputs v[t + 2q]
+#pragma unroll
+            for (int t = 0; t < 2; ++t) {
+                const int b = j + t * T, k = b % span, stride = M / (span * 4);
+                C2<R> a[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) a[q] = v[t + 2 * q];
+#pragma unroll
+                for (int q = 1; q < 4; ++q) a[q] = cmul(a[q], twl<R, FWD>(tw, q * k * stride));
+                dft4<R, FWD>(a[0], a[1], a[2], a[3]);
+#pragma unroll
+                for (int u = 0; u < 4; ++u) v[t + 2 * u] = a[u];
+            }
+        } else {   // P == 2, four butterflies b = j + t*T, inputs v[t], v[t+4]
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const int b = j + t * T, k = b % span, stride = M / (span * 2);
+                C2<R> a0 = v[t], a1 = cmul(v[t + 4], twl<R, FWD>(tw, k * stride));
+                dft2<R, FWD>(a0, a1);
+                v[t] = a0;
+                v[t + 4] = a1;
+            }
+        }
+        span *= P;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// column passes along y or z of the stage arrays
+struct ColArgs {
+    const void* src;
+    void* dst;
+    const void* green;   // z_fused only: normfact * G, [kz'][ky'][kx], kx padded to ldx
+    const void* tw;
+    int nx;              // valid x columns
+    int ldx;             // x row length in complex elements (multiple of 8/16)
+    long long s_col;     // element stride between rows of a column (src and dst share it for z; y differs below)
+    long long s_col_dst;
+    long long s_other_src, s_other_dst;   // stride of the other (batch) dimension
+    int nother;
+    int nin, nout;       // valid input rows, rows to store
+    int gy, gmy, gmz;    // z_fused: ky' extent, M1 (for ky' = min(y, M1-y)), M2
+    int cstride_src, cstride_dst;   // component strides in complex elements (low 32 bits)
+    long long comp_src, comp_dst;
+};
+
+// TX columns per block, M/8 threads per column; block = TX * M/8 threads
+template <typename R, int M, int TX, bool FWD, bool FUSED>
+__global__ void __launch_bounds__(TX * M / 8) col_fft(ColArgs a) {
+    __shared__ C2<R> lds[M * TX];
+    constexpr int T = M / 8;
+    const int c = threadIdx.x % TX;
+    const int j = threadIdx.x / TX;
+    const int ntx = (a.nx + TX - 1) / TX;
+    const int tile = blockIdx.x % ntx;
+    const int other = blockIdx.x / ntx;
+    const int x = tile * TX + c;
+    const bool live = x < a.nx;
+    const C2<R>* src = reinterpret_cast<const C2<R>*>(a.src) + blockIdx.y * a.comp_src + other * a.s_other_src + x;
+    C2<R>* dst = reinterpret_cast<C2<R>*>(a.dst) + blockIdx.y * a.comp_dst + other * a.s_other_dst + x;
+    const C2<R>* tw = reinterpret_cast<const C2<R>*>(a.tw);
+    C2<R> v[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const int row = j + i * T;
+        v[i] = (live && row < a.nin) ? src[row * a.s_col] : C2<R>{0, 0};
+    }
+    column_fft<R, M, FWD, TX>(v, lds + c, j, tw);
+    if (FUSED) {
+        // multiply by normfact * G(kx, ky, kz); G is even in ky and kz
+        const R* g = reinterpret_cast<const R*>(a.green);
+        const int y = other;
+        const int ky = min(y, a.gmy - y);
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const int kz = j + i * T;
+            const int kzs = min(kz, M - kz);
+            const R gv = live ? g[((long long)kzs * a.gy + ky) * a.ldx + x] : (R)0;
+            v[i].x *= gv;
+            v[i].y *= gv;
+        }
+        column_fft<R, M, false, TX>(v, lds + c, j, tw);
+    }
+    if (live) {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const int row = j + i * T;
+            if (row < a.nout) dst[row * a.s_col_dst] = v[i];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// x rows: r2c of the zero-padded real row (length M = 2*H) through the packed
+// complex FFT of length H: z[n] = x[2n] + i x[2n+1], Z = FFT_H(z),
+// X[k] = (Z[k] + conj Z[H-k])/2 + exp(-2 pi i k/M) (Z[k] - conj Z[H-k])/(2i).
+struct RowArgs {
+    const void* src;
+    void* dst;
+    const void* tw;      // H-point table exp(-2 pi i t/H)
+    const void* tw2;     // M-point table exp(-2 pi i t/M)
+    long long s_row_src; // stride between rows (y) in source elements (real for r2c / complex for c2r)
+    long long s_z_src;   // stride between planes (z)
+    long long s_row_dst, s_z_dst;
+    int n1;              // rows per plane
+    int nrows;           // n1 * n2
+    int nvalid;          // valid reals per input row (r2c) / reals to store per output row (c2r)
+    int nx;              // complex outputs per row (H + 1)
+    long long comp_src, comp_dst;
+};
+
+template <typename R, int H, int ROWS>
+__global__ void __launch_bounds__(ROWS * H / 8) x_r2c(RowArgs a) {
+    __shared__ C2<R> lds[H * ROWS];
+    constexpr int T = H / 8;
+    const int rl = threadIdx.x / T;   // row within the block
+    const int j = threadIdx.x % T;
+    const int row = blockIdx.x * ROWS + rl;
+    const bool live = row < a.nrows;
+    const int y = live ? row % a.n1 : 0, z = live ? row / a.n1 : 0;
+    const R* src = reinterpret_cast<const R*>(a.src) + blockIdx.y * a.comp_src + y * a.s_row_src + z * a.s_z_src;
+    C2<R>* dst = reinterpret_cast<C2<R>*>(a.dst) + blockIdx.y * a.comp_dst + y * a.s_row_dst + z * a.s_z_dst;
+    C2<R>* col = lds + rl * H;   // row's buffer (contiguous: element stride 1)
+    C2<R> v[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const int n = j + i * T;
+        const int e0 = 2 * n, e1 = 2 * n + 1;
+        R re = 0, im = 0;
+        if (live && e0 < a.nvalid) re = src[e0];
+        if (live && e1 < a.nvalid) im = src[e1];
+        v[i] = {re, im};
+    }
+    column_fft<R, H, true, 1>(v, col, j, reinterpret_cast<const C2<R>*>(a.tw));
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < 8; ++i) col[j + i * T] = v[i];
+    __syncthreads();
+    const C2<R>* tw2 = reinterpret_cast<const C2<R>*>(a.tw2);
+    if (live) {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const int k = j + i * T;
+            const C2<R> zk = col[k], zc = cconj(col[(H - k) & (H - 1)]);
+            const C2<R> e = {(zk.x + zc.x) * (R)0.5, (zk.y + zc.y) * (R)0.5};
+            const C2<R> df = {(zk.x - zc.x) * (R)0.5, (zk.y - zc.y) * (R)0.5};
+            const C2<R> o = {df.y, -df.x};
+            dst[k] = cadd(e, cmul(tw2[k], o));
+        }
+        if (j == 0) {   // Nyquist: Re Z0 - Im Z0
+            const C2<R> z0 = col[0];
+            dst[H] = {z0.x - z0.y, (R)0};
+        }
+    }
+}
+
+// x rows inverse: Z[k] = (X[k] + conj X[H-k]) + i exp(+2 pi i k/M) (X[k] - conj X[H-k]), z = IFFT_H(Z),
+// x[2n] = Re z[n], x[2n+1] = Im z[n]; the imaginary parts of X[0] and X[H] are ignored.
+template <typename R, int H, int ROWS>
+__global__ void __launch_bounds__(ROWS * H / 8) x_c2r(RowArgs a) {
+    __shared__ C2<R> lds[H * ROWS];
+    constexpr int T = H / 8;
+    const int rl = threadIdx.x / T;
+    const int j = threadIdx.x % T;
+    const int row = blockIdx.x * ROWS + rl;
+    const bool live = row < a.nrows;
+    const int y = live ? row % a.n1 : 0, z = live ? row / a.n1 : 0;
+    const C2<R>* src = reinterpret_cast<const C2<R>*>(a.src) + blockIdx.y * a.comp_src + y * a.s_row_src + z * a.s_z_src;
+    R* dst = reinterpret_cast<R*>(a.dst) + blockIdx.y * a.comp_dst + y * a.s_row_dst + z * a.s_z_dst;
+    const C2<R>* tw2 = reinterpret_cast<const C2<R>*>(a.tw2);
+    C2<R> v[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const int k = j + i * T;
+        C2<R> xa = {0, 0}, xb = {0, 0};
+        if (live) {
+            xa = src[k];
+            xb = src[H - k];
+        }
+        if (k == 0) { xa.y = 0; xb.y = 0; }
+        const C2<R> bc = cconj(xb);
+        const C2<R> e = cadd(xa, bc), f = csub(xa, bc);
+        const C2<R> t = cmul(cconj(tw2[k]), f);
+        v[i] = {e.x - t.y, e.y + t.x};
+    }
+    column_fft<R, H, false, 1>(v, lds + rl * H, j, reinterpret_cast<const C2<R>*>(a.tw));
+    if (live) {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const int n = j + i * T;
+            if (2 * n < a.nvalid) dst[2 * n] = v[i].x;
+            if (2 * n + 1 < a.nvalid) dst[2 * n + 1] = v[i].y;
+        }
+    }
+}
+
+__global__ void compress_green(const double* g, double* out, float* out32, int nx, int n1, int n2, int gy, int gz, int ldx,
+                               double normfact) {
+    const long long tot = (long long)gz * gy * ldx;
+    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < tot; e += (long long)gridDim.x * blockDim.x) {
+        const int x = (int)(e % ldx);
+        const int ky = (int)((e / ldx) % gy);
+        const int kz = (int)(e / ((long long)ldx * gy));
+        double v = 0.0;
+        if (x < nx) v = normfact * g[2 * (x + (long long)nx * (ky + (long long)n1 * kz))];
+        if (out) out[e] = v;
+        if (out32) out32[e] = (float)v;
+    }
+}
+
+std::vector<double> twiddle_table(int M) {
+    std::vector<double> t(2 * (size_t)M);
+    for (int i = 0; i < M; ++i) {
+        const long double a = -2.0L * 3.14159265358979323846264338327950288L * (long double)i / (long double)M;
+        t[2 * i] = (double)cosl(a);
+        t[2 * i + 1] = (double)sinl(a);
+    }
+    return t;
+}
+
+bool pow2_in(int m, int lo, int hi) { return m >= lo && m <= hi && (m & (m - 1)) == 0; }
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+struct FastPlan {
+    int n[3];          // physical extents
+    int M[3];          // transform lengths
+    int nx, ldx;       // complex x extent and padded row length
+    int lda;
+    bool fp32;
+    double* s0 = nullptr;   // (ldx, N1, N2) complex
+    double* s1 = nullptr;   // (ldx, M1, N2) complex
+    void* green = nullptr;  // normfact * G, [M2/2+1][M1/2+1][ldx]
+    void* tw[3] = {nullptr, nullptr, nullptr};   // M1, M2 tables and the M0/2 table
+    void* twx = nullptr;                          // M0 table (x post-twiddles)
+    std::vector<void*> owned;
+    // per-kernel timing (HIP events on the solve stream)
+    bool profile = false;
+    hipEvent_t ev[6] = {};
+    double ms[5] = {0, 0, 0, 0, 0};
+    long calls = 0;
+    double bytes[5] = {0, 0, 0, 0, 0};   // algorithmic HBM bytes per launch
+};
+
+FastPlan* make_fast_plan(SolverImpl& s) {
+    if (s.ndim_ != 3 || s.phys_.axis != 0 || s.odiff_ != NOD) return nullptr;
+    for (int ip = 0; ip < 3; ++ip)
+        if (s.fwd_[ip].type != UNBUNB || s.fwd_[ip].dim != ip) return nullptr;
+    if (getenv("FLUPS_FORCE_GENERIC")) return nullptr;
+    auto* f = new FastPlan;
+    for (int d = 0; d < 3; ++d) {
+        f->n[d] = s.phys_.nglob[d];
+        f->M[d] = s.fwd_[d].n_in[0];
+    }
+    // supported lengths: powers of two (x: packed length M0/2 in [64, 2048]; y, z: [64, 2048])
+    if (!pow2_in(f->M[0] / 2, 64, 1024) || !pow2_in(f->M[1], 64, 1024) || !pow2_in(f->M[2], 64, 1024)) {
+        delete f;
+        return nullptr;
+    }
+    f->lda = s.lda_;
+    f->fp32 = s.fp32_;
+    f->nx = f->M[0] / 2 + 1;
+    const int align = f->fp32 ? 16 : 8;
+    f->ldx = (f->nx + align - 1) / align * align;
+    const size_t es = f->fp32 ? 2 * sizeof(float) : 2 * sizeof(double);
+    const size_t n0 = (size_t)f->ldx * f->n[1] * f->n[2] * f->lda;
+    const size_t n1 = (size_t)f->ldx * f->M[1] * f->n[2] * f->lda;
+    HIP_CHECK(hipMalloc(&f->s0, n0 * es));
+    HIP_CHECK(hipMalloc(&f->s1, n1 * es));
+    HIP_CHECK(hipMemset(f->s0, 0, n0 * es));
+    HIP_CHECK(hipMemset(f->s1, 0, n1 * es));
+    const int gy = f->M[1] / 2 + 1, gz = f->M[2] / 2 + 1;
+    const size_t gn = (size_t)gz * gy * f->ldx;
+    HIP_CHECK(hipMalloc(&f->green, gn * (f->fp32 ? sizeof(float) : sizeof(double))));
+    // the Green's function of the general path: complex (nx, M1, M2), FP64
+    const Shape& gs = s.stage_[2];
+    FLUPS_REQUIRE(gs.nf == 2 && gs.n[0] == f->nx && gs.n[1] == f->M[1] && gs.n[2] == f->M[2], "unexpected Green's layout");
+    hipLaunchKernelGGL(compress_green, dim3(4096), dim3(256), 0, s.stream_, s.dgreen_, f->fp32 ? nullptr : (double*)f->green,
+                       f->fp32 ? (float*)f->green : nullptr, f->nx, f->M[1], f->M[2], gy, gz, f->ldx, s.normfact_);
+    HIP_CHECK(hipGetLastError());
+    auto upload = [&](int M) -> void* {
+        const auto t = twiddle_table(M);
+        void* d = nullptr;
+        if (f->fp32) {
+            std::vector<float> tf(t.begin(), t.end());
+            HIP_CHECK(hipMalloc(&d, tf.size() * sizeof(float)));
+            HIP_CHECK(hipMemcpy(d, tf.data(), tf.size() * sizeof(float), hipMemcpyHostToDevice));
+        } else {
+            HIP_CHECK(hipMalloc(&d, t.size() * sizeof(double)));
+            HIP_CHECK(hipMemcpy(d, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice));
+        }
+        f->owned.push_back(d);
+        return d;
+    };
+    f->tw[0] = upload(f->M[0] / 2);
+    f->tw[1] = upload(f->M[1]);
+    f->tw[2] = upload(f->M[2]);
+    f->twx = upload(f->M[0]);
+    for (int i = 0; i < 6; ++i) HIP_CHECK(hipEventCreate(&f->ev[i]));
+    {
+        // minimum traffic of each pass: valid elements read + written (plus G for the fused pass)
+        const double rs = sizeof(double), cs = 2 * rs;
+        const double N = (double)f->n[0] * f->n[1] * f->n[2];
+        const double A = (double)f->nx * f->n[1] * f->n[2];        // S0 complex elements
+        const double B = (double)f->nx * f->M[1] * f->n[2];        // S1 complex elements
+        const double G = (double)gz * gy * f->nx * rs;
+        const double L = f->lda;
+        f->bytes[0] = L * (N * rs + A * cs);
+        f->bytes[1] = L * (A * cs + B * cs);
+        f->bytes[2] = L * (2 * B * cs) + G;
+        f->bytes[3] = L * (B * cs + A * cs);
+        f->bytes[4] = L * (A * cs + N * rs);
+    }
+    HIP_CHECK(hipStreamSynchronize(s.stream_));
+    return f;
+}
+
+void destroy_fast_plan(FastPlan* f) {
+    if (!f) return;
+    (void)hipFree(f->s0);
+    (void)hipFree(f->s1);
+    (void)hipFree(f->green);
+    for (void* p : f->owned) (void)hipFree(p);
+    for (int i = 0; i < 6; ++i)
+        if (f->ev[i]) (void)hipEventDestroy(f->ev[i]);
+    delete f;
+}
+
+namespace {
+
+template <typename R, int M, int TX, bool FWD, bool FUSED>
+void launch_col(const ColArgs& a, int comps, hipStream_t st) {
+    const int ntx = (a.nx + TX - 1) / TX;
+    dim3 grid(ntx * a.nother, comps);
+    hipLaunchKernelGGL((col_fft<R, M, TX, FWD, FUSED>), grid, dim3(TX * M / 8), 0, st, a);
+    HIP_CHECK(hipGetLastError());
+}
+
+template <typename R, int TX, bool FWD, bool FUSED>
+void col_dispatch(int M, const ColArgs& a, int comps, hipStream_t st) {
+    switch (M) {
+        case 64: launch_col<R, 64, TX, FWD, FUSED>(a, comps, st); break;
+        case 128: launch_col<R, 128, TX, FWD, FUSED>(a, comps, st); break;
+        case 256: launch_col<R, 256, TX, FWD, FUSED>(a, comps, st); break;
+        case 512: launch_col<R, 512, TX, FWD, FUSED>(a, comps, st); break;
+        case 1024: launch_col<R, 1024, TX / 2, FWD, FUSED>(a, comps, st); break;
+        default: fail("unsupported column length %d", M);
+    }
+}
+
+template <typename R>
+void row_dispatch(bool fwd, int H, const RowArgs& a, int comps, hipStream_t st) {
+#define FLUPS_ROW(HH, RR)                                                                                   \
+    case HH: {                                                                                              \
+        dim3 grid((a.nrows + RR - 1) / RR, comps);                                                          \
+        if (fwd) hipLaunchKernelGGL((x_r2c<R, HH, RR>), grid, dim3(RR * HH / 8), 0, st, a);                 \
+        else hipLaunchKernelGGL((x_c2r<R, HH, RR>), grid, dim3(RR * HH / 8), 0, st, a);                     \
+        break;                                                                                              \
+    }
+    switch (H) {
+        FLUPS_ROW(64, 32)
+        FLUPS_ROW(128, 16)
+        FLUPS_ROW(256, 8)
+        FLUPS_ROW(512, 4)
+        FLUPS_ROW(1024, 2)
+        default: fail("unsupported row length %d", 2 * H);
+    }
+#undef FLUPS_ROW
+    HIP_CHECK(hipGetLastError());
+}
+
+template <typename R>
+void run_fast(FastPlan* f, const double* rhs, double* field, const View& uin, const View& uout, hipStream_t st) {
+    const int N0 = f->n[0], N1 = f->n[1], N2 = f->n[2];
+    const int M0 = f->M[0], M1 = f->M[1], M2 = f->M[2];
+    const int H = M0 / 2;
+    const long long ldx = f->ldx;
+    const long long c0 = ldx * N1 * N2, c1 = ldx * M1 * N2;   // component strides (complex)
+    constexpr int TX = sizeof(R) == 8 ? 8 : 16;
+    // 1. x r2c: user (real) -> S0
+    RowArgs r;
+    r.src = rhs;
+    r.dst = f->s0;
+    r.tw = f->tw[0];
+    r.tw2 = f->twx;
+    r.s_row_src = uin.s[1];
+    r.s_z_src = uin.s[2];
+    r.s_row_dst = ldx;
+    r.s_z_dst = ldx * N1;
+    r.n1 = N1;
+    r.nrows = N1 * N2;
+    r.nvalid = N0;
+    r.nx = f->nx;
+    r.comp_src = uin.sc;
+    r.comp_dst = c0;
+    static_assert(sizeof(R) == 8, "FP32 stage buffers are not wired yet");
+    if (f->profile) HIP_CHECK(hipEventRecord(f->ev[0], st));
+    row_dispatch<R>(true, H, r, f->lda, st);
+    if (f->profile) HIP_CHECK(hipEventRecord(f->ev[1], st));
+    // 2. y forward: S0 (ldx, N1, N2) -> S1 (ldx, M1, N2)
+    ColArgs a{};
+    a.src = f->s0;
+    a.dst = f->s1;
+    a.tw = f->tw[1];
+    a.nx = f->nx;
+    a.ldx = f->ldx;
+    a.s_col = ldx;
+    a.s_col_dst = ldx;
+    a.s_other_src = ldx * N1;
+    a.s_other_dst = ldx * M1;
+    a.nother = N2;
+    a.nin = N1;
+    a.nout = M1;
+    a.comp_src = c0;
+    a.comp_dst = c1;
+    col_dispatch<R, TX, true, false>(M1, a, f->lda, st);
+    if (f->profile) HIP_CHECK(hipEventRecord(f->ev[2], st));
+    // 3. z fused: S1 in place, forward, * normfact G, inverse, keep N2
+    ColArgs z{};
+    z.src = f->s1;
+    z.dst = f->s1;
+    z.green = f->green;
+    z.tw = f->tw[2];
+    z.nx = f->nx;
+    z.ldx = f->ldx;
+    z.s_col = ldx * M1;
+    z.s_col_dst = ldx * M1;
+    z.s_other_src = ldx;
+    z.s_other_dst = ldx;
+    z.nother = M1;
+    z.nin = N2;
+    z.nout = N2;
+    z.gy = M1 / 2 + 1;
+    z.gmy = M1;
+    z.gmz = M2;
+    z.comp_src = c1;
+    z.comp_dst = c1;
+    col_dispatch<R, TX, true, true>(M2, z, f->lda, st);
+    if (f->profile) HIP_CHECK(hipEventRecord(f->ev[3], st));
+    // 4. y inverse: S1 -> S0, keep N1
+    ColArgs b = a;
+    b.src = f->s1;
+    b.dst = f->s0;
+    b.s_other_src = ldx * M1;
+    b.s_other_dst = ldx * N1;
+    b.nin = M1;
+    b.nout = N1;
+    b.comp_src = c1;
+    b.comp_dst = c0;
+    col_dispatch<R, TX, false, false>(M1, b, f->lda, st);
+    if (f->profile) HIP_CHECK(hipEventRecord(f->ev[4], st));
+    // 5. x c2r: S0 -> user (real), keep N0
+    RowArgs q;
+    q.src = f->s0;
+    q.dst = field;
+    q.tw = f->tw[0];
+    q.tw2 = f->twx;
+    q.s_row_src = ldx;
+    q.s_z_src = ldx * N1;
+    q.s_row_dst = uout.s[1];
+    q.s_z_dst = uout.s[2];
+    q.n1 = N1;
+    q.nrows = N1 * N2;
+    q.nvalid = N0;
+    q.nx = f->nx;
+    q.comp_src = c0;
+    q.comp_dst = uout.sc;
+    row_dispatch<R>(false, H, q, f->lda, st);
+    if (f->profile) HIP_CHECK(hipEventRecord(f->ev[5], st));
+    (void)M2;
+}
+
+}  // namespace
+
 void fast_solve(FastPlan* f, double* field, const double* rhs, const View& uin, const View& uout, hipStream_t s) {
-    (void)f; (void)field; (void)rhs; (void)uin; (void)uout; (void)s;
+    run_fast<double>(f, rhs, field, uin, uout, s);
+}
+
+// after the stream has completed: accumulate the per-kernel times of the last solve
+void fast_collect(FastPlan* f) {
+    if (!f->profile) return;
+    for (int i = 0; i < 5; ++i) {
+        float ms = 0;
+        HIP_CHECK(hipEventElapsedTime(&ms, f->ev[i], f->ev[i + 1]));
+        f->ms[i] += ms;
+    }
+    ++f->calls;
+}
+void fast_profile(FastPlan* f, bool on) {
+    f->profile = on;
+    for (double& m : f->ms) m = 0;
+    f->calls = 0;
+}
+int fast_profile_get(FastPlan* f, int i, const char** name, double* ms, long* calls, double* bytes) {
+    static const char* names[5] = {"x_r2c", "y_fwd", "z_fused", "y_inv", "x_c2r"};
+    if (i < 0 || i >= 5) return 0;
+    *name = names[i];
+    *ms = f->ms[i];
+    *calls = f->calls;
+    *bytes = f->bytes[i];
+    return 5;
 }
 
 }  // namespace flups

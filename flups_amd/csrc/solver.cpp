@@ -22,6 +22,9 @@ void launch_generic(const PassDesc& P, hipStream_t s);
 FastPlan* make_fast_plan(SolverImpl& s);   // nullptr if the fused pipeline does not apply
 void destroy_fast_plan(FastPlan* f);
 void fast_solve(FastPlan* f, double* field, const double* rhs, const View& uin, const View& uout, hipStream_t s);
+void fast_collect(FastPlan* f);
+void fast_profile(FastPlan* f, bool on);
+int fast_profile_get(FastPlan* f, int i, const char** name, double* ms, long* calls, double* bytes);
 
 // ---------------------------------------------------------------------------
 // topology sizes (src/Topology.cpp:81-98): the fastest direction is padded so
@@ -544,6 +547,14 @@ void SolverImpl::solve(double* field, const double* rhs, int type) {
     float ms = 0;
     HIP_CHECK(hipEventElapsedTime(&ms, ev0_, ev1_));
     last_ns_ = ms * 1e6;
+    if (fast_ && type == 0) fast_collect(fast_);
+}
+
+void SolverImpl::profile(bool on) {
+    if (fast_) fast_profile(fast_, on);
+}
+int SolverImpl::profile_get(int i, const char** name, double* ms, long* calls, double* bytes) const {
+    return fast_ ? fast_profile_get(fast_, i, name, ms, calls, bytes) : 0;
 }
 
 }  // namespace flups

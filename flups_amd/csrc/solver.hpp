@@ -41,6 +41,9 @@ class SolverImpl {
     int path() const { return fast_ ? 1 : 0; }
     double last_solve_ns() const { return last_ns_; }
     int ndim() const { return ndim_; }
+    // per-kernel timing of the fused pipeline (HIP events on the solve stream)
+    void profile(bool on);
+    int profile_get(int i, const char** name, double* ms, long* calls, double* bytes) const;
 
     // building blocks (also used by the fused pipeline)
     hipStream_t stream() const { return stream_; }
@@ -75,6 +78,7 @@ class SolverImpl {
     double last_ns_ = 0;
     std::string kdir_;
     friend struct FastPlan;
+    friend FastPlan* make_fast_plan(SolverImpl&);
 };
 
 std::string kernel_dir();

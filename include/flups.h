@@ -147,6 +147,18 @@ void        flups_set_precision(FLUPS_Solver* s, int fp32);
 int         flups_get_path(FLUPS_Solver* s);
 /* nanoseconds of device time of the last flups_solve (HIP events on the solver's stream) */
 double      flups_get_last_solve_ns(FLUPS_Solver* s);
+/* per-kernel timing of the fused pipeline: enable (resets the counters), then read entry idx:
+   name, accumulated milliseconds, number of solves and algorithmic HBM bytes per launch.
+   Returns the number of entries (0 if the solver runs the general path). */
+void        flups_profile_enable(FLUPS_Solver* s, int on);
+int         flups_profile_get(FLUPS_Solver* s, int idx, char* name, int name_len, double* total_ms, long* calls,
+                              double* bytes_per_call);
+
+/* device memory through the library's HIP runtime (zero-copy HBM buffers for flups_solve) */
+void*       flups_device_malloc(size_t bytes);
+void        flups_device_free(void* p);
+int         flups_memcpy(void* dst, const void* src, size_t bytes);   /* any direction; 1 on success */
+int         flups_pointer_is_device(const void* p);
 
 #ifdef __cplusplus
 }

@@ -14,6 +14,11 @@ import os
 import numpy as np
 import pytest
 
+try:   # torch's HIP runtime must start before libflups' one in a process using both
+    import torch
+except Exception:   # pragma: no cover
+    torch = None
+
 import flups_amd as fl
 from tests import oracle_lib as orc
 
@@ -172,4 +177,75 @@ def test_mixed_bc_lgf(kernel):
     f_gpu = gpu_solve([n] * 3, bc, 1, h, [1, 1, 1], 1, kernel, rhs)
     f_ora = orc.solve([n] * 3, bc, 1, h, [1, 1, 1], 1, kernel, rhs)
     e2, ei = rel_err(f_gpu, f_ora)
+    assert e2 <= TOL and ei <= TOL, (e2, ei)
+
+
+FAST_CASES = [
+    # (nglob, center, lda, kernel)
+    ((64, 64, 64), 1, 1, 2),
+    ((128, 128, 128), 1, 1, 4),
+    ((64, 32, 128), 1, 1, 2),
+    ((65, 65, 65), 0, 1, 2),
+    ((64, 64, 64), 1, 3, 8),
+    ((256, 32, 64), 1, 1, 0),
+]
+
+
+@pytest.mark.parametrize("case", FAST_CASES, ids=lambda c: "x".join(map(str, c[0])) + "-c%d-l%d-k%d" % c[1:])
+def test_fused_pipeline_vs_oracle(case):
+    """the fused unbounded pipeline (path 1) against the oracle, cell and node, scalar and vector"""
+    nglob, center, lda, kernel = case
+    rng = np.random.default_rng(sum(nglob) + lda)
+    n0, n1, n2 = nglob
+    rhs = rng.normal(size=(lda, n2, n1, n0))
+    h = [1.0 / (n - (1 - center)) for n in nglob]
+    if kernel in (1, 2, 3, 4, 5, 6, 8, 9, 10):
+        h = [h[0]] * 3
+    L = [h[d] * (nglob[d] - (1 - center)) for d in range(3)]
+    bc = [[4, 4]] * 3
+    topo = fl.Topology(0, lda, list(nglob), [1, 1, 1], False)
+    s = fl.Solver(topo, bc, h, L, center_type=[center] * 3)
+    s.set_green_type(kernel)
+    s.setup()
+    assert s.path == 1
+    f_gpu = np.zeros_like(rhs)
+    s.solve(f_gpu, rhs)
+    f_ora = orc.solve(list(nglob), bc, lda, h, L, center, kernel, rhs)
+    e2, ei = rel_err(f_gpu, f_ora)
+    assert e2 <= TOL and ei <= TOL, (e2, ei)
+
+
+def test_fused_vs_general_and_device_pointers(monkeypatch):
+    """same problem through the general path, through library device buffers and torch device tensors"""
+    n = 64
+    rhs = blobs_rhs(n, 99)
+    bc = [[4, 4]] * 3
+    h = [1.0 / n] * 3
+    outs = []
+    for force in (False, True):
+        if force:
+            monkeypatch.setenv("FLUPS_FORCE_GENERIC", "1")
+        topo = fl.Topology(0, 1, [n] * 3, [1, 1, 1], False)
+        s = fl.Solver(topo, bc, h, [1, 1, 1])
+        s.set_green_type(2)
+        s.setup()
+        assert s.path == (0 if force else 1)
+        f = np.zeros_like(rhs)
+        s.solve(f, rhs)
+        outs.append(f)
+        if not force:
+            d_rhs = fl.DeviceArray.from_numpy(rhs)
+            d_f = fl.DeviceArray(rhs.shape)
+            assert fl.lib().flups_pointer_is_device(d_rhs.ptr) == 1
+            s.solve(d_f, d_rhs)
+            assert np.array_equal(d_f.numpy(), f)
+            s.solve(d_rhs, d_rhs)   # in place: field and rhs alias
+            assert np.array_equal(d_rhs.numpy(), f)
+            if torch is not None and torch.cuda.is_available():
+                t_rhs = torch.from_numpy(rhs).cuda()
+                t_f = torch.zeros_like(t_rhs)
+                s.solve(t_f, t_rhs)
+                torch.cuda.synchronize()
+                assert np.array_equal(t_f.cpu().numpy(), f)
+    e2, ei = rel_err(outs[0], outs[1])
     assert e2 <= TOL and ei <= TOL, (e2, ei)
