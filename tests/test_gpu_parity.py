@@ -249,3 +249,17 @@ def test_fused_vs_general_and_device_pointers(monkeypatch):
                 assert np.array_equal(t_f.cpu().numpy(), f)
     e2, ei = rel_err(outs[0], outs[1])
     assert e2 <= TOL and ei <= TOL, (e2, ei)
+
+
+@pytest.mark.parametrize("odiff", [1, 2, 4, 6])
+@pytest.mark.parametrize("bc", [[[3, 3]] * 3, [[3, 3], [0, 1], [4, 4]], [[4, 4]] * 3], ids=["per", "mix", "unb"])
+def test_rot_diff_orders(odiff, bc):
+    """lda = 3 ROT with spectral and FD2/4/6 derivative symbols against the oracle"""
+    n = 16
+    rng = np.random.default_rng(odiff)
+    rhs = rng.normal(size=(3, n, n, n))
+    h = [1.0 / n] * 3
+    f_gpu = gpu_solve([n] * 3, bc, 3, h, [1, 1, 1], 1, 2, rhs, typ=1, odiff=odiff)
+    f_ora = orc.solve([n] * 3, bc, 3, h, [1, 1, 1], 1, 2, rhs, typ=1, odiff=odiff)
+    e2, ei = rel_err(f_gpu, f_ora)
+    assert e2 <= TOL and ei <= TOL, (e2, ei)

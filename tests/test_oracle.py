@@ -210,3 +210,77 @@ def test_lgf2_table_known_values():
     t2 = np.fromfile(os.path.join(kdir, "LGF_2_2d_32.ker"), "<f8")
     assert abs(t3[0] - 0.2527310098586626) < 1e-15
     assert abs(t2[1] + 0.25) < 1e-14
+
+
+def lowmode_field(n, lda, seed, kmax=3):
+    """band-limited periodic field (no Nyquist content), zero mean per component"""
+    rng = np.random.default_rng(seed)
+    x = (np.arange(n) + 0.5) / n
+    Z, Y, X = np.meshgrid(x, x, x, indexing="ij")
+    f = np.zeros((lda, n, n, n))
+    for c in range(lda):
+        for _ in range(6):
+            k = rng.integers(-kmax, kmax + 1, 3)
+            if not k.any():
+                continue
+            f[c] += rng.normal() * np.cos(2 * np.pi * (k[0] * X + k[1] * Y + k[2] * Z) + rng.uniform(0, 2 * np.pi))
+        f[c] -= f[c].mean()
+    return f
+
+
+def stencil_d(f, axis, h, order):
+    """central finite difference of order 2/4/6 along a periodic axis"""
+    w = {2: [0.5], 4: [2 / 3, -1 / 12], 6: [0.75, -0.15, 1 / 60]}[order]
+    return sum(c * (np.roll(f, -(m + 1), axis) - np.roll(f, m + 1, axis)) for m, c in enumerate(w)) / h
+
+
+def spectral_d(f, axis, h):
+    k = 2 * np.pi * np.fft.fftfreq(f.shape[axis], h)
+    shape = [1] * f.ndim
+    shape[axis] = -1
+    return np.real(np.fft.ifft(1j * k.reshape(shape) * np.fft.fft(f, axis=axis), axis=axis))
+
+
+def curl(psi, d):
+    """psi[c] indexed (z, y, x): dim 0 = x is array axis 2"""
+    dx, dy, dz = (lambda f: d(f, 2)), (lambda f: d(f, 1)), (lambda f: d(f, 0))
+    return np.stack([dy(psi[2]) - dz(psi[1]), dz(psi[0]) - dx(psi[2]), dx(psi[1]) - dy(psi[0])])
+
+
+@pytest.mark.parametrize("odiff", [2, 4, 6])
+def test_rot_fd_is_stencil_curl_of_std(odiff):
+    """ROT with FD2/4/6 = the central-difference curl of the STD solution (periodic, exact),
+    ROT with SPE = the spectral curl; the sign is shared (dothemagic_rot.ipp:102-349)"""
+    n, h = 16, 1.0 / 16
+    bc = [[3, 3]] * 3
+    w = lowmode_field(n, 3, 11)
+    psi = orc.solve([n] * 3, bc, 3, [h] * 3, [1, 1, 1], 1, 0, w, typ=0)
+    u_spe = orc.solve([n] * 3, bc, 3, [h] * 3, [1, 1, 1], 1, 0, w, typ=1, odiff=1)
+    cs = curl(psi, lambda f, a: spectral_d(f, a, h))
+    s = np.sign(np.sum(u_spe * cs))
+    scale = np.max(np.abs(u_spe))
+    assert np.max(np.abs(u_spe - s * cs)) <= 1e-12 * scale
+    u_fd = orc.solve([n] * 3, bc, 3, [h] * 3, [1, 1, 1], 1, 0, w, typ=1, odiff=odiff)
+    cf = curl(psi, lambda f, a: stencil_d(f, a, h, odiff))
+    assert np.max(np.abs(u_fd - s * cf)) <= 1e-12 * scale
+    assert np.max(np.abs(u_fd - u_spe)) > 1e-6 * scale   # the FD symbol really differs from ik
+
+
+@pytest.mark.parametrize("odiff", [2, 6])
+def test_rot_fd_unbounded_interior(odiff):
+    """z unbounded: the FD curl acts on the padded domain, so interior points equal the stencil
+    curl of the STD solution (x, y periodic)"""
+    n, h = 16, 1.0 / 16
+    bc = [[3, 3], [3, 3], [4, 4]]
+    w = lowmode_field(n, 3, 5)
+    z = (np.arange(n) + 0.5) / n
+    w *= np.exp(-((z - 0.5) / 0.15) ** 2)[None, :, None, None]
+    psi = orc.solve([n] * 3, bc, 3, [h] * 3, [1, 1, 1], 1, 0, w, typ=0)
+    u = orc.solve([n] * 3, bc, 3, [h] * 3, [1, 1, 1], 1, 0, w, typ=1, odiff=odiff)
+    psi_pad = np.concatenate([psi, np.zeros_like(psi)], axis=1)   # z stencil uses interior values only
+    cf = curl(psi_pad, lambda f, a: stencil_d(f, a, h, odiff))[:, :n]
+    r = odiff // 2
+    u_spe = orc.solve([n] * 3, bc, 3, [h] * 3, [1, 1, 1], 1, 0, w, typ=1, odiff=1)
+    s = np.sign(np.sum(u_spe * cf))
+    scale = np.max(np.abs(u))
+    assert np.max(np.abs(u - s * cf)[:, r:n - r]) <= 1e-11 * scale
