@@ -91,12 +91,21 @@ __device__ __forceinline__ C2<R> twl(const C2<R>* tw, int e) {
     return FWD ? w : cconj(w);
 }
 
+// LDS slot of column element e: e*ES for interleaved columns (ES columns side by
+// side), or e + e/8 for a single contiguous column (ES = 0): the pad slot every 8
+// elements keeps the stride-8 writes of the first stage on distinct banks.
+template <int ES>
+__device__ __forceinline__ int lslot(int e) { return ES > 0 ? e * ES : e + (e >> 3); }
+
 // Full column FFT of length M held by a team of T = M/8 threads (j = 0..T-1).
 // On entry v[i] = x[j + i*T]; on exit v[i] = X[j + i*T]. lds points at the
-// column (element e at lds[e*ES]); the whole block must call this together.
+// column (element e at lds[lslot<ES>(e)]); the whole block must call this together.
 template <typename R, int M, bool FWD, int ES>
 __device__ __forceinline__ void column_fft(C2<R>* v, C2<R>* lds, int j, const C2<R>* tw) {
     constexpr int T = M / 8;
+    // table entries w, w^2, w^4 of the next radix-8 stage, loaded one stage ahead
+    // so that their latency hides behind the current stage and its LDS exchange
+    C2<R> r1 = {1, 0}, r2 = {1, 0}, r4 = {1, 0};
     int span = 1;
 #pragma unroll
     for (int st = 0; st < 8; ++st) {
@@ -106,22 +115,41 @@ __device__ __forceinline__ void column_fft(C2<R>* v, C2<R>* lds, int j, const C2
         if (span > 1) {
             __syncthreads();   // previous stage's writes are visible
 #pragma unroll
-            for (int i = 0; i < 8; ++i) v[i] = lds[(j + i * T) * ES];
+            for (int i = 0; i < 8; ++i) v[i] = lds[lslot<ES>(j + i * T)];
         }
         if (P == 8) {
             const int k = j % span;
+            C2<R> n1 = {1, 0}, n2 = {1, 0}, n4 = {1, 0};
+            const int nspan = span * 8;
+            if (nspan * 8 <= M) {
+                const int e = (j % nspan) * (M / (nspan * 8));
+                n1 = twl<R, FWD>(tw, e);
+                n2 = twl<R, FWD>(tw, 2 * e);
+                n4 = twl<R, FWD>(tw, 4 * e);
+            }
             if (span > 1) {
-                const int stride = M / (span * 8);
+                // w^q, q = 1..7, from w, w^2, w^4 and four products
+                C2<R> tw8[8];
+                tw8[1] = r1;
+                tw8[2] = r2;
+                tw8[4] = r4;
+                tw8[3] = cmul(tw8[1], tw8[2]);
+                tw8[5] = cmul(tw8[1], tw8[4]);
+                tw8[6] = cmul(tw8[2], tw8[4]);
+                tw8[7] = cmul(tw8[3], tw8[4]);
 #pragma unroll
-                for (int q = 1; q < 8; ++q) v[q] = cmul(v[q], twl<R, FWD>(tw, q * k * stride));
+                for (int q = 1; q < 8; ++q) v[q] = cmul(v[q], tw8[q]);
             }
             dft8<R, FWD>(v);
             if (span * 8 < M) {
                 __syncthreads();   // everyone has read this stage's inputs
                 const int base = (j / span) * span * 8 + k;
 #pragma unroll
-                for (int u = 0; u < 8; ++u) lds[(base + u * span) * ES] = v[u];
+                for (int u = 0; u < 8; ++u) lds[lslot<ES>(base + u * span)] = v[u];
             }
+            r1 = n1;
+            r2 = n2;
+            r4 = n4;
         } else if (P == 4) {
             // two butterflies b = j + t*T (t = 0, 1), inputs v[t + 2q]
 #pragma unroll
@@ -150,6 +178,14 @@ __device__ __forceinline__ void column_fft(C2<R>* v, C2<R>* lds, int j, const C2
     }
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs (each with its own L2):
+// hardware block b runs on XCD b % 8. Renumber so that consecutive logical
+// blocks run on the same XCD at about the same time (they share cache lines).
+__device__ __forceinline__ int xcd_group(int b, int nb) {
+    const int q = nb / 8, r = nb % 8, x = b % 8, slot = b / 8;
+    return x < r ? x * (q + 1) + slot : r * (q + 1) + (x - r) * q + slot;
+}
+
 // ---------------------------------------------------------------------------
 // column passes along y or z of the stage arrays
 struct ColArgs {
@@ -167,18 +203,21 @@ struct ColArgs {
     int gy, gmy, gmz;    // z_fused: ky' extent, M1 (for ky' = min(y, M1-y)), M2
     int cstride_src, cstride_dst;   // component strides in complex elements (low 32 bits)
     long long comp_src, comp_dst;
+    int rev;             // 1: blocks walk the batch from its last entry
+    int xcd;             // z_fused: 1 = XCD-grouped block numbering
 };
 
 // TX columns per block, M/8 threads per column; block = TX * M/8 threads
-template <typename R, int M, int TX, bool FWD, bool FUSED>
+template <typename R, int M, int TX, bool FWD>
 __global__ void __launch_bounds__(TX * M / 8) col_fft(ColArgs a) {
     __shared__ C2<R> lds[M * TX];
     constexpr int T = M / 8;
     const int c = threadIdx.x % TX;
     const int j = threadIdx.x / TX;
     const int ntx = (a.nx + TX - 1) / TX;
-    const int tile = blockIdx.x % ntx;
-    const int other = blockIdx.x / ntx;
+    const int blk = a.rev ? gridDim.x - 1 - blockIdx.x : blockIdx.x;
+    const int tile = blk % ntx;
+    const int other = blk / ntx;
     const int x = tile * TX + c;
     const bool live = x < a.nx;
     const C2<R>* src = reinterpret_cast<const C2<R>*>(a.src) + blockIdx.y * a.comp_src + other * a.s_other_src + x;
@@ -191,26 +230,93 @@ __global__ void __launch_bounds__(TX * M / 8) col_fft(ColArgs a) {
         v[i] = (live && row < a.nin) ? src[row * a.s_col] : C2<R>{0, 0};
     }
     column_fft<R, M, FWD, TX>(v, lds + c, j, tw);
-    if (FUSED) {
-        // multiply by normfact * G(kx, ky, kz); G is even in ky and kz
-        const R* g = reinterpret_cast<const R*>(a.green);
-        const int y = other;
-        const int ky = min(y, a.gmy - y);
-#pragma unroll
-        for (int i = 0; i < 8; ++i) {
-            const int kz = j + i * T;
-            const int kzs = min(kz, M - kz);
-            const R gv = live ? g[((long long)kzs * a.gy + ky) * a.ldx + x] : (R)0;
-            v[i].x *= gv;
-            v[i].y *= gv;
-        }
-        column_fft<R, M, false, TX>(v, lds + c, j, tw);
-    }
     if (live) {
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
             const int row = j + i * T;
             if (row < a.nout) dst[row * a.s_col_dst] = v[i];
+        }
+    }
+}
+
+// Fused z pass, in place on S1: forward FFT along z, multiply by normfact * G,
+// inverse FFT, keep the first nin rows. G is real and even in ky and kz, so one
+// block serves the y rows ky and M1 - ky of one x tile with the same G values:
+// its slice of the compressed G, [ky'][tile][kz'][TX], is contiguous and read
+// once. PREFETCH issues the loads of both column sets before the first FFT
+// (more bytes in flight, more registers); otherwise the second set is loaded
+// after the first one is stored.
+template <typename R, int M, int TX, int SCHED>
+__global__ void __launch_bounds__(TX * M / 8) z_fused(ColArgs a) {
+    // SCHED 0: G loaded with the first set, second set loaded after the first is stored;
+    //       1: both sets prefetched up front; 2: G loaded just before each multiply.
+    __shared__ C2<R> lds[M * TX];
+    constexpr int T = M / 8;
+    const int c = threadIdx.x % TX;
+    const int j = threadIdx.x / TX;
+    const int ntx = (a.nx + TX - 1) / TX;
+    const int blk = a.xcd ? xcd_group(blockIdx.x, gridDim.x) : blockIdx.x;
+    const int tile = blk % ntx;
+    const int y0 = blk / ntx;           // 0 .. M1/2
+    const int y1 = a.gmy - y0;          // mirror row; none for y0 = 0 (M1) and y0 = M1/2 (itself)
+    const bool pair = y0 != 0 && y1 != y0;
+    const bool live = tile * TX + c < a.nx;
+    // uniform bases (scalar registers) + one 32-bit per-lane byte offset for every access
+    char* base = reinterpret_cast<char*>(a.dst) +
+                 (size_t)sizeof(C2<R>) * ((long long)blockIdx.y * a.comp_src + (long long)tile * TX);
+    char* b0 = base + (size_t)sizeof(C2<R>) * ((long long)y0 * a.s_other_src);
+    char* b1 = base + (size_t)sizeof(C2<R>) * ((long long)y1 * a.s_other_src);
+    const size_t rstride = (size_t)sizeof(C2<R>) * T * a.s_col;                      // i -> i + 1
+    const unsigned voff = (unsigned)(sizeof(C2<R>) * ((size_t)j * a.s_col + c));     // < 4 GiB: j < M/8
+    const char* g = reinterpret_cast<const char*>(a.green) + sizeof(R) * ((long long)y0 * ntx + tile) * (M / 2 + 1) * TX;
+    const C2<R>* tw = reinterpret_cast<const C2<R>*>(a.tw);
+    R gv[8];
+    C2<R> v[8], w[8];
+    auto load_g = [&]() {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const int kz = j + i * T;
+            gv[i] = *reinterpret_cast<const R*>(g + (unsigned)(sizeof(R) * (min(kz, M - kz) * TX + c)));
+        }
+    };
+    if (SCHED != 2) load_g();
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const int row = j + i * T;
+        v[i] = (live && row < a.nin) ? *reinterpret_cast<const C2<R>*>(b0 + i * rstride + voff) : C2<R>{0, 0};
+    }
+    if (SCHED == 1) {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const int row = j + i * T;
+            w[i] = (pair && live && row < a.nin) ? *reinterpret_cast<const C2<R>*>(b1 + i * rstride + voff) : C2<R>{0, 0};
+        }
+    }
+    for (int s = 0; s < 2; ++s) {
+        column_fft<R, M, true, TX>(v, lds + c, j, tw);
+        if (SCHED == 2) load_g();
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            v[i].x *= gv[i];
+            v[i].y *= gv[i];
+        }
+        column_fft<R, M, false, TX>(v, lds + c, j, tw);
+        char* b = s == 0 ? b0 : b1;
+        if (live) {
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                const int row = j + i * T;
+                if (row < a.nout) *reinterpret_cast<C2<R>*>(b + i * rstride + voff) = v[i];
+            }
+        }
+        if (!pair) break;   // uniform over the block
+        if (s == 0) {
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                const int row = j + i * T;
+                v[i] = SCHED == 1 ? w[i]
+                                  : ((live && row < a.nin) ? *reinterpret_cast<const C2<R>*>(b1 + i * rstride + voff) : C2<R>{0, 0});
+            }
         }
     }
 }
@@ -232,20 +338,23 @@ struct RowArgs {
     int nvalid;          // valid reals per input row (r2c) / reals to store per output row (c2r)
     int nx;              // complex outputs per row (H + 1)
     long long comp_src, comp_dst;
+    int rev;             // 1: blocks walk the rows from the last one (reads what the previous pass wrote last)
 };
 
 template <typename R, int H, int ROWS>
 __global__ void __launch_bounds__(ROWS * H / 8) x_r2c(RowArgs a) {
-    __shared__ C2<R> lds[H * ROWS];
+    constexpr int HP = H + H / 8;     // padded row buffer
+    __shared__ C2<R> lds[HP * ROWS];
     constexpr int T = H / 8;
     const int rl = threadIdx.x / T;   // row within the block
     const int j = threadIdx.x % T;
-    const int row = blockIdx.x * ROWS + rl;
+    const int blk = a.rev ? gridDim.x - 1 - blockIdx.x : blockIdx.x;
+    const int row = blk * ROWS + rl;
     const bool live = row < a.nrows;
     const int y = live ? row % a.n1 : 0, z = live ? row / a.n1 : 0;
     const R* src = reinterpret_cast<const R*>(a.src) + blockIdx.y * a.comp_src + y * a.s_row_src + z * a.s_z_src;
     C2<R>* dst = reinterpret_cast<C2<R>*>(a.dst) + blockIdx.y * a.comp_dst + y * a.s_row_dst + z * a.s_z_dst;
-    C2<R>* col = lds + rl * H;   // row's buffer (contiguous: element stride 1)
+    C2<R>* col = lds + rl * HP;   // row's buffer
     C2<R> v[8];
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
@@ -256,17 +365,17 @@ __global__ void __launch_bounds__(ROWS * H / 8) x_r2c(RowArgs a) {
         if (live && e1 < a.nvalid) im = src[e1];
         v[i] = {re, im};
     }
-    column_fft<R, H, true, 1>(v, col, j, reinterpret_cast<const C2<R>*>(a.tw));
+    column_fft<R, H, true, 0>(v, col, j, reinterpret_cast<const C2<R>*>(a.tw));
     __syncthreads();
 #pragma unroll
-    for (int i = 0; i < 8; ++i) col[j + i * T] = v[i];
+    for (int i = 0; i < 8; ++i) col[lslot<0>(j + i * T)] = v[i];
     __syncthreads();
     const C2<R>* tw2 = reinterpret_cast<const C2<R>*>(a.tw2);
     if (live) {
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
             const int k = j + i * T;
-            const C2<R> zk = col[k], zc = cconj(col[(H - k) & (H - 1)]);
+            const C2<R> zk = col[lslot<0>(k)], zc = cconj(col[lslot<0>((H - k) & (H - 1))]);
             const C2<R> e = {(zk.x + zc.x) * (R)0.5, (zk.y + zc.y) * (R)0.5};
             const C2<R> df = {(zk.x - zc.x) * (R)0.5, (zk.y - zc.y) * (R)0.5};
             const C2<R> o = {df.y, -df.x};
@@ -283,11 +392,13 @@ __global__ void __launch_bounds__(ROWS * H / 8) x_r2c(RowArgs a) {
 // x[2n] = Re z[n], x[2n+1] = Im z[n]; the imaginary parts of X[0] and X[H] are ignored.
 template <typename R, int H, int ROWS>
 __global__ void __launch_bounds__(ROWS * H / 8) x_c2r(RowArgs a) {
-    __shared__ C2<R> lds[H * ROWS];
+    constexpr int HP = H + H / 8;
+    __shared__ C2<R> lds[HP * ROWS];
     constexpr int T = H / 8;
     const int rl = threadIdx.x / T;
     const int j = threadIdx.x % T;
-    const int row = blockIdx.x * ROWS + rl;
+    const int blk = a.rev ? gridDim.x - 1 - blockIdx.x : blockIdx.x;
+    const int row = blk * ROWS + rl;
     const bool live = row < a.nrows;
     const int y = live ? row % a.n1 : 0, z = live ? row / a.n1 : 0;
     const C2<R>* src = reinterpret_cast<const C2<R>*>(a.src) + blockIdx.y * a.comp_src + y * a.s_row_src + z * a.s_z_src;
@@ -308,7 +419,7 @@ __global__ void __launch_bounds__(ROWS * H / 8) x_c2r(RowArgs a) {
         const C2<R> t = cmul(cconj(tw2[k]), f);
         v[i] = {e.x - t.y, e.y + t.x};
     }
-    column_fft<R, H, false, 1>(v, lds + rl * H, j, reinterpret_cast<const C2<R>*>(a.tw));
+    column_fft<R, H, false, 0>(v, lds + rl * HP, j, reinterpret_cast<const C2<R>*>(a.tw));
     if (live) {
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
@@ -319,13 +430,17 @@ __global__ void __launch_bounds__(ROWS * H / 8) x_c2r(RowArgs a) {
     }
 }
 
-__global__ void compress_green(const double* g, double* out, float* out32, int nx, int n1, int n2, int gy, int gz, int ldx,
-                               double normfact) {
-    const long long tot = (long long)gz * gy * ldx;
+// normfact * Re G of the general path's complex (nx, n1, n2) Green's function,
+// one octant, laid out [ky'][tile][kz'][tx] with tiles of tx x columns (zero past nx)
+__global__ void compress_green(const double* g, double* out, float* out32, int nx, int n1, int n2, int gy, int gz, int tx,
+                               int ntx, double normfact) {
+    const long long tot = (long long)gy * ntx * gz * tx;
     for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < tot; e += (long long)gridDim.x * blockDim.x) {
-        const int x = (int)(e % ldx);
-        const int ky = (int)((e / ldx) % gy);
-        const int kz = (int)(e / ((long long)ldx * gy));
+        const int c = (int)(e % tx);
+        const int kz = (int)((e / tx) % gz);
+        const int tile = (int)((e / ((long long)tx * gz)) % ntx);
+        const int ky = (int)(e / ((long long)tx * gz * ntx));
+        const int x = tile * tx + c;
         double v = 0.0;
         if (x < nx) v = normfact * g[2 * (x + (long long)nx * (ky + (long long)n1 * kz))];
         if (out) out[e] = v;
@@ -354,9 +469,11 @@ struct FastPlan {
     int nx, ldx;       // complex x extent and padded row length
     int lda;
     bool fp32;
+    int zsched = 20;         // z pass schedule (see launch_col): half-width tiles, XCD-grouped
+    int order = 4;           // traversal: bit 0 y_fwd, 1 y_inv, 2 x_c2r, 3 x_r2c walk their batch backwards
     double* s0 = nullptr;   // (ldx, N1, N2) complex
     double* s1 = nullptr;   // (ldx, M1, N2) complex
-    void* green = nullptr;  // normfact * G, [M2/2+1][M1/2+1][ldx]
+    void* green = nullptr;  // normfact * G, [M1/2+1][x tiles][M2/2+1][tile width of the z kernel]
     void* tw[3] = {nullptr, nullptr, nullptr};   // M1, M2 tables and the M0/2 table
     void* twx = nullptr;                          // M0 table (x post-twiddles)
     std::vector<void*> owned;
@@ -367,6 +484,9 @@ struct FastPlan {
     long calls = 0;
     double bytes[5] = {0, 0, 0, 0, 0};   // algorithmic HBM bytes per launch
 };
+
+void build_fast_plan(SolverImpl& s, FastPlan* f, size_t es, int align);
+void destroy_fast_plan(FastPlan* f);
 
 FastPlan* make_fast_plan(SolverImpl& s) {
     if (s.ndim_ != 3 || s.phys_.axis != 0 || s.odiff_ != NOD) return nullptr;
@@ -389,6 +509,27 @@ FastPlan* make_fast_plan(SolverImpl& s) {
     const int align = f->fp32 ? 16 : 8;
     f->ldx = (f->nx + align - 1) / align * align;
     const size_t es = f->fp32 ? 2 * sizeof(float) : 2 * sizeof(double);
+    // per-lane byte offsets of the column kernels are 32-bit: (M/8) rows of the largest stride
+    if ((double)(f->M[2] / 8) * f->ldx * f->M[1] * es >= 4294967296.0) {
+        delete f;
+        return nullptr;
+    }
+    if (const char* e = getenv("FLUPS_ZSCHED")) f->zsched = atoi(e);
+    if (const char* e = getenv("FLUPS_ORDER")) f->order = atoi(e) & 15;
+    if (f->zsched < 0 || f->zsched > 31 || (f->zsched & 3) == 3 || (f->zsched & 8)) {
+        delete f;
+        fail("FLUPS_ZSCHED: bits 0-1 schedule (0, 1, 2), bit 2 half-width tiles, bit 4 XCD-grouped blocks");
+    }
+    try {
+        build_fast_plan(s, f, es, align);
+    } catch (...) {
+        destroy_fast_plan(f);
+        throw;
+    }
+    return f;
+}
+
+void build_fast_plan(SolverImpl& s, FastPlan* f, size_t es, int align) {
     const size_t n0 = (size_t)f->ldx * f->n[1] * f->n[2] * f->lda;
     const size_t n1 = (size_t)f->ldx * f->M[1] * f->n[2] * f->lda;
     HIP_CHECK(hipMalloc(&f->s0, n0 * es));
@@ -396,13 +537,16 @@ FastPlan* make_fast_plan(SolverImpl& s) {
     HIP_CHECK(hipMemset(f->s0, 0, n0 * es));
     HIP_CHECK(hipMemset(f->s1, 0, n1 * es));
     const int gy = f->M[1] / 2 + 1, gz = f->M[2] / 2 + 1;
-    const size_t gn = (size_t)gz * gy * f->ldx;
+    int gtx = f->M[2] == 1024 ? align / 2 : align;   // x tile width of the z kernel
+    if (f->zsched & 4) gtx /= 2;
+    const size_t gn = (size_t)gz * gy * ((f->nx + gtx - 1) / gtx) * gtx;
     HIP_CHECK(hipMalloc(&f->green, gn * (f->fp32 ? sizeof(float) : sizeof(double))));
     // the Green's function of the general path: complex (nx, M1, M2), FP64
     const Shape& gs = s.stage_[2];
     FLUPS_REQUIRE(gs.nf == 2 && gs.n[0] == f->nx && gs.n[1] == f->M[1] && gs.n[2] == f->M[2], "unexpected Green's layout");
     hipLaunchKernelGGL(compress_green, dim3(4096), dim3(256), 0, s.stream_, s.dgreen_, f->fp32 ? nullptr : (double*)f->green,
-                       f->fp32 ? (float*)f->green : nullptr, f->nx, f->M[1], f->M[2], gy, gz, f->ldx, s.normfact_);
+                       f->fp32 ? (float*)f->green : nullptr, f->nx, f->M[1], f->M[2], gy, gz, gtx, (f->nx + gtx - 1) / gtx,
+                       s.normfact_);
     HIP_CHECK(hipGetLastError());
     auto upload = [&](int M) -> void* {
         const auto t = twiddle_table(M);
@@ -438,7 +582,6 @@ FastPlan* make_fast_plan(SolverImpl& s) {
         f->bytes[4] = L * (A * cs + N * rs);
     }
     HIP_CHECK(hipStreamSynchronize(s.stream_));
-    return f;
 }
 
 void destroy_fast_plan(FastPlan* f) {
@@ -455,21 +598,36 @@ void destroy_fast_plan(FastPlan* f) {
 namespace {
 
 template <typename R, int M, int TX, bool FWD, bool FUSED>
-void launch_col(const ColArgs& a, int comps, hipStream_t st) {
-    const int ntx = (a.nx + TX - 1) / TX;
-    dim3 grid(ntx * a.nother, comps);
-    hipLaunchKernelGGL((col_fft<R, M, TX, FWD, FUSED>), grid, dim3(TX * M / 8), 0, st, a);
+void launch_col(const ColArgs& a, int comps, int zsched, hipStream_t st) {
+    if (FUSED) {
+        // z schedules: bit 0-1 = SCHED of z_fused, 4 = half-width tiles (TX/2 columns)
+        const int tx = (zsched & 4) ? TX / 2 : TX;
+        dim3 grid(((a.nx + tx - 1) / tx) * (a.gmy / 2 + 1), comps);
+        switch (zsched & 7) {
+            case 0: hipLaunchKernelGGL((z_fused<R, M, TX, 0>), grid, dim3(TX * M / 8), 0, st, a); break;
+            case 1: hipLaunchKernelGGL((z_fused<R, M, TX, 1>), grid, dim3(TX * M / 8), 0, st, a); break;
+            case 2: hipLaunchKernelGGL((z_fused<R, M, TX, 2>), grid, dim3(TX * M / 8), 0, st, a); break;
+            case 4: hipLaunchKernelGGL((z_fused<R, M, TX / 2, 0>), grid, dim3(TX / 2 * M / 8), 0, st, a); break;
+            case 5: hipLaunchKernelGGL((z_fused<R, M, TX / 2, 1>), grid, dim3(TX / 2 * M / 8), 0, st, a); break;
+            case 6: hipLaunchKernelGGL((z_fused<R, M, TX / 2, 2>), grid, dim3(TX / 2 * M / 8), 0, st, a); break;
+            default: fail("unknown z schedule %d", zsched);
+        }
+    } else {
+        const int ntx = (a.nx + TX - 1) / TX;
+        dim3 grid(ntx * a.nother, comps);
+        hipLaunchKernelGGL((col_fft<R, M, TX, FWD>), grid, dim3(TX * M / 8), 0, st, a);
+    }
     HIP_CHECK(hipGetLastError());
 }
 
 template <typename R, int TX, bool FWD, bool FUSED>
-void col_dispatch(int M, const ColArgs& a, int comps, hipStream_t st) {
+void col_dispatch(int M, const ColArgs& a, int comps, int zsched, hipStream_t st) {
     switch (M) {
-        case 64: launch_col<R, 64, TX, FWD, FUSED>(a, comps, st); break;
-        case 128: launch_col<R, 128, TX, FWD, FUSED>(a, comps, st); break;
-        case 256: launch_col<R, 256, TX, FWD, FUSED>(a, comps, st); break;
-        case 512: launch_col<R, 512, TX, FWD, FUSED>(a, comps, st); break;
-        case 1024: launch_col<R, 1024, TX / 2, FWD, FUSED>(a, comps, st); break;
+        case 64: launch_col<R, 64, TX, FWD, FUSED>(a, comps, zsched, st); break;
+        case 128: launch_col<R, 128, TX, FWD, FUSED>(a, comps, zsched, st); break;
+        case 256: launch_col<R, 256, TX, FWD, FUSED>(a, comps, zsched, st); break;
+        case 512: launch_col<R, 512, TX, FWD, FUSED>(a, comps, zsched, st); break;
+        case 1024: launch_col<R, 1024, TX / 2, FWD, FUSED>(a, comps, zsched, st); break;
         default: fail("unsupported column length %d", M);
     }
 }
@@ -504,7 +662,8 @@ void run_fast(FastPlan* f, const double* rhs, double* field, const View& uin, co
     const long long c0 = ldx * N1 * N2, c1 = ldx * M1 * N2;   // component strides (complex)
     constexpr int TX = sizeof(R) == 8 ? 8 : 16;
     // 1. x r2c: user (real) -> S0
-    RowArgs r;
+    RowArgs r{};
+    r.rev = (f->order >> 3) & 1;
     r.src = rhs;
     r.dst = f->s0;
     r.tw = f->tw[0];
@@ -525,6 +684,7 @@ void run_fast(FastPlan* f, const double* rhs, double* field, const View& uin, co
     if (f->profile) HIP_CHECK(hipEventRecord(f->ev[1], st));
     // 2. y forward: S0 (ldx, N1, N2) -> S1 (ldx, M1, N2)
     ColArgs a{};
+    a.rev = f->order & 1;
     a.src = f->s0;
     a.dst = f->s1;
     a.tw = f->tw[1];
@@ -539,7 +699,7 @@ void run_fast(FastPlan* f, const double* rhs, double* field, const View& uin, co
     a.nout = M1;
     a.comp_src = c0;
     a.comp_dst = c1;
-    col_dispatch<R, TX, true, false>(M1, a, f->lda, st);
+    col_dispatch<R, TX, true, false>(M1, a, f->lda, 0, st);
     if (f->profile) HIP_CHECK(hipEventRecord(f->ev[2], st));
     // 3. z fused: S1 in place, forward, * normfact G, inverse, keep N2
     ColArgs z{};
@@ -561,10 +721,12 @@ void run_fast(FastPlan* f, const double* rhs, double* field, const View& uin, co
     z.gmz = M2;
     z.comp_src = c1;
     z.comp_dst = c1;
-    col_dispatch<R, TX, true, true>(M2, z, f->lda, st);
+    z.xcd = (f->zsched >> 4) & 1;
+    col_dispatch<R, TX, true, true>(M2, z, f->lda, f->zsched, st);
     if (f->profile) HIP_CHECK(hipEventRecord(f->ev[3], st));
     // 4. y inverse: S1 -> S0, keep N1
     ColArgs b = a;
+    b.rev = (f->order >> 1) & 1;
     b.src = f->s1;
     b.dst = f->s0;
     b.s_other_src = ldx * M1;
@@ -573,10 +735,11 @@ void run_fast(FastPlan* f, const double* rhs, double* field, const View& uin, co
     b.nout = N1;
     b.comp_src = c1;
     b.comp_dst = c0;
-    col_dispatch<R, TX, false, false>(M1, b, f->lda, st);
+    col_dispatch<R, TX, false, false>(M1, b, f->lda, 0, st);
     if (f->profile) HIP_CHECK(hipEventRecord(f->ev[4], st));
     // 5. x c2r: S0 -> user (real), keep N0
-    RowArgs q;
+    RowArgs q{};
+    q.rev = (f->order >> 2) & 1;
     q.src = f->s0;
     q.dst = field;
     q.tw = f->tw[0];

@@ -79,6 +79,7 @@ class SolverImpl {
     std::string kdir_;
     friend struct FastPlan;
     friend FastPlan* make_fast_plan(SolverImpl&);
+    friend void build_fast_plan(SolverImpl&, FastPlan*, size_t, int);
 };
 
 std::string kernel_dir();
