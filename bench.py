@@ -56,6 +56,98 @@ def cpu_baseline(n, kernel, reps, threads):
     return reps / sec[0], sec[0]
 
 
+SERVE_SHAPES = [(64, 64, 64), (96, 128, 64), (128, 128, 128), (256, 256, 256), (1024, 1024, 1)]
+SERVE_TEMPLATES = {"unbounded": [[4, 4], [4, 4], [4, 4]], "periodic-x": [[3, 3], [4, 4], [4, 4]],
+                   "symmetric-x/periodic-y": [[0, 1], [3, 3], [4, 4]], "periodic": [[3, 3], [3, 3], [3, 3]]}
+SERVE_KERNELS = {"HEJ_2": 2, "HEJ_6": 4, "LGF_4": 8}
+
+
+def serve_mix(fl, nreq, rate, workers, seed=7):
+    """Poisson arrivals of mixed requests through the library's serving loop, against the plain path
+    (one request at a time, same arrivals). Solvers are warmed (set up once per request kind) first."""
+    import threading
+    rng = np.random.default_rng(seed)
+    kinds = {}
+    reqs = []
+    for _ in range(nreq):
+        ng = SERVE_SHAPES[rng.integers(len(SERVE_SHAPES))]
+        tpl = list(SERVE_TEMPLATES)[rng.integers(len(SERVE_TEMPLATES))]
+        kern = list(SERVE_KERNELS)[rng.integers(len(SERVE_KERNELS))]
+        fp32 = bool(rng.integers(2))
+        key = (ng, tpl, kern, fp32)
+        if key not in kinds:
+            bc = [list(b) for b in SERVE_TEMPLATES[tpl]]
+            n = max(ng)
+            h = [1.0 / n] * 3
+            L = [h[0] * m for m in ng]
+            if ng[2] == 1:
+                bc[2] = [9, 9]
+                L[2] = h[2]
+            kinds[key] = fl.ServeRequest.make(ng, bc, h, L, green=SERVE_KERNELS[kern], fp32=fp32)
+        reqs.append(key)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, nreq))
+    # device buffers: one input per shape, a pool of outputs per shape
+    shapes = sorted({k[0] for k in reqs})
+    inputs, pools = {}, {}
+    for ng in shapes:
+        n0, n1, n2 = ng
+        x = [(np.arange(m) + 0.5) / max(ng) for m in ng]
+        f = np.exp(-((x[2][:, None, None] - 0.45) ** 2 + (x[1][None, :, None] - 0.5) ** 2 +
+                     (x[0][None, None, :] - 0.55) ** 2) / (2 * 0.04 ** 2))[None]
+        inputs[ng] = fl.DeviceArray.from_numpy(f)
+        pools[ng] = [fl.DeviceArray(f.shape) for _ in range(workers + 2)]
+
+    def run(nw):
+        srv = fl.Server(160 << 30, workers=nw, solvers_per_key=1)
+        for key, r in kinds.items():   # warm the solver cache
+            srv.wait(srv.submit(r, inputs[key[0]], pools[key[0]][0]))
+        srv.reset_stats()
+        free = {ng: list(range(len(pools[ng]))) for ng in shapes}
+        cv = threading.Condition()
+        pending = []
+
+        def waiter():
+            for i in range(nreq):
+                with cv:
+                    while len(pending) <= i:
+                        cv.wait()
+                    t, ng, b = pending[i]
+                srv.wait(t)
+                with cv:
+                    free[ng].append(b)
+                    cv.notify_all()
+
+        th = threading.Thread(target=waiter)
+        th.start()
+        t0 = time.perf_counter()
+        for i, key in enumerate(reqs):
+            dt = t0 + arrivals[i] - time.perf_counter()
+            if dt > 0:
+                time.sleep(dt)
+            ng = key[0]
+            with cv:
+                while not free[ng]:
+                    cv.wait()
+                b = free[ng].pop()
+            t = srv.submit(kinds[key], inputs[ng], pools[ng][b])
+            with cv:
+                pending.append((t, ng, b))
+                cv.notify_all()
+        th.join()
+        st = srv.stats()
+        srv.close()
+        return {"p50_ms": round(st["p50_ms"], 3), "p99_ms": round(st["p99_ms"], 3), "mean_ms": round(st["mean_ms"], 3),
+                "solves_per_s": round(st["solves_per_s"], 1), "completed": int(st["completed"]),
+                "failed": int(st["failed"]), "workers": nw}
+
+    served = run(workers)
+    plain = run(1)
+    return {"requests": nreq, "rate_per_s": rate, "kinds": len(kinds),
+            "mix": "shapes 64^3, 96x128x64, 128^3, 256^3, 2-D 1024^2; BC templates unbounded, periodic-x, "
+                   "symmetric-x/periodic-y, periodic; HEJ_2/HEJ_6/LGF_4; FP32 or FP64; Poisson arrivals",
+            "served": served, "plain_one_at_a_time": plain}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -65,6 +157,10 @@ def main():
     ap.add_argument("--kernel", type=int, default=2, help="Green's kernel (2 = HEJ_2)")
     ap.add_argument("--cpu-reps", type=int, default=2, help="timed oracle solves for the CPU baseline (0 = skip)")
     ap.add_argument("--profile-steps", type=int, default=10)
+    ap.add_argument("--fp32-steps", type=int, default=50, help="timed solves of the FP32 variant (0 = skip)")
+    ap.add_argument("--serve-requests", type=int, default=300, help="requests of the served mix (0 = skip)")
+    ap.add_argument("--serve-rate", type=float, default=800.0, help="Poisson arrival rate of the served mix (1/s)")
+    ap.add_argument("--serve-workers", type=int, default=4)
     ap.add_argument("--copy-calib-mb", type=int, default=0,
                     help="also run one plain device-to-device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
@@ -140,6 +236,32 @@ def main():
                 "algorithmic_bytes": dom["bytes"], "avg_us": round(dom["avg_us"], 2)}
     total_bytes = sum(k["bytes"] for k in kernels) if kernels else None
 
+    # FP32 transforms + multiply on the same workload (FP64 caller buffers), error against the FP64 path
+    fp32 = None
+    if args.fp32_steps > 0:
+        s32 = fl.Solver(topo, [[fl.BoundaryType.UNB, fl.BoundaryType.UNB]] * 3, [1.0 / n] * 3, [1.0] * 3,
+                        center_type=[fl.CenterType.CELL_CENTER] * 3)
+        s32.set_green_type(args.kernel)
+        s32.set_precision(True)
+        s32.setup()
+        f32 = fl.DeviceArray(rhs.shape)
+        for _ in range(args.warmup):
+            s32.solve(f32, rhs, fl.SolverType.STD)
+        t0 = time.perf_counter()
+        for _ in range(args.fp32_steps):
+            s32.solve(f32, rhs, fl.SolverType.STD)
+        el32 = time.perf_counter() - t0
+        a64, a32 = field.numpy(), f32.numpy()
+        e2 = float(np.sqrt(np.sum((a32 - a64) ** 2) / np.sum(a64 ** 2)))
+        ei = float(np.max(np.abs(a32 - a64)) / np.max(np.abs(a64)))
+        v32 = args.fp32_steps / el32
+        fp32 = {"value": round(v32, 3), "unit": "solves/s", "ms_per_step": round(el32 / args.fp32_steps * 1e3, 4),
+                "speedup_vs_fp64": round(v32 / solves_per_s, 3), "rel_l2_vs_fp64": e2, "rel_linf_vs_fp64": ei,
+                "path": "fused-fp32" if s32.path == 2 else "general", "steps": args.fp32_steps}
+        del s32, f32
+
+    served = serve_mix(fl, args.serve_requests, args.serve_rate, args.serve_workers) if args.serve_requests > 0 else None
+
     cpu = None
     if args.cpu_reps > 0:
         threads = int(os.environ.get("OMP_NUM_THREADS", str(min(16, os.cpu_count() or 1))))
@@ -161,6 +283,8 @@ def main():
         "latency_ms": {"p50": round(float(np.percentile(lat_ms, 50)), 4), "p99": round(float(np.percentile(lat_ms, 99)), 4)},
         "kernels": [{k: (round(v, 3) if isinstance(v, float) else v) for k, v in kk.items()} for kk in kernels],
         "setup_s": round(setup_s, 3),
+        "fp32": fp32,
+        "served_mix": served,
     }
     print(json.dumps(out))
 

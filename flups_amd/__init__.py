@@ -12,7 +12,7 @@ construction needs the HIP library and a GPU.
 import ctypes
 import enum
 import os
-from ctypes import POINTER, byref, c_bool, c_char_p, c_double, c_int, c_int64, c_size_t, c_void_p
+from ctypes import POINTER, byref, c_bool, c_char_p, c_double, c_int, c_int64, c_long, c_size_t, c_void_p
 
 import numpy as np
 
@@ -129,6 +129,8 @@ def lib():
         L.flups_get_allocSize.argtypes = [c_void_p]
         L.flups_get_allocSize.restype = c_size_t
         L.flups_get_spectralInfo.argtypes = [c_void_p, POINTER(c_double), POINTER(c_double), POINTER(c_double)]
+        L.flups_set_autotune.argtypes = [c_void_p, c_int]
+        L.flups_get_schedule.argtypes = [c_void_p, POINTER(c_int)]
         L.flups_get_path.argtypes = [c_void_p]
         L.flups_get_path.restype = c_int
         L.flups_get_last_solve_ns.argtypes = [c_void_p]
@@ -142,8 +144,125 @@ def lib():
         L.flups_memcpy.restype = c_int
         L.flups_pointer_is_device.argtypes = [c_void_p]
         L.flups_pointer_is_device.restype = c_int
+        L.flups_server_new.restype = c_void_p
+        L.flups_server_new.argtypes = [c_size_t, c_int, c_int]
+        L.flups_server_free.argtypes = [c_void_p]
+        L.flups_server_submit.restype = c_long
+        L.flups_server_submit.argtypes = [c_void_p, POINTER(ServeRequest), c_void_p, c_void_p]
+        L.flups_server_wait.restype = c_int
+        L.flups_server_wait.argtypes = [c_void_p, c_long, POINTER(c_double)]
+        L.flups_server_drain.argtypes = [c_void_p]
+        L.flups_server_stats.argtypes = [c_void_p, POINTER(c_double)]
+        L.flups_server_reset_stats.argtypes = [c_void_p]
+        L.flups_batch_new.restype = c_void_p
+        L.flups_batch_new.argtypes = [c_int, POINTER(ServeRequest), POINTER(c_void_p), POINTER(c_void_p)]
+        L.flups_batch_free.argtypes = [c_void_p]
+        L.flups_batch_run.restype = c_int
+        L.flups_batch_run.argtypes = [c_void_p]
+        L.flups_batch_run_sequential.restype = c_int
+        L.flups_batch_run_sequential.argtypes = [c_void_p]
+        L.flups_batch_last_ms.restype = c_double
+        L.flups_batch_last_ms.argtypes = [c_void_p]
+        L.flups_batch_device_bytes.restype = c_size_t
+        L.flups_batch_device_bytes.argtypes = [c_void_p]
         _lib = L
     return _lib
+
+
+class ServeRequest(ctypes.Structure):
+    """one whole problem for the serving loop / batches (FLUPS_ServeRequest)"""
+    _fields_ = [("nglob", c_int * 3), ("center", c_int), ("bc", (c_int * 2) * 3), ("lda", c_int),
+                ("h", c_double * 3), ("L", c_double * 3), ("green", c_int), ("alpha", c_double), ("fp32", c_int),
+                ("type", c_int), ("order_diff", c_int)]
+
+    @classmethod
+    def make(cls, nglob, bc, h, L, green=GreenType.HEJ_2, lda=1, center=CenterType.CELL_CENTER, fp32=False,
+             solver_type=SolverType.STD, order_diff=DiffType.NOD, alpha=0.0):
+        r = cls()
+        for d in range(3):
+            r.nglob[d] = int(nglob[d])
+            r.h[d] = float(h[d])
+            r.L[d] = float(L[d])
+            for side in range(2):
+                r.bc[d][side] = int(bc[d][side])
+        r.center, r.lda, r.green, r.alpha = int(center), int(lda), int(green), float(alpha)
+        r.fp32, r.type, r.order_diff = int(bool(fp32)), int(solver_type), int(order_diff)
+        return r
+
+    def shape(self):
+        """caller array shape: (lda, n2, n1, n0), dense"""
+        return (self.lda, self.nglob[2], self.nglob[1], self.nglob[0])
+
+
+SERVER_STATS = ("completed", "failed", "p50_ms", "p99_ms", "mean_ms", "solves_per_s", "cache_hits", "cache_misses",
+                "evictions", "peak_device_bytes", "device_bytes", "cached_solvers")
+
+
+class Server:
+    """serving loop of the library: worker threads, solver cache keyed by the request, HBM-budget admission"""
+
+    def __init__(self, hbm_budget_bytes, workers=4, solvers_per_key=1):
+        self._s = lib().flups_server_new(int(hbm_budget_bytes), int(workers), int(solvers_per_key))
+        _check(self._s, "flups_server_new")
+
+    def submit(self, req, rhs, field):
+        t = lib().flups_server_submit(self._s, ctypes.byref(req), _data_ptr(rhs), _data_ptr(field))
+        _check(t > 0, "flups_server_submit")
+        return t
+
+    def wait(self, ticket):
+        lat = c_double()
+        _check(lib().flups_server_wait(self._s, ticket, ctypes.byref(lat)), "flups_server_wait")
+        return lat.value
+
+    def drain(self):
+        lib().flups_server_drain(self._s)
+
+    def stats(self):
+        out = (c_double * 12)()
+        lib().flups_server_stats(self._s, out)
+        return dict(zip(SERVER_STATS, list(out)))
+
+    def reset_stats(self):
+        lib().flups_server_reset_stats(self._s)
+
+    def close(self):
+        if getattr(self, "_s", None):
+            lib().flups_server_free(self._s)
+            self._s = None
+
+    def __del__(self):
+        self.close()
+
+
+class Batch:
+    """independent problems on device buffers, set up once, solved by one hipGraph launch"""
+
+    def __init__(self, reqs, rhs, fields):
+        n = len(reqs)
+        self._reqs = (ServeRequest * n)(*reqs)
+        self._rhs = (c_void_p * n)(*[_data_ptr(a) for a in rhs])
+        self._fields = (c_void_p * n)(*[_data_ptr(a) for a in fields])
+        self._b = lib().flups_batch_new(n, self._reqs, self._rhs, self._fields)
+        _check(self._b, "flups_batch_new")
+
+    def run(self):
+        _check(lib().flups_batch_run(self._b), "flups_batch_run")
+        return lib().flups_batch_last_ms(self._b)
+
+    def run_sequential(self):
+        _check(lib().flups_batch_run_sequential(self._b), "flups_batch_run_sequential")
+
+    def device_bytes(self):
+        return lib().flups_batch_device_bytes(self._b)
+
+    def close(self):
+        if getattr(self, "_b", None):
+            lib().flups_batch_free(self._b)
+            self._b = None
+
+    def __del__(self):
+        self.close()
 
 
 def last_error():
@@ -289,6 +408,18 @@ class Solver:
     def set_stream(self, stream_handle):
         lib().flups_set_stream(self._solver, stream_handle)
 
+    def set_autotune(self, on=True):
+        lib().flups_clear_last_error()
+        lib().flups_set_autotune(self._solver, 1 if on else 0)
+        _check(not lib().flups_get_last_error(), "flups_set_autotune")
+
+    @property
+    def schedule(self):
+        """(z pass, y passes, x order) schedule of the fused pipeline; -1s on the general path"""
+        out = (c_int * 3)()
+        lib().flups_get_schedule(self._solver, out)
+        return tuple(out)
+
     def set_precision(self, fp32):
         lib().flups_set_precision(self._solver, 1 if fp32 else 0)
 
@@ -319,7 +450,7 @@ class Solver:
 
     @property
     def path(self):
-        """1 = fused unbounded pipeline, 0 = general pencil path"""
+        """0 = general pencil path (FP64), 1 = fused unbounded pipeline in FP64, 2 = fused in FP32"""
         return lib().flups_get_path(self._solver)
 
     @property

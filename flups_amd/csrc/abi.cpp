@@ -33,6 +33,13 @@ void report(const char* where, const std::string& msg) {
     std::fprintf(stderr, "[flups] %s\n", g_last_error.c_str());
     if (g_abort) std::abort();
 }
+}  // namespace
+
+namespace flups {
+void report_error(const char* where, const std::string& msg) { report(where, msg); }
+}  // namespace flups
+
+namespace {
 template <typename F>
 auto guard(const char* where, F&& f) -> decltype(f()) {
     try {
@@ -173,7 +180,11 @@ void flups_set_alpha(FLUPS_Solver* s, const double alpha) {
     if (s) s->impl->set_alpha(alpha);
 }
 void flups_set_precision(FLUPS_Solver* s, int fp32) {
-    if (s) s->impl->set_fp32(fp32 != 0);
+    guard("flups_set_precision", [&]() {
+        FLUPS_REQUIRE(s, "NULL solver");
+        s->impl->set_fp32(fp32 != 0);
+        return 0;
+    });
 }
 void flups_setup(FLUPS_Solver* s, const bool changeComm) {
     (void)changeComm;
@@ -195,6 +206,16 @@ void flups_get_spectralInfo(FLUPS_Solver* s, double kfact[3], double koffset[3],
     if (s) s->impl->spectral_info(kfact, koffset, symstart);
 }
 int flups_get_path(FLUPS_Solver* s) { return s ? s->impl->path() : -1; }
+void flups_set_autotune(FLUPS_Solver* s, int on) {
+    guard("flups_set_autotune", [&]() {
+        FLUPS_REQUIRE(s, "NULL solver");
+        s->impl->set_autotune(on != 0);
+        return 0;
+    });
+}
+void flups_get_schedule(FLUPS_Solver* s, int sched[3]) {
+    if (s && sched) s->impl->schedule(sched);
+}
 void flups_profile_enable(FLUPS_Solver* s, int on) {
     if (s) s->impl->profile(on != 0);
 }

@@ -25,7 +25,12 @@
 // them. The x rows use the half-length packed complex FFT for r2c / c2r.
 #include <hip/hip_runtime.h>
 
+#include <array>
 #include <cmath>
+#include <cstdio>
+#include <map>
+#include <mutex>
+#include <string>
 #include <vector>
 
 #include "solver.hpp"
@@ -215,7 +220,8 @@ __global__ void __launch_bounds__(TX * M / 8) col_fft(ColArgs a) {
     const int c = threadIdx.x % TX;
     const int j = threadIdx.x / TX;
     const int ntx = (a.nx + TX - 1) / TX;
-    const int blk = a.rev ? gridDim.x - 1 - blockIdx.x : blockIdx.x;
+    int blk = a.xcd ? xcd_group(blockIdx.x, gridDim.x) : blockIdx.x;
+    if (a.rev) blk = gridDim.x - 1 - blk;
     const int tile = blk % ntx;
     const int other = blk / ntx;
     const int x = tile * TX + c;
@@ -352,7 +358,7 @@ __global__ void __launch_bounds__(ROWS * H / 8) x_r2c(RowArgs a) {
     const int row = blk * ROWS + rl;
     const bool live = row < a.nrows;
     const int y = live ? row % a.n1 : 0, z = live ? row / a.n1 : 0;
-    const R* src = reinterpret_cast<const R*>(a.src) + blockIdx.y * a.comp_src + y * a.s_row_src + z * a.s_z_src;
+    const double* src = reinterpret_cast<const double*>(a.src) + blockIdx.y * a.comp_src + y * a.s_row_src + z * a.s_z_src;
     C2<R>* dst = reinterpret_cast<C2<R>*>(a.dst) + blockIdx.y * a.comp_dst + y * a.s_row_dst + z * a.s_z_dst;
     C2<R>* col = lds + rl * HP;   // row's buffer
     C2<R> v[8];
@@ -361,8 +367,8 @@ __global__ void __launch_bounds__(ROWS * H / 8) x_r2c(RowArgs a) {
         const int n = j + i * T;
         const int e0 = 2 * n, e1 = 2 * n + 1;
         R re = 0, im = 0;
-        if (live && e0 < a.nvalid) re = src[e0];
-        if (live && e1 < a.nvalid) im = src[e1];
+        if (live && e0 < a.nvalid) re = (R)src[e0];
+        if (live && e1 < a.nvalid) im = (R)src[e1];
         v[i] = {re, im};
     }
     column_fft<R, H, true, 0>(v, col, j, reinterpret_cast<const C2<R>*>(a.tw));
@@ -402,7 +408,7 @@ __global__ void __launch_bounds__(ROWS * H / 8) x_c2r(RowArgs a) {
     const bool live = row < a.nrows;
     const int y = live ? row % a.n1 : 0, z = live ? row / a.n1 : 0;
     const C2<R>* src = reinterpret_cast<const C2<R>*>(a.src) + blockIdx.y * a.comp_src + y * a.s_row_src + z * a.s_z_src;
-    R* dst = reinterpret_cast<R*>(a.dst) + blockIdx.y * a.comp_dst + y * a.s_row_dst + z * a.s_z_dst;
+    double* dst = reinterpret_cast<double*>(a.dst) + blockIdx.y * a.comp_dst + y * a.s_row_dst + z * a.s_z_dst;
     const C2<R>* tw2 = reinterpret_cast<const C2<R>*>(a.tw2);
     C2<R> v[8];
 #pragma unroll
@@ -424,8 +430,8 @@ __global__ void __launch_bounds__(ROWS * H / 8) x_c2r(RowArgs a) {
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
             const int n = j + i * T;
-            if (2 * n < a.nvalid) dst[2 * n] = v[i].x;
-            if (2 * n + 1 < a.nvalid) dst[2 * n + 1] = v[i].y;
+            if (2 * n < a.nvalid) dst[2 * n] = (double)v[i].x;
+            if (2 * n + 1 < a.nvalid) dst[2 * n + 1] = (double)v[i].y;
         }
     }
 }
@@ -470,9 +476,11 @@ struct FastPlan {
     int lda;
     bool fp32;
     int zsched = 20;         // z pass schedule (see launch_col): half-width tiles, XCD-grouped
+    int ysched = 0;          // y passes: bit 2 half-width tiles, bit 4 XCD-grouped
     int order = 4;           // traversal: bit 0 y_fwd, 1 y_inv, 2 x_c2r, 3 x_r2c walk their batch backwards
-    double* s0 = nullptr;   // (ldx, N1, N2) complex
-    double* s1 = nullptr;   // (ldx, M1, N2) complex
+    int gtx = 0;             // x tile width of the compressed Green's layout
+    void* s0 = nullptr;     // (ldx, N1, N2) complex, FP64 or FP32
+    void* s1 = nullptr;     // (ldx, M1, N2) complex
     void* green = nullptr;  // normfact * G, [M1/2+1][x tiles][M2/2+1][tile width of the z kernel]
     void* tw[3] = {nullptr, nullptr, nullptr};   // M1, M2 tables and the M0/2 table
     void* twx = nullptr;                          // M0 table (x post-twiddles)
@@ -483,6 +491,7 @@ struct FastPlan {
     double ms[5] = {0, 0, 0, 0, 0};
     long calls = 0;
     double bytes[5] = {0, 0, 0, 0, 0};   // algorithmic HBM bytes per launch
+    size_t dev_bytes = 0;                 // device memory held by the plan
 };
 
 void build_fast_plan(SolverImpl& s, FastPlan* f, size_t es, int align);
@@ -516,6 +525,7 @@ FastPlan* make_fast_plan(SolverImpl& s) {
     }
     if (const char* e = getenv("FLUPS_ZSCHED")) f->zsched = atoi(e);
     if (const char* e = getenv("FLUPS_ORDER")) f->order = atoi(e) & 15;
+    if (const char* e = getenv("FLUPS_YSCHED")) f->ysched = atoi(e) & 20;
     if (f->zsched < 0 || f->zsched > 31 || (f->zsched & 3) == 3 || (f->zsched & 8)) {
         delete f;
         fail("FLUPS_ZSCHED: bits 0-1 schedule (0, 1, 2), bit 2 half-width tiles, bit 4 XCD-grouped blocks");
@@ -529,25 +539,44 @@ FastPlan* make_fast_plan(SolverImpl& s) {
     return f;
 }
 
+// x tile width of the z kernel for the plan's schedule: half the stage tile for
+// M2 = 1024 (LDS) or for half-width schedules
+static int z_tile(const FastPlan* f) {
+    const int align = f->fp32 ? 16 : 8;
+    return (f->M[2] == 1024 || (f->zsched & 4)) ? align / 2 : align;
+}
+
+// (re)write the compressed Green's function in the layout of the current z schedule
+static void layout_green(FastPlan* f, const double* dgreen, double normfact, hipStream_t st) {
+    const int gtx = z_tile(f);
+    if (gtx == f->gtx) return;
+    FLUPS_REQUIRE(dgreen, "the full Green's function is needed to change the z tile width");
+    const int gy = f->M[1] / 2 + 1, gz = f->M[2] / 2 + 1;
+    hipLaunchKernelGGL(compress_green, dim3(4096), dim3(256), 0, st, dgreen, f->fp32 ? nullptr : (double*)f->green,
+                       f->fp32 ? (float*)f->green : nullptr, f->nx, f->M[1], f->M[2], gy, gz, gtx, (f->nx + gtx - 1) / gtx,
+                       normfact);
+    HIP_CHECK(hipGetLastError());
+    f->gtx = gtx;
+}
+
 void build_fast_plan(SolverImpl& s, FastPlan* f, size_t es, int align) {
     const size_t n0 = (size_t)f->ldx * f->n[1] * f->n[2] * f->lda;
     const size_t n1 = (size_t)f->ldx * f->M[1] * f->n[2] * f->lda;
     HIP_CHECK(hipMalloc(&f->s0, n0 * es));
     HIP_CHECK(hipMalloc(&f->s1, n1 * es));
+    f->dev_bytes += (n0 + n1) * es;
     HIP_CHECK(hipMemset(f->s0, 0, n0 * es));
     HIP_CHECK(hipMemset(f->s1, 0, n1 * es));
     const int gy = f->M[1] / 2 + 1, gz = f->M[2] / 2 + 1;
-    int gtx = f->M[2] == 1024 ? align / 2 : align;   // x tile width of the z kernel
-    if (f->zsched & 4) gtx /= 2;
-    const size_t gn = (size_t)gz * gy * ((f->nx + gtx - 1) / gtx) * gtx;
+    // room for the widest Green's layout (full-width tiles: ldx columns)
+    const size_t gn = (size_t)gz * gy * f->ldx;
     HIP_CHECK(hipMalloc(&f->green, gn * (f->fp32 ? sizeof(float) : sizeof(double))));
+    f->dev_bytes += gn * (f->fp32 ? sizeof(float) : sizeof(double));
     // the Green's function of the general path: complex (nx, M1, M2), FP64
     const Shape& gs = s.stage_[2];
     FLUPS_REQUIRE(gs.nf == 2 && gs.n[0] == f->nx && gs.n[1] == f->M[1] && gs.n[2] == f->M[2], "unexpected Green's layout");
-    hipLaunchKernelGGL(compress_green, dim3(4096), dim3(256), 0, s.stream_, s.dgreen_, f->fp32 ? nullptr : (double*)f->green,
-                       f->fp32 ? (float*)f->green : nullptr, f->nx, f->M[1], f->M[2], gy, gz, gtx, (f->nx + gtx - 1) / gtx,
-                       s.normfact_);
-    HIP_CHECK(hipGetLastError());
+    f->gtx = 0;
+    layout_green(f, s.dgreen_, s.normfact_, s.stream_);
     auto upload = [&](int M) -> void* {
         const auto t = twiddle_table(M);
         void* d = nullptr;
@@ -560,6 +589,7 @@ void build_fast_plan(SolverImpl& s, FastPlan* f, size_t es, int align) {
             HIP_CHECK(hipMemcpy(d, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice));
         }
         f->owned.push_back(d);
+        f->dev_bytes += t.size() * (f->fp32 ? sizeof(float) : sizeof(double));
         return d;
     };
     f->tw[0] = upload(f->M[0] / 2);
@@ -569,17 +599,18 @@ void build_fast_plan(SolverImpl& s, FastPlan* f, size_t es, int align) {
     for (int i = 0; i < 6; ++i) HIP_CHECK(hipEventCreate(&f->ev[i]));
     {
         // minimum traffic of each pass: valid elements read + written (plus G for the fused pass)
-        const double rs = sizeof(double), cs = 2 * rs;
+        const double us = sizeof(double);                          // user arrays are always FP64
+        const double rs = f->fp32 ? sizeof(float) : sizeof(double), cs = 2 * rs;
         const double N = (double)f->n[0] * f->n[1] * f->n[2];
         const double A = (double)f->nx * f->n[1] * f->n[2];        // S0 complex elements
         const double B = (double)f->nx * f->M[1] * f->n[2];        // S1 complex elements
         const double G = (double)gz * gy * f->nx * rs;
         const double L = f->lda;
-        f->bytes[0] = L * (N * rs + A * cs);
+        f->bytes[0] = L * (N * us + A * cs);
         f->bytes[1] = L * (A * cs + B * cs);
         f->bytes[2] = L * (2 * B * cs) + G;
         f->bytes[3] = L * (B * cs + A * cs);
-        f->bytes[4] = L * (A * cs + N * rs);
+        f->bytes[4] = L * (A * cs + N * us);
     }
     HIP_CHECK(hipStreamSynchronize(s.stream_));
 }
@@ -612,9 +643,11 @@ void launch_col(const ColArgs& a, int comps, int zsched, hipStream_t st) {
             case 6: hipLaunchKernelGGL((z_fused<R, M, TX / 2, 2>), grid, dim3(TX / 2 * M / 8), 0, st, a); break;
             default: fail("unknown z schedule %d", zsched);
         }
+    } else if (zsched & 4) {   // y schedule: half-width tiles
+        dim3 grid(((a.nx + TX / 2 - 1) / (TX / 2)) * a.nother, comps);
+        hipLaunchKernelGGL((col_fft<R, M, TX / 2, FWD>), grid, dim3(TX / 2 * M / 8), 0, st, a);
     } else {
-        const int ntx = (a.nx + TX - 1) / TX;
-        dim3 grid(ntx * a.nother, comps);
+        dim3 grid(((a.nx + TX - 1) / TX) * a.nother, comps);
         hipLaunchKernelGGL((col_fft<R, M, TX, FWD>), grid, dim3(TX * M / 8), 0, st, a);
     }
     HIP_CHECK(hipGetLastError());
@@ -627,7 +660,7 @@ void col_dispatch(int M, const ColArgs& a, int comps, int zsched, hipStream_t st
         case 128: launch_col<R, 128, TX, FWD, FUSED>(a, comps, zsched, st); break;
         case 256: launch_col<R, 256, TX, FWD, FUSED>(a, comps, zsched, st); break;
         case 512: launch_col<R, 512, TX, FWD, FUSED>(a, comps, zsched, st); break;
-        case 1024: launch_col<R, 1024, TX / 2, FWD, FUSED>(a, comps, zsched, st); break;
+        case 1024: launch_col<R, 1024, TX / 2, FWD, FUSED>(a, comps, zsched & ~4, st); break;   // already half width
         default: fail("unsupported column length %d", M);
     }
 }
@@ -678,13 +711,13 @@ void run_fast(FastPlan* f, const double* rhs, double* field, const View& uin, co
     r.nx = f->nx;
     r.comp_src = uin.sc;
     r.comp_dst = c0;
-    static_assert(sizeof(R) == 8, "FP32 stage buffers are not wired yet");
     if (f->profile) HIP_CHECK(hipEventRecord(f->ev[0], st));
     row_dispatch<R>(true, H, r, f->lda, st);
     if (f->profile) HIP_CHECK(hipEventRecord(f->ev[1], st));
     // 2. y forward: S0 (ldx, N1, N2) -> S1 (ldx, M1, N2)
     ColArgs a{};
     a.rev = f->order & 1;
+    a.xcd = (f->ysched >> 4) & 1;
     a.src = f->s0;
     a.dst = f->s1;
     a.tw = f->tw[1];
@@ -699,7 +732,7 @@ void run_fast(FastPlan* f, const double* rhs, double* field, const View& uin, co
     a.nout = M1;
     a.comp_src = c0;
     a.comp_dst = c1;
-    col_dispatch<R, TX, true, false>(M1, a, f->lda, 0, st);
+    col_dispatch<R, TX, true, false>(M1, a, f->lda, f->ysched, st);
     if (f->profile) HIP_CHECK(hipEventRecord(f->ev[2], st));
     // 3. z fused: S1 in place, forward, * normfact G, inverse, keep N2
     ColArgs z{};
@@ -735,7 +768,7 @@ void run_fast(FastPlan* f, const double* rhs, double* field, const View& uin, co
     b.nout = N1;
     b.comp_src = c1;
     b.comp_dst = c0;
-    col_dispatch<R, TX, false, false>(M1, b, f->lda, 0, st);
+    col_dispatch<R, TX, false, false>(M1, b, f->lda, f->ysched, st);
     if (f->profile) HIP_CHECK(hipEventRecord(f->ev[4], st));
     // 5. x c2r: S0 -> user (real), keep N0
     RowArgs q{};
@@ -762,7 +795,84 @@ void run_fast(FastPlan* f, const double* rhs, double* field, const View& uin, co
 }  // namespace
 
 void fast_solve(FastPlan* f, double* field, const double* rhs, const View& uin, const View& uout, hipStream_t s) {
-    run_fast<double>(f, rhs, field, uin, uout, s);
+    if (f->fp32) run_fast<float>(f, rhs, field, uin, uout, s);
+    else run_fast<double>(f, rhs, field, uin, uout, s);
+}
+
+size_t fast_device_bytes(const FastPlan* f) { return f->dev_bytes; }
+
+void fast_schedule(const FastPlan* f, int out[3]) {
+    out[0] = f->zsched;
+    out[1] = f->ysched;
+    out[2] = f->order;
+}
+
+// Schedule autotuner: times the candidate schedules of the column passes (tile
+// width and XCD grouping of the z pass, XCD grouping of the y passes, traversal
+// order of the x passes) on the plan's own shapes and keeps the fastest. Every
+// candidate computes the same arithmetic per column, so results do not depend
+// on the choice. Choices are remembered per shape for the life of the process.
+namespace {
+std::mutex g_tune_mu;
+std::map<std::string, std::array<int, 3>> g_tune;
+}  // namespace
+
+void fast_autotune(FastPlan* f, double* out, const double* in, const View& uin, const View& uout, const double* dgreen,
+                   double normfact, hipStream_t st) {
+    char key[160];
+    std::snprintf(key, sizeof(key), "%d,%d,%d|%d,%d,%d|%d|%d", f->n[0], f->n[1], f->n[2], f->M[0], f->M[1], f->M[2], f->lda,
+                  (int)f->fp32);
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto it = g_tune.find(key);
+        if (it != g_tune.end()) {
+            f->zsched = it->second[0];
+            f->ysched = it->second[1];
+            f->order = it->second[2];
+            layout_green(f, dgreen, normfact, st);
+            return;
+        }
+    }
+    auto run = [&]() {
+        if (f->fp32) run_fast<float>(f, in, out, uin, uout, st);
+        else run_fast<double>(f, in, out, uin, uout, st);
+    };
+    const bool prof = f->profile;
+    f->profile = false;
+    hipEvent_t e0, e1;
+    HIP_CHECK(hipEventCreate(&e0));
+    HIP_CHECK(hipEventCreate(&e1));
+    float best = 1e30f;
+    std::array<int, 3> pick = {f->zsched, f->ysched, f->order};
+    for (int z : {20, 16, 4, 0})
+        for (int y : {0, 16})
+            for (int o : {4, 0}) {
+                f->zsched = z;
+                f->ysched = y;
+                f->order = o;
+                layout_green(f, dgreen, normfact, st);
+                run();
+                HIP_CHECK(hipEventRecord(e0, st));
+                for (int r = 0; r < 3; ++r) run();
+                HIP_CHECK(hipEventRecord(e1, st));
+                HIP_CHECK(hipEventSynchronize(e1));
+                float ms = 0;
+                HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+                if (ms < best) {
+                    best = ms;
+                    pick = {z, y, o};
+                }
+            }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    f->profile = prof;
+    f->zsched = pick[0];
+    f->ysched = pick[1];
+    f->order = pick[2];
+    layout_green(f, dgreen, normfact, st);
+    HIP_CHECK(hipStreamSynchronize(st));
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune[key] = pick;
 }
 
 // after the stream has completed: accumulate the per-kernel times of the last solve

@@ -1,0 +1,555 @@
+// Serving loop and batches on one GPU. Neither has a counterpart in the
+// reference, whose solver serves one problem per process.
+//
+// Server: solve requests of mixed shapes, boundary conditions, kernels and
+// precisions are admitted as worker threads and device memory free up.
+// Solvers (plan, Green's function on the device, stage arrays) are cached by
+// the full problem description and evicted least-recently-used when the HBM
+// budget needs room. Every cached solver has its own stream; a worker owns a
+// solver for the duration of one solve, so requests that arrive while others
+// run overlap on the device. A request whose solver would not fit waits; later
+// requests may still run on solvers that are already cached (no new memory),
+// so a large request is not starved by small ones.
+//
+// Batch: a fixed list of problems, set up once; their solves are captured
+// into one hipGraph (one independent branch per problem) and replayed with a
+// single launch.
+#define FLUPS_BUILDING_LIBRARY 1
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstdio>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "../../include/flups.h"
+#include "solver.hpp"
+
+namespace flups {
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+std::string key_of(const FLUPS_ServeRequest& r) {
+    char b[640];
+    std::snprintf(b, sizeof(b), "n%d,%d,%d c%d bc%d%d%d%d%d%d l%d h%a,%a,%a L%a,%a,%a g%d al%a p%d d%d", r.nglob[0],
+                  r.nglob[1], r.nglob[2], r.center, r.bc[0][0], r.bc[0][1], r.bc[1][0], r.bc[1][1], r.bc[2][0], r.bc[2][1],
+                  r.lda, r.h[0], r.h[1], r.h[2], r.L[0], r.L[1], r.L[2], r.green, r.alpha, r.fp32, r.order_diff);
+    return b;
+}
+
+void validate(const FLUPS_ServeRequest& r) {
+    FLUPS_REQUIRE(r.lda >= 1 && r.lda <= 3, "request lda must be 1..3");
+    FLUPS_REQUIRE(r.type == 0 || r.type == 1, "request type must be STD (0) or ROT (1)");
+    for (int d = 0; d < 3; ++d) FLUPS_REQUIRE(r.nglob[d] >= 1, "request nglob must be >= 1");
+}
+
+std::unique_ptr<SolverImpl> make_solver(const FLUPS_ServeRequest& r) {
+    Topo t;
+    t.axis = 0;
+    t.lda = r.lda;
+    t.nf = 1;
+    t.alignment = sizeof(double);   // dense caller arrays: (lda, n2, n1, n0), no row padding
+    for (int d = 0; d < 3; ++d) t.nglob[d] = r.nglob[d];
+    t.cmpt_sizes();
+    std::vector<int> bc(6 * r.lda);
+    for (int d = 0; d < 3; ++d)
+        for (int s = 0; s < 2; ++s)
+            for (int c = 0; c < r.lda; ++c) bc[(d * 2 + s) * r.lda + c] = r.bc[d][s];
+    const int center[3] = {r.center, r.center, r.center};
+    auto s = std::make_unique<SolverImpl>(&t, bc.data(), r.h, r.L, r.order_diff, center);
+    s->set_green(r.green);
+    if (r.alpha > 0) s->set_alpha(r.alpha);
+    s->set_fp32(r.fp32 != 0);
+    return s;
+}
+
+struct Cached {
+    std::string key;
+    std::unique_ptr<SolverImpl> solver;
+    hipStream_t stream = nullptr;
+    size_t bytes = 0;   // reserved before setup, measured after
+    bool busy = true;
+    bool ready = false;
+    uint64_t last_use = 0;
+    ~Cached() {
+        solver.reset();
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+struct Job {
+    long id = 0;
+    FLUPS_ServeRequest req{};
+    std::string key;
+    size_t need = 0;
+    const double* rhs = nullptr;
+    double* field = nullptr;
+    Clock::time_point t_submit, t_done;
+    int state = 0;   // 0 queued, 1 running, 2 done, 3 failed
+    std::string error;
+};
+
+double pct(std::vector<double> v, double p) {
+    if (v.empty()) return 0.0;
+    std::sort(v.begin(), v.end());
+    const double x = p / 100.0 * (double)(v.size() - 1);
+    const size_t i = (size_t)x;
+    const double f = x - (double)i;
+    return i + 1 < v.size() ? v[i] * (1 - f) + v[i + 1] * f : v[i];
+}
+
+}  // namespace
+
+class Server {
+   public:
+    Server(size_t budget, int nworkers, int per_key) : budget_(budget), per_key_(std::max(1, per_key)) {
+        HIP_CHECK(hipGetDevice(&device_));
+        for (int i = 0; i < std::max(1, nworkers); ++i) workers_.emplace_back([this] { work(); });
+    }
+    ~Server() {
+        {
+            std::lock_guard<std::mutex> lk(m_);
+            stop_ = true;
+            for (Job* j : queue_) {
+                j->state = 3;
+                j->error = "server shut down";
+            }
+            queue_.clear();
+        }
+        cv_work_.notify_all();
+        cv_done_.notify_all();
+        for (auto& t : workers_) t.join();
+        cache_.clear();
+    }
+
+    long submit(const FLUPS_ServeRequest& r, const double* rhs, double* field) {
+        validate(r);
+        FLUPS_REQUIRE(rhs && field, "request buffers must not be NULL");
+        auto j = std::make_unique<Job>();
+        j->req = r;
+        j->key = key_of(r);
+        {
+            std::lock_guard<std::mutex> lk(m_);
+            auto it = need_.find(j->key);
+            if (it != need_.end()) j->need = it->second;
+        }
+        if (!j->need) {   // host-side plan only: sizes for the admission decision
+            auto s = make_solver(r);
+            j->need = s->predicted_device_bytes(r.type);
+        }
+        FLUPS_REQUIRE(j->need <= budget_, "request needs %zu bytes of device memory, above the server budget %zu", j->need,
+                      budget_);
+        j->rhs = rhs;
+        j->field = field;
+        std::lock_guard<std::mutex> lk(m_);
+        need_[j->key] = j->need;
+        j->id = next_id_++;
+        j->t_submit = Clock::now();
+        if (!any_) {
+            first_submit_ = j->t_submit;
+            any_ = true;
+        }
+        queue_.push_back(j.get());
+        const long id = j->id;
+        jobs_[id] = std::move(j);
+        cv_work_.notify_one();
+        return id;
+    }
+
+    // 1 = done, 0 = failed (message in *err); removes the request
+    int wait(long id, double* latency_ms, std::string* err) {
+        std::unique_lock<std::mutex> lk(m_);
+        auto it = jobs_.find(id);
+        FLUPS_REQUIRE(it != jobs_.end(), "unknown request %ld", id);
+        Job* j = it->second.get();
+        cv_done_.wait(lk, [&] { return j->state >= 2; });
+        const int ok = j->state == 2;
+        if (latency_ms) *latency_ms = std::chrono::duration<double, std::milli>(j->t_done - j->t_submit).count();
+        if (err) *err = j->error;
+        jobs_.erase(it);
+        return ok;
+    }
+
+    void drain() {
+        std::unique_lock<std::mutex> lk(m_);
+        cv_done_.wait(lk, [&] {
+            for (auto& kv : jobs_)
+                if (kv.second->state < 2) return false;
+            return true;
+        });
+    }
+
+    void stats(double out[12]) {
+        std::lock_guard<std::mutex> lk(m_);
+        const double win = any_ && done_ ? std::chrono::duration<double>(last_done_ - first_submit_).count() : 0.0;
+        out[0] = (double)lat_.size();
+        out[1] = (double)failed_;
+        out[2] = pct(lat_, 50);
+        out[3] = pct(lat_, 99);
+        double sum = 0;
+        for (double v : lat_) sum += v;
+        out[4] = lat_.empty() ? 0.0 : sum / (double)lat_.size();
+        out[5] = win > 0 ? (double)lat_.size() / win : 0.0;
+        out[6] = (double)hits_;
+        out[7] = (double)misses_;
+        out[8] = (double)evictions_;
+        out[9] = (double)peak_;
+        out[10] = (double)used_;
+        out[11] = (double)cache_.size();
+    }
+
+    void reset_stats() {
+        std::lock_guard<std::mutex> lk(m_);
+        lat_.clear();
+        failed_ = hits_ = misses_ = evictions_ = 0;
+        peak_ = used_;
+        any_ = done_ = false;
+    }
+
+   private:
+    // under the lock: the first queued request that can start now
+    bool pick(Job*& job, Cached*& slot, bool& create, std::vector<std::unique_ptr<Cached>>& evict) {
+        bool blocked = false;   // an earlier request waits for memory: later ones may not take new memory
+        for (auto it = queue_.begin(); it != queue_.end(); ++it) {
+            Job* j = *it;
+            Cached* idle = nullptr;
+            int count = 0;
+            for (auto& e : cache_)
+                if (e->key == j->key) {
+                    ++count;
+                    if (!e->busy && e->ready && !idle) idle = e.get();
+                }
+            if (idle) {
+                idle->busy = true;
+                job = j;
+                slot = idle;
+                create = false;
+                ++hits_;
+                queue_.erase(it);
+                return true;
+            }
+            if (blocked || count >= per_key_) continue;
+            // room for a new solver: evict idle solvers, least recently used first, if that is enough
+            size_t idle_bytes = 0;
+            for (auto& e : cache_)
+                if (!e->busy) idle_bytes += e->bytes;
+            if (used_ - idle_bytes + j->need > budget_) {
+                blocked = true;
+                continue;
+            }
+            while (used_ + j->need > budget_) {
+                auto lru = cache_.end();
+                for (auto e = cache_.begin(); e != cache_.end(); ++e)
+                    if (!(*e)->busy && (lru == cache_.end() || (*e)->last_use < (*lru)->last_use)) lru = e;
+                if (lru == cache_.end()) break;
+                used_ -= (*lru)->bytes;
+                evict.push_back(std::move(*lru));
+                cache_.erase(lru);
+                ++evictions_;
+            }
+            if (used_ + j->need > budget_) {
+                blocked = true;
+                continue;
+            }
+            auto c = std::make_unique<Cached>();
+            c->key = j->key;
+            c->bytes = j->need;
+            used_ += j->need;
+            peak_ = std::max(peak_, used_);
+            slot = c.get();
+            cache_.push_back(std::move(c));
+            job = j;
+            create = true;
+            ++misses_;
+            queue_.erase(it);
+            return true;
+        }
+        return false;
+    }
+
+    void work() {
+        (void)hipSetDevice(device_);
+        std::unique_lock<std::mutex> lk(m_);
+        while (true) {
+            Job* j = nullptr;
+            Cached* c = nullptr;
+            bool create = false;
+            std::vector<std::unique_ptr<Cached>> evict;
+            cv_work_.wait(lk, [&] { return stop_ || pick(j, c, create, evict); });
+            if (!j) return;   // stop
+            j->state = 1;
+            lk.unlock();
+            evict.clear();   // frees device memory outside the lock
+            std::string err;
+            try {
+                if (create) {
+                    c->solver = make_solver(j->req);
+                    HIP_CHECK(hipStreamCreate(&c->stream));
+                    c->solver->set_stream(c->stream);
+                    c->solver->setup();
+                    c->solver->prepare(j->req.type);
+                }
+                c->solver->solve(j->field, j->rhs, j->req.type);
+            } catch (const Error& e) {
+                err = e.msg;
+            } catch (const std::exception& e) {
+                err = e.what();
+            }
+            lk.lock();
+            j->t_done = Clock::now();
+            if (create) {
+                const size_t real = err.empty() ? c->solver->device_bytes() : 0;
+                used_ = used_ - c->bytes + real;
+                peak_ = std::max(peak_, used_);
+                c->bytes = real;
+                c->ready = err.empty();
+            }
+            c->busy = false;
+            c->last_use = ++tick_;
+            std::unique_ptr<Cached> dead;
+            if (!c->ready) {   // failed set-up: drop the slot
+                for (auto e = cache_.begin(); e != cache_.end(); ++e)
+                    if (e->get() == c) {
+                        used_ -= (*e)->bytes;
+                        dead = std::move(*e);
+                        cache_.erase(e);
+                        break;
+                    }
+            }
+            if (err.empty()) {
+                j->state = 2;
+                lat_.push_back(std::chrono::duration<double, std::milli>(j->t_done - j->t_submit).count());
+                last_done_ = j->t_done;
+                done_ = true;
+            } else {
+                j->state = 3;
+                j->error = err;
+                ++failed_;
+            }
+            cv_done_.notify_all();
+            cv_work_.notify_all();
+            if (dead) {
+                lk.unlock();
+                dead.reset();
+                lk.lock();
+            }
+        }
+    }
+
+    size_t budget_;
+    int per_key_;
+    int device_ = 0;
+    std::mutex m_;
+    std::condition_variable cv_work_, cv_done_;
+    std::deque<Job*> queue_;
+    std::map<long, std::unique_ptr<Job>> jobs_;
+    std::map<std::string, size_t> need_;
+    std::vector<std::unique_ptr<Cached>> cache_;
+    std::vector<std::thread> workers_;
+    size_t used_ = 0, peak_ = 0;
+    long next_id_ = 1;
+    uint64_t tick_ = 0;
+    bool stop_ = false, any_ = false, done_ = false;
+    std::vector<double> lat_;
+    long hits_ = 0, misses_ = 0, evictions_ = 0, failed_ = 0;
+    Clock::time_point first_submit_, last_done_;
+};
+
+class Batch {
+   public:
+    Batch(int n, const FLUPS_ServeRequest* reqs, const double* const* rhs, double* const* field) {
+        FLUPS_REQUIRE(n >= 1, "a batch needs at least one problem");
+        HIP_CHECK(hipStreamCreate(&main_));
+        for (int i = 0; i < n; ++i) {
+            validate(reqs[i]);
+            FLUPS_REQUIRE(rhs[i] && field[i], "batch buffers must not be NULL");
+            FLUPS_REQUIRE(is_device(rhs[i]) && is_device(field[i]), "batch problem %d: buffers must be device memory", i);
+            Item it;
+            it.solver = make_solver(reqs[i]);
+            HIP_CHECK(hipStreamCreate(&it.stream));
+            it.solver->set_stream(it.stream);
+            it.solver->setup();
+            it.solver->prepare(reqs[i].type);
+            HIP_CHECK(hipEventCreateWithFlags(&it.done, hipEventDisableTiming));
+            it.rhs = rhs[i];
+            it.field = field[i];
+            it.type = reqs[i].type;
+            items_.push_back(std::move(it));
+        }
+        HIP_CHECK(hipEventCreateWithFlags(&fork_, hipEventDisableTiming));
+        HIP_CHECK(hipEventCreate(&t0_));
+        HIP_CHECK(hipEventCreate(&t1_));
+        // one graph: fork from the main stream, one branch per problem, join
+        HIP_CHECK(hipStreamBeginCapture(main_, hipStreamCaptureModeThreadLocal));
+        HIP_CHECK(hipEventRecord(fork_, main_));
+        for (auto& it : items_) {
+            HIP_CHECK(hipStreamWaitEvent(it.stream, fork_, 0));
+            it.solver->enqueue(it.field, it.rhs, it.type);
+            HIP_CHECK(hipEventRecord(it.done, it.stream));
+            HIP_CHECK(hipStreamWaitEvent(main_, it.done, 0));
+        }
+        HIP_CHECK(hipStreamEndCapture(main_, &graph_));
+        HIP_CHECK(hipGraphInstantiate(&exec_, graph_, nullptr, nullptr, 0));
+    }
+    ~Batch() {
+        if (exec_) (void)hipGraphExecDestroy(exec_);
+        if (graph_) (void)hipGraphDestroy(graph_);
+        for (auto& it : items_) {
+            it.solver.reset();
+            if (it.done) (void)hipEventDestroy(it.done);
+            if (it.stream) (void)hipStreamDestroy(it.stream);
+        }
+        if (fork_) (void)hipEventDestroy(fork_);
+        if (t0_) (void)hipEventDestroy(t0_);
+        if (t1_) (void)hipEventDestroy(t1_);
+        if (main_) (void)hipStreamDestroy(main_);
+    }
+    void run() {
+        HIP_CHECK(hipEventRecord(t0_, main_));
+        HIP_CHECK(hipGraphLaunch(exec_, main_));
+        HIP_CHECK(hipEventRecord(t1_, main_));
+        HIP_CHECK(hipStreamSynchronize(main_));
+        float ms = 0;
+        HIP_CHECK(hipEventElapsedTime(&ms, t0_, t1_));
+        last_ms_ = ms;
+    }
+    // the same problems one after the other on the main stream, without the graph
+    void run_sequential() {
+        for (auto& it : items_) {
+            it.solver->enqueue(it.field, it.rhs, it.type);
+            HIP_CHECK(hipStreamSynchronize(it.stream));
+        }
+    }
+    double last_ms() const { return last_ms_; }
+    size_t device_bytes() const {
+        size_t b = 0;
+        for (auto& it : items_) b += it.solver->device_bytes();
+        return b;
+    }
+
+   private:
+    static bool is_device(const void* p) {
+        hipPointerAttribute_t a;
+        if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+            (void)hipGetLastError();
+            return false;
+        }
+        return a.type == hipMemoryTypeDevice;
+    }
+    struct Item {
+        std::unique_ptr<SolverImpl> solver;
+        hipStream_t stream = nullptr;
+        hipEvent_t done = nullptr;
+        const double* rhs = nullptr;
+        double* field = nullptr;
+        int type = 0;
+    };
+    std::vector<Item> items_;
+    hipStream_t main_ = nullptr;
+    hipEvent_t fork_ = nullptr, t0_ = nullptr, t1_ = nullptr;
+    hipGraph_t graph_ = nullptr;
+    hipGraphExec_t exec_ = nullptr;
+    double last_ms_ = 0;
+};
+
+}  // namespace flups
+
+// ---- C entry points ----
+struct FLUPS_ServerS {
+    flups::Server s;
+    FLUPS_ServerS(size_t b, int w, int k) : s(b, w, k) {}
+};
+struct FLUPS_BatchS {
+    flups::Batch b;
+    FLUPS_BatchS(int n, const FLUPS_ServeRequest* r, const double* const* x, double* const* y) : b(n, r, x, y) {}
+};
+
+namespace flups {
+void report_error(const char* where, const std::string& msg);   // abi.cpp
+}
+
+template <typename F>
+static auto sguard(const char* where, F&& f) -> decltype(f()) {
+    try {
+        return f();
+    } catch (const flups::Error& e) {
+        flups::report_error(where, e.msg);
+    } catch (const std::exception& e) {
+        flups::report_error(where, e.what());
+    }
+    return decltype(f())();
+}
+
+extern "C" {
+
+FLUPS_Server* flups_server_new(size_t hbm_budget_bytes, int workers, int solvers_per_key) {
+    return sguard("flups_server_new", [&]() -> FLUPS_Server* {
+        return reinterpret_cast<FLUPS_Server*>(new FLUPS_ServerS(hbm_budget_bytes, workers, solvers_per_key));
+    });
+}
+void flups_server_free(FLUPS_Server* s) { delete reinterpret_cast<FLUPS_ServerS*>(s); }
+long flups_server_submit(FLUPS_Server* s, const FLUPS_ServeRequest* req, const double* rhs, double* field) {
+    return sguard("flups_server_submit", [&]() -> long {
+        FLUPS_REQUIRE(s && req, "NULL server or request");
+        return reinterpret_cast<FLUPS_ServerS*>(s)->s.submit(*req, rhs, field);
+    });
+}
+int flups_server_wait(FLUPS_Server* s, long ticket, double* latency_ms) {
+    return sguard("flups_server_wait", [&]() -> int {
+        FLUPS_REQUIRE(s, "NULL server");
+        std::string err;
+        const int ok = reinterpret_cast<FLUPS_ServerS*>(s)->s.wait(ticket, latency_ms, &err);
+        FLUPS_REQUIRE(ok, "request %ld failed: %s", ticket, err.c_str());
+        return 1;
+    });
+}
+void flups_server_drain(FLUPS_Server* s) {
+    sguard("flups_server_drain", [&]() {
+        FLUPS_REQUIRE(s, "NULL server");
+        reinterpret_cast<FLUPS_ServerS*>(s)->s.drain();
+        return 0;
+    });
+}
+void flups_server_stats(FLUPS_Server* s, double out[12]) {
+    sguard("flups_server_stats", [&]() {
+        FLUPS_REQUIRE(s && out, "NULL server or output");
+        reinterpret_cast<FLUPS_ServerS*>(s)->s.stats(out);
+        return 0;
+    });
+}
+void flups_server_reset_stats(FLUPS_Server* s) {
+    if (s) reinterpret_cast<FLUPS_ServerS*>(s)->s.reset_stats();
+}
+
+FLUPS_Batch* flups_batch_new(int n, const FLUPS_ServeRequest* reqs, const double* const* rhs, double* const* field) {
+    return sguard("flups_batch_new", [&]() -> FLUPS_Batch* {
+        FLUPS_REQUIRE(reqs && rhs && field, "NULL batch description");
+        return reinterpret_cast<FLUPS_Batch*>(new FLUPS_BatchS(n, reqs, rhs, field));
+    });
+}
+void flups_batch_free(FLUPS_Batch* b) { delete reinterpret_cast<FLUPS_BatchS*>(b); }
+int flups_batch_run(FLUPS_Batch* b) {
+    return sguard("flups_batch_run", [&]() -> int {
+        FLUPS_REQUIRE(b, "NULL batch");
+        reinterpret_cast<FLUPS_BatchS*>(b)->b.run();
+        return 1;
+    });
+}
+int flups_batch_run_sequential(FLUPS_Batch* b) {
+    return sguard("flups_batch_run_sequential", [&]() -> int {
+        FLUPS_REQUIRE(b, "NULL batch");
+        reinterpret_cast<FLUPS_BatchS*>(b)->b.run_sequential();
+        return 1;
+    });
+}
+double flups_batch_last_ms(FLUPS_Batch* b) { return b ? reinterpret_cast<FLUPS_BatchS*>(b)->b.last_ms() : 0.0; }
+size_t flups_batch_device_bytes(FLUPS_Batch* b) { return b ? reinterpret_cast<FLUPS_BatchS*>(b)->b.device_bytes() : 0; }
+
+}  // extern "C"

@@ -25,6 +25,10 @@ void fast_solve(FastPlan* f, double* field, const double* rhs, const View& uin, 
 void fast_collect(FastPlan* f);
 void fast_profile(FastPlan* f, bool on);
 int fast_profile_get(FastPlan* f, int i, const char** name, double* ms, long* calls, double* bytes);
+size_t fast_device_bytes(const FastPlan* f);
+void fast_schedule(const FastPlan* f, int out[3]);
+void fast_autotune(FastPlan* f, double* out, const double* in, const View& uin, const View& uout, const double* dgreen,
+                   double normfact, hipStream_t st);
 
 // ---------------------------------------------------------------------------
 // topology sizes (src/Topology.cpp:81-98): the fastest direction is padded so
@@ -452,6 +456,27 @@ void SolverImpl::setup() {
     HIP_CHECK(hipEventCreate(&ev1_));
     compute_green();
     fast_ = make_fast_plan(*this);
+    if (fast_ && (autotune_ || std::getenv("FLUPS_AUTOTUNE"))) {
+        // time the candidate schedules on scratch arrays of the caller's layout
+        const size_t n = phys_.memsize();
+        double *din = nullptr, *dout = nullptr;
+        HIP_CHECK(hipMalloc(&din, n * sizeof(double)));
+        HIP_CHECK(hipMalloc(&dout, n * sizeof(double)));
+        HIP_CHECK(hipMemsetAsync(din, 0, n * sizeof(double), stream_));
+        try {
+            fast_autotune(fast_, dout, din, user_view(din), user_view(dout), dgreen_, normfact_, stream_);
+        } catch (...) {
+            (void)hipFree(din);
+            (void)hipFree(dout);
+            throw;
+        }
+        HIP_CHECK(hipFree(din));
+        HIP_CHECK(hipFree(dout));
+    }
+    if (fast_ && lda_ != 3) {   // the fused pipeline keeps its own compressed copy; ROT needs lda = 3
+        HIP_CHECK(hipFree(dgreen_));
+        dgreen_ = nullptr;
+    }
     if (!fast_) {
         for (int ip = 0; ip < ndim_; ++ip) {
             const size_t n = std::max(hat_[ip].count(), stage_[ip].count());
@@ -503,12 +528,65 @@ void SolverImpl::solve_generic(double* field, const double* rhs, int type) {
     }
 }
 
-void SolverImpl::solve(double* field, const double* rhs, int type) {
+void SolverImpl::check_type(int type) const {
     FLUPS_REQUIRE(setup_done_, "flups_setup must be called before flups_solve");
-    FLUPS_REQUIRE(field && rhs, "field and rhs must not be NULL");
     FLUPS_REQUIRE(type == 0 || type == 1, "unknown solver type %d", type);
     FLUPS_REQUIRE(type == 0 || lda_ == 3, "the ROT solver needs lda = 3 (vectors)");
     FLUPS_REQUIRE(type == 0 || odiff_ != NOD, "the ROT solver needs a differentiation order (SPE, FD2, FD4, FD6)");
+}
+
+// buffers a solve of this type needs beyond those of setup (the general path's stage
+// arrays when the fused pipeline cannot run it)
+void SolverImpl::prepare(int type) {
+    check_type(type);
+    if (fast_ && type == 0) return;
+    if (!dstage_[0]) {
+        for (int ip = 0; ip < ndim_; ++ip) {
+            const size_t m = std::max(hat_[ip].count(), stage_[ip].count());
+            HIP_CHECK(hipMalloc(&dstage_[ip], m * sizeof(double)));
+        }
+    }
+}
+
+// queue the solve's kernels on the solver's stream; field and rhs are device
+// pointers, nothing is allocated or synchronised (usable under graph capture
+// once prepare(type) has run)
+void SolverImpl::enqueue(double* field, const double* rhs, int type) {
+    if (fast_ && type == 0) fast_solve(fast_, field, rhs, user_view(const_cast<double*>(rhs)), user_view(field), stream_);
+    else {
+        FLUPS_REQUIRE(dstage_[0] && dgreen_, "solver not prepared for solve type %d", type);
+        solve_generic(field, rhs, type);
+    }
+}
+
+size_t SolverImpl::device_bytes() const {
+    size_t b = 0;
+    for (const auto& kv : tw_) b += 2 * sizeof(double) * (size_t)kv.first;
+    for (int ip = 0; ip < 3; ++ip)
+        if (dstage_[ip]) b += sizeof(double) * std::max(hat_[ip].count(), stage_[ip].count());
+    if (dgreen_) b += sizeof(double) * gpost_[ndim_ - 1].count();
+    b += sizeof(double) * staging_n_;
+    if (fast_) b += fast_device_bytes(fast_);
+    return b;
+}
+
+// upper bound of the device memory setup + a solve of this type will hold (setup's
+// transient Green's buffers included), for admission decisions before setup
+size_t SolverImpl::predicted_device_bytes(int type) const {
+    size_t cap = 0, stages = 0;
+    for (int ip = 0; ip < ndim_; ++ip) {
+        cap = std::max(cap, std::max(gpre_[ip].count(), gpost_[ip].count()));
+        stages += std::max(hat_[ip].count(), stage_[ip].count());
+    }
+    const size_t setup_peak = 2 * cap * sizeof(double);
+    const size_t steady = (gpost_[ndim_ - 1].count() + stages) * sizeof(double);
+    (void)type;
+    return std::max(setup_peak, steady) + (size_t)(4u << 20);   // + twiddle tables, LGF tables, slack
+}
+
+void SolverImpl::solve(double* field, const double* rhs, int type) {
+    FLUPS_REQUIRE(field && rhs, "field and rhs must not be NULL");
+    prepare(type);
     hipPointerAttribute_t at_f, at_r;
     auto is_dev = [](const void* p, hipPointerAttribute_t* a) {
         if (hipPointerGetAttributes(a, p) != hipSuccess) { (void)hipGetLastError(); return false; }
@@ -531,16 +609,7 @@ void SolverImpl::solve(double* field, const double* rhs, int type) {
         if (!fdev) dout = dstaging_ + n;
     }
     HIP_CHECK(hipEventRecord(ev0_, stream_));
-    if (fast_ && type == 0) fast_solve(fast_, dout, din, user_view(din), user_view(dout), stream_);
-    else {
-        if (fast_ && !dstage_[0]) {
-            for (int ip = 0; ip < ndim_; ++ip) {
-                const size_t m = std::max(hat_[ip].count(), stage_[ip].count());
-                HIP_CHECK(hipMalloc(&dstage_[ip], m * sizeof(double)));
-            }
-        }
-        solve_generic(dout, din, type);
-    }
+    enqueue(dout, din, type);
     HIP_CHECK(hipEventRecord(ev1_, stream_));
     if (!fdev) HIP_CHECK(hipMemcpyAsync(field, dout, n * sizeof(double), hipMemcpyDeviceToHost, stream_));
     HIP_CHECK(hipStreamSynchronize(stream_));
@@ -552,6 +621,10 @@ void SolverImpl::solve(double* field, const double* rhs, int type) {
 
 void SolverImpl::profile(bool on) {
     if (fast_) fast_profile(fast_, on);
+}
+void SolverImpl::schedule(int out[3]) const {
+    out[0] = out[1] = out[2] = -1;
+    if (fast_) fast_schedule(fast_, out);
 }
 int SolverImpl::profile_get(int i, const char** name, double* ms, long* calls, double* bytes) const {
     return fast_ ? fast_profile_get(fast_, i, name, ms, calls, bytes) : 0;

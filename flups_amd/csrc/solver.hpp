@@ -33,12 +33,25 @@ class SolverImpl {
     void set_green(int t) { gtype_ = t; }
     void set_alpha(double a) { alpha_ = a; }
     void set_stream(hipStream_t s) { stream_ = s; }
-    void set_fp32(bool f) { fp32_ = f; }
+    void set_fp32(bool f) {
+        FLUPS_REQUIRE(!setup_done_, "flups_set_precision must be called before flups_setup");
+        fp32_ = f;
+    }
     void setup();
     void solve(double* field, const double* rhs, int type);
+    void prepare(int type);                                          // allocate what a solve of this type needs
+    void enqueue(double* field, const double* rhs, int type);        // device pointers, no sync, no allocation
+    size_t device_bytes() const;                                     // device memory held now
+    size_t predicted_device_bytes(int type) const;                   // bound for setup + solve, before setup
+    bool is_setup() const { return setup_done_; }
+    void set_autotune(bool on) {
+        FLUPS_REQUIRE(!setup_done_, "flups_set_autotune must be called before flups_setup");
+        autotune_ = on;
+    }
+    void schedule(int out[3]) const;   // fused pipeline schedule (z, y, order), -1 on the general path
     size_t alloc_size() const;
     void spectral_info(double kf[3], double ko[3], double ss[3]) const;
-    int path() const { return fast_ ? 1 : 0; }
+    int path() const { return fast_ ? (fp32_ ? 2 : 1) : 0; }   // 0 general FP64, 1 fused FP64, 2 fused FP32
     double last_solve_ns() const { return last_ns_; }
     int ndim() const { return ndim_; }
     // per-kernel timing of the fused pipeline (HIP events on the solve stream)
@@ -59,10 +72,11 @@ class SolverImpl {
                          const double symstart[3], double length, int nunb);
     View user_view(double* p) const;
     void solve_generic(double* field, const double* rhs, int type);
+    void check_type(int type) const;
 
     Topo phys_;
     int lda_ = 1, ndim_ = 3, odiff_ = NOD, gtype_ = CHAT_2;
-    bool node_ = false, fp32_ = false, setup_done_ = false;
+    bool node_ = false, fp32_ = false, setup_done_ = false, autotune_ = false;
     double h_[3], L_[3], alpha_ = 2.0, normfact_ = 1, volfact_ = 1;
     DirPlan fwd_[3], bwd_[3], bwdd_[3], gpl_[3];
     Shape hat_[3], stage_[3], gpre_[3], gpost_[3];

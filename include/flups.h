@@ -141,10 +141,17 @@ const char* flups_get_last_error(void);        /* message of the last failed che
 void        flups_clear_last_error(void);
 void        flups_set_abort_on_error(int on);  /* 1: abort() on a failed check, as the reference */
 /* 0 = FP64 throughout (default), 1 = FP32 transforms and multiply with FP64-computed
-   Green's function; caller buffers stay double. Call before flups_setup. */
+   Green's function and twiddles, rounded once; caller buffers stay double. Call
+   before flups_setup. Applies to the fused unbounded pipeline; the general path
+   stays FP64 (flups_get_path tells which one runs). */
 void        flups_set_precision(FLUPS_Solver* s, int fp32);
-/* 1 if the solver runs the fused unbounded pipeline, 0 for the general pencil path */
+/* 0 = general pencil path (FP64), 1 = fused unbounded pipeline in FP64, 2 = fused in FP32 */
 int         flups_get_path(FLUPS_Solver* s);
+/* 1: flups_setup times the fused pipeline's candidate schedules on this problem and keeps
+   the fastest (remembered per shape for the process; FLUPS_AUTOTUNE=1 turns it on for all) */
+void        flups_set_autotune(FLUPS_Solver* s, int on);
+/* schedule of the fused pipeline: z pass, y passes, x traversal order (-1 on the general path) */
+void        flups_get_schedule(FLUPS_Solver* s, int sched[3]);
 /* nanoseconds of device time of the last flups_solve (HIP events on the solver's stream) */
 double      flups_get_last_solve_ns(FLUPS_Solver* s);
 /* per-kernel timing of the fused pipeline: enable (resets the counters), then read entry idx:
@@ -159,6 +166,48 @@ void*       flups_device_malloc(size_t bytes);
 void        flups_device_free(void* p);
 int         flups_memcpy(void* dst, const void* src, size_t bytes);   /* any direction; 1 on success */
 int         flups_pointer_is_device(const void* p);
+
+/* ---- serving and batching (no counterpart in the reference: its solver serves one
+   problem per process). A request describes a whole problem; caller arrays are dense
+   doubles shaped (lda, n2, n1, n0), device or host memory. ---- */
+typedef struct {
+    int    nglob[3];
+    int    center;        /* FLUPS_CenterType, all directions */
+    int    bc[3][2];      /* FLUPS_BoundaryType per direction and side, same for every component */
+    int    lda;           /* 1 (scalar) or 3 (vector) */
+    double h[3], L[3];
+    int    green;         /* FLUPS_GreenType */
+    double alpha;         /* regularisation factor; <= 0 keeps the default */
+    int    fp32;          /* 1: FP32 transforms where the fused pipeline runs */
+    int    type;          /* FLUPS_SolverType: STD or ROT */
+    int    order_diff;    /* FLUPS_DiffType (ROT) */
+} FLUPS_ServeRequest;
+
+typedef struct FLUPS_ServerS FLUPS_Server;
+typedef struct FLUPS_BatchS  FLUPS_Batch;
+
+/* serving loop: `workers` threads; solvers cached by request description and evicted
+   least-recently-used within `hbm_budget_bytes`; at most `solvers_per_key` solvers per
+   description. Requests arriving while others run are admitted as workers and memory
+   free up; their solves overlap on the device. */
+FLUPS_Server* flups_server_new(size_t hbm_budget_bytes, int workers, int solvers_per_key);
+void          flups_server_free(FLUPS_Server* s);
+long          flups_server_submit(FLUPS_Server* s, const FLUPS_ServeRequest* req, const double* rhs, double* field);
+int           flups_server_wait(FLUPS_Server* s, long ticket, double* latency_ms);   /* 1 done, 0 failed */
+void          flups_server_drain(FLUPS_Server* s);
+/* out: completed, failed, p50 ms, p99 ms, mean ms, solves/s (first submit to last completion),
+   cache hits, cache misses, evictions, peak device bytes, device bytes now, cached solvers */
+void          flups_server_stats(FLUPS_Server* s, double out[12]);
+void          flups_server_reset_stats(FLUPS_Server* s);
+
+/* batch: n independent problems (device buffers), set up once, solved by one hipGraph
+   launch (one branch per problem) */
+FLUPS_Batch*  flups_batch_new(int n, const FLUPS_ServeRequest* reqs, const double* const* rhs, double* const* field);
+void          flups_batch_free(FLUPS_Batch* b);
+int           flups_batch_run(FLUPS_Batch* b);              /* 1 on success; waits for completion */
+int           flups_batch_run_sequential(FLUPS_Batch* b);   /* same problems one by one, no graph */
+double        flups_batch_last_ms(FLUPS_Batch* b);          /* device time of the last flups_batch_run */
+size_t        flups_batch_device_bytes(FLUPS_Batch* b);
 
 #ifdef __cplusplus
 }

@@ -263,3 +263,72 @@ def test_rot_diff_orders(odiff, bc):
     f_ora = orc.solve([n] * 3, bc, 3, h, [1, 1, 1], 1, 2, rhs, typ=1, odiff=odiff)
     e2, ei = rel_err(f_gpu, f_ora)
     assert e2 <= TOL and ei <= TOL, (e2, ei)
+
+
+FP32_TOL = 1e-5
+
+
+@pytest.mark.parametrize("case", [((64, 64, 64), 1, 1, 2), ((128, 96, 64), 1, 1, 4), ((65, 65, 65), 0, 1, 2),
+                                  ((64, 64, 64), 1, 3, 8)], ids=["64-hej2", "128x96x64", "node65", "lda3-lgf4"])
+def test_fused_fp32_vs_oracle(case):
+    """FP32 transforms + multiply (FP64-computed Green's function) against the FP64 oracle: relative
+    L2 and Linf errors <= 1e-5"""
+    nglob, center, lda, kernel = case
+    n0, n1, n2 = nglob
+    h = [1.0 / (max(nglob) - (1 - center))] * 3
+    L = [h[0] * (n - (1 - center)) for n in nglob]
+    rhs = np.concatenate([blobs_rhs(max(nglob), 5 + c)[:, :n2, :n1, :n0] for c in range(lda)])
+    bc = [[4, 4]] * 3
+    topo = fl.Topology(0, lda, list(nglob), [1, 1, 1], False)
+    s = fl.Solver(topo, bc, h, L, center_type=[center] * 3)
+    s.set_green_type(kernel)
+    s.set_precision(True)
+    s.setup()
+    assert s.path == 2
+    f32 = np.zeros_like(rhs)
+    s.solve(f32, rhs)
+    f_ora = orc.solve(list(nglob), bc, lda, h, L, center, kernel, rhs)
+    e2, ei = rel_err(f32, f_ora)
+    assert e2 <= FP32_TOL and ei <= FP32_TOL, (e2, ei)
+    assert e2 > 1e-12   # really computed in single precision
+
+
+@pytest.mark.parametrize("nglob", [(64, 64, 64), (128, 64, 32)])
+def test_schedules_are_bitwise_equal_and_autotune(monkeypatch, nglob):
+    """every schedule of the fused pipeline computes the same arithmetic: results are bitwise
+    equal to the default schedule's; the autotuner picks one of them"""
+    rng = np.random.default_rng(3)
+    n0, n1, n2 = nglob
+    rhs = rng.normal(size=(1, n2, n1, n0))
+    h = [1.0 / max(nglob)] * 3
+    L = [h[0] * n for n in nglob]
+
+    def run(env, tune=False):
+        for k in ("FLUPS_ZSCHED", "FLUPS_YSCHED", "FLUPS_ORDER"):
+            monkeypatch.delenv(k, raising=False)
+        for k, v in env.items():
+            monkeypatch.setenv(k, str(v))
+        topo = fl.Topology(0, 1, list(nglob), [1, 1, 1], False)
+        s = fl.Solver(topo, [[4, 4]] * 3, h, L)
+        s.set_green_type(2)
+        if tune:
+            s.set_autotune(True)
+        s.setup()
+        assert s.path == 1
+        f = np.zeros_like(rhs)
+        s.solve(f, rhs)
+        return f, s.schedule
+
+    ref, sched = run({})
+    assert sched == (20, 0, 4)
+    for z in (0, 1, 2, 4, 16, 20):
+        for y, o in ((0, 0), (16, 15), (4, 4)):
+            f, sc = run({"FLUPS_ZSCHED": z, "FLUPS_YSCHED": y, "FLUPS_ORDER": o})
+            assert sc == (z, y, o)
+            assert np.array_equal(f, ref), (z, y, o)
+    f, sc = run({}, tune=True)
+    assert sc[0] in (0, 4, 16, 20) and sc[1] in (0, 16) and sc[2] in (0, 4)
+    assert np.array_equal(f, ref)
+    f_ora = orc.solve(list(nglob), [[4, 4]] * 3, 1, h, L, 1, 2, rhs)
+    e2, ei = rel_err(ref, f_ora)
+    assert e2 <= TOL and ei <= TOL

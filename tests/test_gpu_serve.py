@@ -1,0 +1,163 @@
+"""GPU tests of the serving loop and of batched (hipGraph) solves.
+
+Each problem's result must be the same alone and in the mix: the kernels are
+deterministic (no atomics, no order-dependent reductions), so the check is
+bitwise. FP64 results are also checked against the oracle (relative L2 / Linf
+<= 1e-12) and FP32 ones against it at <= 1e-5.
+"""
+import numpy as np
+import pytest
+
+try:   # torch's HIP runtime must start before libflups' one in a process using both
+    import torch  # noqa: F401
+except Exception:   # pragma: no cover
+    torch = None
+
+import flups_amd as fl
+from tests import oracle_lib as orc
+
+pytestmark = pytest.mark.gpu
+
+# BC templates of the mixed workloads
+TEMPLATES = {
+    "unb": [[4, 4], [4, 4], [4, 4]],
+    "perx": [[3, 3], [4, 4], [4, 4]],
+    "symx-pery": [[0, 1], [3, 3], [4, 4]],
+    "per": [[3, 3], [3, 3], [3, 3]],
+}
+
+
+def blob_field(shape, seed):
+    """lda components of 4 Gaussian blobs on the cell grid of [0, n_d h]"""
+    lda, n2, n1, n0 = shape
+    rng = np.random.default_rng(seed)
+    n = max(n0, n1, n2)
+    x = [(np.arange(m) + 0.5) / n for m in (n0, n1, n2)]
+    out = np.zeros(shape)
+    for c in range(lda):
+        for _ in range(4):
+            cen = rng.uniform(0.3, 0.7, 3) * np.array([n0, n1, n2]) / n
+            s = rng.uniform(0.03, 0.06)
+            g = [np.exp(-(x[d] - cen[d]) ** 2 / (2 * s * s)) for d in range(3)]
+            out[c] += rng.normal() * g[2][:, None, None] * g[1][None, :, None] * g[0][None, None, :]
+    return out
+
+
+def problems():
+    """a mix of shapes, BC templates, kernels, lda, precision and a 2-D grid"""
+    P = []
+    spec = [((32, 32, 32), "unb", 2, 1, False), ((48, 40, 32), "unb", 4, 1, True), ((32, 32, 32), "perx", 2, 1, False),
+            ((32, 48, 32), "symx-pery", 8, 1, False), ((32, 32, 32), "per", 4, 1, False), ((64, 64, 64), "unb", 2, 1, True),
+            ((32, 32, 32), "unb", 4, 3, False), ((64, 64, 1), "unb2d", 2, 1, False), ((40, 32, 48), "perx", 8, 1, False),
+            ((32, 32, 32), "unb", 2, 1, False)]
+    for k, (ng, tpl, green, lda, fp32) in enumerate(spec):
+        n = max(ng)
+        h = [1.0 / n] * 3
+        L = [h[0] * m for m in ng]
+        bc = [[4, 4], [4, 4], [9, 9]] if tpl == "unb2d" else TEMPLATES[tpl]
+        if tpl == "unb2d":
+            L[2] = h[2]
+        r = fl.ServeRequest.make(ng, bc, h, L, green=green, lda=lda, fp32=fp32)
+        rhs = blob_field(r.shape(), 100 + k)
+        if tpl in ("per",):
+            rhs -= rhs.mean(axis=(1, 2, 3), keepdims=True)
+        if tpl == "symx-pery":
+            rhs -= rhs.mean(axis=2, keepdims=True)
+        P.append((r, rhs, ng, bc, h, L, green, lda, fp32))
+    return P
+
+
+def solve_alone(r, rhs, ng, bc, h, L, green, lda, fp32):
+    topo = fl.Topology(0, lda, list(ng), [1, 1, 1], False, alignment=8)
+    s = fl.Solver(topo, bc, h, L)
+    s.set_green_type(green)
+    if fp32:
+        s.set_precision(True)
+    s.setup()
+    f = np.zeros_like(rhs)
+    s.solve(f, np.ascontiguousarray(rhs))
+    return f
+
+
+def rel(a, b):
+    return (np.sqrt(np.sum((a - b) ** 2) / max(np.sum(b * b), 1e-300)), np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def test_server_alone_vs_mix_and_oracle():
+    P = problems()
+    alone = [solve_alone(*p) for p in P]
+    for (r, rhs, ng, bc, h, L, green, lda, fp32), fa in zip(P, alone):
+        fo = orc.solve(list(ng), bc, lda, h, L, 1, green, rhs)
+        e2, ei = rel(fa, fo)
+        tol = 1e-5 if fp32 else 1e-12
+        assert e2 <= tol and ei <= tol, (ng, bc, green, lda, fp32, e2, ei)
+    srv = fl.Server(8 << 30, workers=4, solvers_per_key=2)
+    dev_in = [fl.DeviceArray.from_numpy(p[1]) for p in P]
+    dev_out = [fl.DeviceArray(p[1].shape) for p in P]
+    # every problem in flight together
+    tickets = [srv.submit(p[0], dev_in[i], dev_out[i]) for i, p in enumerate(P)]
+    for t in tickets:
+        srv.wait(t)
+    for i, fa in enumerate(alone):
+        assert np.array_equal(dev_out[i].numpy(), fa), ("mix differs from alone", i)
+    # second round on cached solvers, host buffers this time
+    outs = [np.zeros_like(p[1]) for p in P]
+    ts = [srv.submit(p[0], p[1], outs[i]) for i, p in enumerate(P)]
+    for t in ts:
+        srv.wait(t)
+    for i, fa in enumerate(alone):
+        assert np.array_equal(outs[i], fa), ("cached solver differs", i)
+    st = srv.stats()
+    assert st["completed"] == 2 * len(P) and st["failed"] == 0
+    assert st["cache_hits"] >= len(P)
+    assert st["p99_ms"] >= st["p50_ms"] > 0
+    srv.close()
+
+
+def test_server_budget_evicts_and_completes():
+    P = problems()[:6]
+    alone = [solve_alone(*p) for p in P]
+    # room for about two of the larger solvers at a time
+    srv = fl.Server(96 << 20, workers=3)
+    outs = [np.zeros_like(p[1]) for p in P]
+    ts = [srv.submit(p[0], p[1], outs[i]) for i, p in enumerate(P)] + \
+         [srv.submit(p[0], p[1], outs[i]) for i, p in enumerate(P)]
+    for t in ts:
+        srv.wait(t)
+    st = srv.stats()
+    assert st["failed"] == 0 and st["completed"] == len(ts)
+    assert st["evictions"] > 0
+    assert st["peak_device_bytes"] <= 96 << 20
+    for i, fa in enumerate(alone):
+        assert np.array_equal(outs[i], fa), i
+    srv.close()
+
+
+def test_server_rejects_oversized_request():
+    P = problems()
+    srv = fl.Server(1 << 20, workers=1)
+    with pytest.raises(RuntimeError):
+        srv.submit(P[0][0], P[0][1], np.zeros_like(P[0][1]))
+    assert "budget" in fl.last_error()
+    srv.close()
+
+
+def test_batch_graph_replays_match_eager():
+    P = problems()
+    alone = [solve_alone(*p) for p in P]
+    dev_in = [fl.DeviceArray.from_numpy(p[1]) for p in P]
+    dev_out = [fl.DeviceArray(p[1].shape) for p in P]
+    b = fl.Batch([p[0] for p in P], dev_in, dev_out)
+    for rep in range(3):
+        for d in dev_out:   # eager work between replays: clear the outputs through the library
+            zero = fl.DeviceArray.from_numpy(np.zeros(d.shape))
+            assert fl.lib().flups_memcpy(d.ptr, zero.ptr, d.nbytes) == 1
+            del zero
+        ms = b.run()
+        assert ms > 0
+        for i, fa in enumerate(alone):
+            assert np.array_equal(dev_out[i].numpy(), fa), ("graph replay", rep, i)
+    b.run_sequential()
+    for i, fa in enumerate(alone):
+        assert np.array_equal(dev_out[i].numpy(), fa), ("sequential", i)
+    b.close()
