@@ -129,6 +129,16 @@ def lib():
         L.flups_get_allocSize.argtypes = [c_void_p]
         L.flups_get_allocSize.restype = c_size_t
         L.flups_get_spectralInfo.argtypes = [c_void_p, POINTER(c_double), POINTER(c_double), POINTER(c_double)]
+        L.flups_get_innerBuffer.restype = c_void_p
+        L.flups_get_innerBuffer.argtypes = [c_void_p]
+        for name in ["flups_get_innerTopo_physical", "flups_get_innerTopo_spectral"]:
+            getattr(L, name).restype = c_void_p
+            getattr(L, name).argtypes = [c_void_p]
+        L.flups_skip_firstSwitchtopo.argtypes = [c_void_p]
+        L.flups_do_copy.argtypes = [c_void_p, c_void_p, c_void_p, c_int]
+        L.flups_do_FFT.argtypes = [c_void_p, c_void_p, c_int]
+        L.flups_do_mult.argtypes = [c_void_p, c_void_p, c_int]
+        L.flups_switchtopo_info.argtypes = [c_void_p]
         L.flups_set_autotune.argtypes = [c_void_p, c_int]
         L.flups_get_schedule.argtypes = [c_void_p, POINTER(c_int)]
         L.flups_get_path.argtypes = [c_void_p]
@@ -154,6 +164,8 @@ def lib():
         L.flups_server_drain.argtypes = [c_void_p]
         L.flups_server_stats.argtypes = [c_void_p, POINTER(c_double)]
         L.flups_server_reset_stats.argtypes = [c_void_p]
+        L.flups_server_request_bytes.restype = c_size_t
+        L.flups_server_request_bytes.argtypes = [POINTER(ServeRequest)]
         L.flups_batch_new.restype = c_void_p
         L.flups_batch_new.argtypes = [c_int, POINTER(ServeRequest), POINTER(c_void_p), POINTER(c_void_p)]
         L.flups_batch_free.argtypes = [c_void_p]
@@ -188,6 +200,10 @@ class ServeRequest(ctypes.Structure):
         r.center, r.lda, r.green, r.alpha = int(center), int(lda), int(green), float(alpha)
         r.fp32, r.type, r.order_diff = int(bool(fp32)), int(solver_type), int(order_diff)
         return r
+
+    def device_bytes(self):
+        """device memory the serving loop reserves to admit this request when its solver is not cached"""
+        return lib().flups_server_request_bytes(ctypes.byref(self))
 
     def shape(self):
         """caller array shape: (lda, n2, n1, n0), dense"""
@@ -332,6 +348,38 @@ class Topology:
     def _ptr(self):
         return self._topo
 
+    @classmethod
+    def _own(cls, handle):
+        """wrap a topology handle returned by the library (freed with the wrapper)"""
+        _check(handle, "inner topology")
+        t = cls.__new__(cls)
+        t._topo = handle
+        return t
+
+
+FORWARD, BACKWARD, BACKWARD_DIFF = -1, 1, 2   # flups_interface.h signs
+
+
+class DeviceView:
+    """float64 device memory owned by the library (e.g. the solver's inner buffer)"""
+
+    def __init__(self, ptr, n, owner):
+        self.ptr, self.shape, self.nbytes, self._owner = int(ptr), (int(n),), int(n) * 8, owner
+
+    def numpy(self):
+        out = np.empty(self.shape)
+        _check(lib().flups_memcpy(out.ctypes.data, self.ptr, self.nbytes), "flups_memcpy")
+        return out
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, dtype=np.float64).ravel()
+        if a.size > self.shape[0]:
+            raise ValueError("array larger than the device buffer")
+        _check(lib().flups_memcpy(self.ptr, a.ctypes.data, a.size * 8), "flups_memcpy")
+
+    def data_ptr(self):
+        return self.ptr
+
 
 class DeviceArray:
     """float64 array in HBM allocated by libflups (shape in C order, dimension 0 of FLUPS last)."""
@@ -442,6 +490,38 @@ class Solver:
 
     def get_alloc_size(self):
         return lib().flups_get_allocSize(self._solver)
+
+    # ---- advanced interface (python_api/pyflups/flups.py, src/flups.h:467-578) ----
+    def inner_buffer(self):
+        L = lib()
+        L.flups_clear_last_error()
+        p = L.flups_get_innerBuffer(self._solver)
+        _check(p, "flups_get_innerBuffer")
+        return DeviceView(p, self.get_alloc_size(), self)
+
+    def inner_topo_physical(self):
+        return Topology._own(lib().flups_get_innerTopo_physical(self._solver))
+
+    def inner_topo_spectral(self):
+        return Topology._own(lib().flups_get_innerTopo_spectral(self._solver))
+
+    def skip_first_switchtopo(self):
+        lib().flups_skip_firstSwitchtopo(self._solver)
+
+    def _adv(self, fn, *args):
+        L = lib()
+        L.flups_clear_last_error()
+        fn(self._solver, *args)
+        _check(not L.flups_get_last_error(), fn.__name__)
+
+    def do_copy(self, topology, data, sign):
+        self._adv(lib().flups_do_copy, topology._ptr(), _data_ptr(data), int(sign))
+
+    def do_fft(self, data, sign):
+        self._adv(lib().flups_do_FFT, _data_ptr(data), int(sign))
+
+    def do_mult(self, data, solver_type=SolverType.STD):
+        self._adv(lib().flups_do_mult, _data_ptr(data), int(solver_type))
 
     def get_spectral_info(self):
         kf, ko, ss = (c_double * 3)(), (c_double * 3)(), (c_double * 3)()

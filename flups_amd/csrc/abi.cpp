@@ -206,6 +206,61 @@ void flups_get_spectralInfo(FLUPS_Solver* s, double kfact[3], double koffset[3],
     if (s) s->impl->spectral_info(kfact, koffset, symstart);
 }
 int flups_get_path(FLUPS_Solver* s) { return s ? s->impl->path() : -1; }
+// ---- advanced interface (src/flups.h:467-578) ----
+double* flups_get_innerBuffer(FLUPS_Solver* s) {
+    return guard("flups_get_innerBuffer", [&]() -> double* {
+        FLUPS_REQUIRE(s, "NULL solver");
+        return s->impl->inner_buffer();
+    });
+}
+FLUPS_Topology* flups_get_innerTopo_physical(FLUPS_Solver* s) {
+    return guard("flups_get_innerTopo_physical", [&]() -> FLUPS_Topology* {
+        FLUPS_REQUIRE(s, "NULL solver");
+        auto* T = new Topology;
+        T->t = s->impl->inner_topo_physical();
+        return T;
+    });
+}
+FLUPS_Topology* flups_get_innerTopo_spectral(FLUPS_Solver* s) {
+    return guard("flups_get_innerTopo_spectral", [&]() -> FLUPS_Topology* {
+        FLUPS_REQUIRE(s, "NULL solver");
+        auto* T = new Topology;
+        T->t = s->impl->inner_topo_spectral();
+        return T;
+    });
+}
+void flups_skip_firstSwitchtopo(FLUPS_Solver* s) {
+    (void)s;   // the physical layout is always the user topology: there is no first switch to skip
+}
+void flups_do_copy(FLUPS_Solver* s, const FLUPS_Topology* topo, double* data, const int sign) {
+    guard("flups_do_copy", [&]() {
+        FLUPS_REQUIRE(s && topo, "NULL solver or topology");
+        s->impl->do_copy(&topo->t, data, sign);
+        return 0;
+    });
+}
+void flups_do_FFT(FLUPS_Solver* s, double* data, const int sign) {
+    guard("flups_do_FFT", [&]() {
+        FLUPS_REQUIRE(s, "NULL solver");
+        s->impl->do_fft(data, sign);
+        return 0;
+    });
+}
+void flups_do_mult(FLUPS_Solver* s, double* data, const FLUPS_SolverType type) {
+    guard("flups_do_mult", [&]() {
+        FLUPS_REQUIRE(s, "NULL solver");
+        s->impl->do_mult(data, (int)type);
+        return 0;
+    });
+}
+void flups_switchtopo_info(FLUPS_Solver* s) {
+    if (!s) return;
+    int sched[3];
+    s->impl->schedule(sched);
+    std::fprintf(stderr, "[flups] one GPU: pencil switches are fused into the transform passes (path %d, schedule %d %d %d)\n",
+                 s->impl->path(), sched[0], sched[1], sched[2]);
+}
+
 void flups_set_autotune(FLUPS_Solver* s, int on) {
     guard("flups_set_autotune", [&]() {
         FLUPS_REQUIRE(s, "NULL solver");

@@ -25,6 +25,8 @@ enum Green {
 };
 enum Diff { NOD = 0, SPE = 1, FD2 = 2, FD4 = 4, FD6 = 6 };
 constexpr int FWD = -1, BWD = 1;
+// signs of the advanced interface (FLUPS_FORWARD, FLUPS_BACKWARD, FLUPS_BACKWARD_DIFF)
+constexpr int SIGN_FORWARD = -1, SIGN_BACKWARD = 1, SIGN_BACKWARD_DIFF = 2;
 
 // failed checks unwind to the C entry point, which records the message
 struct Error {

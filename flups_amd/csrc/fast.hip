@@ -526,9 +526,10 @@ FastPlan* make_fast_plan(SolverImpl& s) {
     if (const char* e = getenv("FLUPS_ZSCHED")) f->zsched = atoi(e);
     if (const char* e = getenv("FLUPS_ORDER")) f->order = atoi(e) & 15;
     if (const char* e = getenv("FLUPS_YSCHED")) f->ysched = atoi(e) & 20;
-    if (f->zsched < 0 || f->zsched > 31 || (f->zsched & 3) == 3 || (f->zsched & 8)) {
+    if (f->zsched < 0 || f->zsched > 31 || (f->zsched & 3) == 3 || ((f->zsched & 8) && (f->zsched & 7))) {
         delete f;
-        fail("FLUPS_ZSCHED: bits 0-1 schedule (0, 1, 2), bit 2 half-width tiles, bit 4 XCD-grouped blocks");
+        fail("FLUPS_ZSCHED: bits 0-1 schedule (0, 1, 2), bit 2 half-width tiles, bit 3 quarter-width tiles (schedule 0), "
+             "bit 4 XCD-grouped blocks");
     }
     try {
         build_fast_plan(s, f, es, align);
@@ -543,7 +544,8 @@ FastPlan* make_fast_plan(SolverImpl& s) {
 // M2 = 1024 (LDS) or for half-width schedules
 static int z_tile(const FastPlan* f) {
     const int align = f->fp32 ? 16 : 8;
-    return (f->M[2] == 1024 || (f->zsched & 4)) ? align / 2 : align;
+    if (f->M[2] == 1024) return align / 2;
+    return (f->zsched & 8) ? align / 4 : (f->zsched & 4) ? align / 2 : align;
 }
 
 // (re)write the compressed Green's function in the layout of the current z schedule
@@ -632,15 +634,16 @@ template <typename R, int M, int TX, bool FWD, bool FUSED>
 void launch_col(const ColArgs& a, int comps, int zsched, hipStream_t st) {
     if (FUSED) {
         // z schedules: bit 0-1 = SCHED of z_fused, 4 = half-width tiles (TX/2 columns)
-        const int tx = (zsched & 4) ? TX / 2 : TX;
+        const int tx = (zsched & 8) ? TX / 4 : (zsched & 4) ? TX / 2 : TX;
         dim3 grid(((a.nx + tx - 1) / tx) * (a.gmy / 2 + 1), comps);
-        switch (zsched & 7) {
+        switch (zsched & 15) {
             case 0: hipLaunchKernelGGL((z_fused<R, M, TX, 0>), grid, dim3(TX * M / 8), 0, st, a); break;
             case 1: hipLaunchKernelGGL((z_fused<R, M, TX, 1>), grid, dim3(TX * M / 8), 0, st, a); break;
             case 2: hipLaunchKernelGGL((z_fused<R, M, TX, 2>), grid, dim3(TX * M / 8), 0, st, a); break;
             case 4: hipLaunchKernelGGL((z_fused<R, M, TX / 2, 0>), grid, dim3(TX / 2 * M / 8), 0, st, a); break;
             case 5: hipLaunchKernelGGL((z_fused<R, M, TX / 2, 1>), grid, dim3(TX / 2 * M / 8), 0, st, a); break;
             case 6: hipLaunchKernelGGL((z_fused<R, M, TX / 2, 2>), grid, dim3(TX / 2 * M / 8), 0, st, a); break;
+            case 8: hipLaunchKernelGGL((z_fused<R, M, (TX >= 8 ? TX / 4 : 1), 0>), grid, dim3((TX >= 8 ? TX / 4 : 1) * M / 8), 0, st, a); break;
             default: fail("unknown z schedule %d", zsched);
         }
     } else if (zsched & 4) {   // y schedule: half-width tiles
@@ -660,7 +663,7 @@ void col_dispatch(int M, const ColArgs& a, int comps, int zsched, hipStream_t st
         case 128: launch_col<R, 128, TX, FWD, FUSED>(a, comps, zsched, st); break;
         case 256: launch_col<R, 256, TX, FWD, FUSED>(a, comps, zsched, st); break;
         case 512: launch_col<R, 512, TX, FWD, FUSED>(a, comps, zsched, st); break;
-        case 1024: launch_col<R, 1024, TX / 2, FWD, FUSED>(a, comps, zsched & ~4, st); break;   // already half width
+        case 1024: launch_col<R, 1024, TX / 2, FWD, FUSED>(a, comps, zsched & ~12, st); break;   // already half width
         default: fail("unsupported column length %d", M);
     }
 }

@@ -524,6 +524,13 @@ void flups_server_stats(FLUPS_Server* s, double out[12]) {
         return 0;
     });
 }
+size_t flups_server_request_bytes(const FLUPS_ServeRequest* req) {
+    return sguard("flups_server_request_bytes", [&]() -> size_t {
+        FLUPS_REQUIRE(req, "NULL request");
+        flups::validate(*req);
+        return flups::make_solver(*req)->predicted_device_bytes(req->type);
+    });
+}
 void flups_server_reset_stats(FLUPS_Server* s) {
     if (s) reinterpret_cast<FLUPS_ServerS*>(s)->s.reset_stats();
 }

@@ -180,6 +180,9 @@ SolverImpl::~SolverImpl() {
     for (double* p : dstage_) if (p) (void)hipFree(p);
     if (dgreen_) (void)hipFree(dgreen_);
     if (dstaging_) (void)hipFree(dstaging_);
+    if (dinner_) (void)hipFree(dinner_);
+    if (gexec_) (void)hipGraphExecDestroy(gexec_);
+    if (cap_) (void)hipStreamDestroy(cap_);
     if (ev0_) (void)hipEventDestroy(ev0_);
     if (ev1_) (void)hipEventDestroy(ev1_);
 }
@@ -452,6 +455,7 @@ void SolverImpl::compute_green() {
 
 void SolverImpl::setup() {
     FLUPS_REQUIRE(!setup_done_, "setup was already done");
+    if (const char* e = std::getenv("FLUPS_GRAPH")) use_graph_ = std::atoi(e) != 0;
     HIP_CHECK(hipEventCreate(&ev0_));
     HIP_CHECK(hipEventCreate(&ev1_));
     compute_green();
@@ -488,44 +492,136 @@ void SolverImpl::setup() {
 }
 
 // ---------------------------------------------------------------------------
-void SolverImpl::solve_generic(double* field, const double* rhs, int type) {
-    const View uin = user_view(const_cast<double*>(rhs));
-    const View uout = user_view(field);
+// forward transforms: user layout (uin) -> dense spectral array `spec`
+void SolverImpl::forward_passes(const View& uin, double* spec) {
     for (int ip = 0; ip < ndim_; ++ip) {
         const DirPlan& p = fwd_[ip];
         const View src = (ip == 0) ? uin : dense_view(dstage_[ip - 1], stage_[ip - 1]);
-        const View dst = dense_view(dstage_[ip], stage_[ip]);
+        const View dst = dense_view(ip == ndim_ - 1 ? spec : dstage_[ip], stage_[ip]);
         for (int c = 0; c < lda_; ++c) run_pass(pass_desc(p, c, true, src, dst, hat_[ip], stage_[ip], p.fieldstart, 0));
     }
+}
+
+// multiply the dense spectral array by normfact * G (STD) or take the curl of G * data (ROT)
+void SolverImpl::multiply(double* spec, int type) {
     const Shape& sp = stage_[ndim_ - 1];
     if (type == 0) {
-        launch_mult_std(dstage_[ndim_ - 1], dgreen_, (size_t)sp.n[0] * sp.n[1] * sp.n[2], sp.nf, lda_, normfact_, stream_);
-    } else {
-        RotArgs R;
-        for (int d = 0; d < 3; ++d) { R.n[d] = sp.n[d]; R.h[d] = h_[d]; }
-        R.nf = sp.nf;
-        R.order = odiff_;
-        R.normfact = normfact_;
-        double kabs[3];
-        spectral_info(kabs, R.koffset, R.symstart);
-        for (int ip = 0; ip < 3; ++ip) {
-            const int d = fwd_[ip].dim;
-            for (int c = 0; c < 3; ++c) {
-                const int ph = (bwdd_[ip].imult[c] ? 1 : 0) - (fwd_[ip].imult[c] ? 1 : 0);
-                R.kf[d][c][0] = ph == 0 ? 0.0 : (ph == 1 ? -kabs[d] : kabs[d]);
-                R.kf[d][c][1] = ph == 0 ? kabs[d] : 0.0;
-            }
-        }
-        launch_mult_rot(dstage_[ndim_ - 1], dgreen_, R, stream_);
+        launch_mult_std(spec, dgreen_, (size_t)sp.n[0] * sp.n[1] * sp.n[2], sp.nf, lda_, normfact_, stream_);
+        return;
     }
-    const DirPlan* back = (type == 0) ? bwd_ : bwdd_;
+    RotArgs R;
+    for (int d = 0; d < 3; ++d) { R.n[d] = sp.n[d]; R.h[d] = h_[d]; }
+    R.nf = sp.nf;
+    R.order = odiff_;
+    R.normfact = normfact_;
+    double kabs[3];
+    spectral_info(kabs, R.koffset, R.symstart);
+    for (int ip = 0; ip < 3; ++ip) {
+        const int d = fwd_[ip].dim;
+        for (int c = 0; c < 3; ++c) {
+            const int ph = (bwdd_[ip].imult[c] ? 1 : 0) - (fwd_[ip].imult[c] ? 1 : 0);
+            R.kf[d][c][0] = ph == 0 ? 0.0 : (ph == 1 ? -kabs[d] : kabs[d]);
+            R.kf[d][c][1] = ph == 0 ? kabs[d] : 0.0;
+        }
+    }
+    launch_mult_rot(spec, dgreen_, R, stream_);
+}
+
+// backward transforms: dense spectral array `spec` -> user layout (uout); `diff`
+// selects the plans of the differentiated field (after a ROT multiply)
+void SolverImpl::backward_passes(const double* spec, const View& uout, bool diff) {
+    const DirPlan* back = diff ? bwdd_ : bwd_;
     for (int ip = ndim_ - 1; ip >= 0; --ip) {
         const DirPlan& p = back[ip];
-        const View src = dense_view(dstage_[ip], stage_[ip]);
+        const View src = dense_view(ip == ndim_ - 1 ? const_cast<double*>(spec) : dstage_[ip], stage_[ip]);
         const View dst = (ip == 0) ? uout : dense_view(dstage_[ip - 1], stage_[ip - 1]);
         for (int c = 0; c < lda_; ++c)
             run_pass(pass_desc(p, c, false, src, dst, stage_[ip], hat_[ip], 0, fwd_[ip].fieldstart));
     }
+}
+
+void SolverImpl::solve_generic(double* field, const double* rhs, int type) {
+    forward_passes(user_view(const_cast<double*>(rhs)), dstage_[ndim_ - 1]);
+    multiply(dstage_[ndim_ - 1], type);
+    backward_passes(dstage_[ndim_ - 1], user_view(field), type != 0);
+}
+
+// ---------------------------------------------------------------------------
+// advanced interface (src/Solver.cpp:1281-1542): the solve's steps one by one on
+// device buffers of flups_get_allocSize doubles. The physical layout is the
+// user topology; the spectral layout is the dense array of inner_topo_spectral.
+void SolverImpl::prepare_advanced() {
+    FLUPS_REQUIRE(setup_done_, "flups_setup must be called first");
+    if (!dstage_[0]) {
+        for (int ip = 0; ip < ndim_; ++ip) {
+            const size_t m = std::max(hat_[ip].count(), stage_[ip].count());
+            HIP_CHECK(hipMalloc(&dstage_[ip], m * sizeof(double)));
+        }
+    }
+    if (!dgreen_) compute_green();   // the fused pipeline released the full Green's function
+}
+
+static bool device_ptr(const void* p) {
+    hipPointerAttribute_t a;
+    if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return a.type == hipMemoryTypeDevice;
+}
+
+double* SolverImpl::inner_buffer() {
+    FLUPS_REQUIRE(setup_done_, "flups_setup must be called first");
+    if (!dinner_) {
+        HIP_CHECK(hipMalloc(&dinner_, alloc_size() * sizeof(double)));
+        HIP_CHECK(hipMemset(dinner_, 0, alloc_size() * sizeof(double)));
+    }
+    return dinner_;
+}
+
+void SolverImpl::do_copy(const Topo* t, double* data, int sign) {
+    FLUPS_REQUIRE(data, "data is NULL");
+    FLUPS_REQUIRE(t && t->lda == lda_, "the topology's lda must match the solver's (%d)", lda_);
+    FLUPS_REQUIRE(t->memsize() == phys_.memsize() && t->axis == phys_.axis,
+                  "do_copy works on the solver's physical topology");
+    FLUPS_REQUIRE(sign == SIGN_FORWARD || sign == SIGN_BACKWARD, "sign must be SIGN_FORWARD or SIGN_BACKWARD");
+    double* own = inner_buffer();
+    const size_t n = phys_.memsize() * sizeof(double);
+    if (sign == SIGN_FORWARD) HIP_CHECK(hipMemcpyAsync(own, data, n, hipMemcpyDefault, stream_));
+    else HIP_CHECK(hipMemcpyAsync(data, own, n, hipMemcpyDefault, stream_));
+    HIP_CHECK(hipStreamSynchronize(stream_));
+}
+
+void SolverImpl::do_fft(double* data, int sign) {
+    FLUPS_REQUIRE(data, "data is NULL");
+    FLUPS_REQUIRE(device_ptr(data), "do_FFT works on device buffers (flups_get_innerBuffer)");
+    FLUPS_REQUIRE(sign == SIGN_FORWARD || sign == SIGN_BACKWARD || sign == SIGN_BACKWARD_DIFF, "unknown sign %d", sign);
+    prepare_advanced();
+    // in place: the first pass reads `data` entirely before the last one writes it (ndim >= 2)
+    if (sign == SIGN_FORWARD) forward_passes(user_view(data), data);
+    else backward_passes(data, user_view(data), sign == SIGN_BACKWARD_DIFF);
+    HIP_CHECK(hipStreamSynchronize(stream_));
+}
+
+void SolverImpl::do_mult(double* data, int type) {
+    FLUPS_REQUIRE(data, "data is NULL");
+    FLUPS_REQUIRE(device_ptr(data), "do_mult works on device buffers (flups_get_innerBuffer)");
+    check_type(type);
+    prepare_advanced();
+    multiply(data, type);
+    HIP_CHECK(hipStreamSynchronize(stream_));
+}
+
+Topo SolverImpl::inner_topo_spectral() const {
+    const Shape& sp = stage_[ndim_ - 1];
+    Topo t;
+    t.axis = 0;
+    t.lda = lda_;
+    t.nf = sp.nf;
+    t.alignment = sizeof(double);
+    for (int d = 0; d < 3; ++d) t.nglob[d] = sp.n[d];
+    t.cmpt_sizes();
+    return t;
 }
 
 void SolverImpl::check_type(int type) const {
@@ -546,6 +642,22 @@ void SolverImpl::prepare(int type) {
             HIP_CHECK(hipMalloc(&dstage_[ip], m * sizeof(double)));
         }
     }
+    if (!warmed_) {
+        warm_tables();
+        warmed_ = true;
+    }
+}
+
+// build every pass descriptor once so that the twiddle tables they use exist
+// before a solve is queued (a solve under graph capture may not allocate)
+void SolverImpl::warm_tables() {
+    const View u = user_view(dstage_[0]);
+    for (int ip = 0; ip < ndim_; ++ip)
+        for (int c = 0; c < lda_; ++c) {
+            (void)pass_desc(fwd_[ip], c, true, u, u, hat_[ip], stage_[ip], fwd_[ip].fieldstart, 0);
+            (void)pass_desc(bwd_[ip], c, false, u, u, stage_[ip], hat_[ip], 0, fwd_[ip].fieldstart);
+            if (odiff_ != NOD) (void)pass_desc(bwdd_[ip], c, false, u, u, stage_[ip], hat_[ip], 0, fwd_[ip].fieldstart);
+        }
 }
 
 // queue the solve's kernels on the solver's stream; field and rhs are device
@@ -609,7 +721,8 @@ void SolverImpl::solve(double* field, const double* rhs, int type) {
         if (!fdev) dout = dstaging_ + n;
     }
     HIP_CHECK(hipEventRecord(ev0_, stream_));
-    enqueue(dout, din, type);
+    if (use_graph_ && !profiling_) launch_graph(dout, din, type);
+    else enqueue(dout, din, type);
     HIP_CHECK(hipEventRecord(ev1_, stream_));
     if (!fdev) HIP_CHECK(hipMemcpyAsync(field, dout, n * sizeof(double), hipMemcpyDeviceToHost, stream_));
     HIP_CHECK(hipStreamSynchronize(stream_));
@@ -619,7 +732,43 @@ void SolverImpl::solve(double* field, const double* rhs, int type) {
     if (fast_ && type == 0) fast_collect(fast_);
 }
 
+// The solve's kernels are recorded once per (field, rhs, type) into a hipGraph and
+// replayed with one launch: the same kernels with the same arguments, fewer
+// host launches. The capture runs on a private stream (the default stream
+// cannot be captured); the replay goes on the solver's stream.
+void SolverImpl::launch_graph(double* field, const double* rhs, int type) {
+    if (!(gexec_ && gfield_ == field && grhs_ == rhs && gtype_key_ == type)) {
+        if (gexec_) {
+            HIP_CHECK(hipGraphExecDestroy(gexec_));
+            gexec_ = nullptr;
+        }
+        if (!cap_) HIP_CHECK(hipStreamCreateWithFlags(&cap_, hipStreamNonBlocking));
+        hipGraph_t g = nullptr;
+        HIP_CHECK(hipStreamBeginCapture(cap_, hipStreamCaptureModeThreadLocal));
+        hipStream_t keep = stream_;
+        stream_ = cap_;
+        try {
+            enqueue(field, rhs, type);
+        } catch (...) {
+            stream_ = keep;
+            (void)hipStreamEndCapture(cap_, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        stream_ = keep;
+        HIP_CHECK(hipStreamEndCapture(cap_, &g));
+        const hipError_t e = hipGraphInstantiate(&gexec_, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        HIP_CHECK(e);
+        gfield_ = field;
+        grhs_ = rhs;
+        gtype_key_ = type;
+    }
+    HIP_CHECK(hipGraphLaunch(gexec_, stream_));
+}
+
 void SolverImpl::profile(bool on) {
+    profiling_ = on;
     if (fast_) fast_profile(fast_, on);
 }
 void SolverImpl::schedule(int out[3]) const {

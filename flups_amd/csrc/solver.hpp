@@ -49,6 +49,13 @@ class SolverImpl {
         autotune_ = on;
     }
     void schedule(int out[3]) const;   // fused pipeline schedule (z, y, order), -1 on the general path
+    // advanced interface: the solve's steps on device buffers
+    double* inner_buffer();
+    void do_copy(const Topo* t, double* data, int sign);
+    void do_fft(double* data, int sign);
+    void do_mult(double* data, int type);
+    const Topo& inner_topo_physical() const { return phys_; }
+    Topo inner_topo_spectral() const;
     size_t alloc_size() const;
     void spectral_info(double kf[3], double ko[3], double ss[3]) const;
     int path() const { return fast_ ? (fp32_ ? 2 : 1) : 0; }   // 0 general FP64, 1 fused FP64, 2 fused FP32
@@ -72,6 +79,12 @@ class SolverImpl {
                          const double symstart[3], double length, int nunb);
     View user_view(double* p) const;
     void solve_generic(double* field, const double* rhs, int type);
+    void forward_passes(const View& uin, double* spec);
+    void multiply(double* spec, int type);
+    void backward_passes(const double* spec, const View& uout, bool diff);
+    void prepare_advanced();
+    void warm_tables();
+    void launch_graph(double* field, const double* rhs, int type);
     void check_type(int type) const;
 
     Topo phys_;
@@ -85,6 +98,13 @@ class SolverImpl {
     double* dstage_[3] = {nullptr, nullptr, nullptr};
     double* dgreen_ = nullptr;
     double* dstaging_ = nullptr;   // rhs/field copies for host pointers
+    double* dinner_ = nullptr;     // inner buffer of the advanced interface
+    bool use_graph_ = true, profiling_ = false, warmed_ = false;
+    hipStream_t cap_ = nullptr;    // capture stream of the solve graph
+    hipGraphExec_t gexec_ = nullptr;
+    double* gfield_ = nullptr;
+    const double* grhs_ = nullptr;
+    int gtype_key_ = -1;
     size_t staging_n_ = 0;
     size_t lds_budget_ = 65536;
     FastPlan* fast_ = nullptr;

@@ -130,6 +130,21 @@ void   flups_get_spectralInfo(FLUPS_Solver* s, double kfact[3], double koffset[3
 void   flups_set_alpha(FLUPS_Solver* s, const double alpha);                                      /* src/flups.h:494 */
 void   flups_pencilDirs(const FLUPS_BoundaryType* bc[3][2], int dirs[3]);                          /* src/flups.h:570 */
 
+/* ---- advanced interface (src/flups.h:467-578): the steps of flups_solve one by one.
+   Buffers are device memory of flups_get_allocSize doubles (flups_get_innerBuffer returns
+   one); flups_do_copy accepts host or device user arrays. The physical layout is the user
+   topology; the spectral layout is described by flups_get_innerTopo_spectral (complex
+   interleaved, dimension 0 fastest, dense); mode indices follow flups_get_spectralInfo and
+   flups_symID as in the reference. ---- */
+double*         flups_get_innerBuffer(FLUPS_Solver* s);                                          /* src/flups.h:509 */
+FLUPS_Topology* flups_get_innerTopo_physical(FLUPS_Solver* s);                                   /* src/flups.h:517 */
+FLUPS_Topology* flups_get_innerTopo_spectral(FLUPS_Solver* s);                                   /* src/flups.h:525 */
+void            flups_skip_firstSwitchtopo(FLUPS_Solver* s);                                     /* src/flups.h:534 */
+void            flups_do_copy(FLUPS_Solver* s, const FLUPS_Topology* topo, double* data, const int sign); /* src/flups.h:544 */
+void            flups_do_FFT(FLUPS_Solver* s, double* data, const int sign);                    /* src/flups.h:552 */
+void            flups_do_mult(FLUPS_Solver* s, double* data, const FLUPS_SolverType type);      /* src/flups.h:560 */
+void            flups_switchtopo_info(FLUPS_Solver* s);                                          /* src/flups.h:578 */
+
 /* ---- profiler handles (src/flups.h:592-611): accepted, reports solve timings ---- */
 FLUPS_Profiler* flups_profiler_new(void);
 FLUPS_Profiler* flups_profiler_new_n(const char name[]);
@@ -199,6 +214,8 @@ void          flups_server_drain(FLUPS_Server* s);
    cache hits, cache misses, evictions, peak device bytes, device bytes now, cached solvers */
 void          flups_server_stats(FLUPS_Server* s, double out[12]);
 void          flups_server_reset_stats(FLUPS_Server* s);
+/* device bytes the server reserves to admit a request whose solver is not cached */
+size_t        flups_server_request_bytes(const FLUPS_ServeRequest* req);
 
 /* batch: n independent problems (device buffers), set up once, solved by one hipGraph
    launch (one branch per problem) */
