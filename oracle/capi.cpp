@@ -137,5 +137,6 @@ double oracle_lgf_symbol(int order, double kh) { return lgf_symbol(order, kh); }
 double oracle_bessel_k0(double x) { return bessel_k0(x); }
 double oracle_bessel_k1(double x) { return bessel_k1(x); }
 double oracle_expint_e1(double x) { return expint_e1(x); }
+double oracle_sine_integral(double x) { return sine_integral(x); }
 
 }  // extern "C"

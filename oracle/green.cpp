@@ -1,12 +1,16 @@
 // Oracle Green's functions.
 //   drivers: src/green_functions.cpp:27-167 (3 unbounded), 192-391 (2), 407-538 (1), 560-708 (0)
-//   kernels: src/green_kernels.hpp (HEJ_2..10, CHAT_2, LGF_2..8)
+//   kernels: src/green_kernels.hpp (HEJ_0..10, CHAT_2, LGF_2..8, MEHR_4L/6L/4F/6F)
 //   tables:  src/green_functions.hpp:30-92 (.ker files, 32^3 / 32^2 raw float64)
-// Special functions come from libstdc++ (std::cyl_bessel_k, std::expint), an
-// implementation independent of both the reference and the product.
-// HEJ_0 and the MEHR kernels are not restated here (parity unpinned for them).
+// Special functions come from libstdc++ (std::cyl_bessel_k, std::expint) and,
+// for the sine integral, a series / complex continued fraction written here:
+// implementations independent of both the reference and the product.
+// Not restated (parity unpinned, the reference's own tests hold no case):
+// HEJ_0 with two unbounded directions (its Ji0c kernel) and the Mehrstellen
+// kernels with one unbounded direction.
 // Test infrastructure only.
 #include <cmath>
+#include <complex>
 #include <cstdio>
 #include <map>
 #include <mutex>
@@ -16,8 +20,39 @@
 
 namespace oracle {
 
-double lgf3_expansion(const std::string& kdir, int order, double n1, double n2, double n3);
-double lgf2_expansion(const std::string& kdir, int order, double n1, double n2);
+double kernel_expansion(const std::string& kdir, const std::string& name, double n1, double n2, double n3);
+
+// sine integral Si(x), x >= 0: power series below 4, above it Si = pi/2 + Im E1(ix)
+// with E1 from its continued fraction (Abramowitz & Stegun 5.1.22), modified Lentz
+double sine_integral(double x) {
+    if (x < 0) return -sine_integral(-x);
+    if (x <= 4.0) {
+        long double t = x, s = x, x2 = (long double)x * x;
+        for (int k = 1; k < 60; ++k) {
+            t *= -x2 / ((2.0L * k) * (2.0L * k + 1));
+            const long double term = t / (2 * k + 1);
+            s += term;
+            if (fabsl(term) < 1e-22L * fabsl(s)) break;
+        }
+        return (double)s;
+    }
+    using C = std::complex<long double>;
+    const C z(0.0L, (long double)x);
+    // E1(z) = exp(-z) / (z + 1 - 1/(z + 3 - 4/(z + 5 - 9/(z + 7 - ...))))
+    const long double tiny = 1e-300L;
+    C b = z + 1.0L, c = 1.0L / tiny, d = 1.0L / b, h = d;
+    for (int i = 1; i < 1000; ++i) {
+        const long double an = -(long double)i * i;
+        b += 2.0L;
+        d = 1.0L / (an * d + b);
+        c = b + an / c;
+        const C del = c * d;
+        h *= del;
+        if (std::abs(del - 1.0L) < 1e-21L) break;
+    }
+    const C e1 = std::exp(-z) * h;
+    return (double)(1.57079632679489661923132169163975144L + e1.imag());
+}
 
 namespace {
 
@@ -36,19 +71,42 @@ int lgf_order(GreenType t) {
     }
 }
 
+bool is_mehr(GreenType t) { return t == MEHR_4L || t == MEHR_6L || t == MEHR_4F || t == MEHR_6F; }
+
+// kernels given near the origin by a 32^3 / 32^2 table and beyond it by an expansion
+// (green_functions.hpp:30-92 for the file names, lgf_3unb0spe.hpp for the expansions)
+bool table_kernel(GreenType t, int dim, std::string* file, std::string* exp) {
+    const int ord = lgf_order(t);
+    char f[64], e[64];
+    if (ord) {
+        std::snprintf(f, sizeof(f), "LGF_%d_%dd_32.ker", ord, dim);
+        std::snprintf(e, sizeof(e), "lgf_%d_%dunb0spe", ord, dim);
+    } else if (is_mehr(t)) {
+        const char* nm = t == MEHR_4L ? "4L" : t == MEHR_6L ? "6L" : t == MEHR_4F ? "4F" : "6F";
+        const char* lo = t == MEHR_4L ? "4l" : t == MEHR_6L ? "6l" : t == MEHR_4F ? "4f" : "6f";
+        const bool lump = (t == MEHR_4L || t == MEHR_6L) && dim == 2;   // 4L and 6L share the 2-D table
+        std::snprintf(f, sizeof(f), "MEHR_%s_%dd_32.ker", lump ? "4L6L" : nm, dim);
+        std::snprintf(e, sizeof(e), "mehr_%s_%dunb0spe", lump ? "4l6l" : lo, dim);
+    } else {
+        return false;
+    }
+    *file = f;
+    *exp = e;
+    return true;
+}
+
 std::mutex g_tab_mtx;
 std::map<std::string, std::vector<double>> g_tabs;
 
-const std::vector<double>& lgf_table(const std::string& kdir, int order, int dim) {
-    char name[512];
-    std::snprintf(name, sizeof(name), "%s/LGF_%d_%dd_32.ker", kdir.c_str(), order, dim);
+const std::vector<double>& kernel_table(const std::string& kdir, const std::string& file, int dim) {
+    const std::string name = kdir + "/" + file;
     std::lock_guard<std::mutex> lk(g_tab_mtx);
     auto it = g_tabs.find(name);
     if (it != g_tabs.end()) return it->second;
     const size_t n = (dim == 3) ? 32 * 32 * 32 : 32 * 32;
     std::vector<double> v(n);
-    FILE* f = std::fopen(name, "rb");
-    if (!f) throw std::runtime_error(std::string("unable to read ") + name);
+    FILE* f = std::fopen(name.c_str(), "rb");
+    if (!f) throw std::runtime_error("unable to read " + name);
     if (std::fread(v.data(), sizeof(double), n, f) != n) { std::fclose(f); throw std::runtime_error("short .ker file"); }
     std::fclose(f);
     return g_tabs[name] = v;
@@ -116,6 +174,23 @@ double hej_spectral(GreenType t, double ksq, double sig) {
     return -p * std::exp(-h) / ksq;
 }
 
+// Mehrstellen symbols in y = sin^2(k h / 2) (green_kernels.hpp:744-773)
+double mehr_spectral(GreenType t, const double k[3], double h) {
+    double y[3];
+    for (int d = 0; d < 3; ++d) {
+        const double sv = std::sin(0.5 * k[d] * h);
+        y[d] = sv * sv;
+    }
+    const double s1 = y[0] + y[1] + y[2], s2 = y[0] * y[1] + y[1] * y[2] + y[0] * y[2], s3 = y[0] * y[1] * y[2];
+    const double q = y[0] * y[0] + y[1] * y[1] + y[2] * y[2];
+    const bool six = (t == MEHR_6L || t == MEHR_6F);
+    const double left = -4.0 * s1 + 8.0 / 3.0 * s2 - (six ? 32.0 / 15.0 * s3 : 0.0);
+    double right = 1.0;
+    if (t == MEHR_4F) right = 1.0 - s1 / 3.0;
+    if (t == MEHR_6F) right = 1.0 - s1 / 3.0 + 8.0 / 45.0 * s2 - q / 15.0;
+    return h * h * right / left;
+}
+
 double lgf_spectral(int order, const double k[3], double h) {
     double s = 0;
     for (int d = 0; d < 3; ++d) s += lgf_symbol(order, k[d] * h);
@@ -128,12 +203,13 @@ void green_fill_3unb(const Topo& t, const double hfact[3], const double symstart
                      double length, const std::string& kdir) {
     for (int d = 0; d < 3; ++d)
         if (hfact[d] == 0.0) throw std::runtime_error("hfact cannot be 0");
-    const int ord = lgf_order(type);
+    std::string file, expn;
+    const bool ord = table_kernel(type, 3, &file, &expn);
     const std::vector<double>* tab = nullptr;
     if (ord) {
         if (!(hfact[0] == hfact[1] && hfact[1] == hfact[2])) throw std::runtime_error("LGF needs an isotropic grid");
-        tab = &lgf_table(kdir, ord, 3);
-    } else if (!(type == CHAT_2 || (type >= HEJ_2 && type <= HEJ_10))) {
+        tab = &kernel_table(kdir, file, 3);
+    } else if (!(type == CHAT_2 || (type >= HEJ_2 && type <= HEJ_0))) {
         throw std::runtime_error("kernel not restated in the oracle");
     }
     for (int i2 = 0; i2 < t.n[2]; ++i2)
@@ -148,10 +224,13 @@ void green_fill_3unb(const Topo& t, const double hfact[3], const double symstart
                     const int a = std::abs(is[ax0]), b = std::abs(is[ax1]), c = std::abs(is[ax2]);
                     double gv;
                     if (a < 32 && b < 32 && c < 32) gv = (*tab)[a + 32 * b + 32 * 32 * c];
-                    else gv = lgf3_expansion(kdir, ord, a, b, c);
+                    else gv = kernel_expansion(kdir, expn, a, b, c);
                     v = -gv / hfact[0];
                 } else if (type == CHAT_2) {
                     v = -1.0 / (4.0 * PI * r);
+                } else if (type == HEJ_0) {   // green_kernels.hpp:55-61, length = h / pi
+                    const double rho = r / length;
+                    v = -sine_integral(rho) / (rho * 2.0 * PI * PI * length);
                 } else {
                     const double rho = r / length;
                     v = -1.0 / (4.0 * PI * r) *
@@ -164,6 +243,8 @@ void green_fill_3unb(const Topo& t, const double hfact[3], const double symstart
         if (type == CHAT_2) {
             const double vol = hfact[0] * hfact[1] * hfact[2];
             g0 = -0.5 * std::pow(1.5 / (2.0 * PI) * vol, 2.0 / 3.0) / vol;
+        } else if (type == HEJ_0) {
+            g0 = -1.0 / (2.0 * PI * PI * length);   // green_functions.cpp:66-68
         } else {
             g0 = hej3_g0(type, length);
         }
@@ -175,11 +256,12 @@ void green_fill_2unb(const Topo& t, const double hfact[3], const double kfact[3]
                      const double symstart[3], double* g, GreenType type, double length, const std::string& kdir) {
     const bool iso = (hfact[0] == 0.0 && hfact[1] == hfact[2]) || (hfact[1] == 0.0 && hfact[0] == hfact[2]) ||
                      (hfact[2] == 0.0 && hfact[0] == hfact[1]);
-    const int ord = lgf_order(type);
+    std::string file, expn;
+    const bool ord = table_kernel(type, 2, &file, &expn);
     const std::vector<double>* tab = nullptr;
     if (ord) {
         if (!iso) throw std::runtime_error("LGF needs an isotropic grid");
-        tab = &lgf_table(kdir, ord, 2);
+        tab = &kernel_table(kdir, file, 2);
     } else if (!(type == CHAT_2 || (type >= HEJ_2 && type <= HEJ_10))) {
         throw std::runtime_error("kernel not restated in the oracle");
     }
@@ -206,7 +288,7 @@ void green_fill_2unb(const Topo& t, const double hfact[3], const double kfact[3]
                         const int j2 = (a[0] == 0) ? a[2] : (a[1] == 0 ? a[2] : a[1]);
                         double gv;
                         if (j1 < 32 && j2 < 32) gv = (*tab)[j1 + 32 * j2];
-                        else gv = lgf2_expansion(kdir, ord, j1, j2);
+                        else gv = kernel_expansion(kdir, expn, j1, j2, 0.0);
                         v = -gv;
                     } else {
                         v = 0.0;
@@ -307,7 +389,7 @@ void green_fill_0unb(const Topo& t, double hgrid, const double kfact[3], const d
                      double* g, GreenType type, double length, const int* istart_custom) {
     if (kfact[0] == 0.0 || kfact[1] == 0.0) throw std::runtime_error("dk cannot be 0");
     const int ord = lgf_order(type);
-    if (!ord && !(type == CHAT_2 || type == HEJ_0 || (type >= HEJ_2 && type <= HEJ_10)))
+    if (!ord && !is_mehr(type) && !(type == CHAT_2 || type == HEJ_0 || (type >= HEJ_2 && type <= HEJ_10)))
         throw std::runtime_error("kernel not restated in the oracle");
     int lo[3] = {0, 0, 0};
     if (istart_custom)
@@ -321,6 +403,7 @@ void green_fill_0unb(const Topo& t, double hgrid, const double kfact[3], const d
                 const double ksq = k[0] * k[0] + k[1] * k[1] + k[2] * k[2];
                 double v;
                 if (ord) v = lgf_spectral(ord, k, hgrid);
+                else if (is_mehr(type)) v = mehr_spectral(type, k, hgrid);
                 else if (type == CHAT_2 || type == HEJ_0) v = -1.0 / ksq;
                 else v = hej_spectral(type, ksq, length);
                 gat(t, g, i0, i1, i2) = v;

@@ -1,5 +1,5 @@
-// Oracle evaluation of the far-field LGF expansions (reference:
-// src/lgf_3unb0spe.hpp:5-215), read from the coefficient table
+// Oracle evaluation of the far-field LGF / Mehrstellen expansions (reference:
+// src/lgf_3unb0spe.hpp:5-374), read from the coefficient table
 // flups_amd/kernel/lgf_expansion.txt: each line "name a b c p l coef" adds
 // coef * n1^a n2^b n3^c n^-p log(n)^l. Test infrastructure only.
 #include <cmath>
@@ -50,6 +50,9 @@ double eval(const std::vector<Term>& ts, double n1, double n2, double n3) {
 }
 }  // namespace
 
+double kernel_expansion(const std::string& kdir, const std::string& name, double n1, double n2, double n3) {
+    return eval(terms(kdir, name), n1, n2, n3);
+}
 double lgf3_expansion(const std::string& kdir, int order, double n1, double n2, double n3) {
     return eval(terms(kdir, "lgf_" + std::to_string(order) + "_3unb0spe"), n1, n2, n3);
 }

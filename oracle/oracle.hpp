@@ -46,6 +46,7 @@ double bessel_k0(double x);
 double bessel_k1(double x);
 double expint_e1(double x);
 double lgf1d(int order, int n, double c);          // 1-D lattice Green's function
+double sine_integral(double x);                     // Si(x)
 double lgf_symbol(int order, double kh);
 
 // ---- one direction of the transform ----

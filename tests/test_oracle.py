@@ -2,7 +2,7 @@
 
 The oracle is the checker of the HIP path, so it is pinned first:
   * against every golden error line the reference ships
-    (samples/validation/data_ref: 2408 lines restated, samples/solve_vtube/data_ref: 27),
+    (samples/validation/data_ref: 2492 lines, samples/solve_vtube/data_ref: 27),
     at the reference's tolerance (check_res.py:9,40-47);
   * its 1-D transforms against numpy / scipy definitions and brute-force sums;
   * its special functions against scipy;
@@ -20,7 +20,7 @@ import scipy.special as sp
 from tests import oracle_lib as orc
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-RESTATED = {0, 1, 2, 3, 4, 5, 6, 8, 9, 10}   # HEJ_0 and MEHR_* are not restated: parity unpinned
+RESTATED = set(range(15))   # every kernel of the reference
 
 
 def golden_ok(got, want):
@@ -32,7 +32,7 @@ def golden_ok(got, want):
 
 
 def test_validation_golden_all():
-    """every golden line whose kernel the oracle restates (HEJ_0 only in fully spectral cases)"""
+    """every golden error line of samples/validation/data_ref (all 15 kernels)"""
     ref = json.load(open(os.path.join(HERE, "golden", "validation_ref.json")))
     n_ok, unpinned, bad = 0, 0, []
     for e in ref:
@@ -49,7 +49,7 @@ def test_validation_golden_all():
         else:
             bad.append((e, got))
     assert not bad, bad[:5]
-    assert n_ok == 2408 and unpinned == 84
+    assert n_ok == 2492 and unpinned == 0
 
 
 def test_vtube_golden_all():
@@ -144,6 +144,15 @@ def test_special_functions(x):
     assert abs(L.oracle_expint_e1(x) - sp.exp1(x)) <= 1e-13 * sp.exp1(x)
 
 
+@pytest.mark.parametrize("x", [1e-8, 1e-3, 0.3, 1.0, 3.9, 4.0, 4.1, 7.5, 12.0, 33.0, 150.0, 1e4])
+def test_sine_integral(x):
+    """Si against scipy's sici on both sides of the series / continued-fraction switch; Si(-x) = -Si(x)"""
+    L = orc.lib()
+    want = sp.sici(x)[0]
+    assert abs(L.oracle_sine_integral(x) - want) <= 2e-15 * abs(want)
+    assert L.oracle_sine_integral(-x) == -L.oracle_sine_integral(x)
+
+
 def stencil(order):
     t = 2 * np.pi * np.arange(64) / 64
     sym = np.array([orc.lib().oracle_lgf_symbol(order, v) for v in t])
@@ -176,13 +185,21 @@ def test_lgf1d_matches_reference_closed_form_lgf2():
             assert abs(L.oracle_lgf1d(2, n, c) - r ** n / (1 / r - r)) <= 1e-12 * (1 / (1 / r - r))
 
 
-@pytest.mark.parametrize("order", [2, 4, 6, 8])
-def test_lgf_expansion_matches_table_at_seam(order):
+SEAM_KERNELS = [("LGF_%d_3d_32.ker" % o, "LGF_%d_2d_32.ker" % o, "lgf_%d_3unb0spe" % o, "lgf_%d_2unb0spe" % o)
+                for o in (2, 4, 6, 8)] + \
+               [("MEHR_4L_3d_32.ker", "MEHR_4L6L_2d_32.ker", "mehr_4l_3unb0spe", "mehr_4l6l_2unb0spe"),
+                ("MEHR_6L_3d_32.ker", "MEHR_4L6L_2d_32.ker", "mehr_6l_3unb0spe", "mehr_4l6l_2unb0spe"),
+                ("MEHR_4F_3d_32.ker", "MEHR_4F_2d_32.ker", "mehr_4f_3unb0spe", "mehr_4f_2unb0spe"),
+                ("MEHR_6F_3d_32.ker", "MEHR_6F_2d_32.ker", "mehr_6f_3unb0spe", "mehr_6f_2unb0spe")]
+
+
+@pytest.mark.parametrize("files", SEAM_KERNELS, ids=[k[2] for k in SEAM_KERNELS])
+def test_lgf_expansion_matches_table_at_seam(files):
     """the far-field expansion continues the 32^3 / 32^2 tables (error << value at n ~ 31)"""
-    import ctypes
+    f3, f2, e3, e2 = files
     kdir = orc.KDIR.decode()
-    t3 = np.fromfile(os.path.join(kdir, "LGF_%d_3d_32.ker" % order), "<f8").reshape(32, 32, 32)
-    t2 = np.fromfile(os.path.join(kdir, "LGF_%d_2d_32.ker" % order), "<f8").reshape(32, 32)
+    t3 = np.fromfile(os.path.join(kdir, f3), "<f8").reshape(32, 32, 32)
+    t2 = np.fromfile(os.path.join(kdir, f2), "<f8").reshape(32, 32)
     terms = {}
     for line in open(os.path.join(kdir, "lgf_expansion.txt")):
         if line.startswith("#"):
@@ -195,12 +212,12 @@ def test_lgf_expansion_matches_table_at_seam(order):
         return sum(co * n1 ** a * n2 ** b * n3 ** c / n ** p * (np.log(n) if l else 1) for a, b, c, p, l, co in terms[name])
 
     for (i, j, k) in [(31, 0, 0), (31, 10, 3), (20, 20, 20), (31, 31, 31)]:
-        v = ev("lgf_%d_3unb0spe" % order, i, j, k)
+        v = ev(e3, i, j, k)
         assert abs(v - t3[k, j, i]) <= 1e-9 * abs(t3[k, j, i])
     for (i, j) in [(31, 0), (31, 17), (25, 25)]:
-        v = ev("lgf_%d_2unb0spe" % order, i, j, 0)
+        v = ev(e2, i, j, 0)
         assert abs(v - t2[j, i]) <= 1e-9 * abs(t2[j, i])
-    assert abs(ev("lgf_%d_3unb0spe" % order, 1000, 0, 0) * 4 * np.pi * 1000 - 1) < 1e-6   # 1/(4 pi n)
+    assert abs(ev(e3, 1000, 0, 0) * 4 * np.pi * 1000 - 1) < 1e-6   # 1/(4 pi n)
 
 
 def test_lgf2_table_known_values():
