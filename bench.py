@@ -148,6 +148,79 @@ def serve_mix(fl, nreq, rate, workers, seed=7):
             "served": served, "plain_one_at_a_time": plain}
 
 
+def rhs_mixed(n, seed):
+    """seeded N(0,1) field smoothed by a Gaussian of width 3h, mean removed along the periodic direction (y)"""
+    from scipy.ndimage import gaussian_filter
+    rng = np.random.default_rng(seed)
+    f = gaussian_filter(rng.normal(size=(n, n, n)), sigma=3.0, mode="wrap")
+    f -= f.mean(axis=1, keepdims=True)
+    return f[None]
+
+
+def rhs_ring(n, seed, h):
+    """vorticity of a vortex ring (radius 0.2, Gaussian core sigma 0.03, seeded centre and axis) on the node grid
+    x_i = i h, plus 1e-3 N(0,1) noise; shape (3, n, n, n)"""
+    rng = np.random.default_rng(seed)
+    c = rng.uniform(0.4, 0.6, 3)
+    a = rng.normal(size=3)
+    a /= np.linalg.norm(a)
+    R, sig = 0.2, 0.03
+    x = np.arange(n) * h
+    out = 1e-3 * rng.normal(size=(3, n, n, n))
+    # the core is negligible (exp(-36)) beyond R + 6 sigma from the centre: work in that box only
+    rb = R + 6 * sig
+    lo = np.clip(np.floor((c - rb) / h).astype(int), 0, n - 1)
+    hi = np.clip(np.ceil((c + rb) / h).astype(int) + 1, 1, n)
+    Y, X = np.meshgrid(x[lo[1]:hi[1]], x[lo[0]:hi[0]], indexing="ij")
+    for k in range(lo[2], hi[2]):   # one z plane at a time keeps the temporaries small
+        d = np.stack([X - c[0], Y - c[1], np.full_like(X, x[k] - c[2])])
+        za = np.tensordot(a, d, axes=1)
+        rho = d - a[:, None, None] * za
+        rn = np.sqrt(np.sum(rho * rho, axis=0)) + 1e-300
+        s2 = (rn - R) ** 2 + za ** 2
+        w = np.exp(-s2 / sig ** 2) / (np.pi * sig ** 2)
+        t = np.cross(a[:, None, None], rho, axis=0) / rn
+        out[:, k, lo[1]:hi[1], lo[0]:hi[0]] += w * t
+    return out
+
+
+def run_config(fl, name, nglob, bc, center, kernel, lda, typ, odiff, rhs, h, L, steps, warmup):
+    topo = fl.Topology(0, lda, list(nglob), [1, 1, 1], False)
+    s = fl.Solver(topo, bc, h, L, order_diff=odiff, center_type=[center] * 3)
+    s.set_green_type(kernel)
+    t0 = time.time()
+    s.setup()
+    setup_s = time.time() - t0
+    din = fl.DeviceArray.from_numpy(rhs)
+    dout = fl.DeviceArray(rhs.shape)
+    for _ in range(warmup):
+        s.solve(dout, din, typ)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        s.solve(dout, din, typ)
+    el = (time.perf_counter() - t0) / steps
+    by = s.solve_bytes(typ)
+    return {"workload": name, "solves_per_s": round(1.0 / el, 3), "ms_per_step": round(el * 1e3, 3),
+            "algorithmic_GB": round(by / 1e9, 3), "achieved_GBs": round(by / el / 1e9, 1),
+            "roofline_frac": round(by / el / PEAK_HBM, 4), "path": ["general", "fused-fp64", "fused-fp32"][s.path],
+            "setup_s": round(setup_s, 2), "steps": steps}
+
+
+def other_configs(fl, which):
+    out = {}
+    if "mixed" in which:
+        n = 256
+        out["mixed"] = run_config(fl, "cell 256^3, x EVEN-ODD, y PER, z UNB (padded 512), LGF_4, lda 1, FP64",
+                                  [n] * 3, [[0, 1], [3, 3], [4, 4]], 1, 8, 1, 0, 0, rhs_mixed(n, 11),
+                                  [1.0 / n] * 3, [1.0] * 3, 20, 2)
+    if "ring" in which:
+        n = 513
+        h = 1.0 / (n - 1)
+        out["ring"] = run_config(fl, "node 513^3 all faces UNB (padded 1024^3), HEJ_6, lda 3 ROT (SPE), FP64",
+                                 [n] * 3, [[4, 4]] * 3, 0, 4, 3, 1, 1, rhs_ring(n, 5, h), [h] * 3, [1.0] * 3, 3, 1)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -159,8 +232,9 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=10)
     ap.add_argument("--fp32-steps", type=int, default=50, help="timed solves of the FP32 variant (0 = skip)")
     ap.add_argument("--serve-requests", type=int, default=300, help="requests of the served mix (0 = skip)")
-    ap.add_argument("--serve-rate", type=float, default=800.0, help="Poisson arrival rate of the served mix (1/s)")
+    ap.add_argument("--serve-rate", type=float, default=400.0, help="Poisson arrival rate of the served mix (1/s)")
     ap.add_argument("--serve-workers", type=int, default=4)
+    ap.add_argument("--configs", default="mixed,ring", help="other workloads to time (comma list of mixed, ring; '' = none)")
     ap.add_argument("--copy-calib-mb", type=int, default=0,
                     help="also run one plain device-to-device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
@@ -261,6 +335,7 @@ def main():
         del s32, f32
 
     served = serve_mix(fl, args.serve_requests, args.serve_rate, args.serve_workers) if args.serve_requests > 0 else None
+    others = other_configs(fl, [c for c in args.configs.split(",") if c]) if args.configs else None
 
     cpu = None
     if args.cpu_reps > 0:
@@ -285,6 +360,7 @@ def main():
         "setup_s": round(setup_s, 3),
         "fp32": fp32,
         "served_mix": served,
+        "other_configs": others,
     }
     print(json.dumps(out))
 

@@ -141,6 +141,8 @@ def lib():
         L.flups_switchtopo_info.argtypes = [c_void_p]
         L.flups_set_autotune.argtypes = [c_void_p, c_int]
         L.flups_get_schedule.argtypes = [c_void_p, POINTER(c_int)]
+        L.flups_get_solve_bytes.restype = c_double
+        L.flups_get_solve_bytes.argtypes = [c_void_p, c_int]
         L.flups_get_path.argtypes = [c_void_p]
         L.flups_get_path.restype = c_int
         L.flups_get_last_solve_ns.argtypes = [c_void_p]
@@ -461,6 +463,10 @@ class Solver:
         lib().flups_set_autotune(self._solver, 1 if on else 0)
         _check(not lib().flups_get_last_error(), "flups_set_autotune")
 
+    def solve_bytes(self, solver_type=SolverType.STD):
+        """minimum HBM bytes of one solve (roofline numerator)"""
+        return lib().flups_get_solve_bytes(self._solver, int(solver_type))
+
     @property
     def schedule(self):
         """(z pass, y passes, x order) schedule of the fused pipeline; -1s on the general path"""
@@ -515,7 +521,7 @@ class Solver:
         _check(not L.flups_get_last_error(), fn.__name__)
 
     def do_copy(self, topology, data, sign):
-        self._adv(lib().flups_do_copy, topology._ptr(), _data_ptr(data), int(sign))
+        self._adv(lib().flups_do_copy, topology._ptr, _data_ptr(data), int(sign))
 
     def do_fft(self, data, sign):
         self._adv(lib().flups_do_FFT, _data_ptr(data), int(sign))

@@ -268,6 +268,12 @@ void flups_set_autotune(FLUPS_Solver* s, int on) {
         return 0;
     });
 }
+double flups_get_solve_bytes(FLUPS_Solver* s, const FLUPS_SolverType type) {
+    return guard("flups_get_solve_bytes", [&]() -> double {
+        FLUPS_REQUIRE(s, "NULL solver");
+        return s->impl->algorithmic_bytes((int)type);
+    });
+}
 void flups_get_schedule(FLUPS_Solver* s, int sched[3]) {
     if (s && sched) s->impl->schedule(sched);
 }

@@ -327,6 +327,106 @@ __global__ void __launch_bounds__(TX * M / 8) z_fused(ColArgs a) {
     }
 }
 
+// Rotational (Biot-Savart) z pass, in place on the three components of S1:
+// forward FFT of omega_x, omega_y, omega_z along z, psi = normfact G omega, then
+// u = curl psi = i K x psi with the derivative symbol K of each direction (the
+// wave number itself, or the symbol of the 2nd/4th/6th-order centred
+// difference), inverse FFTs, keep the first nin rows. The same y-row pairing
+// and Green's slice as z_fused. src/dothemagic_rot.ipp:102-349.
+struct RotK {
+    double kfact[3], h[3], symstart[3];
+    int order;   // 1 spectral, 2 / 4 / 6 centred differences
+};
+
+template <typename R>
+__device__ __forceinline__ R dsym(const RotK& k, int d, int idx) {
+    // symmetrised mode index (mirrored half negative), then the derivative symbol
+    const double ss = k.symstart[d];
+    int is = idx;
+    if (ss != 0.0 && idx > ss) {
+        const int v = (int)fabs(2.0 * ss - idx);
+        is = -(v > 1 ? v : 1);
+    }
+    const double kk = is * k.kfact[d], hh = k.h[d];
+    switch (k.order) {
+        case 2: return (R)(sin(kk * hh) / hh);
+        case 4: return (R)((4.0 / 3.0) * sin(kk * hh) / hh - (1.0 / 6.0) * sin(2.0 * kk * hh) / hh);
+        case 6: return (R)(1.5 * sin(kk * hh) / hh - 0.3 * sin(2.0 * kk * hh) / hh + (1.0 / 30.0) * sin(3.0 * kk * hh) / hh);
+        default: return (R)kk;
+    }
+}
+
+template <typename R, int M, int TX>
+__global__ void __launch_bounds__(TX * M / 8) z_rot(ColArgs a, RotK rk) {
+    __shared__ C2<R> lds[M * TX];
+    constexpr int T = M / 8;
+    const int c = threadIdx.x % TX;
+    const int j = threadIdx.x / TX;
+    const int ntx = (a.nx + TX - 1) / TX;
+    const int blk = a.xcd ? xcd_group(blockIdx.x, gridDim.x) : blockIdx.x;
+    const int tile = blk % ntx;
+    const int y0 = blk / ntx;
+    const int y1 = a.gmy - y0;
+    const bool pair = y0 != 0 && y1 != y0;
+    const int x = tile * TX + c;
+    const bool live = x < a.nx;
+    char* base = reinterpret_cast<char*>(a.dst) + (size_t)sizeof(C2<R>) * ((long long)tile * TX);
+    const size_t cstride = (size_t)sizeof(C2<R>) * a.comp_src;
+    const size_t rstride = (size_t)sizeof(C2<R>) * T * a.s_col;
+    const unsigned voff = (unsigned)(sizeof(C2<R>) * ((size_t)j * a.s_col + c));
+    const char* g = reinterpret_cast<const char*>(a.green) + sizeof(R) * ((long long)y0 * ntx + tile) * (M / 2 + 1) * TX;
+    const C2<R>* tw = reinterpret_cast<const C2<R>*>(a.tw);
+    R gv[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const int kz = j + i * T;
+        gv[i] = *reinterpret_cast<const R*>(g + (unsigned)(sizeof(R) * (min(kz, M - kz) * TX + c)));
+    }
+    const R kx = dsym<R>(rk, 0, x);
+    for (int s = 0; s < 2; ++s) {
+        const int y = s == 0 ? y0 : y1;
+        char* b = base + (size_t)sizeof(C2<R>) * ((long long)y * a.s_other_src);
+        const R ky = dsym<R>(rk, 1, y);
+        C2<R> v[3][8];
+#pragma unroll
+        for (int cc = 0; cc < 3; ++cc) {
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                const int row = j + i * T;
+                v[cc][i] = (live && row < a.nin) ? *reinterpret_cast<const C2<R>*>(b + cc * cstride + i * rstride + voff)
+                                                 : C2<R>{0, 0};
+            }
+            column_fft<R, M, true, TX>(v[cc], lds + c, j, tw);
+        }
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const R kz = dsym<R>(rk, 2, j + i * T);
+            const C2<R> wx = {v[0][i].x * gv[i], v[0][i].y * gv[i]};
+            const C2<R> wy = {v[1][i].x * gv[i], v[1][i].y * gv[i]};
+            const C2<R> wz = {v[2][i].x * gv[i], v[2][i].y * gv[i]};
+            // u = i K x w: (K x w) then multiply by i: (p + i q) i = -q + i p
+            const C2<R> cx = {ky * wz.x - kz * wy.x, ky * wz.y - kz * wy.y};
+            const C2<R> cy = {kz * wx.x - kx * wz.x, kz * wx.y - kx * wz.y};
+            const C2<R> cz = {kx * wy.x - ky * wx.x, kx * wy.y - ky * wx.y};
+            v[0][i] = {-cx.y, cx.x};
+            v[1][i] = {-cy.y, cy.x};
+            v[2][i] = {-cz.y, cz.x};
+        }
+#pragma unroll
+        for (int cc = 0; cc < 3; ++cc) {
+            column_fft<R, M, false, TX>(v[cc], lds + c, j, tw);
+            if (live) {
+#pragma unroll
+                for (int i = 0; i < 8; ++i) {
+                    const int row = j + i * T;
+                    if (row < a.nout) *reinterpret_cast<C2<R>*>(b + cc * cstride + i * rstride + voff) = v[cc][i];
+                }
+            }
+        }
+        if (!pair) break;   // uniform over the block
+    }
+}
+
 // ---------------------------------------------------------------------------
 // x rows: r2c of the zero-padded real row (length M = 2*H) through the packed
 // complex FFT of length H: z[n] = x[2n] + i x[2n+1], Z = FFT_H(z),
@@ -479,6 +579,7 @@ struct FastPlan {
     int ysched = 0;          // y passes: bit 2 half-width tiles, bit 4 XCD-grouped
     int order = 4;           // traversal: bit 0 y_fwd, 1 y_inv, 2 x_c2r, 3 x_r2c walk their batch backwards
     int gtx = 0;             // x tile width of the compressed Green's layout
+    RotK rot{};              // wave-number data of the rotational pass
     void* s0 = nullptr;     // (ldx, N1, N2) complex, FP64 or FP32
     void* s1 = nullptr;     // (ldx, M1, N2) complex
     void* green = nullptr;  // normfact * G, [M1/2+1][x tiles][M2/2+1][tile width of the z kernel]
@@ -498,7 +599,7 @@ void build_fast_plan(SolverImpl& s, FastPlan* f, size_t es, int align);
 void destroy_fast_plan(FastPlan* f);
 
 FastPlan* make_fast_plan(SolverImpl& s) {
-    if (s.ndim_ != 3 || s.phys_.axis != 0 || s.odiff_ != NOD) return nullptr;
+    if (s.ndim_ != 3 || s.phys_.axis != 0) return nullptr;
     for (int ip = 0; ip < 3; ++ip)
         if (s.fwd_[ip].type != UNBUNB || s.fwd_[ip].dim != ip) return nullptr;
     if (getenv("FLUPS_FORCE_GENERIC")) return nullptr;
@@ -514,6 +615,12 @@ FastPlan* make_fast_plan(SolverImpl& s) {
     }
     f->lda = s.lda_;
     f->fp32 = s.fp32_;
+    for (int ip = 0; ip < 3; ++ip) {
+        f->rot.kfact[ip] = s.fwd_[ip].kfact;
+        f->rot.symstart[ip] = s.fwd_[ip].symstart;
+        f->rot.h[ip] = s.h_[ip];
+    }
+    f->rot.order = s.odiff_;
     f->nx = f->M[0] / 2 + 1;
     const int align = f->fp32 ? 16 : 8;
     f->ldx = (f->nx + align - 1) / align * align;
@@ -668,6 +775,20 @@ void col_dispatch(int M, const ColArgs& a, int comps, int zsched, hipStream_t st
     }
 }
 
+template <typename R, int TX>
+void launch_rot(int M, const ColArgs& a, const RotK& rk, hipStream_t st) {
+    dim3 grid(((a.nx + TX - 1) / TX) * (a.gmy / 2 + 1), 1);
+    switch (M) {
+        case 64: hipLaunchKernelGGL((z_rot<R, 64, TX>), grid, dim3(TX * 8), 0, st, a, rk); break;
+        case 128: hipLaunchKernelGGL((z_rot<R, 128, TX>), grid, dim3(TX * 16), 0, st, a, rk); break;
+        case 256: hipLaunchKernelGGL((z_rot<R, 256, TX>), grid, dim3(TX * 32), 0, st, a, rk); break;
+        case 512: hipLaunchKernelGGL((z_rot<R, 512, TX>), grid, dim3(TX * 64), 0, st, a, rk); break;
+        case 1024: hipLaunchKernelGGL((z_rot<R, 1024, TX>), grid, dim3(TX * 128), 0, st, a, rk); break;
+        default: fail("unsupported column length %d", M);
+    }
+    HIP_CHECK(hipGetLastError());
+}
+
 template <typename R>
 void row_dispatch(bool fwd, int H, const RowArgs& a, int comps, hipStream_t st) {
 #define FLUPS_ROW(HH, RR)                                                                                   \
@@ -690,7 +811,7 @@ void row_dispatch(bool fwd, int H, const RowArgs& a, int comps, hipStream_t st) 
 }
 
 template <typename R>
-void run_fast(FastPlan* f, const double* rhs, double* field, const View& uin, const View& uout, hipStream_t st) {
+void run_fast(FastPlan* f, const double* rhs, double* field, const View& uin, const View& uout, hipStream_t st, int type = 0) {
     const int N0 = f->n[0], N1 = f->n[1], N2 = f->n[2];
     const int M0 = f->M[0], M1 = f->M[1], M2 = f->M[2];
     const int H = M0 / 2;
@@ -758,7 +879,15 @@ void run_fast(FastPlan* f, const double* rhs, double* field, const View& uin, co
     z.comp_src = c1;
     z.comp_dst = c1;
     z.xcd = (f->zsched >> 4) & 1;
-    col_dispatch<R, TX, true, true>(M2, z, f->lda, f->zsched, st);
+    if (type == 1) {
+        // rotational pass: all three components per block, tile width of the Green's layout
+        FLUPS_REQUIRE(f->lda == 3, "the rotational solve needs lda = 3");
+        if (f->gtx == TX) launch_rot<R, TX>(M2, z, f->rot, st);
+        else if (f->gtx == TX / 2) launch_rot<R, TX / 2>(M2, z, f->rot, st);
+        else launch_rot<R, (TX >= 8 ? TX / 4 : 1)>(M2, z, f->rot, st);
+    } else {
+        col_dispatch<R, TX, true, true>(M2, z, f->lda, f->zsched, st);
+    }
     if (f->profile) HIP_CHECK(hipEventRecord(f->ev[3], st));
     // 4. y inverse: S1 -> S0, keep N1
     ColArgs b = a;
@@ -797,12 +926,14 @@ void run_fast(FastPlan* f, const double* rhs, double* field, const View& uin, co
 
 }  // namespace
 
-void fast_solve(FastPlan* f, double* field, const double* rhs, const View& uin, const View& uout, hipStream_t s) {
-    if (f->fp32) run_fast<float>(f, rhs, field, uin, uout, s);
-    else run_fast<double>(f, rhs, field, uin, uout, s);
+void fast_solve(FastPlan* f, double* field, const double* rhs, const View& uin, const View& uout, hipStream_t s, int type) {
+    if (f->fp32) run_fast<float>(f, rhs, field, uin, uout, s, type);
+    else run_fast<double>(f, rhs, field, uin, uout, s, type);
 }
 
 size_t fast_device_bytes(const FastPlan* f) { return f->dev_bytes; }
+
+double fast_bytes(const FastPlan* f) { return f->bytes[0] + f->bytes[1] + f->bytes[2] + f->bytes[3] + f->bytes[4]; }
 
 void fast_schedule(const FastPlan* f, int out[3]) {
     out[0] = f->zsched;

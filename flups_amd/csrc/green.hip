@@ -6,7 +6,7 @@
 //   1 unbounded + 2 spectral        :407-538
 //   fully spectral                  :560-708
 // with the closed-form kernels of src/green_kernels.hpp (HEJ_2..10, CHAT_2,
-// LGF_2..8) evaluated in FP64. Special functions are computed here from their
+// LGF_2..8, MEHR_4L/6L/4F/6F) evaluated in FP64. Special functions are computed here from their
 // integral or series definitions:
 //   K_nu(x) = int_0^inf exp(-x cosh t) cosh(nu t) dt, trapezoidal rule with
 //             step 1/4 (error ~ exp(-pi^2 / h) for an integrand analytic in
@@ -94,6 +94,75 @@ __device__ double hej2_q(int t, double r2) {
 __device__ int lgf_order(int t) {
     return t == LGF_2 ? 2 : t == LGF_4 ? 4 : t == LGF_6 ? 6 : t == LGF_8 ? 8 : 0;
 }
+__device__ bool is_mehr(int t) { return t == MEHR_4L || t == MEHR_6L || t == MEHR_4F || t == MEHR_6F; }
+// kernels given by a table near the origin and an expansion beyond (LGF, Mehrstellen)
+__device__ bool is_table(int t) { return lgf_order(t) != 0 || is_mehr(t); }
+
+// Mehrstellen symbols in y = sin^2(k h / 2) (src/green_kernels.hpp:744-773)
+__device__ double mehr_spectral(int t, const double k[3], double h) {
+    double y[3];
+    for (int d = 0; d < 3; ++d) {
+        const double sv = sin(0.5 * k[d] * h);
+        y[d] = sv * sv;
+    }
+    const double s1 = y[0] + y[1] + y[2], s2 = y[0] * y[1] + y[1] * y[2] + y[0] * y[2], s3 = y[0] * y[1] * y[2];
+    const double q = y[0] * y[0] + y[1] * y[1] + y[2] * y[2];
+    const bool six = (t == MEHR_6L || t == MEHR_6F);
+    const double left = -4.0 * s1 + 8.0 / 3.0 * s2 - (six ? 32.0 / 15.0 * s3 : 0.0);
+    double right = 1.0;
+    if (t == MEHR_4F) right = 1.0 - s1 / 3.0;
+    if (t == MEHR_6F) right = 1.0 - s1 / 3.0 + 8.0 / 45.0 * s2 - q / 15.0;
+    return h * h * right / left;
+}
+
+// sine integral Si(x), x >= 0: power series up to 4; beyond, Si = pi/2 + Im E1(ix)
+// with E1 from its continued fraction (Abramowitz & Stegun 5.1.22, modified Lentz)
+__device__ double sine_integral(double x) {
+    if (x <= 4.0) {
+        double t = x, s = x;
+        const double x2 = x * x;
+        for (int k = 1; k < 40; ++k) {
+            t *= -x2 / ((2.0 * k) * (2.0 * k + 1));
+            const double term = t / (2 * k + 1);
+            s += term;
+            if (fabs(term) < 1e-18 * fabs(s)) break;
+        }
+        return s;
+    }
+    // complex arithmetic on (re, im) pairs, z = i x
+    double br = 1.0, bi = x;                   // b = z + 1
+    double cr = 1e300, ci = 0.0;               // c = 1 / tiny
+    double dr, di;
+    {
+        const double m = br * br + bi * bi;
+        dr = br / m;
+        di = -bi / m;
+    }
+    double hr = dr, hi = di;
+    for (int i = 1; i < 2000; ++i) {
+        const double an = -(double)i * i;
+        br += 2.0;
+        // d = 1 / (an d + b)
+        double tr = an * dr + br, ti = an * di + bi;
+        double m = tr * tr + ti * ti;
+        dr = tr / m;
+        di = -ti / m;
+        // c = b + an / c
+        m = cr * cr + ci * ci;
+        const double qr = an * cr / m, qi = -an * ci / m;
+        cr = br + qr;
+        ci = bi + qi;
+        const double er = cr * dr - ci * di, ei = cr * di + ci * dr;   // del = c d
+        const double nr = hr * er - hi * ei, ni = hr * ei + hi * er;
+        hr = nr;
+        hi = ni;
+        if (fabs(er - 1.0) + fabs(ei) < 1e-17) break;
+    }
+    // E1 = exp(-i x) h
+    const double cx = cos(x), sx = sin(x);
+    const double e1i = cx * hi - sx * hr;
+    return 0.5 * kPi + e1i;
+}
 __device__ double lgf_symbol(int order, double k) {
     const double s1 = sin(0.5 * k), s2 = sin(k), s3 = sin(1.5 * k), s4 = sin(2.0 * k);
     switch (order) {
@@ -134,7 +203,7 @@ __global__ void fill_3unb(double* g, FillGeom geo, GreenArgs A) {
         const int is[3] = {symid(i[0], A.symstart[0]), symid(i[1], A.symstart[1]), symid(i[2], A.symstart[2])};
         const double x0 = is[0] * A.hfact[0], x1 = is[1] * A.hfact[1], x2 = is[2] * A.hfact[2];
         const double r = sqrt(x0 * x0 + x1 * x1 + x2 * x2);
-        const int ord = lgf_order(A.type);
+        const bool ord = is_table(A.type);
         double v;
         if (ord) {
             const int ax0 = geo.axis, ax1 = (ax0 + 1) % 3, ax2 = (ax0 + 2) % 3;
@@ -143,6 +212,9 @@ __global__ void fill_3unb(double* g, FillGeom geo, GreenArgs A) {
             v = -gv / A.hfact[0];
         } else if (A.type == CHAT_2) {
             v = -1.0 / (4.0 * kPi * r);
+        } else if (A.type == HEJ_0) {   // -Si(rho) / (2 pi^2 sigma rho), sigma = h / pi
+            const double rho = r / A.length;
+            v = rho > 0.0 ? -sine_integral(rho) / (rho * 2.0 * kPi * kPi * A.length) : 0.0;
         } else {
             const double rho = r / A.length;
             v = -1.0 / (4.0 * kPi * r) * (erf(rho / sqrt(2.0)) + hej3_poly(A.type, rho) * exp(-0.5 * rho * rho) / sqrt(2.0 * kPi));
@@ -157,7 +229,7 @@ __global__ void fill_2unb(double* g, FillGeom geo, GreenArgs A) {
     const double sh = A.hfact[0] + A.hfact[1] + A.hfact[2];
     const double sk = A.kfact[0] + A.kfact[1] + A.kfact[2];
     const double req = sqrt(A.hfact[0] * A.hfact[1] + A.hfact[1] * A.hfact[2] + A.hfact[2] * A.hfact[0]) / sqrt(kPi);
-    const int ord = lgf_order(A.type);
+    const bool ord = is_table(A.type);
     for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < tot; e += (long long)gridDim.x * blockDim.x) {
         int i[3];
         unflat(e, geo, i);
@@ -394,6 +466,8 @@ __global__ void fill_0unb(double* g, FillGeom geo, GreenArgs A, int lo0, int lo1
         double v;
         if (ord) {
             v = -A.hgrid * A.hgrid / (lgf_symbol(ord, k[0] * A.hgrid) + lgf_symbol(ord, k[1] * A.hgrid) + lgf_symbol(ord, k[2] * A.hgrid));
+        } else if (is_mehr(A.type)) {
+            v = mehr_spectral(A.type, k, A.hgrid);
         } else if (A.type == CHAT_2 || A.type == HEJ_0) {
             v = -1.0 / ksq;
         } else {

@@ -21,12 +21,14 @@ size_t generic_lds_bytes(const PassDesc& P, int ppb);
 void launch_generic(const PassDesc& P, hipStream_t s);
 FastPlan* make_fast_plan(SolverImpl& s);   // nullptr if the fused pipeline does not apply
 void destroy_fast_plan(FastPlan* f);
-void fast_solve(FastPlan* f, double* field, const double* rhs, const View& uin, const View& uout, hipStream_t s);
+void fast_solve(FastPlan* f, double* field, const double* rhs, const View& uin, const View& uout, hipStream_t s,
+                int type);
 void fast_collect(FastPlan* f);
 void fast_profile(FastPlan* f, bool on);
 int fast_profile_get(FastPlan* f, int i, const char** name, double* ms, long* calls, double* bytes);
 size_t fast_device_bytes(const FastPlan* f);
 void fast_schedule(const FastPlan* f, int out[3]);
+double fast_bytes(const FastPlan* f);
 void fast_autotune(FastPlan* f, double* out, const double* in, const View& uin, const View& uout, const double* dgreen,
                    double normfact, hipStream_t st);
 
@@ -309,6 +311,8 @@ void SolverImpl::fill_green_args(GreenArgs& A, const double hfact[3], const doub
         } else if (gtype_ >= HEJ_2 && gtype_ <= HEJ_10) {
             static const double mult[5] = {1.0 / 4.0, 3.0 / 8.0, 15.0 / 32.0, 35.0 / 64.0, 315.0 / 512.0};
             A.g0 = -M_SQRT2 * mult[gtype_ - HEJ_2] / (length * std::sqrt(M_PI * M_PI * M_PI));
+        } else if (gtype_ == HEJ_0) {
+            A.g0 = -1.0 / (2.0 * M_PI * M_PI * length);   // src/green_functions.cpp:66-68
         }
     }
     // stencil coefficients of the 1-D lattice Green's functions: s_m (symbol s_0 + 2 sum s_m cos(m t))
@@ -370,10 +374,10 @@ void SolverImpl::compute_green() {
     double hfact[3] = {0, 0, 0}, kfact[3] = {0, 0, 0}, koffset[3] = {0, 0, 0}, symstart[3] = {0, 0, 0};
     double length = alpha_ * h_[0];
     if (gtype_ == HEJ_0) length = h_[0] / M_PI;
-    const bool iso = (gtype_ >= HEJ_2 && gtype_ <= HEJ_0) || gtype_ == LGF_2;
+    const bool iso = (gtype_ >= HEJ_2 && gtype_ <= MEHR_6F) || gtype_ == LGF_2;   // regularised, LGF and Mehrstellen
     FLUPS_REQUIRE(!(iso && ((ndim_ == 3 && (h_[0] != h_[1] || h_[1] != h_[2])) || (ndim_ == 2 && h_[0] != h_[1]))),
                   "a regularized kernel or an LGF requires dx = dy = dz");
-    FLUPS_REQUIRE(gtype_ != HEJ_0 && gtype_ < MEHR_4L, "Green's kernel %d is not available on the device yet", gtype_);
+    FLUPS_REQUIRE(gtype_ >= CHAT_2 && gtype_ <= MEHR_6F, "unknown Green's kernel %d", gtype_);
     for (int ip = 0; ip < ndim_; ++ip) {
         const DirPlan& p = gpl_[ip];
         const int d = p.dim;
@@ -386,19 +390,32 @@ void SolverImpl::compute_green() {
     int nspec = 0;
     for (int d = 0; d < ndim_; ++d) if (spec[d]) ++nspec;
     const int nunb = ndim_ - nspec;
+    const bool mehr = gtype_ >= MEHR_4L && gtype_ <= MEHR_6F;
+    // src/green_functions.cpp:444-446 (HEJ_0, one unbounded direction: unavailable in the reference too)
+    FLUPS_REQUIRE(!(gtype_ == HEJ_0 && nunb == 1), "HEJ_0 kernel not available for 1D unbounded problems");
+    FLUPS_REQUIRE(!(gtype_ == HEJ_0 && nunb == 2), "HEJ_0 with two unbounded directions is not available on the device");
+    FLUPS_REQUIRE(!(mehr && nunb == 1), "Mehrstellen kernels with one unbounded direction are not available on the device");
 
     GreenArgs A;
     fill_green_args(A, hfact, kfact, koffset, symstart, length, nunb);
     const int ord = (gtype_ == LGF_2) ? 2 : (gtype_ == LGF_4) ? 4 : (gtype_ == LGF_6) ? 6 : (gtype_ == LGF_8) ? 8 : 0;
     double* dtab = nullptr;
     ExpTerm* dexp = nullptr;
-    if (ord && (nunb == 3 || nunb == 2)) {
+    if ((ord || mehr) && (nunb == 3 || nunb == 2)) {
+        // table near the origin + far-field expansion (src/green_functions.hpp:30-92 for the file names)
         const int gd = nunb;
-        char name[256];
-        std::snprintf(name, sizeof(name), "%s/LGF_%d_%dd_32.ker", kdir_.c_str(), ord, gd);
+        char name[256], ename[64];
+        if (ord) {
+            std::snprintf(name, sizeof(name), "%s/LGF_%d_%dd_32.ker", kdir_.c_str(), ord, gd);
+            std::snprintf(ename, sizeof(ename), "lgf_%d_%dunb0spe", ord, gd);
+        } else {
+            const char* up = gtype_ == MEHR_4L ? "4L" : gtype_ == MEHR_6L ? "6L" : gtype_ == MEHR_4F ? "4F" : "6F";
+            const char* lo = gtype_ == MEHR_4L ? "4l" : gtype_ == MEHR_6L ? "6l" : gtype_ == MEHR_4F ? "4f" : "6f";
+            const bool lump = (gtype_ == MEHR_4L || gtype_ == MEHR_6L) && gd == 2;   // 4L and 6L share the 2-D table
+            std::snprintf(name, sizeof(name), "%s/MEHR_%s_%dd_32.ker", kdir_.c_str(), lump ? "4L6L" : up, gd);
+            std::snprintf(ename, sizeof(ename), "mehr_%s_%dunb0spe", lump ? "4l6l" : lo, gd);
+        }
         const auto tab = read_table(name, gd == 3 ? 32 * 32 * 32 : 32 * 32);
-        char ename[64];
-        std::snprintf(ename, sizeof(ename), "lgf_%d_%dunb0spe", ord, gd);
         const auto ex = read_expansion(kdir_ + "/lgf_expansion.txt", ename);
         HIP_CHECK(hipMalloc(&dtab, tab.size() * sizeof(double)));
         HIP_CHECK(hipMemcpyAsync(dtab, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice, stream_));
@@ -431,7 +448,7 @@ void SolverImpl::compute_green() {
     launch_scale(a, gfin.count(), volfact_, stream_);
     const bool hej = gtype_ >= HEJ_2 && gtype_ <= HEJ_10;
     const bool lgf = ord != 0;
-    if (ndim_ == 3 && nspec == 1 && (hej || lgf)) {
+    if (ndim_ == 3 && nspec == 1 && (hej || lgf || mehr)) {
         // 2 unbounded + 1 spectral: the spectral kernel away from the k = 0 plane (src/Solver.cpp:1092-1101)
         int lo[3];
         for (int ip = 0; ip < 3; ++ip) {
@@ -635,7 +652,7 @@ void SolverImpl::check_type(int type) const {
 // arrays when the fused pipeline cannot run it)
 void SolverImpl::prepare(int type) {
     check_type(type);
-    if (fast_ && type == 0) return;
+    if (fast_ && (type == 0 || lda_ == 3)) return;
     if (!dstage_[0]) {
         for (int ip = 0; ip < ndim_; ++ip) {
             const size_t m = std::max(hat_[ip].count(), stage_[ip].count());
@@ -664,7 +681,8 @@ void SolverImpl::warm_tables() {
 // pointers, nothing is allocated or synchronised (usable under graph capture
 // once prepare(type) has run)
 void SolverImpl::enqueue(double* field, const double* rhs, int type) {
-    if (fast_ && type == 0) fast_solve(fast_, field, rhs, user_view(const_cast<double*>(rhs)), user_view(field), stream_);
+    if (fast_ && (type == 0 || lda_ == 3))
+        fast_solve(fast_, field, rhs, user_view(const_cast<double*>(rhs)), user_view(field), stream_, type);
     else {
         FLUPS_REQUIRE(dstage_[0] && dgreen_, "solver not prepared for solve type %d", type);
         solve_generic(field, rhs, type);
@@ -729,7 +747,7 @@ void SolverImpl::solve(double* field, const double* rhs, int type) {
     float ms = 0;
     HIP_CHECK(hipEventElapsedTime(&ms, ev0_, ev1_));
     last_ns_ = ms * 1e6;
-    if (fast_ && type == 0) fast_collect(fast_);
+    if (fast_ && (type == 0 || lda_ == 3)) fast_collect(fast_);
 }
 
 // The solve's kernels are recorded once per (field, rhs, type) into a hipGraph and
@@ -765,6 +783,24 @@ void SolverImpl::launch_graph(double* field, const double* rhs, int type) {
         gtype_key_ = type;
     }
     HIP_CHECK(hipGraphLaunch(gexec_, stream_));
+}
+
+// minimum HBM traffic of one solve: every pass reads its input and writes its
+// output once (zero padding not read, truncated halves not written), the
+// multiply reads the field and the Green's function and writes the field
+double SolverImpl::algorithmic_bytes(int type) const {
+    if (fast_ && (type == 0 || lda_ == 3)) return fast_bytes(fast_);
+    const double w = sizeof(double);
+    const double user = (double)phys_.nglob[0] * phys_.nglob[1] * phys_.nglob[2] * lda_;
+    double b = 0;
+    for (int ip = 0; ip < ndim_; ++ip) {
+        const double in = ip == 0 ? user : (double)stage_[ip - 1].count();
+        b += 2.0 * w * (in + (double)stage_[ip].count());   // forward pass and its backward mirror
+    }
+    const Shape& sp = stage_[ndim_ - 1];
+    b += w * (2.0 * (double)sp.count() + (double)gpost_[ndim_ - 1].count());
+    (void)type;
+    return b;
 }
 
 void SolverImpl::profile(bool on) {

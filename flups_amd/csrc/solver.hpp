@@ -49,6 +49,7 @@ class SolverImpl {
         autotune_ = on;
     }
     void schedule(int out[3]) const;   // fused pipeline schedule (z, y, order), -1 on the general path
+    double algorithmic_bytes(int type) const;   // minimum HBM bytes of one solve
     // advanced interface: the solve's steps on device buffers
     double* inner_buffer();
     void do_copy(const Topo* t, double* data, int sign);

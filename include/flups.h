@@ -165,6 +165,9 @@ int         flups_get_path(FLUPS_Solver* s);
 /* 1: flups_setup times the fused pipeline's candidate schedules on this problem and keeps
    the fastest (remembered per shape for the process; FLUPS_AUTOTUNE=1 turns it on for all) */
 void        flups_set_autotune(FLUPS_Solver* s, int on);
+/* minimum HBM bytes of one solve of this type (each pass reads its input and writes its
+   output once; the multiply reads the field and the Green's function): the roofline numerator */
+double      flups_get_solve_bytes(FLUPS_Solver* s, const FLUPS_SolverType type);
 /* schedule of the fused pipeline: z pass, y passes, x traversal order (-1 on the general path) */
 void        flups_get_schedule(FLUPS_Solver* s, int sched[3]);
 /* nanoseconds of device time of the last flups_solve (HIP events on the solver's stream) */
