@@ -1,4 +1,5 @@
-// Fused pipeline for fully unbounded problems (every direction UNB-UNB).
+// Fused pipeline for problems whose every direction is unbounded (UNB-UNB) or,
+// cell-centred, periodic (PER-PER), transformed in the order x, y, z.
 //
 // The reference runs, per solve, 3 x (pencil switch + batched 1-D FFT), the
 // multiply, then 3 x (FFT + switch back) (src/Solver.cpp:1207-1272). For the
@@ -11,6 +12,7 @@
 //   z_fused   S1 along z: forward FFT, x normfact*G, inverse FFT, keep N2 (in place)
 //   col_fft   S1 along y, inverse, keep N1 -> S0
 //   x_c2r     S0 rows -> user rows (real), keep N0
+// Periodic directions are transformed without padding (M = N, nothing truncated).
 // Stage arrays keep x fastest with the row padded to a multiple of 8 complex
 // values, so a tile of 8 consecutive x columns is one aligned 128-byte segment
 // for FP64 (16 columns for FP32). The pencil switches of the reference become
@@ -600,16 +602,25 @@ void destroy_fast_plan(FastPlan* f);
 
 FastPlan* make_fast_plan(SolverImpl& s) {
     if (s.ndim_ != 3 || s.phys_.axis != 0) return nullptr;
-    for (int ip = 0; ip < 3; ++ip)
-        if (s.fwd_[ip].type != UNBUNB || s.fwd_[ip].dim != ip) return nullptr;
+    // every direction unbounded or (cell-centred) periodic, transformed in the order x, y, z,
+    // with no offsets or post-corrections (node-centred periodic directions need one)
+    for (int ip = 0; ip < 3; ++ip) {
+        const DirPlan* ps[3] = {&s.fwd_[ip], &s.bwd_[ip], &s.gpl_[ip]};
+        const PType t = s.fwd_[ip].type;
+        if (!(t == UNBUNB || (t == PERPER && !s.node_)) || s.fwd_[ip].dim != ip) return nullptr;
+        for (const DirPlan* p : ps)
+            for (int c = 0; c < p->lda; ++c)
+                if (p->post[c] != 0 || p->start_in[c] != 0 || p->start_out[c] != 0 || p->fieldstart != 0 || p->koffset != 0.0)
+                    return nullptr;
+    }
     if (getenv("FLUPS_FORCE_GENERIC")) return nullptr;
     auto* f = new FastPlan;
     for (int d = 0; d < 3; ++d) {
         f->n[d] = s.phys_.nglob[d];
         f->M[d] = s.fwd_[d].n_in[0];
     }
-    // supported lengths: powers of two (x: packed length M0/2 in [64, 2048]; y, z: [64, 2048])
-    if (!pow2_in(f->M[0] / 2, 64, 1024) || !pow2_in(f->M[1], 64, 1024) || !pow2_in(f->M[2], 64, 1024)) {
+    // supported lengths: powers of two (x: packed length M0/2 in [16, 1024]; y, z: [32, 1024])
+    if (!pow2_in(f->M[0] / 2, 16, 1024) || !pow2_in(f->M[1], 32, 1024) || !pow2_in(f->M[2], 32, 1024)) {
         delete f;
         return nullptr;
     }
@@ -766,6 +777,7 @@ void launch_col(const ColArgs& a, int comps, int zsched, hipStream_t st) {
 template <typename R, int TX, bool FWD, bool FUSED>
 void col_dispatch(int M, const ColArgs& a, int comps, int zsched, hipStream_t st) {
     switch (M) {
+        case 32: launch_col<R, 32, TX, FWD, FUSED>(a, comps, zsched, st); break;
         case 64: launch_col<R, 64, TX, FWD, FUSED>(a, comps, zsched, st); break;
         case 128: launch_col<R, 128, TX, FWD, FUSED>(a, comps, zsched, st); break;
         case 256: launch_col<R, 256, TX, FWD, FUSED>(a, comps, zsched, st); break;
@@ -779,6 +791,7 @@ template <typename R, int TX>
 void launch_rot(int M, const ColArgs& a, const RotK& rk, hipStream_t st) {
     dim3 grid(((a.nx + TX - 1) / TX) * (a.gmy / 2 + 1), 1);
     switch (M) {
+        case 32: hipLaunchKernelGGL((z_rot<R, 32, TX>), grid, dim3(TX * 4), 0, st, a, rk); break;
         case 64: hipLaunchKernelGGL((z_rot<R, 64, TX>), grid, dim3(TX * 8), 0, st, a, rk); break;
         case 128: hipLaunchKernelGGL((z_rot<R, 128, TX>), grid, dim3(TX * 16), 0, st, a, rk); break;
         case 256: hipLaunchKernelGGL((z_rot<R, 256, TX>), grid, dim3(TX * 32), 0, st, a, rk); break;
@@ -799,6 +812,8 @@ void row_dispatch(bool fwd, int H, const RowArgs& a, int comps, hipStream_t st) 
         break;                                                                                              \
     }
     switch (H) {
+        FLUPS_ROW(16, 128)
+        FLUPS_ROW(32, 64)
         FLUPS_ROW(64, 32)
         FLUPS_ROW(128, 16)
         FLUPS_ROW(256, 8)

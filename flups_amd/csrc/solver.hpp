@@ -100,7 +100,7 @@ class SolverImpl {
     double* dgreen_ = nullptr;
     double* dstaging_ = nullptr;   // rhs/field copies for host pointers
     double* dinner_ = nullptr;     // inner buffer of the advanced interface
-    bool use_graph_ = true, profiling_ = false, warmed_ = false;
+    bool use_graph_ = false, profiling_ = false, warmed_ = false;   // FLUPS_GRAPH=1: replay solves as graphs
     hipStream_t cap_ = nullptr;    // capture stream of the solve graph
     hipGraphExec_t gexec_ = nullptr;
     double* gfield_ = nullptr;
