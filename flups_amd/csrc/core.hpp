@@ -99,6 +99,9 @@ struct PassDesc {
     long long dst_s[3] = {0, 0, 0};
     int n1 = 1, n2 = 1;     // extents of the two other dims ((d+1)%3, (d+2)%3)
     int ppb = 1;            // pencils per block
+    int fa = 1, fb = 2;     // pencil grid: dim fa varies fastest between pencils (the one with the smaller stride)
+    int na = 1;             // extent along fa
+    int alias = 0;          // 1: the second FFT buffer reuses the pencil buffer
 };
 
 }  // namespace flups

@@ -37,29 +37,34 @@ __device__ __forceinline__ cd expmpi(double t) {
     return {c, -s};
 }
 
-// twiddle omega_M^e with the pass sign (table holds exp(-2 pi i t/M))
-__device__ __forceinline__ cd twid(const double* tw, long long e, int M, int sign) {
-    e %= M;
-    if (e < 0) e += M;
+// twiddle omega_M^e, 0 <= e < M, with the pass sign (table holds exp(-2 pi i t/M))
+__device__ __forceinline__ cd twid(const double* tw, unsigned e, int sign) {
     cd w = {tw[2 * e], tw[2 * e + 1]};
     return sign > 0 ? conj(w) : w;
 }
 
 // one Stockham stage, all outputs of all pencils of the block
 __device__ void stockham_stage(const cd* in, cd* out, int M, int p, int span, const double* tw, int sign, int npen) {
-    const int mp = M / p;
-    const int scale = M / (span * p);
-    for (int i = threadIdx.x; i < npen * M; i += blockDim.x) {
-        const int pen = i / M, o = i - pen * M;
-        const int k = o % span;
-        const int u = (o / span) % p;
-        const int b = o / (span * p);
-        const int j = b * span + k;
+    const unsigned mp = M / p;
+    const unsigned sp = span * p;
+    const unsigned scale = M / sp;
+    const unsigned um = M;
+    for (unsigned i = threadIdx.x; i < (unsigned)(npen * M); i += blockDim.x) {
+        const unsigned pen = i / um, o = i - pen * um;
+        const unsigned b = o / sp, r = o - b * sp;          // r = k + u * span
+        const unsigned u = r / span, k = r - u * span;
+        const unsigned j = b * span + k;
         const cd* a = in + (size_t)pen * M;
-        cd acc = {0, 0};
-        const long long base = (long long)(k + u * span) * scale;
-        for (int q = 0; q < p; ++q) acc = add(acc, mul(a[j + q * mp], twid(tw, q * base, M, sign)));
+        const unsigned base = r * scale;                    // < M
+        cd acc = a[j];
+        unsigned e = 0;
+        for (int q = 1; q < p; ++q) {
+            e += base;
+            if (e >= um) e -= um;
+            acc = add(acc, mul(a[j + q * mp], twid(tw, e, sign)));
+        }
         out[(size_t)pen * M + o] = acc;
+        (void)u;
     }
 }
 
@@ -74,22 +79,31 @@ __global__ void __launch_bounds__(256) generic_pass(PassDesc P) {
     if (npen <= 0) return;
     double* pb = smem;                                       // pencils, ppb * len doubles
     cd* A = reinterpret_cast<cd*>(smem + (size_t)P.ppb * len + ((P.ppb * len) & 1));
-    cd* B = A + (size_t)P.ppb * P.M;
-    const int d = P.d, d1 = (d + 1) % 3, d2 = (d + 2) % 3;
+    cd* B = P.alias ? reinterpret_cast<cd*>(smem) : A + (size_t)P.ppb * P.M;   // free while the FFT runs
+    const int d = P.d;
 
     // ---- gather (pencil switch: shift, zero padding) ----
-    for (int i = threadIdx.x; i < npen * len; i += blockDim.x) {
-        const int pen = i / len, e = i - pen * len;
-        const int g = pen0 + pen;
-        const int j1 = g % P.n1, j2 = g / P.n1;
-        const int k = e / P.nf_in, c = e - k * P.nf_in;
-        const int ks = k - P.gshift;
-        double v = 0.0;
-        if (ks >= 0 && ks < P.src_nd) {
-            long long idx = (long long)ks * P.src_s[d] + (long long)j1 * P.src_s[d1] + (long long)j2 * P.src_s[d2];
-            v = P.src[idx * P.src_nf + c];
+    // Neighbouring threads take neighbouring elements of a pencil when the pencil
+    // is contiguous in the source, neighbouring pencils (adjacent along fa) otherwise.
+    {
+        const bool efast = P.src_s[d] == 1;
+        const int nfi = P.nf_in;
+        const int per = P.pen_in;   // elements per pencil
+        for (int i = threadIdx.x; i < npen * per; i += blockDim.x) {
+            const int pen = efast ? i / per : i % npen;
+            const int k = efast ? i - pen * per : i / npen;
+            const int g = pen0 + pen;
+            const int ja = g % P.na, jb = g / P.na;
+            const int ks = k - P.gshift;
+            double* o = pb + (size_t)pen * len + (size_t)k * nfi;
+            if (ks >= 0 && ks < P.src_nd) {
+                const long long idx = (long long)ks * P.src_s[d] + (long long)ja * P.src_s[P.fa] + (long long)jb * P.src_s[P.fb];
+                const double* sp = P.src + idx * P.src_nf;
+                for (int c = 0; c < nfi; ++c) o[c] = sp[c];
+            } else {
+                for (int c = 0; c < nfi; ++c) o[c] = 0.0;
+            }
         }
-        pb[(size_t)pen * len + e] = v;
     }
     __syncthreads();
 
@@ -163,6 +177,11 @@ __global__ void __launch_bounds__(256) generic_pass(PassDesc P) {
         __syncthreads();
         cd* t = in; in = out; out = t;
     }
+    if (P.alias && in != A) {   // the result sits in the pencil buffer, which the post-map writes: move it
+        for (int i = threadIdx.x; i < npen * M; i += blockDim.x) A[i] = in[i];
+        __syncthreads();
+        in = A;
+    }
     const cd* R = in;
 
     // ---- post-map: sequence -> pencil ----
@@ -230,23 +249,32 @@ __global__ void __launch_bounds__(256) generic_pass(PassDesc P) {
     __syncthreads();
 
     // ---- scatter (truncation on the way back) ----
-    const int lo = P.dst_nd * P.dst_nf;
-    for (int i = threadIdx.x; i < npen * lo; i += blockDim.x) {
-        const int pen = i / lo, e = i - pen * lo;
-        const int g = pen0 + pen;
-        const int j1 = g % P.n1, j2 = g / P.n1;
-        const int k = e / P.dst_nf, c = e - k * P.dst_nf;
-        const int kt = k + P.sshift;
-        if (kt >= P.pen_out) continue;
-        const double v = pb[(size_t)pen * len + (size_t)kt * P.nf_out + c];
-        long long idx = (long long)k * P.dst_s[d] + (long long)j1 * P.dst_s[d1] + (long long)j2 * P.dst_s[d2];
-        P.dst[idx * P.dst_nf + c] = v;
+    {
+        const bool efast = P.dst_s[d] == 1;
+        const int nfo = P.dst_nf;
+        const int per = P.dst_nd;
+        for (int i = threadIdx.x; i < npen * per; i += blockDim.x) {
+            const int pen = efast ? i / per : i % npen;
+            const int k = efast ? i - pen * per : i / npen;
+            const int kt = k + P.sshift;
+            if (kt >= P.pen_out) continue;
+            const int g = pen0 + pen;
+            const int ja = g % P.na, jb = g / P.na;
+            const double* v = pb + (size_t)pen * len + (size_t)kt * P.nf_out;
+            const long long idx = (long long)k * P.dst_s[d] + (long long)ja * P.dst_s[P.fa] + (long long)jb * P.dst_s[P.fb];
+            double* dp = P.dst + idx * nfo;
+            for (int c = 0; c < nfo; ++c) dp[c] = v[c];
+        }
     }
 }
 
+// B aliases the pencil buffer when that is at least as large (c2c and r2c/c2r pencils)
+bool generic_alias(const PassDesc& P) { return (size_t)P.pen_in * P.nf_in >= 2 * (size_t)P.M; }
+
 size_t generic_lds_bytes(const PassDesc& P, int ppb) {
     const size_t len = (size_t)P.pen_in * P.nf_in;
-    return (ppb * len + ((ppb * len) & 1)) * sizeof(double) + 2 * (size_t)ppb * P.M * 2 * sizeof(double);
+    const size_t nbuf = generic_alias(P) ? 1 : 2;
+    return (ppb * len + ((ppb * len) & 1)) * sizeof(double) + nbuf * (size_t)ppb * P.M * 2 * sizeof(double);
 }
 
 void launch_generic(const PassDesc& P, hipStream_t s) {

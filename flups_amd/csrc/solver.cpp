@@ -18,6 +18,7 @@
 namespace flups {
 
 size_t generic_lds_bytes(const PassDesc& P, int ppb);
+bool generic_alias(const PassDesc& P);
 void launch_generic(const PassDesc& P, hipStream_t s);
 FastPlan* make_fast_plan(SolverImpl& s);   // nullptr if the fused pipeline does not apply
 void destroy_fast_plan(FastPlan* f);
@@ -173,6 +174,7 @@ SolverImpl::SolverImpl(const Topo* t, const int* bc, const double h[3], const do
     int lds = 0;
     HIP_CHECK(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
     lds_budget_ = (size_t)lds;
+    if (const char* e = std::getenv("FLUPS_GENERIC_LDS")) generic_lds_cap_ = (size_t)std::atoll(e);
     kdir_ = kernel_dir();
 }
 
@@ -281,11 +283,23 @@ PassDesc SolverImpl::pass_desc(const DirPlan& p, int comp, bool forward, const V
     for (int i = 0; i < 3; ++i) { P.src_s[i] = src.s[i]; P.dst_s[i] = dst.s[i]; }
     P.n1 = tin.n[(d + 1) % 3];
     P.n2 = tin.n[(d + 2) % 3];
+    // pencil grid: the other dim with the smaller source stride varies fastest between pencils,
+    // so that neighbouring pencils of a block are neighbours in memory
+    {
+        const int d1 = (d + 1) % 3, d2 = (d + 2) % 3;
+        const bool swap = src.s[d2] < src.s[d1];
+        P.fa = swap ? d2 : d1;
+        P.fb = swap ? d1 : d2;
+        P.na = swap ? P.n2 : P.n1;
+    }
+    P.alias = generic_alias(P) ? 1 : 0;
     const size_t per_pen = generic_lds_bytes(P, 1);
     FLUPS_REQUIRE(per_pen <= lds_budget_, "pencil of %d elements (transform length %d) does not fit the general path's LDS",
                   P.pen_in, M);
-    int ppb = (int)std::min<size_t>(16, lds_budget_ / std::max<size_t>(per_pen, 1));
-    while (ppb > 1 && generic_lds_bytes(P, ppb) > lds_budget_) --ppb;
+    // several blocks per CU: cap the LDS of one block
+    const size_t cap = std::max(per_pen, std::min(lds_budget_, generic_lds_cap_));
+    int ppb = (int)std::min<size_t>(16, cap / std::max<size_t>(per_pen, 1));
+    while (ppb > 1 && generic_lds_bytes(P, ppb) > cap) --ppb;
     P.ppb = std::max(1, std::min(ppb, P.n1 * P.n2));
     return P;
 }

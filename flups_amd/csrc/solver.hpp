@@ -108,6 +108,7 @@ class SolverImpl {
     int gtype_key_ = -1;
     size_t staging_n_ = 0;
     size_t lds_budget_ = 65536;
+    size_t generic_lds_cap_ = 32768;   // LDS of one general-path block (FLUPS_GENERIC_LDS)
     FastPlan* fast_ = nullptr;
     hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
     double last_ns_ = 0;
