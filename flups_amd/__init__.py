@@ -536,7 +536,7 @@ class Solver:
 
     @property
     def path(self):
-        """0 = general pencil path (FP64), 1 = fused unbounded pipeline in FP64, 2 = fused in FP32"""
+        """0 = general pencil path (FP64), 1 = fused pipeline FP64, 2 = fused FP32, 3 = general + fused last direction"""
         return lib().flups_get_path(self._solver)
 
     @property

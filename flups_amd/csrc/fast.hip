@@ -212,6 +212,7 @@ struct ColArgs {
     long long comp_src, comp_dst;
     int rev;             // 1: blocks walk the batch from its last entry
     int xcd;             // z_fused: 1 = XCD-grouped block numbering
+    int nopair;          // z_fused: 1 = the other dim has no mirror rows (one row per block, G stored per row)
 };
 
 // TX columns per block, M/8 threads per column; block = TX * M/8 threads
@@ -265,9 +266,9 @@ __global__ void __launch_bounds__(TX * M / 8) z_fused(ColArgs a) {
     const int ntx = (a.nx + TX - 1) / TX;
     const int blk = a.xcd ? xcd_group(blockIdx.x, gridDim.x) : blockIdx.x;
     const int tile = blk % ntx;
-    const int y0 = blk / ntx;           // 0 .. M1/2
+    const int y0 = blk / ntx;           // 0 .. M1/2 (every row with nopair)
     const int y1 = a.gmy - y0;          // mirror row; none for y0 = 0 (M1) and y0 = M1/2 (itself)
-    const bool pair = y0 != 0 && y1 != y0;
+    const bool pair = !a.nopair && y0 != 0 && y1 != y0;
     const bool live = tile * TX + c < a.nx;
     // uniform bases (scalar registers) + one 32-bit per-lane byte offset for every access
     char* base = reinterpret_cast<char*>(a.dst) +
@@ -753,7 +754,7 @@ void launch_col(const ColArgs& a, int comps, int zsched, hipStream_t st) {
     if (FUSED) {
         // z schedules: bit 0-1 = SCHED of z_fused, 4 = half-width tiles (TX/2 columns)
         const int tx = (zsched & 8) ? TX / 4 : (zsched & 4) ? TX / 2 : TX;
-        dim3 grid(((a.nx + tx - 1) / tx) * (a.gmy / 2 + 1), comps);
+        dim3 grid(((a.nx + tx - 1) / tx) * (a.nopair ? a.nother : a.gmy / 2 + 1), comps);
         switch (zsched & 15) {
             case 0: hipLaunchKernelGGL((z_fused<R, M, TX, 0>), grid, dim3(TX * M / 8), 0, st, a); break;
             case 1: hipLaunchKernelGGL((z_fused<R, M, TX, 1>), grid, dim3(TX * M / 8), 0, st, a); break;
@@ -947,6 +948,110 @@ void fast_solve(FastPlan* f, double* field, const double* rhs, const View& uin, 
 }
 
 size_t fast_device_bytes(const FastPlan* f) { return f->dev_bytes; }
+
+// ---------------------------------------------------------------------------
+// Last direction of the general path fused with the multiply: when the final
+// forward transform is a c2c along z (unbounded or cell-centred periodic) of a
+// power-of-two length, the forward z FFT, the Green's multiply and the inverse
+// z FFT run as one z_fused launch in place on the stage array before it (the
+// general passes do the other directions). The other dim (y) may be a
+// half-spectrum (r2c) or real-to-real direction: rows are not paired then.
+struct HybridZ {
+    int M = 0, nx = 0, ny = 0, nz = 0, lda = 1, gtx = 8;
+    bool pair = false;
+    void* green = nullptr;   // normfact * Re G, [y'][tile][kz'][gtx]
+    void* tw = nullptr;
+    size_t dev_bytes = 0;
+    double bytes = 0;        // algorithmic bytes of one launch
+};
+
+HybridZ* make_hybrid_z(SolverImpl& s) {
+    if (s.ndim_ != 3 || s.fp32_) return nullptr;
+    if (getenv("FLUPS_FORCE_GENERIC")) return nullptr;
+    const int ip = 2;
+    const DirPlan &f = s.fwd_[ip], &b = s.bwd_[ip];
+    if (f.dim != 2 || !(f.type == UNBUNB || (f.type == PERPER && !s.node_)) || f.r2c) return nullptr;
+    for (const DirPlan* p : {&f, &b})
+        for (int c = 0; c < p->lda; ++c)
+            if (p->post[c] != 0 || p->start_in[c] != 0 || p->start_out[c] != 0 || p->fieldstart != 0 || p->koffset != 0.0)
+                return nullptr;
+    const Shape& in = s.stage_[1];   // field before the z transform
+    const Shape& sp = s.stage_[2];   // spectral field (and Green's function)
+    if (in.nf != 2 || sp.nf != 2 || in.n[0] != sp.n[0] || in.n[1] != sp.n[1]) return nullptr;
+    const int M = f.n_in[0];
+    if (!pow2_in(M, 32, 1024) || sp.n[2] != M) return nullptr;
+    if ((double)(M / 8) * in.n[0] * in.n[1] * 16.0 >= 4294967296.0) return nullptr;
+    auto* h = new HybridZ;
+    h->M = M;
+    h->nx = in.n[0];
+    h->ny = in.n[1];
+    h->nz = in.n[2];
+    h->lda = s.lda_;
+    // y rows pair as (y, M1 - y) when y is a full unbounded / periodic c2c direction (G even in ky)
+    const DirPlan& fy = s.fwd_[1];
+    h->pair = fy.dim == 1 && !fy.r2c && (fy.type == UNBUNB || (fy.type == PERPER && !s.node_)) && fy.koffset == 0.0 &&
+              fy.n_in[0] == h->ny;
+    h->gtx = 4;   // half-width x tiles
+    const int ntx = (h->nx + h->gtx - 1) / h->gtx;
+    const int gy = h->pair ? h->ny / 2 + 1 : h->ny, gz = M / 2 + 1;
+    const size_t gn = (size_t)gy * ntx * gz * h->gtx;
+    try {
+        HIP_CHECK(hipMalloc(&h->green, gn * sizeof(double)));
+        h->dev_bytes += gn * sizeof(double);
+        hipLaunchKernelGGL(compress_green, dim3(4096), dim3(256), 0, s.stream_, s.dgreen_, (double*)h->green, (float*)nullptr,
+                           h->nx, h->ny, M, gy, gz, h->gtx, ntx, s.normfact_);
+        HIP_CHECK(hipGetLastError());
+        const auto t = twiddle_table(M);
+        HIP_CHECK(hipMalloc(&h->tw, t.size() * sizeof(double)));
+        HIP_CHECK(hipMemcpy(h->tw, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice));
+        h->dev_bytes += t.size() * sizeof(double);
+        HIP_CHECK(hipStreamSynchronize(s.stream_));
+    } catch (...) {
+        (void)hipFree(h->green);
+        (void)hipFree(h->tw);
+        delete h;
+        throw;
+    }
+    const double cnt = (double)h->nx * h->ny * h->nz * 16.0;
+    h->bytes = h->lda * 2.0 * cnt + (double)gy * gz * h->nx * 8.0;
+    return h;
+}
+
+void destroy_hybrid_z(HybridZ* h) {
+    if (!h) return;
+    (void)hipFree(h->green);
+    (void)hipFree(h->tw);
+    delete h;
+}
+size_t hybrid_device_bytes(const HybridZ* h) { return h->dev_bytes; }
+double hybrid_bytes(const HybridZ* h) { return h->bytes; }
+
+// in place on the stage array before the z transform: (nx, ny, nz) complex, lda components
+void hybrid_z(const HybridZ* h, double* data, hipStream_t st) {
+    ColArgs z{};
+    z.src = data;
+    z.dst = data;
+    z.green = h->green;
+    z.tw = h->tw;
+    z.nx = h->nx;
+    z.ldx = h->nx;
+    z.s_col = (long long)h->nx * h->ny;
+    z.s_col_dst = z.s_col;
+    z.s_other_src = h->nx;
+    z.s_other_dst = h->nx;
+    z.nother = h->ny;
+    z.nin = h->nz;
+    z.nout = h->nz;
+    z.gy = h->pair ? h->ny / 2 + 1 : h->ny;
+    z.gmy = h->pair ? h->ny : 0;
+    z.gmz = h->M;
+    z.comp_src = (long long)h->nx * h->ny * h->nz;
+    z.comp_dst = z.comp_src;
+    z.xcd = 1;
+    z.nopair = h->pair ? 0 : 1;
+    // half-width tiles (schedule bit 2), XCD-grouped; M = 1024 is already half width
+    col_dispatch<double, 8, true, true>(h->M, z, h->lda, 4 | 16, st);
+}
 
 double fast_bytes(const FastPlan* f) { return f->bytes[0] + f->bytes[1] + f->bytes[2] + f->bytes[3] + f->bytes[4]; }
 

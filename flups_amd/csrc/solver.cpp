@@ -28,6 +28,11 @@ void fast_collect(FastPlan* f);
 void fast_profile(FastPlan* f, bool on);
 int fast_profile_get(FastPlan* f, int i, const char** name, double* ms, long* calls, double* bytes);
 size_t fast_device_bytes(const FastPlan* f);
+HybridZ* make_hybrid_z(SolverImpl& s);
+void destroy_hybrid_z(HybridZ* h);
+size_t hybrid_device_bytes(const HybridZ* h);
+double hybrid_bytes(const HybridZ* h);
+void hybrid_z(const HybridZ* h, double* data, hipStream_t st);
 void fast_schedule(const FastPlan* f, int out[3]);
 double fast_bytes(const FastPlan* f);
 void fast_autotune(FastPlan* f, double* out, const double* in, const View& uin, const View& uout, const double* dgreen,
@@ -180,6 +185,7 @@ SolverImpl::SolverImpl(const Topo* t, const int* bc, const double h[3], const do
 
 SolverImpl::~SolverImpl() {
     if (fast_) destroy_fast_plan(fast_);
+    if (hybrid_) destroy_hybrid_z(hybrid_);
     for (auto& kv : tw_) (void)hipFree(kv.second);
     for (double* p : dstage_) if (p) (void)hipFree(p);
     if (dgreen_) (void)hipFree(dgreen_);
@@ -512,6 +518,7 @@ void SolverImpl::setup() {
         HIP_CHECK(hipFree(dgreen_));
         dgreen_ = nullptr;
     }
+    if (!fast_) hybrid_ = make_hybrid_z(*this);
     if (!fast_) {
         for (int ip = 0; ip < ndim_; ++ip) {
             const size_t n = std::max(hat_[ip].count(), stage_[ip].count());
@@ -572,6 +579,25 @@ void SolverImpl::backward_passes(const double* spec, const View& uout, bool diff
 }
 
 void SolverImpl::solve_generic(double* field, const double* rhs, int type) {
+    if (hybrid_ && type == 0) {
+        // general passes for the first directions, the last one fused with the multiply
+        const View uin = user_view(const_cast<double*>(rhs)), uout = user_view(field);
+        for (int ip = 0; ip < ndim_ - 1; ++ip) {
+            const DirPlan& p = fwd_[ip];
+            const View src = (ip == 0) ? uin : dense_view(dstage_[ip - 1], stage_[ip - 1]);
+            const View dst = dense_view(dstage_[ip], stage_[ip]);
+            for (int c = 0; c < lda_; ++c) run_pass(pass_desc(p, c, true, src, dst, hat_[ip], stage_[ip], p.fieldstart, 0));
+        }
+        hybrid_z(hybrid_, dstage_[ndim_ - 2], stream_);
+        for (int ip = ndim_ - 2; ip >= 0; --ip) {
+            const DirPlan& p = bwd_[ip];
+            const View src = dense_view(dstage_[ip], stage_[ip]);
+            const View dst = (ip == 0) ? uout : dense_view(dstage_[ip - 1], stage_[ip - 1]);
+            for (int c = 0; c < lda_; ++c)
+                run_pass(pass_desc(p, c, false, src, dst, stage_[ip], hat_[ip], 0, fwd_[ip].fieldstart));
+        }
+        return;
+    }
     forward_passes(user_view(const_cast<double*>(rhs)), dstage_[ndim_ - 1]);
     multiply(dstage_[ndim_ - 1], type);
     backward_passes(dstage_[ndim_ - 1], user_view(field), type != 0);
@@ -711,6 +737,7 @@ size_t SolverImpl::device_bytes() const {
     if (dgreen_) b += sizeof(double) * gpost_[ndim_ - 1].count();
     b += sizeof(double) * staging_n_;
     if (fast_) b += fast_device_bytes(fast_);
+    if (hybrid_) b += hybrid_device_bytes(hybrid_);
     return b;
 }
 
@@ -807,10 +834,12 @@ double SolverImpl::algorithmic_bytes(int type) const {
     const double w = sizeof(double);
     const double user = (double)phys_.nglob[0] * phys_.nglob[1] * phys_.nglob[2] * lda_;
     double b = 0;
-    for (int ip = 0; ip < ndim_; ++ip) {
+    const int nfull = (hybrid_ && type == 0) ? ndim_ - 1 : ndim_;
+    for (int ip = 0; ip < nfull; ++ip) {
         const double in = ip == 0 ? user : (double)stage_[ip - 1].count();
         b += 2.0 * w * (in + (double)stage_[ip].count());   // forward pass and its backward mirror
     }
+    if (hybrid_ && type == 0) return b + hybrid_bytes(hybrid_);
     const Shape& sp = stage_[ndim_ - 1];
     b += w * (2.0 * (double)sp.count() + (double)gpost_[ndim_ - 1].count());
     (void)type;

@@ -24,6 +24,7 @@ struct Topo {
 };
 
 struct FastPlan;   // fused unbounded pipeline, fast.hip
+struct HybridZ;    // general path with its last direction fused with the multiply, fast.hip
 
 class SolverImpl {
    public:
@@ -59,7 +60,8 @@ class SolverImpl {
     Topo inner_topo_spectral() const;
     size_t alloc_size() const;
     void spectral_info(double kf[3], double ko[3], double ss[3]) const;
-    int path() const { return fast_ ? (fp32_ ? 2 : 1) : 0; }   // 0 general FP64, 1 fused FP64, 2 fused FP32
+    // 0 general FP64, 1 fused FP64, 2 fused FP32, 3 general with the last direction fused with the multiply
+    int path() const { return fast_ ? (fp32_ ? 2 : 1) : hybrid_ ? 3 : 0; }
     double last_solve_ns() const { return last_ns_; }
     int ndim() const { return ndim_; }
     // per-kernel timing of the fused pipeline (HIP events on the solve stream)
@@ -110,12 +112,14 @@ class SolverImpl {
     size_t lds_budget_ = 65536;
     size_t generic_lds_cap_ = 32768;   // LDS of one general-path block (FLUPS_GENERIC_LDS)
     FastPlan* fast_ = nullptr;
+    HybridZ* hybrid_ = nullptr;
     hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
     double last_ns_ = 0;
     std::string kdir_;
     friend struct FastPlan;
     friend FastPlan* make_fast_plan(SolverImpl&);
     friend void build_fast_plan(SolverImpl&, FastPlan*, size_t, int);
+    friend HybridZ* make_hybrid_z(SolverImpl&);
 };
 
 std::string kernel_dir();

@@ -160,7 +160,8 @@ void        flups_set_abort_on_error(int on);  /* 1: abort() on a failed check, 
    before flups_setup. Applies to the fused unbounded pipeline; the general path
    stays FP64 (flups_get_path tells which one runs). */
 void        flups_set_precision(FLUPS_Solver* s, int fp32);
-/* 0 = general pencil path (FP64), 1 = fused unbounded pipeline in FP64, 2 = fused in FP32 */
+/* 0 = general pencil path (FP64), 1 = fused pipeline in FP64, 2 = fused in FP32,
+   3 = general passes with the last direction fused with the multiply (FP64) */
 int         flups_get_path(FLUPS_Solver* s);
 /* 1: flups_setup times the fused pipeline's candidate schedules on this problem and keeps
    the fastest (remembered per shape for the process; FLUPS_AUTOTUNE=1 turns it on for all) */

@@ -400,3 +400,41 @@ def test_fused_periodic_and_small_vs_oracle(case):
     f_ora = orc.solve(list(nglob), bc, lda, h, L, 1, kernel, rhs)
     e2, ei = rel_err(f_gpu, f_ora)
     assert e2 <= TOL and ei <= TOL, (e2, ei)
+
+
+HYBRID_CASES = [
+    # (nglob, bc, center, kernel, lda): the last direction (z) c2c of a power-of-two length
+    ((64, 64, 64), [[0, 1], [3, 3], [4, 4]], 1, 8, 1),      # x EVEN-ODD, y PER, z UNB (mixed config)
+    ((48, 40, 32), [[4, 4], [0, 4], [4, 4]], 1, 2, 1),      # y semi-unbounded first, x r2c, z fused
+    ((33, 17, 33), [[1, 4], [4, 4], [4, 4]], 0, 4, 3),      # node, x semi-unbounded, lda 3
+    ((32, 24, 64), [[1, 1], [3, 3], [3, 3]], 1, 0, 1),      # x ODD-ODD, y PER (r2c), z PER
+]
+
+
+@pytest.mark.parametrize("case", HYBRID_CASES, ids=["mixed64", "semiunb-y", "node-lda3", "sym-per-per"])
+def test_hybrid_last_direction_vs_general_and_oracle(case, monkeypatch):
+    """general passes + the last direction fused with the multiply, against the all-general path and the oracle"""
+    nglob, bc, center, kernel, lda = case
+    rng = np.random.default_rng(sum(nglob))
+    n0, n1, n2 = nglob
+    rhs = rng.normal(size=(lda, n2, n1, n0))
+    rhs -= rhs.mean(axis=(1, 2, 3), keepdims=True)
+    h = [1.0 / (max(nglob) - (1 - center))] * 3
+    L = [h[0] * (n - (1 - center)) for n in nglob]
+    outs = []
+    for force in (False, True):
+        if force:
+            monkeypatch.setenv("FLUPS_FORCE_GENERIC", "1")
+        topo = fl.Topology(0, lda, list(nglob), [1, 1, 1], False)
+        s = fl.Solver(topo, bc, h, L, center_type=[center] * 3)
+        s.set_green_type(kernel)
+        s.setup()
+        assert s.path == (0 if force else 3)
+        f = np.zeros_like(rhs)
+        s.solve(f, rhs)
+        outs.append(f)
+    e2, ei = rel_err(outs[0], outs[1])
+    assert e2 <= TOL and ei <= TOL, (e2, ei)
+    f_ora = orc.solve(list(nglob), bc, lda, h, L, center, kernel, rhs)
+    e2, ei = rel_err(outs[0], f_ora)
+    assert e2 <= TOL and ei <= TOL, (e2, ei)
