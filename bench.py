@@ -325,13 +325,30 @@ def main():
         for _ in range(args.fp32_steps):
             s32.solve(f32, rhs, fl.SolverType.STD)
         el32 = time.perf_counter() - t0
+        L.flups_profile_enable(s32._solver, 1)
+        for _ in range(args.profile_steps):
+            s32.solve(f32, rhs, fl.SolverType.STD)
+        k32 = []
+        i = 0
+        while True:
+            name = ctypes.create_string_buffer(64)
+            ms, calls, by = ctypes.c_double(), ctypes.c_long(), ctypes.c_double()
+            cnt = L.flups_profile_get(s32._solver, i, name, 64, ctypes.byref(ms), ctypes.byref(calls), ctypes.byref(by))
+            if cnt == 0 or i >= cnt:
+                break
+            avg = ms.value / max(calls.value, 1) * 1e-3
+            k32.append({"kernel": name.value.decode(), "avg_us": round(avg * 1e6, 3), "bytes": by.value,
+                        "gbs": round(by.value / avg / 1e9, 1) if avg > 0 else 0.0})
+            i += 1
+        L.flups_profile_enable(s32._solver, 0)
         a64, a32 = field.numpy(), f32.numpy()
         e2 = float(np.sqrt(np.sum((a32 - a64) ** 2) / np.sum(a64 ** 2)))
         ei = float(np.max(np.abs(a32 - a64)) / np.max(np.abs(a64)))
         v32 = args.fp32_steps / el32
         fp32 = {"value": round(v32, 3), "unit": "solves/s", "ms_per_step": round(el32 / args.fp32_steps * 1e3, 4),
                 "speedup_vs_fp64": round(v32 / solves_per_s, 3), "rel_l2_vs_fp64": e2, "rel_linf_vs_fp64": ei,
-                "path": "fused-fp32" if s32.path == 2 else "general", "steps": args.fp32_steps}
+                "path": "fused-fp32" if s32.path == 2 else "general", "steps": args.fp32_steps,
+                "schedule": list(s32.schedule), "kernels": k32}
         del s32, f32
 
     served = serve_mix(fl, args.serve_requests, args.serve_rate, args.serve_workers) if args.serve_requests > 0 else None
@@ -358,6 +375,7 @@ def main():
         "latency_ms": {"p50": round(float(np.percentile(lat_ms, 50)), 4), "p99": round(float(np.percentile(lat_ms, 99)), 4)},
         "kernels": [{k: (round(v, 3) if isinstance(v, float) else v) for k, v in kk.items()} for kk in kernels],
         "setup_s": round(setup_s, 3),
+        "schedule": list(s.schedule),
         "fp32": fp32,
         "served_mix": served,
         "other_configs": others,

@@ -497,7 +497,8 @@ void SolverImpl::setup() {
     HIP_CHECK(hipEventCreate(&ev1_));
     compute_green();
     fast_ = make_fast_plan(*this);
-    if (fast_ && (autotune_ || std::getenv("FLUPS_AUTOTUNE"))) {
+    const char* at_env = std::getenv("FLUPS_AUTOTUNE");
+    if (fast_ && (autotune_ || (at_env && std::atoi(at_env) != 0))) {
         // time the candidate schedules on scratch arrays of the caller's layout
         const size_t n = phys_.memsize();
         double *din = nullptr, *dout = nullptr;
