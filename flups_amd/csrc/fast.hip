@@ -72,15 +72,23 @@ __device__ __forceinline__ void dft4(C2<R>& a0, C2<R>& a1, C2<R>& a2, C2<R>& a3)
     a1 = cadd(d0, d1);
     a3 = csub(d0, d1);
 }
-template <typename R, bool FWD>
+// HIN: v[4..7] are zero (zero-padded input) and are not read
+template <typename R, bool FWD, bool HIN = false>
 __device__ __forceinline__ void dft8(C2<R>* v) {
     const R h = (R)0.70710678118654752440;
-    C2<R> a0 = v[0], a1 = v[1], a2 = v[2], a3 = v[3], a4 = v[4], a5 = v[5], a6 = v[6], a7 = v[7];
     // radix-2 split into even/odd halves, then 4-point DFTs
-    C2<R> e0 = cadd(a0, a4), o0 = csub(a0, a4);
-    C2<R> e1 = cadd(a1, a5), o1 = csub(a1, a5);
-    C2<R> e2 = cadd(a2, a6), o2 = csub(a2, a6);
-    C2<R> e3 = cadd(a3, a7), o3 = csub(a3, a7);
+    C2<R> e0, e1, e2, e3, o0, o1, o2, o3;
+    if (HIN) {
+        e0 = o0 = v[0];
+        e1 = o1 = v[1];
+        e2 = o2 = v[2];
+        e3 = o3 = v[3];
+    } else {
+        e0 = cadd(v[0], v[4]), o0 = csub(v[0], v[4]);
+        e1 = cadd(v[1], v[5]), o1 = csub(v[1], v[5]);
+        e2 = cadd(v[2], v[6]), o2 = csub(v[2], v[6]);
+        e3 = cadd(v[3], v[7]), o3 = csub(v[3], v[7]);
+    }
     // twiddles w8^k on the odd half
     o1 = FWD ? C2<R>{h * (o1.x + o1.y), h * (o1.y - o1.x)} : C2<R>{h * (o1.x - o1.y), h * (o1.y + o1.x)};
     o2 = rot90<R, FWD>(o2);
@@ -107,7 +115,9 @@ __device__ __forceinline__ int lslot(int e) { return ES > 0 ? e * ES : e + (e >>
 // Full column FFT of length M held by a team of T = M/8 threads (j = 0..T-1).
 // On entry v[i] = x[j + i*T]; on exit v[i] = X[j + i*T]. lds points at the
 // column (element e at lds[lslot<ES>(e)]); the whole block must call this together.
-template <typename R, int M, bool FWD, int ES>
+// HIN: the upper half of the input is zero (v[4..7] not read). Outputs the caller
+// never uses (the upper half of a truncated inverse) are removed by the compiler.
+template <typename R, int M, bool FWD, int ES, bool HIN = false>
 __device__ __forceinline__ void column_fft(C2<R>* v, C2<R>* lds, int j, const C2<R>* tw) {
     constexpr int T = M / 8;
     // table entries w, w^2, w^4 of the next radix-8 stage, loaded one stage ahead
@@ -147,7 +157,8 @@ __device__ __forceinline__ void column_fft(C2<R>* v, C2<R>* lds, int j, const C2
 #pragma unroll
                 for (int q = 1; q < 8; ++q) v[q] = cmul(v[q], tw8[q]);
             }
-            dft8<R, FWD>(v);
+            if (HIN && span == 1) dft8<R, FWD, true>(v);
+            else dft8<R, FWD>(v);
             if (span * 8 < M) {
                 __syncthreads();   // everyone has read this stage's inputs
                 const int base = (j / span) * span * 8 + k;
@@ -215,8 +226,11 @@ struct ColArgs {
     int nopair;          // z_fused: 1 = the other dim has no mirror rows (one row per block, G stored per row)
 };
 
-// TX columns per block, M/8 threads per column; block = TX * M/8 threads
-template <typename R, int M, int TX, bool FWD>
+// TX columns per block, M/8 threads per column; block = TX * M/8 threads.
+// HALF: zero-padded input of M/2 rows (forward) or output truncated to M/2 rows
+// (inverse); nin / nout are then not consulted. Lanes past nx load a valid
+// column (the last one) and store nothing.
+template <typename R, int M, int TX, bool FWD, bool HALF>
 __global__ void __launch_bounds__(TX * M / 8) col_fft(ColArgs a) {
     __shared__ C2<R> lds[M * TX];
     constexpr int T = M / 8;
@@ -229,21 +243,24 @@ __global__ void __launch_bounds__(TX * M / 8) col_fft(ColArgs a) {
     const int other = blk / ntx;
     const int x = tile * TX + c;
     const bool live = x < a.nx;
-    const C2<R>* src = reinterpret_cast<const C2<R>*>(a.src) + blockIdx.y * a.comp_src + other * a.s_other_src + x;
+    const C2<R>* src = reinterpret_cast<const C2<R>*>(a.src) + blockIdx.y * a.comp_src + other * a.s_other_src + min(x, a.nx - 1);
     C2<R>* dst = reinterpret_cast<C2<R>*>(a.dst) + blockIdx.y * a.comp_dst + other * a.s_other_dst + x;
     const C2<R>* tw = reinterpret_cast<const C2<R>*>(a.tw);
+    constexpr bool HIN = HALF && FWD, HOUT = HALF && !FWD;
     C2<R> v[8];
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
         const int row = j + i * T;
-        v[i] = (live && row < a.nin) ? src[row * a.s_col] : C2<R>{0, 0};
+        if (HIN) v[i] = i < 4 ? src[row * a.s_col] : C2<R>{0, 0};
+        else if (HOUT) v[i] = src[row * a.s_col];
+        else v[i] = row < a.nin ? src[row * a.s_col] : C2<R>{0, 0};
     }
-    column_fft<R, M, FWD, TX>(v, lds + c, j, tw);
+    column_fft<R, M, FWD, TX, HIN>(v, lds + c, j, tw);
     if (live) {
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
             const int row = j + i * T;
-            if (row < a.nout) dst[row * a.s_col_dst] = v[i];
+            if (HOUT ? i < 4 : (HIN || row < a.nout)) dst[row * a.s_col_dst] = v[i];
         }
     }
 }
@@ -255,7 +272,9 @@ __global__ void __launch_bounds__(TX * M / 8) col_fft(ColArgs a) {
 // once. PREFETCH issues the loads of both column sets before the first FFT
 // (more bytes in flight, more registers); otherwise the second set is loaded
 // after the first one is stored.
-template <typename R, int M, int TX, int SCHED>
+// HALF: nin = nout = M/2 (unbounded direction): only four rows per thread are
+// loaded and stored, the zero half is pruned from the first forward stage.
+template <typename R, int M, int TX, int SCHED, bool HALF>
 __global__ void __launch_bounds__(TX * M / 8) z_fused(ColArgs a) {
     // SCHED 0: G loaded with the first set, second set loaded after the first is stored;
     //       1: both sets prefetched up front; 2: G loaded just before each multiply.
@@ -270,13 +289,16 @@ __global__ void __launch_bounds__(TX * M / 8) z_fused(ColArgs a) {
     const int y1 = a.gmy - y0;          // mirror row; none for y0 = 0 (M1) and y0 = M1/2 (itself)
     const bool pair = !a.nopair && y0 != 0 && y1 != y0;
     const bool live = tile * TX + c < a.nx;
-    // uniform bases (scalar registers) + one 32-bit per-lane byte offset for every access
+    // uniform bases (scalar registers) + one 32-bit per-lane byte offset for every access;
+    // lanes past nx load the last valid column and store nothing
     char* base = reinterpret_cast<char*>(a.dst) +
                  (size_t)sizeof(C2<R>) * ((long long)blockIdx.y * a.comp_src + (long long)tile * TX);
     char* b0 = base + (size_t)sizeof(C2<R>) * ((long long)y0 * a.s_other_src);
     char* b1 = base + (size_t)sizeof(C2<R>) * ((long long)y1 * a.s_other_src);
     const size_t rstride = (size_t)sizeof(C2<R>) * T * a.s_col;                      // i -> i + 1
     const unsigned voff = (unsigned)(sizeof(C2<R>) * ((size_t)j * a.s_col + c));     // < 4 GiB: j < M/8
+    const unsigned loff = (unsigned)(sizeof(C2<R>) * ((size_t)j * a.s_col + min(c, a.nx - 1 - tile * TX)));
+    constexpr int NL = HALF ? 4 : 8;   // rows loaded / stored per thread
     const char* g = reinterpret_cast<const char*>(a.green) + sizeof(R) * ((long long)y0 * ntx + tile) * (M / 2 + 1) * TX;
     const C2<R>* tw = reinterpret_cast<const C2<R>*>(a.tw);
     R gv[8];
@@ -288,21 +310,19 @@ __global__ void __launch_bounds__(TX * M / 8) z_fused(ColArgs a) {
             gv[i] = *reinterpret_cast<const R*>(g + (unsigned)(sizeof(R) * (min(kz, M - kz) * TX + c)));
         }
     };
-    if (SCHED != 2) load_g();
-#pragma unroll
-    for (int i = 0; i < 8; ++i) {
-        const int row = j + i * T;
-        v[i] = (live && row < a.nin) ? *reinterpret_cast<const C2<R>*>(b0 + i * rstride + voff) : C2<R>{0, 0};
-    }
-    if (SCHED == 1) {
+    auto load = [&](C2<R>* u, const char* b) {
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
             const int row = j + i * T;
-            w[i] = (pair && live && row < a.nin) ? *reinterpret_cast<const C2<R>*>(b1 + i * rstride + voff) : C2<R>{0, 0};
+            if (HALF) u[i] = i < 4 ? *reinterpret_cast<const C2<R>*>(b + i * rstride + loff) : C2<R>{0, 0};
+            else u[i] = row < a.nin ? *reinterpret_cast<const C2<R>*>(b + i * rstride + loff) : C2<R>{0, 0};
         }
-    }
+    };
+    if (SCHED != 2) load_g();
+    load(v, b0);
+    if (SCHED == 1 && pair) load(w, b1);
     for (int s = 0; s < 2; ++s) {
-        column_fft<R, M, true, TX>(v, lds + c, j, tw);
+        column_fft<R, M, true, TX, HALF>(v, lds + c, j, tw);
         if (SCHED == 2) load_g();
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
@@ -313,18 +333,18 @@ __global__ void __launch_bounds__(TX * M / 8) z_fused(ColArgs a) {
         char* b = s == 0 ? b0 : b1;
         if (live) {
 #pragma unroll
-            for (int i = 0; i < 8; ++i) {
+            for (int i = 0; i < NL; ++i) {
                 const int row = j + i * T;
-                if (row < a.nout) *reinterpret_cast<C2<R>*>(b + i * rstride + voff) = v[i];
+                if (HALF || row < a.nout) *reinterpret_cast<C2<R>*>(b + i * rstride + voff) = v[i];
             }
         }
         if (!pair) break;   // uniform over the block
         if (s == 0) {
+            if (SCHED == 1) {
 #pragma unroll
-            for (int i = 0; i < 8; ++i) {
-                const int row = j + i * T;
-                v[i] = SCHED == 1 ? w[i]
-                                  : ((live && row < a.nin) ? *reinterpret_cast<const C2<R>*>(b1 + i * rstride + voff) : C2<R>{0, 0});
+                for (int i = 0; i < 8; ++i) v[i] = w[i];
+            } else {
+                load(v, b1);
             }
         }
     }
@@ -584,7 +604,8 @@ struct FastPlan {
     int gtx = 0;             // x tile width of the compressed Green's layout
     RotK rot{};              // wave-number data of the rotational pass
     void* s0 = nullptr;     // (ldx, N1, N2) complex, FP64 or FP32
-    void* s1 = nullptr;     // (ldx, M1, N2) complex
+    void* s1 = nullptr;     // (ldx, M1, N2) complex, z stride zs
+    long long zs = 0;       // z stride of S1 in complex elements: ldx * M1 + zpad
     void* green = nullptr;  // normfact * G, [M1/2+1][x tiles][M2/2+1][tile width of the z kernel]
     void* tw[3] = {nullptr, nullptr, nullptr};   // M1, M2 tables and the M0/2 table
     void* twx = nullptr;                          // M0 table (x post-twiddles)
@@ -637,8 +658,18 @@ FastPlan* make_fast_plan(SolverImpl& s) {
     const int align = f->fp32 ? 16 : 8;
     f->ldx = (f->nx + align - 1) / align * align;
     const size_t es = f->fp32 ? 2 * sizeof(float) : 2 * sizeof(double);
+    // S1 rows along z are ldx * M1 elements apart, a large multiple of a power of two: every
+    // row of a z column then falls on the same few HBM channels and the z pass runs at about
+    // 70% of the bandwidth of the y passes. A pad of 1 KiB after each z plane spreads them.
+    long long zpad = 1024 / (long long)es;
+    if (const char* e = getenv("FLUPS_ZPAD")) zpad = atoll(e);
+    if (zpad < 0 || zpad % align) {
+        delete f;
+        fail("FLUPS_ZPAD: a non-negative multiple of %d (complex elements)", align);
+    }
+    f->zs = (long long)f->ldx * f->M[1] + zpad;
     // per-lane byte offsets of the column kernels are 32-bit: (M/8) rows of the largest stride
-    if ((double)(f->M[2] / 8) * f->ldx * f->M[1] * es >= 4294967296.0) {
+    if ((double)(f->M[2] / 8) * f->zs * es >= 4294967296.0) {
         delete f;
         return nullptr;
     }
@@ -682,7 +713,7 @@ static void layout_green(FastPlan* f, const double* dgreen, double normfact, hip
 
 void build_fast_plan(SolverImpl& s, FastPlan* f, size_t es, int align) {
     const size_t n0 = (size_t)f->ldx * f->n[1] * f->n[2] * f->lda;
-    const size_t n1 = (size_t)f->ldx * f->M[1] * f->n[2] * f->lda;
+    const size_t n1 = (size_t)f->zs * f->n[2] * f->lda;
     HIP_CHECK(hipMalloc(&f->s0, n0 * es));
     HIP_CHECK(hipMalloc(&f->s1, n1 * es));
     f->dev_bytes += (n0 + n1) * es;
@@ -749,30 +780,43 @@ void destroy_fast_plan(FastPlan* f) {
 
 namespace {
 
-template <typename R, int M, int TX, bool FWD, bool FUSED>
-void launch_col(const ColArgs& a, int comps, int zsched, hipStream_t st) {
+template <typename R, int M, int TX, bool FWD, bool FUSED, bool HALF>
+void launch_col_h(const ColArgs& a, int comps, int zsched, hipStream_t st) {
     if (FUSED) {
         // z schedules: bit 0-1 = SCHED of z_fused, 4 = half-width tiles (TX/2 columns)
         const int tx = (zsched & 8) ? TX / 4 : (zsched & 4) ? TX / 2 : TX;
         dim3 grid(((a.nx + tx - 1) / tx) * (a.nopair ? a.nother : a.gmy / 2 + 1), comps);
         switch (zsched & 15) {
-            case 0: hipLaunchKernelGGL((z_fused<R, M, TX, 0>), grid, dim3(TX * M / 8), 0, st, a); break;
-            case 1: hipLaunchKernelGGL((z_fused<R, M, TX, 1>), grid, dim3(TX * M / 8), 0, st, a); break;
-            case 2: hipLaunchKernelGGL((z_fused<R, M, TX, 2>), grid, dim3(TX * M / 8), 0, st, a); break;
-            case 4: hipLaunchKernelGGL((z_fused<R, M, TX / 2, 0>), grid, dim3(TX / 2 * M / 8), 0, st, a); break;
-            case 5: hipLaunchKernelGGL((z_fused<R, M, TX / 2, 1>), grid, dim3(TX / 2 * M / 8), 0, st, a); break;
-            case 6: hipLaunchKernelGGL((z_fused<R, M, TX / 2, 2>), grid, dim3(TX / 2 * M / 8), 0, st, a); break;
-            case 8: hipLaunchKernelGGL((z_fused<R, M, (TX >= 8 ? TX / 4 : 1), 0>), grid, dim3((TX >= 8 ? TX / 4 : 1) * M / 8), 0, st, a); break;
+            case 0: hipLaunchKernelGGL((z_fused<R, M, TX, 0, HALF>), grid, dim3(TX * M / 8), 0, st, a); break;
+            case 1: hipLaunchKernelGGL((z_fused<R, M, TX, 1, HALF>), grid, dim3(TX * M / 8), 0, st, a); break;
+            case 2: hipLaunchKernelGGL((z_fused<R, M, TX, 2, HALF>), grid, dim3(TX * M / 8), 0, st, a); break;
+            case 4: hipLaunchKernelGGL((z_fused<R, M, TX / 2, 0, HALF>), grid, dim3(TX / 2 * M / 8), 0, st, a); break;
+            case 5: hipLaunchKernelGGL((z_fused<R, M, TX / 2, 1, HALF>), grid, dim3(TX / 2 * M / 8), 0, st, a); break;
+            case 6: hipLaunchKernelGGL((z_fused<R, M, TX / 2, 2, HALF>), grid, dim3(TX / 2 * M / 8), 0, st, a); break;
+            case 8:
+                hipLaunchKernelGGL((z_fused<R, M, (TX >= 8 ? TX / 4 : 1), 0, HALF>), grid, dim3((TX >= 8 ? TX / 4 : 1) * M / 8), 0, st,
+                                   a);
+                break;
             default: fail("unknown z schedule %d", zsched);
         }
     } else if (zsched & 4) {   // y schedule: half-width tiles
         dim3 grid(((a.nx + TX / 2 - 1) / (TX / 2)) * a.nother, comps);
-        hipLaunchKernelGGL((col_fft<R, M, TX / 2, FWD>), grid, dim3(TX / 2 * M / 8), 0, st, a);
+        hipLaunchKernelGGL((col_fft<R, M, TX / 2, FWD, HALF>), grid, dim3(TX / 2 * M / 8), 0, st, a);
     } else {
         dim3 grid(((a.nx + TX - 1) / TX) * a.nother, comps);
-        hipLaunchKernelGGL((col_fft<R, M, TX, FWD>), grid, dim3(TX * M / 8), 0, st, a);
+        hipLaunchKernelGGL((col_fft<R, M, TX, FWD, HALF>), grid, dim3(TX * M / 8), 0, st, a);
     }
     HIP_CHECK(hipGetLastError());
+}
+
+// HALF kernels when the column is zero-padded to twice its length (unbounded
+// directions): fused pass nin = nout = M/2, forward nin = M/2, inverse nout = M/2
+template <typename R, int M, int TX, bool FWD, bool FUSED>
+void launch_col(const ColArgs& a, int comps, int zsched, hipStream_t st) {
+    const int h = M / 2;
+    const bool half = FUSED ? (a.nin == h && a.nout == h) : FWD ? (a.nin == h && a.nout == M) : (a.nin == M && a.nout == h);
+    if (half) launch_col_h<R, M, TX, FWD, FUSED, true>(a, comps, zsched, st);
+    else launch_col_h<R, M, TX, FWD, FUSED, false>(a, comps, zsched, st);
 }
 
 template <typename R, int TX, bool FWD, bool FUSED>
@@ -832,7 +876,8 @@ void run_fast(FastPlan* f, const double* rhs, double* field, const View& uin, co
     const int M0 = f->M[0], M1 = f->M[1], M2 = f->M[2];
     const int H = M0 / 2;
     const long long ldx = f->ldx;
-    const long long c0 = ldx * N1 * N2, c1 = ldx * M1 * N2;   // component strides (complex)
+    const long long zs = f->zs;                                // z stride of S1
+    const long long c0 = ldx * N1 * N2, c1 = zs * N2;          // component strides (complex)
     constexpr int TX = sizeof(R) == 8 ? 8 : 16;
     // 1. x r2c: user (real) -> S0
     RowArgs r{};
@@ -866,7 +911,7 @@ void run_fast(FastPlan* f, const double* rhs, double* field, const View& uin, co
     a.s_col = ldx;
     a.s_col_dst = ldx;
     a.s_other_src = ldx * N1;
-    a.s_other_dst = ldx * M1;
+    a.s_other_dst = zs;
     a.nother = N2;
     a.nin = N1;
     a.nout = M1;
@@ -882,8 +927,8 @@ void run_fast(FastPlan* f, const double* rhs, double* field, const View& uin, co
     z.tw = f->tw[2];
     z.nx = f->nx;
     z.ldx = f->ldx;
-    z.s_col = ldx * M1;
-    z.s_col_dst = ldx * M1;
+    z.s_col = zs;
+    z.s_col_dst = zs;
     z.s_other_src = ldx;
     z.s_other_dst = ldx;
     z.nother = M1;
@@ -910,7 +955,7 @@ void run_fast(FastPlan* f, const double* rhs, double* field, const View& uin, co
     b.rev = (f->order >> 1) & 1;
     b.src = f->s1;
     b.dst = f->s0;
-    b.s_other_src = ldx * M1;
+    b.s_other_src = zs;
     b.s_other_dst = ldx * N1;
     b.nin = M1;
     b.nout = N1;
