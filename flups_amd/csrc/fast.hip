@@ -224,6 +224,8 @@ struct ColArgs {
     int rev;             // 1: blocks walk the batch from its last entry
     int xcd;             // z_fused: 1 = XCD-grouped block numbering
     int nopair;          // z_fused: 1 = the other dim has no mirror rows (one row per block, G stored per row)
+    int x0;              // z_fused / z_rot: spectral x index of column 0 (slab offset, a multiple of TX)
+    int gntx;            // z_fused / z_rot: x tiles of one ky' row of the compressed G
 };
 
 // TX columns per block, M/8 threads per column; block = TX * M/8 threads.
@@ -299,7 +301,8 @@ __global__ void __launch_bounds__(TX * M / 8) z_fused(ColArgs a) {
     const unsigned voff = (unsigned)(sizeof(C2<R>) * ((size_t)j * a.s_col + c));     // < 4 GiB: j < M/8
     const unsigned loff = (unsigned)(sizeof(C2<R>) * ((size_t)j * a.s_col + min(c, a.nx - 1 - tile * TX)));
     constexpr int NL = HALF ? 4 : 8;   // rows loaded / stored per thread
-    const char* g = reinterpret_cast<const char*>(a.green) + sizeof(R) * ((long long)y0 * ntx + tile) * (M / 2 + 1) * TX;
+    const char* g = reinterpret_cast<const char*>(a.green) +
+                    sizeof(R) * ((long long)y0 * a.gntx + a.x0 / TX + tile) * (M / 2 + 1) * TX;
     const C2<R>* tw = reinterpret_cast<const C2<R>*>(a.tw);
     R gv[8];
     C2<R> v[8], w[8];
@@ -397,7 +400,8 @@ __global__ void __launch_bounds__(TX * M / 8) z_rot(ColArgs a, RotK rk) {
     const size_t cstride = (size_t)sizeof(C2<R>) * a.comp_src;
     const size_t rstride = (size_t)sizeof(C2<R>) * T * a.s_col;
     const unsigned voff = (unsigned)(sizeof(C2<R>) * ((size_t)j * a.s_col + c));
-    const char* g = reinterpret_cast<const char*>(a.green) + sizeof(R) * ((long long)y0 * ntx + tile) * (M / 2 + 1) * TX;
+    const char* g = reinterpret_cast<const char*>(a.green) +
+                    sizeof(R) * ((long long)y0 * a.gntx + a.x0 / TX + tile) * (M / 2 + 1) * TX;
     const C2<R>* tw = reinterpret_cast<const C2<R>*>(a.tw);
     R gv[8];
 #pragma unroll
@@ -405,7 +409,7 @@ __global__ void __launch_bounds__(TX * M / 8) z_rot(ColArgs a, RotK rk) {
         const int kz = j + i * T;
         gv[i] = *reinterpret_cast<const R*>(g + (unsigned)(sizeof(R) * (min(kz, M - kz) * TX + c)));
     }
-    const R kx = dsym<R>(rk, 0, x);
+    const R kx = dsym<R>(rk, 0, a.x0 + x);
     for (int s = 0; s < 2; ++s) {
         const int y = s == 0 ? y0 : y1;
         char* b = base + (size_t)sizeof(C2<R>) * ((long long)y * a.s_other_src);
@@ -604,15 +608,17 @@ struct FastPlan {
     int gtx = 0;             // x tile width of the compressed Green's layout
     RotK rot{};              // wave-number data of the rotational pass
     void* s0 = nullptr;     // (ldx, N1, N2) complex, FP64 or FP32
-    void* s1 = nullptr;     // (ldx, M1, N2) complex, z stride zs
-    long long zs = 0;       // z stride of S1 in complex elements: ldx * M1 + zpad
+    void* s1 = nullptr;     // one x slab of the y-transformed field: (ldxs, M1, N2) complex, z stride zs
+    long long ldxs = 0;     // S1 row length: the widest slab
+    long long zs = 0;       // z stride of S1 in complex elements: ldxs * M1 + zpad
+    std::vector<int> slab;  // first x column of each slab, then nx
     void* green = nullptr;  // normfact * G, [M1/2+1][x tiles][M2/2+1][tile width of the z kernel]
     void* tw[3] = {nullptr, nullptr, nullptr};   // M1, M2 tables and the M0/2 table
     void* twx = nullptr;                          // M0 table (x post-twiddles)
     std::vector<void*> owned;
     // per-kernel timing (HIP events on the solve stream)
     bool profile = false;
-    hipEvent_t ev[6] = {};
+    std::vector<hipEvent_t> ev;   // x_r2c, then y_fwd / z / y_inv of every slab, then x_c2r
     double ms[5] = {0, 0, 0, 0, 0};
     long calls = 0;
     double bytes[5] = {0, 0, 0, 0, 0};   // algorithmic HBM bytes per launch
@@ -658,16 +664,37 @@ FastPlan* make_fast_plan(SolverImpl& s) {
     const int align = f->fp32 ? 16 : 8;
     f->ldx = (f->nx + align - 1) / align * align;
     const size_t es = f->fp32 ? 2 * sizeof(float) : 2 * sizeof(double);
-    // S1 rows along z are ldx * M1 elements apart, a large multiple of a power of two: every
-    // row of a z column then falls on the same few HBM channels and the z pass runs at about
-    // 70% of the bandwidth of the y passes. A pad of 1 KiB after each z plane spreads them.
+    // x slabs: with FLUPS_SLAB_MB = S > 0, y_fwd, z and y_inv run slab after slab on one S1
+    // buffer of about S MiB instead of the whole y-transformed spectrum (S1 of a 256^3 FP64
+    // solve: 553 MB), for callers short of device memory. The slabs split the tiles of
+    // align columns evenly. Default: one slab. Narrow slabs cost time: each launch is only
+    // a few rounds of blocks deep (64 MiB slabs: about 10% slower at 256^3).
+    {
+        const int tiles = f->ldx / align;
+        const double col_bytes = (double)f->M[1] * f->n[2] * es * f->lda;   // one x column of S1
+        double target = 0;
+        if (const char* e = getenv("FLUPS_SLAB_MB")) target = atof(e) * (1 << 20);
+        const int per = target > 0 ? std::max(1, (int)(target / (col_bytes * align))) : tiles;
+        const int nslab = (tiles + per - 1) / per;
+        int maxw = 0;
+        for (int k = 0; k < nslab; ++k) {
+            const int t0 = (int)((long long)k * tiles / nslab), t1 = (int)((long long)(k + 1) * tiles / nslab);
+            f->slab.push_back(t0 * align);
+            maxw = std::max(maxw, (t1 - t0) * align);
+        }
+        f->slab.push_back(f->nx);
+        f->ldxs = maxw;
+    }
+    // A pad of 1 KiB after each z plane of S1: with ldxs * M1 a large power of two, the rows
+    // of a z column would otherwise fall on the same few HBM channels (a plain copy along z
+    // then runs at 3.8 instead of 5 TB/s).
     long long zpad = 1024 / (long long)es;
     if (const char* e = getenv("FLUPS_ZPAD")) zpad = atoll(e);
     if (zpad < 0 || zpad % align) {
         delete f;
         fail("FLUPS_ZPAD: a non-negative multiple of %d (complex elements)", align);
     }
-    f->zs = (long long)f->ldx * f->M[1] + zpad;
+    f->zs = f->ldxs * f->M[1] + zpad;
     // per-lane byte offsets of the column kernels are 32-bit: (M/8) rows of the largest stride
     if ((double)(f->M[2] / 8) * f->zs * es >= 4294967296.0) {
         delete f;
@@ -748,7 +775,8 @@ void build_fast_plan(SolverImpl& s, FastPlan* f, size_t es, int align) {
     f->tw[1] = upload(f->M[1]);
     f->tw[2] = upload(f->M[2]);
     f->twx = upload(f->M[0]);
-    for (int i = 0; i < 6; ++i) HIP_CHECK(hipEventCreate(&f->ev[i]));
+    f->ev.assign(3 + 3 * (f->slab.size() - 1), nullptr);
+    for (auto& e : f->ev) HIP_CHECK(hipEventCreate(&e));
     {
         // minimum traffic of each pass: valid elements read + written (plus G for the fused pass)
         const double us = sizeof(double);                          // user arrays are always FP64
@@ -773,8 +801,8 @@ void destroy_fast_plan(FastPlan* f) {
     (void)hipFree(f->s1);
     (void)hipFree(f->green);
     for (void* p : f->owned) (void)hipFree(p);
-    for (int i = 0; i < 6; ++i)
-        if (f->ev[i]) (void)hipEventDestroy(f->ev[i]);
+    for (hipEvent_t e : f->ev)
+        if (e) (void)hipEventDestroy(e);
     delete f;
 }
 
@@ -899,70 +927,79 @@ void run_fast(FastPlan* f, const double* rhs, double* field, const View& uin, co
     if (f->profile) HIP_CHECK(hipEventRecord(f->ev[0], st));
     row_dispatch<R>(true, H, r, f->lda, st);
     if (f->profile) HIP_CHECK(hipEventRecord(f->ev[1], st));
-    // 2. y forward: S0 (ldx, N1, N2) -> S1 (ldx, M1, N2)
-    ColArgs a{};
-    a.rev = f->order & 1;
-    a.xcd = (f->ysched >> 4) & 1;
-    a.src = f->s0;
-    a.dst = f->s1;
-    a.tw = f->tw[1];
-    a.nx = f->nx;
-    a.ldx = f->ldx;
-    a.s_col = ldx;
-    a.s_col_dst = ldx;
-    a.s_other_src = ldx * N1;
-    a.s_other_dst = zs;
-    a.nother = N2;
-    a.nin = N1;
-    a.nout = M1;
-    a.comp_src = c0;
-    a.comp_dst = c1;
-    col_dispatch<R, TX, true, false>(M1, a, f->lda, f->ysched, st);
-    if (f->profile) HIP_CHECK(hipEventRecord(f->ev[2], st));
-    // 3. z fused: S1 in place, forward, * normfact G, inverse, keep N2
-    ColArgs z{};
-    z.src = f->s1;
-    z.dst = f->s1;
-    z.green = f->green;
-    z.tw = f->tw[2];
-    z.nx = f->nx;
-    z.ldx = f->ldx;
-    z.s_col = zs;
-    z.s_col_dst = zs;
-    z.s_other_src = ldx;
-    z.s_other_dst = ldx;
-    z.nother = M1;
-    z.nin = N2;
-    z.nout = N2;
-    z.gy = M1 / 2 + 1;
-    z.gmy = M1;
-    z.gmz = M2;
-    z.comp_src = c1;
-    z.comp_dst = c1;
-    z.xcd = (f->zsched >> 4) & 1;
-    if (type == 1) {
-        // rotational pass: all three components per block, tile width of the Green's layout
-        FLUPS_REQUIRE(f->lda == 3, "the rotational solve needs lda = 3");
-        if (f->gtx == TX) launch_rot<R, TX>(M2, z, f->rot, st);
-        else if (f->gtx == TX / 2) launch_rot<R, TX / 2>(M2, z, f->rot, st);
-        else launch_rot<R, (TX >= 8 ? TX / 4 : 1)>(M2, z, f->rot, st);
-    } else {
-        col_dispatch<R, TX, true, true>(M2, z, f->lda, f->zsched, st);
+    const size_t es = sizeof(C2<R>);
+    const int nslab = (int)f->slab.size() - 1;
+    for (int k = 0; k < nslab; ++k) {
+        const int x0 = f->slab[k], w = f->slab[k + 1] - x0;
+        // 2. y forward: columns [x0, x0 + w) of S0 (ldx, N1, N2) -> S1 (ldxs, M1, N2)
+        ColArgs a{};
+        a.rev = f->order & 1;
+        a.xcd = (f->ysched >> 4) & 1;
+        a.src = reinterpret_cast<const char*>(f->s0) + es * x0;
+        a.dst = f->s1;
+        a.tw = f->tw[1];
+        a.nx = w;
+        a.ldx = (int)f->ldxs;
+        a.s_col = ldx;
+        a.s_col_dst = f->ldxs;
+        a.s_other_src = ldx * N1;
+        a.s_other_dst = zs;
+        a.nother = N2;
+        a.nin = N1;
+        a.nout = M1;
+        a.comp_src = c0;
+        a.comp_dst = c1;
+        col_dispatch<R, TX, true, false>(M1, a, f->lda, f->ysched, st);
+        if (f->profile) HIP_CHECK(hipEventRecord(f->ev[2 + 3 * k], st));
+        // 3. z fused: S1 in place, forward, * normfact G, inverse, keep N2
+        ColArgs z{};
+        z.src = f->s1;
+        z.dst = f->s1;
+        z.green = f->green;
+        z.tw = f->tw[2];
+        z.nx = w;
+        z.ldx = (int)f->ldxs;
+        z.s_col = zs;
+        z.s_col_dst = zs;
+        z.s_other_src = f->ldxs;
+        z.s_other_dst = f->ldxs;
+        z.nother = M1;
+        z.nin = N2;
+        z.nout = N2;
+        z.gy = M1 / 2 + 1;
+        z.gmy = M1;
+        z.gmz = M2;
+        z.comp_src = c1;
+        z.comp_dst = c1;
+        z.xcd = (f->zsched >> 4) & 1;
+        z.x0 = x0;
+        z.gntx = (f->nx + f->gtx - 1) / f->gtx;
+        if (type == 1) {
+            // rotational pass: all three components per block, tile width of the Green's layout
+            FLUPS_REQUIRE(f->lda == 3, "the rotational solve needs lda = 3");
+            if (f->gtx == TX) launch_rot<R, TX>(M2, z, f->rot, st);
+            else if (f->gtx == TX / 2) launch_rot<R, TX / 2>(M2, z, f->rot, st);
+            else launch_rot<R, (TX >= 8 ? TX / 4 : 1)>(M2, z, f->rot, st);
+        } else {
+            col_dispatch<R, TX, true, true>(M2, z, f->lda, f->zsched, st);
+        }
+        if (f->profile) HIP_CHECK(hipEventRecord(f->ev[3 + 3 * k], st));
+        // 4. y inverse: S1 -> columns [x0, x0 + w) of S0, keep N1
+        ColArgs b = a;
+        b.rev = (f->order >> 1) & 1;
+        b.src = f->s1;
+        b.dst = reinterpret_cast<char*>(f->s0) + es * x0;
+        b.s_col = f->ldxs;
+        b.s_col_dst = ldx;
+        b.s_other_src = zs;
+        b.s_other_dst = ldx * N1;
+        b.nin = M1;
+        b.nout = N1;
+        b.comp_src = c1;
+        b.comp_dst = c0;
+        col_dispatch<R, TX, false, false>(M1, b, f->lda, f->ysched, st);
+        if (f->profile) HIP_CHECK(hipEventRecord(f->ev[4 + 3 * k], st));
     }
-    if (f->profile) HIP_CHECK(hipEventRecord(f->ev[3], st));
-    // 4. y inverse: S1 -> S0, keep N1
-    ColArgs b = a;
-    b.rev = (f->order >> 1) & 1;
-    b.src = f->s1;
-    b.dst = f->s0;
-    b.s_other_src = zs;
-    b.s_other_dst = ldx * N1;
-    b.nin = M1;
-    b.nout = N1;
-    b.comp_src = c1;
-    b.comp_dst = c0;
-    col_dispatch<R, TX, false, false>(M1, b, f->lda, f->ysched, st);
-    if (f->profile) HIP_CHECK(hipEventRecord(f->ev[4], st));
     // 5. x c2r: S0 -> user (real), keep N0
     RowArgs q{};
     q.rev = (f->order >> 2) & 1;
@@ -981,8 +1018,7 @@ void run_fast(FastPlan* f, const double* rhs, double* field, const View& uin, co
     q.comp_src = c0;
     q.comp_dst = uout.sc;
     row_dispatch<R>(false, H, q, f->lda, st);
-    if (f->profile) HIP_CHECK(hipEventRecord(f->ev[5], st));
-    (void)M2;
+    if (f->profile) HIP_CHECK(hipEventRecord(f->ev.back(), st));
 }
 
 }  // namespace
@@ -1094,6 +1130,8 @@ void hybrid_z(const HybridZ* h, double* data, hipStream_t st) {
     z.comp_dst = z.comp_src;
     z.xcd = 1;
     z.nopair = h->pair ? 0 : 1;
+    z.x0 = 0;
+    z.gntx = (h->nx + h->gtx - 1) / h->gtx;
     // half-width tiles (schedule bit 2), XCD-grouped; M = 1024 is already half width
     col_dispatch<double, 8, true, true>(h->M, z, h->lda, 4 | 16, st);
 }
@@ -1175,12 +1213,15 @@ void fast_autotune(FastPlan* f, double* out, const double* in, const View& uin, 
 }
 
 // after the stream has completed: accumulate the per-kernel times of the last solve
+// kernel kinds of one solve: x_r2c, y_fwd / z / y_inv of each slab (summed), x_c2r
 void fast_collect(FastPlan* f) {
     if (!f->profile) return;
-    for (int i = 0; i < 5; ++i) {
+    const int ne = (int)f->ev.size();
+    for (int i = 0; i + 1 < ne; ++i) {
         float ms = 0;
         HIP_CHECK(hipEventElapsedTime(&ms, f->ev[i], f->ev[i + 1]));
-        f->ms[i] += ms;
+        const int kind = i == 0 ? 0 : i == ne - 2 ? 4 : 1 + (i - 1) % 3;
+        f->ms[kind] += ms;
     }
     ++f->calls;
 }

@@ -438,3 +438,49 @@ def test_hybrid_last_direction_vs_general_and_oracle(case, monkeypatch):
     f_ora = orc.solve(list(nglob), bc, lda, h, L, center, kernel, rhs)
     e2, ei = rel_err(outs[0], f_ora)
     assert e2 <= TOL and ei <= TOL, (e2, ei)
+
+
+SLAB_CASES = [
+    # (nglob, bc, lda, kernel, typ, odiff, fp32)
+    ((128, 64, 64), [[4, 4]] * 3, 1, 2, 0, 0, False),
+    ((128, 64, 32), [[3, 3], [4, 4], [4, 4]], 1, 8, 0, 0, False),
+    ((128, 64, 64), [[4, 4]] * 3, 3, 4, 1, 1, False),
+    ((128, 64, 64), [[4, 4]] * 3, 1, 2, 0, 0, True),
+]
+
+
+@pytest.mark.parametrize("case", SLAB_CASES, ids=["unb", "perx-lgf4", "rot-lda3", "unb-fp32"])
+def test_fused_x_slabs_bitwise_and_oracle(case, monkeypatch):
+    """the fused pipeline split into x slabs (FLUPS_SLAB_MB: the y-transformed spectrum in
+    slabs of about that many MiB, run one after the other) computes the same arithmetic per
+    column as one slab: bitwise equal results, and the oracle's to 1e-12 (FP32: 1e-5)"""
+    nglob, bc, lda, kernel, typ, odiff, fp32 = case
+    rng = np.random.default_rng(21)
+    n0, n1, n2 = nglob
+    rhs = rng.normal(size=(lda, n2, n1, n0))
+    if any(b == [3, 3] for b in bc):
+        rhs -= rhs.mean(axis=3, keepdims=True)
+    h = [1.0 / max(nglob)] * 3
+    L = [h[0] * n for n in nglob]
+
+    def run(slab_mb):
+        monkeypatch.setenv("FLUPS_SLAB_MB", str(slab_mb))
+        topo = fl.Topology(0, lda, list(nglob), [1, 1, 1], False)
+        s = fl.Solver(topo, bc, h, L, order_diff=odiff)
+        s.set_green_type(kernel)
+        if fp32:
+            s.set_precision(True)
+        s.setup()
+        assert s.path == (2 if fp32 else 1)
+        f = np.zeros_like(rhs)
+        s.solve(f, rhs, typ)
+        return f
+
+    one = run(0)
+    # one x column of S1: M1 * n2 complex values per component; 0.3 MiB slabs give several slabs
+    for mb in (0.3, 4):
+        assert np.array_equal(run(mb), one), mb
+    ora = orc.solve(list(nglob), bc, lda, h, L, 1, kernel, rhs, typ=typ, odiff=odiff)
+    e2, ei = rel_err(one, ora)
+    tol = 1e-5 if fp32 else TOL
+    assert e2 <= tol and ei <= tol, (e2, ei)
