@@ -29,6 +29,7 @@
 
 #include <array>
 #include <cmath>
+#include <cstdint>
 #include <cstdio>
 #include <map>
 #include <mutex>
@@ -122,6 +123,8 @@ __device__ __forceinline__ void column_fft(C2<R>* v, C2<R>* lds, int j, const C2
     constexpr int T = M / 8;
     // table entries w, w^2, w^4 of the next radix-8 stage, loaded one stage ahead
     // so that their latency hides behind the current stage and its LDS exchange
+    // (loading all seven w^q instead of forming w^3, w^5, w^6, w^7 by products
+    // competes with the data loads: FP32 z pass 184 -> 236 us at 256^3)
     C2<R> r1 = {1, 0}, r2 = {1, 0}, r4 = {1, 0};
     int span = 1;
 #pragma unroll
@@ -474,7 +477,10 @@ struct RowArgs {
     int rev;             // 1: blocks walk the rows from the last one (reads what the previous pass wrote last)
 };
 
-template <typename R, int H, int ROWS>
+// MODE 0: any nvalid, scalar loads; 1: nvalid = H (zero-padded row: the upper half of the
+// packed input is zero and pruned from the first stage), 16-byte loads; 2: nvalid = 2H,
+// 16-byte loads. Modes 1 and 2 need 16-byte aligned user rows.
+template <typename R, int H, int ROWS, int MODE>
 __global__ void __launch_bounds__(ROWS * H / 8) x_r2c(RowArgs a) {
     constexpr int HP = H + H / 8;     // padded row buffer
     __shared__ C2<R> lds[HP * ROWS];
@@ -493,12 +499,20 @@ __global__ void __launch_bounds__(ROWS * H / 8) x_r2c(RowArgs a) {
     for (int i = 0; i < 8; ++i) {
         const int n = j + i * T;
         const int e0 = 2 * n, e1 = 2 * n + 1;
-        R re = 0, im = 0;
-        if (live && e0 < a.nvalid) re = (R)src[e0];
-        if (live && e1 < a.nvalid) im = (R)src[e1];
-        v[i] = {re, im};
+        if (MODE == 0) {
+            R re = 0, im = 0;
+            if (live && e0 < a.nvalid) re = (R)src[e0];
+            if (live && e1 < a.nvalid) im = (R)src[e1];
+            v[i] = {re, im};
+        } else if (MODE == 2 || i < 4) {
+            // dead rows (past nrows) read row 0 of the plane and store nothing
+            const double2 p = *reinterpret_cast<const double2*>(src + e0);
+            v[i] = {(R)p.x, (R)p.y};
+        } else {
+            v[i] = {0, 0};
+        }
     }
-    column_fft<R, H, true, 0>(v, col, j, reinterpret_cast<const C2<R>*>(a.tw));
+    column_fft<R, H, true, 0, MODE == 1>(v, col, j, reinterpret_cast<const C2<R>*>(a.tw));
     __syncthreads();
 #pragma unroll
     for (int i = 0; i < 8; ++i) col[lslot<0>(j + i * T)] = v[i];
@@ -523,7 +537,9 @@ __global__ void __launch_bounds__(ROWS * H / 8) x_r2c(RowArgs a) {
 
 // x rows inverse: Z[k] = (X[k] + conj X[H-k]) + i exp(+2 pi i k/M) (X[k] - conj X[H-k]), z = IFFT_H(Z),
 // x[2n] = Re z[n], x[2n+1] = Im z[n]; the imaginary parts of X[0] and X[H] are ignored.
-template <typename R, int H, int ROWS>
+// MODE as for x_r2c: 1 stores the first H reals of the row (the upper half of the
+// inverse is not computed), 2 all 2H; both with 16-byte stores to aligned user rows.
+template <typename R, int H, int ROWS, int MODE>
 __global__ void __launch_bounds__(ROWS * H / 8) x_c2r(RowArgs a) {
     constexpr int HP = H + H / 8;
     __shared__ C2<R> lds[HP * ROWS];
@@ -557,8 +573,12 @@ __global__ void __launch_bounds__(ROWS * H / 8) x_c2r(RowArgs a) {
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
             const int n = j + i * T;
-            if (2 * n < a.nvalid) dst[2 * n] = (double)v[i].x;
-            if (2 * n + 1 < a.nvalid) dst[2 * n + 1] = (double)v[i].y;
+            if (MODE == 0) {
+                if (2 * n < a.nvalid) dst[2 * n] = (double)v[i].x;
+                if (2 * n + 1 < a.nvalid) dst[2 * n + 1] = (double)v[i].y;
+            } else if (MODE == 2 || i < 4) {
+                *reinterpret_cast<double2*>(dst + 2 * n) = double2{(double)v[i].x, (double)v[i].y};
+            }
         }
     }
 }
@@ -875,13 +895,13 @@ void launch_rot(int M, const ColArgs& a, const RotK& rk, hipStream_t st) {
     HIP_CHECK(hipGetLastError());
 }
 
-template <typename R>
-void row_dispatch(bool fwd, int H, const RowArgs& a, int comps, hipStream_t st) {
+template <typename R, int MODE>
+void row_launch(bool fwd, int H, const RowArgs& a, int comps, hipStream_t st) {
 #define FLUPS_ROW(HH, RR)                                                                                   \
     case HH: {                                                                                              \
         dim3 grid((a.nrows + RR - 1) / RR, comps);                                                          \
-        if (fwd) hipLaunchKernelGGL((x_r2c<R, HH, RR>), grid, dim3(RR * HH / 8), 0, st, a);                 \
-        else hipLaunchKernelGGL((x_c2r<R, HH, RR>), grid, dim3(RR * HH / 8), 0, st, a);                     \
+        if (fwd) hipLaunchKernelGGL((x_r2c<R, HH, RR, MODE>), grid, dim3(RR * HH / 8), 0, st, a);           \
+        else hipLaunchKernelGGL((x_c2r<R, HH, RR, MODE>), grid, dim3(RR * HH / 8), 0, st, a);               \
         break;                                                                                              \
     }
     switch (H) {
@@ -896,6 +916,19 @@ void row_dispatch(bool fwd, int H, const RowArgs& a, int comps, hipStream_t st) 
     }
 #undef FLUPS_ROW
     HIP_CHECK(hipGetLastError());
+}
+
+// user rows of 16-byte aligned pairs and a zero-padded (nvalid = H) or full (2H) row:
+// the vectorised kernels; anything else the scalar one
+template <typename R>
+void row_dispatch(bool fwd, int H, const RowArgs& a, int comps, hipStream_t st) {
+    const void* user = fwd ? a.src : a.dst;
+    const long long s_row = fwd ? a.s_row_src : a.s_row_dst, s_z = fwd ? a.s_z_src : a.s_z_dst;
+    const long long sc = fwd ? a.comp_src : a.comp_dst;
+    const bool aligned = ((uintptr_t)user % 16) == 0 && s_row % 2 == 0 && s_z % 2 == 0 && (comps == 1 || sc % 2 == 0);
+    if (aligned && a.nvalid == H) row_launch<R, 1>(fwd, H, a, comps, st);
+    else if (aligned && a.nvalid == 2 * H) row_launch<R, 2>(fwd, H, a, comps, st);
+    else row_launch<R, 0>(fwd, H, a, comps, st);
 }
 
 template <typename R>
