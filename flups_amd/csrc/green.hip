@@ -258,12 +258,28 @@ __global__ void fill_2unb(double* g, FillGeom geo, GreenArgs A) {
             } else {
                 v = -bessel_k(0, fabs(k) * r) / (2.0 * kPi);
             }
+        } else if (A.type == HEJ_0) {
+            // src/green_kernels.hpp:416-428, the same expression at r = 0 and on k = 0, with
+            // Ji0c(rho) from the table of the integer squared distances s = |i|^2 (isotropic
+            // grid: rho = pi sqrt(s) < 30 for s <= 91); elsewhere zero_ (0.5 |r|), which no
+            // spectral rewrite replaces for HEJ_0
+            if (r0 || k0) {
+                const double rho = r / A.length;
+                const int s = (A.hfact[0] != 0.0 ? is[0] * is[0] : 0) + (A.hfact[1] != 0.0 ? is[1] * is[1] : 0) +
+                              (A.hfact[2] != 0.0 ? is[2] * is[2] : 0);
+                v = (rho < 30.0 && s < A.ntable) ? (A.table[s] + log(2.0 * A.length) - kEuler) / (2.0 * kPi)
+                                                 : log(r) / (2.0 * kPi);
+            } else {
+                v = 0.5 * fabs(r);
+            }
         } else {
             if (r0) {
                 v = -(0.5 * kEuler - log(sqrt(2.0) * A.length) + hej2_q(A.type, 0.0)) / (2.0 * kPi);
             } else if (k0) {
                 const double rho2 = (r / A.length) * (r / A.length);
                 v = (log(r) - hej2_q(A.type, rho2) * exp(-0.5 * rho2) + 0.5 * expint_e1(0.5 * rho2)) / (2.0 * kPi);
+            } else {
+                v = 0.5 * fabs(r);   // zero_ (src/green_kernels.hpp:430-434); the spectral rewrite replaces these planes
             }
         }
         if (e == 0 && A.type == CHAT_2) v = 0.25 / (2.0 * kPi) * (kPi - 6.0 + 2.0 * log(0.5 * kPi * req * req));
@@ -389,6 +405,57 @@ __device__ double lgf1d_eval(const GreenArgs& A, int order, int n, double c) {
     return g;
 }
 
+// 1-D lattice Green's functions of the Mehrstellen stencils along the unbounded direction
+// (src/lgf_1unb2spe.hpp:500-611): transverse wave numbers k1 >= k2 (times h), the root r of
+// a1 r^2 + a0 r + a1 = 0 inside the unit circle, r = lambda - sqrt(lambda - 1) sqrt(lambda + 1)
+// with lambda = -a0 / (2 a1) (src/lgf_1unb2spe.hpp:80-98); "left": a1 G(n-1) + a0 G(n) +
+// a1 G(n+1) = delta_n, "full": the right-hand stencil b applied to it.
+__device__ double r_from_lambda(double lambda) {
+    if (isinf(lambda)) return 0.0;
+    if (fabs(lambda) > 1.0e2) {   // 1/(2l) + 1/(8l^3) + 1/(16l^5) + 5/(128l^7) + 7/(256l^9)
+        const double y = 1.0 / lambda;
+        const double e[10] = {0, 1. / 2., 0, 1. / 8., 0, 1. / 16., 0, 5. / 128., 0, 7. / 256.};
+        double out = e[9];
+        for (int i = 8; i >= 0; --i) out = y * out + e[i];
+        return out;
+    }
+    if (lambda >= 1.0) return lambda - sqrt(lambda - 1.0) * sqrt(lambda + 1.0);
+    if (lambda > -1.0) return lambda;   // both roots on the unit circle: the real part
+    return lambda + sqrt(1.0 - lambda) * sqrt(-1.0 - lambda);
+}
+__device__ double mehr_1d(int type, int n, double k1, double k2) {
+    const bool six = type == MEHR_6L || type == MEHR_6F, full = type == MEHR_4F || type == MEHR_6F;
+    const int an = abs(n);
+    if (k1 == 0 && k2 == 0) {
+        double v = -0.5 * an;
+        if (full && !six) v -= (n == 0) * (1. / 12.);
+        if (full && six) v -= (an == 1) * (-1. / 240.) + (n == 0) * (11. / 120.);
+        return v;
+    }
+    const double s0 = sin(k1 / 2), s1 = sin(k2 / 2);
+    const double y0 = s0 * s0, y1 = s1 * s1;
+    const double a0 = six ? -8. / 5. * y0 * y1 + 8. / 3. * (y0 + y1) + 2.0 : -8. / 3. * (y0 * y1 - y0 - y1) + 2.0;
+    const double a1 = six ? -8. / 15. * y0 * y1 + 2. / 3. * (y0 + y1) - 1.0 : 2. / 3. * (y0 + y1) - 1.0;
+    const double lambda = -a0 / (2. * a1);
+    const double r = r_from_lambda(lambda);
+    if (!full) {
+        if (!isfinite(lambda)) return six ? -(double)(n == 0) / a0 : (double)(n == 0) / a0;
+        return (r / a1) * pow(r, (double)an) / (r * r - 1.0);
+    }
+    if (!six) {
+        const double b0 = 5. / 6. - 1. / 3. * (y0 + y1), b1 = 1. / 12.;
+        if (!isfinite(lambda)) return (n == 0) * (b0 / a0) + (n == 1) * (b1 / a0);
+        if (n == 0) return (r / a1) * (b0 + 2. * b1 * r) / (r * r - 1.);
+        return (b1 + b0 * r + b1 * r * r) * pow(r, (double)an) / (a1 * (r * r - 1));
+    }
+    const double b0 = -1. / 15. * (y0 * y0 + y1 * y1) + 8. / 45. * y0 * y1 - 11. / 45. * (y0 + y1) + 97. / 120.;
+    const double b1 = -2. / 45. * (y0 + y1) + 1. / 10.;
+    const double b2 = -1. / 240.;
+    const double pr = b0 * r * r + b1 * (r + r * r * r) + b2 * (1.0 + r * r * r * r);
+    const double base = pr * pow(r, (double)(an - 1)) / (a1 * (r * r - 1.0));
+    return base + (an == 1) * (b2 / a1) + (n == 0) * (b1 * a1 - b2 * a0) / (a1 * a1);
+}
+
 __global__ void fill_1unb(double* g, FillGeom geo, GreenArgs A) {
     const long long tot = (long long)geo.n[0] * geo.n[1] * geo.n[2];
     const double sk = A.kfact[0] + A.kfact[1] + A.kfact[2];
@@ -409,6 +476,12 @@ __global__ void fill_1unb(double* g, FillGeom geo, GreenArgs A) {
             for (int dd = 0; dd < 3; ++dd) c += lgf_symbol(ord, kk[dd] * h);
             const int nn = (int)round(fabs(r) / h);
             v = -h * lgf1d_eval(A, ord, nn, c);
+        } else if (is_mehr(A.type)) {
+            // src/green_kernels.hpp:624-671: one k is zero; |k1| >= |k2| >= 0
+            const double a0 = fabs(kk[0]), a1 = fabs(kk[1]), a2 = fabs(kk[2]);
+            const double k1 = fmax(a0, fmax(a1, a2)), k2 = a0 + a1 + a2 - k1;
+            const int nn = (int)round(fabs(r) / h);
+            v = -h * mehr_1d(A.type, nn, k1 * h, k2 * h);
         } else if (A.type == CHAT_2) {
             v = k0 ? 0.5 * fabs(r) : -0.5 * exp(-k * r) / k;
         } else {

@@ -15,7 +15,8 @@ struct GreenArgs {
     int type = 0;
     double length = 0, g0 = 0, hgrid = 0;
     double hfact[3] = {0, 0, 0}, kfact[3] = {0, 0, 0}, koffset[3] = {0, 0, 0}, symstart[3] = {0, 0, 0};
-    const double* table = nullptr;   // LGF table near the origin (32^3 or 32^2)
+    const double* table = nullptr;   // LGF table near the origin (32^3 or 32^2); HEJ_0, 2 unbounded: Ji0c by s = |i|^2
+    int ntable = 0;
     const ExpTerm* exp = nullptr;    // LGF far-field expansion
     int nexp = 0;
     // 1-D LGF stencils, orders 2,4,6,8 at rows 0..3: s_0..s_w and p_0..p_w

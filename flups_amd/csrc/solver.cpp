@@ -362,6 +362,22 @@ void SolverImpl::fill_green_args(GreenArgs& A, const double hfact[3], const doub
 }
 
 namespace {
+// Ji0c(x) = int_0^x (1 - J0(u)) / u du = -1/2 sum_{n=1}^{50} (-x^2/4)^n / (n!^2 n): long double,
+// summed from n = 50 down, 1/n!^2 from the running product (src/ji0.hpp:23-41)
+double ji0c_series(double x) {
+    long double c[51];
+    long double f = 1.0L;
+    c[0] = 1.0L;
+    for (int n = 1; n <= 50; ++n) {
+        f *= (long double)n;
+        c[n] = 1.0L / (f * f);
+    }
+    const long double t = (long double)(-(0.25 * x * x));
+    long double val = 0.0L;
+    for (int n = 50; n > 0; --n) val -= std::pow(t, (long double)n) * c[n] / (long double)n * 0.5L;
+    return (double)val;
+}
+
 std::vector<double> read_table(const std::string& path, size_t n) {
     std::vector<double> v(n);
     FILE* f = std::fopen(path.c_str(), "rb");
@@ -413,8 +429,6 @@ void SolverImpl::compute_green() {
     const bool mehr = gtype_ >= MEHR_4L && gtype_ <= MEHR_6F;
     // src/green_functions.cpp:444-446 (HEJ_0, one unbounded direction: unavailable in the reference too)
     FLUPS_REQUIRE(!(gtype_ == HEJ_0 && nunb == 1), "HEJ_0 kernel not available for 1D unbounded problems");
-    FLUPS_REQUIRE(!(gtype_ == HEJ_0 && nunb == 2), "HEJ_0 with two unbounded directions is not available on the device");
-    FLUPS_REQUIRE(!(mehr && nunb == 1), "Mehrstellen kernels with one unbounded direction are not available on the device");
 
     GreenArgs A;
     fill_green_args(A, hfact, kfact, koffset, symstart, length, nunb);
@@ -445,6 +459,19 @@ void SolverImpl::compute_green() {
         A.table = dtab;
         A.exp = dexp;
         A.nexp = (int)ex.size();
+    }
+    if (gtype_ == HEJ_0 && nunb == 2) {
+        // Ji0c at rho = pi sqrt(s) for the integer squared distances s of the isotropic grid
+        // with rho < 30 (s <= 91), summed on the host in long double as src/ji0.hpp:23-41 does
+        // (its cancellation near rho = 30 leaves about 1e-10 relative error; the same summation
+        // keeps the Green's function equal to the reference's)
+        std::vector<double> tab(92);
+        for (int q = 0; q < 92; ++q) tab[q] = ji0c_series(std::sqrt((double)q) * h_[0] / length);
+        HIP_CHECK(hipMalloc(&dtab, tab.size() * sizeof(double)));
+        HIP_CHECK(hipMemcpyAsync(dtab, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice, stream_));
+        HIP_CHECK(hipStreamSynchronize(stream_));
+        A.table = dtab;
+        A.ntable = (int)tab.size();
     }
     // fill on the first Green's topology, then forward transforms of the non-spectral directions
     double* a = nullptr;

@@ -2,12 +2,11 @@
 //   drivers: src/green_functions.cpp:27-167 (3 unbounded), 192-391 (2), 407-538 (1), 560-708 (0)
 //   kernels: src/green_kernels.hpp (HEJ_0..10, CHAT_2, LGF_2..8, MEHR_4L/6L/4F/6F)
 //   tables:  src/green_functions.hpp:30-92 (.ker files, 32^3 / 32^2 raw float64)
+//   1-D Mehrstellen lattice Green's functions: src/lgf_1unb2spe.hpp:80-98, 500-611
+//   Ji0c: src/ji0.hpp:23-41 (long double series, the reference's summation order)
 // Special functions come from libstdc++ (std::cyl_bessel_k, std::expint) and,
 // for the sine integral, a series / complex continued fraction written here:
 // implementations independent of both the reference and the product.
-// Not restated (parity unpinned, the reference's own tests hold no case):
-// HEJ_0 with two unbounded directions (its Ji0c kernel) and the Mehrstellen
-// kernels with one unbounded direction.
 // Test infrastructure only.
 #include <cmath>
 #include <complex>
@@ -52,6 +51,89 @@ double sine_integral(double x) {
     }
     const C e1 = std::exp(-z) * h;
     return (double)(1.57079632679489661923132169163975144L + e1.imag());
+}
+
+// Ji0c(x) = int_0^x (1 - J0(u)) / u du = -1/2 sum_{n=1}^{50} (-x^2/4)^n / (n!^2 n), summed in
+// long double from n = 50 down, with 1/n!^2 from the running product n! (src/ji0.hpp:23-41).
+// Near x = 30 the terms reach 1e9 and the sum keeps about 1e-10 relative accuracy: this is
+// the reference's value, kept as it is.
+double ji0c(double x) {
+    static long double c[51];
+    static std::once_flag once;
+    std::call_once(once, [] {
+        long double f = 1.0L;
+        c[0] = 1.0L;
+        for (int n = 1; n <= 50; ++n) {
+            f *= (long double)n;
+            c[n] = 1.0L / (f * f);
+        }
+    });
+    const long double t = (long double)(-(0.25 * x * x));
+    long double val = 0.0L;
+    for (int n = 50; n > 0; --n) val -= std::pow(t, (long double)n) * c[n] / (long double)n * 0.5L;
+    return (double)val;
+}
+
+// 1-D lattice Green's functions of the Mehrstellen stencils along the unbounded direction,
+// transverse wave numbers k1 >= k2 (times h): the root r of a1 r^2 + a0 r + a1 = 0 inside the
+// unit circle gives G(n) = r^{|n|+1} / (a1 (r^2 - 1)) ("left": a1 G(n-1) + a0 G(n) + a1 G(n+1)
+// = delta_n); "full" applies the right-hand stencil b to it (src/lgf_1unb2spe.hpp:500-611).
+namespace {
+double r_from_lambda(double lambda) {   // lambda - sqrt(lambda - 1) sqrt(lambda + 1), |r| <= 1
+    if (std::isinf(lambda)) return 0.0;
+    if (std::fabs(lambda) > 1.0e2) {   // 1/(2l) + 1/(8l^3) + 1/(16l^5) + 5/(128l^7) + 7/(256l^9)
+        static const double e[10] = {0, 1. / 2., 0, 1. / 8., 0, 1. / 16., 0, 5. / 128., 0, 7. / 256.};
+        const double y = 1.0 / lambda;
+        double out = e[9];
+        for (int i = 8; i >= 0; --i) out = y * out + e[i];
+        return out;
+    }
+    const std::complex<double> a = std::sqrt(std::complex<double>(lambda - 1.0)), b = std::sqrt(std::complex<double>(lambda + 1.0));
+    return std::real(lambda - a * b);
+}
+}  // namespace
+
+double mehr_1d(int kind, int n, double k1, double k2) {
+    // kind: 0 = 4 left, 1 = 6 left, 2 = 4 full, 3 = 6 full
+    const bool six = kind == 1 || kind == 3, full = kind >= 2;
+    const int an = std::abs(n);
+    if (k1 == 0 && k2 == 0) {
+        double v = -0.5 * an;
+        if (full && !six) v -= (n == 0) * 1. / 12.;
+        if (full && six) v -= (an == 1) * (-1. / 240.) + (n == 0) * (11. / 120.);
+        return v;
+    }
+    const double y0 = std::pow(std::sin(k1 / 2), 2), y1 = std::pow(std::sin(k2 / 2), 2);
+    double a0, a1;
+    if (!six) {
+        a0 = -8. / 3. * (y0 * y1 - y0 - y1) + 2.0;
+        a1 = 2. / 3. * (y0 + y1) - 1.0;
+    } else {
+        a0 = -8. / 5. * y0 * y1 + 8. / 3. * (y0 + y1) + 2.0;
+        a1 = -8. / 15. * y0 * y1 + 2. / 3. * (y0 + y1) - 1.0;
+    }
+    const double lambda = -a0 / (2. * a1);
+    const double r = r_from_lambda(lambda);
+    if (!full) {
+        const double base = (r / a1) * std::pow(r, an) / (r * r - 1.0);
+        if (std::isfinite(lambda)) return base;
+        return six ? -(double)(n == 0) / a0 : (double)(n == 0) / a0;
+    }
+    if (!six) {
+        const double b0 = 5. / 6. - 1. / 3. * (y0 + y1), b1 = 1. / 12.;
+        const double pr = b1 + b0 * r + b1 * r * r;
+        if (std::isfinite(lambda)) {
+            const double base = pr * std::pow(r, an) / (a1 * (r * r - 1));
+            return n == 0 ? (r / a1) * (b0 + 2. * b1 * r) / (r * r - 1.) : base;
+        }
+        return (n == 0) * (b0 / a0) + (n == 1) * (b1 / a0);
+    }
+    const double b0 = -1. / 15. * (std::pow(y0, 2) + std::pow(y1, 2)) + 8. / 45. * y0 * y1 - 11. / 45. * (y0 + y1) + 97. / 120.;
+    const double b1 = -2. / 45. * (y0 + y1) + 1. / 10.;
+    const double b2 = -1. / 240.;
+    const double pr = b0 * std::pow(r, 2) + b1 * (r + std::pow(r, 3)) + b2 * (1.0 + std::pow(r, 4));
+    const double base = pr * std::pow(r, an - 1) / (a1 * (r * r - 1.0));
+    return base + (an == 1) * (b2 / a1) + (n == 0) * (b1 * a1 - b2 * a0) / (a1 * a1);
 }
 
 namespace {
@@ -262,7 +344,7 @@ void green_fill_2unb(const Topo& t, const double hfact[3], const double kfact[3]
     if (ord) {
         if (!iso) throw std::runtime_error("LGF needs an isotropic grid");
         tab = &kernel_table(kdir, file, 2);
-    } else if (!(type == CHAT_2 || (type >= HEJ_2 && type <= HEJ_10))) {
+    } else if (!(type == CHAT_2 || (type >= HEJ_2 && type <= HEJ_0))) {
         throw std::runtime_error("kernel not restated in the oracle");
     }
     const double sh = hfact[0] + hfact[1] + hfact[2];
@@ -302,6 +384,15 @@ void green_fill_2unb(const Topo& t, const double hfact[3], const double kfact[3]
                     } else {
                         v = -std::cyl_bessel_k(0.0, std::fabs(k) * r) / (2.0 * PI);
                     }
+                } else if (type == HEJ_0) {
+                    // green_kernels.hpp:416-428: the same expression at r = 0 and on k = 0; elsewhere
+                    // zero_ (green_kernels.hpp:430-434), which no spectral rewrite replaces for HEJ_0
+                    if (r0 || k0) {
+                        const double rho = r / length;
+                        v = rho < 30.0 ? (ji0c(rho) + std::log(2 * length) - EULER) / (2.0 * PI) : std::log(r) / (2.0 * PI);
+                    } else {
+                        v = 0.5 * std::fabs(r);
+                    }
                 } else {
                     if (r0) {
                         v = -(0.5 * EULER - std::log(M_SQRT2 * length) + hej2_q0(type)) / (2.0 * PI);
@@ -309,7 +400,7 @@ void green_fill_2unb(const Topo& t, const double hfact[3], const double kfact[3]
                         const double rho2 = (r / length) * (r / length);
                         v = (std::log(r) - hej2_q(type, rho2) * std::exp(-0.5 * rho2) + 0.5 * e1(0.5 * rho2)) / (2.0 * PI);
                     } else {
-                        v = 0.0;
+                        v = 0.5 * std::fabs(r);   // zero_; the spectral rewrite replaces these planes
                     }
                 }
                 gat(t, g, i0, i1, i2) = v;
@@ -320,7 +411,8 @@ void green_fill_2unb(const Topo& t, const double hfact[3], const double kfact[3]
 void green_fill_1unb(const Topo& t, const double hfact[3], const double kfact[3], const double koffset[3],
                      const double symstart[3], double* g, GreenType type, double length) {
     const int ord = lgf_order(type);
-    if (!ord && !(type == CHAT_2 || (type >= HEJ_2 && type <= HEJ_10)))
+    const bool mehr = type == MEHR_4L || type == MEHR_6L || type == MEHR_4F || type == MEHR_6F;
+    if (!ord && !mehr && !(type == CHAT_2 || (type >= HEJ_2 && type <= HEJ_10)))
         throw std::runtime_error("kernel not restated in the oracle");
     const double sk = kfact[0] + kfact[1] + kfact[2];
     const double h = hfact[0] + hfact[1] + hfact[2];
@@ -340,6 +432,14 @@ void green_fill_1unb(const Topo& t, const double hfact[3], const double kfact[3]
                     for (int d = 0; d < 3; ++d) c += lgf_symbol(ord, kk[d] * h);
                     const int n = (int)std::round(std::fabs(r) / h);
                     v = -h * lgf1d(ord, n, c);
+                } else if (mehr) {
+                    // green_kernels.hpp:624-671: one k is zero; |k1| >= |k2| >= 0
+                    const double a0 = std::fabs(kk[0]), a1 = std::fabs(kk[1]), a2 = std::fabs(kk[2]);
+                    const double k1 = std::max(a0, std::max(a1, a2));
+                    const double k2 = a0 + a1 + a2 - k1;
+                    const int n = (int)std::round(std::fabs(r) / h);
+                    const int kind = type == MEHR_4L ? 0 : type == MEHR_6L ? 1 : type == MEHR_4F ? 2 : 3;
+                    v = -h * mehr_1d(kind, n, k1 * h, k2 * h);
                 } else if (type == CHAT_2) {
                     v = k0 ? 0.5 * std::fabs(r) : -0.5 * std::exp(-k * r) / k;
                 } else {

@@ -40,9 +40,11 @@ def lib():
             getattr(L, n).restype = c_double
         L.oracle_lgf_symbol.argtypes = [c_int, c_double]
         L.oracle_lgf_symbol.restype = c_double
-        for n in ["oracle_bessel_k0", "oracle_bessel_k1", "oracle_expint_e1", "oracle_sine_integral"]:
+        for n in ["oracle_bessel_k0", "oracle_bessel_k1", "oracle_expint_e1", "oracle_sine_integral", "oracle_ji0c"]:
             getattr(L, n).argtypes = [c_double]
             getattr(L, n).restype = c_double
+        L.oracle_mehr_1d.argtypes = [c_int, c_int, c_double, c_double]
+        L.oracle_mehr_1d.restype = c_double
         _lib = L
     return _lib
 

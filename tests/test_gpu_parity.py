@@ -108,20 +108,54 @@ DEVICE_KERNELS = list(range(15))   # every kernel of the reference
 
 @pytest.mark.parametrize("center", [1, 0])
 def test_kernel_sweep(center):
-    """9 BC sets x 15 kernels of test_3D_kerns.py at N = 16 (cell) / 17 (node); HEJ_0 on the 2-D
-    unbounded set is not available (the reference's tests hold no such case either)"""
+    """9 BC sets x 15 kernels of test_3D_kerns.py at N = 16 (cell) / 17 (node), against the oracle
+    and the golden norms; the reference's data hold no entry for HEJ_0 on the 2-D unbounded set
+    (its Ji0c kernel), which is checked against the oracle only"""
     golden = golden_table()
     N = 16 if center == 1 else 17
     matched = 0
     for bc6 in KERN_BCS:
         for k in DEVICE_KERNELS:
-            if k == 7 and bc6 == (4, 0, 1, 4, 9, 9):
-                with pytest.raises(RuntimeError):
-                    run_validation_case(center, bc6, k, N, 1, golden)
-                assert "not available" in fl.last_error()
-                continue
             matched += run_validation_case(center, bc6, k, N, 1, golden)
     assert matched == len(KERN_BCS) * len(DEVICE_KERNELS) - 1
+
+
+# the reference's tests hold no case of these: HEJ_0 with two unbounded directions (Ji0c on the
+# k = 0 plane, 0.5 |r| elsewhere, no spectral rewrite) and the Mehrstellen kernels with one
+# unbounded direction (1-D lattice Green's functions); oracle parity at 1e-12
+TWO_ONE_CASES = [
+    # (nglob, bc, center, kernel)
+    ((32, 32, 16), [[4, 4], [4, 4], [3, 3]], 1, 7),
+    ((33, 33, 16), [[4, 4], [1, 4], [3, 3]], 0, 7),
+    ((24, 32, 32), [[3, 3], [4, 0], [4, 4]], 1, 7),
+    ((64, 64, 1), [[4, 4], [4, 4], [9, 9]], 1, 7),
+    ((32, 16, 16), [[4, 4], [3, 3], [3, 3]], 1, 11),
+    ((32, 16, 16), [[4, 4], [3, 3], [3, 3]], 1, 12),
+    ((32, 16, 16), [[4, 4], [3, 3], [0, 1]], 1, 13),
+    ((17, 17, 17), [[4, 4], [3, 3], [3, 3]], 0, 14),
+    ((16, 32, 16), [[3, 3], [0, 4], [3, 3]], 1, 14),
+]
+
+
+@pytest.mark.parametrize("case", TWO_ONE_CASES, ids=["hej0-unb2-per", "hej0-node-mix", "hej0-per-yz", "hej0-2d",
+                                                     "mehr4l-unb1", "mehr6l-unb1", "mehr4f-unb1-sym", "mehr6f-node",
+                                                     "mehr6f-semi-y"])
+def test_hej0_two_unbounded_and_mehr_one_unbounded(case):
+    nglob, bc, center, kernel = case
+    rng = np.random.default_rng(5)
+    n0, n1, n2 = nglob
+    rhs = rng.normal(size=(1, n2, n1, n0))
+    if any(b == [3, 3] for b in bc):
+        rhs -= rhs.mean(axis=(1, 2, 3), keepdims=True)
+    n = max(nglob)
+    h = [1.0 / (n - (1 - center))] * 3
+    L = [h[0] * (m - (1 - center)) for m in nglob]
+    if nglob[2] == 1:
+        L[2] = h[2]
+    f = gpu_solve(list(nglob), bc, 1, h, L, center, kernel, rhs)
+    ora = orc.solve(list(nglob), bc, 1, h, L, center, kernel, rhs)
+    e2, ei = rel_err(f, ora)
+    assert e2 <= TOL and ei <= TOL, (e2, ei)
 
 
 def test_vtube_rot():

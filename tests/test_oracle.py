@@ -153,6 +153,80 @@ def test_sine_integral(x):
     assert L.oracle_sine_integral(-x) == -L.oracle_sine_integral(x)
 
 
+@pytest.mark.parametrize("x", [0.5, 3.0, 10.0, 20.0, 25.0, 28.27, 29.6, 29.97])
+def test_ji0c_against_quadrature(x):
+    """Ji0c(x) = int_0^x (1 - J0(u)) / u du by adaptive quadrature of scipy's j0; near x = 30 the
+    reference's long double series keeps only about 1e-10 (its value is what the solve uses)"""
+    import scipy.integrate as si
+    want, _ = si.quad(lambda u: (1.0 - sp.j0(u)) / u, 0.0, x, limit=400, epsabs=0.0, epsrel=1e-13)
+    tol = 1e-12 if x <= 25.0 else 3e-10
+    assert abs(orc.lib().oracle_ji0c(x) - want) <= tol * want
+
+
+def test_ji0c_small_argument_series():
+    """Ji0c(x) = x^2/8 - x^4/256 + x^6/(6 * 2304) - ... for small x; Ji0c(0) = 0"""
+    L = orc.lib()
+    assert L.oracle_ji0c(0.0) == 0.0
+    for x in (1e-3, 0.05, 0.2):
+        want = x ** 2 / 8 - x ** 4 / 256 + x ** 6 / 13824   # next term -x^8 / 1179648
+        assert abs(L.oracle_ji0c(x) - want) <= 1e-15 * want + 1e-6 * x ** 8
+
+
+def mehr_stencil_a(six, k1, k2):
+    """(a0, a1): minus the 3-D Mehrstellen left stencil Fourier-transformed in the two transverse
+    directions (h = 1). Fourth order: 19 points (centre -4, faces 1/3, edges 1/6); sixth order:
+    27 points (-128, 14, 3, 1) / 30"""
+    c1, c2 = np.cos(k1), np.cos(k2)
+    if not six:
+        off0 = -4.0 + (2.0 / 3.0) * (c1 + c2) + (2.0 / 3.0) * c1 * c2
+        off1 = 1.0 / 3.0 + (1.0 / 3.0) * (c1 + c2)
+    else:
+        off0 = (-128.0 + 28.0 * (c1 + c2) + 12.0 * c1 * c2) / 30.0
+        off1 = (14.0 + 6.0 * (c1 + c2) + 4.0 * c1 * c2) / 30.0
+    return -off0, -off1
+
+
+@pytest.mark.parametrize("six", [False, True])
+def test_mehr_1d_left_solves_stencil(six):
+    """a1 G(n-1) + a0 G(n) + a1 G(n+1) = delta_n for the left Mehrstellen LGF, with (a0, a1) from
+    the 3-D stencil; k1 = k2 = 0 is the regularised -|n|/2 of a0 = 2, a1 = -1"""
+    L = orc.lib()
+    rng = np.random.default_rng(7)
+    ks = [tuple(sorted(rng.uniform(0.0, np.pi, 2), reverse=True)) for _ in range(40)] + [(0.3, 0.0), (np.pi, np.pi)]
+    for k1, k2 in ks:
+        a0, a1 = mehr_stencil_a(six, k1, k2)
+        g = {n: L.oracle_mehr_1d(int(six), n, k1, k2) for n in range(-9, 10)}
+        for n in range(-8, 9):
+            res = a1 * g[n - 1] + a0 * g[n] + a1 * g[n + 1] - (1.0 if n == 0 else 0.0)
+            # the root lambda - sqrt(lambda^2 - 1) loses about lambda^2 eps to cancellation (lambda <= 100)
+            scale = abs(a1) * (abs(g[n - 1]) + abs(g[n + 1])) + abs(a0 * g[n]) + 1.0
+            assert abs(res) <= 1e-11 * scale, (six, k1, k2, n, res)
+        assert g[4] == g[-4]
+    g = [L.oracle_mehr_1d(int(six), n, 0.0, 0.0) for n in range(-3, 4)]
+    assert g == [-0.5 * abs(n) for n in range(-3, 4)]
+
+
+@pytest.mark.parametrize("six", [False, True])
+def test_mehr_1d_full_is_right_stencil_of_left(six):
+    """the full Mehrstellen LGF is the right-hand stencil b applied to the left one:
+    G_full(n) = sum_m b_|m| G_left(n - m). Fourth order: b = (1/2 + (1/12)(2 cos k1 + 2 cos k2), 1/12)
+    (f + h^2/12 laplacian f); sixth order: b from src/lgf_1unb2spe.hpp:574-578"""
+    L = orc.lib()
+    rng = np.random.default_rng(8)
+    for k1, k2 in [tuple(sorted(rng.uniform(0.0, np.pi, 2), reverse=True)) for _ in range(30)] + [(0.0, 0.0)]:
+        y0, y1 = np.sin(k1 / 2) ** 2, np.sin(k2 / 2) ** 2
+        if not six:
+            b = [0.5 + (2 * np.cos(k1) + 2 * np.cos(k2)) / 12.0, 1.0 / 12.0]
+        else:
+            b = [-1 / 15 * (y0 ** 2 + y1 ** 2) + 8 / 45 * y0 * y1 - 11 / 45 * (y0 + y1) + 97 / 120, -2 / 45 * (y0 + y1) + 1 / 10,
+                 -1 / 240]
+        left = {n: L.oracle_mehr_1d(int(six), n, k1, k2) for n in range(-12, 13)}
+        for n in range(-6, 7):
+            want = sum(b[abs(m)] * left[n - m] for m in range(1 - len(b), len(b)))
+            got = L.oracle_mehr_1d(2 + int(six), n, k1, k2)
+            assert abs(got - want) <= 1e-12 * max(1.0, abs(want)), (six, k1, k2, n, got, want)
+
+
 def stencil(order):
     t = 2 * np.pi * np.arange(64) / 64
     sym = np.array([orc.lib().oracle_lgf_symbol(order, v) for v in t])
