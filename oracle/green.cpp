@@ -7,6 +7,10 @@
 // Special functions come from libstdc++ (std::cyl_bessel_k, std::expint) and,
 // for the sine integral, a series / complex continued fraction written here:
 // implementations independent of both the reference and the product.
+// Parity unpinned: the per-point kernel choice of HEJ_0 with two unbounded directions in 3-D
+// (zero_ = 0.5 |r| off the k = 0 plane, kept because the reference's spectral rewrite skips
+// HEJ_0: src/Solver.cpp:1086-1101, src/green_functions.cpp:251-256) follows the reference's code;
+// no golden line or invariant checks it (the 2-D case and Ji0c itself are pinned).
 // Test infrastructure only.
 #include <cmath>
 #include <complex>

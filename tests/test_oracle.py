@@ -227,6 +227,25 @@ def test_mehr_1d_full_is_right_stencil_of_left(six):
             assert abs(got - want) <= 1e-12 * max(1.0, abs(want)), (six, k1, k2, n, got, want)
 
 
+def test_hej0_two_unbounded_2d_converges():
+    """HEJ_0 on a 2-D semi-unbounded grid (Ji0c kernel on the whole plane; the reference's data
+    hold no line for it): the error against the manufactured solution falls with N (order ~1.8)"""
+    e16 = orc.validation_errors(1, (4, 0, 1, 4, 9, 9), 7, 16)
+    e32 = orc.validation_errors(1, (4, 0, 1, 4, 9, 9), 7, 32)
+    assert e16[0] < 2e-4 and e16[1] < 6e-4
+    assert e16[0] / e32[0] > 3.0 and e16[1] / e32[1] > 2.5
+
+
+@pytest.mark.parametrize("kernel,order", [(11, 2), (12, 2), (13, 4), (14, 6)])
+def test_mehr_one_unbounded_convergence_order(kernel, order):
+    """Mehrstellen kernels with one unbounded direction (x unbounded, y and z periodic): the
+    L2 error falls at the kernel's order between N = 16 and 32 (left stencils: second order,
+    4F: fourth, 6F: sixth)"""
+    e16 = orc.validation_errors(1, (4, 4, 3, 3, 3, 3), kernel, 16)
+    e32 = orc.validation_errors(1, (4, 4, 3, 3, 3, 3), kernel, 32)
+    assert np.log2(e16[0] / e32[0]) > order - 0.3
+
+
 def stencil(order):
     t = 2 * np.pi * np.arange(64) / 64
     sym = np.array([orc.lib().oracle_lgf_symbol(order, v) for v in t])
