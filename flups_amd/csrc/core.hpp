@@ -102,6 +102,9 @@ struct PassDesc {
     int fa = 1, fb = 2;     // pencil grid: dim fa varies fastest between pencils (the one with the smaller stride)
     int na = 1;             // extent along fa
     int alias = 0;          // 1: the second FFT buffer reuses the pencil buffer
+    int fastk = 0;          // 1: the pass runs the row r2r kernel (rowpass.hip) instead of the generic one
+    int vec = 0;            // fast kernels: pencils start 16-byte aligned in source and destination
+    const double* aux = nullptr;   // fast kernels: pre / post twiddle table
 };
 
 }  // namespace flups

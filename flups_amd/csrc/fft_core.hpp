@@ -1,0 +1,175 @@
+// Register-resident FFT building blocks shared by the fused pipeline (fast.hip)
+// and the fast kernels of the general path (rowpass.hip): complex helpers,
+// radix-2/4/8 butterflies and the column FFT of length M held by M/8 threads
+// (8 values each) with Stockham exchanges through LDS.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace flups {
+
+template <typename R>
+struct C2 {
+    R x, y;
+};
+template <typename R>
+__device__ __forceinline__ C2<R> cadd(C2<R> a, C2<R> b) { return {a.x + b.x, a.y + b.y}; }
+template <typename R>
+__device__ __forceinline__ C2<R> csub(C2<R> a, C2<R> b) { return {a.x - b.x, a.y - b.y}; }
+template <typename R>
+__device__ __forceinline__ C2<R> cmul(C2<R> a, C2<R> b) { return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
+template <typename R>
+__device__ __forceinline__ C2<R> cconj(C2<R> a) { return {a.x, -a.y}; }
+// multiply by -i (forward) or +i (inverse)
+template <typename R, bool FWD>
+__device__ __forceinline__ C2<R> rot90(C2<R> a) { return FWD ? C2<R>{a.y, -a.x} : C2<R>{-a.y, a.x}; }
+
+template <typename R, bool FWD>
+__device__ __forceinline__ void dft2(C2<R>& a, C2<R>& b) {
+    const C2<R> t = a;
+    a = cadd(t, b);
+    b = csub(t, b);
+}
+template <typename R, bool FWD>
+__device__ __forceinline__ void dft4(C2<R>& a0, C2<R>& a1, C2<R>& a2, C2<R>& a3) {
+    const C2<R> s0 = cadd(a0, a2), d0 = csub(a0, a2);
+    const C2<R> s1 = cadd(a1, a3), d1 = rot90<R, FWD>(csub(a1, a3));
+    a0 = cadd(s0, s1);
+    a2 = csub(s0, s1);
+    a1 = cadd(d0, d1);
+    a3 = csub(d0, d1);
+}
+// HIN: v[4..7] are zero (zero-padded input) and are not read
+template <typename R, bool FWD, bool HIN = false>
+__device__ __forceinline__ void dft8(C2<R>* v) {
+    const R h = (R)0.70710678118654752440;
+    // radix-2 split into even/odd halves, then 4-point DFTs
+    C2<R> e0, e1, e2, e3, o0, o1, o2, o3;
+    if (HIN) {
+        e0 = o0 = v[0];
+        e1 = o1 = v[1];
+        e2 = o2 = v[2];
+        e3 = o3 = v[3];
+    } else {
+        e0 = cadd(v[0], v[4]), o0 = csub(v[0], v[4]);
+        e1 = cadd(v[1], v[5]), o1 = csub(v[1], v[5]);
+        e2 = cadd(v[2], v[6]), o2 = csub(v[2], v[6]);
+        e3 = cadd(v[3], v[7]), o3 = csub(v[3], v[7]);
+    }
+    // twiddles w8^k on the odd half
+    o1 = FWD ? C2<R>{h * (o1.x + o1.y), h * (o1.y - o1.x)} : C2<R>{h * (o1.x - o1.y), h * (o1.y + o1.x)};
+    o2 = rot90<R, FWD>(o2);
+    o3 = FWD ? C2<R>{h * (o3.y - o3.x), -h * (o3.x + o3.y)} : C2<R>{-h * (o3.x + o3.y), h * (o3.x - o3.y)};
+    dft4<R, FWD>(e0, e1, e2, e3);
+    dft4<R, FWD>(o0, o1, o2, o3);
+    v[0] = e0; v[2] = e1; v[4] = e2; v[6] = e3;
+    v[1] = o0; v[3] = o1; v[5] = o2; v[7] = o3;
+}
+
+// twiddle exp(-+2 pi i e / M) from the table tw[t] = exp(-2 pi i t / M)
+template <typename R, bool FWD>
+__device__ __forceinline__ C2<R> twl(const C2<R>* tw, int e) {
+    const C2<R> w = tw[e];
+    return FWD ? w : cconj(w);
+}
+
+// LDS slot of column element e: e*ES for interleaved columns (ES columns side by
+// side), or e + e/8 for a single contiguous column (ES = 0): the pad slot every 8
+// elements keeps the stride-8 writes of the first stage on distinct banks.
+template <int ES>
+__device__ __forceinline__ int lslot(int e) { return ES > 0 ? e * ES : e + (e >> 3); }
+
+// Full column FFT of length M held by a team of T = M/8 threads (j = 0..T-1).
+// On entry v[i] = x[j + i*T]; on exit v[i] = X[j + i*T]. lds points at the
+// column (element e at lds[lslot<ES>(e)]); the whole block must call this together.
+// HIN: the upper half of the input is zero (v[4..7] not read). Outputs the caller
+// never uses (the upper half of a truncated inverse) are removed by the compiler.
+template <typename R, int M, bool FWD, int ES, bool HIN = false>
+__device__ __forceinline__ void column_fft(C2<R>* v, C2<R>* lds, int j, const C2<R>* tw) {
+    constexpr int T = M / 8;
+    // table entries w, w^2, w^4 of the next radix-8 stage, loaded one stage ahead
+    // so that their latency hides behind the current stage and its LDS exchange
+    // (loading all seven w^q instead of forming w^3, w^5, w^6, w^7 by products
+    // competes with the data loads: FP32 z pass 184 -> 236 us at 256^3)
+    C2<R> r1 = {1, 0}, r2 = {1, 0}, r4 = {1, 0};
+    int span = 1;
+#pragma unroll
+    for (int st = 0; st < 8; ++st) {
+        // stages of radix 8 while 8*span <= M, then one radix 4 or 2 stage
+        if (span >= M) break;
+        const int P = (span * 8 <= M) ? 8 : (M / span);
+        if (span > 1) {
+            __syncthreads();   // previous stage's writes are visible
+#pragma unroll
+            for (int i = 0; i < 8; ++i) v[i] = lds[lslot<ES>(j + i * T)];
+        }
+        if (P == 8) {
+            const int k = j % span;
+            C2<R> n1 = {1, 0}, n2 = {1, 0}, n4 = {1, 0};
+            const int nspan = span * 8;
+            if (nspan * 8 <= M) {
+                const int e = (j % nspan) * (M / (nspan * 8));
+                n1 = twl<R, FWD>(tw, e);
+                n2 = twl<R, FWD>(tw, 2 * e);
+                n4 = twl<R, FWD>(tw, 4 * e);
+            }
+            if (span > 1) {
+                // w^q, q = 1..7, from w, w^2, w^4 and four products
+                C2<R> tw8[8];
+                tw8[1] = r1;
+                tw8[2] = r2;
+                tw8[4] = r4;
+                tw8[3] = cmul(tw8[1], tw8[2]);
+                tw8[5] = cmul(tw8[1], tw8[4]);
+                tw8[6] = cmul(tw8[2], tw8[4]);
+                tw8[7] = cmul(tw8[3], tw8[4]);
+#pragma unroll
+                for (int q = 1; q < 8; ++q) v[q] = cmul(v[q], tw8[q]);
+            }
+            if (HIN && span == 1) dft8<R, FWD, true>(v);
+            else dft8<R, FWD>(v);
+            if (span * 8 < M) {
+                __syncthreads();   // everyone has read this stage's inputs
+                const int base = (j / span) * span * 8 + k;
+#pragma unroll
+                for (int u = 0; u < 8; ++u) lds[lslot<ES>(base + u * span)] = v[u];
+            }
+            r1 = n1;
+            r2 = n2;
+            r4 = n4;
+        } else if (P == 4) {
+            // two butterflies b = j + t*T (t = 0, 1), inputs v[t + 2q]
+#pragma unroll
+            for (int t = 0; t < 2; ++t) {
+                const int b = j + t * T, k = b % span, stride = M / (span * 4);
+                C2<R> a[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) a[q] = v[t + 2 * q];
+#pragma unroll
+                for (int q = 1; q < 4; ++q) a[q] = cmul(a[q], twl<R, FWD>(tw, q * k * stride));
+                dft4<R, FWD>(a[0], a[1], a[2], a[3]);
+#pragma unroll
+                for (int u = 0; u < 4; ++u) v[t + 2 * u] = a[u];
+            }
+        } else {   // P == 2, four butterflies b = j + t*T, inputs v[t], v[t+4]
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const int b = j + t * T, k = b % span, stride = M / (span * 2);
+                C2<R> a0 = v[t], a1 = cmul(v[t + 4], twl<R, FWD>(tw, k * stride));
+                dft2<R, FWD>(a0, a1);
+                v[t] = a0;
+                v[t + 4] = a1;
+            }
+        }
+        span *= P;
+    }
+}
+
+// Workgroups are dispatched round-robin over the 8 XCDs (each with its own L2):
+// hardware block b runs on XCD b % 8. Renumber so that consecutive logical
+// blocks run on the same XCD at about the same time (they share cache lines).
+__device__ __forceinline__ int xcd_group(int b, int nb) {
+    const int q = nb / 8, r = nb % 8, x = b % 8, slot = b / 8;
+    return x < r ? x * (q + 1) + slot : r * (q + 1) + (x - r) * q + slot;
+}
+
+}  // namespace flups

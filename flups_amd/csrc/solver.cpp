@@ -10,7 +10,9 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <fstream>
 #include <sstream>
@@ -20,6 +22,9 @@ namespace flups {
 size_t generic_lds_bytes(const PassDesc& P, int ppb);
 bool generic_alias(const PassDesc& P);
 void launch_generic(const PassDesc& P, hipStream_t s);
+bool row_r2r_fits(const PassDesc& P);
+std::vector<double> row_r2r_table(int kind, int n);
+void launch_row_r2r(const PassDesc& P, hipStream_t s);
 FastPlan* make_fast_plan(SolverImpl& s);   // nullptr if the fused pipeline does not apply
 void destroy_fast_plan(FastPlan* f);
 void fast_solve(FastPlan* f, double* field, const double* rhs, const View& uin, const View& uout, hipStream_t s,
@@ -187,6 +192,7 @@ SolverImpl::~SolverImpl() {
     if (fast_) destroy_fast_plan(fast_);
     if (hybrid_) destroy_hybrid_z(hybrid_);
     for (auto& kv : tw_) (void)hipFree(kv.second);
+    for (auto& kv : r2r_tab_) (void)hipFree(kv.second);
     for (double* p : dstage_) if (p) (void)hipFree(p);
     if (dgreen_) (void)hipFree(dgreen_);
     if (dstaging_) (void)hipFree(dstaging_);
@@ -226,6 +232,18 @@ const double* SolverImpl::twiddles(int M) {
     HIP_CHECK(hipMalloc(&d, t.size() * sizeof(double)));
     HIP_CHECK(hipMemcpy(d, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice));
     tw_[M] = d;
+    return d;
+}
+
+const double* SolverImpl::r2r_table(int kind, int n) {
+    const auto key = std::make_pair(kind, n);
+    auto it = r2r_tab_.find(key);
+    if (it != r2r_tab_.end()) return it->second;
+    const auto t = row_r2r_table(kind, n);
+    double* d = nullptr;
+    HIP_CHECK(hipMalloc(&d, t.size() * sizeof(double)));
+    HIP_CHECK(hipMemcpy(d, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice));
+    r2r_tab_[key] = d;
     return d;
 }
 
@@ -298,6 +316,16 @@ PassDesc SolverImpl::pass_desc(const DirPlan& p, int comp, bool forward, const V
         P.fb = swap ? d1 : d2;
         P.na = swap ? P.n2 : P.n1;
     }
+    // contiguous real-to-real pencils of a power-of-two length: the row r2r kernel
+    if (!std::getenv("FLUPS_GENERIC_PASS") && row_r2r_fits(P)) {
+        P.fastk = 1;
+        P.tw = twiddles(P.n / 2);
+        P.aux = r2r_table(P.kind, P.n);
+        const int fa = P.fa, fb = P.fb;
+        P.vec = ((uintptr_t)P.src % 16 == 0) && ((uintptr_t)P.dst % 16 == 0) && P.src_s[fa] % 2 == 0 && P.src_s[fb] % 2 == 0 &&
+                P.dst_s[fa] % 2 == 0 && P.dst_s[fb] % 2 == 0;
+        return P;
+    }
     P.alias = generic_alias(P) ? 1 : 0;
     const size_t per_pen = generic_lds_bytes(P, 1);
     FLUPS_REQUIRE(per_pen <= lds_budget_, "pencil of %d elements (transform length %d) does not fit the general path's LDS",
@@ -310,7 +338,10 @@ PassDesc SolverImpl::pass_desc(const DirPlan& p, int comp, bool forward, const V
     return P;
 }
 
-void SolverImpl::run_pass(PassDesc P) { launch_generic(P, stream_); }
+void SolverImpl::run_pass(PassDesc P) {
+    if (P.fastk == 1) launch_row_r2r(P, stream_);
+    else launch_generic(P, stream_);
+}
 
 // ---------------------------------------------------------------------------
 void SolverImpl::fill_green_args(GreenArgs& A, const double hfact[3], const double kfact[3], const double koffset[3],
@@ -760,6 +791,7 @@ void SolverImpl::enqueue(double* field, const double* rhs, int type) {
 size_t SolverImpl::device_bytes() const {
     size_t b = 0;
     for (const auto& kv : tw_) b += 2 * sizeof(double) * (size_t)kv.first;
+    for (const auto& kv : r2r_tab_) b += 4 * sizeof(double) * (size_t)(kv.first.second / 2 + 1);
     for (int ip = 0; ip < 3; ++ip)
         if (dstage_[ip]) b += sizeof(double) * std::max(hat_[ip].count(), stage_[ip].count());
     if (dgreen_) b += sizeof(double) * gpost_[ndim_ - 1].count();

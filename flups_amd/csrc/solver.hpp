@@ -98,6 +98,8 @@ class SolverImpl {
     Shape hat_[3], stage_[3], gpre_[3], gpost_[3];
     hipStream_t stream_ = nullptr;
     std::map<int, double*> tw_;
+    std::map<std::pair<int, int>, double*> r2r_tab_;   // (kind, n) -> pre / post twiddles of the row r2r kernel
+    const double* r2r_table(int kind, int n);
     double* dstage_[3] = {nullptr, nullptr, nullptr};
     double* dgreen_ = nullptr;
     double* dstaging_ = nullptr;   // rhs/field copies for host pointers

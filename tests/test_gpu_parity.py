@@ -518,3 +518,57 @@ def test_fused_x_slabs_bitwise_and_oracle(case, monkeypatch):
     e2, ei = rel_err(one, ora)
     tol = 1e-5 if fp32 else TOL
     assert e2 <= tol and ei <= tol, (e2, ei)
+
+
+ROW_CASES = [
+    # (nglob, bc, center, kernel): symmetric x (cell-centred: REDFT10/01, RODFT10/01, REDFT11, RODFT11)
+    ((64, 32, 32), [[0, 0], [3, 3], [4, 4]], 1, 2),
+    ((64, 32, 32), [[1, 1], [3, 3], [4, 4]], 1, 2),
+    ((256, 16, 16), [[0, 1], [3, 3], [4, 4]], 1, 8),
+    ((32, 32, 32), [[1, 0], [4, 4], [3, 3]], 1, 0),
+    ((128, 16, 16), [[0, 1], [1, 0], [0, 0]], 1, 0),
+    ((64, 24, 16), [[1, 1], [0, 1], [4, 4]], 1, 4),
+]
+
+
+@pytest.mark.parametrize("case", ROW_CASES, ids=["ee", "oo", "eo-256", "oe", "sym3", "oo-mix"])
+@pytest.mark.parametrize("misalign", [False, True])
+def test_row_r2r_kernel_vs_generic_and_oracle(case, misalign, monkeypatch):
+    """the row kernel of the contiguous real-to-real passes (one half-length complex FFT with pre-
+    and post-twiddles) against the generic pass kernel (FLUPS_GENERIC_PASS) and the oracle at
+    1e-12; misaligned user rows take its scalar loads and stores"""
+    nglob, bc, center, kernel = case
+    rng = np.random.default_rng(31)
+    n0, n1, n2 = nglob
+    rhs = rng.normal(size=(1, n2, n1, n0))
+    rhs -= rhs.mean()
+    h = [1.0 / max(nglob)] * 3
+    L = [h[0] * m for m in nglob]
+
+    def run(generic):
+        if generic:
+            monkeypatch.setenv("FLUPS_GENERIC_PASS", "1")
+        else:
+            monkeypatch.delenv("FLUPS_GENERIC_PASS", raising=False)
+        topo = fl.Topology(0, 1, list(nglob), [1, 1, 1], False)
+        s = fl.Solver(topo, bc, h, L)
+        s.set_green_type(kernel)
+        s.setup()
+        n = rhs.size
+        off = 1 if misalign else 0   # one double in: rows start 8 bytes off a 16-byte boundary
+        flat = np.zeros(n + 1)
+        flat[off:off + n] = rhs.ravel()
+        buf_in = fl.DeviceArray.from_numpy(flat)
+        buf_out = fl.DeviceArray((n + 1,))
+        fl.lib().flups_clear_last_error()
+        fl.lib().flups_solve(s._solver, buf_out.ptr + 8 * off, buf_in.ptr + 8 * off, 0)
+        assert fl.last_error() == "", fl.last_error()
+        return buf_out.numpy()[off:off + n].reshape(rhs.shape)
+
+    fast = run(False)
+    gen = run(True)
+    e2, ei = rel_err(fast, gen)
+    assert e2 <= TOL and ei <= TOL, ("fast vs generic", e2, ei)
+    ora = orc.solve(list(nglob), bc, 1, h, L, 1, kernel, rhs)
+    e2, ei = rel_err(fast, ora)
+    assert e2 <= TOL and ei <= TOL, ("fast vs oracle", e2, ei)
