@@ -16,6 +16,16 @@
 //   RODFT10 = reversed REDFT10 of (-1)^n x, RODFT01/11 = (-1)^k REDFT01/11 of reversed x.
 // ROWS pencils per block, H/8 threads per pencil; the pencil is staged through
 // LDS so that HBM is read and written in 16-byte pieces when rows are aligned.
+//
+// col_real: r2c / c2r of length n (a power of two in [32, 1024]) along a strided
+// direction whose pencils are neighbours along a unit-stride dimension (the
+// y or z passes of the general path, x fastest). Two neighbouring real pencils
+// a, b are one complex column a + i b: Z = FFT_n(a + i b), A[k] = (Z[k] + conj
+// Z[n-k]) / 2, B[k] = (Z[k] - conj Z[n-k]) / (2i) for k <= n/2; the inverse
+// builds Z[k] = A[k] + i B[k] (conj A[n-k] + i conj B[n-k] above n/2; the
+// imaginary parts of the DC and Nyquist terms are ignored, as FFTW's c2r does)
+// and a + i b = IFFT_n(Z). TX pairs side by side per block: each row of a
+// tile is one contiguous run of 2 TX reals or complex values.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -157,6 +167,95 @@ __global__ void __launch_bounds__(ROWS * H / 8) row_r2r(PassDesc P) {
     }
 }
 
+template <int M, int TX, bool FWD>
+__global__ void __launch_bounds__(TX * M / 8) col_real(PassDesc P) {
+    constexpr int T = M / 8, NH = M / 2;
+    __shared__ C2<double> lds[(M + 2) * TX];
+    const int c = threadIdx.x % TX, j = threadIdx.x / TX;
+    const int npair = (P.na + 1) / 2;
+    const int ntile = (npair + TX - 1) / TX;
+    const int tile = blockIdx.x % ntile, jb = blockIdx.x / ntile;
+    const int pr = tile * TX + c;
+    const bool la = 2 * pr < P.na, lb = 2 * pr + 1 < P.na;
+    const int xa = la ? 2 * pr : 2 * (npair - 1);   // dead pairs read the last pair and store nothing
+    const int d = P.d;
+    const long long sr = FWD ? P.src_s[d] : P.dst_s[d];       // row stride of the real array
+    const long long sc = FWD ? P.dst_s[d] : P.src_s[d];       // row stride of the complex array
+    const double* rsrc = P.src + (long long)jb * P.src_s[P.fb] + xa;              // FWD: real source
+    double* rdst = P.dst + (long long)jb * P.dst_s[P.fb] + xa;                    // !FWD: real destination
+    const double* csrc = P.src + 2 * ((long long)jb * P.src_s[P.fb] + xa);       // !FWD: complex source
+    double* cdst = P.dst + 2 * ((long long)jb * P.dst_s[P.fb] + xa);             // FWD: complex destination
+    const bool hasb = lb;   // a real partner column exists
+    const C2<double>* tw = reinterpret_cast<const C2<double>*>(P.tw);
+    C2<double> v[8];
+    if (FWD) {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const double* p = rsrc + (long long)(j + i * T) * sr;
+            if (P.vec) {
+                const double2 q = *reinterpret_cast<const double2*>(p);
+                v[i] = {q.x, q.y};
+            } else {
+                v[i] = {p[0], hasb ? p[1] : 0.0};
+            }
+        }
+        column_fft<double, M, true, TX>(v, lds + c, j, tw);
+        __syncthreads();   // the FFT's last LDS reads are done
+#pragma unroll
+        for (int i = 0; i < 8; ++i) lds[(j + i * T) * TX + c] = v[i];
+        __syncthreads();
+        if (!la) return;
+#pragma unroll
+        for (int i = 0; i < 5; ++i) {
+            const int k = j + i * T;
+            if (i == 4 && j != 0) break;   // k = n/2 once
+            const C2<double> zk = lds[k * TX + c], zc = cconj(lds[((M - k) & (M - 1)) * TX + c]);
+            const C2<double> A = {0.5 * (zk.x + zc.x), 0.5 * (zk.y + zc.y)};
+            const C2<double> df = {0.5 * (zk.x - zc.x), 0.5 * (zk.y - zc.y)};
+            double* o = cdst + 2 * (long long)k * sc;
+            *reinterpret_cast<double2*>(o) = double2{A.x, A.y};
+            if (hasb) *reinterpret_cast<double2*>(o + 2) = double2{df.y, -df.x};   // (zk - zc) / (2i)
+        }
+    } else {
+        // rows 0..n/2 of the two spectra into LDS: A at row k, B at row n/2 + 1 + k
+#pragma unroll
+        for (int i = 0; i < 5; ++i) {
+            const int k = j + i * T;
+            if (i == 4 && j != 0) break;
+            const double* p = csrc + 2 * (long long)k * sc;
+            const double2 qa = *reinterpret_cast<const double2*>(p);
+            double2 qb = hasb ? *reinterpret_cast<const double2*>(p + 2) : double2{0.0, 0.0};
+            C2<double> A = {qa.x, qa.y}, B = {qb.x, qb.y};
+            if (k == 0 || k == NH) {   // DC and Nyquist are real
+                A.y = 0.0;
+                B.y = 0.0;
+            }
+            lds[k * TX + c] = A;
+            lds[(NH + 1 + k) * TX + c] = B;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const int k = j + i * T;
+            const int m = k <= NH ? k : M - k;
+            const C2<double> A = lds[m * TX + c], B = lds[(NH + 1 + m) * TX + c];
+            v[i] = k <= NH ? C2<double>{A.x - B.y, A.y + B.x} : C2<double>{A.x + B.y, B.x - A.y};
+        }
+        column_fft<double, M, false, TX>(v, lds + c, j, tw);   // syncs before its first LDS write
+        if (!la) return;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            double* p = rdst + (long long)(j + i * T) * sr;
+            if (P.vec) {
+                *reinterpret_cast<double2*>(p) = double2{v[i].x, v[i].y};
+            } else {
+                p[0] = v[i].x;
+                if (hasb) p[1] = v[i].y;
+            }
+        }
+    }
+}
+
 template <int H, int ROWS>
 void launch_row_h(const PassDesc& P, hipStream_t s) {
     const int npen = P.n1 * P.n2;
@@ -171,6 +270,16 @@ void launch_row_h(const PassDesc& P, hipStream_t s) {
         case RODFT11: hipLaunchKernelGGL((row_r2r<H, ROWS, RODFT11>), grid, block, 0, s, P); break;
         default: fail("row r2r: kind %d", P.kind);
     }
+}
+
+template <int M, int TX>
+void launch_col_real_m(const PassDesc& P, hipStream_t s) {
+    const int npair = (P.na + 1) / 2;
+    const int ntile = (npair + TX - 1) / TX;
+    const int nb = P.n1 * P.n2 / P.na;
+    dim3 grid(ntile * nb);
+    if (P.op == OP_R2C) hipLaunchKernelGGL((col_real<M, TX, true>), grid, dim3(TX * M / 8), 0, s, P);
+    else hipLaunchKernelGGL((col_real<M, TX, false>), grid, dim3(TX * M / 8), 0, s, P);
 }
 
 }  // namespace
@@ -210,6 +319,36 @@ std::vector<double> row_r2r_table(int kind, int n) {
         }
     }
     return t;
+}
+
+bool col_real_fits(const PassDesc& P) {
+    const int n = P.n;
+    if (!(P.op == OP_R2C || P.op == OP_C2R) || n < 32 || n > 1024 || (n & (n - 1))) return false;
+    const bool fwd = P.op == OP_R2C;
+    const int d = P.d, fa = P.fa;
+    if (fa == d || P.src_s[fa] != 1 || P.dst_s[fa] != 1) return false;
+    // the real side may carry the two padding reals of the in-place r2c layout (n + 2 per
+    // pencil): only rows below n are data, the rest is neither read nor written here
+    if (fwd && !(P.nf_in == 1 && P.nf_out == 2 && P.src_nf == 1 && P.dst_nf == 2 && P.pen_in >= n && P.pen_out == n / 2 + 1 &&
+                 P.src_nd >= n && P.dst_nd == n / 2 + 1))
+        return false;
+    if (!fwd && !(P.nf_in == 2 && P.nf_out == 1 && P.src_nf == 2 && P.dst_nf == 1 && P.pen_in == n / 2 + 1 && P.pen_out >= n &&
+                  P.src_nd == n / 2 + 1 && P.dst_nd >= n))
+        return false;
+    return P.si == 0 && P.so == 0 && P.gshift == 0 && P.sshift == 0 && P.post == 0 && (P.n1 * P.n2) % P.na == 0;
+}
+
+void launch_col_real(const PassDesc& P, hipStream_t s) {
+    switch (P.n) {
+        case 32: launch_col_real_m<32, 16>(P, s); break;
+        case 64: launch_col_real_m<64, 16>(P, s); break;
+        case 128: launch_col_real_m<128, 8>(P, s); break;
+        case 256: launch_col_real_m<256, 8>(P, s); break;
+        case 512: launch_col_real_m<512, 4>(P, s); break;     // LDS (M + 2) TX complex: 33 KB at most
+        case 1024: launch_col_real_m<1024, 2>(P, s); break;
+        default: fail("col real: length %d", P.n);
+    }
+    HIP_CHECK(hipGetLastError());
 }
 
 void launch_row_r2r(const PassDesc& P, hipStream_t s) {

@@ -23,6 +23,8 @@ size_t generic_lds_bytes(const PassDesc& P, int ppb);
 bool generic_alias(const PassDesc& P);
 void launch_generic(const PassDesc& P, hipStream_t s);
 bool row_r2r_fits(const PassDesc& P);
+bool col_real_fits(const PassDesc& P);
+void launch_col_real(const PassDesc& P, hipStream_t s);
 std::vector<double> row_r2r_table(int kind, int n);
 void launch_row_r2r(const PassDesc& P, hipStream_t s);
 FastPlan* make_fast_plan(SolverImpl& s);   // nullptr if the fused pipeline does not apply
@@ -326,6 +328,17 @@ PassDesc SolverImpl::pass_desc(const DirPlan& p, int comp, bool forward, const V
                 P.dst_s[fa] % 2 == 0 && P.dst_s[fb] % 2 == 0;
         return P;
     }
+    // real-to-complex pencils along a strided direction, neighbours along a unit-stride one:
+    // pairs of real pencils as one complex column
+    if (!std::getenv("FLUPS_GENERIC_PASS") && col_real_fits(P)) {
+        P.fastk = 2;
+        P.tw = twiddles(P.n);
+        const bool fwd = P.op == OP_R2C;
+        const double* real = fwd ? P.src : P.dst;
+        const long long* rs = fwd ? P.src_s : P.dst_s;
+        P.vec = ((uintptr_t)real % 16 == 0) && rs[P.d] % 2 == 0 && rs[P.fb] % 2 == 0 && P.na % 2 == 0;
+        return P;
+    }
     P.alias = generic_alias(P) ? 1 : 0;
     const size_t per_pen = generic_lds_bytes(P, 1);
     FLUPS_REQUIRE(per_pen <= lds_budget_, "pencil of %d elements (transform length %d) does not fit the general path's LDS",
@@ -340,6 +353,7 @@ PassDesc SolverImpl::pass_desc(const DirPlan& p, int comp, bool forward, const V
 
 void SolverImpl::run_pass(PassDesc P) {
     if (P.fastk == 1) launch_row_r2r(P, stream_);
+    else if (P.fastk == 2) launch_col_real(P, stream_);
     else launch_generic(P, stream_);
 }
 

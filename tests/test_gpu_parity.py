@@ -528,10 +528,16 @@ ROW_CASES = [
     ((32, 32, 32), [[1, 0], [4, 4], [3, 3]], 1, 0),
     ((128, 16, 16), [[0, 1], [1, 0], [0, 0]], 1, 0),
     ((64, 24, 16), [[1, 1], [0, 1], [4, 4]], 1, 4),
+    # periodic y or z after symmetric directions: r2c / c2r of paired real columns
+    ((32, 64, 32), [[0, 0], [3, 3], [4, 4]], 1, 2),
+    ((33, 64, 16), [[1, 1], [3, 3], [4, 4]], 1, 0),
+    ((64, 32, 128), [[0, 1], [1, 0], [3, 3]], 1, 0),
+    ((256, 256, 8), [[0, 1], [3, 3], [0, 0]], 1, 0),
 ]
 
 
-@pytest.mark.parametrize("case", ROW_CASES, ids=["ee", "oo", "eo-256", "oe", "sym3", "oo-mix"])
+@pytest.mark.parametrize("case", ROW_CASES, ids=["ee", "oo", "eo-256", "oe", "sym3", "oo-mix", "ee-pery", "odd-x-pery",
+                                                 "perz-128", "pery-256"])
 @pytest.mark.parametrize("misalign", [False, True])
 def test_row_r2r_kernel_vs_generic_and_oracle(case, misalign, monkeypatch):
     """the row kernel of the contiguous real-to-real passes (one half-length complex FFT with pre-
