@@ -65,6 +65,7 @@ struct ColArgs {
     int nopair;          // z_fused: 1 = the other dim has no mirror rows (one row per block, G stored per row)
     int x0;              // z_fused / z_rot: spectral x index of column 0 (slab offset, a multiple of TX)
     int gntx;            // z_fused / z_rot: x tiles of one ky' row of the compressed G
+    int gtx;             // z_rot: tile width of the compressed G (a multiple of the kernel's TX)
 };
 
 // TX columns per block, M/8 threads per column; block = TX * M/8 threads.
@@ -200,24 +201,27 @@ __global__ void __launch_bounds__(TX * M / 8) z_fused(ColArgs a) {
 // and Green's slice as z_fused. src/dothemagic_rot.ipp:102-349.
 struct RotK {
     double kfact[3], h[3], symstart[3];
-    int order;   // 1 spectral, 2 / 4 / 6 centred differences
+    int order;          // 1 spectral, 2 / 4 / 6 centred differences
+    const void* sym;    // derivative symbols by mode index: x (nx), then y (M1), then z (M2), type R
+    int off[3];         // start of each direction's table
 };
 
-template <typename R>
-__device__ __forceinline__ R dsym(const RotK& k, int d, int idx) {
-    // symmetrised mode index (mirrored half negative), then the derivative symbol
+// derivative symbol of mode idx along d: symmetrised index (mirrored half negative),
+// k = is * kfact, then k itself (spectral) or the symbol of the 2nd/4th/6th-order
+// centred difference (src/dothemagic_rot.ipp:14-101); tabulated on the host
+double rot_symbol(const RotK& k, int d, int idx) {
     const double ss = k.symstart[d];
     int is = idx;
     if (ss != 0.0 && idx > ss) {
-        const int v = (int)fabs(2.0 * ss - idx);
+        const int v = (int)std::fabs(2.0 * ss - idx);
         is = -(v > 1 ? v : 1);
     }
     const double kk = is * k.kfact[d], hh = k.h[d];
     switch (k.order) {
-        case 2: return (R)(sin(kk * hh) / hh);
-        case 4: return (R)((4.0 / 3.0) * sin(kk * hh) / hh - (1.0 / 6.0) * sin(2.0 * kk * hh) / hh);
-        case 6: return (R)(1.5 * sin(kk * hh) / hh - 0.3 * sin(2.0 * kk * hh) / hh + (1.0 / 30.0) * sin(3.0 * kk * hh) / hh);
-        default: return (R)kk;
+        case 2: return std::sin(kk * hh) / hh;
+        case 4: return (4.0 / 3.0) * std::sin(kk * hh) / hh - (1.0 / 6.0) * std::sin(2.0 * kk * hh) / hh;
+        case 6: return 1.5 * std::sin(kk * hh) / hh - 0.3 * std::sin(2.0 * kk * hh) / hh + (1.0 / 30.0) * std::sin(3.0 * kk * hh) / hh;
+        default: return kk;
     }
 }
 
@@ -239,20 +243,24 @@ __global__ void __launch_bounds__(TX * M / 8) z_rot(ColArgs a, RotK rk) {
     const size_t cstride = (size_t)sizeof(C2<R>) * a.comp_src;
     const size_t rstride = (size_t)sizeof(C2<R>) * T * a.s_col;
     const unsigned voff = (unsigned)(sizeof(C2<R>) * ((size_t)j * a.s_col + c));
+    // the block's TX columns lie in one tile of the G layout (gtx a multiple of TX)
+    const int xg = a.x0 + tile * TX;
+    const int cg = xg % a.gtx + c;
     const char* g = reinterpret_cast<const char*>(a.green) +
-                    sizeof(R) * ((long long)y0 * a.gntx + a.x0 / TX + tile) * (M / 2 + 1) * TX;
+                    sizeof(R) * ((long long)y0 * a.gntx + xg / a.gtx) * (M / 2 + 1) * a.gtx;
     const C2<R>* tw = reinterpret_cast<const C2<R>*>(a.tw);
     R gv[8];
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
         const int kz = j + i * T;
-        gv[i] = *reinterpret_cast<const R*>(g + (unsigned)(sizeof(R) * (min(kz, M - kz) * TX + c)));
+        gv[i] = *reinterpret_cast<const R*>(g + (unsigned)(sizeof(R) * (min(kz, M - kz) * a.gtx + cg)));
     }
-    const R kx = dsym<R>(rk, 0, a.x0 + x);
+    const R* sym = reinterpret_cast<const R*>(rk.sym);
+    const R kx = sym[rk.off[0] + a.x0 + min(x, a.nx - 1)];
     for (int s = 0; s < 2; ++s) {
         const int y = s == 0 ? y0 : y1;
         char* b = base + (size_t)sizeof(C2<R>) * ((long long)y * a.s_other_src);
-        const R ky = dsym<R>(rk, 1, y);
+        const R ky = sym[rk.off[1] + y];
         C2<R> v[3][8];
 #pragma unroll
         for (int cc = 0; cc < 3; ++cc) {
@@ -266,7 +274,7 @@ __global__ void __launch_bounds__(TX * M / 8) z_rot(ColArgs a, RotK rk) {
         }
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
-            const R kz = dsym<R>(rk, 2, j + i * T);
+            const R kz = sym[rk.off[2] + j + i * T];
             const C2<R> wx = {v[0][i].x * gv[i], v[0][i].y * gv[i]};
             const C2<R> wy = {v[1][i].x * gv[i], v[1][i].y * gv[i]};
             const C2<R> wz = {v[2][i].x * gv[i], v[2][i].y * gv[i]};
@@ -631,6 +639,26 @@ void build_fast_plan(SolverImpl& s, FastPlan* f, size_t es, int align) {
     f->tw[1] = upload(f->M[1]);
     f->tw[2] = upload(f->M[2]);
     f->twx = upload(f->M[0]);
+    if (f->lda == 3) {   // derivative symbols of the rotational pass
+        const int nsym[3] = {f->nx, f->M[1], f->M[2]};
+        std::vector<double> t;
+        for (int d = 0; d < 3; ++d) {
+            f->rot.off[d] = (int)t.size();
+            for (int i = 0; i < nsym[d]; ++i) t.push_back(rot_symbol(f->rot, d, i));
+        }
+        void* d = nullptr;
+        if (f->fp32) {
+            std::vector<float> tf(t.begin(), t.end());
+            HIP_CHECK(hipMalloc(&d, tf.size() * sizeof(float)));
+            HIP_CHECK(hipMemcpy(d, tf.data(), tf.size() * sizeof(float), hipMemcpyHostToDevice));
+        } else {
+            HIP_CHECK(hipMalloc(&d, t.size() * sizeof(double)));
+            HIP_CHECK(hipMemcpy(d, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice));
+        }
+        f->owned.push_back(d);
+        f->dev_bytes += t.size() * (f->fp32 ? sizeof(float) : sizeof(double));
+        f->rot.sym = d;
+    }
     f->ev.assign(3 + 3 * (f->slab.size() - 1), nullptr);
     for (auto& e : f->ev) HIP_CHECK(hipEventCreate(&e));
     {
@@ -844,9 +872,13 @@ void run_fast(FastPlan* f, const double* rhs, double* field, const View& uin, co
         z.x0 = x0;
         z.gntx = (f->nx + f->gtx - 1) / f->gtx;
         if (type == 1) {
-            // rotational pass: all three components per block, tile width of the Green's layout
+            // rotational pass: all three components per block, tile width of the Green's layout;
+            // M2 = 1024: half of it (256 threads per block for FP64, 512 for FP32), so that the
+            // three components' registers fit without spilling
             FLUPS_REQUIRE(f->lda == 3, "the rotational solve needs lda = 3");
-            if (f->gtx == TX) launch_rot<R, TX>(M2, z, f->rot, st);
+            z.gtx = f->gtx;
+            if (M2 == 1024 && f->gtx == TX / 2) launch_rot<R, TX / 4>(M2, z, f->rot, st);
+            else if (f->gtx == TX) launch_rot<R, TX>(M2, z, f->rot, st);
             else if (f->gtx == TX / 2) launch_rot<R, TX / 2>(M2, z, f->rot, st);
             else launch_rot<R, (TX >= 8 ? TX / 4 : 1)>(M2, z, f->rot, st);
         } else {

@@ -402,6 +402,32 @@ def test_fused_rot_vs_oracle(case):
     assert e2 <= TOL and ei <= TOL, (e2, ei)
 
 
+@pytest.mark.parametrize("fp32", [False, True])
+@pytest.mark.parametrize("odiff", [1, 4])
+def test_fused_rot_long_z_vs_oracle(fp32, odiff):
+    """the rotational pass at M2 = 1024 (tiles narrower than the Green's layout, symbols from the
+    host tables): oracle parity at 1e-12 (FP32: 1e-5)"""
+    nglob = (16, 16, 512)
+    rng = np.random.default_rng(77 + odiff)
+    rhs = rng.normal(size=(3, 512, 16, 16))
+    h = [1.0 / 512] * 3
+    L = [h[0] * n for n in nglob]
+    bc = [[4, 4]] * 3
+    topo = fl.Topology(0, 3, list(nglob), [1, 1, 1], False)
+    s = fl.Solver(topo, bc, h, L, order_diff=odiff)
+    s.set_green_type(4)
+    if fp32:
+        s.set_precision(True)
+    s.setup()
+    assert s.path == (2 if fp32 else 1)
+    f_gpu = np.zeros_like(rhs)
+    s.solve(f_gpu, rhs, 1)
+    f_ora = orc.solve(list(nglob), bc, 3, h, L, 1, 4, rhs, typ=1, odiff=odiff)
+    e2, ei = rel_err(f_gpu, f_ora)
+    tol = 1e-5 if fp32 else TOL
+    assert e2 <= tol and ei <= tol, (e2, ei)
+
+
 PER_CASES = [
     # (nglob, bc, kernel, lda)
     ((64, 32, 32), [[3, 3], [4, 4], [4, 4]], 2, 1),
