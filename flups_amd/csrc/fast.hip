@@ -226,8 +226,7 @@ double rot_symbol(const RotK& k, int d, int idx) {
 }
 
 template <typename R, int M, int TX>
-__global__ void __launch_bounds__(TX * M / 8) z_rot(ColArgs a, RotK rk) {
-    __shared__ C2<R> lds[M * TX];
+__device__ __forceinline__ void z_rot_body(const ColArgs& a, const RotK& rk, C2<R>* lds) {
     constexpr int T = M / 8;
     const int c = threadIdx.x % TX;
     const int j = threadIdx.x / TX;
@@ -243,18 +242,14 @@ __global__ void __launch_bounds__(TX * M / 8) z_rot(ColArgs a, RotK rk) {
     const size_t cstride = (size_t)sizeof(C2<R>) * a.comp_src;
     const size_t rstride = (size_t)sizeof(C2<R>) * T * a.s_col;
     const unsigned voff = (unsigned)(sizeof(C2<R>) * ((size_t)j * a.s_col + c));
+    // lanes past nx load the last valid column and store nothing
+    const unsigned loff = (unsigned)(sizeof(C2<R>) * ((size_t)j * a.s_col + min(c, a.nx - 1 - tile * TX)));
     // the block's TX columns lie in one tile of the G layout (gtx a multiple of TX)
     const int xg = a.x0 + tile * TX;
     const int cg = xg % a.gtx + c;
     const char* g = reinterpret_cast<const char*>(a.green) +
                     sizeof(R) * ((long long)y0 * a.gntx + xg / a.gtx) * (M / 2 + 1) * a.gtx;
     const C2<R>* tw = reinterpret_cast<const C2<R>*>(a.tw);
-    R gv[8];
-#pragma unroll
-    for (int i = 0; i < 8; ++i) {
-        const int kz = j + i * T;
-        gv[i] = *reinterpret_cast<const R*>(g + (unsigned)(sizeof(R) * (min(kz, M - kz) * a.gtx + cg)));
-    }
     const R* sym = reinterpret_cast<const R*>(rk.sym);
     const R kx = sym[rk.off[0] + a.x0 + min(x, a.nx - 1)];
     for (int s = 0; s < 2; ++s) {
@@ -267,10 +262,16 @@ __global__ void __launch_bounds__(TX * M / 8) z_rot(ColArgs a, RotK rk) {
 #pragma unroll
             for (int i = 0; i < 8; ++i) {
                 const int row = j + i * T;
-                v[cc][i] = (live && row < a.nin) ? *reinterpret_cast<const C2<R>*>(b + cc * cstride + i * rstride + voff)
-                                                 : C2<R>{0, 0};
+                v[cc][i] = row < a.nin ? *reinterpret_cast<const C2<R>*>(b + cc * cstride + i * rstride + loff) : C2<R>{0, 0};
             }
             column_fft<R, M, true, TX>(v[cc], lds + c, j, tw);
+        }
+        // G of this tile, loaded after the forward transforms (fewer registers live during them)
+        R gv[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const int kz = j + i * T;
+            gv[i] = *reinterpret_cast<const R*>(g + (unsigned)(sizeof(R) * (min(kz, M - kz) * a.gtx + cg)));
         }
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
@@ -299,6 +300,15 @@ __global__ void __launch_bounds__(TX * M / 8) z_rot(ColArgs a, RotK rk) {
         }
         if (!pair) break;   // uniform over the block
     }
+}
+
+// at most 256 registers per lane (two waves per SIMD): the three components need 250-285
+// (FP64), and the few that spill cost less than halving the waves (513^3 x 3 solve: 34.0 ->
+// 29.8 ms)
+template <typename R, int M, int TX>
+__global__ void __launch_bounds__(TX * M / 8) __attribute__((amdgpu_waves_per_eu(2))) z_rot(ColArgs a, RotK rk) {
+    __shared__ C2<R> lds[M * TX];
+    z_rot_body<R, M, TX>(a, rk, lds);
 }
 
 // ---------------------------------------------------------------------------
