@@ -375,34 +375,77 @@ __device__ int poly_roots(const double* a, int deg, cz* r) {
         }
     return deg;
 }
-__device__ double lgf1d_eval(const GreenArgs& A, int order, int n, double c) {
+// roots and residues of the 1-D LGF for one shift c (they do not depend on n)
+struct Lgf1d {
+    int nr;
+    bool c0;
+    cz L[4], Ai[4];
+};
+__device__ Lgf1d lgf1d_setup(const GreenArgs& A, int order, double c) {
     const int w = order / 2;
     const double* s = A.lgf_s + 5 * (w - 1);   // s_0..s_w
     const double* p = A.lgf_p + 5 * (w - 1);   // p_0 (=0)..p_w
     double a[5];
     cz mu[4];
-    int nr;
-    if (c == 0.0) {
+    Lgf1d R;
+    R.c0 = c == 0.0;
+    if (R.c0) {
         for (int j = 1; j <= w; ++j) a[j - 1] = p[j];
-        nr = poly_roots(a, w - 1, mu);
+        R.nr = poly_roots(a, w - 1, mu);
     } else {
         a[0] = c;
         for (int j = 1; j <= w; ++j) a[j] = p[j];
-        nr = poly_roots(a, w, mu);
+        R.nr = poly_roots(a, w, mu);
     }
-    const double an = fabs((double)n);
-    double g = (c == 0.0) ? -0.5 * an : 0.0;
-    for (int i = 0; i < nr; ++i) {
+    for (int i = 0; i < R.nr; ++i) {
         cz L = 2.0 * casinh_(csqrt_(0.5 * mu[i]));
         if (L.x < 0.0) L = {-L.x, -L.y};
         cz den = {0.0, 0.0};
         for (int m = 1; m <= w; ++m) den = den + (m * s[m]) * csinh_((double)m * L);
-        const cz Ai = cdiv({-0.5, 0.0}, den);
-        cz term = Ai * cexp_((-an) * L);
-        if (c == 0.0) term = term - Ai;
+        R.L[i] = L;
+        R.Ai[i] = cdiv({-0.5, 0.0}, den);
+    }
+    return R;
+}
+__device__ double lgf1d_at(const Lgf1d& R, int n) {
+    const double an = fabs((double)n);
+    double g = R.c0 ? -0.5 * an : 0.0;
+    for (int i = 0; i < R.nr; ++i) {
+        cz term = R.Ai[i] * cexp_((-an) * R.L[i]);
+        if (R.c0) term = term - R.Ai[i];
         g += term.x;
     }
     return g;
+}
+__device__ double lgf1d_eval(const GreenArgs& A, int order, int n, double c) { return lgf1d_at(lgf1d_setup(A, order, c), n); }
+
+// 1-unbounded LGF kernels: one thread per column along the unbounded direction ud (the
+// roots depend on the transverse wave numbers only), then the column's values for every n
+__global__ void fill_1unb_lgf(double* g, FillGeom geo, GreenArgs A, int ud) {
+    const int da = (ud + 1) % 3, db = (ud + 2) % 3;
+    const long long ncol = (long long)geo.n[da] * geo.n[db];
+    const long long sd[3] = {1, geo.n[0], (long long)geo.n[0] * geo.n[1]};
+    const double h = A.hfact[0] + A.hfact[1] + A.hfact[2];
+    const int ord = lgf_order(A.type);
+    for (long long col = blockIdx.x * (long long)blockDim.x + threadIdx.x; col < ncol; col += (long long)gridDim.x * blockDim.x) {
+        int i[3];
+        i[da] = (int)(col % geo.n[da]);
+        i[db] = (int)(col / geo.n[da]);
+        i[ud] = 0;
+        double c = 0.0;
+        for (int dd = 0; dd < 3; ++dd) {
+            const int is = symid(i[dd], A.symstart[dd]);
+            c += lgf_symbol(ord, (is + A.koffset[dd]) * A.kfact[dd] * h);
+        }
+        const Lgf1d R = lgf1d_setup(A, ord, c);
+        const long long e0 = i[da] * sd[da] + i[db] * sd[db];
+        for (int iu = 0; iu < geo.n[ud]; ++iu) {
+            const double x = symid(iu, A.symstart[ud]) * A.hfact[ud];
+            const double r = sqrt(x * x);
+            const int nn = (int)round(fabs(r) / h);
+            g[(e0 + iu * sd[ud]) * geo.nf] = -h * lgf1d_at(R, nn);
+        }
+    }
 }
 
 // 1-D lattice Green's functions of the Mehrstellen stencils along the unbounded direction
@@ -642,7 +685,11 @@ void launch_green_fill(int nunb, double* g, const int n[3], int nf, int axis, co
     const unsigned gb = grid_for(tot);
     if (nunb == 3) hipLaunchKernelGGL(fill_3unb, dim3(gb), dim3(256), 0, s, g, geo, A);
     else if (nunb == 2) hipLaunchKernelGGL(fill_2unb, dim3(gb), dim3(256), 0, s, g, geo, A);
-    else if (nunb == 1) hipLaunchKernelGGL(fill_1unb, dim3(gb), dim3(256), 0, s, g, geo, A);
+    else if (nunb == 1 && (A.type == LGF_2 || A.type == LGF_4 || A.type == LGF_6 || A.type == LGF_8)) {
+        const int ud = A.hfact[0] != 0.0 ? 0 : A.hfact[1] != 0.0 ? 1 : 2;   // the unbounded direction
+        const size_t ncol = tot / (size_t)n[ud];
+        hipLaunchKernelGGL(fill_1unb_lgf, dim3(grid_for(ncol)), dim3(256), 0, s, g, geo, A, ud);
+    } else if (nunb == 1) hipLaunchKernelGGL(fill_1unb, dim3(gb), dim3(256), 0, s, g, geo, A);
     else hipLaunchKernelGGL(fill_0unb, dim3(gb), dim3(256), 0, s, g, geo, A, lo ? lo[0] : 0, lo ? lo[1] : 0, lo ? lo[2] : 0);
     HIP_CHECK(hipGetLastError());
 }
