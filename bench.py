@@ -184,10 +184,13 @@ def rhs_ring(n, seed, h):
     return out
 
 
-def run_config(fl, name, nglob, bc, center, kernel, lda, typ, odiff, rhs, h, L, steps, warmup):
+def run_config(fl, name, nglob, bc, center, kernel, lda, typ, odiff, rhs, h, L, steps, warmup, fp32=False, ref=None):
+    """time one workload; with `ref` (the FP64 result on the device) also its relative L2 / Linf error against it"""
     topo = fl.Topology(0, lda, list(nglob), [1, 1, 1], False)
     s = fl.Solver(topo, bc, h, L, order_diff=odiff, center_type=[center] * 3)
     s.set_green_type(kernel)
+    if fp32:
+        s.set_precision(True)
     t0 = time.time()
     s.setup()
     setup_s = time.time() - t0
@@ -200,24 +203,115 @@ def run_config(fl, name, nglob, bc, center, kernel, lda, typ, odiff, rhs, h, L, 
         s.solve(dout, din, typ)
     el = (time.perf_counter() - t0) / steps
     by = s.solve_bytes(typ)
-    return {"workload": name, "solves_per_s": round(1.0 / el, 3), "ms_per_step": round(el * 1e3, 3),
-            "algorithmic_GB": round(by / 1e9, 3), "achieved_GBs": round(by / el / 1e9, 1),
-            "roofline_frac": round(by / el / PEAK_HBM, 4), "path": ["general", "fused-fp64", "fused-fp32", "general+fused-z"][s.path],
-            "setup_s": round(setup_s, 2), "steps": steps}
+    out = {"workload": name, "solves_per_s": round(1.0 / el, 3), "ms_per_step": round(el * 1e3, 3),
+           "algorithmic_GB": round(by / 1e9, 3), "achieved_GBs": round(by / el / 1e9, 1),
+           "roofline_frac": round(by / el / PEAK_HBM, 4), "path": ["general", "fused-fp64", "fused-fp32", "general+fused-z"][s.path],
+           "setup_s": round(setup_s, 2), "steps": steps}
+    if ref is not None:
+        a, b = dout.numpy(), ref.numpy()
+        out["rel_l2_vs_fp64"] = float(np.sqrt(np.sum((a - b) ** 2) / np.sum(b * b)))
+        out["rel_linf_vs_fp64"] = float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+        del a, b
+    del s
+    return out, dout
+
+
+def blob_rhs(shape, seed):
+    """lda components, each a sum of 4 Gaussian blobs (centres U[0.3,0.7] of the box, sigma U[0.03,0.06],
+    amplitudes N(0,1)) on the cell grid of a box whose longest side is 1"""
+    lda, n2, n1, n0 = shape
+    rng = np.random.default_rng(seed)
+    n = max(n0, n1, n2)
+    xs = [(np.arange(m) + 0.5) / n for m in (n0, n1, n2)]
+    out = np.zeros(shape)
+    for c in range(lda):
+        for _ in range(4):
+            cen = rng.uniform(0.3, 0.7, 3) * np.array([n0, n1, n2]) / n
+            sg = rng.uniform(0.03, 0.06)
+            g = [np.exp(-(xs[d] - cen[d]) ** 2 / (2 * sg * sg)) for d in range(3)]
+            out[c] += rng.normal() * g[2][:, None, None] * g[1][None, :, None] * g[0][None, None, :]
+    return out
+
+
+def batch_config(fl, nprob, seed, reps):
+    """independent problems solved by one hipGraph launch (flups_batch_run) against the plain path, the same
+    solves one after the other with a synchronisation each (flups_batch_run_sequential)"""
+    rng = np.random.default_rng(seed)
+    sizes = [32, 48, 64, 80, 96, 112, 128]
+    tpls = {"unbounded": [[4, 4]] * 3, "periodic-x/unbounded-yz": [[3, 3], [4, 4], [4, 4]],
+            "symmetric-x/periodic-y/unbounded-z": [[0, 1], [3, 3], [4, 4]], "periodic": [[3, 3]] * 3}
+    names = list(tpls)
+    reqs, dins, douts, kinds = [], [], [], {}
+    for p in range(nprob):
+        ng = [int(rng.choice(sizes)) for _ in range(3)]
+        while len(set(ng)) == 1:   # unequal axes
+            ng = [int(rng.choice(sizes)) for _ in range(3)]
+        tpl = names[int(rng.integers(len(names)))]
+        lda = 3 if rng.random() < 0.25 else 1
+        fp32 = bool(rng.random() < 0.5)
+        typ = 1 if lda == 3 else 0   # three-component problems are Biot-Savart (ROT, spectral derivative)
+        n = max(ng)
+        h = [1.0 / n] * 3
+        L = [h[0] * m for m in ng]
+        rhs = blob_rhs((lda, ng[2], ng[1], ng[0]), seed * 1000 + p)
+        if tpl == "periodic":
+            rhs -= rhs.mean(axis=(1, 2, 3), keepdims=True)
+        elif tpl.startswith("symmetric"):
+            rhs -= rhs.mean(axis=2, keepdims=True)
+        reqs.append(fl.ServeRequest.make(ng, tpls[tpl], h, L, green=2, lda=lda, fp32=fp32, solver_type=typ,
+                                         order_diff=1 if typ else 0))
+        dins.append(fl.DeviceArray.from_numpy(rhs))
+        douts.append(fl.DeviceArray(rhs.shape))
+        key = "%s/lda%d/%s" % (tpl, lda, "fp32" if fp32 else "fp64")
+        kinds[key] = kinds.get(key, 0) + 1
+    t0 = time.time()
+    b = fl.Batch(reqs, dins, douts)
+    setup_s = time.time() - t0
+    for _ in range(2):
+        b.run()
+        b.run_sequential()
+    t0 = time.perf_counter()
+    ev = []
+    for _ in range(reps):
+        ev.append(b.run())
+    graph = (time.perf_counter() - t0) / reps
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        b.run_sequential()
+    seq = (time.perf_counter() - t0) / reps
+    out = {"workload": "%d independent problems, shapes from {32..128}^3 with unequal axes, 4 BC templates, lda 1 or 3 "
+                       "(lda 3: ROT), FP32 or FP64, HEJ_2, Gaussian-blob right-hand sides" % nprob,
+           "problems": nprob, "mix": kinds, "graph_ms_per_batch": round(graph * 1e3, 3),
+           "graph_device_ms": round(float(np.median(ev)), 3), "solves_per_s_graph": round(nprob / graph, 1),
+           "sequential_ms_per_batch": round(seq * 1e3, 3), "solves_per_s_sequential": round(nprob / seq, 1),
+           "speedup": round(seq / graph, 3), "setup_s": round(setup_s, 2), "reps": reps,
+           "device_MB": round(b.device_bytes() / 1e6, 1)}
+    b.close()
+    return out
 
 
 def other_configs(fl, which):
     out = {}
     if "mixed" in which:
         n = 256
-        out["mixed"] = run_config(fl, "cell 256^3, x EVEN-ODD, y PER, z UNB (padded 512), LGF_4, lda 1, FP64",
-                                  [n] * 3, [[0, 1], [3, 3], [4, 4]], 1, 8, 1, 0, 0, rhs_mixed(n, 11),
-                                  [1.0 / n] * 3, [1.0] * 3, 20, 2)
+        out["mixed"], _ = run_config(fl, "cell 256^3, x EVEN-ODD, y PER, z UNB (padded 512), LGF_4, lda 1, FP64",
+                                     [n] * 3, [[0, 1], [3, 3], [4, 4]], 1, 8, 1, 0, 0, rhs_mixed(n, 11),
+                                     [1.0 / n] * 3, [1.0] * 3, 20, 2)
     if "ring" in which:
         n = 513
         h = 1.0 / (n - 1)
-        out["ring"] = run_config(fl, "node 513^3 all faces UNB (padded 1024^3), HEJ_6, lda 3 ROT (SPE), FP64",
-                                 [n] * 3, [[4, 4]] * 3, 0, 4, 3, 1, 1, rhs_ring(n, 5, h), [h] * 3, [1.0] * 3, 3, 1)
+        rhs = rhs_ring(n, 5, h)
+        out["ring"], ref = run_config(fl, "node 513^3 all faces UNB (padded 1024^3), HEJ_6, lda 3 ROT (SPE), FP64",
+                                      [n] * 3, [[4, 4]] * 3, 0, 4, 3, 1, 1, rhs, [h] * 3, [1.0] * 3, 3, 1)
+        # the FP32 transforms and multiply on the same problem, error against the FP64 result
+        # (itself within 1e-12 of the oracle: tests/test_gpu_parity.py)
+        out["ring_fp32"], d32 = run_config(fl, "node 513^3 all faces UNB (padded 1024^3), HEJ_6, lda 3 ROT (SPE), FP32",
+                                           [n] * 3, [[4, 4]] * 3, 0, 4, 3, 1, 1, rhs, [h] * 3, [1.0] * 3, 3, 1,
+                                           fp32=True, ref=ref)
+        out["ring_fp32"]["speedup_vs_fp64"] = round(out["ring_fp32"]["solves_per_s"] / out["ring"]["solves_per_s"], 3)
+        del ref, d32, rhs
+    if "batch" in which:
+        out["batch"] = batch_config(fl, 64, 7, 10)
     return out
 
 
@@ -234,7 +328,8 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=300, help="requests of the served mix (0 = skip)")
     ap.add_argument("--serve-rate", type=float, default=400.0, help="Poisson arrival rate of the served mix (1/s)")
     ap.add_argument("--serve-workers", type=int, default=4)
-    ap.add_argument("--configs", default="mixed,ring", help="other workloads to time (comma list of mixed, ring; '' = none)")
+    ap.add_argument("--configs", default="mixed,ring,batch",
+                    help="other workloads to time (comma list of mixed, ring, batch; '' = none)")
     ap.add_argument("--copy-calib-mb", type=int, default=0,
                     help="also run one plain device-to-device copy of this many MiB (counter calibration)")
     args = ap.parse_args()

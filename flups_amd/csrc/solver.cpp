@@ -778,16 +778,20 @@ void SolverImpl::prepare(int type) {
     }
 }
 
-// build every pass descriptor once so that the twiddle tables they use exist
-// before a solve is queued (a solve under graph capture may not allocate)
+// build every pass descriptor once, with the views the solve uses (stage arrays; the
+// user layout for the first and last pass), so that the tables of the kernels they
+// select exist before a solve is queued (a solve under graph capture may not allocate)
 void SolverImpl::warm_tables() {
     const View u = user_view(dstage_[0]);
-    for (int ip = 0; ip < ndim_; ++ip)
+    for (int ip = 0; ip < ndim_; ++ip) {
+        const View prev = (ip == 0) ? u : dense_view(dstage_[ip - 1], stage_[ip - 1]);
+        const View cur = dense_view(dstage_[ip], stage_[ip]);
         for (int c = 0; c < lda_; ++c) {
-            (void)pass_desc(fwd_[ip], c, true, u, u, hat_[ip], stage_[ip], fwd_[ip].fieldstart, 0);
-            (void)pass_desc(bwd_[ip], c, false, u, u, stage_[ip], hat_[ip], 0, fwd_[ip].fieldstart);
-            if (odiff_ != NOD) (void)pass_desc(bwdd_[ip], c, false, u, u, stage_[ip], hat_[ip], 0, fwd_[ip].fieldstart);
+            (void)pass_desc(fwd_[ip], c, true, prev, cur, hat_[ip], stage_[ip], fwd_[ip].fieldstart, 0);
+            (void)pass_desc(bwd_[ip], c, false, cur, prev, stage_[ip], hat_[ip], 0, fwd_[ip].fieldstart);
+            if (odiff_ != NOD) (void)pass_desc(bwdd_[ip], c, false, cur, prev, stage_[ip], hat_[ip], 0, fwd_[ip].fieldstart);
         }
+    }
 }
 
 // queue the solve's kernels on the solver's stream; field and rhs are device

@@ -44,12 +44,13 @@ def blob_field(shape, seed):
 
 
 def problems():
-    """a mix of shapes, BC templates, kernels, lda, precision and a 2-D grid"""
+    """a mix of shapes, BC templates, kernels, lda, precision and a 2-D grid (the last problem runs the general
+    path's row r2r and paired-column kernels)"""
     P = []
     spec = [((32, 32, 32), "unb", 2, 1, False), ((48, 40, 32), "unb", 4, 1, True), ((32, 32, 32), "perx", 2, 1, False),
             ((32, 48, 32), "symx-pery", 8, 1, False), ((32, 32, 32), "per", 4, 1, False), ((64, 64, 64), "unb", 2, 1, True),
             ((32, 32, 32), "unb", 4, 3, False), ((64, 64, 1), "unb2d", 2, 1, False), ((40, 32, 48), "perx", 8, 1, False),
-            ((32, 32, 32), "unb", 2, 1, False)]
+            ((32, 32, 32), "unb", 2, 1, False), ((64, 64, 32), "symx-pery", 2, 1, False)]
     for k, (ng, tpl, green, lda, fp32) in enumerate(spec):
         n = max(ng)
         h = [1.0 / n] * 3
