@@ -127,15 +127,22 @@ __global__ void __launch_bounds__(TX * M / 8) z_fused(ColArgs a) {
     const int ntx = (a.nx + TX - 1) / TX;
     const int blk = a.xcd ? xcd_group(blockIdx.x, gridDim.x) : blockIdx.x;
     const int tile = blk % ntx;
-    const int y0 = blk / ntx;           // 0 .. M1/2 (every row with nopair)
+    int y0 = blk / ntx;                 // 0 .. M1/2 (every row with nopair = 1)
+    int yrow = y0;                      // the row of the first (or only) column set
+    if (a.nopair == 2) {                // the two rows of a pair in neighbouring blocks (same G slice)
+        const int r = y0;
+        y0 = r >> 1;
+        yrow = (r & 1) ? a.gmy - y0 : y0;
+        if ((r & 1) && (y0 == 0 || 2 * y0 == a.gmy)) return;   // no mirror row: the whole block leaves
+    }
     const int y1 = a.gmy - y0;          // mirror row; none for y0 = 0 (M1) and y0 = M1/2 (itself)
-    const bool pair = !a.nopair && y0 != 0 && y1 != y0;
+    const bool pair = a.nopair == 0 && y0 != 0 && y1 != y0;
     const bool live = tile * TX + c < a.nx;
     // uniform bases (scalar registers) + one 32-bit per-lane byte offset for every access;
     // lanes past nx load the last valid column and store nothing
     char* base = reinterpret_cast<char*>(a.dst) +
                  (size_t)sizeof(C2<R>) * ((long long)blockIdx.y * a.comp_src + (long long)tile * TX);
-    char* b0 = base + (size_t)sizeof(C2<R>) * ((long long)y0 * a.s_other_src);
+    char* b0 = base + (size_t)sizeof(C2<R>) * ((long long)yrow * a.s_other_src);
     char* b1 = base + (size_t)sizeof(C2<R>) * ((long long)y1 * a.s_other_src);
     const size_t rstride = (size_t)sizeof(C2<R>) * T * a.s_col;                      // i -> i + 1
     const unsigned voff = (unsigned)(sizeof(C2<R>) * ((size_t)j * a.s_col + c));     // < 4 GiB: j < M/8
@@ -577,10 +584,10 @@ FastPlan* make_fast_plan(SolverImpl& s) {
     if (const char* e = getenv("FLUPS_ZSCHED")) f->zsched = atoi(e);
     if (const char* e = getenv("FLUPS_ORDER")) f->order = atoi(e) & 15;
     if (const char* e = getenv("FLUPS_YSCHED")) f->ysched = atoi(e) & 20;
-    if (f->zsched < 0 || f->zsched > 31 || (f->zsched & 3) == 3 || ((f->zsched & 8) && (f->zsched & 7))) {
+    if (f->zsched < 0 || f->zsched > 63 || (f->zsched & 3) == 3 || ((f->zsched & 8) && (f->zsched & 7))) {
         delete f;
         fail("FLUPS_ZSCHED: bits 0-1 schedule (0, 1, 2), bit 2 half-width tiles, bit 3 quarter-width tiles (schedule 0), "
-             "bit 4 XCD-grouped blocks");
+             "bit 4 XCD-grouped blocks, bit 5 mirror rows in neighbouring blocks");
     }
     try {
         build_fast_plan(s, f, es, align);
@@ -707,7 +714,7 @@ void launch_col_h(const ColArgs& a, int comps, int zsched, hipStream_t st) {
     if (FUSED) {
         // z schedules: bit 0-1 = SCHED of z_fused, 4 = half-width tiles (TX/2 columns)
         const int tx = (zsched & 8) ? TX / 4 : (zsched & 4) ? TX / 2 : TX;
-        dim3 grid(((a.nx + tx - 1) / tx) * (a.nopair ? a.nother : a.gmy / 2 + 1), comps);
+        dim3 grid(((a.nx + tx - 1) / tx) * (a.nopair == 1 ? a.nother : a.nopair == 2 ? 2 * (a.gmy / 2 + 1) : a.gmy / 2 + 1), comps);
         switch (zsched & 15) {
             case 0: hipLaunchKernelGGL((z_fused<R, M, TX, 0, HALF>), grid, dim3(TX * M / 8), 0, st, a); break;
             case 1: hipLaunchKernelGGL((z_fused<R, M, TX, 1, HALF>), grid, dim3(TX * M / 8), 0, st, a); break;
@@ -879,6 +886,7 @@ void run_fast(FastPlan* f, const double* rhs, double* field, const View& uin, co
         z.comp_src = c1;
         z.comp_dst = c1;
         z.xcd = (f->zsched >> 4) & 1;
+        z.nopair = (f->zsched & 32) ? 2 : 0;
         z.x0 = x0;
         z.gntx = (f->nx + f->gtx - 1) / f->gtx;
         if (type == 1) {

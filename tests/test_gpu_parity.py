@@ -361,7 +361,7 @@ def test_schedules_are_bitwise_equal_and_autotune(monkeypatch, nglob):
 
     ref, sched = run({})
     assert sched == (20, 0, 4)
-    for z in (0, 1, 2, 4, 16, 20):
+    for z in (0, 1, 2, 4, 16, 20, 48, 52):
         for y, o in ((0, 0), (16, 15), (4, 4)):
             f, sc = run({"FLUPS_ZSCHED": z, "FLUPS_YSCHED": y, "FLUPS_ORDER": o})
             assert sc == (z, y, o)
