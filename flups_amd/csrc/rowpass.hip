@@ -16,6 +16,9 @@
 //   RODFT10 = reversed REDFT10 of (-1)^n x, RODFT01/11 = (-1)^k REDFT01/11 of reversed x.
 // ROWS pencils per block, H/8 threads per pencil; the pencil is staged through
 // LDS so that HBM is read and written in 16-byte pieces when rows are aligned.
+// The pencil switch and plan semantics of the generic kernel are kept: gather
+// with the field shift and zero padding, input / output offsets of 0 or 1,
+// the first / last point zeroed after the transform, truncated scatter.
 //
 // col_real: r2c / c2r of length n (a power of two in [32, 1024]) along a strided
 // direction whose pencils are neighbours along a unit-stride dimension (the
@@ -58,7 +61,9 @@ __global__ void __launch_bounds__(ROWS * H / 8) row_r2r(PassDesc P) {
     const C2<double>* B = A + (H + 1);
     const C2<double>* tw = reinterpret_cast<const C2<double>*>(P.tw);
 
-    // 1. stage the pencil in LDS
+    const int pin = P.pen_in;   // pencil buffer length (the same after the transform), at most N + 2
+
+    // 1. gather the pencil into LDS (field shift, zero padding)
     if (P.vec) {
 #pragma unroll
         for (int t = 0; t < 8; ++t) {
@@ -68,12 +73,15 @@ __global__ void __launch_bounds__(ROWS * H / 8) row_r2r(PassDesc P) {
             rd[2 * q + 1] = p.y;
         }
     } else {
-#pragma unroll
-        for (int t = 0; t < 16; ++t) rd[j + t * T] = src[j + t * T];
+        for (int k = j; k < pin; k += T) {
+            const int ks = k - P.gshift;
+            rd[k] = (ks >= 0 && ks < P.src_nd) ? src[ks] : 0.0;
+        }
     }
     __syncthreads();
+    const int si = P.si;
     auto xin = [&](int q) -> double {
-        const double v = rd[REV ? N - 1 - q : q];
+        const double v = rd[si + (REV ? N - 1 - q : q)];
         return (ALT && (q & 1)) ? -v : v;
     };
 
@@ -149,9 +157,14 @@ __global__ void __launch_bounds__(ROWS * H / 8) row_r2r(PassDesc P) {
     }
     __syncthreads();
 
-    // 5. store (DST output reflections)
+    // 5. scatter: output pencil position p holds transform value p - so (DST reflections
+    // applied), the first / last point zeroed by the plan's post-correction
     if (!live) return;
-    auto out = [&](int k) -> double {
+    const int so = P.so, post = P.post;
+    auto val = [&](int p) -> double {
+        if (((post & POST_FIRST) && p == 0) || ((post & POST_LAST) && p == pin - 1)) return 0.0;
+        const int k = p - so;
+        if (k < 0 || k >= N) return 0.0;
         const double y = rd[ALT ? N - 1 - k : k];
         return (REV && (k & 1)) ? -y : y;
     };
@@ -159,11 +172,13 @@ __global__ void __launch_bounds__(ROWS * H / 8) row_r2r(PassDesc P) {
 #pragma unroll
         for (int t = 0; t < 8; ++t) {
             const int q = j + t * T;
-            reinterpret_cast<double2*>(dst)[q] = double2{out(2 * q), out(2 * q + 1)};
+            reinterpret_cast<double2*>(dst)[q] = double2{val(2 * q), val(2 * q + 1)};
         }
     } else {
-#pragma unroll
-        for (int t = 0; t < 16; ++t) dst[j + t * T] = out(j + t * T);
+        for (int k = j; k < P.dst_nd; k += T) {
+            const int p = k + P.sshift;
+            if (p < pin) dst[k] = val(p);
+        }
     }
 }
 
@@ -291,9 +306,14 @@ bool row_r2r_fits(const PassDesc& P) {
         return false;
     if (n % 2 || h < 16 || h > 1024 || (h & (h - 1))) return false;
     const int d = P.d;
-    return P.src_s[d] == 1 && P.dst_s[d] == 1 && P.src_nf == 1 && P.dst_nf == 1 && P.nf_in == 1 && P.nf_out == 1 &&
-           P.r2r_stride == 1 && P.si == 0 && P.so == 0 && P.gshift == 0 && P.sshift == 0 && P.src_nd == n &&
-           P.dst_nd == n && P.pen_in == n && P.pen_out == n && P.post == 0;
+    if (!(P.src_s[d] == 1 && P.dst_s[d] == 1 && P.src_nf == 1 && P.dst_nf == 1 && P.nf_in == 1 && P.nf_out == 1 &&
+          P.r2r_stride == 1 && P.pen_in == P.pen_out && P.pen_in >= n && P.pen_in <= n + 2 && P.gshift >= 0 && P.sshift >= 0))
+        return false;
+    if (P.si < 0 || P.si > 1 || P.so < 0 || P.so > 1 || P.si + n > P.pen_in || (P.post & POST_PERIOD)) return false;
+    // every output position is a transform value or a point the post-correction zeroes
+    const bool head = P.so == 0 || (P.post & POST_FIRST);
+    const bool tail = P.so + n == P.pen_out || (P.so + n == P.pen_out - 1 && (P.post & POST_LAST));
+    return head && tail;
 }
 
 // pre / post twiddles of the row r2r kernels, (H + 1) complex each, exact to rounding

@@ -324,8 +324,10 @@ PassDesc SolverImpl::pass_desc(const DirPlan& p, int comp, bool forward, const V
         P.tw = twiddles(P.n / 2);
         P.aux = r2r_table(P.kind, P.n);
         const int fa = P.fa, fb = P.fb;
-        P.vec = ((uintptr_t)P.src % 16 == 0) && ((uintptr_t)P.dst % 16 == 0) && P.src_s[fa] % 2 == 0 && P.src_s[fb] % 2 == 0 &&
-                P.dst_s[fa] % 2 == 0 && P.dst_s[fb] % 2 == 0;
+        const bool plain = P.gshift == 0 && P.sshift == 0 && P.src_nd == P.n && P.dst_nd == P.n && P.pen_in == P.n &&
+                           P.si == 0 && P.so == 0 && P.post == 0;
+        P.vec = plain && ((uintptr_t)P.src % 16 == 0) && ((uintptr_t)P.dst % 16 == 0) && P.src_s[fa] % 2 == 0 &&
+                P.src_s[fb] % 2 == 0 && P.dst_s[fa] % 2 == 0 && P.dst_s[fb] % 2 == 0;
         return P;
     }
     // real-to-complex pencils along a strided direction, neighbours along a unit-stride one:

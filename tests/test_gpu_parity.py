@@ -559,11 +559,15 @@ ROW_CASES = [
     ((33, 64, 16), [[1, 1], [3, 3], [4, 4]], 1, 0),
     ((64, 32, 128), [[0, 1], [1, 0], [3, 3]], 1, 0),
     ((256, 256, 8), [[0, 1], [3, 3], [0, 0]], 1, 0),
+    # semi-unbounded x (transforms of length 2N over the zero-padded, possibly shifted field)
+    ((32, 32, 32), [[0, 4], [3, 3], [4, 4]], 1, 2),
+    ((64, 16, 32), [[4, 1], [4, 4], [3, 3]], 1, 4),
+    ((32, 64, 16), [[1, 4], [0, 0], [4, 4]], 1, 0),
 ]
 
 
 @pytest.mark.parametrize("case", ROW_CASES, ids=["ee", "oo", "eo-256", "oe", "sym3", "oo-mix", "ee-pery", "odd-x-pery",
-                                                 "perz-128", "pery-256"])
+                                                 "perz-128", "pery-256", "even-unb", "unb-odd", "odd-unb-ee"])
 @pytest.mark.parametrize("misalign", [False, True])
 def test_row_r2r_kernel_vs_generic_and_oracle(case, misalign, monkeypatch):
     """the row kernel of the contiguous real-to-real passes (one half-length complex FFT with pre-
