@@ -206,8 +206,11 @@ __global__ void __launch_bounds__(TX * M / 8) col_real(PassDesc P) {
     if (FWD) {
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
-            const double* p = rsrc + (long long)(j + i * T) * sr;
-            if (P.vec) {
+            const int row = j + i * T;
+            const double* p = rsrc + (long long)row * sr;
+            if (row >= P.src_nd) {   // zero padding of an unbounded direction
+                v[i] = {0.0, 0.0};
+            } else if (P.vec) {
                 const double2 q = *reinterpret_cast<const double2*>(p);
                 v[i] = {q.x, q.y};
             } else {
@@ -260,7 +263,9 @@ __global__ void __launch_bounds__(TX * M / 8) col_real(PassDesc P) {
         if (!la) return;
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
-            double* p = rdst + (long long)(j + i * T) * sr;
+            const int row = j + i * T;
+            if (row >= P.dst_nd) continue;   // truncation back to the field's extent
+            double* p = rdst + (long long)row * sr;
             if (P.vec) {
                 *reinterpret_cast<double2*>(p) = double2{v[i].x, v[i].y};
             } else {
@@ -348,12 +353,13 @@ bool col_real_fits(const PassDesc& P) {
     const int d = P.d, fa = P.fa;
     if (fa == d || P.src_s[fa] != 1 || P.dst_s[fa] != 1) return false;
     // the real side may carry the two padding reals of the in-place r2c layout (n + 2 per
-    // pencil): only rows below n are data, the rest is neither read nor written here
+    // pencil): only rows below n are data, the rest is neither read nor written here. A
+    // zero-padded (unbounded) direction reads src_nd < n rows and stores dst_nd < n back.
     if (fwd && !(P.nf_in == 1 && P.nf_out == 2 && P.src_nf == 1 && P.dst_nf == 2 && P.pen_in >= n && P.pen_out == n / 2 + 1 &&
-                 P.src_nd >= n && P.dst_nd == n / 2 + 1))
+                 P.src_nd >= 1 && P.src_nd <= P.pen_in && P.dst_nd == n / 2 + 1))
         return false;
     if (!fwd && !(P.nf_in == 2 && P.nf_out == 1 && P.src_nf == 2 && P.dst_nf == 1 && P.pen_in == n / 2 + 1 && P.pen_out >= n &&
-                  P.src_nd == n / 2 + 1 && P.dst_nd >= n))
+                  P.src_nd == n / 2 + 1 && P.dst_nd >= 1 && P.dst_nd <= P.pen_out))
         return false;
     return P.si == 0 && P.so == 0 && P.gshift == 0 && P.sshift == 0 && P.post == 0 && (P.n1 * P.n2) % P.na == 0;
 }

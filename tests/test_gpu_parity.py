@@ -563,11 +563,15 @@ ROW_CASES = [
     ((32, 32, 32), [[0, 4], [3, 3], [4, 4]], 1, 2),
     ((64, 16, 32), [[4, 1], [4, 4], [3, 3]], 1, 4),
     ((32, 64, 16), [[1, 4], [0, 0], [4, 4]], 1, 0),
+    # unbounded y after a symmetric x: zero-padded r2c / truncated c2r of paired real columns
+    ((32, 64, 32), [[0, 0], [4, 4], [3, 3]], 1, 2),
+    ((64, 32, 32), [[1, 0], [4, 4], [4, 4]], 1, 8),
 ]
 
 
 @pytest.mark.parametrize("case", ROW_CASES, ids=["ee", "oo", "eo-256", "oe", "sym3", "oo-mix", "ee-pery", "odd-x-pery",
-                                                 "perz-128", "pery-256", "even-unb", "unb-odd", "odd-unb-ee"])
+                                                 "perz-128", "pery-256", "even-unb", "unb-odd", "odd-unb-ee", "ee-unby-perz",
+                                                 "oe-unby-unbz"])
 @pytest.mark.parametrize("misalign", [False, True])
 def test_row_r2r_kernel_vs_generic_and_oracle(case, misalign, monkeypatch):
     """the row kernel of the contiguous real-to-real passes (one half-length complex FFT with pre-
