@@ -102,7 +102,8 @@ struct PassDesc {
     int fa = 1, fb = 2;     // pencil grid: dim fa varies fastest between pencils (the one with the smaller stride)
     int na = 1;             // extent along fa
     int alias = 0;          // 1: the second FFT buffer reuses the pencil buffer
-    int fastk = 0;          // fast kernels of rowpass.hip instead of the generic one: 1 row r2r, 2 paired real columns
+    int fastk = 0;          // fast kernels of rowpass.hip instead of the generic one: 1 row r2r, 2 paired real columns,
+                            // 3 row r2c / c2r
     int vec = 0;            // fast kernels: pencils start 16-byte aligned in source and destination
     const double* aux = nullptr;   // fast kernels: pre / post twiddle table
 };

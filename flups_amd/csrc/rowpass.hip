@@ -29,6 +29,12 @@
 // imaginary parts of the DC and Nyquist terms are ignored, as FFTW's c2r does)
 // and a + i b = IFFT_n(Z). TX pairs side by side per block: each row of a
 // tile is one contiguous run of 2 TX reals or complex values.
+//
+// row_rc: r2c / c2r of even length n = 2H (H a power of two in [16, 1024]) along a
+// contiguous pencil through the packed complex FFT of length H: z[m] = x[2m] + i x[2m+1],
+// X[k] = (Z[k] + conj Z[H-k])/2 + e^{-2 pi i k/n} (Z[k] - conj Z[H-k])/(2i), k <= H; the
+// inverse Z[k] = (X[k] + conj X[H-k]) + i e^{2 pi i k/n} (X[k] - conj X[H-k]), x[2m] + i x[2m+1]
+// = IFFT_H(Z) (the imaginary parts of X[0] and X[H] ignored).
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -182,6 +188,92 @@ __global__ void __launch_bounds__(ROWS * H / 8) row_r2r(PassDesc P) {
     }
 }
 
+template <int H, int ROWS, bool FWD>
+__global__ void __launch_bounds__(ROWS * H / 8) row_rc(PassDesc P) {
+    constexpr int T = H / 8, HP = H + H / 8;
+    __shared__ C2<double> lds[HP * ROWS];
+    const int rl = threadIdx.x / T, j = threadIdx.x % T;
+    const int g = blockIdx.x * ROWS + rl;
+    const bool live = g < P.n1 * P.n2;
+    const int gg = live ? g : 0;   // dead rows read pencil 0 and store nothing
+    const int ja = gg % P.na, jb = gg / P.na;
+    const double* src = P.src + (long long)ja * P.src_s[P.fa] * P.src_nf + (long long)jb * P.src_s[P.fb] * P.src_nf;
+    double* dst = P.dst + (long long)ja * P.dst_s[P.fa] * P.dst_nf + (long long)jb * P.dst_s[P.fb] * P.dst_nf;
+    C2<double>* row = lds + rl * HP;
+    double* rd = reinterpret_cast<double*>(row);
+    const C2<double>* tw = reinterpret_cast<const C2<double>*>(P.tw);    // H-point table
+    const C2<double>* twn = reinterpret_cast<const C2<double>*>(P.aux);  // n-point table
+    // 1. gather: the pencil's doubles (reals for r2c, interleaved complex for c2r)
+    const int nfi = P.nf_in, len = P.pen_in * nfi;
+    for (int q = j; q < len; q += T) {
+        const int k = q / nfi, ks = k - P.gshift;
+        rd[q] = (ks >= 0 && ks < P.src_nd) ? src[(long long)ks * nfi + (q - k * nfi)] : 0.0;
+    }
+    __syncthreads();
+    C2<double> v[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const int m = j + i * T;
+        if (FWD) {
+            v[i] = {rd[2 * m], rd[2 * m + 1]};
+        } else {
+            C2<double> a = {rd[2 * m], rd[2 * m + 1]}, b = {rd[2 * (H - m)], rd[2 * (H - m) + 1]};
+            if (m == 0) {   // DC and Nyquist are real
+                a.y = 0.0;
+                b.y = 0.0;
+            }
+            const C2<double> bc = cconj(b);
+            const C2<double> e = cadd(a, bc), f = csub(a, bc);
+            const C2<double> t = cmul(cconj(twn[m]), f);   // e^{+2 pi i m/n} (a - conj b)
+            v[i] = {e.x - t.y, e.y + t.x};
+        }
+    }
+    column_fft<double, H, FWD, 0>(v, row, j, tw);
+    __syncthreads();
+    const int nfo = P.nf_out, plen = P.pen_out;
+    if (FWD) {
+        // X[k], k <= H, into the row as complex values
+#pragma unroll
+        for (int i = 0; i < 8; ++i) row[j + i * T] = v[i];
+        __syncthreads();
+        C2<double> X[8], XH = {0, 0};
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const int k = j + i * T;
+            const C2<double> zk = row[k], zc = cconj(row[(H - k) & (H - 1)]);
+            const C2<double> e = {0.5 * (zk.x + zc.x), 0.5 * (zk.y + zc.y)};
+            const C2<double> df = {0.5 * (zk.x - zc.x), 0.5 * (zk.y - zc.y)};
+            X[i] = cadd(e, cmul(twn[k], C2<double>{df.y, -df.x}));
+        }
+        if (j == 0) {
+            const C2<double> z0 = row[0];
+            XH = {z0.x - z0.y, 0.0};
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < 8; ++i) row[j + i * T] = X[i];
+        if (j == 0) row[H] = XH;
+        // positions past H + 1 complex values keep nothing (zero)
+        for (int q = H + 1 + j; q < plen; q += T) row[q] = {0.0, 0.0};
+        // periodic node-centred direction: the first value again at the end (n_out - 1)
+        if ((P.post & POST_PERIOD) && j == 0) row[P.n_out_plan - 1] = X[0];
+    } else {
+        // x[2m] = Re z[m], x[2m+1] = Im z[m]
+#pragma unroll
+        for (int i = 0; i < 8; ++i) row[j + i * T] = v[i];
+        for (int q = 2 * H + j; q < plen; q += T) rd[q] = 0.0;
+        if ((P.post & POST_PERIOD) && j == 0) rd[2 * H] = v[0].x;   // x[n] = x[0]
+    }
+    __syncthreads();
+    if (!live) return;
+    // 2. scatter (truncation), element k of the destination = pencil position k + sshift
+    const int dn = P.dst_nd;
+    for (int q = j; q < dn * nfo; q += T) {
+        const int k = q / nfo, p = k + P.sshift;
+        if (p < plen) dst[(long long)k * nfo + (q - k * nfo)] = rd[p * nfo + (q - k * nfo)];
+    }
+}
+
 template <int M, int TX, bool FWD>
 __global__ void __launch_bounds__(TX * M / 8) col_real(PassDesc P) {
     constexpr int T = M / 8, NH = M / 2;
@@ -292,6 +384,13 @@ void launch_row_h(const PassDesc& P, hipStream_t s) {
     }
 }
 
+template <int H, int ROWS>
+void launch_row_rc_h(const PassDesc& P, hipStream_t s) {
+    dim3 grid((P.n1 * P.n2 + ROWS - 1) / ROWS);
+    if (P.op == OP_R2C) hipLaunchKernelGGL((row_rc<H, ROWS, true>), grid, dim3(ROWS * H / 8), 0, s, P);
+    else hipLaunchKernelGGL((row_rc<H, ROWS, false>), grid, dim3(ROWS * H / 8), 0, s, P);
+}
+
 template <int M, int TX>
 void launch_col_real_m(const PassDesc& P, hipStream_t s) {
     const int npair = (P.na + 1) / 2;
@@ -344,6 +443,35 @@ std::vector<double> row_r2r_table(int kind, int n) {
         }
     }
     return t;
+}
+
+bool row_rc_fits(const PassDesc& P) {
+    const int n = P.n, h = n / 2;
+    if (!(P.op == OP_R2C || P.op == OP_C2R) || n % 2 || h < 16 || h > 1024 || (h & (h - 1))) return false;
+    const bool fwd = P.op == OP_R2C;
+    const int d = P.d;
+    if (P.src_s[d] != 1 || P.dst_s[d] != 1 || P.si != 0 || P.so != 0 || (P.post & ~POST_PERIOD) || P.gshift < 0 || P.sshift < 0)
+        return false;
+    // the periodic copy lands inside the pencil (r2c: complex value n_out - 1, c2r: real value n)
+    if ((P.post & POST_PERIOD) && (fwd ? P.n_out_plan - 1 >= P.pen_out || P.n_out_plan < 1 : n >= P.pen_out)) return false;
+    // buffer of the pencil in LDS: at most 2H + H/4 doubles
+    if (P.pen_in * P.nf_in > 2 * h + h / 4 || P.pen_out * P.nf_out > 2 * h + h / 4) return false;
+    if (fwd) return P.nf_in == 1 && P.src_nf == 1 && P.nf_out == 2 && P.dst_nf == 2 && P.pen_in >= n && P.pen_out >= h + 1;
+    return P.nf_in == 2 && P.src_nf == 2 && P.nf_out == 1 && P.dst_nf == 1 && P.pen_in >= h + 1 && P.pen_out >= n;
+}
+
+void launch_row_rc(const PassDesc& P, hipStream_t s) {
+    switch (P.n / 2) {
+        case 16: launch_row_rc_h<16, 128>(P, s); break;
+        case 32: launch_row_rc_h<32, 64>(P, s); break;
+        case 64: launch_row_rc_h<64, 32>(P, s); break;
+        case 128: launch_row_rc_h<128, 16>(P, s); break;
+        case 256: launch_row_rc_h<256, 8>(P, s); break;
+        case 512: launch_row_rc_h<512, 4>(P, s); break;
+        case 1024: launch_row_rc_h<1024, 2>(P, s); break;
+        default: fail("row rc: length %d", P.n);
+    }
+    HIP_CHECK(hipGetLastError());
 }
 
 bool col_real_fits(const PassDesc& P) {

@@ -25,6 +25,8 @@ void launch_generic(const PassDesc& P, hipStream_t s);
 bool row_r2r_fits(const PassDesc& P);
 bool col_real_fits(const PassDesc& P);
 void launch_col_real(const PassDesc& P, hipStream_t s);
+bool row_rc_fits(const PassDesc& P);
+void launch_row_rc(const PassDesc& P, hipStream_t s);
 std::vector<double> row_r2r_table(int kind, int n);
 void launch_row_r2r(const PassDesc& P, hipStream_t s);
 FastPlan* make_fast_plan(SolverImpl& s);   // nullptr if the fused pipeline does not apply
@@ -330,6 +332,13 @@ PassDesc SolverImpl::pass_desc(const DirPlan& p, int comp, bool forward, const V
                 P.src_s[fb] % 2 == 0 && P.dst_s[fa] % 2 == 0 && P.dst_s[fb] % 2 == 0;
         return P;
     }
+    // real-to-complex pencils along the contiguous direction: packed half-length FFT in registers
+    if (!std::getenv("FLUPS_GENERIC_PASS") && row_rc_fits(P)) {
+        P.fastk = 3;
+        P.tw = twiddles(P.n / 2);
+        P.aux = twiddles(P.n);
+        return P;
+    }
     // real-to-complex pencils along a strided direction, neighbours along a unit-stride one:
     // pairs of real pencils as one complex column
     if (!std::getenv("FLUPS_GENERIC_PASS") && col_real_fits(P)) {
@@ -356,6 +365,7 @@ PassDesc SolverImpl::pass_desc(const DirPlan& p, int comp, bool forward, const V
 void SolverImpl::run_pass(PassDesc P) {
     if (P.fastk == 1) launch_row_r2r(P, stream_);
     else if (P.fastk == 2) launch_col_real(P, stream_);
+    else if (P.fastk == 3) launch_row_rc(P, stream_);
     else launch_generic(P, stream_);
 }
 

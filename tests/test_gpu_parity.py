@@ -566,12 +566,17 @@ ROW_CASES = [
     # unbounded y after a symmetric x: zero-padded r2c / truncated c2r of paired real columns
     ((32, 64, 32), [[0, 0], [4, 4], [3, 3]], 1, 2),
     ((64, 32, 32), [[1, 0], [4, 4], [4, 4]], 1, 8),
+    # symmetric y first, then a contiguous r2c / c2r along x (packed half-length FFT)
+    ((64, 32, 32), [[4, 4], [0, 0], [4, 4]], 1, 2),
+    ((64, 32, 32), [[3, 3], [1, 1], [4, 4]], 1, 0),
+    # node-centred periodic: r2c with the periodic copy
+    ((65, 33, 17), [[3, 3], [3, 3], [3, 3]], 0, 0),
 ]
 
 
 @pytest.mark.parametrize("case", ROW_CASES, ids=["ee", "oo", "eo-256", "oe", "sym3", "oo-mix", "ee-pery", "odd-x-pery",
                                                  "perz-128", "pery-256", "even-unb", "unb-odd", "odd-unb-ee", "ee-unby-perz",
-                                                 "oe-unby-unbz"])
+                                                 "oe-unby-unbz", "unbx-eey", "perx-ooy", "node-per"])
 @pytest.mark.parametrize("misalign", [False, True])
 def test_row_r2r_kernel_vs_generic_and_oracle(case, misalign, monkeypatch):
     """the row kernel of the contiguous real-to-real passes (one half-length complex FFT with pre-
@@ -582,8 +587,8 @@ def test_row_r2r_kernel_vs_generic_and_oracle(case, misalign, monkeypatch):
     n0, n1, n2 = nglob
     rhs = rng.normal(size=(1, n2, n1, n0))
     rhs -= rhs.mean()
-    h = [1.0 / max(nglob)] * 3
-    L = [h[0] * m for m in nglob]
+    h = [1.0 / (max(nglob) - (1 - center))] * 3
+    L = [h[0] * (m - (1 - center)) for m in nglob]
 
     def run(generic):
         if generic:
@@ -591,7 +596,7 @@ def test_row_r2r_kernel_vs_generic_and_oracle(case, misalign, monkeypatch):
         else:
             monkeypatch.delenv("FLUPS_GENERIC_PASS", raising=False)
         topo = fl.Topology(0, 1, list(nglob), [1, 1, 1], False)
-        s = fl.Solver(topo, bc, h, L)
+        s = fl.Solver(topo, bc, h, L, center_type=[center] * 3)
         s.set_green_type(kernel)
         s.setup()
         n = rhs.size
@@ -609,6 +614,6 @@ def test_row_r2r_kernel_vs_generic_and_oracle(case, misalign, monkeypatch):
     gen = run(True)
     e2, ei = rel_err(fast, gen)
     assert e2 <= TOL and ei <= TOL, ("fast vs generic", e2, ei)
-    ora = orc.solve(list(nglob), bc, 1, h, L, 1, kernel, rhs)
+    ora = orc.solve(list(nglob), bc, 1, h, L, center, kernel, rhs)
     e2, ei = rel_err(fast, ora)
     assert e2 <= TOL and ei <= TOL, ("fast vs oracle", e2, ei)
