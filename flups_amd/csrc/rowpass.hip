@@ -16,6 +16,9 @@
 //   RODFT10 = reversed REDFT10 of (-1)^n x, RODFT01/11 = (-1)^k REDFT01/11 of reversed x.
 // ROWS pencils per block, H/8 threads per pencil; the pencil is staged through
 // LDS so that HBM is read and written in 16-byte pieces when rows are aligned.
+// col_r2r runs the same transforms along a strided direction: R pencils that are
+// neighbours along the unit-stride dimension share a block, interleaved in LDS,
+// and each row of the tile is one contiguous run of R doubles in HBM.
 // The pencil switch and plan semantics of the generic kernel are kept: gather
 // with the field shift and zero padding, input / output offsets of 0 or 1,
 // the first / last point zeroed after the transform, truncated scatter.
@@ -47,47 +50,68 @@ namespace flups {
 
 namespace {
 
-template <int H, int ROWS, int KIND>
-__global__ void __launch_bounds__(ROWS * H / 8) row_r2r(PassDesc P) {
+// COL = false: R contiguous pencils per block (H/8 threads each, one padded LDS row each);
+// COL = true: R strided pencils, neighbours along the unit-stride dimension fa, interleaved
+// in LDS (element k of pencil c at slot k * R + c) and read / written as tile rows.
+template <int H, int R, int KIND, bool COL>
+__global__ void __launch_bounds__(R * H / 8) pen_r2r(PassDesc P) {
     constexpr int T = H / 8, N = 2 * H, HP = H + H / 8;
-    __shared__ C2<double> lds[HP * ROWS];
+    constexpr int S = COL ? R : 1;   // LDS stride between consecutive elements of a pencil
+    __shared__ C2<double> lds[COL ? (H + 1) * R : HP * R];
     constexpr int DCT = KIND >= RODFT10 ? KIND - 4 : KIND;
     constexpr bool ALT = KIND == RODFT10;                       // input (-1)^n x, output reversed
     constexpr bool REV = KIND == RODFT01 || KIND == RODFT11;    // input reversed, output (-1)^k
-    const int rl = threadIdx.x / T, j = threadIdx.x % T;
-    const int g = blockIdx.x * ROWS + rl;
-    const bool live = g < P.n1 * P.n2;
-    const int gg = live ? g : 0;   // dead rows read pencil 0 and store nothing
-    const int ja = gg % P.na, jb = gg / P.na;
+    int rl, j, ja, jb;
+    bool live;
+    if (COL) {
+        rl = threadIdx.x % R;
+        j = threadIdx.x / R;
+        const int ntile = (P.na + R - 1) / R;
+        const int tile = blockIdx.x % ntile;
+        jb = blockIdx.x / ntile;
+        ja = tile * R + rl;
+        live = ja < P.na;
+        if (!live) ja = P.na - 1;   // dead pencils read the last one and store nothing
+    } else {
+        rl = threadIdx.x / T;
+        j = threadIdx.x % T;
+        const int g = blockIdx.x * R + rl;
+        live = g < P.n1 * P.n2;
+        const int gg = live ? g : 0;   // dead rows read pencil 0 and store nothing
+        ja = gg % P.na;
+        jb = gg / P.na;
+    }
     const double* src = P.src + (long long)ja * P.src_s[P.fa] + (long long)jb * P.src_s[P.fb];
     double* dst = P.dst + (long long)ja * P.dst_s[P.fa] + (long long)jb * P.dst_s[P.fb];
-    C2<double>* row = lds + rl * HP;
-    double* rd = reinterpret_cast<double*>(row);   // the pencil as N reals
+    const long long sd = P.src_s[P.d], dd = P.dst_s[P.d];
+    C2<double>* row = COL ? lds + rl : lds + rl * HP;                                    // complex view
+    double* rdb = COL ? reinterpret_cast<double*>(lds) + rl : reinterpret_cast<double*>(row);   // real view
+    auto RD = [&](int k) -> double& { return rdb[k * S]; };
+    auto RW = [&](int k) -> C2<double>& { return row[k * S]; };
     const C2<double>* A = reinterpret_cast<const C2<double>*>(P.aux);
     const C2<double>* B = A + (H + 1);
     const C2<double>* tw = reinterpret_cast<const C2<double>*>(P.tw);
-
     const int pin = P.pen_in;   // pencil buffer length (the same after the transform), at most N + 2
 
     // 1. gather the pencil into LDS (field shift, zero padding)
-    if (P.vec) {
+    if (!COL && P.vec) {
 #pragma unroll
         for (int t = 0; t < 8; ++t) {
             const int q = j + t * T;
             const double2 p = reinterpret_cast<const double2*>(src)[q];
-            rd[2 * q] = p.x;
-            rd[2 * q + 1] = p.y;
+            RD(2 * q) = p.x;
+            RD(2 * q + 1) = p.y;
         }
     } else {
         for (int k = j; k < pin; k += T) {
             const int ks = k - P.gshift;
-            rd[k] = (ks >= 0 && ks < P.src_nd) ? src[ks] : 0.0;
+            RD(k) = (ks >= 0 && ks < P.src_nd) ? src[(long long)ks * sd] : 0.0;
         }
     }
     __syncthreads();
     const int si = P.si;
     auto xin = [&](int q) -> double {
-        const double v = rd[si + (REV ? N - 1 - q : q)];
+        const double v = RD(si + (REV ? N - 1 - q : q));
         return (ALT && (q & 1)) ? -v : v;
     };
 
@@ -114,33 +138,33 @@ __global__ void __launch_bounds__(ROWS * H / 8) row_r2r(PassDesc P) {
     }
 
     // 3. FFT of length H (forward for DCT-II / IV, inverse for DCT-III)
-    column_fft<double, H, DCT != REDFT01, 0>(v, row, j, tw);
+    column_fft<double, H, DCT != REDFT01, COL ? R : 0>(v, row, j, tw);
     __syncthreads();   // every thread is done with the FFT's LDS before the buffer is reused
 
-    // 4. post-map into the LDS row
+    // 4. post-map into the LDS pencil
     if (DCT == REDFT11) {
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
             const int k = j + i * T;
             const C2<double> w = cmul(v[i], B[k]);
-            rd[2 * k] = 2.0 * w.x;
-            rd[N - 1 - 2 * k] = -2.0 * w.y;
+            RD(2 * k) = 2.0 * w.x;
+            RD(N - 1 - 2 * k) = -2.0 * w.y;
         }
     } else if (DCT == REDFT10) {
 #pragma unroll
-        for (int i = 0; i < 8; ++i) row[j + i * T] = v[i];
+        for (int i = 0; i < 8; ++i) RW(j + i * T) = v[i];
         __syncthreads();
         C2<double> U[8], UH = {0, 0};
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
             const int k = j + i * T;
-            const C2<double> zk = row[k], zc = cconj(row[(H - k) & (H - 1)]);
+            const C2<double> zk = RW(k), zc = cconj(RW((H - k) & (H - 1)));
             const C2<double> e = {0.5 * (zk.x + zc.x), 0.5 * (zk.y + zc.y)};
             const C2<double> df = {0.5 * (zk.x - zc.x), 0.5 * (zk.y - zc.y)};
             U[i] = cadd(e, cmul(A[k], C2<double>{df.y, -df.x}));   // e + A[k] (zk - zc) / (2i)
         }
         if (j == 0) {   // U[H] = Re Z[0] - Im Z[0]
-            const C2<double> z0 = row[0];
+            const C2<double> z0 = RW(0);
             UH = {z0.x - z0.y, 0.0};
         }
         __syncthreads();
@@ -148,17 +172,17 @@ __global__ void __launch_bounds__(ROWS * H / 8) row_r2r(PassDesc P) {
         for (int i = 0; i < 8; ++i) {
             const int k = j + i * T;
             const C2<double> c = cmul(B[k], U[i]);
-            rd[k] = 2.0 * c.x;
-            if (k > 0) rd[N - k] = -2.0 * c.y;
+            RD(k) = 2.0 * c.x;
+            if (k > 0) RD(N - k) = -2.0 * c.y;
         }
-        if (j == 0) rd[H] = 2.0 * cmul(B[H], UH).x;
+        if (j == 0) RD(H) = 2.0 * cmul(B[H], UH).x;
     } else {   // REDFT01: v[i] = z[m], u[2m] = Re, u[2m+1] = Im
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
             const int m = j + i * T;
             const int q0 = 2 * m, q1 = 2 * m + 1;
-            rd[q0 < H ? 2 * q0 : 2 * (N - 1 - q0) + 1] = v[i].x;
-            rd[q1 < H ? 2 * q1 : 2 * (N - 1 - q1) + 1] = v[i].y;
+            RD(q0 < H ? 2 * q0 : 2 * (N - 1 - q0) + 1) = v[i].x;
+            RD(q1 < H ? 2 * q1 : 2 * (N - 1 - q1) + 1) = v[i].y;
         }
     }
     __syncthreads();
@@ -171,10 +195,10 @@ __global__ void __launch_bounds__(ROWS * H / 8) row_r2r(PassDesc P) {
         if (((post & POST_FIRST) && p == 0) || ((post & POST_LAST) && p == pin - 1)) return 0.0;
         const int k = p - so;
         if (k < 0 || k >= N) return 0.0;
-        const double y = rd[ALT ? N - 1 - k : k];
+        const double y = RD(ALT ? N - 1 - k : k);
         return (REV && (k & 1)) ? -y : y;
     };
-    if (P.vec) {
+    if (!COL && P.vec) {
 #pragma unroll
         for (int t = 0; t < 8; ++t) {
             const int q = j + t * T;
@@ -183,7 +207,7 @@ __global__ void __launch_bounds__(ROWS * H / 8) row_r2r(PassDesc P) {
     } else {
         for (int k = j; k < P.dst_nd; k += T) {
             const int p = k + P.sshift;
-            if (p < pin) dst[k] = val(p);
+            if (p < pin) dst[(long long)k * dd] = val(p);
         }
     }
 }
@@ -368,19 +392,20 @@ __global__ void __launch_bounds__(TX * M / 8) col_real(PassDesc P) {
     }
 }
 
-template <int H, int ROWS>
-void launch_row_h(const PassDesc& P, hipStream_t s) {
+template <int H, int R, bool COL>
+void launch_pen_h(const PassDesc& P, hipStream_t s) {
     const int npen = P.n1 * P.n2;
-    dim3 grid((npen + ROWS - 1) / ROWS);
-    dim3 block(ROWS * H / 8);
+    const int nb = COL ? ((P.na + R - 1) / R) * (npen / P.na) : (npen + R - 1) / R;
+    dim3 grid(nb);
+    dim3 block(R * H / 8);
     switch (P.kind) {
-        case REDFT10: hipLaunchKernelGGL((row_r2r<H, ROWS, REDFT10>), grid, block, 0, s, P); break;
-        case REDFT01: hipLaunchKernelGGL((row_r2r<H, ROWS, REDFT01>), grid, block, 0, s, P); break;
-        case REDFT11: hipLaunchKernelGGL((row_r2r<H, ROWS, REDFT11>), grid, block, 0, s, P); break;
-        case RODFT10: hipLaunchKernelGGL((row_r2r<H, ROWS, RODFT10>), grid, block, 0, s, P); break;
-        case RODFT01: hipLaunchKernelGGL((row_r2r<H, ROWS, RODFT01>), grid, block, 0, s, P); break;
-        case RODFT11: hipLaunchKernelGGL((row_r2r<H, ROWS, RODFT11>), grid, block, 0, s, P); break;
-        default: fail("row r2r: kind %d", P.kind);
+        case REDFT10: hipLaunchKernelGGL((pen_r2r<H, R, REDFT10, COL>), grid, block, 0, s, P); break;
+        case REDFT01: hipLaunchKernelGGL((pen_r2r<H, R, REDFT01, COL>), grid, block, 0, s, P); break;
+        case REDFT11: hipLaunchKernelGGL((pen_r2r<H, R, REDFT11, COL>), grid, block, 0, s, P); break;
+        case RODFT10: hipLaunchKernelGGL((pen_r2r<H, R, RODFT10, COL>), grid, block, 0, s, P); break;
+        case RODFT01: hipLaunchKernelGGL((pen_r2r<H, R, RODFT01, COL>), grid, block, 0, s, P); break;
+        case RODFT11: hipLaunchKernelGGL((pen_r2r<H, R, RODFT11, COL>), grid, block, 0, s, P); break;
+        default: fail("r2r pencil kernel: kind %d", P.kind);
     }
 }
 
@@ -403,21 +428,29 @@ void launch_col_real_m(const PassDesc& P, hipStream_t s) {
 
 }  // namespace
 
-bool row_r2r_fits(const PassDesc& P) {
+namespace {
+// the plan semantics the r2r pencil kernels reproduce (kinds, length, offsets, corrections)
+bool r2r_plan_fits(const PassDesc& P) {
     const int k = P.kind;
     const int n = P.n, h = n / 2;
     if (P.op != OP_R2R || !(k == REDFT10 || k == REDFT01 || k == REDFT11 || k == RODFT10 || k == RODFT01 || k == RODFT11))
         return false;
     if (n % 2 || h < 16 || h > 1024 || (h & (h - 1))) return false;
-    const int d = P.d;
-    if (!(P.src_s[d] == 1 && P.dst_s[d] == 1 && P.src_nf == 1 && P.dst_nf == 1 && P.nf_in == 1 && P.nf_out == 1 &&
-          P.r2r_stride == 1 && P.pen_in == P.pen_out && P.pen_in >= n && P.pen_in <= n + 2 && P.gshift >= 0 && P.sshift >= 0))
+    if (!(P.src_nf == 1 && P.dst_nf == 1 && P.nf_in == 1 && P.nf_out == 1 && P.r2r_stride == 1 && P.pen_in == P.pen_out &&
+          P.pen_in >= n && P.pen_in <= n + 2 && P.gshift >= 0 && P.sshift >= 0))
         return false;
     if (P.si < 0 || P.si > 1 || P.so < 0 || P.so > 1 || P.si + n > P.pen_in || (P.post & POST_PERIOD)) return false;
     // every output position is a transform value or a point the post-correction zeroes
     const bool head = P.so == 0 || (P.post & POST_FIRST);
     const bool tail = P.so + n == P.pen_out || (P.so + n == P.pen_out - 1 && (P.post & POST_LAST));
     return head && tail;
+}
+}  // namespace
+
+bool row_r2r_fits(const PassDesc& P) { return r2r_plan_fits(P) && P.src_s[P.d] == 1 && P.dst_s[P.d] == 1; }
+
+bool col_r2r_fits(const PassDesc& P) {
+    return r2r_plan_fits(P) && P.fa != P.d && P.src_s[P.fa] == 1 && P.dst_s[P.fa] == 1 && (P.n1 * P.n2) % P.na == 0;
 }
 
 // pre / post twiddles of the row r2r kernels, (H + 1) complex each, exact to rounding
@@ -507,14 +540,29 @@ void launch_col_real(const PassDesc& P, hipStream_t s) {
 
 void launch_row_r2r(const PassDesc& P, hipStream_t s) {
     switch (P.n / 2) {
-        case 16: launch_row_h<16, 128>(P, s); break;
-        case 32: launch_row_h<32, 64>(P, s); break;
-        case 64: launch_row_h<64, 32>(P, s); break;
-        case 128: launch_row_h<128, 16>(P, s); break;
-        case 256: launch_row_h<256, 8>(P, s); break;
-        case 512: launch_row_h<512, 4>(P, s); break;
-        case 1024: launch_row_h<1024, 2>(P, s); break;
+        case 16: launch_pen_h<16, 128, false>(P, s); break;
+        case 32: launch_pen_h<32, 64, false>(P, s); break;
+        case 64: launch_pen_h<64, 32, false>(P, s); break;
+        case 128: launch_pen_h<128, 16, false>(P, s); break;
+        case 256: launch_pen_h<256, 8, false>(P, s); break;
+        case 512: launch_pen_h<512, 4, false>(P, s); break;
+        case 1024: launch_pen_h<1024, 2, false>(P, s); break;
         default: fail("row r2r: length %d", P.n);
+    }
+    HIP_CHECK(hipGetLastError());
+}
+
+// strided pencils: R pencils side by side, LDS (H + 1) R complex values (at most 33 KB)
+void launch_col_r2r(const PassDesc& P, hipStream_t s) {
+    switch (P.n / 2) {
+        case 16: launch_pen_h<16, 32, true>(P, s); break;
+        case 32: launch_pen_h<32, 16, true>(P, s); break;
+        case 64: launch_pen_h<64, 16, true>(P, s); break;
+        case 128: launch_pen_h<128, 16, true>(P, s); break;
+        case 256: launch_pen_h<256, 8, true>(P, s); break;
+        case 512: launch_pen_h<512, 4, true>(P, s); break;
+        case 1024: launch_pen_h<1024, 2, true>(P, s); break;
+        default: fail("column r2r: length %d", P.n);
     }
     HIP_CHECK(hipGetLastError());
 }

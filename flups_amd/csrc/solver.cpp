@@ -29,6 +29,8 @@ bool row_rc_fits(const PassDesc& P);
 void launch_row_rc(const PassDesc& P, hipStream_t s);
 std::vector<double> row_r2r_table(int kind, int n);
 void launch_row_r2r(const PassDesc& P, hipStream_t s);
+bool col_r2r_fits(const PassDesc& P);
+void launch_col_r2r(const PassDesc& P, hipStream_t s);
 FastPlan* make_fast_plan(SolverImpl& s);   // nullptr if the fused pipeline does not apply
 void destroy_fast_plan(FastPlan* f);
 void fast_solve(FastPlan* f, double* field, const double* rhs, const View& uin, const View& uout, hipStream_t s,
@@ -332,6 +334,14 @@ PassDesc SolverImpl::pass_desc(const DirPlan& p, int comp, bool forward, const V
                 P.src_s[fb] % 2 == 0 && P.dst_s[fa] % 2 == 0 && P.dst_s[fb] % 2 == 0;
         return P;
     }
+    // strided real-to-real pencils, neighbours along a unit-stride dimension: the same transforms
+    // on tiles of neighbouring pencils
+    if (!std::getenv("FLUPS_GENERIC_PASS") && col_r2r_fits(P)) {
+        P.fastk = 4;
+        P.tw = twiddles(P.n / 2);
+        P.aux = r2r_table(P.kind, P.n);
+        return P;
+    }
     // real-to-complex pencils along the contiguous direction: packed half-length FFT in registers
     if (!std::getenv("FLUPS_GENERIC_PASS") && row_rc_fits(P)) {
         P.fastk = 3;
@@ -366,6 +376,7 @@ void SolverImpl::run_pass(PassDesc P) {
     if (P.fastk == 1) launch_row_r2r(P, stream_);
     else if (P.fastk == 2) launch_col_real(P, stream_);
     else if (P.fastk == 3) launch_row_rc(P, stream_);
+    else if (P.fastk == 4) launch_col_r2r(P, stream_);
     else launch_generic(P, stream_);
 }
 

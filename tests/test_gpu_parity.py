@@ -571,12 +571,21 @@ ROW_CASES = [
     ((64, 32, 32), [[3, 3], [1, 1], [4, 4]], 1, 0),
     # node-centred periodic: r2c with the periodic copy
     ((65, 33, 17), [[3, 3], [3, 3], [3, 3]], 0, 0),
+    # symmetric y or z on strided pencils: tiles of neighbouring pencils (ragged last tile,
+    # semi-unbounded shift, long pencils with narrow tiles)
+    ((32, 32, 128), [[4, 4], [3, 3], [0, 1]], 1, 2),
+    ((33, 64, 16), [[3, 3], [1, 1], [4, 4]], 1, 0),
+    ((32, 64, 32), [[3, 3], [0, 4], [4, 4]], 1, 2),
+    ((20, 16, 64), [[4, 4], [4, 1], [1, 0]], 1, 4),
+    ((16, 32, 512), [[4, 4], [4, 4], [1, 1]], 1, 0),
+    ((8, 2048, 4), [[3, 3], [0, 0], [3, 3]], 1, 0),
 ]
 
 
 @pytest.mark.parametrize("case", ROW_CASES, ids=["ee", "oo", "eo-256", "oe", "sym3", "oo-mix", "ee-pery", "odd-x-pery",
                                                  "perz-128", "pery-256", "even-unb", "unb-odd", "odd-unb-ee", "ee-unby-perz",
-                                                 "oe-unby-unbz", "unbx-eey", "perx-ooy", "node-per"])
+                                                 "oe-unby-unbz", "unbx-eey", "perx-ooy", "node-per", "eoz-128", "ragged-ooy",
+                                                 "eunb-y", "unbo-y-oez", "ooz-512", "eey-2048"])
 @pytest.mark.parametrize("misalign", [False, True])
 def test_row_r2r_kernel_vs_generic_and_oracle(case, misalign, monkeypatch):
     """the row kernel of the contiguous real-to-real passes (one half-length complex FFT with pre-
