@@ -145,6 +145,8 @@ def lib():
         L.flups_get_solve_bytes.argtypes = [c_void_p, c_int]
         L.flups_get_path.argtypes = [c_void_p]
         L.flups_get_path.restype = c_int
+        L.flups_get_pass_kernels.argtypes = [c_void_p, POINTER(c_int), c_int]
+        L.flups_get_pass_kernels.restype = c_int
         L.flups_get_last_solve_ns.argtypes = [c_void_p]
         L.flups_get_last_solve_ns.restype = c_double
         L.flups_get_last_error.restype = c_char_p
@@ -538,6 +540,14 @@ class Solver:
     def path(self):
         """0 = general pencil path (FP64), 1 = fused pipeline FP64, 2 = fused FP32, 3 = general + fused last direction"""
         return lib().flups_get_path(self._solver)
+
+    @property
+    def pass_kernels(self):
+        """kernel of each general-path pass (forward passes, then backward): 0 generic, 1 contiguous r2r,
+        2 paired real columns, 3 contiguous r2c / c2r, 4 strided r2r, 5 strided c2c, -1 none"""
+        buf = (c_int * 8)()
+        n = lib().flups_get_pass_kernels(self._solver, buf, 8)
+        return list(buf[:max(n, 0)])
 
     @property
     def last_solve_ns(self):

@@ -206,6 +206,12 @@ void flups_get_spectralInfo(FLUPS_Solver* s, double kfact[3], double koffset[3],
     if (s) s->impl->spectral_info(kfact, koffset, symstart);
 }
 int flups_get_path(FLUPS_Solver* s) { return s ? s->impl->path() : -1; }
+int flups_get_pass_kernels(FLUPS_Solver* s, int* kinds, int max) {
+    if (!s) return -1;
+    const std::vector<int>& k = s->impl->pass_kernels();
+    for (int i = 0; i < (int)k.size() && i < max; ++i) kinds[i] = k[i];
+    return (int)k.size();
+}
 // ---- advanced interface (src/flups.h:467-578) ----
 double* flups_get_innerBuffer(FLUPS_Solver* s) {
     return guard("flups_get_innerBuffer", [&]() -> double* {

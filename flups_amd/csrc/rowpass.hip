@@ -33,6 +33,11 @@
 // and a + i b = IFFT_n(Z). TX pairs side by side per block: each row of a
 // tile is one contiguous run of 2 TX reals or complex values.
 //
+// col_c2c: complex-to-complex transforms of length M (a power of two in [32, 2048]) along a
+// strided direction, TX neighbouring pencils per block, with the zero padding of the gather
+// and the truncation of the scatter (the second and third transforms of unbounded and
+// periodic directions in the general path).
+//
 // row_rc: r2c / c2r of even length n = 2H (H a power of two in [16, 1024]) along a
 // contiguous pencil through the packed complex FFT of length H: z[m] = x[2m] + i x[2m+1],
 // X[k] = (Z[k] + conj Z[H-k])/2 + e^{-2 pi i k/n} (Z[k] - conj Z[H-k])/(2i), k <= H; the
@@ -392,6 +397,44 @@ __global__ void __launch_bounds__(TX * M / 8) col_real(PassDesc P) {
     }
 }
 
+// Strided complex pencils, TX neighbours along the unit-stride dimension fa per block,
+// interleaved in LDS; the gather zero-pads (rows below gshift or from src_nd on read as
+// zero) and the scatter truncates (pencil position p lands in row p - sshift when that
+// row is below dst_nd). HIN: rows M/2 and up are padding (gshift 0, src_nd <= M/2), the
+// first stage skips them; HOUT: only rows below M/2 are stored (sshift 0, dst_nd <= M/2).
+template <int M, int TX, bool FWD, bool HIN, bool HOUT>
+__global__ void __launch_bounds__(TX * M / 8) col_c2c(PassDesc P) {
+    constexpr int T = M / 8;
+    __shared__ C2<double> lds[M * TX];
+    const int c = threadIdx.x % TX, j = threadIdx.x / TX;
+    const int ntile = (P.na + TX - 1) / TX;
+    const int tile = blockIdx.x % ntile, jb = blockIdx.x / ntile;
+    int ja = tile * TX + c;
+    const bool live = ja < P.na;
+    if (!live) ja = P.na - 1;   // dead columns read the last one and store nothing
+    const C2<double>* src =
+        reinterpret_cast<const C2<double>*>(P.src) + (long long)ja * P.src_s[P.fa] + (long long)jb * P.src_s[P.fb];
+    C2<double>* dst = reinterpret_cast<C2<double>*>(P.dst) + (long long)ja * P.dst_s[P.fa] + (long long)jb * P.dst_s[P.fb];
+    const long long sd = P.src_s[P.d], dd = P.dst_s[P.d];
+    C2<double> v[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        if (HIN && i >= 4) {
+            v[i] = {0.0, 0.0};
+            continue;
+        }
+        const int ks = j + i * T - P.gshift;
+        v[i] = (ks >= 0 && ks < P.src_nd) ? src[ks * sd] : C2<double>{0.0, 0.0};
+    }
+    column_fft<double, M, FWD, TX, HIN>(v, lds + c, j, reinterpret_cast<const C2<double>*>(P.tw));
+    if (!live) return;
+#pragma unroll
+    for (int i = 0; i < (HOUT ? 4 : 8); ++i) {
+        const int k = j + i * T - P.sshift;
+        if (k >= 0 && k < P.dst_nd) dst[k * dd] = v[i];
+    }
+}
+
 template <int H, int R, bool COL>
 void launch_pen_h(const PassDesc& P, hipStream_t s) {
     const int npen = P.n1 * P.n2;
@@ -424,6 +467,21 @@ void launch_col_real_m(const PassDesc& P, hipStream_t s) {
     dim3 grid(ntile * nb);
     if (P.op == OP_R2C) hipLaunchKernelGGL((col_real<M, TX, true>), grid, dim3(TX * M / 8), 0, s, P);
     else hipLaunchKernelGGL((col_real<M, TX, false>), grid, dim3(TX * M / 8), 0, s, P);
+}
+
+template <int M, int TX>
+void launch_col_c2c_m(const PassDesc& P, hipStream_t s) {
+    const int ntile = (P.na + TX - 1) / TX;
+    dim3 grid(ntile * (P.n1 * P.n2 / P.na));
+    dim3 block(TX * M / 8);
+    const bool hin = P.gshift == 0 && P.src_nd <= M / 2, hout = P.sshift == 0 && P.dst_nd <= M / 2;
+    if (P.sign == FWD) {
+        if (hin) hipLaunchKernelGGL((col_c2c<M, TX, true, true, false>), grid, block, 0, s, P);
+        else hipLaunchKernelGGL((col_c2c<M, TX, true, false, false>), grid, block, 0, s, P);
+    } else {
+        if (hout) hipLaunchKernelGGL((col_c2c<M, TX, false, false, true>), grid, block, 0, s, P);
+        else hipLaunchKernelGGL((col_c2c<M, TX, false, false, false>), grid, block, 0, s, P);
+    }
 }
 
 }  // namespace
@@ -534,6 +592,30 @@ void launch_col_real(const PassDesc& P, hipStream_t s) {
         case 512: launch_col_real_m<512, 4>(P, s); break;     // LDS (M + 2) TX complex: 33 KB at most
         case 1024: launch_col_real_m<1024, 2>(P, s); break;
         default: fail("col real: length %d", P.n);
+    }
+    HIP_CHECK(hipGetLastError());
+}
+
+bool col_c2c_fits(const PassDesc& P) {
+    const int n = P.n;
+    if (P.op != OP_C2C || n < 32 || n > 2048 || (n & (n - 1)) || P.M != n) return false;
+    const int d = P.d, fa = P.fa;
+    if (fa == d || P.src_s[fa] != 1 || P.dst_s[fa] != 1 || (P.n1 * P.n2) % P.na != 0) return false;
+    if (!(P.nf_in == 2 && P.nf_out == 2 && P.src_nf == 2 && P.dst_nf == 2 && P.pen_in == n && P.pen_out == n)) return false;
+    return P.si == 0 && P.so == 0 && P.post == 0 && P.gshift >= 0 && P.sshift >= 0 && P.src_nd >= 1 && P.dst_nd >= 1;
+}
+
+// LDS M TX complex values: 64 KB at most
+void launch_col_c2c(const PassDesc& P, hipStream_t s) {
+    switch (P.n) {
+        case 32: launch_col_c2c_m<32, 16>(P, s); break;
+        case 64: launch_col_c2c_m<64, 16>(P, s); break;
+        case 128: launch_col_c2c_m<128, 8>(P, s); break;
+        case 256: launch_col_c2c_m<256, 8>(P, s); break;
+        case 512: launch_col_c2c_m<512, 8>(P, s); break;
+        case 1024: launch_col_c2c_m<1024, 4>(P, s); break;
+        case 2048: launch_col_c2c_m<2048, 2>(P, s); break;
+        default: fail("column c2c: length %d", P.n);
     }
     HIP_CHECK(hipGetLastError());
 }

@@ -30,6 +30,8 @@ void launch_row_rc(const PassDesc& P, hipStream_t s);
 std::vector<double> row_r2r_table(int kind, int n);
 void launch_row_r2r(const PassDesc& P, hipStream_t s);
 bool col_r2r_fits(const PassDesc& P);
+bool col_c2c_fits(const PassDesc& P);
+void launch_col_c2c(const PassDesc& P, hipStream_t s);
 void launch_col_r2r(const PassDesc& P, hipStream_t s);
 FastPlan* make_fast_plan(SolverImpl& s);   // nullptr if the fused pipeline does not apply
 void destroy_fast_plan(FastPlan* f);
@@ -342,6 +344,11 @@ PassDesc SolverImpl::pass_desc(const DirPlan& p, int comp, bool forward, const V
         P.aux = r2r_table(P.kind, P.n);
         return P;
     }
+    // strided complex pencils: tiles of neighbouring pencils through the register FFT
+    if (!std::getenv("FLUPS_GENERIC_PASS") && col_c2c_fits(P)) {
+        P.fastk = 5;
+        return P;
+    }
     // real-to-complex pencils along the contiguous direction: packed half-length FFT in registers
     if (!std::getenv("FLUPS_GENERIC_PASS") && row_rc_fits(P)) {
         P.fastk = 3;
@@ -377,6 +384,7 @@ void SolverImpl::run_pass(PassDesc P) {
     else if (P.fastk == 2) launch_col_real(P, stream_);
     else if (P.fastk == 3) launch_row_rc(P, stream_);
     else if (P.fastk == 4) launch_col_r2r(P, stream_);
+    else if (P.fastk == 5) launch_col_c2c(P, stream_);
     else launch_generic(P, stream_);
 }
 
@@ -620,6 +628,8 @@ void SolverImpl::setup() {
             const size_t n = std::max(hat_[ip].count(), stage_[ip].count());
             HIP_CHECK(hipMalloc(&dstage_[ip], n * sizeof(double)));
         }
+        warm_tables();
+        warmed_ = true;
     }
     HIP_CHECK(hipStreamSynchronize(stream_));
     setup_done_ = true;
@@ -806,15 +816,21 @@ void SolverImpl::prepare(int type) {
 // select exist before a solve is queued (a solve under graph capture may not allocate)
 void SolverImpl::warm_tables() {
     const View u = user_view(dstage_[0]);
+    std::vector<int> fk(2 * ndim_, 0);
     for (int ip = 0; ip < ndim_; ++ip) {
         const View prev = (ip == 0) ? u : dense_view(dstage_[ip - 1], stage_[ip - 1]);
         const View cur = dense_view(dstage_[ip], stage_[ip]);
         for (int c = 0; c < lda_; ++c) {
-            (void)pass_desc(fwd_[ip], c, true, prev, cur, hat_[ip], stage_[ip], fwd_[ip].fieldstart, 0);
-            (void)pass_desc(bwd_[ip], c, false, cur, prev, stage_[ip], hat_[ip], 0, fwd_[ip].fieldstart);
+            const PassDesc f = pass_desc(fwd_[ip], c, true, prev, cur, hat_[ip], stage_[ip], fwd_[ip].fieldstart, 0);
+            const PassDesc b = pass_desc(bwd_[ip], c, false, cur, prev, stage_[ip], hat_[ip], 0, fwd_[ip].fieldstart);
             if (odiff_ != NOD) (void)pass_desc(bwdd_[ip], c, false, cur, prev, stage_[ip], hat_[ip], 0, fwd_[ip].fieldstart);
+            if (c == 0) {
+                fk[ip] = f.op == OP_NONE ? -1 : f.fastk;
+                fk[ndim_ + ip] = b.op == OP_NONE ? -1 : b.fastk;
+            }
         }
     }
+    pass_kernels_ = fk;
 }
 
 // queue the solve's kernels on the solver's stream; field and rhs are device

@@ -62,6 +62,8 @@ class SolverImpl {
     void spectral_info(double kf[3], double ko[3], double ss[3]) const;
     // 0 general FP64, 1 fused FP64, 2 fused FP32, 3 general with the last direction fused with the multiply
     int path() const { return fast_ ? (fp32_ ? 2 : 1) : hybrid_ ? 3 : 0; }
+    // kernel of each general-path pass (component 0): forward passes in order, then the backward ones
+    const std::vector<int>& pass_kernels() const { return pass_kernels_; }
     double last_solve_ns() const { return last_ns_; }
     int ndim() const { return ndim_; }
     // per-kernel timing of the fused pipeline (HIP events on the solve stream)
@@ -104,6 +106,7 @@ class SolverImpl {
     double* dgreen_ = nullptr;
     double* dstaging_ = nullptr;   // rhs/field copies for host pointers
     double* dinner_ = nullptr;     // inner buffer of the advanced interface
+    std::vector<int> pass_kernels_;
     bool use_graph_ = false, profiling_ = false, warmed_ = false;   // FLUPS_GRAPH=1: replay solves as graphs
     hipStream_t cap_ = nullptr;    // capture stream of the solve graph
     hipGraphExec_t gexec_ = nullptr;

@@ -163,6 +163,12 @@ void        flups_set_precision(FLUPS_Solver* s, int fp32);
 /* 0 = general pencil path (FP64), 1 = fused pipeline in FP64, 2 = fused in FP32,
    3 = general passes with the last direction fused with the multiply (FP64) */
 int         flups_get_path(FLUPS_Solver* s);
+/* kernel of each pass of the general path after flups_setup (component 0; the forward
+   passes in order, then the backward ones): 0 generic pencil kernel, 1 contiguous
+   real-to-real, 2 paired real columns (r2c / c2r), 3 contiguous r2c / c2r, 4 strided
+   real-to-real, 5 strided c2c, -1 no transform. Fills at most max entries, returns the
+   number of passes (0 when the fused pipeline runs everything, -1 for a NULL solver) */
+int         flups_get_pass_kernels(FLUPS_Solver* s, int* kinds, int max);
 /* 1: flups_setup times the fused pipeline's candidate schedules on this problem and keeps
    the fastest (remembered per shape for the process; FLUPS_AUTOTUNE=1 turns it on for all) */
 void        flups_set_autotune(FLUPS_Solver* s, int on);

@@ -547,51 +547,58 @@ def test_fused_x_slabs_bitwise_and_oracle(case, monkeypatch):
 
 
 ROW_CASES = [
-    # (nglob, bc, center, kernel): symmetric x (cell-centred: REDFT10/01, RODFT10/01, REDFT11, RODFT11)
-    ((64, 32, 32), [[0, 0], [3, 3], [4, 4]], 1, 2),
-    ((64, 32, 32), [[1, 1], [3, 3], [4, 4]], 1, 2),
-    ((256, 16, 16), [[0, 1], [3, 3], [4, 4]], 1, 8),
-    ((32, 32, 32), [[1, 0], [4, 4], [3, 3]], 1, 0),
-    ((128, 16, 16), [[0, 1], [1, 0], [0, 0]], 1, 0),
-    ((64, 24, 16), [[1, 1], [0, 1], [4, 4]], 1, 4),
+    # (nglob, bc, center, kernel, fast kernel that must run: flups_get_pass_kernels codes)
+    # symmetric x (cell-centred: REDFT10/01, RODFT10/01, REDFT11, RODFT11)
+    ((64, 32, 32), [[0, 0], [3, 3], [4, 4]], 1, 2, 1),
+    ((64, 32, 32), [[1, 1], [3, 3], [4, 4]], 1, 2, 1),
+    ((256, 16, 16), [[0, 1], [3, 3], [4, 4]], 1, 8, 1),
+    ((32, 32, 32), [[1, 0], [4, 4], [3, 3]], 1, 0, 1),
+    ((128, 16, 16), [[0, 1], [1, 0], [0, 0]], 1, 0, 1),
+    ((64, 24, 16), [[1, 1], [0, 1], [4, 4]], 1, 4, 1),
     # periodic y or z after symmetric directions: r2c / c2r of paired real columns
-    ((32, 64, 32), [[0, 0], [3, 3], [4, 4]], 1, 2),
-    ((33, 64, 16), [[1, 1], [3, 3], [4, 4]], 1, 0),
-    ((64, 32, 128), [[0, 1], [1, 0], [3, 3]], 1, 0),
-    ((256, 256, 8), [[0, 1], [3, 3], [0, 0]], 1, 0),
+    ((32, 64, 32), [[0, 0], [3, 3], [4, 4]], 1, 2, 2),
+    ((33, 64, 16), [[1, 1], [3, 3], [4, 4]], 1, 0, 2),
+    ((64, 32, 128), [[0, 1], [1, 0], [3, 3]], 1, 0, 2),
+    ((256, 256, 8), [[0, 1], [3, 3], [0, 0]], 1, 0, 2),
     # semi-unbounded x (transforms of length 2N over the zero-padded, possibly shifted field)
-    ((32, 32, 32), [[0, 4], [3, 3], [4, 4]], 1, 2),
-    ((64, 16, 32), [[4, 1], [4, 4], [3, 3]], 1, 4),
-    ((32, 64, 16), [[1, 4], [0, 0], [4, 4]], 1, 0),
+    ((32, 32, 32), [[0, 4], [3, 3], [4, 4]], 1, 2, 1),
+    ((64, 16, 32), [[4, 1], [4, 4], [3, 3]], 1, 4, 1),
+    ((32, 64, 16), [[1, 4], [0, 0], [4, 4]], 1, 0, 4),
     # unbounded y after a symmetric x: zero-padded r2c / truncated c2r of paired real columns
-    ((32, 64, 32), [[0, 0], [4, 4], [3, 3]], 1, 2),
-    ((64, 32, 32), [[1, 0], [4, 4], [4, 4]], 1, 8),
+    ((32, 64, 32), [[0, 0], [4, 4], [3, 3]], 1, 2, 2),
+    ((64, 32, 32), [[1, 0], [4, 4], [4, 4]], 1, 8, 2),
     # symmetric y first, then a contiguous r2c / c2r along x (packed half-length FFT)
-    ((64, 32, 32), [[4, 4], [0, 0], [4, 4]], 1, 2),
-    ((64, 32, 32), [[3, 3], [1, 1], [4, 4]], 1, 0),
+    ((64, 32, 32), [[4, 4], [0, 0], [4, 4]], 1, 2, 4),
+    ((64, 32, 32), [[3, 3], [1, 1], [4, 4]], 1, 0, 4),
     # node-centred periodic: r2c with the periodic copy
-    ((65, 33, 17), [[3, 3], [3, 3], [3, 3]], 0, 0),
+    ((65, 33, 17), [[3, 3], [3, 3], [3, 3]], 0, 0, 3),
     # symmetric y or z on strided pencils: tiles of neighbouring pencils (ragged last tile,
     # semi-unbounded shift, long pencils with narrow tiles)
-    ((32, 32, 128), [[4, 4], [3, 3], [0, 1]], 1, 2),
-    ((33, 64, 16), [[3, 3], [1, 1], [4, 4]], 1, 0),
-    ((32, 64, 32), [[3, 3], [0, 4], [4, 4]], 1, 2),
-    ((20, 16, 64), [[4, 4], [4, 1], [1, 0]], 1, 4),
-    ((16, 32, 512), [[4, 4], [4, 4], [1, 1]], 1, 0),
-    ((8, 2048, 4), [[3, 3], [0, 0], [3, 3]], 1, 0),
+    ((32, 32, 128), [[4, 4], [3, 3], [0, 1]], 1, 2, 4),
+    ((33, 64, 16), [[3, 3], [1, 1], [4, 4]], 1, 0, 4),
+    ((32, 64, 32), [[3, 3], [0, 4], [4, 4]], 1, 2, 4),
+    ((20, 16, 64), [[4, 4], [4, 1], [1, 0]], 1, 4, 4),
+    ((16, 32, 512), [[4, 4], [4, 4], [1, 1]], 1, 0, 4),
+    ((8, 2048, 4), [[3, 3], [0, 0], [3, 3]], 1, 0, 4),
+    # complex-to-complex along strided directions (zero-padded / truncated, plain periodic, 2-D)
+    ((64, 128, 24), [[4, 4], [4, 4], [4, 4]], 1, 2, 5),
+    ((48, 32, 20), [[3, 3], [3, 3], [3, 3]], 1, 0, 5),
+    ((256, 512, 1), [[4, 4], [4, 4], [9, 9]], 1, 2, 5),
+    ((128, 1024, 1), [[3, 3], [4, 4], [9, 9]], 1, 4, 5),
 ]
 
 
 @pytest.mark.parametrize("case", ROW_CASES, ids=["ee", "oo", "eo-256", "oe", "sym3", "oo-mix", "ee-pery", "odd-x-pery",
                                                  "perz-128", "pery-256", "even-unb", "unb-odd", "odd-unb-ee", "ee-unby-perz",
                                                  "oe-unby-unbz", "unbx-eey", "perx-ooy", "node-per", "eoz-128", "ragged-ooy",
-                                                 "eunb-y", "unbo-y-oez", "ooz-512", "eey-2048"])
+                                                 "eunb-y", "unbo-y-oez", "ooz-512", "eey-2048", "c2c-y-256", "c2c-y-32",
+                                                 "c2c-2d-1024", "c2c-2d-2048"])
 @pytest.mark.parametrize("misalign", [False, True])
-def test_row_r2r_kernel_vs_generic_and_oracle(case, misalign, monkeypatch):
-    """the row kernel of the contiguous real-to-real passes (one half-length complex FFT with pre-
-    and post-twiddles) against the generic pass kernel (FLUPS_GENERIC_PASS) and the oracle at
-    1e-12; misaligned user rows take its scalar loads and stores"""
-    nglob, bc, center, kernel = case
+def test_fast_pass_kernels_vs_generic_and_oracle(case, misalign, monkeypatch):
+    """the fast kernels of the general path (real-to-real pencils and tiles, paired real columns,
+    packed r2c / c2r rows, complex columns) against the generic pass kernel (FLUPS_GENERIC_PASS)
+    and the oracle at 1e-12; misaligned user rows take the scalar loads and stores"""
+    nglob, bc, center, kernel, want = case
     rng = np.random.default_rng(31)
     n0, n1, n2 = nglob
     rhs = rng.normal(size=(1, n2, n1, n0))
@@ -608,6 +615,11 @@ def test_row_r2r_kernel_vs_generic_and_oracle(case, misalign, monkeypatch):
         s = fl.Solver(topo, bc, h, L, center_type=[center] * 3)
         s.set_green_type(kernel)
         s.setup()
+        kinds = s.pass_kernels
+        if generic:
+            assert all(k <= 0 for k in kinds), kinds
+        else:
+            assert want in kinds, (want, kinds)
         n = rhs.size
         off = 1 if misalign else 0   # one double in: rows start 8 bytes off a 16-byte boundary
         flat = np.zeros(n + 1)
