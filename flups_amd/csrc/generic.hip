@@ -15,6 +15,8 @@
 // r2c/c2r of even length n use the half-length packed complex FFT.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "core.hpp"
 
 namespace flups {
@@ -43,17 +45,66 @@ __device__ __forceinline__ cd twid(const double* tw, unsigned e, int sign) {
     return sign > 0 ? conj(w) : w;
 }
 
-// one Stockham stage, all outputs of all pencils of the block
+// One radix-p butterfly of a Stockham stage: inputs a[j + q mp] (q < p, j = b span + k),
+// twiddled by w_sp^(q k), then a p-point DFT whose output u lands at o[u span]:
+//   out[b sp + u span + k] = sum_q a[j + q mp] w_sp^(q k) w_p^(q u),  sp = span p, mp = M / p.
+template <int P>
+__device__ __forceinline__ void butterfly(const cd* a, cd* o, unsigned j, unsigned mp, unsigned span, unsigned k,
+                                          unsigned scale, const double* tw, int sign, unsigned M) {
+    cd x[P];
+    x[0] = a[j];
+#pragma unroll
+    for (int q = 1; q < P; ++q) x[q] = mul(a[j + q * mp], twid(tw, q * k * scale, sign));   // q k scale < M
+    if (P == 2) {
+        o[0] = add(x[0], x[1]);
+        o[span] = sub(x[0], x[1]);
+    } else if (P == 4) {
+        const cd s0 = add(x[0], x[2]), d0 = sub(x[0], x[2]), s1 = add(x[1], x[3]), t = sub(x[1], x[3]);
+        const cd d1 = sign > 0 ? mk(-t.y, t.x) : mk(t.y, -t.x);   // w_4 (x1 - x3), w_4 = +-i
+        o[0] = add(s0, s1);
+        o[span] = add(d0, d1);
+        o[2 * span] = sub(s0, s1);
+        o[3 * span] = sub(d0, d1);
+    } else {
+        cd w[P];
+        w[0] = mk(1.0, 0.0);
+#pragma unroll
+        for (int m = 1; m < P; ++m) w[m] = twid(tw, m * (M / P), sign);   // w_p^m
+#pragma unroll
+        for (int u = 0; u < P; ++u) {
+            cd acc = x[0];
+#pragma unroll
+            for (int q = 1; q < P; ++q) acc = add(acc, mul(x[q], w[(q * u) % P]));
+            o[u * span] = acc;
+        }
+    }
+}
+
+// one Stockham stage, all butterflies of all pencils of the block (radix 2, 3, 4, 5, 7
+// unrolled; any other prime one output at a time)
 __device__ void stockham_stage(const cd* in, cd* out, int M, int p, int span, const double* tw, int sign, int npen) {
-    const unsigned mp = M / p;
-    const unsigned sp = span * p;
-    const unsigned scale = M / sp;
-    const unsigned um = M;
+    const unsigned um = M, mp = M / p, sp = span * p, scale = M / sp, us = span;
+    if (p == 2 || p == 3 || p == 4 || p == 5 || p == 7) {
+        for (unsigned i = threadIdx.x; i < (unsigned)npen * mp; i += blockDim.x) {
+            const unsigned pen = i / mp, j = i - pen * mp;
+            const unsigned b = j / us, k = j - b * us;
+            const cd* a = in + (size_t)pen * M;
+            cd* o = out + (size_t)pen * M + b * sp + k;
+            switch (p) {
+                case 2: butterfly<2>(a, o, j, mp, us, k, scale, tw, sign, um); break;
+                case 3: butterfly<3>(a, o, j, mp, us, k, scale, tw, sign, um); break;
+                case 4: butterfly<4>(a, o, j, mp, us, k, scale, tw, sign, um); break;
+                case 5: butterfly<5>(a, o, j, mp, us, k, scale, tw, sign, um); break;
+                default: butterfly<7>(a, o, j, mp, us, k, scale, tw, sign, um); break;
+            }
+        }
+        return;
+    }
     for (unsigned i = threadIdx.x; i < (unsigned)(npen * M); i += blockDim.x) {
         const unsigned pen = i / um, o = i - pen * um;
         const unsigned b = o / sp, r = o - b * sp;          // r = k + u * span
-        const unsigned u = r / span, k = r - u * span;
-        const unsigned j = b * span + k;
+        const unsigned k = r % us;
+        const unsigned j = b * us + k;
         const cd* a = in + (size_t)pen * M;
         const unsigned base = r * scale;                    // < M
         cd acc = a[j];
@@ -64,7 +115,6 @@ __device__ void stockham_stage(const cd* in, cd* out, int M, int p, int span, co
             acc = add(acc, mul(a[j + q * mp], twid(tw, e, sign)));
         }
         out[(size_t)pen * M + o] = acc;
-        (void)u;
     }
 }
 
@@ -281,7 +331,10 @@ void launch_generic(const PassDesc& P, hipStream_t s) {
     const int npen = P.n1 * P.n2;
     const int blocks = (npen + P.ppb - 1) / P.ppb;
     const size_t lds = generic_lds_bytes(P, P.ppb);
-    hipLaunchKernelGGL(generic_pass, dim3(blocks), dim3(256), lds, s, P);
+    // threads: enough for one element (or butterfly) each of the block's pencils, whole waves, at most 256
+    const int work = P.ppb * std::max(P.M, P.pen_in);
+    const int threads = std::min(256, std::max(64, (work + 63) / 64 * 64));
+    hipLaunchKernelGGL(generic_pass, dim3(blocks), dim3(threads), lds, s, P);
     HIP_CHECK(hipGetLastError());
 }
 

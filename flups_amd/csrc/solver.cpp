@@ -375,6 +375,8 @@ PassDesc SolverImpl::pass_desc(const DirPlan& p, int comp, bool forward, const V
     const size_t cap = std::max(per_pen, std::min(lds_budget_, generic_lds_cap_));
     int ppb = (int)std::min<size_t>(16, cap / std::max<size_t>(per_pen, 1));
     while (ppb > 1 && generic_lds_bytes(P, ppb) > cap) --ppb;
+    // small problems: fewer pencils per block, so that about 1024 blocks share the 256 CUs
+    ppb = std::min(ppb, std::max(1, (P.n1 * P.n2 + 1023) / 1024));
     P.ppb = std::max(1, std::min(ppb, P.n1 * P.n2));
     return P;
 }
