@@ -956,9 +956,11 @@ size_t fast_device_bytes(const FastPlan* f) { return f->dev_bytes; }
 // z FFT run as one z_fused launch in place on the stage array before it (the
 // general passes do the other directions). The other dim (y) may be a
 // half-spectrum (r2c) or real-to-real direction: rows are not paired then.
+// 2-D problems fuse their last direction (y) the same way, as a 3-D field with
+// one row (columns up to 2048 long, two columns per block).
 struct HybridZ {
     int M = 0, nx = 0, ny = 0, nz = 0, lda = 1, gtx = 8;
-    bool pair = false;
+    bool pair = false, twod = false;
     void* green = nullptr;   // normfact * Re G, [y'][tile][kz'][gtx]
     void* tw = nullptr;
     size_t dev_bytes = 0;
@@ -966,32 +968,36 @@ struct HybridZ {
 };
 
 HybridZ* make_hybrid_z(SolverImpl& s) {
-    if (s.ndim_ != 3 || s.fp32_) return nullptr;
+    const int nd = s.ndim_;
+    if ((nd != 3 && nd != 2) || s.fp32_) return nullptr;
     if (getenv("FLUPS_FORCE_GENERIC")) return nullptr;
-    const int ip = 2;
+    const int ip = nd - 1;
     const DirPlan &f = s.fwd_[ip], &b = s.bwd_[ip];
-    if (f.dim != 2 || !(f.type == UNBUNB || (f.type == PERPER && !s.node_)) || f.r2c) return nullptr;
+    if (f.dim != nd - 1 || !(f.type == UNBUNB || (f.type == PERPER && !s.node_)) || f.r2c) return nullptr;
     for (const DirPlan* p : {&f, &b})
         for (int c = 0; c < p->lda; ++c)
             if (p->post[c] != 0 || p->start_in[c] != 0 || p->start_out[c] != 0 || p->fieldstart != 0 || p->koffset != 0.0)
                 return nullptr;
-    const Shape& in = s.stage_[1];   // field before the z transform
-    const Shape& sp = s.stage_[2];   // spectral field (and Green's function)
-    if (in.nf != 2 || sp.nf != 2 || in.n[0] != sp.n[0] || in.n[1] != sp.n[1]) return nullptr;
+    const Shape& in = s.stage_[nd - 2];   // field before the last transform
+    const Shape& sp = s.stage_[nd - 1];   // spectral field (and Green's function)
+    if (in.nf != 2 || sp.nf != 2 || in.n[0] != sp.n[0] || (nd == 3 && in.n[1] != sp.n[1])) return nullptr;
     const int M = f.n_in[0];
-    if (!pow2_in(M, 32, 1024) || sp.n[2] != M) return nullptr;
-    if ((double)(M / 8) * in.n[0] * in.n[1] * 16.0 >= 4294967296.0) return nullptr;
+    if (!pow2_in(M, 32, nd == 2 ? 2048 : 1024) || sp.n[nd - 1] != M) return nullptr;
+    const int ny = nd == 3 ? in.n[1] : 1;
+    if ((double)(M / 8) * in.n[0] * ny * 16.0 >= 4294967296.0) return nullptr;
     auto* h = new HybridZ;
     h->M = M;
     h->nx = in.n[0];
-    h->ny = in.n[1];
-    h->nz = in.n[2];
+    h->ny = ny;
+    h->nz = in.n[nd - 1];
     h->lda = s.lda_;
     // y rows pair as (y, M1 - y) when y is a full unbounded / periodic c2c direction (G even in ky)
     const DirPlan& fy = s.fwd_[1];
-    h->pair = fy.dim == 1 && !fy.r2c && (fy.type == UNBUNB || (fy.type == PERPER && !s.node_)) && fy.koffset == 0.0 &&
-              fy.n_in[0] == h->ny;
-    h->gtx = 4;   // half-width x tiles
+    h->pair = nd == 3 && fy.dim == 1 && !fy.r2c && (fy.type == UNBUNB || (fy.type == PERPER && !s.node_)) &&
+              fy.koffset == 0.0 && fy.n_in[0] == h->ny;
+    // half-width x tiles in 3-D; two columns per block in 2-D (few columns, cache-resident field)
+    h->gtx = nd == 2 ? 2 : 4;
+    h->twod = nd == 2;
     const int ntx = (h->nx + h->gtx - 1) / h->gtx;
     const int gy = h->pair ? h->ny / 2 + 1 : h->ny, gz = M / 2 + 1;
     const size_t gn = (size_t)gy * ntx * gz * h->gtx;
@@ -1051,6 +1057,19 @@ void hybrid_z(const HybridZ* h, double* data, hipStream_t st) {
     z.nopair = h->pair ? 0 : 1;
     z.x0 = 0;
     z.gntx = (h->nx + h->gtx - 1) / h->gtx;
+    if (h->twod) {   // two columns per block (the tile width of the compressed G), XCD-grouped
+        switch (h->M) {
+            case 32: launch_col<double, 32, 2, true, true>(z, h->lda, 16, st); break;
+            case 64: launch_col<double, 64, 2, true, true>(z, h->lda, 16, st); break;
+            case 128: launch_col<double, 128, 2, true, true>(z, h->lda, 16, st); break;
+            case 256: launch_col<double, 256, 2, true, true>(z, h->lda, 16, st); break;
+            case 512: launch_col<double, 512, 2, true, true>(z, h->lda, 16, st); break;
+            case 1024: launch_col<double, 1024, 2, true, true>(z, h->lda, 16, st); break;
+            case 2048: launch_col<double, 2048, 2, true, true>(z, h->lda, 16, st); break;
+            default: fail("unsupported column length %d", h->M);
+        }
+        return;
+    }
     // half-width tiles (schedule bit 2), XCD-grouped; M = 1024 is already half width
     col_dispatch<double, 8, true, true>(h->M, z, h->lda, 4 | 16, st);
 }

@@ -827,8 +827,9 @@ void SolverImpl::warm_tables() {
             const PassDesc b = pass_desc(bwd_[ip], c, false, cur, prev, stage_[ip], hat_[ip], 0, fwd_[ip].fieldstart);
             if (odiff_ != NOD) (void)pass_desc(bwdd_[ip], c, false, cur, prev, stage_[ip], hat_[ip], 0, fwd_[ip].fieldstart);
             if (c == 0) {
-                fk[ip] = f.op == OP_NONE ? -1 : f.fastk;
-                fk[ndim_ + ip] = b.op == OP_NONE ? -1 : b.fastk;
+                const bool fused = hybrid_ && ip == ndim_ - 1;   // the last direction runs with the multiply
+                fk[ip] = fused ? -2 : f.op == OP_NONE ? -1 : f.fastk;
+                fk[ndim_ + ip] = fused ? -2 : b.op == OP_NONE ? -1 : b.fastk;
             }
         }
     }

@@ -468,10 +468,16 @@ HYBRID_CASES = [
     ((48, 40, 32), [[4, 4], [0, 4], [4, 4]], 1, 2, 1),      # y semi-unbounded first, x r2c, z fused
     ((33, 17, 33), [[1, 4], [4, 4], [4, 4]], 0, 4, 3),      # node, x semi-unbounded, lda 3
     ((32, 24, 64), [[1, 1], [3, 3], [3, 3]], 1, 0, 1),      # x ODD-ODD, y PER (r2c), z PER
+    # 2-D: the last direction is y (columns of up to 2048 points)
+    ((256, 512, 1), [[4, 4], [4, 4], [9, 9]], 1, 2, 1),     # UNB 2-D, y 1024 padded
+    ((128, 1024, 1), [[3, 3], [4, 4], [9, 9]], 1, 4, 1),    # x PER, y UNB: columns of 2048
+    ((96, 128, 1), [[3, 3], [3, 3], [9, 9]], 1, 0, 2),      # fully periodic 2-D, lda 2
+    ((80, 64, 1), [[4, 4], [4, 4], [9, 9]], 1, 8, 1),       # UNB 2-D, LGF_4, x of 160 (generic pass)
 ]
 
 
-@pytest.mark.parametrize("case", HYBRID_CASES, ids=["mixed64", "semiunb-y", "node-lda3", "sym-per-per"])
+@pytest.mark.parametrize("case", HYBRID_CASES, ids=["mixed64", "semiunb-y", "node-lda3", "sym-per-per", "2d-unb-1024",
+                                                    "2d-per-unb-2048", "2d-per-lda2", "2d-unb-lgf4"])
 def test_hybrid_last_direction_vs_general_and_oracle(case, monkeypatch):
     """general passes + the last direction fused with the multiply, against the all-general path and the oracle"""
     nglob, bc, center, kernel, lda = case
@@ -605,6 +611,9 @@ def test_fast_pass_kernels_vs_generic_and_oracle(case, misalign, monkeypatch):
     rhs -= rhs.mean()
     h = [1.0 / (max(nglob) - (1 - center))] * 3
     L = [h[0] * (m - (1 - center)) for m in nglob]
+
+    # every direction through the pass kernels (no fused last direction)
+    monkeypatch.setenv("FLUPS_FORCE_GENERIC", "1")
 
     def run(generic):
         if generic:
