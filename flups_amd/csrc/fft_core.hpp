@@ -164,6 +164,54 @@ __device__ __forceinline__ void column_fft(C2<R>* v, C2<R>* lds, int j, const C2
     }
 }
 
+// Column FFT of length H = P Q (P = 1, 3, 5 or 7; Q a power of two, at least 8) held by
+// T = Q/8 threads with 8P values each: on entry v[i] = x[j + i T], on exit v[i] = X[j + i T]
+// (i < 8P). For P > 1 (Cooley-Tukey with n = Q n1 + n2, k = k1 + P k2): the P-point DFTs
+// over n1 (a thread holds every n1 of its n2 = j + i T), the twiddles w_H^(n2 k1), the P
+// Q-point column FFTs over n2 one after the other on the same LDS, and an LDS exchange back
+// to the natural order. twq: Q-point table; twh[e * hs] = exp(-2 pi i e / H). The whole block
+// calls this together; it ends with LDS reads (sync before the next LDS write).
+template <typename R, int P, int Q, bool FWD, int ES, bool HIN = false>
+__device__ __forceinline__ void column_fft_pq(C2<R>* v, C2<R>* lds, int j, const C2<R>* twq, const C2<R>* twh, int hs) {
+    if constexpr (P == 1) {
+        column_fft<R, Q, FWD, ES, HIN>(v, lds, j, twq);
+    } else {
+        constexpr int T = Q / 8;
+        C2<R> wp[P];   // w_P^m = w_H^(m Q)
+        wp[0] = {1, 0};
+#pragma unroll
+        for (int m = 1; m < P; ++m) wp[m] = twl<R, FWD>(twh, m * Q * hs);
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            C2<R> a[P];
+#pragma unroll
+            for (int n1 = 0; n1 < P; ++n1) a[n1] = v[i + 8 * n1];
+            const int n2 = j + i * T;
+#pragma unroll
+            for (int k1 = 0; k1 < P; ++k1) {
+                C2<R> acc = a[0];
+#pragma unroll
+                for (int n1 = 1; n1 < P; ++n1) acc = cadd(acc, cmul(a[n1], wp[(n1 * k1) % P]));
+                if (k1 > 0) acc = cmul(acc, twl<R, FWD>(twh, n2 * k1 * hs));   // n2 k1 < H
+                v[i + 8 * k1] = acc;
+            }
+        }
+#pragma unroll
+        for (int k1 = 0; k1 < P; ++k1) {
+            if (k1 > 0) __syncthreads();   // the previous column FFT's last LDS reads are done
+            column_fft<R, Q, FWD, ES>(v + 8 * k1, lds, j, twq);   // v[i + 8 k1] = X[k1 + P (j + i T)]
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k1 = 0; k1 < P; ++k1)
+#pragma unroll
+            for (int i = 0; i < 8; ++i) lds[lslot<ES>(k1 + P * (j + i * T))] = v[i + 8 * k1];
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < 8 * P; ++i) v[i] = lds[lslot<ES>(j + i * T)];
+    }
+}
+
 // Workgroups are dispatched round-robin over the 8 XCDs (each with its own L2):
 // hardware block b runs on XCD b % 8. Renumber so that consecutive logical
 // blocks run on the same XCD at about the same time (they share cache lines).

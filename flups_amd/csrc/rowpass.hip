@@ -217,9 +217,10 @@ __global__ void __launch_bounds__(R * H / 8) pen_r2r(PassDesc P) {
     }
 }
 
-template <int H, int ROWS, bool FWD>
-__global__ void __launch_bounds__(ROWS * H / 8) row_rc(PassDesc P) {
-    constexpr int T = H / 8, HP = H + H / 8;
+// H = PF Q: PF = 1 (a power of two; the only one launched) or 3, 5, 7 (mixed radix, column_fft_pq)
+template <int PF, int Q, int ROWS, bool FWD>
+__global__ void __launch_bounds__(ROWS * Q / 8) row_rc(PassDesc P) {
+    constexpr int H = PF * Q, T = Q / 8, HP = H + H / 8, NV = 8 * PF;
     __shared__ C2<double> lds[HP * ROWS];
     const int rl = threadIdx.x / T, j = threadIdx.x % T;
     const int g = blockIdx.x * ROWS + rl;
@@ -230,7 +231,7 @@ __global__ void __launch_bounds__(ROWS * H / 8) row_rc(PassDesc P) {
     double* dst = P.dst + (long long)ja * P.dst_s[P.fa] * P.dst_nf + (long long)jb * P.dst_s[P.fb] * P.dst_nf;
     C2<double>* row = lds + rl * HP;
     double* rd = reinterpret_cast<double*>(row);
-    const C2<double>* tw = reinterpret_cast<const C2<double>*>(P.tw);    // H-point table
+    const C2<double>* tw = reinterpret_cast<const C2<double>*>(P.tw);    // Q-point table
     const C2<double>* twn = reinterpret_cast<const C2<double>*>(P.aux);  // n-point table
     // 1. gather: the pencil's doubles (reals for r2c, interleaved complex for c2r)
     const int nfi = P.nf_in, len = P.pen_in * nfi;
@@ -239,9 +240,9 @@ __global__ void __launch_bounds__(ROWS * H / 8) row_rc(PassDesc P) {
         rd[q] = (ks >= 0 && ks < P.src_nd) ? src[(long long)ks * nfi + (q - k * nfi)] : 0.0;
     }
     __syncthreads();
-    C2<double> v[8];
+    C2<double> v[NV];
 #pragma unroll
-    for (int i = 0; i < 8; ++i) {
+    for (int i = 0; i < NV; ++i) {
         const int m = j + i * T;
         if (FWD) {
             v[i] = {rd[2 * m], rd[2 * m + 1]};
@@ -257,22 +258,22 @@ __global__ void __launch_bounds__(ROWS * H / 8) row_rc(PassDesc P) {
             v[i] = {e.x - t.y, e.y + t.x};
         }
     }
-    column_fft<double, H, FWD, 0>(v, row, j, tw);
+    column_fft_pq<double, PF, Q, FWD, 0>(v, row, j, tw, twn, 2);   // w_H^e = twn[2 e]
     __syncthreads();
     const int nfo = P.nf_out, plen = P.pen_out;
     if (FWD) {
-        // X[k], k <= H, into the row as complex values
+        // X[k], k <= H, into the row as complex values (v holds X after the exchange)
 #pragma unroll
-        for (int i = 0; i < 8; ++i) row[j + i * T] = v[i];
+        for (int i = 0; i < NV; ++i) row[j + i * T] = v[i];
         __syncthreads();
-        C2<double> X[8], XH = {0, 0};
+        C2<double> XH = {0, 0};
 #pragma unroll
-        for (int i = 0; i < 8; ++i) {
+        for (int i = 0; i < NV; ++i) {
             const int k = j + i * T;
-            const C2<double> zk = row[k], zc = cconj(row[(H - k) & (H - 1)]);
+            const C2<double> zk = row[k], zc = cconj(row[k == 0 ? 0 : H - k]);
             const C2<double> e = {0.5 * (zk.x + zc.x), 0.5 * (zk.y + zc.y)};
             const C2<double> df = {0.5 * (zk.x - zc.x), 0.5 * (zk.y - zc.y)};
-            X[i] = cadd(e, cmul(twn[k], C2<double>{df.y, -df.x}));
+            v[i] = cadd(e, cmul(twn[k], C2<double>{df.y, -df.x}));
         }
         if (j == 0) {
             const C2<double> z0 = row[0];
@@ -280,16 +281,16 @@ __global__ void __launch_bounds__(ROWS * H / 8) row_rc(PassDesc P) {
         }
         __syncthreads();
 #pragma unroll
-        for (int i = 0; i < 8; ++i) row[j + i * T] = X[i];
+        for (int i = 0; i < NV; ++i) row[j + i * T] = v[i];
         if (j == 0) row[H] = XH;
         // positions past H + 1 complex values keep nothing (zero)
         for (int q = H + 1 + j; q < plen; q += T) row[q] = {0.0, 0.0};
         // periodic node-centred direction: the first value again at the end (n_out - 1)
-        if ((P.post & POST_PERIOD) && j == 0) row[P.n_out_plan - 1] = X[0];
+        if ((P.post & POST_PERIOD) && j == 0) row[P.n_out_plan - 1] = v[0];
     } else {
         // x[2m] = Re z[m], x[2m+1] = Im z[m]
 #pragma unroll
-        for (int i = 0; i < 8; ++i) row[j + i * T] = v[i];
+        for (int i = 0; i < NV; ++i) row[j + i * T] = v[i];
         for (int q = 2 * H + j; q < plen; q += T) rd[q] = 0.0;
         if ((P.post & POST_PERIOD) && j == 0) rd[2 * H] = v[0].x;   // x[n] = x[0]
     }
@@ -402,9 +403,10 @@ __global__ void __launch_bounds__(TX * M / 8) col_real(PassDesc P) {
 // zero) and the scatter truncates (pencil position p lands in row p - sshift when that
 // row is below dst_nd). HIN: rows M/2 and up are padding (gshift 0, src_nd <= M/2), the
 // first stage skips them; HOUT: only rows below M/2 are stored (sshift 0, dst_nd <= M/2).
-template <int M, int TX, bool FWD, bool HIN, bool HOUT>
-__global__ void __launch_bounds__(TX * M / 8) col_c2c(PassDesc P) {
-    constexpr int T = M / 8;
+// M = PF Q: PF = 1 (a power of two) or 3, 5 (mixed radix, column_fft_pq)
+template <int PF, int Q, int TX, bool FWD, bool HIN, bool HOUT>
+__global__ void __launch_bounds__(TX * Q / 8) col_c2c(PassDesc P) {
+    constexpr int M = PF * Q, T = Q / 8, NV = 8 * PF;
     __shared__ C2<double> lds[M * TX];
     const int c = threadIdx.x % TX, j = threadIdx.x / TX;
     const int ntile = (P.na + TX - 1) / TX;
@@ -416,20 +418,21 @@ __global__ void __launch_bounds__(TX * M / 8) col_c2c(PassDesc P) {
         reinterpret_cast<const C2<double>*>(P.src) + (long long)ja * P.src_s[P.fa] + (long long)jb * P.src_s[P.fb];
     C2<double>* dst = reinterpret_cast<C2<double>*>(P.dst) + (long long)ja * P.dst_s[P.fa] + (long long)jb * P.dst_s[P.fb];
     const long long sd = P.src_s[P.d], dd = P.dst_s[P.d];
-    C2<double> v[8];
+    C2<double> v[NV];
 #pragma unroll
-    for (int i = 0; i < 8; ++i) {
-        if (HIN && i >= 4) {
+    for (int i = 0; i < NV; ++i) {
+        if (HIN && i >= NV / 2) {
             v[i] = {0.0, 0.0};
             continue;
         }
         const int ks = j + i * T - P.gshift;
         v[i] = (ks >= 0 && ks < P.src_nd) ? src[ks * sd] : C2<double>{0.0, 0.0};
     }
-    column_fft<double, M, FWD, TX, HIN>(v, lds + c, j, reinterpret_cast<const C2<double>*>(P.tw));
+    column_fft_pq<double, PF, Q, FWD, TX, HIN && PF == 1>(v, lds + c, j, reinterpret_cast<const C2<double>*>(P.tw),
+                                                         reinterpret_cast<const C2<double>*>(P.aux), 1);
     if (!live) return;
 #pragma unroll
-    for (int i = 0; i < (HOUT ? 4 : 8); ++i) {
+    for (int i = 0; i < (HOUT ? NV / 2 : NV); ++i) {
         const int k = j + i * T - P.sshift;
         if (k >= 0 && k < P.dst_nd) dst[k * dd] = v[i];
     }
@@ -452,11 +455,11 @@ void launch_pen_h(const PassDesc& P, hipStream_t s) {
     }
 }
 
-template <int H, int ROWS>
+template <int PF, int Q, int ROWS>
 void launch_row_rc_h(const PassDesc& P, hipStream_t s) {
     dim3 grid((P.n1 * P.n2 + ROWS - 1) / ROWS);
-    if (P.op == OP_R2C) hipLaunchKernelGGL((row_rc<H, ROWS, true>), grid, dim3(ROWS * H / 8), 0, s, P);
-    else hipLaunchKernelGGL((row_rc<H, ROWS, false>), grid, dim3(ROWS * H / 8), 0, s, P);
+    if (P.op == OP_R2C) hipLaunchKernelGGL((row_rc<PF, Q, ROWS, true>), grid, dim3(ROWS * Q / 8), 0, s, P);
+    else hipLaunchKernelGGL((row_rc<PF, Q, ROWS, false>), grid, dim3(ROWS * Q / 8), 0, s, P);
 }
 
 template <int M, int TX>
@@ -469,18 +472,32 @@ void launch_col_real_m(const PassDesc& P, hipStream_t s) {
     else hipLaunchKernelGGL((col_real<M, TX, false>), grid, dim3(TX * M / 8), 0, s, P);
 }
 
-template <int M, int TX>
+template <int PF, int Q, int TX>
 void launch_col_c2c_m(const PassDesc& P, hipStream_t s) {
+    constexpr int M = PF * Q;
     const int ntile = (P.na + TX - 1) / TX;
     dim3 grid(ntile * (P.n1 * P.n2 / P.na));
-    dim3 block(TX * M / 8);
+    dim3 block(TX * Q / 8);
     const bool hin = P.gshift == 0 && P.src_nd <= M / 2, hout = P.sshift == 0 && P.dst_nd <= M / 2;
     if (P.sign == FWD) {
-        if (hin) hipLaunchKernelGGL((col_c2c<M, TX, true, true, false>), grid, block, 0, s, P);
-        else hipLaunchKernelGGL((col_c2c<M, TX, true, false, false>), grid, block, 0, s, P);
+        if (hin) hipLaunchKernelGGL((col_c2c<PF, Q, TX, true, true, false>), grid, block, 0, s, P);
+        else hipLaunchKernelGGL((col_c2c<PF, Q, TX, true, false, false>), grid, block, 0, s, P);
     } else {
-        if (hout) hipLaunchKernelGGL((col_c2c<M, TX, false, false, true>), grid, block, 0, s, P);
-        else hipLaunchKernelGGL((col_c2c<M, TX, false, false, false>), grid, block, 0, s, P);
+        if (hout) hipLaunchKernelGGL((col_c2c<PF, Q, TX, false, false, true>), grid, block, 0, s, P);
+        else hipLaunchKernelGGL((col_c2c<PF, Q, TX, false, false, false>), grid, block, 0, s, P);
+    }
+}
+// mixed radix: TX Q = 1024 (PF 3) or 512 (PF 5), LDS 48 / 40 KB, two or one waves per block
+template <int PF>
+void launch_col_c2c_pf(int q, const PassDesc& P, hipStream_t s) {
+    constexpr int TQ = PF == 3 ? 1024 : 512;
+    switch (q) {
+        case 8: launch_col_c2c_m<PF, 8, TQ / 8>(P, s); break;
+        case 16: launch_col_c2c_m<PF, 16, TQ / 16>(P, s); break;
+        case 32: launch_col_c2c_m<PF, 32, TQ / 32>(P, s); break;
+        case 64: launch_col_c2c_m<PF, 64, TQ / 64>(P, s); break;
+        case 128: launch_col_c2c_m<PF, 128, TQ / 128>(P, s); break;
+        default: fail("column c2c: length %d", P.n);
     }
 }
 
@@ -536,9 +553,37 @@ std::vector<double> row_r2r_table(int kind, int n) {
     return t;
 }
 
+// n = pf q with pf in {1, 3, 5, 7} and q a power of two (false otherwise)
+bool split_pq(int n, int& pf, int& q) {
+    for (int f : {1, 3, 5, 7}) {
+        if (n % f) continue;
+        const int r = n / f;
+        if (r > 0 && (r & (r - 1)) == 0) {
+            pf = f;
+            q = r;
+            return true;
+        }
+    }
+    return false;
+}
+
+namespace {
+// lengths of the register column FFT: powers of two in [lo, hi], or (mixed) 3 or 5 times a power
+// of two in [8, 128]. A thread holds 8 PF values, so a pencil of PF Q points has only Q/8 threads:
+// for the contiguous r2c rows of small problems that leaves too few waves (96 x 128 x 64, x of
+// 96: 24 us against 14 us of the generic kernel), and PF = 7 needs 300 registers (no gain
+// either); strided c2c with PF = 3 halves the generic kernel's time.
+bool fft_len_ok(int n, int lo, int hi, bool mixed) {
+    int pf = 0, q = 0;
+    if (!split_pq(n, pf, q)) return false;
+    if (pf == 1) return q >= lo && q <= hi;
+    return mixed && pf <= 5 && q >= 8 && q <= 128;
+}
+}  // namespace
+
 bool row_rc_fits(const PassDesc& P) {
     const int n = P.n, h = n / 2;
-    if (!(P.op == OP_R2C || P.op == OP_C2R) || n % 2 || h < 16 || h > 1024 || (h & (h - 1))) return false;
+    if (!(P.op == OP_R2C || P.op == OP_C2R) || n % 2 || !fft_len_ok(h, 16, 1024, false)) return false;
     const bool fwd = P.op == OP_R2C;
     const int d = P.d;
     if (P.src_s[d] != 1 || P.dst_s[d] != 1 || P.si != 0 || P.so != 0 || (P.post & ~POST_PERIOD) || P.gshift < 0 || P.sshift < 0)
@@ -553,13 +598,13 @@ bool row_rc_fits(const PassDesc& P) {
 
 void launch_row_rc(const PassDesc& P, hipStream_t s) {
     switch (P.n / 2) {
-        case 16: launch_row_rc_h<16, 128>(P, s); break;
-        case 32: launch_row_rc_h<32, 64>(P, s); break;
-        case 64: launch_row_rc_h<64, 32>(P, s); break;
-        case 128: launch_row_rc_h<128, 16>(P, s); break;
-        case 256: launch_row_rc_h<256, 8>(P, s); break;
-        case 512: launch_row_rc_h<512, 4>(P, s); break;
-        case 1024: launch_row_rc_h<1024, 2>(P, s); break;
+        case 16: launch_row_rc_h<1, 16, 128>(P, s); break;
+        case 32: launch_row_rc_h<1, 32, 64>(P, s); break;
+        case 64: launch_row_rc_h<1, 64, 32>(P, s); break;
+        case 128: launch_row_rc_h<1, 128, 16>(P, s); break;
+        case 256: launch_row_rc_h<1, 256, 8>(P, s); break;
+        case 512: launch_row_rc_h<1, 512, 4>(P, s); break;
+        case 1024: launch_row_rc_h<1, 1024, 2>(P, s); break;
         default: fail("row rc: length %d", P.n);
     }
     HIP_CHECK(hipGetLastError());
@@ -598,7 +643,7 @@ void launch_col_real(const PassDesc& P, hipStream_t s) {
 
 bool col_c2c_fits(const PassDesc& P) {
     const int n = P.n;
-    if (P.op != OP_C2C || n < 32 || n > 2048 || (n & (n - 1)) || P.M != n) return false;
+    if (P.op != OP_C2C || !fft_len_ok(n, 32, 2048, true) || P.M != n) return false;
     const int d = P.d, fa = P.fa;
     if (fa == d || P.src_s[fa] != 1 || P.dst_s[fa] != 1 || (P.n1 * P.n2) % P.na != 0) return false;
     if (!(P.nf_in == 2 && P.nf_out == 2 && P.src_nf == 2 && P.dst_nf == 2 && P.pen_in == n && P.pen_out == n)) return false;
@@ -607,14 +652,22 @@ bool col_c2c_fits(const PassDesc& P) {
 
 // LDS M TX complex values: 64 KB at most
 void launch_col_c2c(const PassDesc& P, hipStream_t s) {
+    int pf = 1, q = 0;
+    split_pq(P.n, pf, q);
+    if (pf != 1) {
+        if (pf == 3) launch_col_c2c_pf<3>(q, P, s);
+        else launch_col_c2c_pf<5>(q, P, s);
+        HIP_CHECK(hipGetLastError());
+        return;
+    }
     switch (P.n) {
-        case 32: launch_col_c2c_m<32, 16>(P, s); break;
-        case 64: launch_col_c2c_m<64, 16>(P, s); break;
-        case 128: launch_col_c2c_m<128, 8>(P, s); break;
-        case 256: launch_col_c2c_m<256, 8>(P, s); break;
-        case 512: launch_col_c2c_m<512, 8>(P, s); break;
-        case 1024: launch_col_c2c_m<1024, 4>(P, s); break;
-        case 2048: launch_col_c2c_m<2048, 2>(P, s); break;
+        case 32: launch_col_c2c_m<1, 32, 16>(P, s); break;
+        case 64: launch_col_c2c_m<1, 64, 16>(P, s); break;
+        case 128: launch_col_c2c_m<1, 128, 8>(P, s); break;
+        case 256: launch_col_c2c_m<1, 256, 8>(P, s); break;
+        case 512: launch_col_c2c_m<1, 512, 8>(P, s); break;
+        case 1024: launch_col_c2c_m<1, 1024, 4>(P, s); break;
+        case 2048: launch_col_c2c_m<1, 2048, 2>(P, s); break;
         default: fail("column c2c: length %d", P.n);
     }
     HIP_CHECK(hipGetLastError());

@@ -31,6 +31,7 @@ std::vector<double> row_r2r_table(int kind, int n);
 void launch_row_r2r(const PassDesc& P, hipStream_t s);
 bool col_r2r_fits(const PassDesc& P);
 bool col_c2c_fits(const PassDesc& P);
+bool split_pq(int n, int& pf, int& q);
 void launch_col_c2c(const PassDesc& P, hipStream_t s);
 void launch_col_r2r(const PassDesc& P, hipStream_t s);
 FastPlan* make_fast_plan(SolverImpl& s);   // nullptr if the fused pipeline does not apply
@@ -345,14 +346,21 @@ PassDesc SolverImpl::pass_desc(const DirPlan& p, int comp, bool forward, const V
         return P;
     }
     // strided complex pencils: tiles of neighbouring pencils through the register FFT
+    // (lengths 3, 5, 7 times a power of two: Q-point column FFTs after the small DFTs)
     if (!std::getenv("FLUPS_GENERIC_PASS") && col_c2c_fits(P)) {
         P.fastk = 5;
+        int pf = 1, q = P.n;
+        split_pq(P.n, pf, q);
+        P.tw = twiddles(q);
+        P.aux = twiddles(P.n);
         return P;
     }
     // real-to-complex pencils along the contiguous direction: packed half-length FFT in registers
     if (!std::getenv("FLUPS_GENERIC_PASS") && row_rc_fits(P)) {
         P.fastk = 3;
-        P.tw = twiddles(P.n / 2);
+        int pf = 1, q = P.n / 2;
+        split_pq(P.n / 2, pf, q);
+        P.tw = twiddles(q);
         P.aux = twiddles(P.n);
         return P;
     }

@@ -591,6 +591,11 @@ ROW_CASES = [
     ((48, 32, 20), [[3, 3], [3, 3], [3, 3]], 1, 0, 5),
     ((256, 512, 1), [[4, 4], [4, 4], [9, 9]], 1, 2, 5),
     ((128, 1024, 1), [[3, 3], [4, 4], [9, 9]], 1, 4, 5),
+    # strided c2c of 3 or 5 times a power of two (small DFTs, then power-of-two column FFTs)
+    ((32, 48, 40), [[3, 3], [4, 4], [3, 3]], 1, 2, 5),
+    ((96, 56, 24), [[4, 4], [3, 3], [4, 4]], 1, 8, 5),
+    ((96, 80, 1), [[4, 4], [4, 4], [9, 9]], 1, 2, 5),
+    ((16, 80, 160), [[3, 3], [3, 3], [3, 3]], 1, 0, 5),
 ]
 
 
@@ -598,7 +603,8 @@ ROW_CASES = [
                                                  "perz-128", "pery-256", "even-unb", "unb-odd", "odd-unb-ee", "ee-unby-perz",
                                                  "oe-unby-unbz", "unbx-eey", "perx-ooy", "node-per", "eoz-128", "ragged-ooy",
                                                  "eunb-y", "unbo-y-oez", "ooz-512", "eey-2048", "c2c-y-256", "c2c-y-32",
-                                                 "c2c-2d-1024", "c2c-2d-2048"])
+                                                 "c2c-2d-1024", "c2c-2d-2048", "c2c-5x8-3x32", "c2c-3x16", "2d-5x32",
+                                                 "c2c-5x16-5x32"])
 @pytest.mark.parametrize("misalign", [False, True])
 def test_fast_pass_kernels_vs_generic_and_oracle(case, misalign, monkeypatch):
     """the fast kernels of the general path (real-to-real pencils and tiles, paired real columns,
