@@ -212,6 +212,37 @@ __device__ __forceinline__ void column_fft_pq(C2<R>* v, C2<R>* lds, int j, const
     }
 }
 
+// Column FFT of length H = PF Q by PF teams of T = Q/8 threads (team k1, thread j of the
+// team), 8 values per thread: team k1 forms its inputs y[n2] = w_H^(n2 k1) sum_n1 z[Q n1 + n2]
+// w_PF^(n1 k1) (n2 = j + i T, z read through load(m)), then runs the Q-point column FFT on its
+// own LDS region. On exit v[i] = Z[k1 + PF (j + i T)]. All of load()'s reads happen before the
+// column FFT's first LDS write, so the input may sit in the same LDS. ES = 0: regions of
+// Q + Q/8 slots; ES > 0: Q ES slots (interleaved columns). PF = 1 is the plain column FFT.
+template <typename R, int PF, int Q, bool FWD, int ES, typename Load>
+__device__ __forceinline__ void fft_pq_team(C2<R>* v, C2<R>* lds, int k1, int j, Load&& load, const C2<R>* twq,
+                                            const C2<R>* twh, int hs) {
+    constexpr int T = Q / 8;
+    if constexpr (PF == 1) {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) v[i] = load(j + i * T);
+        column_fft<R, Q, FWD, ES>(v, lds, j, twq);
+    } else {
+        C2<R> wk[PF];   // w_PF^(n1 k1) = w_H^(Q ((n1 k1) mod PF))
+        wk[0] = {1, 0};
+#pragma unroll
+        for (int n1 = 1; n1 < PF; ++n1) wk[n1] = twl<R, FWD>(twh, ((n1 * k1) % PF) * Q * hs);
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const int n2 = j + i * T;
+            C2<R> acc = load(n2);
+#pragma unroll
+            for (int n1 = 1; n1 < PF; ++n1) acc = cadd(acc, cmul(load(Q * n1 + n2), wk[n1]));
+            v[i] = k1 > 0 ? cmul(acc, twl<R, FWD>(twh, n2 * k1 * hs)) : acc;   // n2 k1 < H
+        }
+        column_fft<R, Q, FWD, ES>(v, lds + k1 * (ES > 0 ? Q * ES : Q + Q / 8), j, twq);
+    }
+}
+
 // Workgroups are dispatched round-robin over the 8 XCDs (each with its own L2):
 // hardware block b runs on XCD b % 8. Renumber so that consecutive logical
 // blocks run on the same XCD at about the same time (they share cache lines).

@@ -217,12 +217,14 @@ __global__ void __launch_bounds__(R * H / 8) pen_r2r(PassDesc P) {
     }
 }
 
-// H = PF Q: PF = 1 (a power of two; the only one launched) or 3, 5, 7 (mixed radix, column_fft_pq)
+// H = PF Q: PF = 1 (a power of two) or 3, 5, 7 (PF teams of Q/8 threads, fft_pq_team);
+// H/8 threads per row either way
 template <int PF, int Q, int ROWS, bool FWD>
-__global__ void __launch_bounds__(ROWS * Q / 8) row_rc(PassDesc P) {
-    constexpr int H = PF * Q, T = Q / 8, HP = H + H / 8, NV = 8 * PF;
+__global__ void __launch_bounds__(ROWS * PF * Q / 8) row_rc(PassDesc P) {
+    constexpr int H = PF * Q, T = Q / 8, TT = H / 8, HP = H + H / 8;
     __shared__ C2<double> lds[HP * ROWS];
-    const int rl = threadIdx.x / T, j = threadIdx.x % T;
+    const int rl = threadIdx.x / TT, tt = threadIdx.x % TT;
+    const int k1 = tt / T, j = tt % T;   // team and thread in the team
     const int g = blockIdx.x * ROWS + rl;
     const bool live = g < P.n1 * P.n2;
     const int gg = live ? g : 0;   // dead rows read pencil 0 and store nothing
@@ -235,70 +237,66 @@ __global__ void __launch_bounds__(ROWS * Q / 8) row_rc(PassDesc P) {
     const C2<double>* twn = reinterpret_cast<const C2<double>*>(P.aux);  // n-point table
     // 1. gather: the pencil's doubles (reals for r2c, interleaved complex for c2r)
     const int nfi = P.nf_in, len = P.pen_in * nfi;
-    for (int q = j; q < len; q += T) {
+    for (int q = tt; q < len; q += TT) {
         const int k = q / nfi, ks = k - P.gshift;
         rd[q] = (ks >= 0 && ks < P.src_nd) ? src[(long long)ks * nfi + (q - k * nfi)] : 0.0;
     }
     __syncthreads();
-    C2<double> v[NV];
-#pragma unroll
-    for (int i = 0; i < NV; ++i) {
-        const int m = j + i * T;
-        if (FWD) {
-            v[i] = {rd[2 * m], rd[2 * m + 1]};
-        } else {
-            C2<double> a = {rd[2 * m], rd[2 * m + 1]}, b = {rd[2 * (H - m)], rd[2 * (H - m) + 1]};
-            if (m == 0) {   // DC and Nyquist are real
-                a.y = 0.0;
-                b.y = 0.0;
-            }
-            const C2<double> bc = cconj(b);
-            const C2<double> e = cadd(a, bc), f = csub(a, bc);
-            const C2<double> t = cmul(cconj(twn[m]), f);   // e^{+2 pi i m/n} (a - conj b)
-            v[i] = {e.x - t.y, e.y + t.x};
+    // element m of the packed complex sequence: z[m] = x[2m] + i x[2m+1] (r2c), or built from
+    // X[m] and X[H-m] (c2r)
+    auto load = [&](int m) -> C2<double> {
+        if (FWD) return {rd[2 * m], rd[2 * m + 1]};
+        C2<double> a = {rd[2 * m], rd[2 * m + 1]}, b = {rd[2 * (H - m)], rd[2 * (H - m) + 1]};
+        if (m == 0) {   // DC and Nyquist are real
+            a.y = 0.0;
+            b.y = 0.0;
         }
-    }
-    column_fft_pq<double, PF, Q, FWD, 0>(v, row, j, tw, twn, 2);   // w_H^e = twn[2 e]
+        const C2<double> bc = cconj(b);
+        const C2<double> e = cadd(a, bc), f = csub(a, bc);
+        const C2<double> t = cmul(cconj(twn[m]), f);   // e^{+2 pi i m/n} (a - conj b)
+        return {e.x - t.y, e.y + t.x};
+    };
+    C2<double> v[8];
+    fft_pq_team<double, PF, Q, FWD, 0>(v, row, k1, j, load, tw, twn, 2);   // w_H^e = twn[2 e]
     __syncthreads();
     const int nfo = P.nf_out, plen = P.pen_out;
-    if (FWD) {
-        // X[k], k <= H, into the row as complex values (v holds X after the exchange)
+    // v[i] = Z[k1 + PF (j + i T)]
 #pragma unroll
-        for (int i = 0; i < NV; ++i) row[j + i * T] = v[i];
+    for (int i = 0; i < 8; ++i) row[k1 + PF * (j + i * T)] = v[i];
+    if (FWD) {
+        // X[k], k <= H, into the row as complex values
         __syncthreads();
         C2<double> XH = {0, 0};
 #pragma unroll
-        for (int i = 0; i < NV; ++i) {
-            const int k = j + i * T;
+        for (int i = 0; i < 8; ++i) {
+            const int k = tt + i * TT;
             const C2<double> zk = row[k], zc = cconj(row[k == 0 ? 0 : H - k]);
             const C2<double> e = {0.5 * (zk.x + zc.x), 0.5 * (zk.y + zc.y)};
             const C2<double> df = {0.5 * (zk.x - zc.x), 0.5 * (zk.y - zc.y)};
             v[i] = cadd(e, cmul(twn[k], C2<double>{df.y, -df.x}));
         }
-        if (j == 0) {
+        if (tt == 0) {
             const C2<double> z0 = row[0];
             XH = {z0.x - z0.y, 0.0};
         }
         __syncthreads();
 #pragma unroll
-        for (int i = 0; i < NV; ++i) row[j + i * T] = v[i];
-        if (j == 0) row[H] = XH;
+        for (int i = 0; i < 8; ++i) row[tt + i * TT] = v[i];
+        if (tt == 0) row[H] = XH;
         // positions past H + 1 complex values keep nothing (zero)
-        for (int q = H + 1 + j; q < plen; q += T) row[q] = {0.0, 0.0};
+        for (int q = H + 1 + tt; q < plen; q += TT) row[q] = {0.0, 0.0};
         // periodic node-centred direction: the first value again at the end (n_out - 1)
-        if ((P.post & POST_PERIOD) && j == 0) row[P.n_out_plan - 1] = v[0];
+        if ((P.post & POST_PERIOD) && tt == 0) row[P.n_out_plan - 1] = v[0];
     } else {
-        // x[2m] = Re z[m], x[2m+1] = Im z[m]
-#pragma unroll
-        for (int i = 0; i < NV; ++i) row[j + i * T] = v[i];
-        for (int q = 2 * H + j; q < plen; q += T) rd[q] = 0.0;
-        if ((P.post & POST_PERIOD) && j == 0) rd[2 * H] = v[0].x;   // x[n] = x[0]
+        // x[2m] = Re z[m], x[2m+1] = Im z[m] (written above)
+        for (int q = 2 * H + tt; q < plen; q += TT) rd[q] = 0.0;
+        if ((P.post & POST_PERIOD) && tt == 0) rd[2 * H] = v[0].x;   // x[n] = x[0]
     }
     __syncthreads();
     if (!live) return;
     // 2. scatter (truncation), element k of the destination = pencil position k + sshift
     const int dn = P.dst_nd;
-    for (int q = j; q < dn * nfo; q += T) {
+    for (int q = tt; q < dn * nfo; q += TT) {
         const int k = q / nfo, p = k + P.sshift;
         if (p < plen) dst[(long long)k * nfo + (q - k * nfo)] = rd[p * nfo + (q - k * nfo)];
     }
@@ -458,8 +456,21 @@ void launch_pen_h(const PassDesc& P, hipStream_t s) {
 template <int PF, int Q, int ROWS>
 void launch_row_rc_h(const PassDesc& P, hipStream_t s) {
     dim3 grid((P.n1 * P.n2 + ROWS - 1) / ROWS);
-    if (P.op == OP_R2C) hipLaunchKernelGGL((row_rc<PF, Q, ROWS, true>), grid, dim3(ROWS * Q / 8), 0, s, P);
-    else hipLaunchKernelGGL((row_rc<PF, Q, ROWS, false>), grid, dim3(ROWS * Q / 8), 0, s, P);
+    dim3 block(ROWS * PF * Q / 8);
+    if (P.op == OP_R2C) hipLaunchKernelGGL((row_rc<PF, Q, ROWS, true>), grid, block, 0, s, P);
+    else hipLaunchKernelGGL((row_rc<PF, Q, ROWS, false>), grid, block, 0, s, P);
+}
+// H = PF Q, 3 <= PF <= 7: about 256 threads (2048 / H rows) and 36 KB of LDS per block
+template <int PF>
+void launch_row_rc_pf(int q, const PassDesc& P, hipStream_t s) {
+    switch (q) {
+        case 8: launch_row_rc_h<PF, 8, 2048 / (PF * 8)>(P, s); break;
+        case 16: launch_row_rc_h<PF, 16, 2048 / (PF * 16)>(P, s); break;
+        case 32: launch_row_rc_h<PF, 32, 2048 / (PF * 32)>(P, s); break;
+        case 64: launch_row_rc_h<PF, 64, 2048 / (PF * 64)>(P, s); break;
+        case 128: launch_row_rc_h<PF, 128, 2048 / (PF * 128)>(P, s); break;
+        default: fail("row rc: length %d", P.n);
+    }
 }
 
 template <int M, int TX>
@@ -568,22 +579,23 @@ bool split_pq(int n, int& pf, int& q) {
 }
 
 namespace {
-// lengths of the register column FFT: powers of two in [lo, hi], or (mixed) 3 or 5 times a power
-// of two in [8, 128]. A thread holds 8 PF values, so a pencil of PF Q points has only Q/8 threads:
-// for the contiguous r2c rows of small problems that leaves too few waves (96 x 128 x 64, x of
-// 96: 24 us against 14 us of the generic kernel), and PF = 7 needs 300 registers (no gain
-// either); strided c2c with PF = 3 halves the generic kernel's time.
-bool fft_len_ok(int n, int lo, int hi, bool mixed) {
+// lengths of the register column FFT: powers of two in [lo, hi], or PF times a power of two in
+// [8, 128] with 3 <= PF <= maxpf. The strided c2c kernel holds 8 PF values per thread (Q/8
+// threads per pencil): PF = 7 needs 300 registers there and gains nothing over the generic
+// kernel, PF = 3 halves its time. The row r2c / c2r kernel splits a pencil over PF teams
+// (H/8 threads per pencil, 8 values each); holding all 8 PF values in one thread instead left
+// too few waves (x of 96 in 96 x 128 x 64: 24 us against 14 us of the generic kernel).
+bool fft_len_ok(int n, int lo, int hi, int maxpf) {
     int pf = 0, q = 0;
     if (!split_pq(n, pf, q)) return false;
     if (pf == 1) return q >= lo && q <= hi;
-    return mixed && pf <= 5 && q >= 8 && q <= 128;
+    return pf <= maxpf && q >= 8 && q <= 128;
 }
 }  // namespace
 
 bool row_rc_fits(const PassDesc& P) {
     const int n = P.n, h = n / 2;
-    if (!(P.op == OP_R2C || P.op == OP_C2R) || n % 2 || !fft_len_ok(h, 16, 1024, false)) return false;
+    if (!(P.op == OP_R2C || P.op == OP_C2R) || n % 2 || !fft_len_ok(h, 16, 1024, 7)) return false;
     const bool fwd = P.op == OP_R2C;
     const int d = P.d;
     if (P.src_s[d] != 1 || P.dst_s[d] != 1 || P.si != 0 || P.so != 0 || (P.post & ~POST_PERIOD) || P.gshift < 0 || P.sshift < 0)
@@ -597,6 +609,15 @@ bool row_rc_fits(const PassDesc& P) {
 }
 
 void launch_row_rc(const PassDesc& P, hipStream_t s) {
+    int pf = 1, q = 0;
+    split_pq(P.n / 2, pf, q);
+    if (pf != 1) {
+        if (pf == 3) launch_row_rc_pf<3>(q, P, s);
+        else if (pf == 5) launch_row_rc_pf<5>(q, P, s);
+        else launch_row_rc_pf<7>(q, P, s);
+        HIP_CHECK(hipGetLastError());
+        return;
+    }
     switch (P.n / 2) {
         case 16: launch_row_rc_h<1, 16, 128>(P, s); break;
         case 32: launch_row_rc_h<1, 32, 64>(P, s); break;
@@ -643,7 +664,7 @@ void launch_col_real(const PassDesc& P, hipStream_t s) {
 
 bool col_c2c_fits(const PassDesc& P) {
     const int n = P.n;
-    if (P.op != OP_C2C || !fft_len_ok(n, 32, 2048, true) || P.M != n) return false;
+    if (P.op != OP_C2C || !fft_len_ok(n, 32, 2048, 5) || P.M != n) return false;
     const int d = P.d, fa = P.fa;
     if (fa == d || P.src_s[fa] != 1 || P.dst_s[fa] != 1 || (P.n1 * P.n2) % P.na != 0) return false;
     if (!(P.nf_in == 2 && P.nf_out == 2 && P.src_nf == 2 && P.dst_nf == 2 && P.pen_in == n && P.pen_out == n)) return false;

@@ -596,6 +596,11 @@ ROW_CASES = [
     ((96, 56, 24), [[4, 4], [3, 3], [4, 4]], 1, 8, 5),
     ((96, 80, 1), [[4, 4], [4, 4], [9, 9]], 1, 2, 5),
     ((16, 80, 160), [[3, 3], [3, 3], [3, 3]], 1, 0, 5),
+    # contiguous r2c / c2r of 3, 5, 7 times a power of two (teams of threads, one per factor)
+    ((48, 32, 32), [[3, 3], [0, 0], [4, 4]], 1, 2, 3),
+    ((80, 32, 16), [[4, 4], [1, 1], [4, 4]], 1, 0, 3),
+    ((112, 16, 16), [[4, 4], [0, 0], [0, 0]], 1, 2, 3),
+    ((49, 17, 17), [[3, 3], [3, 3], [3, 3]], 0, 0, 3),
 ]
 
 
@@ -604,7 +609,7 @@ ROW_CASES = [
                                                  "oe-unby-unbz", "unbx-eey", "perx-ooy", "node-per", "eoz-128", "ragged-ooy",
                                                  "eunb-y", "unbo-y-oez", "ooz-512", "eey-2048", "c2c-y-256", "c2c-y-32",
                                                  "c2c-2d-1024", "c2c-2d-2048", "c2c-5x8-3x32", "c2c-3x16", "2d-5x32",
-                                                 "c2c-5x16-5x32"])
+                                                 "c2c-5x16-5x32", "rc-3x8", "rc-5x16", "rc-7x16", "rc-node-3x8"])
 @pytest.mark.parametrize("misalign", [False, True])
 def test_fast_pass_kernels_vs_generic_and_oracle(case, misalign, monkeypatch):
     """the fast kernels of the general path (real-to-real pencils and tiles, paired real columns,
