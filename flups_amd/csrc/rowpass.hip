@@ -58,19 +58,21 @@ namespace {
 // COL = false: R contiguous pencils per block (H/8 threads each, one padded LDS row each);
 // COL = true: R strided pencils, neighbours along the unit-stride dimension fa, interleaved
 // in LDS (element k of pencil c at slot k * R + c) and read / written as tile rows.
-template <int H, int R, int KIND, bool COL>
-__global__ void __launch_bounds__(R * H / 8) pen_r2r(PassDesc P) {
-    constexpr int T = H / 8, N = 2 * H, HP = H + H / 8;
+// H = PF Q: PF = 1 (a power of two) or 3, 5, 7 (PF teams of Q/8 threads, fft_pq_team);
+// H/8 threads per pencil either way
+template <int PF, int Q, int R, int KIND, bool COL>
+__global__ void __launch_bounds__(R * PF * Q / 8) pen_r2r(PassDesc P) {
+    constexpr int H = PF * Q, T = Q / 8, TT = H / 8, N = 2 * H, HP = H + H / 8;
     constexpr int S = COL ? R : 1;   // LDS stride between consecutive elements of a pencil
     __shared__ C2<double> lds[COL ? (H + 1) * R : HP * R];
     constexpr int DCT = KIND >= RODFT10 ? KIND - 4 : KIND;
     constexpr bool ALT = KIND == RODFT10;                       // input (-1)^n x, output reversed
     constexpr bool REV = KIND == RODFT01 || KIND == RODFT11;    // input reversed, output (-1)^k
-    int rl, j, ja, jb;
+    int rl, tt, ja, jb;
     bool live;
     if (COL) {
         rl = threadIdx.x % R;
-        j = threadIdx.x / R;
+        tt = threadIdx.x / R;
         const int ntile = (P.na + R - 1) / R;
         const int tile = blockIdx.x % ntile;
         jb = blockIdx.x / ntile;
@@ -78,14 +80,15 @@ __global__ void __launch_bounds__(R * H / 8) pen_r2r(PassDesc P) {
         live = ja < P.na;
         if (!live) ja = P.na - 1;   // dead pencils read the last one and store nothing
     } else {
-        rl = threadIdx.x / T;
-        j = threadIdx.x % T;
+        rl = threadIdx.x / TT;
+        tt = threadIdx.x % TT;
         const int g = blockIdx.x * R + rl;
         live = g < P.n1 * P.n2;
         const int gg = live ? g : 0;   // dead rows read pencil 0 and store nothing
         ja = gg % P.na;
         jb = gg / P.na;
     }
+    const int k1 = tt / T, j = tt % T;   // team and thread in the team
     const double* src = P.src + (long long)ja * P.src_s[P.fa] + (long long)jb * P.src_s[P.fb];
     double* dst = P.dst + (long long)ja * P.dst_s[P.fa] + (long long)jb * P.dst_s[P.fb];
     const long long sd = P.src_s[P.d], dd = P.dst_s[P.d];
@@ -95,20 +98,21 @@ __global__ void __launch_bounds__(R * H / 8) pen_r2r(PassDesc P) {
     auto RW = [&](int k) -> C2<double>& { return row[k * S]; };
     const C2<double>* A = reinterpret_cast<const C2<double>*>(P.aux);
     const C2<double>* B = A + (H + 1);
-    const C2<double>* tw = reinterpret_cast<const C2<double>*>(P.tw);
+    const C2<double>* tw = reinterpret_cast<const C2<double>*>(P.tw);   // Q-point table
+    const C2<double>* twh = B + (H + 1);                                 // H-point table (PF > 1)
     const int pin = P.pen_in;   // pencil buffer length (the same after the transform), at most N + 2
 
     // 1. gather the pencil into LDS (field shift, zero padding)
     if (!COL && P.vec) {
 #pragma unroll
         for (int t = 0; t < 8; ++t) {
-            const int q = j + t * T;
+            const int q = tt + t * TT;
             const double2 p = reinterpret_cast<const double2*>(src)[q];
             RD(2 * q) = p.x;
             RD(2 * q + 1) = p.y;
         }
     } else {
-        for (int k = j; k < pin; k += T) {
+        for (int k = tt; k < pin; k += TT) {
             const int ks = k - P.gshift;
             RD(k) = (ks >= 0 && ks < P.src_nd) ? src[(long long)ks * sd] : 0.0;
         }
@@ -120,16 +124,13 @@ __global__ void __launch_bounds__(R * H / 8) pen_r2r(PassDesc P) {
         return (ALT && (q & 1)) ? -v : v;
     };
 
-    // 2. pre-map to the complex sequence of length H
-    C2<double> v[8];
-#pragma unroll
-    for (int i = 0; i < 8; ++i) {
-        const int m = j + i * T;
+    // 2. pre-map to the complex sequence of length H: element m
+    auto load = [&](int m) -> C2<double> {
         if (DCT == REDFT11) {
-            v[i] = cmul(C2<double>{xin(2 * m), xin(N - 1 - 2 * m)}, A[m]);
+            return cmul(C2<double>{xin(2 * m), xin(N - 1 - 2 * m)}, A[m]);
         } else if (DCT == REDFT10) {
             auto u = [&](int q) { return q < H ? xin(2 * q) : xin(2 * (N - 1 - q) + 1); };
-            v[i] = {u(2 * m), u(2 * m + 1)};
+            return {u(2 * m), u(2 * m + 1)};
         } else {   // REDFT01
             auto U = [&](int k) {
                 const double a = k < N ? xin(k) : 0.0, b = (N - k) < N ? xin(N - k) : 0.0;
@@ -138,53 +139,54 @@ __global__ void __launch_bounds__(R * H / 8) pen_r2r(PassDesc P) {
             const C2<double> uk = U(m), uc = cconj(U(H - m));
             const C2<double> e = cadd(uk, uc), f = csub(uk, uc);
             const C2<double> t = cmul(B[m], f);
-            v[i] = {e.x - t.y, e.y + t.x};   // e + i t
+            return {e.x - t.y, e.y + t.x};   // e + i t
         }
-    }
+    };
 
-    // 3. FFT of length H (forward for DCT-II / IV, inverse for DCT-III)
-    column_fft<double, H, DCT != REDFT01, COL ? R : 0>(v, row, j, tw);
+    // 3. FFT of length H (forward for DCT-II / IV, inverse for DCT-III): v[i] = Z[k1 + PF (j + i T)]
+    C2<double> v[8];
+    fft_pq_team<double, PF, Q, DCT != REDFT01, COL ? R : 0>(v, row, k1, j, load, tw, twh, 1);
     __syncthreads();   // every thread is done with the FFT's LDS before the buffer is reused
 
     // 4. post-map into the LDS pencil
     if (DCT == REDFT11) {
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
-            const int k = j + i * T;
+            const int k = k1 + PF * (j + i * T);
             const C2<double> w = cmul(v[i], B[k]);
             RD(2 * k) = 2.0 * w.x;
             RD(N - 1 - 2 * k) = -2.0 * w.y;
         }
     } else if (DCT == REDFT10) {
 #pragma unroll
-        for (int i = 0; i < 8; ++i) RW(j + i * T) = v[i];
+        for (int i = 0; i < 8; ++i) RW(k1 + PF * (j + i * T)) = v[i];
         __syncthreads();
         C2<double> U[8], UH = {0, 0};
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
-            const int k = j + i * T;
-            const C2<double> zk = RW(k), zc = cconj(RW((H - k) & (H - 1)));
+            const int k = tt + i * TT;
+            const C2<double> zk = RW(k), zc = cconj(RW(k == 0 ? 0 : H - k));
             const C2<double> e = {0.5 * (zk.x + zc.x), 0.5 * (zk.y + zc.y)};
             const C2<double> df = {0.5 * (zk.x - zc.x), 0.5 * (zk.y - zc.y)};
             U[i] = cadd(e, cmul(A[k], C2<double>{df.y, -df.x}));   // e + A[k] (zk - zc) / (2i)
         }
-        if (j == 0) {   // U[H] = Re Z[0] - Im Z[0]
+        if (tt == 0) {   // U[H] = Re Z[0] - Im Z[0]
             const C2<double> z0 = RW(0);
             UH = {z0.x - z0.y, 0.0};
         }
         __syncthreads();
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
-            const int k = j + i * T;
+            const int k = tt + i * TT;
             const C2<double> c = cmul(B[k], U[i]);
             RD(k) = 2.0 * c.x;
             if (k > 0) RD(N - k) = -2.0 * c.y;
         }
-        if (j == 0) RD(H) = 2.0 * cmul(B[H], UH).x;
+        if (tt == 0) RD(H) = 2.0 * cmul(B[H], UH).x;
     } else {   // REDFT01: v[i] = z[m], u[2m] = Re, u[2m+1] = Im
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
-            const int m = j + i * T;
+            const int m = k1 + PF * (j + i * T);
             const int q0 = 2 * m, q1 = 2 * m + 1;
             RD(q0 < H ? 2 * q0 : 2 * (N - 1 - q0) + 1) = v[i].x;
             RD(q1 < H ? 2 * q1 : 2 * (N - 1 - q1) + 1) = v[i].y;
@@ -206,11 +208,11 @@ __global__ void __launch_bounds__(R * H / 8) pen_r2r(PassDesc P) {
     if (!COL && P.vec) {
 #pragma unroll
         for (int t = 0; t < 8; ++t) {
-            const int q = j + t * T;
+            const int q = tt + t * TT;
             reinterpret_cast<double2*>(dst)[q] = double2{val(2 * q), val(2 * q + 1)};
         }
     } else {
-        for (int k = j; k < P.dst_nd; k += T) {
+        for (int k = tt; k < P.dst_nd; k += TT) {
             const int p = k + P.sshift;
             if (p < pin) dst[(long long)k * dd] = val(p);
         }
@@ -436,20 +438,34 @@ __global__ void __launch_bounds__(TX * Q / 8) col_c2c(PassDesc P) {
     }
 }
 
-template <int H, int R, bool COL>
+template <int PF, int Q, int R, bool COL>
 void launch_pen_h(const PassDesc& P, hipStream_t s) {
     const int npen = P.n1 * P.n2;
     const int nb = COL ? ((P.na + R - 1) / R) * (npen / P.na) : (npen + R - 1) / R;
     dim3 grid(nb);
-    dim3 block(R * H / 8);
+    dim3 block(R * PF * Q / 8);
     switch (P.kind) {
-        case REDFT10: hipLaunchKernelGGL((pen_r2r<H, R, REDFT10, COL>), grid, block, 0, s, P); break;
-        case REDFT01: hipLaunchKernelGGL((pen_r2r<H, R, REDFT01, COL>), grid, block, 0, s, P); break;
-        case REDFT11: hipLaunchKernelGGL((pen_r2r<H, R, REDFT11, COL>), grid, block, 0, s, P); break;
-        case RODFT10: hipLaunchKernelGGL((pen_r2r<H, R, RODFT10, COL>), grid, block, 0, s, P); break;
-        case RODFT01: hipLaunchKernelGGL((pen_r2r<H, R, RODFT01, COL>), grid, block, 0, s, P); break;
-        case RODFT11: hipLaunchKernelGGL((pen_r2r<H, R, RODFT11, COL>), grid, block, 0, s, P); break;
+        case REDFT10: hipLaunchKernelGGL((pen_r2r<PF, Q, R, REDFT10, COL>), grid, block, 0, s, P); break;
+        case REDFT01: hipLaunchKernelGGL((pen_r2r<PF, Q, R, REDFT01, COL>), grid, block, 0, s, P); break;
+        case REDFT11: hipLaunchKernelGGL((pen_r2r<PF, Q, R, REDFT11, COL>), grid, block, 0, s, P); break;
+        case RODFT10: hipLaunchKernelGGL((pen_r2r<PF, Q, R, RODFT10, COL>), grid, block, 0, s, P); break;
+        case RODFT01: hipLaunchKernelGGL((pen_r2r<PF, Q, R, RODFT01, COL>), grid, block, 0, s, P); break;
+        case RODFT11: hipLaunchKernelGGL((pen_r2r<PF, Q, R, RODFT11, COL>), grid, block, 0, s, P); break;
         default: fail("r2r pencil kernel: kind %d", P.kind);
+    }
+}
+// H = PF Q (3 <= PF <= 7): rows, 2048 / H per block (about 256 threads, 36 KB of LDS);
+// columns, 32 pencils per block up to H = 64, then 16, 8, 4 (LDS (H + 1) R complex <= 64 KB)
+template <int PF, bool COL>
+void launch_pen_pf(int q, const PassDesc& P, hipStream_t s) {
+    constexpr int H8 = PF * 8, H16 = PF * 16, H32 = PF * 32, H64 = PF * 64, H128 = PF * 128;
+    switch (q) {
+        case 8: launch_pen_h<PF, 8, COL ? (H8 <= 64 ? 32 : 16) : 2048 / H8, COL>(P, s); break;
+        case 16: launch_pen_h<PF, 16, COL ? (H16 <= 64 ? 32 : 16) : 2048 / H16, COL>(P, s); break;
+        case 32: launch_pen_h<PF, 32, COL ? (H32 <= 64 ? 32 : H32 <= 256 ? 16 : 8) : 2048 / H32, COL>(P, s); break;
+        case 64: launch_pen_h<PF, 64, COL ? (H64 <= 256 ? 16 : H64 <= 512 ? 8 : 4) : 2048 / H64, COL>(P, s); break;
+        case 128: launch_pen_h<PF, 128, COL ? (H128 <= 512 ? 8 : 4) : 2048 / H128, COL>(P, s); break;
+        default: fail("r2r pencil kernel: length %d", P.n);
     }
 }
 
@@ -514,56 +530,6 @@ void launch_col_c2c_pf(int q, const PassDesc& P, hipStream_t s) {
 
 }  // namespace
 
-namespace {
-// the plan semantics the r2r pencil kernels reproduce (kinds, length, offsets, corrections)
-bool r2r_plan_fits(const PassDesc& P) {
-    const int k = P.kind;
-    const int n = P.n, h = n / 2;
-    if (P.op != OP_R2R || !(k == REDFT10 || k == REDFT01 || k == REDFT11 || k == RODFT10 || k == RODFT01 || k == RODFT11))
-        return false;
-    if (n % 2 || h < 16 || h > 1024 || (h & (h - 1))) return false;
-    if (!(P.src_nf == 1 && P.dst_nf == 1 && P.nf_in == 1 && P.nf_out == 1 && P.r2r_stride == 1 && P.pen_in == P.pen_out &&
-          P.pen_in >= n && P.pen_in <= n + 2 && P.gshift >= 0 && P.sshift >= 0))
-        return false;
-    if (P.si < 0 || P.si > 1 || P.so < 0 || P.so > 1 || P.si + n > P.pen_in || (P.post & POST_PERIOD)) return false;
-    // every output position is a transform value or a point the post-correction zeroes
-    const bool head = P.so == 0 || (P.post & POST_FIRST);
-    const bool tail = P.so + n == P.pen_out || (P.so + n == P.pen_out - 1 && (P.post & POST_LAST));
-    return head && tail;
-}
-}  // namespace
-
-bool row_r2r_fits(const PassDesc& P) { return r2r_plan_fits(P) && P.src_s[P.d] == 1 && P.dst_s[P.d] == 1; }
-
-bool col_r2r_fits(const PassDesc& P) {
-    return r2r_plan_fits(P) && P.fa != P.d && P.src_s[P.fa] == 1 && P.dst_s[P.fa] == 1 && (P.n1 * P.n2) % P.na == 0;
-}
-
-// pre / post twiddles of the row r2r kernels, (H + 1) complex each, exact to rounding
-std::vector<double> row_r2r_table(int kind, int n) {
-    const int dct = kind >= RODFT10 ? kind - 4 : kind;
-    const int h = n / 2;
-    const long double pi = 3.14159265358979323846264338327950288L, N = n;
-    std::vector<double> t(4 * (size_t)(h + 1));
-    auto put = [&](size_t i, long double angle) {
-        t[2 * i] = (double)cosl(angle);
-        t[2 * i + 1] = (double)sinl(angle);
-    };
-    for (int k = 0; k <= h; ++k) {
-        if (dct == REDFT11) {
-            put(k, -pi * (k + 0.25L) / N);
-            put(h + 1 + k, -pi * k / N);
-        } else if (dct == REDFT10) {
-            put(k, -2.0L * pi * k / N);
-            put(h + 1 + k, -pi * k / (2.0L * N));
-        } else {
-            put(k, pi * k / (2.0L * N));
-            put(h + 1 + k, 2.0L * pi * k / N);
-        }
-    }
-    return t;
-}
-
 // n = pf q with pf in {1, 3, 5, 7} and q a power of two (false otherwise)
 bool split_pq(int n, int& pf, int& q) {
     for (int f : {1, 3, 5, 7}) {
@@ -592,6 +558,58 @@ bool fft_len_ok(int n, int lo, int hi, int maxpf) {
     return pf <= maxpf && q >= 8 && q <= 128;
 }
 }  // namespace
+
+namespace {
+// the plan semantics the r2r pencil kernels reproduce (kinds, length, offsets, corrections)
+bool r2r_plan_fits(const PassDesc& P) {
+    const int k = P.kind;
+    const int n = P.n, h = n / 2;
+    if (P.op != OP_R2R || !(k == REDFT10 || k == REDFT01 || k == REDFT11 || k == RODFT10 || k == RODFT01 || k == RODFT11))
+        return false;
+    if (n % 2 || !fft_len_ok(h, 16, 1024, 7)) return false;
+    if (!(P.src_nf == 1 && P.dst_nf == 1 && P.nf_in == 1 && P.nf_out == 1 && P.r2r_stride == 1 && P.pen_in == P.pen_out &&
+          P.pen_in >= n && P.pen_in <= n + 2 && P.gshift >= 0 && P.sshift >= 0))
+        return false;
+    if (P.si < 0 || P.si > 1 || P.so < 0 || P.so > 1 || P.si + n > P.pen_in || (P.post & POST_PERIOD)) return false;
+    // every output position is a transform value or a point the post-correction zeroes
+    const bool head = P.so == 0 || (P.post & POST_FIRST);
+    const bool tail = P.so + n == P.pen_out || (P.so + n == P.pen_out - 1 && (P.post & POST_LAST));
+    return head && tail;
+}
+}  // namespace
+
+bool row_r2r_fits(const PassDesc& P) { return r2r_plan_fits(P) && P.src_s[P.d] == 1 && P.dst_s[P.d] == 1; }
+
+bool col_r2r_fits(const PassDesc& P) {
+    return r2r_plan_fits(P) && P.fa != P.d && P.src_s[P.fa] == 1 && P.dst_s[P.fa] == 1 && (P.n1 * P.n2) % P.na == 0;
+}
+
+// pre / post twiddles of the row r2r kernels, (H + 1) complex each, exact to rounding
+std::vector<double> row_r2r_table(int kind, int n) {
+    const int dct = kind >= RODFT10 ? kind - 4 : kind;
+    const int h = n / 2;
+    const long double pi = 3.14159265358979323846264338327950288L, N = n;
+    std::vector<double> t(4 * (size_t)(h + 1) + 2 * (size_t)h);   // A, B, then exp(-2 pi i e / h)
+    auto put = [&](size_t i, long double angle) {
+        t[2 * i] = (double)cosl(angle);
+        t[2 * i + 1] = (double)sinl(angle);
+    };
+    for (int k = 0; k <= h; ++k) {
+        if (dct == REDFT11) {
+            put(k, -pi * (k + 0.25L) / N);
+            put(h + 1 + k, -pi * k / N);
+        } else if (dct == REDFT10) {
+            put(k, -2.0L * pi * k / N);
+            put(h + 1 + k, -pi * k / (2.0L * N));
+        } else {
+            put(k, pi * k / (2.0L * N));
+            put(h + 1 + k, 2.0L * pi * k / N);
+        }
+    }
+    for (int e = 0; e < h; ++e) put(2 * (size_t)(h + 1) + e, -2.0L * pi * e / h);
+    return t;
+}
+
 
 bool row_rc_fits(const PassDesc& P) {
     const int n = P.n, h = n / 2;
@@ -695,14 +713,23 @@ void launch_col_c2c(const PassDesc& P, hipStream_t s) {
 }
 
 void launch_row_r2r(const PassDesc& P, hipStream_t s) {
+    int pf = 1, q = 0;
+    split_pq(P.n / 2, pf, q);
+    if (pf != 1) {
+        if (pf == 3) launch_pen_pf<3, false>(q, P, s);
+        else if (pf == 5) launch_pen_pf<5, false>(q, P, s);
+        else launch_pen_pf<7, false>(q, P, s);
+        HIP_CHECK(hipGetLastError());
+        return;
+    }
     switch (P.n / 2) {
-        case 16: launch_pen_h<16, 128, false>(P, s); break;
-        case 32: launch_pen_h<32, 64, false>(P, s); break;
-        case 64: launch_pen_h<64, 32, false>(P, s); break;
-        case 128: launch_pen_h<128, 16, false>(P, s); break;
-        case 256: launch_pen_h<256, 8, false>(P, s); break;
-        case 512: launch_pen_h<512, 4, false>(P, s); break;
-        case 1024: launch_pen_h<1024, 2, false>(P, s); break;
+        case 16: launch_pen_h<1, 16, 128, false>(P, s); break;
+        case 32: launch_pen_h<1, 32, 64, false>(P, s); break;
+        case 64: launch_pen_h<1, 64, 32, false>(P, s); break;
+        case 128: launch_pen_h<1, 128, 16, false>(P, s); break;
+        case 256: launch_pen_h<1, 256, 8, false>(P, s); break;
+        case 512: launch_pen_h<1, 512, 4, false>(P, s); break;
+        case 1024: launch_pen_h<1, 1024, 2, false>(P, s); break;
         default: fail("row r2r: length %d", P.n);
     }
     HIP_CHECK(hipGetLastError());
@@ -710,14 +737,23 @@ void launch_row_r2r(const PassDesc& P, hipStream_t s) {
 
 // strided pencils: R pencils side by side, LDS (H + 1) R complex values (at most 33 KB)
 void launch_col_r2r(const PassDesc& P, hipStream_t s) {
+    int pf = 1, q = 0;
+    split_pq(P.n / 2, pf, q);
+    if (pf != 1) {
+        if (pf == 3) launch_pen_pf<3, true>(q, P, s);
+        else if (pf == 5) launch_pen_pf<5, true>(q, P, s);
+        else launch_pen_pf<7, true>(q, P, s);
+        HIP_CHECK(hipGetLastError());
+        return;
+    }
     switch (P.n / 2) {
-        case 16: launch_pen_h<16, 32, true>(P, s); break;
-        case 32: launch_pen_h<32, 16, true>(P, s); break;
-        case 64: launch_pen_h<64, 16, true>(P, s); break;
-        case 128: launch_pen_h<128, 16, true>(P, s); break;
-        case 256: launch_pen_h<256, 8, true>(P, s); break;
-        case 512: launch_pen_h<512, 4, true>(P, s); break;
-        case 1024: launch_pen_h<1024, 2, true>(P, s); break;
+        case 16: launch_pen_h<1, 16, 32, true>(P, s); break;
+        case 32: launch_pen_h<1, 32, 16, true>(P, s); break;
+        case 64: launch_pen_h<1, 64, 16, true>(P, s); break;
+        case 128: launch_pen_h<1, 128, 16, true>(P, s); break;
+        case 256: launch_pen_h<1, 256, 8, true>(P, s); break;
+        case 512: launch_pen_h<1, 512, 4, true>(P, s); break;
+        case 1024: launch_pen_h<1, 1024, 2, true>(P, s); break;
         default: fail("column r2r: length %d", P.n);
     }
     HIP_CHECK(hipGetLastError());

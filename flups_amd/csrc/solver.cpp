@@ -328,7 +328,9 @@ PassDesc SolverImpl::pass_desc(const DirPlan& p, int comp, bool forward, const V
     // contiguous real-to-real pencils of a power-of-two length: the row r2r kernel
     if (!std::getenv("FLUPS_GENERIC_PASS") && row_r2r_fits(P)) {
         P.fastk = 1;
-        P.tw = twiddles(P.n / 2);
+        int pf = 1, q = P.n / 2;
+        split_pq(P.n / 2, pf, q);
+        P.tw = twiddles(q);
         P.aux = r2r_table(P.kind, P.n);
         const int fa = P.fa, fb = P.fb;
         const bool plain = P.gshift == 0 && P.sshift == 0 && P.src_nd == P.n && P.dst_nd == P.n && P.pen_in == P.n &&
@@ -341,7 +343,9 @@ PassDesc SolverImpl::pass_desc(const DirPlan& p, int comp, bool forward, const V
     // on tiles of neighbouring pencils
     if (!std::getenv("FLUPS_GENERIC_PASS") && col_r2r_fits(P)) {
         P.fastk = 4;
-        P.tw = twiddles(P.n / 2);
+        int pf = 1, q = P.n / 2;
+        split_pq(P.n / 2, pf, q);
+        P.tw = twiddles(q);
         P.aux = r2r_table(P.kind, P.n);
         return P;
     }

@@ -601,6 +601,15 @@ ROW_CASES = [
     ((80, 32, 16), [[4, 4], [1, 1], [4, 4]], 1, 0, 3),
     ((112, 16, 16), [[4, 4], [0, 0], [0, 0]], 1, 2, 3),
     ((49, 17, 17), [[3, 3], [3, 3], [3, 3]], 0, 0, 3),
+    # real-to-real pencils of 3, 5, 7 times a power of two: rows (x) and strided tiles (y, z)
+    ((48, 32, 32), [[0, 1], [3, 3], [4, 4]], 1, 8, 1),
+    ((96, 32, 16), [[0, 0], [4, 4], [3, 3]], 1, 2, 1),
+    ((80, 16, 16), [[1, 1], [3, 3], [4, 4]], 1, 0, 1),
+    ((112, 16, 16), [[1, 0], [4, 4], [4, 4]], 1, 4, 1),
+    ((48, 32, 16), [[0, 4], [3, 3], [3, 3]], 1, 2, 1),
+    ((32, 48, 32), [[4, 4], [0, 0], [3, 3]], 1, 2, 4),
+    ((16, 32, 160), [[3, 3], [4, 4], [1, 0]], 1, 0, 4),
+    ((32, 56, 16), [[3, 3], [4, 1], [4, 4]], 1, 4, 4),
 ]
 
 
@@ -609,7 +618,9 @@ ROW_CASES = [
                                                  "oe-unby-unbz", "unbx-eey", "perx-ooy", "node-per", "eoz-128", "ragged-ooy",
                                                  "eunb-y", "unbo-y-oez", "ooz-512", "eey-2048", "c2c-y-256", "c2c-y-32",
                                                  "c2c-2d-1024", "c2c-2d-2048", "c2c-5x8-3x32", "c2c-3x16", "2d-5x32",
-                                                 "c2c-5x16-5x32", "rc-3x8", "rc-5x16", "rc-7x16", "rc-node-3x8"])
+                                                 "c2c-5x16-5x32", "rc-3x8", "rc-5x16", "rc-7x16", "rc-node-3x8",
+                                                 "r2r-eo-3x8", "r2r-ee-3x16", "r2r-oo-5x8", "r2r-oe-7x8", "r2r-eu-3x16",
+                                                 "col-ee-3x8", "col-oe-5x16", "col-uo-7x8"])
 @pytest.mark.parametrize("misalign", [False, True])
 def test_fast_pass_kernels_vs_generic_and_oracle(case, misalign, monkeypatch):
     """the fast kernels of the general path (real-to-real pencils and tiles, paired real columns,
