@@ -304,9 +304,10 @@ __global__ void __launch_bounds__(ROWS * PF * Q / 8) row_rc(PassDesc P) {
     }
 }
 
-template <int M, int TX, bool FWD>
-__global__ void __launch_bounds__(TX * M / 8) col_real(PassDesc P) {
-    constexpr int T = M / 8, NH = M / 2;
+// M = PF Q: PF = 1 (a power of two) or 3, 5 (8 PF values per thread, column_fft_pq)
+template <int PF, int Q, int TX, bool FWD>
+__global__ void __launch_bounds__(TX * Q / 8) col_real(PassDesc P) {
+    constexpr int M = PF * Q, T = Q / 8, NH = M / 2, NV = 8 * PF;
     __shared__ C2<double> lds[(M + 2) * TX];
     const int c = threadIdx.x % TX, j = threadIdx.x / TX;
     const int npair = (P.na + 1) / 2;
@@ -323,11 +324,12 @@ __global__ void __launch_bounds__(TX * M / 8) col_real(PassDesc P) {
     const double* csrc = P.src + 2 * ((long long)jb * P.src_s[P.fb] + xa);       // !FWD: complex source
     double* cdst = P.dst + 2 * ((long long)jb * P.dst_s[P.fb] + xa);             // FWD: complex destination
     const bool hasb = lb;   // a real partner column exists
-    const C2<double>* tw = reinterpret_cast<const C2<double>*>(P.tw);
-    C2<double> v[8];
+    const C2<double>* tw = reinterpret_cast<const C2<double>*>(P.tw);    // Q-point table
+    const C2<double>* twm = reinterpret_cast<const C2<double>*>(P.aux);  // M-point table (PF > 1)
+    C2<double> v[NV];
     if (FWD) {
 #pragma unroll
-        for (int i = 0; i < 8; ++i) {
+        for (int i = 0; i < NV; ++i) {
             const int row = j + i * T;
             const double* p = rsrc + (long long)row * sr;
             if (row >= P.src_nd) {   // zero padding of an unbounded direction
@@ -339,17 +341,17 @@ __global__ void __launch_bounds__(TX * M / 8) col_real(PassDesc P) {
                 v[i] = {p[0], hasb ? p[1] : 0.0};
             }
         }
-        column_fft<double, M, true, TX>(v, lds + c, j, tw);
+        column_fft_pq<double, PF, Q, true, TX>(v, lds + c, j, tw, twm, 1);
         __syncthreads();   // the FFT's last LDS reads are done
 #pragma unroll
-        for (int i = 0; i < 8; ++i) lds[(j + i * T) * TX + c] = v[i];
+        for (int i = 0; i < NV; ++i) lds[(j + i * T) * TX + c] = v[i];
         __syncthreads();
         if (!la) return;
 #pragma unroll
-        for (int i = 0; i < 5; ++i) {
+        for (int i = 0; i <= NV / 2; ++i) {
             const int k = j + i * T;
-            if (i == 4 && j != 0) break;   // k = n/2 once
-            const C2<double> zk = lds[k * TX + c], zc = cconj(lds[((M - k) & (M - 1)) * TX + c]);
+            if (i == NV / 2 && j != 0) break;   // k = n/2 once
+            const C2<double> zk = lds[k * TX + c], zc = cconj(lds[(k == 0 ? 0 : M - k) * TX + c]);
             const C2<double> A = {0.5 * (zk.x + zc.x), 0.5 * (zk.y + zc.y)};
             const C2<double> df = {0.5 * (zk.x - zc.x), 0.5 * (zk.y - zc.y)};
             double* o = cdst + 2 * (long long)k * sc;
@@ -359,9 +361,9 @@ __global__ void __launch_bounds__(TX * M / 8) col_real(PassDesc P) {
     } else {
         // rows 0..n/2 of the two spectra into LDS: A at row k, B at row n/2 + 1 + k
 #pragma unroll
-        for (int i = 0; i < 5; ++i) {
+        for (int i = 0; i <= NV / 2; ++i) {
             const int k = j + i * T;
-            if (i == 4 && j != 0) break;
+            if (i == NV / 2 && j != 0) break;
             const double* p = csrc + 2 * (long long)k * sc;
             const double2 qa = *reinterpret_cast<const double2*>(p);
             double2 qb = hasb ? *reinterpret_cast<const double2*>(p + 2) : double2{0.0, 0.0};
@@ -375,16 +377,16 @@ __global__ void __launch_bounds__(TX * M / 8) col_real(PassDesc P) {
         }
         __syncthreads();
 #pragma unroll
-        for (int i = 0; i < 8; ++i) {
+        for (int i = 0; i < NV; ++i) {
             const int k = j + i * T;
             const int m = k <= NH ? k : M - k;
             const C2<double> A = lds[m * TX + c], B = lds[(NH + 1 + m) * TX + c];
             v[i] = k <= NH ? C2<double>{A.x - B.y, A.y + B.x} : C2<double>{A.x + B.y, B.x - A.y};
         }
-        column_fft<double, M, false, TX>(v, lds + c, j, tw);   // syncs before its first LDS write
+        column_fft_pq<double, PF, Q, false, TX>(v, lds + c, j, tw, twm, 1);   // syncs before its first LDS write
         if (!la) return;
 #pragma unroll
-        for (int i = 0; i < 8; ++i) {
+        for (int i = 0; i < NV; ++i) {
             const int row = j + i * T;
             if (row >= P.dst_nd) continue;   // truncation back to the field's extent
             double* p = rdst + (long long)row * sr;
@@ -489,14 +491,27 @@ void launch_row_rc_pf(int q, const PassDesc& P, hipStream_t s) {
     }
 }
 
-template <int M, int TX>
+template <int PF, int Q, int TX>
 void launch_col_real_m(const PassDesc& P, hipStream_t s) {
     const int npair = (P.na + 1) / 2;
     const int ntile = (npair + TX - 1) / TX;
     const int nb = P.n1 * P.n2 / P.na;
     dim3 grid(ntile * nb);
-    if (P.op == OP_R2C) hipLaunchKernelGGL((col_real<M, TX, true>), grid, dim3(TX * M / 8), 0, s, P);
-    else hipLaunchKernelGGL((col_real<M, TX, false>), grid, dim3(TX * M / 8), 0, s, P);
+    if (P.op == OP_R2C) hipLaunchKernelGGL((col_real<PF, Q, TX, true>), grid, dim3(TX * Q / 8), 0, s, P);
+    else hipLaunchKernelGGL((col_real<PF, Q, TX, false>), grid, dim3(TX * Q / 8), 0, s, P);
+}
+// mixed radix: TX Q = 1024 (PF 3) or 512 (PF 5), LDS about 48 / 40 KB
+template <int PF>
+void launch_col_real_pf(int q, const PassDesc& P, hipStream_t s) {
+    constexpr int TQ = PF == 3 ? 1024 : 512;
+    switch (q) {
+        case 8: launch_col_real_m<PF, 8, TQ / 8>(P, s); break;
+        case 16: launch_col_real_m<PF, 16, TQ / 16>(P, s); break;
+        case 32: launch_col_real_m<PF, 32, TQ / 32>(P, s); break;
+        case 64: launch_col_real_m<PF, 64, TQ / 64>(P, s); break;
+        case 128: launch_col_real_m<PF, 128, TQ / 128>(P, s); break;
+        default: fail("col real: length %d", P.n);
+    }
 }
 
 template <int PF, int Q, int TX>
@@ -651,7 +666,7 @@ void launch_row_rc(const PassDesc& P, hipStream_t s) {
 
 bool col_real_fits(const PassDesc& P) {
     const int n = P.n;
-    if (!(P.op == OP_R2C || P.op == OP_C2R) || n < 32 || n > 1024 || (n & (n - 1))) return false;
+    if (!(P.op == OP_R2C || P.op == OP_C2R) || !fft_len_ok(n, 32, 1024, 5)) return false;
     const bool fwd = P.op == OP_R2C;
     const int d = P.d, fa = P.fa;
     if (fa == d || P.src_s[fa] != 1 || P.dst_s[fa] != 1) return false;
@@ -668,13 +683,21 @@ bool col_real_fits(const PassDesc& P) {
 }
 
 void launch_col_real(const PassDesc& P, hipStream_t s) {
+    int pf = 1, q = 0;
+    split_pq(P.n, pf, q);
+    if (pf != 1) {
+        if (pf == 3) launch_col_real_pf<3>(q, P, s);
+        else launch_col_real_pf<5>(q, P, s);
+        HIP_CHECK(hipGetLastError());
+        return;
+    }
     switch (P.n) {
-        case 32: launch_col_real_m<32, 16>(P, s); break;
-        case 64: launch_col_real_m<64, 16>(P, s); break;
-        case 128: launch_col_real_m<128, 8>(P, s); break;
-        case 256: launch_col_real_m<256, 8>(P, s); break;
-        case 512: launch_col_real_m<512, 4>(P, s); break;     // LDS (M + 2) TX complex: 33 KB at most
-        case 1024: launch_col_real_m<1024, 2>(P, s); break;
+        case 32: launch_col_real_m<1, 32, 16>(P, s); break;
+        case 64: launch_col_real_m<1, 64, 16>(P, s); break;
+        case 128: launch_col_real_m<1, 128, 8>(P, s); break;
+        case 256: launch_col_real_m<1, 256, 8>(P, s); break;
+        case 512: launch_col_real_m<1, 512, 4>(P, s); break;     // LDS (M + 2) TX complex: 33 KB at most
+        case 1024: launch_col_real_m<1, 1024, 2>(P, s); break;
         default: fail("col real: length %d", P.n);
     }
     HIP_CHECK(hipGetLastError());

@@ -372,7 +372,10 @@ PassDesc SolverImpl::pass_desc(const DirPlan& p, int comp, bool forward, const V
     // pairs of real pencils as one complex column
     if (!std::getenv("FLUPS_GENERIC_PASS") && col_real_fits(P)) {
         P.fastk = 2;
-        P.tw = twiddles(P.n);
+        int pf = 1, q = P.n;
+        split_pq(P.n, pf, q);
+        P.tw = twiddles(q);
+        P.aux = twiddles(P.n);
         const bool fwd = P.op == OP_R2C;
         const double* real = fwd ? P.src : P.dst;
         const long long* rs = fwd ? P.src_s : P.dst_s;
