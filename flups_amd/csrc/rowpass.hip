@@ -46,6 +46,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdlib>
 #include <vector>
 
 #include "core.hpp"
@@ -405,7 +406,7 @@ __global__ void __launch_bounds__(TX * Q / 8) col_real(PassDesc P) {
 // zero) and the scatter truncates (pencil position p lands in row p - sshift when that
 // row is below dst_nd). HIN: rows M/2 and up are padding (gshift 0, src_nd <= M/2), the
 // first stage skips them; HOUT: only rows below M/2 are stored (sshift 0, dst_nd <= M/2).
-// M = PF Q: PF = 1 (a power of two) or 3, 5 (mixed radix, column_fft_pq)
+// M = PF Q: PF = 1 (a power of two) or 3 (mixed radix, 8 PF values per thread, column_fft_pq)
 template <int PF, int Q, int TX, bool FWD, bool HIN, bool HOUT>
 __global__ void __launch_bounds__(TX * Q / 8) col_c2c(PassDesc P) {
     constexpr int M = PF * Q, T = Q / 8, NV = 8 * PF;
@@ -436,6 +437,42 @@ __global__ void __launch_bounds__(TX * Q / 8) col_c2c(PassDesc P) {
 #pragma unroll
     for (int i = 0; i < (HOUT ? NV / 2 : NV); ++i) {
         const int k = j + i * T - P.sshift;
+        if (k >= 0 && k < P.dst_nd) dst[k * dd] = v[i];
+    }
+}
+
+// Strided complex pencils of M = PF Q points through thread teams (fft_pq_team): the TX
+// columns are staged in LDS (rows read as contiguous runs of TX values), M/8 threads per
+// column form the PF twiddled inputs and run the Q-point FFTs. Same plan semantics as col_c2c.
+template <int PF, int Q, int TX, bool FWD>
+__global__ void __launch_bounds__(TX * PF * Q / 8) col_c2c_team(PassDesc P) {
+    constexpr int M = PF * Q, T = Q / 8, TT = M / 8;
+    __shared__ C2<double> lds[M * TX];
+    const int c = threadIdx.x % TX, tt = threadIdx.x / TX;
+    const int k1 = tt / T, j = tt % T;
+    const int ntile = (P.na + TX - 1) / TX;
+    const int tile = blockIdx.x % ntile, jb = blockIdx.x / ntile;
+    int ja = tile * TX + c;
+    const bool live = ja < P.na;
+    if (!live) ja = P.na - 1;   // dead columns read the last one and store nothing
+    const C2<double>* src =
+        reinterpret_cast<const C2<double>*>(P.src) + (long long)ja * P.src_s[P.fa] + (long long)jb * P.src_s[P.fb];
+    C2<double>* dst = reinterpret_cast<C2<double>*>(P.dst) + (long long)ja * P.dst_s[P.fa] + (long long)jb * P.dst_s[P.fb];
+    const long long sd = P.src_s[P.d], dd = P.dst_s[P.d];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const int r = tt + i * TT, ks = r - P.gshift;
+        lds[r * TX + c] = (ks >= 0 && ks < P.src_nd) ? src[ks * sd] : C2<double>{0.0, 0.0};
+    }
+    __syncthreads();
+    auto load = [&](int m) -> C2<double> { return lds[m * TX + c]; };
+    C2<double> v[8];
+    fft_pq_team<double, PF, Q, FWD, TX>(v, lds + c, k1, j, load, reinterpret_cast<const C2<double>*>(P.tw),
+                                         reinterpret_cast<const C2<double>*>(P.aux), 1);
+    if (!live) return;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const int k = k1 + PF * (j + i * T) - P.sshift;
         if (k >= 0 && k < P.dst_nd) dst[k * dd] = v[i];
     }
 }
@@ -529,7 +566,27 @@ void launch_col_c2c_m(const PassDesc& P, hipStream_t s) {
         else hipLaunchKernelGGL((col_c2c<PF, Q, TX, false, false, false>), grid, block, 0, s, P);
     }
 }
-// mixed radix: TX Q = 1024 (PF 3) or 512 (PF 5), LDS 48 / 40 KB, two or one waves per block
+template <int PF, int Q, int TX>
+void launch_col_c2c_t(const PassDesc& P, hipStream_t s) {
+    const int ntile = (P.na + TX - 1) / TX;
+    dim3 grid(ntile * (P.n1 * P.n2 / P.na));
+    dim3 block(TX * PF * Q / 8);
+    if (P.sign == FWD) hipLaunchKernelGGL((col_c2c_team<PF, Q, TX, true>), grid, block, 0, s, P);
+    else hipLaunchKernelGGL((col_c2c_team<PF, Q, TX, false>), grid, block, 0, s, P);
+}
+// thread teams: TX M <= 4096 (64 KB of LDS), TX >= 4
+template <int PF>
+void launch_col_c2c_team_pf(int q, const PassDesc& P, hipStream_t s) {
+    switch (q) {
+        case 8: launch_col_c2c_t<PF, 8, 16>(P, s); break;
+        case 16: launch_col_c2c_t<PF, 16, 16>(P, s); break;
+        case 32: launch_col_c2c_t<PF, 32, (PF * 32 <= 128 ? 16 : 8)>(P, s); break;
+        case 64: launch_col_c2c_t<PF, 64, (PF * 64 <= 256 ? 16 : PF * 64 <= 512 ? 8 : 4)>(P, s); break;
+        case 128: launch_col_c2c_t<PF, 128, (PF * 128 <= 512 ? 8 : 4)>(P, s); break;
+        default: fail("column c2c: length %d", P.n);
+    }
+}
+// mixed radix, 8 PF values per thread: TX Q = 1024 (PF 3) or 512, LDS 48 / 40 KB
 template <int PF>
 void launch_col_c2c_pf(int q, const PassDesc& P, hipStream_t s) {
     constexpr int TQ = PF == 3 ? 1024 : 512;
@@ -561,11 +618,11 @@ bool split_pq(int n, int& pf, int& q) {
 
 namespace {
 // lengths of the register column FFT: powers of two in [lo, hi], or PF times a power of two in
-// [8, 128] with 3 <= PF <= maxpf. The strided c2c kernel holds 8 PF values per thread (Q/8
-// threads per pencil): PF = 7 needs 300 registers there and gains nothing over the generic
-// kernel, PF = 3 halves its time. The row r2c / c2r kernel splits a pencil over PF teams
-// (H/8 threads per pencil, 8 values each); holding all 8 PF values in one thread instead left
-// too few waves (x of 96 in 96 x 128 x 64: 24 us against 14 us of the generic kernel).
+// [8, 128] with 3 <= PF <= maxpf. Holding 8 PF values per thread (Q/8 threads per pencil) suits
+// the strided kernels that load from HBM at PF = 3 (half the generic kernel's time); at PF = 7
+// it needs 300 registers and gains nothing, and for contiguous r2c rows it left too few waves
+// (x of 96 in 96 x 128 x 64: 24 us against 14 us of the generic kernel). Thread teams (PF
+// teams of Q/8 threads, 8 values each, inputs from LDS) serve the rest.
 bool fft_len_ok(int n, int lo, int hi, int maxpf) {
     int pf = 0, q = 0;
     if (!split_pq(n, pf, q)) return false;
@@ -666,7 +723,7 @@ void launch_row_rc(const PassDesc& P, hipStream_t s) {
 
 bool col_real_fits(const PassDesc& P) {
     const int n = P.n;
-    if (!(P.op == OP_R2C || P.op == OP_C2R) || !fft_len_ok(n, 32, 1024, 5)) return false;
+    if (!(P.op == OP_R2C || P.op == OP_C2R) || !fft_len_ok(n, 32, 1024, 5)) return false;   // 8 PF values per thread
     const bool fwd = P.op == OP_R2C;
     const int d = P.d, fa = P.fa;
     if (fa == d || P.src_s[fa] != 1 || P.dst_s[fa] != 1) return false;
@@ -705,7 +762,7 @@ void launch_col_real(const PassDesc& P, hipStream_t s) {
 
 bool col_c2c_fits(const PassDesc& P) {
     const int n = P.n;
-    if (P.op != OP_C2C || !fft_len_ok(n, 32, 2048, 5) || P.M != n) return false;
+    if (P.op != OP_C2C || !fft_len_ok(n, 32, 2048, 7) || P.M != n) return false;
     const int d = P.d, fa = P.fa;
     if (fa == d || P.src_s[fa] != 1 || P.dst_s[fa] != 1 || (P.n1 * P.n2) % P.na != 0) return false;
     if (!(P.nf_in == 2 && P.nf_out == 2 && P.src_nf == 2 && P.dst_nf == 2 && P.pen_in == n && P.pen_out == n)) return false;
@@ -717,8 +774,11 @@ void launch_col_c2c(const PassDesc& P, hipStream_t s) {
     int pf = 1, q = 0;
     split_pq(P.n, pf, q);
     if (pf != 1) {
+        // 24 values per thread for PF = 3 (HBM loads straight into registers); thread teams over
+        // the LDS-staged columns for 5 and 7 (40 or 56 values would crowd the registers)
         if (pf == 3) launch_col_c2c_pf<3>(q, P, s);
-        else launch_col_c2c_pf<5>(q, P, s);
+        else if (pf == 5) launch_col_c2c_team_pf<5>(q, P, s);
+        else launch_col_c2c_team_pf<7>(q, P, s);
         HIP_CHECK(hipGetLastError());
         return;
     }

@@ -614,6 +614,9 @@ ROW_CASES = [
     ((32, 48, 32), [[0, 0], [3, 3], [4, 4]], 1, 2, 2),
     ((32, 40, 32), [[1, 1], [4, 4], [3, 3]], 1, 2, 2),
     ((33, 96, 16), [[1, 0], [3, 3], [4, 4]], 1, 0, 2),
+    # strided c2c of 7 times a power of two (thread teams over the LDS-staged columns)
+    ((16, 56, 32), [[3, 3], [3, 3], [4, 4]], 1, 2, 5),
+    ((16, 32, 112), [[4, 4], [3, 3], [4, 4]], 1, 0, 5),
 ]
 
 
@@ -625,7 +628,7 @@ ROW_CASES = [
                                                  "c2c-5x16-5x32", "rc-3x8", "rc-5x16", "rc-7x16", "rc-node-3x8",
                                                  "r2r-eo-3x8", "r2r-ee-3x16", "r2r-oo-5x8", "r2r-oe-7x8", "r2r-eu-3x16",
                                                  "col-ee-3x8", "col-oe-5x16", "col-uo-7x8", "real-3x16", "real-unb-5x16",
-                                                 "real-odd-3x32"])
+                                                 "real-odd-3x32", "c2c-7x8", "c2c-unb-7x32"])
 @pytest.mark.parametrize("misalign", [False, True])
 def test_fast_pass_kernels_vs_generic_and_oracle(case, misalign, monkeypatch):
     """the fast kernels of the general path (real-to-real pencils and tiles, paired real columns,
