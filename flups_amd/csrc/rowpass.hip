@@ -1,9 +1,12 @@
 // Fast kernels of the general path for its most common pencil shapes, used in
 // place of the generic pass kernel (generic.hip) when a pass fits them.
 //
-// row_r2r: real-to-real transforms of kinds REDFT10/01/11 and RODFT10/01/11
+// Lengths: a power of two, or 3, 5, 7 times a power of two (mixed radix, see
+// column_fft_pq and fft_pq_team in fft_core.hpp); the ranges are in the fit tests.
+//
+// row_r2r (pen_r2r, rows): real-to-real transforms of kinds REDFT10/01/11 and RODFT10/01/11
 // (src/FFTW_plan_dim.cpp:635-720 runs them through FFTW) along a contiguous
-// pencil of even length N = 2H, H a power of two in [16, 1024]. Each is one
+// pencil of even length N = 2H, H in [16, 1024]. Each is one
 // complex FFT of length H held in registers (the column FFT of fft_core.hpp)
 // with pre- and post-twiddles, instead of the 2N-point extension of the
 // generic kernel:
@@ -23,7 +26,7 @@
 // with the field shift and zero padding, input / output offsets of 0 or 1,
 // the first / last point zeroed after the transform, truncated scatter.
 //
-// col_real: r2c / c2r of length n (a power of two in [32, 1024]) along a strided
+// col_real: r2c / c2r of length n (in [32, 1024]) along a strided
 // direction whose pencils are neighbours along a unit-stride dimension (the
 // y or z passes of the general path, x fastest). Two neighbouring real pencils
 // a, b are one complex column a + i b: Z = FFT_n(a + i b), A[k] = (Z[k] + conj
@@ -33,12 +36,13 @@
 // and a + i b = IFFT_n(Z). TX pairs side by side per block: each row of a
 // tile is one contiguous run of 2 TX reals or complex values.
 //
-// col_c2c: complex-to-complex transforms of length M (a power of two in [32, 2048]) along a
+// col_c2c: complex-to-complex transforms of length M (in [32, 2048]) along a
 // strided direction, TX neighbouring pencils per block, with the zero padding of the gather
 // and the truncation of the scatter (the second and third transforms of unbounded and
-// periodic directions in the general path).
+// periodic directions in the general path). col_c2c_team: the same for M = 5 or 7 times a
+// power of two, the columns staged in LDS first.
 //
-// row_rc: r2c / c2r of even length n = 2H (H a power of two in [16, 1024]) along a
+// row_rc: r2c / c2r of even length n = 2H (H in [16, 1024]) along a
 // contiguous pencil through the packed complex FFT of length H: z[m] = x[2m] + i x[2m+1],
 // X[k] = (Z[k] + conj Z[H-k])/2 + e^{-2 pi i k/n} (Z[k] - conj Z[H-k])/(2i), k <= H; the
 // inverse Z[k] = (X[k] + conj X[H-k]) + i e^{2 pi i k/n} (X[k] - conj X[H-k]), x[2m] + i x[2m+1]
