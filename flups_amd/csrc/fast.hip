@@ -66,6 +66,7 @@ struct ColArgs {
     int x0;              // z_fused / z_rot: spectral x index of column 0 (slab offset, a multiple of TX)
     int gntx;            // z_fused / z_rot: x tiles of one ky' row of the compressed G
     int gtx;             // z_rot: tile width of the compressed G (a multiple of the kernel's TX)
+    const void* twm;     // z_fused with PF > 1: M-point twiddle table (tw is then the M/PF-point one)
 };
 
 // TX columns per block, M/8 threads per column; block = TX * M/8 threads.
@@ -116,12 +117,15 @@ __global__ void __launch_bounds__(TX * M / 8) col_fft(ColArgs a) {
 // after the first one is stored.
 // HALF: nin = nout = M/2 (unbounded direction): only four rows per thread are
 // loaded and stored, the zero half is pruned from the first forward stage.
-template <typename R, int M, int TX, int SCHED, bool HALF>
-__global__ void __launch_bounds__(TX * M / 8) z_fused(ColArgs a) {
+// PF = 3: M = 3 Q (the general path's fused last direction), 8 PF values per thread through
+// column_fft_pq, no HALF pruning.
+template <typename R, int M, int TX, int SCHED, bool HALF, int PF = 1>
+__global__ void __launch_bounds__(TX * M / (8 * PF)) z_fused(ColArgs a) {
     // SCHED 0: G loaded with the first set, second set loaded after the first is stored;
     //       1: both sets prefetched up front; 2: G loaded just before each multiply.
+    static_assert(PF == 1 || !HALF, "the zero-padded half is pruned for powers of two only");
     __shared__ C2<R> lds[M * TX];
-    constexpr int T = M / 8;
+    constexpr int Q = M / PF, T = Q / 8, NV = 8 * PF;
     const int c = threadIdx.x % TX;
     const int j = threadIdx.x / TX;
     const int ntx = (a.nx + TX - 1) / TX;
@@ -147,22 +151,23 @@ __global__ void __launch_bounds__(TX * M / 8) z_fused(ColArgs a) {
     const size_t rstride = (size_t)sizeof(C2<R>) * T * a.s_col;                      // i -> i + 1
     const unsigned voff = (unsigned)(sizeof(C2<R>) * ((size_t)j * a.s_col + c));     // < 4 GiB: j < M/8
     const unsigned loff = (unsigned)(sizeof(C2<R>) * ((size_t)j * a.s_col + min(c, a.nx - 1 - tile * TX)));
-    constexpr int NL = HALF ? 4 : 8;   // rows loaded / stored per thread
+    constexpr int NL = HALF ? 4 : NV;   // rows loaded / stored per thread
     const char* g = reinterpret_cast<const char*>(a.green) +
                     sizeof(R) * ((long long)y0 * a.gntx + a.x0 / TX + tile) * (M / 2 + 1) * TX;
     const C2<R>* tw = reinterpret_cast<const C2<R>*>(a.tw);
-    R gv[8];
-    C2<R> v[8], w[8];
+    const C2<R>* twm = reinterpret_cast<const C2<R>*>(a.twm);
+    R gv[NV];
+    C2<R> v[NV], w[NV];
     auto load_g = [&]() {
 #pragma unroll
-        for (int i = 0; i < 8; ++i) {
+        for (int i = 0; i < NV; ++i) {
             const int kz = j + i * T;
             gv[i] = *reinterpret_cast<const R*>(g + (unsigned)(sizeof(R) * (min(kz, M - kz) * TX + c)));
         }
     };
     auto load = [&](C2<R>* u, const char* b) {
 #pragma unroll
-        for (int i = 0; i < 8; ++i) {
+        for (int i = 0; i < NV; ++i) {
             const int row = j + i * T;
             if (HALF) u[i] = i < 4 ? *reinterpret_cast<const C2<R>*>(b + i * rstride + loff) : C2<R>{0, 0};
             else u[i] = row < a.nin ? *reinterpret_cast<const C2<R>*>(b + i * rstride + loff) : C2<R>{0, 0};
@@ -172,14 +177,14 @@ __global__ void __launch_bounds__(TX * M / 8) z_fused(ColArgs a) {
     load(v, b0);
     if (SCHED == 1 && pair) load(w, b1);
     for (int s = 0; s < 2; ++s) {
-        column_fft<R, M, true, TX, HALF>(v, lds + c, j, tw);
+        column_fft_pq<R, PF, Q, true, TX, HALF>(v, lds + c, j, tw, twm, 1);
         if (SCHED == 2) load_g();
 #pragma unroll
-        for (int i = 0; i < 8; ++i) {
+        for (int i = 0; i < NV; ++i) {
             v[i].x *= gv[i];
             v[i].y *= gv[i];
         }
-        column_fft<R, M, false, TX>(v, lds + c, j, tw);
+        column_fft_pq<R, PF, Q, false, TX>(v, lds + c, j, tw, twm, 1);   // syncs before its first LDS write
         char* b = s == 0 ? b0 : b1;
         if (live) {
 #pragma unroll
@@ -192,7 +197,7 @@ __global__ void __launch_bounds__(TX * M / 8) z_fused(ColArgs a) {
         if (s == 0) {
             if (SCHED == 1) {
 #pragma unroll
-                for (int i = 0; i < 8; ++i) v[i] = w[i];
+                for (int i = 0; i < NV; ++i) v[i] = w[i];
             } else {
                 load(v, b1);
             }
@@ -960,9 +965,11 @@ size_t fast_device_bytes(const FastPlan* f) { return f->dev_bytes; }
 // one row (columns up to 2048 long, two columns per block).
 struct HybridZ {
     int M = 0, nx = 0, ny = 0, nz = 0, lda = 1, gtx = 8;
+    int pf = 1;              // M = pf 2^k: 1, or 3 (mixed-radix column FFT, 8 pf values per thread)
     bool pair = false, twod = false;
     void* green = nullptr;   // normfact * Re G, [y'][tile][kz'][gtx]
-    void* tw = nullptr;
+    void* tw = nullptr;      // (M / pf)-point table
+    void* twm = nullptr;     // M-point table (pf 3)
     size_t dev_bytes = 0;
     double bytes = 0;        // algorithmic bytes of one launch
 };
@@ -982,7 +989,8 @@ HybridZ* make_hybrid_z(SolverImpl& s) {
     const Shape& sp = s.stage_[nd - 1];   // spectral field (and Green's function)
     if (in.nf != 2 || sp.nf != 2 || in.n[0] != sp.n[0] || (nd == 3 && in.n[1] != sp.n[1])) return nullptr;
     const int M = f.n_in[0];
-    if (!pow2_in(M, 32, nd == 2 ? 2048 : 1024) || sp.n[nd - 1] != M) return nullptr;
+    const bool p2 = pow2_in(M, 32, nd == 2 ? 2048 : 1024), p3 = M % 3 == 0 && pow2_in(M / 3, 8, 128);
+    if (!(p2 || p3) || sp.n[nd - 1] != M) return nullptr;
     const int ny = nd == 3 ? in.n[1] : 1;
     if ((double)(M / 8) * in.n[0] * ny * 16.0 >= 4294967296.0) return nullptr;
     auto* h = new HybridZ;
@@ -998,6 +1006,7 @@ HybridZ* make_hybrid_z(SolverImpl& s) {
     // half-width x tiles in 3-D; two columns per block in 2-D (few columns, cache-resident field)
     h->gtx = nd == 2 ? 2 : 4;
     h->twod = nd == 2;
+    h->pf = p2 ? 1 : 3;
     const int ntx = (h->nx + h->gtx - 1) / h->gtx;
     const int gy = h->pair ? h->ny / 2 + 1 : h->ny, gz = M / 2 + 1;
     const size_t gn = (size_t)gy * ntx * gz * h->gtx;
@@ -1007,14 +1016,21 @@ HybridZ* make_hybrid_z(SolverImpl& s) {
         hipLaunchKernelGGL(compress_green, dim3(4096), dim3(256), 0, s.stream_, s.dgreen_, (double*)h->green, (float*)nullptr,
                            h->nx, h->ny, M, gy, gz, h->gtx, ntx, s.normfact_);
         HIP_CHECK(hipGetLastError());
-        const auto t = twiddle_table(M);
+        const auto t = twiddle_table(M / h->pf);
         HIP_CHECK(hipMalloc(&h->tw, t.size() * sizeof(double)));
         HIP_CHECK(hipMemcpy(h->tw, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice));
         h->dev_bytes += t.size() * sizeof(double);
+        if (h->pf > 1) {
+            const auto tm = twiddle_table(M);
+            HIP_CHECK(hipMalloc(&h->twm, tm.size() * sizeof(double)));
+            HIP_CHECK(hipMemcpy(h->twm, tm.data(), tm.size() * sizeof(double), hipMemcpyHostToDevice));
+            h->dev_bytes += tm.size() * sizeof(double);
+        }
         HIP_CHECK(hipStreamSynchronize(s.stream_));
     } catch (...) {
         (void)hipFree(h->green);
         (void)hipFree(h->tw);
+        (void)hipFree(h->twm);
         delete h;
         throw;
     }
@@ -1027,10 +1043,30 @@ void destroy_hybrid_z(HybridZ* h) {
     if (!h) return;
     (void)hipFree(h->green);
     (void)hipFree(h->tw);
+    (void)hipFree(h->twm);
     delete h;
 }
 size_t hybrid_device_bytes(const HybridZ* h) { return h->dev_bytes; }
 double hybrid_bytes(const HybridZ* h) { return h->bytes; }
+
+// M = 3 Q: TX columns per block (the compressed G's tile width), Q/8 threads per column
+template <int M, int TX>
+void launch_z_pf3(const ColArgs& a, int comps, hipStream_t st) {
+    dim3 grid(((a.nx + TX - 1) / TX) * (a.nopair == 1 ? a.nother : a.gmy / 2 + 1), comps);
+    hipLaunchKernelGGL((z_fused<double, M, TX, 0, false, 3>), grid, dim3(TX * M / 24), 0, st, a);
+    HIP_CHECK(hipGetLastError());
+}
+template <int TX>
+void dispatch_z_pf3(int M, const ColArgs& a, int comps, hipStream_t st) {
+    switch (M) {
+        case 24: launch_z_pf3<24, TX>(a, comps, st); break;
+        case 48: launch_z_pf3<48, TX>(a, comps, st); break;
+        case 96: launch_z_pf3<96, TX>(a, comps, st); break;
+        case 192: launch_z_pf3<192, TX>(a, comps, st); break;
+        case 384: launch_z_pf3<384, TX>(a, comps, st); break;
+        default: fail("unsupported column length %d", M);
+    }
+}
 
 // in place on the stage array before the z transform: (nx, ny, nz) complex, lda components
 void hybrid_z(const HybridZ* h, double* data, hipStream_t st) {
@@ -1057,6 +1093,12 @@ void hybrid_z(const HybridZ* h, double* data, hipStream_t st) {
     z.nopair = h->pair ? 0 : 1;
     z.x0 = 0;
     z.gntx = (h->nx + h->gtx - 1) / h->gtx;
+    z.twm = h->twm;
+    if (h->pf == 3) {
+        if (h->twod) dispatch_z_pf3<2>(h->M, z, h->lda, st);
+        else dispatch_z_pf3<4>(h->M, z, h->lda, st);
+        return;
+    }
     if (h->twod) {   // two columns per block (the tile width of the compressed G), XCD-grouped
         switch (h->M) {
             case 32: launch_col<double, 32, 2, true, true>(z, h->lda, 16, st); break;

@@ -473,11 +473,17 @@ HYBRID_CASES = [
     ((128, 1024, 1), [[3, 3], [4, 4], [9, 9]], 1, 4, 1),    # x PER, y UNB: columns of 2048
     ((96, 128, 1), [[3, 3], [3, 3], [9, 9]], 1, 0, 2),      # fully periodic 2-D, lda 2
     ((80, 64, 1), [[4, 4], [4, 4], [9, 9]], 1, 8, 1),       # UNB 2-D, LGF_4, x of 160 (generic pass)
+    # last direction of 3 times a power of two (mixed-radix fused column)
+    ((64, 32, 48), [[0, 1], [3, 3], [3, 3]], 1, 0, 1),      # z PER 48
+    ((48, 40, 48), [[4, 4], [4, 4], [4, 4]], 1, 2, 1),      # all UNB, paired y rows, z of 96
+    ((32, 24, 96), [[0, 0], [4, 4], [4, 4]], 1, 4, 3),      # lda 3, z of 192
+    ((64, 48, 1), [[4, 4], [4, 4], [9, 9]], 1, 2, 1),       # 2-D, y of 96
 ]
 
 
 @pytest.mark.parametrize("case", HYBRID_CASES, ids=["mixed64", "semiunb-y", "node-lda3", "sym-per-per", "2d-unb-1024",
-                                                    "2d-per-unb-2048", "2d-per-lda2", "2d-unb-lgf4"])
+                                                    "2d-per-unb-2048", "2d-per-lda2", "2d-unb-lgf4", "pf3-per48", "pf3-unb-pairs",
+                                                    "pf3-lda3-192", "pf3-2d-96"])
 def test_hybrid_last_direction_vs_general_and_oracle(case, monkeypatch):
     """general passes + the last direction fused with the multiply, against the all-general path and the oracle"""
     nglob, bc, center, kernel, lda = case
