@@ -71,6 +71,10 @@ __global__ void __launch_bounds__(R * PF * Q / 8) pen_r2r(PassDesc P) {
     constexpr int S = COL ? R : 1;   // LDS stride between consecutive elements of a pencil
     __shared__ C2<double> lds[COL ? (H + 1) * R : HP * R];
     constexpr int DCT = KIND >= RODFT10 ? KIND - 4 : KIND;
+    // DCT-I / DST-I (node-centred symmetric directions): the even / odd extension of length 2H
+    // through the packed real FFT; n = H + 1 (DCT-I) or H - 1 (DST-I) points in and out
+    constexpr bool E00 = KIND == REDFT00 || KIND == RODFT00;
+    constexpr int NT = KIND == REDFT00 ? H + 1 : KIND == RODFT00 ? H - 1 : N;   // transform outputs
     constexpr bool ALT = KIND == RODFT10;                       // input (-1)^n x, output reversed
     constexpr bool REV = KIND == RODFT01 || KIND == RODFT11;    // input reversed, output (-1)^k
     int rl, tt, ja, jb;
@@ -131,7 +135,13 @@ __global__ void __launch_bounds__(R * PF * Q / 8) pen_r2r(PassDesc P) {
 
     // 2. pre-map to the complex sequence of length H: element m
     auto load = [&](int m) -> C2<double> {
-        if (DCT == REDFT11) {
+        if (KIND == REDFT00) {   // e[q] = x[q] (q <= H), x[2H - q] (q > H)
+            auto e = [&](int q) { return xin(q <= H ? q : N - q); };
+            return {e(2 * m), e(2 * m + 1)};
+        } else if (KIND == RODFT00) {   // o[0] = o[H] = 0, o[q] = x[q-1] (q < H), -x[2H-q-1] (q > H)
+            auto o = [&](int q) { return (q == 0 || q == H) ? 0.0 : q < H ? xin(q - 1) : -xin(N - q - 1); };
+            return {o(2 * m), o(2 * m + 1)};
+        } else if (DCT == REDFT11) {
             return cmul(C2<double>{xin(2 * m), xin(N - 1 - 2 * m)}, A[m]);
         } else if (DCT == REDFT10) {
             auto u = [&](int q) { return q < H ? xin(2 * q) : xin(2 * (N - 1 - q) + 1); };
@@ -154,7 +164,34 @@ __global__ void __launch_bounds__(R * PF * Q / 8) pen_r2r(PassDesc P) {
     __syncthreads();   // every thread is done with the FFT's LDS before the buffer is reused
 
     // 4. post-map into the LDS pencil
-    if (DCT == REDFT11) {
+    if (E00) {
+        // X[k] = (Z[k] + conj Z[H-k])/2 + e^{-i pi k/H} (Z[k] - conj Z[H-k])/(2i), the real DFT of the
+        // extension; DCT-I y[k] = Re X[k] (k <= H), DST-I y[k] = -Im X[k+1] (k < H - 1)
+#pragma unroll
+        for (int i = 0; i < 8; ++i) RW(k1 + PF * (j + i * T)) = v[i];
+        __syncthreads();
+        double XH = 0.0;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const int k = tt + i * TT;
+            const C2<double> zk = RW(k), zc = cconj(RW(k == 0 ? 0 : H - k));
+            const C2<double> e = {0.5 * (zk.x + zc.x), 0.5 * (zk.y + zc.y)};
+            const C2<double> df = {0.5 * (zk.x - zc.x), 0.5 * (zk.y - zc.y)};
+            v[i] = cadd(e, cmul(A[k], C2<double>{df.y, -df.x}));
+        }
+        if (tt == 0) {
+            const C2<double> z0 = RW(0);
+            XH = z0.x - z0.y;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const int k = tt + i * TT;
+            if (KIND == REDFT00) RD(k) = v[i].x;
+            else if (k >= 1) RD(k - 1) = -v[i].y;
+        }
+        if (KIND == REDFT00 && tt == 0) RD(H) = XH;
+    } else if (DCT == REDFT11) {
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
             const int k = k1 + PF * (j + i * T);
@@ -206,7 +243,7 @@ __global__ void __launch_bounds__(R * PF * Q / 8) pen_r2r(PassDesc P) {
     auto val = [&](int p) -> double {
         if (((post & POST_FIRST) && p == 0) || ((post & POST_LAST) && p == pin - 1)) return 0.0;
         const int k = p - so;
-        if (k < 0 || k >= N) return 0.0;
+        if (k < 0 || k >= NT) return 0.0;
         const double y = RD(ALT ? N - 1 - k : k);
         return (REV && (k & 1)) ? -y : y;
     };
@@ -494,6 +531,8 @@ void launch_pen_h(const PassDesc& P, hipStream_t s) {
         case RODFT10: hipLaunchKernelGGL((pen_r2r<PF, Q, R, RODFT10, COL>), grid, block, 0, s, P); break;
         case RODFT01: hipLaunchKernelGGL((pen_r2r<PF, Q, R, RODFT01, COL>), grid, block, 0, s, P); break;
         case RODFT11: hipLaunchKernelGGL((pen_r2r<PF, Q, R, RODFT11, COL>), grid, block, 0, s, P); break;
+        case REDFT00: hipLaunchKernelGGL((pen_r2r<PF, Q, R, REDFT00, COL>), grid, block, 0, s, P); break;
+        case RODFT00: hipLaunchKernelGGL((pen_r2r<PF, Q, R, RODFT00, COL>), grid, block, 0, s, P); break;
         default: fail("r2r pencil kernel: kind %d", P.kind);
     }
 }
@@ -606,6 +645,10 @@ void launch_col_c2c_pf(int q, const PassDesc& P, hipStream_t s) {
 
 }  // namespace
 
+// length H of the complex FFT behind a real-to-real transform of n points: n/2 for the
+// DCT/DST-II, III, IV, n - 1 for DCT-I (extension of 2(n-1)), n + 1 for DST-I (2(n+1))
+int r2r_h(int kind, int n) { return kind == REDFT00 ? n - 1 : kind == RODFT00 ? n + 1 : n / 2; }
+
 // n = pf q with pf in {1, 3, 5, 7} and q a power of two (false otherwise)
 bool split_pq(int n, int& pf, int& q) {
     for (int f : {1, 3, 5, 7}) {
@@ -639,10 +682,11 @@ namespace {
 // the plan semantics the r2r pencil kernels reproduce (kinds, length, offsets, corrections)
 bool r2r_plan_fits(const PassDesc& P) {
     const int k = P.kind;
-    const int n = P.n, h = n / 2;
-    if (P.op != OP_R2R || !(k == REDFT10 || k == REDFT01 || k == REDFT11 || k == RODFT10 || k == RODFT01 || k == RODFT11))
-        return false;
-    if (n % 2 || !fft_len_ok(h, 16, 1024, 7)) return false;
+    const int n = P.n, h = r2r_h(k, n);
+    if (P.op != OP_R2R) return false;
+    const bool e00 = k == REDFT00 || k == RODFT00;
+    if (!(e00 || k == REDFT10 || k == REDFT01 || k == REDFT11 || k == RODFT10 || k == RODFT01 || k == RODFT11)) return false;
+    if ((!e00 && n % 2) || !fft_len_ok(h, 16, 1024, 7)) return false;
     if (!(P.src_nf == 1 && P.dst_nf == 1 && P.nf_in == 1 && P.nf_out == 1 && P.r2r_stride == 1 && P.pen_in == P.pen_out &&
           P.pen_in >= n && P.pen_in <= n + 2 && P.gshift >= 0 && P.sshift >= 0))
         return false;
@@ -663,7 +707,8 @@ bool col_r2r_fits(const PassDesc& P) {
 // pre / post twiddles of the row r2r kernels, (H + 1) complex each, exact to rounding
 std::vector<double> row_r2r_table(int kind, int n) {
     const int dct = kind >= RODFT10 ? kind - 4 : kind;
-    const int h = n / 2;
+    const int h = r2r_h(kind, n);
+    const bool e00 = kind == REDFT00 || kind == RODFT00;
     const long double pi = 3.14159265358979323846264338327950288L, N = n;
     std::vector<double> t(4 * (size_t)(h + 1) + 2 * (size_t)h);   // A, B, then exp(-2 pi i e / h)
     auto put = [&](size_t i, long double angle) {
@@ -671,7 +716,10 @@ std::vector<double> row_r2r_table(int kind, int n) {
         t[2 * i + 1] = (double)sinl(angle);
     };
     for (int k = 0; k <= h; ++k) {
-        if (dct == REDFT11) {
+        if (e00) {   // A[k] = e^{-i pi k / h} (the 2h-point twiddle of the packed real FFT); B unused
+            put(k, -pi * k / h);
+            put(h + 1 + k, 0.0L);
+        } else if (dct == REDFT11) {
             put(k, -pi * (k + 0.25L) / N);
             put(h + 1 + k, -pi * k / N);
         } else if (dct == REDFT10) {
@@ -800,8 +848,9 @@ void launch_col_c2c(const PassDesc& P, hipStream_t s) {
 }
 
 void launch_row_r2r(const PassDesc& P, hipStream_t s) {
+    const int h = r2r_h(P.kind, P.n);
     int pf = 1, q = 0;
-    split_pq(P.n / 2, pf, q);
+    split_pq(h, pf, q);
     if (pf != 1) {
         if (pf == 3) launch_pen_pf<3, false>(q, P, s);
         else if (pf == 5) launch_pen_pf<5, false>(q, P, s);
@@ -809,7 +858,7 @@ void launch_row_r2r(const PassDesc& P, hipStream_t s) {
         HIP_CHECK(hipGetLastError());
         return;
     }
-    switch (P.n / 2) {
+    switch (h) {
         case 16: launch_pen_h<1, 16, 128, false>(P, s); break;
         case 32: launch_pen_h<1, 32, 64, false>(P, s); break;
         case 64: launch_pen_h<1, 64, 32, false>(P, s); break;
@@ -824,8 +873,9 @@ void launch_row_r2r(const PassDesc& P, hipStream_t s) {
 
 // strided pencils: R pencils side by side, LDS (H + 1) R complex values (at most 33 KB)
 void launch_col_r2r(const PassDesc& P, hipStream_t s) {
+    const int h = r2r_h(P.kind, P.n);
     int pf = 1, q = 0;
-    split_pq(P.n / 2, pf, q);
+    split_pq(h, pf, q);
     if (pf != 1) {
         if (pf == 3) launch_pen_pf<3, true>(q, P, s);
         else if (pf == 5) launch_pen_pf<5, true>(q, P, s);
@@ -833,7 +883,7 @@ void launch_col_r2r(const PassDesc& P, hipStream_t s) {
         HIP_CHECK(hipGetLastError());
         return;
     }
-    switch (P.n / 2) {
+    switch (h) {
         case 16: launch_pen_h<1, 16, 32, true>(P, s); break;
         case 32: launch_pen_h<1, 32, 16, true>(P, s); break;
         case 64: launch_pen_h<1, 64, 16, true>(P, s); break;

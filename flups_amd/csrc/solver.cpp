@@ -32,6 +32,7 @@ void launch_row_r2r(const PassDesc& P, hipStream_t s);
 bool col_r2r_fits(const PassDesc& P);
 bool col_c2c_fits(const PassDesc& P);
 bool split_pq(int n, int& pf, int& q);
+int r2r_h(int kind, int n);
 void launch_col_c2c(const PassDesc& P, hipStream_t s);
 void launch_col_r2r(const PassDesc& P, hipStream_t s);
 FastPlan* make_fast_plan(SolverImpl& s);   // nullptr if the fused pipeline does not apply
@@ -328,13 +329,13 @@ PassDesc SolverImpl::pass_desc(const DirPlan& p, int comp, bool forward, const V
     // contiguous real-to-real pencils of a power-of-two length: the row r2r kernel
     if (!std::getenv("FLUPS_GENERIC_PASS") && row_r2r_fits(P)) {
         P.fastk = 1;
-        int pf = 1, q = P.n / 2;
-        split_pq(P.n / 2, pf, q);
+        int pf = 1, q = r2r_h(P.kind, P.n);
+        split_pq(r2r_h(P.kind, P.n), pf, q);
         P.tw = twiddles(q);
         P.aux = r2r_table(P.kind, P.n);
         const int fa = P.fa, fb = P.fb;
         const bool plain = P.gshift == 0 && P.sshift == 0 && P.src_nd == P.n && P.dst_nd == P.n && P.pen_in == P.n &&
-                           P.si == 0 && P.so == 0 && P.post == 0;
+                           P.si == 0 && P.so == 0 && P.post == 0 && P.kind != REDFT00 && P.kind != RODFT00;
         P.vec = plain && ((uintptr_t)P.src % 16 == 0) && ((uintptr_t)P.dst % 16 == 0) && P.src_s[fa] % 2 == 0 &&
                 P.src_s[fb] % 2 == 0 && P.dst_s[fa] % 2 == 0 && P.dst_s[fb] % 2 == 0;
         return P;
@@ -343,8 +344,8 @@ PassDesc SolverImpl::pass_desc(const DirPlan& p, int comp, bool forward, const V
     // on tiles of neighbouring pencils
     if (!std::getenv("FLUPS_GENERIC_PASS") && col_r2r_fits(P)) {
         P.fastk = 4;
-        int pf = 1, q = P.n / 2;
-        split_pq(P.n / 2, pf, q);
+        int pf = 1, q = r2r_h(P.kind, P.n);
+        split_pq(r2r_h(P.kind, P.n), pf, q);
         P.tw = twiddles(q);
         P.aux = r2r_table(P.kind, P.n);
         return P;

@@ -623,6 +623,14 @@ ROW_CASES = [
     # strided c2c of 7 times a power of two (thread teams over the LDS-staged columns)
     ((16, 56, 32), [[3, 3], [3, 3], [4, 4]], 1, 2, 5),
     ((16, 32, 112), [[4, 4], [3, 3], [4, 4]], 1, 0, 5),
+    # node-centred symmetric and semi-unbounded directions: DCT-I / DST-I through the packed
+    # real FFT of the 2(n -+ 1)-point extension (rows and strided tiles)
+    ((33, 17, 17), [[0, 0], [3, 3], [4, 4]], 0, 2, 1),
+    ((17, 33, 17), [[4, 4], [1, 1], [3, 3]], 0, 0, 4),
+    ((65, 33, 17), [[1, 1], [0, 0], [4, 4]], 0, 4, 1),
+    ((33, 17, 17), [[0, 4], [3, 3], [3, 3]], 0, 2, 1),
+    ((33, 17, 17), [[4, 1], [3, 3], [3, 3]], 0, 2, 1),
+    ((49, 17, 25), [[0, 0], [3, 3], [1, 1]], 0, 0, 1),
 ]
 
 
@@ -634,7 +642,9 @@ ROW_CASES = [
                                                  "c2c-5x16-5x32", "rc-3x8", "rc-5x16", "rc-7x16", "rc-node-3x8",
                                                  "r2r-eo-3x8", "r2r-ee-3x16", "r2r-oo-5x8", "r2r-oe-7x8", "r2r-eu-3x16",
                                                  "col-ee-3x8", "col-oe-5x16", "col-uo-7x8", "real-3x16", "real-unb-5x16",
-                                                 "real-odd-3x32", "c2c-7x8", "c2c-unb-7x32"])
+                                                 "real-odd-3x32", "c2c-7x8", "c2c-unb-7x32", "node-ee-32",
+                                                 "node-oo-col-32", "node-oo-ee-64", "node-eu-64", "node-uo-64",
+                                                 "node-ee-3x16-oo-3x8"])
 @pytest.mark.parametrize("misalign", [False, True])
 def test_fast_pass_kernels_vs_generic_and_oracle(case, misalign, monkeypatch):
     """the fast kernels of the general path (real-to-real pencils and tiles, paired real columns,
