@@ -16,7 +16,9 @@
 //                     packed FFT_H of u[2m] + i u[2m+1], y[k] = 2 Re(e^{-i pi k/(2N)} U[k]);
 //   REDFT01 (DCT-III): U[k] = e^{i pi k/(2N)} (x[k] - i x[N-k]) (x[N] = 0), u = N IFFT_N(U)
 //                     through the packed inverse FFT_H, y[2m] = u[m], y[2m+1] = u[N-1-m];
-//   RODFT10 = reversed REDFT10 of (-1)^n x, RODFT01/11 = (-1)^k REDFT01/11 of reversed x.
+//   RODFT10 = reversed REDFT10 of (-1)^n x, RODFT01/11 = (-1)^k REDFT01/11 of reversed x;
+//   REDFT00 (DCT-I, n = H + 1) and RODFT00 (DST-I, n = H - 1): the even / odd extension of
+//                     2H points through the packed real FFT of length H, y = Re X[k] resp. -Im X[k+1].
 // ROWS pencils per block, H/8 threads per pencil; the pencil is staged through
 // LDS so that HBM is read and written in 16-byte pieces when rows are aligned.
 // col_r2r runs the same transforms along a strided direction: R pencils that are
