@@ -544,7 +544,7 @@ class Solver:
     @property
     def pass_kernels(self):
         """kernel of each general-path pass (forward passes, then backward): 0 generic, 1 contiguous r2r,
-        2 paired real columns, 3 contiguous r2c / c2r, 4 strided r2r, 5 strided c2c, -1 none, -2 fused with the multiply"""
+        2 paired real columns, 3 contiguous r2c / c2r, 4 strided r2r, 5 strided c2c, 6 contiguous c2c, -1 none, -2 fused with the multiply"""
         buf = (c_int * 8)()
         n = lib().flups_get_pass_kernels(self._solver, buf, 8)
         return list(buf[:max(n, 0)])

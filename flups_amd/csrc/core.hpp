@@ -103,7 +103,7 @@ struct PassDesc {
     int na = 1;             // extent along fa
     int alias = 0;          // 1: the second FFT buffer reuses the pencil buffer
     int fastk = 0;          // fast kernels of rowpass.hip instead of the generic one: 1 row r2r, 2 paired real columns,
-                            // 3 row r2c / c2r, 4 column r2r, 5 column c2c
+                            // 3 row r2c / c2r, 4 column r2r, 5 column c2c, 6 row c2c
     int vec = 0;            // fast kernels: pencils start 16-byte aligned in source and destination
     const double* aux = nullptr;   // fast kernels: pre / post twiddle table
 };

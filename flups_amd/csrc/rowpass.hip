@@ -44,6 +44,9 @@
 // periodic directions in the general path). col_c2c_team: the same for M = 5 or 7 times a
 // power of two, the columns staged in LDS first.
 //
+// row_c2c: c2c of length M (in [16, 1024]) along a contiguous pencil (x after an r2c
+// direction), staged in LDS, PF teams of M/(8 PF) threads.
+//
 // row_rc: r2c / c2r of even length n = 2H (H in [16, 1024]) along a
 // contiguous pencil through the packed complex FFT of length H: z[m] = x[2m] + i x[2m+1],
 // X[k] = (Z[k] + conj Z[H-k])/2 + e^{-2 pi i k/n} (Z[k] - conj Z[H-k])/(2i), k <= H; the
@@ -484,6 +487,44 @@ __global__ void __launch_bounds__(TX * Q / 8) col_c2c(PassDesc P) {
     }
 }
 
+// Contiguous complex pencils of M = PF Q points (a c2c direction along x, after an r2c one),
+// ROWS pencils per block, M/8 threads per pencil in PF teams (fft_pq_team); each pencil is
+// staged in LDS so that HBM is read and written in whole rows, with the zero padding of the
+// gather and the truncation of the scatter.
+template <int PF, int Q, int ROWS, bool FWD>
+__global__ void __launch_bounds__(ROWS * PF * Q / 8) row_c2c(PassDesc P) {
+    constexpr int M = PF * Q, T = Q / 8, TT = M / 8, HP = M + M / 8;
+    __shared__ C2<double> lds[HP * ROWS];
+    const int rl = threadIdx.x / TT, tt = threadIdx.x % TT;
+    const int k1 = tt / T, j = tt % T;
+    const int g = blockIdx.x * ROWS + rl;
+    const bool live = g < P.n1 * P.n2;
+    const int gg = live ? g : 0;   // dead rows read pencil 0 and store nothing
+    const int ja = gg % P.na, jb = gg / P.na;
+    const C2<double>* src =
+        reinterpret_cast<const C2<double>*>(P.src) + (long long)ja * P.src_s[P.fa] + (long long)jb * P.src_s[P.fb];
+    C2<double>* dst = reinterpret_cast<C2<double>*>(P.dst) + (long long)ja * P.dst_s[P.fa] + (long long)jb * P.dst_s[P.fb];
+    C2<double>* row = lds + rl * HP;
+    for (int k = tt; k < M; k += TT) {
+        const int ks = k - P.gshift;
+        row[k] = (ks >= 0 && ks < P.src_nd) ? src[ks] : C2<double>{0.0, 0.0};
+    }
+    __syncthreads();
+    auto load = [&](int m) -> C2<double> { return row[m]; };
+    C2<double> v[8];
+    fft_pq_team<double, PF, Q, FWD, 0>(v, row, k1, j, load, reinterpret_cast<const C2<double>*>(P.tw),
+                                        reinterpret_cast<const C2<double>*>(P.aux), 1);
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < 8; ++i) row[k1 + PF * (j + i * T)] = v[i];
+    __syncthreads();
+    if (!live) return;
+    for (int k = tt; k < P.dst_nd; k += TT) {
+        const int p = k + P.sshift;
+        if (p < M) dst[k] = row[p];
+    }
+}
+
 // Strided complex pencils of M = PF Q points through thread teams (fft_pq_team): the TX
 // columns are staged in LDS (rows read as contiguous runs of TX values), M/8 threads per
 // column form the PF twiddled inputs and run the Q-point FFTs. Same plan semantics as col_c2c.
@@ -618,6 +659,25 @@ void launch_col_c2c_t(const PassDesc& P, hipStream_t s) {
     dim3 block(TX * PF * Q / 8);
     if (P.sign == FWD) hipLaunchKernelGGL((col_c2c_team<PF, Q, TX, true>), grid, block, 0, s, P);
     else hipLaunchKernelGGL((col_c2c_team<PF, Q, TX, false>), grid, block, 0, s, P);
+}
+template <int PF, int Q>
+void launch_row_c2c_h(const PassDesc& P, hipStream_t s) {
+    constexpr int ROWS = 2048 / (PF * Q) > 0 ? 2048 / (PF * Q) : 1;   // about 256 threads, 36 KB of LDS
+    dim3 grid((P.n1 * P.n2 + ROWS - 1) / ROWS);
+    dim3 block(ROWS * PF * Q / 8);
+    if (P.sign == FWD) hipLaunchKernelGGL((row_c2c<PF, Q, ROWS, true>), grid, block, 0, s, P);
+    else hipLaunchKernelGGL((row_c2c<PF, Q, ROWS, false>), grid, block, 0, s, P);
+}
+template <int PF>
+void launch_row_c2c_pf(int q, const PassDesc& P, hipStream_t s) {
+    switch (q) {
+        case 8: launch_row_c2c_h<PF, 8>(P, s); break;
+        case 16: launch_row_c2c_h<PF, 16>(P, s); break;
+        case 32: launch_row_c2c_h<PF, 32>(P, s); break;
+        case 64: launch_row_c2c_h<PF, 64>(P, s); break;
+        case 128: launch_row_c2c_h<PF, 128>(P, s); break;
+        default: fail("row c2c: length %d", P.n);
+    }
 }
 // thread teams: TX M <= 4096 (64 KB of LDS), TX >= 4
 template <int PF>
@@ -845,6 +905,38 @@ void launch_col_c2c(const PassDesc& P, hipStream_t s) {
         case 1024: launch_col_c2c_m<1, 1024, 4>(P, s); break;
         case 2048: launch_col_c2c_m<1, 2048, 2>(P, s); break;
         default: fail("column c2c: length %d", P.n);
+    }
+    HIP_CHECK(hipGetLastError());
+}
+
+bool row_c2c_fits(const PassDesc& P) {
+    const int n = P.n, d = P.d;
+    if (P.op != OP_C2C || !fft_len_ok(n, 16, 1024, 7) || P.M != n) return false;
+    if (P.src_s[d] != 1 || P.dst_s[d] != 1) return false;
+    if (!(P.nf_in == 2 && P.nf_out == 2 && P.src_nf == 2 && P.dst_nf == 2 && P.pen_in == n && P.pen_out == n)) return false;
+    return P.si == 0 && P.so == 0 && P.post == 0 && P.gshift >= 0 && P.sshift >= 0 && P.src_nd >= 1 && P.dst_nd >= 1;
+}
+
+void launch_row_c2c(const PassDesc& P, hipStream_t s) {
+    int pf = 1, q = 0;
+    split_pq(P.n, pf, q);
+    if (pf == 1) {
+        switch (q) {
+            case 16: launch_row_c2c_h<1, 16>(P, s); break;
+            case 32: launch_row_c2c_h<1, 32>(P, s); break;
+            case 64: launch_row_c2c_h<1, 64>(P, s); break;
+            case 128: launch_row_c2c_h<1, 128>(P, s); break;
+            case 256: launch_row_c2c_h<1, 256>(P, s); break;
+            case 512: launch_row_c2c_h<1, 512>(P, s); break;
+            case 1024: launch_row_c2c_h<1, 1024>(P, s); break;
+            default: fail("row c2c: length %d", P.n);
+        }
+    } else if (pf == 3) {
+        launch_row_c2c_pf<3>(q, P, s);
+    } else if (pf == 5) {
+        launch_row_c2c_pf<5>(q, P, s);
+    } else {
+        launch_row_c2c_pf<7>(q, P, s);
     }
     HIP_CHECK(hipGetLastError());
 }

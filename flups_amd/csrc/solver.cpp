@@ -33,6 +33,8 @@ bool col_r2r_fits(const PassDesc& P);
 bool col_c2c_fits(const PassDesc& P);
 bool split_pq(int n, int& pf, int& q);
 int r2r_h(int kind, int n);
+bool row_c2c_fits(const PassDesc& P);
+void launch_row_c2c(const PassDesc& P, hipStream_t s);
 void launch_col_c2c(const PassDesc& P, hipStream_t s);
 void launch_col_r2r(const PassDesc& P, hipStream_t s);
 FastPlan* make_fast_plan(SolverImpl& s);   // nullptr if the fused pipeline does not apply
@@ -350,6 +352,15 @@ PassDesc SolverImpl::pass_desc(const DirPlan& p, int comp, bool forward, const V
         P.aux = r2r_table(P.kind, P.n);
         return P;
     }
+    // contiguous complex pencils (x after an r2c direction): staged rows, thread teams
+    if (!std::getenv("FLUPS_GENERIC_PASS") && row_c2c_fits(P)) {
+        P.fastk = 6;
+        int pf = 1, q = P.n;
+        split_pq(P.n, pf, q);
+        P.tw = twiddles(q);
+        P.aux = twiddles(P.n);
+        return P;
+    }
     // strided complex pencils: tiles of neighbouring pencils through the register FFT
     // (lengths 3, 5, 7 times a power of two: Q-point column FFTs after the small DFTs)
     if (!std::getenv("FLUPS_GENERIC_PASS") && col_c2c_fits(P)) {
@@ -403,6 +414,7 @@ void SolverImpl::run_pass(PassDesc P) {
     else if (P.fastk == 3) launch_row_rc(P, stream_);
     else if (P.fastk == 4) launch_col_r2r(P, stream_);
     else if (P.fastk == 5) launch_col_c2c(P, stream_);
+    else if (P.fastk == 6) launch_row_c2c(P, stream_);
     else launch_generic(P, stream_);
 }
 

@@ -166,7 +166,7 @@ int         flups_get_path(FLUPS_Solver* s);
 /* kernel of each pass of the general path after flups_setup (component 0; the forward
    passes in order, then the backward ones): 0 generic pencil kernel, 1 contiguous
    real-to-real, 2 paired real columns (r2c / c2r), 3 contiguous r2c / c2r, 4 strided
-   real-to-real, 5 strided c2c, -1 no transform, -2 fused with the multiply (the last
+   real-to-real, 5 strided c2c, 6 contiguous c2c, -1 no transform, -2 fused with the multiply (the last
    direction of flups_get_path 3). Fills at most max entries, returns the
    number of passes (0 when the fused pipeline runs everything, -1 for a NULL solver) */
 int         flups_get_pass_kernels(FLUPS_Solver* s, int* kinds, int max);

@@ -631,6 +631,11 @@ ROW_CASES = [
     ((33, 17, 17), [[0, 4], [3, 3], [3, 3]], 0, 2, 1),
     ((33, 17, 17), [[4, 1], [3, 3], [3, 3]], 0, 2, 1),
     ((49, 17, 25), [[0, 0], [3, 3], [1, 1]], 0, 0, 1),
+    # c2c along x after an r2c along y or z (contiguous complex pencils, zero padding / truncation)
+    ((64, 32, 16), [[4, 4], [3, 3], [0, 0]], 1, 2, 6),
+    ((48, 32, 32), [[4, 4], [3, 3], [4, 4]], 1, 2, 6),
+    ((40, 64, 16), [[4, 4], [4, 4], [3, 3]], 1, 0, 6),
+    ((56, 64, 1), [[4, 4], [3, 3], [9, 9]], 1, 2, 6),
 ]
 
 
@@ -644,7 +649,8 @@ ROW_CASES = [
                                                  "col-ee-3x8", "col-oe-5x16", "col-uo-7x8", "real-3x16", "real-unb-5x16",
                                                  "real-odd-3x32", "c2c-7x8", "c2c-unb-7x32", "node-ee-32",
                                                  "node-oo-col-32", "node-oo-ee-64", "node-eu-64", "node-uo-64",
-                                                 "node-ee-3x16-oo-3x8"])
+                                                 "node-ee-3x16-oo-3x8", "rowc2c-128", "rowc2c-3x32", "rowc2c-5x16",
+                                                 "rowc2c-2d-7x16"])
 @pytest.mark.parametrize("misalign", [False, True])
 def test_fast_pass_kernels_vs_generic_and_oracle(case, misalign, monkeypatch):
     """the fast kernels of the general path (real-to-real pencils and tiles, paired real columns,
