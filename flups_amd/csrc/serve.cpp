@@ -19,6 +19,7 @@
 #include <chrono>
 #include <condition_variable>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <map>
@@ -386,10 +387,20 @@ class Batch {
         HIP_CHECK(hipEventCreateWithFlags(&fork_, hipEventDisableTiming));
         HIP_CHECK(hipEventCreate(&t0_));
         HIP_CHECK(hipEventCreate(&t1_));
-        // one graph: fork from the main stream, one branch per problem, join
+        // one graph: fork from the main stream, one branch per problem, join. Branches are
+        // recorded largest problem first (algorithmic bytes; FLUPS_BATCH_ORDER=0 keeps the
+        // given order), so that the small ones fill in behind the large ones.
+        std::vector<int> order(items_.size());
+        for (size_t i = 0; i < order.size(); ++i) order[i] = (int)i;
+        const char* bo = std::getenv("FLUPS_BATCH_ORDER");
+        if (!bo || std::atoi(bo) != 0)
+            std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+                return items_[a].solver->algorithmic_bytes(items_[a].type) > items_[b].solver->algorithmic_bytes(items_[b].type);
+            });
         HIP_CHECK(hipStreamBeginCapture(main_, hipStreamCaptureModeThreadLocal));
         HIP_CHECK(hipEventRecord(fork_, main_));
-        for (auto& it : items_) {
+        for (int oi : order) {
+            auto& it = items_[oi];
             HIP_CHECK(hipStreamWaitEvent(it.stream, fork_, 0));
             it.solver->enqueue(it.field, it.rhs, it.type);
             HIP_CHECK(hipEventRecord(it.done, it.stream));
