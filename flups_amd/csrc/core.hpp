@@ -108,4 +108,16 @@ struct PassDesc {
     const double* aux = nullptr;   // fast kernels: pre / post twiddle table
 };
 
+// the last direction of the general path as one r2c -> normfact G -> c2r launch (col_rz in
+// rowpass.hip): real columns along the last dimension, pairs of neighbours along x
+struct RzArgs {
+    double* d = nullptr;         // field before the transform, in place: (nx, ny, nin) reals, x fastest
+    const double* g = nullptr;   // Green's function on the spectral layout (nx, ny, M/2 + 1) complex
+    const double* tw = nullptr;  // M-point twiddle table
+    int nx = 1, ny = 1, nin = 1, M = 1, lda = 1;
+    long long s_col = 1;         // reals between rows of a column (nx ny)
+    long long comp = 0;          // reals between components
+    double normfact = 1;
+};
+
 }  // namespace flups

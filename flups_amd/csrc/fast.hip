@@ -963,8 +963,13 @@ size_t fast_device_bytes(const FastPlan* f) { return f->dev_bytes; }
 // half-spectrum (r2c) or real-to-real direction: rows are not paired then.
 // 2-D problems fuse their last direction (y) the same way, as a 3-D field with
 // one row (columns up to 2048 long, two columns per block).
+void launch_col_rz(const RzArgs& A, hipStream_t s);   // rowpass.hip
+
 struct HybridZ {
     int M = 0, nx = 0, ny = 0, nz = 0, lda = 1, gtx = 8;
+    bool real = false;       // r2c last direction (col_rz): the general path's Green's function, real columns
+    double normfact = 1;
+    const double* g = nullptr;   // real: the solver's spectral Green's function (not owned)
     int pf = 1;              // M = pf 2^k: 1, or 3 (mixed-radix column FFT, 8 pf values per thread)
     bool pair = false, twod = false;
     void* green = nullptr;   // normfact * Re G, [y'][tile][kz'][gtx]
@@ -980,13 +985,41 @@ HybridZ* make_hybrid_z(SolverImpl& s) {
     if (getenv("FLUPS_FORCE_GENERIC")) return nullptr;
     const int ip = nd - 1;
     const DirPlan &f = s.fwd_[ip], &b = s.bwd_[ip];
-    if (f.dim != nd - 1 || !(f.type == UNBUNB || (f.type == PERPER && !s.node_)) || f.r2c) return nullptr;
+    if (f.dim != nd - 1 || !(f.type == UNBUNB || (f.type == PERPER && !s.node_))) return nullptr;
     for (const DirPlan* p : {&f, &b})
         for (int c = 0; c < p->lda; ++c)
             if (p->post[c] != 0 || p->start_in[c] != 0 || p->start_out[c] != 0 || p->fieldstart != 0 || p->koffset != 0.0)
                 return nullptr;
     const Shape& in = s.stage_[nd - 2];   // field before the last transform
     const Shape& sp = s.stage_[nd - 1];   // spectral field (and Green's function)
+    if (f.r2c) {
+        // every other direction real-to-real: the field before the last transform is real and
+        // the Green's function keeps the general path's spectral layout
+        const int M = f.n_in[0];
+        if (in.nf != 1 || sp.nf != 2 || in.n[0] != sp.n[0] || (nd == 3 && in.n[1] != sp.n[1])) return nullptr;
+        if (!pow2_in(M, 32, 1024) || sp.n[nd - 1] != M / 2 + 1 || in.n[nd - 1] > M || !s.dgreen_) return nullptr;
+        auto* h = new HybridZ;
+        h->real = true;
+        h->M = M;
+        h->nx = in.n[0];
+        h->ny = nd == 3 ? in.n[1] : 1;
+        h->nz = in.n[nd - 1];
+        h->lda = s.lda_;
+        h->normfact = s.normfact_;
+        h->g = s.dgreen_;
+        const auto t = twiddle_table(M);
+        try {
+            HIP_CHECK(hipMalloc(&h->tw, t.size() * sizeof(double)));
+            HIP_CHECK(hipMemcpy(h->tw, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice));
+        } catch (...) {
+            delete h;
+            throw;
+        }
+        h->dev_bytes = t.size() * sizeof(double);
+        // the field in and out, the Green's function's (M/2 + 1) rows of complex values
+        h->bytes = h->lda * 2.0 * (double)h->nx * h->ny * h->nz * 8.0 + (double)h->nx * h->ny * (M / 2 + 1) * 16.0;
+        return h;
+    }
     if (in.nf != 2 || sp.nf != 2 || in.n[0] != sp.n[0] || (nd == 3 && in.n[1] != sp.n[1])) return nullptr;
     const int M = f.n_in[0];
     const bool p2 = pow2_in(M, 32, nd == 2 ? 2048 : 1024), p3 = M % 3 == 0 && pow2_in(M / 3, 8, 128);
@@ -1068,8 +1101,25 @@ void dispatch_z_pf3(int M, const ColArgs& a, int comps, hipStream_t st) {
     }
 }
 
-// in place on the stage array before the z transform: (nx, ny, nz) complex, lda components
+// in place on the stage array before the z transform: (nx, ny, nz) complex (real for an r2c
+// last direction), lda components
 void hybrid_z(const HybridZ* h, double* data, hipStream_t st) {
+    if (h->real) {
+        RzArgs A;
+        A.d = data;
+        A.g = h->g;
+        A.tw = static_cast<const double*>(h->tw);
+        A.nx = h->nx;
+        A.ny = h->ny;
+        A.nin = h->nz;
+        A.M = h->M;
+        A.lda = h->lda;
+        A.s_col = (long long)h->nx * h->ny;
+        A.comp = (long long)h->nx * h->ny * h->nz;
+        A.normfact = h->normfact;
+        launch_col_rz(A, st);
+        return;
+    }
     ColArgs z{};
     z.src = data;
     z.dst = data;

@@ -447,6 +447,61 @@ __global__ void __launch_bounds__(TX * Q / 8) col_real(PassDesc P) {
     }
 }
 
+// The general path's last direction when it is an r2c one (symmetric x and y, unbounded or
+// periodic z): forward r2c, normfact G, inverse c2r in one launch, in place. Two neighbouring
+// real columns a, b are one complex column a + i b (as in col_real): with Z = FFT(a + i b) and
+// Z' the product spectrum, A' = Ga (Z[k] + conj Z[M-k])/2 and B' = Gb (Z[k] - conj Z[M-k])/(2i)
+// recombine to Z'[k] = A' + i B' = (Ga + Gb)/2 Z[k] + (Ga - Gb)/2 conj Z[M-k] (G real and even
+// in kz), and IFFT(Z') = a' + i b'. Rows from nin on are zero (padding) and are not stored.
+template <int M, int TX>
+__global__ void __launch_bounds__(TX * M / 8) col_rz(RzArgs A) {
+    constexpr int T = M / 8;
+    __shared__ C2<double> lds[M * TX];
+    const int c = threadIdx.x % TX, j = threadIdx.x / TX;
+    const int npair = (A.nx + 1) / 2;
+    const int ntile = (npair + TX - 1) / TX;
+    const int tile = blockIdx.x % ntile, y = blockIdx.x / ntile;
+    const int pr = tile * TX + c;
+    const bool la = 2 * pr < A.nx, lb = 2 * pr + 1 < A.nx;
+    const int xa = la ? 2 * pr : 2 * (npair - 1);   // dead pairs read the last pair and store nothing
+    const bool hb = 2 * pr + 1 < A.nx || (!la && xa + 1 < A.nx);   // partner column present (for the loads)
+    double* base = A.d + (long long)blockIdx.y * A.comp + (long long)y * A.nx + xa;
+    const C2<double>* tw = reinterpret_cast<const C2<double>*>(A.tw);
+    C2<double> v[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const int row = j + i * T;
+        const double* p = base + (long long)row * A.s_col;
+        v[i] = row < A.nin ? C2<double>{p[0], hb ? p[1] : 0.0} : C2<double>{0.0, 0.0};
+    }
+    column_fft<double, M, true, TX>(v, lds + c, j, tw);
+    __syncthreads();   // the FFT's last LDS reads are done
+#pragma unroll
+    for (int i = 0; i < 8; ++i) lds[(j + i * T) * TX + c] = v[i];
+    __syncthreads();
+    const double* g = A.g + 2 * ((long long)y * A.nx + xa);
+    const long long gs = 2LL * A.nx * A.ny;   // doubles between kz rows of G
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const int k = j + i * T;
+        const C2<double> zc = cconj(lds[(k == 0 ? 0 : M - k) * TX + c]);
+        const int kz = k <= M / 2 ? k : M - k;
+        const double ga = g[kz * gs], gb = hb ? g[kz * gs + 2] : ga;
+        const double p = 0.5 * (ga + gb) * A.normfact, q = 0.5 * (ga - gb) * A.normfact;
+        v[i] = {p * v[i].x + q * zc.x, p * v[i].y + q * zc.y};
+    }
+    column_fft<double, M, false, TX>(v, lds + c, j, tw);   // syncs before its first LDS write
+    if (!la) return;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const int row = j + i * T;
+        if (row >= A.nin) continue;
+        double* p = base + (long long)row * A.s_col;
+        p[0] = v[i].x;
+        if (lb) p[1] = v[i].y;
+    }
+}
+
 // Strided complex pencils, TX neighbours along the unit-stride dimension fa per block,
 // interleaved in LDS; the gather zero-pads (rows below gshift or from src_nd on read as
 // zero) and the scatter truncates (pencil position p lands in row p - sshift when that
@@ -937,6 +992,25 @@ void launch_row_c2c(const PassDesc& P, hipStream_t s) {
         launch_row_c2c_pf<5>(q, P, s);
     } else {
         launch_row_c2c_pf<7>(q, P, s);
+    }
+    HIP_CHECK(hipGetLastError());
+}
+
+template <int M, int TX>
+void launch_col_rz_m(const RzArgs& A, hipStream_t s) {
+    const int ntile = ((A.nx + 1) / 2 + TX - 1) / TX;
+    hipLaunchKernelGGL((col_rz<M, TX>), dim3(ntile * A.ny, A.lda), dim3(TX * M / 8), 0, s, A);
+}
+// LDS M TX complex values: 32 KB at most; TX pairs of doubles per row segment
+void launch_col_rz(const RzArgs& A, hipStream_t s) {
+    switch (A.M) {
+        case 32: launch_col_rz_m<32, 16>(A, s); break;
+        case 64: launch_col_rz_m<64, 16>(A, s); break;
+        case 128: launch_col_rz_m<128, 8>(A, s); break;
+        case 256: launch_col_rz_m<256, 8>(A, s); break;
+        case 512: launch_col_rz_m<512, 4>(A, s); break;
+        case 1024: launch_col_rz_m<1024, 2>(A, s); break;
+        default: fail("fused r2c column: length %d", A.M);
     }
     HIP_CHECK(hipGetLastError());
 }

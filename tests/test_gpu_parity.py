@@ -478,12 +478,19 @@ HYBRID_CASES = [
     ((48, 40, 48), [[4, 4], [4, 4], [4, 4]], 1, 2, 1),      # all UNB, paired y rows, z of 96
     ((32, 24, 96), [[0, 0], [4, 4], [4, 4]], 1, 4, 3),      # lda 3, z of 192
     ((64, 48, 1), [[4, 4], [4, 4], [9, 9]], 1, 2, 1),       # 2-D, y of 96
+    # r2c last direction after real-to-real ones: r2c, G and c2r of paired real columns in one launch
+    ((64, 32, 32), [[0, 1], [0, 0], [4, 4]], 1, 8, 1),      # x EO, y EE, z UNB
+    ((33, 48, 64), [[1, 1], [0, 1], [3, 3]], 1, 0, 1),      # odd x count (an unpaired column), z PER
+    ((32, 32, 32), [[0, 0], [1, 1], [4, 4]], 1, 2, 3),      # lda 3
+    ((64, 128, 1), [[0, 0], [4, 4], [9, 9]], 1, 2, 1),      # 2-D, y UNB
+    ((33, 33, 33), [[0, 0], [1, 1], [4, 4]], 0, 2, 1),      # node-centred
 ]
 
 
 @pytest.mark.parametrize("case", HYBRID_CASES, ids=["mixed64", "semiunb-y", "node-lda3", "sym-per-per", "2d-unb-1024",
                                                     "2d-per-unb-2048", "2d-per-lda2", "2d-unb-lgf4", "pf3-per48", "pf3-unb-pairs",
-                                                    "pf3-lda3-192", "pf3-2d-96"])
+                                                    "pf3-lda3-192", "pf3-2d-96", "rz-eo-ee-unb", "rz-odd-per",
+                                                    "rz-lda3", "rz-2d", "rz-node"])
 def test_hybrid_last_direction_vs_general_and_oracle(case, monkeypatch):
     """general passes + the last direction fused with the multiply, against the all-general path and the oracle"""
     nglob, bc, center, kernel, lda = case
