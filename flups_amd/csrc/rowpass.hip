@@ -47,6 +47,9 @@
 // row_c2c: c2c of length M (in [16, 1024]) along a contiguous pencil (x after an r2c
 // direction), staged in LDS, PF teams of M/(8 PF) threads.
 //
+// col_rz: the general path's last direction when it is an r2c one, with the multiply: r2c,
+// normfact G, c2r of paired real columns in one in-place launch.
+//
 // row_rc: r2c / c2r of even length n = 2H (H in [16, 1024]) along a
 // contiguous pencil through the packed complex FFT of length H: z[m] = x[2m] + i x[2m+1],
 // X[k] = (Z[k] + conj Z[H-k])/2 + e^{-2 pi i k/n} (Z[k] - conj Z[H-k])/(2i), k <= H; the
