@@ -108,8 +108,9 @@ struct PassDesc {
     const double* aux = nullptr;   // fast kernels: pre / post twiddle table
 };
 
-// the last direction of the general path as one r2c -> normfact G -> c2r launch (col_rz in
-// rowpass.hip): real columns along the last dimension, pairs of neighbours along x
+// the last direction of the general path as one r2c -> normfact G -> c2r launch (rowpass.hip):
+// col_rz, real columns along the last dimension, pairs of neighbours along x; row_rz, real rows
+// along x (x the last direction); nin real values per column / row, transform length M
 struct RzArgs {
     double* d = nullptr;         // field before the transform, in place: (nx, ny, nin) reals, x fastest
     const double* g = nullptr;   // Green's function on the spectral layout (nx, ny, M/2 + 1) complex
@@ -118,6 +119,8 @@ struct RzArgs {
     long long s_col = 1;         // reals between rows of a column (nx ny)
     long long comp = 0;          // reals between components
     double normfact = 1;
+    const double* twn = nullptr; // row_rz: n-point table (n = 2H); tw is then the (H / pf)-point one
+    int rows = 1;                // row_rz: rows of the field (ny nz)
 };
 
 }  // namespace flups

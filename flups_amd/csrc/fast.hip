@@ -964,10 +964,15 @@ size_t fast_device_bytes(const FastPlan* f) { return f->dev_bytes; }
 // 2-D problems fuse their last direction (y) the same way, as a 3-D field with
 // one row (columns up to 2048 long, two columns per block).
 void launch_col_rz(const RzArgs& A, hipStream_t s);   // rowpass.hip
+void launch_row_rz(const RzArgs& A, hipStream_t s);
+bool row_rz_len_ok(int M);
+bool split_pq(int n, int& pf, int& q);
 
 struct HybridZ {
     int M = 0, nx = 0, ny = 0, nz = 0, lda = 1, gtx = 8;
     bool real = false;       // r2c last direction (col_rz): the general path's Green's function, real columns
+    bool xrow = false;       // r2c last direction along x (row_rz): real rows
+    void* twn = nullptr;     // row_rz: n-point table
     double normfact = 1;
     const double* g = nullptr;   // real: the solver's spectral Green's function (not owned)
     int pf = 1;              // M = pf 2^k: 1, or 3 (mixed-radix column FFT, 8 pf values per thread)
@@ -985,13 +990,48 @@ HybridZ* make_hybrid_z(SolverImpl& s) {
     if (getenv("FLUPS_FORCE_GENERIC")) return nullptr;
     const int ip = nd - 1;
     const DirPlan &f = s.fwd_[ip], &b = s.bwd_[ip];
-    if (f.dim != nd - 1 || !(f.type == UNBUNB || (f.type == PERPER && !s.node_))) return nullptr;
+    if (!(f.type == UNBUNB || (f.type == PERPER && !s.node_))) return nullptr;
+    if (f.dim != nd - 1 && !(f.dim == 0 && f.r2c)) return nullptr;
     for (const DirPlan* p : {&f, &b})
         for (int c = 0; c < p->lda; ++c)
             if (p->post[c] != 0 || p->start_in[c] != 0 || p->start_out[c] != 0 || p->fieldstart != 0 || p->koffset != 0.0)
                 return nullptr;
     const Shape& in = s.stage_[nd - 2];   // field before the last transform
     const Shape& sp = s.stage_[nd - 1];   // spectral field (and Green's function)
+    if (f.r2c && f.dim == 0) {
+        // x last (every other direction real-to-real): rows of the real field, the Green's
+        // function's rows of the general path's spectral layout
+        const int M = f.n_in[0];
+        if (in.nf != 1 || sp.nf != 2 || !row_rz_len_ok(M) || sp.n[0] != M / 2 + 1 || in.n[0] > M || !s.dgreen_) return nullptr;
+        for (int d = 1; d < nd; ++d)
+            if (in.n[d] != sp.n[d]) return nullptr;
+        auto* h = new HybridZ;
+        h->real = true;
+        h->xrow = true;
+        h->M = M;
+        h->nx = in.n[0];
+        h->ny = nd == 3 ? in.n[1] * in.n[2] : in.n[1];   // rows
+        h->nz = 1;
+        h->lda = s.lda_;
+        h->normfact = s.normfact_;
+        h->g = s.dgreen_;
+        int pf = 1, q = 0;
+        split_pq(M / 2, pf, q);
+        const auto t = twiddle_table(q), tn = twiddle_table(M);
+        try {
+            HIP_CHECK(hipMalloc(&h->tw, t.size() * sizeof(double)));
+            HIP_CHECK(hipMemcpy(h->tw, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice));
+            HIP_CHECK(hipMalloc(&h->twn, tn.size() * sizeof(double)));
+            HIP_CHECK(hipMemcpy(h->twn, tn.data(), tn.size() * sizeof(double), hipMemcpyHostToDevice));
+        } catch (...) {
+            (void)hipFree(h->tw);
+            delete h;
+            throw;
+        }
+        h->dev_bytes = (t.size() + tn.size()) * sizeof(double);
+        h->bytes = h->lda * 2.0 * (double)h->nx * h->ny * 8.0 + (double)h->ny * (M / 2 + 1) * 16.0;
+        return h;
+    }
     if (f.r2c) {
         // every other direction real-to-real: the field before the last transform is real and
         // the Green's function keeps the general path's spectral layout
@@ -1077,6 +1117,7 @@ void destroy_hybrid_z(HybridZ* h) {
     (void)hipFree(h->green);
     (void)hipFree(h->tw);
     (void)hipFree(h->twm);
+    (void)hipFree(h->twn);
     delete h;
 }
 size_t hybrid_device_bytes(const HybridZ* h) { return h->dev_bytes; }
@@ -1104,6 +1145,21 @@ void dispatch_z_pf3(int M, const ColArgs& a, int comps, hipStream_t st) {
 // in place on the stage array before the z transform: (nx, ny, nz) complex (real for an r2c
 // last direction), lda components
 void hybrid_z(const HybridZ* h, double* data, hipStream_t st) {
+    if (h->xrow) {
+        RzArgs A;
+        A.d = data;
+        A.g = h->g;
+        A.tw = static_cast<const double*>(h->tw);
+        A.twn = static_cast<const double*>(h->twn);
+        A.nin = h->nx;
+        A.M = h->M;
+        A.lda = h->lda;
+        A.rows = h->ny;
+        A.comp = (long long)h->nx * h->ny;
+        A.normfact = h->normfact;
+        launch_row_rz(A, st);
+        return;
+    }
     if (h->real) {
         RzArgs A;
         A.d = data;

@@ -505,6 +505,76 @@ __global__ void __launch_bounds__(TX * M / 8) col_rz(RzArgs A) {
     }
 }
 
+// The general path's last direction when it is an r2c along x (every other direction real-
+// to-real): r2c of each row (length n = 2H, rows zero-padded from nin), X[k] normfact G[k]
+// (G real, one row of the spectral layout per field row), c2r, the first nin values stored,
+// all in LDS, in place. PF teams of Q/8 threads as row_rc.
+template <int PF, int Q, int ROWS>
+__global__ void __launch_bounds__(ROWS * PF * Q / 8) row_rz(RzArgs A) {
+    constexpr int H = PF * Q, T = Q / 8, TT = H / 8, HP = H + H / 8;
+    __shared__ C2<double> lds[HP * ROWS];
+    const int rl = threadIdx.x / TT, tt = threadIdx.x % TT;
+    const int k1 = tt / T, j = tt % T;
+    const int g = blockIdx.x * ROWS + rl;
+    const bool live = g < A.rows;
+    const int r = live ? g : 0;   // dead rows read row 0 and store nothing
+    double* d = A.d + (long long)blockIdx.y * A.comp + (long long)r * A.nin;
+    const double* gr = A.g + 2LL * r * (H + 1);   // real parts at even offsets
+    C2<double>* row = lds + rl * HP;
+    double* rd = reinterpret_cast<double*>(row);
+    const C2<double>* tw = reinterpret_cast<const C2<double>*>(A.tw);    // Q-point table
+    const C2<double>* twn = reinterpret_cast<const C2<double>*>(A.twn);  // n-point table
+    for (int q = tt; q < 2 * H; q += TT) rd[q] = q < A.nin ? d[q] : 0.0;
+    __syncthreads();
+    C2<double> v[8];
+    auto loadf = [&](int m) -> C2<double> { return {rd[2 * m], rd[2 * m + 1]}; };
+    fft_pq_team<double, PF, Q, true, 0>(v, row, k1, j, loadf, tw, twn, 2);
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < 8; ++i) row[k1 + PF * (j + i * T)] = v[i];
+    __syncthreads();
+    // X[k] = (Z[k] + conj Z[H-k])/2 + e^{-2 pi i k/n} (Z[k] - conj Z[H-k])/(2i), times normfact G[k]
+    C2<double> XH = {0, 0};
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const int k = tt + i * TT;
+        const C2<double> zk = row[k], zc = cconj(row[k == 0 ? 0 : H - k]);
+        const C2<double> e = {0.5 * (zk.x + zc.x), 0.5 * (zk.y + zc.y)};
+        const C2<double> df = {0.5 * (zk.x - zc.x), 0.5 * (zk.y - zc.y)};
+        const C2<double> X = cadd(e, cmul(twn[k], C2<double>{df.y, -df.x}));
+        const double gk = A.normfact * gr[2 * k];
+        v[i] = {X.x * gk, X.y * gk};
+    }
+    if (tt == 0) {
+        const C2<double> z0 = row[0];
+        XH = {(z0.x - z0.y) * A.normfact * gr[2 * H], 0.0};
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < 8; ++i) row[tt + i * TT] = v[i];
+    if (tt == 0) row[H] = XH;
+    __syncthreads();
+    // c2r: Z[m] = (X[m] + conj X[H-m]) + i e^{2 pi i m/n} (X[m] - conj X[H-m]), z = IFFT_H(Z)
+    auto loadb = [&](int m) -> C2<double> {
+        C2<double> a = {rd[2 * m], rd[2 * m + 1]}, b = {rd[2 * (H - m)], rd[2 * (H - m) + 1]};
+        if (m == 0) {   // DC and Nyquist are real
+            a.y = 0.0;
+            b.y = 0.0;
+        }
+        const C2<double> bc = cconj(b);
+        const C2<double> e = cadd(a, bc), f = csub(a, bc);
+        const C2<double> t = cmul(cconj(twn[m]), f);
+        return {e.x - t.y, e.y + t.x};
+    };
+    fft_pq_team<double, PF, Q, false, 0>(v, row, k1, j, loadb, tw, twn, 2);
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < 8; ++i) row[k1 + PF * (j + i * T)] = v[i];   // x[2m] = Re z[m], x[2m+1] = Im z[m]
+    __syncthreads();
+    if (!live) return;
+    for (int q = tt; q < A.nin; q += TT) d[q] = rd[q];
+}
+
 // Strided complex pencils, TX neighbours along the unit-stride dimension fa per block,
 // interleaved in LDS; the gather zero-pads (rows below gshift or from src_nd on read as
 // zero) and the scatter truncates (pencil position p lands in row p - sshift when that
@@ -1014,6 +1084,48 @@ void launch_col_rz(const RzArgs& A, hipStream_t s) {
         case 512: launch_col_rz_m<512, 4>(A, s); break;
         case 1024: launch_col_rz_m<1024, 2>(A, s); break;
         default: fail("fused r2c column: length %d", A.M);
+    }
+    HIP_CHECK(hipGetLastError());
+}
+
+template <int PF, int Q>
+void launch_row_rz_h(const RzArgs& A, hipStream_t s) {
+    constexpr int ROWS = 2048 / (PF * Q) > 0 ? 2048 / (PF * Q) : 1;   // about 256 threads, 36 KB of LDS
+    hipLaunchKernelGGL((row_rz<PF, Q, ROWS>), dim3((A.rows + ROWS - 1) / ROWS, A.lda), dim3(ROWS * PF * Q / 8), 0, s, A);
+}
+template <int PF>
+void launch_row_rz_pf(int q, const RzArgs& A, hipStream_t s) {
+    switch (q) {
+        case 8: launch_row_rz_h<PF, 8>(A, s); break;
+        case 16: launch_row_rz_h<PF, 16>(A, s); break;
+        case 32: launch_row_rz_h<PF, 32>(A, s); break;
+        case 64: launch_row_rz_h<PF, 64>(A, s); break;
+        case 128: launch_row_rz_h<PF, 128>(A, s); break;
+        default: fail("fused r2c row: length %d", A.M);
+    }
+}
+// H = M / 2: a power of two in [16, 1024] or 3, 5, 7 times one in [8, 128]
+bool row_rz_len_ok(int M) { return M % 2 == 0 && fft_len_ok(M / 2, 16, 1024, 7); }
+void launch_row_rz(const RzArgs& A, hipStream_t s) {
+    int pf = 1, q = 0;
+    split_pq(A.M / 2, pf, q);
+    if (pf == 1) {
+        switch (q) {
+            case 16: launch_row_rz_h<1, 16>(A, s); break;
+            case 32: launch_row_rz_h<1, 32>(A, s); break;
+            case 64: launch_row_rz_h<1, 64>(A, s); break;
+            case 128: launch_row_rz_h<1, 128>(A, s); break;
+            case 256: launch_row_rz_h<1, 256>(A, s); break;
+            case 512: launch_row_rz_h<1, 512>(A, s); break;
+            case 1024: launch_row_rz_h<1, 1024>(A, s); break;
+            default: fail("fused r2c row: length %d", A.M);
+        }
+    } else if (pf == 3) {
+        launch_row_rz_pf<3>(q, A, s);
+    } else if (pf == 5) {
+        launch_row_rz_pf<5>(q, A, s);
+    } else {
+        launch_row_rz_pf<7>(q, A, s);
     }
     HIP_CHECK(hipGetLastError());
 }

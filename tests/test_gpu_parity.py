@@ -484,13 +484,19 @@ HYBRID_CASES = [
     ((32, 32, 32), [[0, 0], [1, 1], [4, 4]], 1, 2, 3),      # lda 3
     ((64, 128, 1), [[0, 0], [4, 4], [9, 9]], 1, 2, 1),      # 2-D, y UNB
     ((33, 33, 33), [[0, 0], [1, 1], [4, 4]], 0, 2, 1),      # node-centred
+    # x last (r2c rows after real-to-real y, z): r2c, G and c2r of each row in LDS
+    ((64, 32, 32), [[3, 3], [1, 1], [0, 0]], 1, 2, 1),      # x PER
+    ((48, 32, 16), [[4, 4], [0, 0], [1, 1]], 1, 2, 1),      # x UNB, rows of 96 (3 x 32)
+    ((80, 32, 1), [[3, 3], [0, 1], [9, 9]], 1, 0, 2),       # 2-D, lda 2, rows of 80 (5 x 16)
+    ((33, 17, 17), [[4, 4], [0, 0], [1, 1]], 0, 2, 1),      # node-centred x UNB
 ]
 
 
 @pytest.mark.parametrize("case", HYBRID_CASES, ids=["mixed64", "semiunb-y", "node-lda3", "sym-per-per", "2d-unb-1024",
                                                     "2d-per-unb-2048", "2d-per-lda2", "2d-unb-lgf4", "pf3-per48", "pf3-unb-pairs",
                                                     "pf3-lda3-192", "pf3-2d-96", "rz-eo-ee-unb", "rz-odd-per",
-                                                    "rz-lda3", "rz-2d", "rz-node"])
+                                                    "rz-lda3", "rz-2d", "rz-node", "xrow-per", "xrow-unb-3x32",
+                                                    "xrow-2d-lda2", "xrow-node"])
 def test_hybrid_last_direction_vs_general_and_oracle(case, monkeypatch):
     """general passes + the last direction fused with the multiply, against the all-general path and the oracle"""
     nglob, bc, center, kernel, lda = case
