@@ -48,7 +48,7 @@
 // direction), staged in LDS, PF teams of M/(8 PF) threads.
 //
 // col_rz: the general path's last direction when it is an r2c one, with the multiply: r2c,
-// normfact G, c2r of paired real columns in one in-place launch.
+// normfact G, c2r of paired real columns in one in-place launch; row_rz: the same along x.
 //
 // row_rc: r2c / c2r of even length n = 2H (H in [16, 1024]) along a
 // contiguous pencil through the packed complex FFT of length H: z[m] = x[2m] + i x[2m+1],
