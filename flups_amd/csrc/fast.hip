@@ -1080,6 +1080,9 @@ HybridZ* make_hybrid_z(SolverImpl& s) {
     h->gtx = nd == 2 ? 2 : 4;
     h->twod = nd == 2;
     h->pf = p2 ? 1 : 3;
+    // 3 Q points: Q/8 threads per column, so wider tiles keep the blocks larger (LDS M gtx
+    // complex values, 48 KB at most)
+    if (h->pf == 3) h->gtx = nd == 2 ? (M <= 96 ? 16 : 8) : (M <= 48 ? 32 : M <= 192 ? 16 : 8);
     const int ntx = (h->nx + h->gtx - 1) / h->gtx;
     const int gy = h->pair ? h->ny / 2 + 1 : h->ny, gz = M / 2 + 1;
     const size_t gn = (size_t)gy * ntx * gz * h->gtx;
@@ -1135,9 +1138,18 @@ void dispatch_z_pf3(int M, const ColArgs& a, int comps, hipStream_t st) {
     switch (M) {
         case 24: launch_z_pf3<24, TX>(a, comps, st); break;
         case 48: launch_z_pf3<48, TX>(a, comps, st); break;
-        case 96: launch_z_pf3<96, TX>(a, comps, st); break;
-        case 192: launch_z_pf3<192, TX>(a, comps, st); break;
-        case 384: launch_z_pf3<384, TX>(a, comps, st); break;
+        case 96:
+            if constexpr (TX <= 16) launch_z_pf3<96, TX>(a, comps, st);
+            else fail("column length %d with tiles of %d", M, TX);
+            break;
+        case 192:
+            if constexpr (TX <= 16) launch_z_pf3<192, TX>(a, comps, st);
+            else fail("column length %d with tiles of %d", M, TX);
+            break;
+        case 384:
+            if constexpr (TX <= 8) launch_z_pf3<384, TX>(a, comps, st);
+            else fail("column length %d with tiles of %d", M, TX);
+            break;
         default: fail("unsupported column length %d", M);
     }
 }
@@ -1201,8 +1213,11 @@ void hybrid_z(const HybridZ* h, double* data, hipStream_t st) {
     z.gntx = (h->nx + h->gtx - 1) / h->gtx;
     z.twm = h->twm;
     if (h->pf == 3) {
-        if (h->twod) dispatch_z_pf3<2>(h->M, z, h->lda, st);
-        else dispatch_z_pf3<4>(h->M, z, h->lda, st);
+        switch (h->gtx) {
+            case 32: dispatch_z_pf3<32>(h->M, z, h->lda, st); break;
+            case 16: dispatch_z_pf3<16>(h->M, z, h->lda, st); break;
+            default: dispatch_z_pf3<8>(h->M, z, h->lda, st); break;
+        }
         return;
     }
     if (h->twod) {   // two columns per block (the tile width of the compressed G), XCD-grouped
