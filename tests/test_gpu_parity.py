@@ -3,7 +3,7 @@
 Every case solves on the GPU through the C interface (flups_amd -> libflups.so)
 and compares:
   * the field with the oracle's field on the same input: relative L2 and Linf
-    errors <= 1e-12 (FP64 path, the north-star bar);
+    errors <= 1e-12 (FP64 path);
   * for the reference's validation cases, the error norms against the exact
     solution with samples/validation/data_ref at the reference's own tolerance
     (1e-10 relative, absolute below 1e-14; check_res.py:9,40-47).
