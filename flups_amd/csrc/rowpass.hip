@@ -57,6 +57,7 @@
 // = IFFT_H(Z) (the imaginary parts of X[0] and X[H] ignored).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdlib>
 #include <vector>
@@ -881,9 +882,12 @@ bool r2r_plan_fits(const PassDesc& P) {
           P.pen_in >= n && P.pen_in <= n + 2 && P.gshift >= 0 && P.sshift >= 0))
         return false;
     if (P.si < 0 || P.si > 1 || P.so < 0 || P.so > 1 || P.si + n > P.pen_in || (P.post & POST_PERIOD)) return false;
-    // every output position is a transform value or a point the post-correction zeroes
-    const bool head = P.so == 0 || (P.post & POST_FIRST);
-    const bool tail = P.so + n == P.pen_out || (P.so + n == P.pen_out - 1 && (P.post & POST_LAST));
+    // every stored output position (k + sshift for k < dst_nd) is a transform value or a point
+    // the post-correction zeroes (the backward pass of an EVEN-EVEN direction stores n of its
+    // n + 1 positions: the last one, left over from the forward pass, is not stored)
+    const int last = std::min(P.pen_out, P.dst_nd + P.sshift);
+    const bool head = P.so == 0 || (P.post & POST_FIRST) || P.sshift >= P.so;
+    const bool tail = P.so + n >= last || (P.so + n == P.pen_out - 1 && (P.post & POST_LAST));
     return head && tail;
 }
 }  // namespace

@@ -694,6 +694,9 @@ def test_fast_pass_kernels_vs_generic_and_oracle(case, misalign, monkeypatch):
             assert all(k <= 0 for k in kinds), kinds
         else:
             assert want in kinds, (want, kinds)
+            # a direction's forward and backward passes are both fast or both generic
+            nd = len(kinds) // 2
+            assert all((kinds[i] == 0) == (kinds[nd + i] == 0) for i in range(nd)), kinds
         n = rhs.size
         off = 1 if misalign else 0   # one double in: rows start 8 bytes off a 16-byte boundary
         flat = np.zeros(n + 1)
