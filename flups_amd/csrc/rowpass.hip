@@ -614,6 +614,10 @@ __global__ void __launch_bounds__(TX * Q / 8) col_c2c(PassDesc P) {
         const int k = j + i * T - P.sshift;
         if (k >= 0 && k < P.dst_nd) dst[k * dd] = v[i];
     }
+    if ((P.post & POST_PERIOD) && j == 0) {   // node-centred periodic: X[0] again at position n
+        const int k = M - P.sshift;
+        if (k >= 0 && k < P.dst_nd) dst[k * dd] = v[0];
+    }
 }
 
 // Contiguous complex pencils of M = PF Q points (a c2c direction along x, after an r2c one),
@@ -646,11 +650,12 @@ __global__ void __launch_bounds__(ROWS * PF * Q / 8) row_c2c(PassDesc P) {
     __syncthreads();
 #pragma unroll
     for (int i = 0; i < 8; ++i) row[k1 + PF * (j + i * T)] = v[i];
+    if ((P.post & POST_PERIOD) && k1 == 0 && j == 0) row[M] = v[0];   // node-centred periodic copy
     __syncthreads();
     if (!live) return;
     for (int k = tt; k < P.dst_nd; k += TT) {
         const int p = k + P.sshift;
-        if (p < M) dst[k] = row[p];
+        if (p < P.pen_out) dst[k] = row[p];
     }
 }
 
@@ -687,6 +692,10 @@ __global__ void __launch_bounds__(TX * PF * Q / 8) col_c2c_team(PassDesc P) {
     for (int i = 0; i < 8; ++i) {
         const int k = k1 + PF * (j + i * T) - P.sshift;
         if (k >= 0 && k < P.dst_nd) dst[k * dd] = v[i];
+    }
+    if ((P.post & POST_PERIOD) && k1 == 0 && j == 0) {   // node-centred periodic: X[0] again at position n
+        const int k = M - P.sshift;
+        if (k >= 0 && k < P.dst_nd) dst[k * dd] = v[0];
     }
 }
 
@@ -870,6 +879,17 @@ bool fft_len_ok(int n, int lo, int hi, int maxpf) {
 }  // namespace
 
 namespace {
+// c2c pencils: exactly the transform's n points, or n + 1 with the periodic copy of a node-
+// centred periodic direction (position n gets X[0], forward and backward)
+bool c2c_pencil_ok(const PassDesc& P) {
+    const int n = P.n;
+    if (P.pen_in == n && P.pen_out == n) return P.post == 0;
+    const int q = P.sign == FWD ? P.n_out_plan - 1 : n;
+    return P.pen_in == n + 1 && P.pen_out == n + 1 && P.post == POST_PERIOD && q == n;
+}
+}  // namespace
+
+namespace {
 // the plan semantics the r2r pencil kernels reproduce (kinds, length, offsets, corrections)
 bool r2r_plan_fits(const PassDesc& P) {
     const int k = P.kind;
@@ -1011,8 +1031,8 @@ bool col_c2c_fits(const PassDesc& P) {
     if (P.op != OP_C2C || !fft_len_ok(n, 32, 2048, 7) || P.M != n) return false;
     const int d = P.d, fa = P.fa;
     if (fa == d || P.src_s[fa] != 1 || P.dst_s[fa] != 1 || (P.n1 * P.n2) % P.na != 0) return false;
-    if (!(P.nf_in == 2 && P.nf_out == 2 && P.src_nf == 2 && P.dst_nf == 2 && P.pen_in == n && P.pen_out == n)) return false;
-    return P.si == 0 && P.so == 0 && P.post == 0 && P.gshift >= 0 && P.sshift >= 0 && P.src_nd >= 1 && P.dst_nd >= 1;
+    if (!(P.nf_in == 2 && P.nf_out == 2 && P.src_nf == 2 && P.dst_nf == 2 && c2c_pencil_ok(P))) return false;
+    return P.si == 0 && P.so == 0 && P.gshift >= 0 && P.sshift >= 0 && P.src_nd >= 1 && P.dst_nd >= 1;
 }
 
 // LDS M TX complex values: 64 KB at most
@@ -1045,8 +1065,8 @@ bool row_c2c_fits(const PassDesc& P) {
     const int n = P.n, d = P.d;
     if (P.op != OP_C2C || !fft_len_ok(n, 16, 1024, 7) || P.M != n) return false;
     if (P.src_s[d] != 1 || P.dst_s[d] != 1) return false;
-    if (!(P.nf_in == 2 && P.nf_out == 2 && P.src_nf == 2 && P.dst_nf == 2 && P.pen_in == n && P.pen_out == n)) return false;
-    return P.si == 0 && P.so == 0 && P.post == 0 && P.gshift >= 0 && P.sshift >= 0 && P.src_nd >= 1 && P.dst_nd >= 1;
+    if (!(P.nf_in == 2 && P.nf_out == 2 && P.src_nf == 2 && P.dst_nf == 2 && c2c_pencil_ok(P))) return false;
+    return P.si == 0 && P.so == 0 && P.gshift >= 0 && P.sshift >= 0 && P.src_nd >= 1 && P.dst_nd >= 1;
 }
 
 void launch_row_c2c(const PassDesc& P, hipStream_t s) {

@@ -649,6 +649,10 @@ ROW_CASES = [
     ((48, 32, 32), [[4, 4], [3, 3], [4, 4]], 1, 2, 6),
     ((40, 64, 16), [[4, 4], [4, 4], [3, 3]], 1, 0, 6),
     ((56, 64, 1), [[4, 4], [3, 3], [9, 9]], 1, 2, 6),
+    # node-centred periodic c2c along y or z: the periodic copy (position n = X[0])
+    ((65, 65, 17), [[3, 3], [3, 3], [4, 4]], 0, 2, 5),
+    ((65, 33, 33), [[4, 4], [3, 3], [3, 3]], 0, 2, 5),
+    ((65, 33, 1), [[3, 3], [3, 3], [9, 9]], 0, 0, 5),
 ]
 
 
@@ -663,7 +667,7 @@ ROW_CASES = [
                                                  "real-odd-3x32", "c2c-7x8", "c2c-unb-7x32", "node-ee-32",
                                                  "node-oo-col-32", "node-oo-ee-64", "node-eu-64", "node-uo-64",
                                                  "node-ee-3x16-oo-3x8", "rowc2c-128", "rowc2c-3x32", "rowc2c-5x16",
-                                                 "rowc2c-2d-7x16"])
+                                                 "rowc2c-2d-7x16", "node-per-c2c-y", "node-per-c2c-z", "node-per-2d"])
 @pytest.mark.parametrize("misalign", [False, True])
 def test_fast_pass_kernels_vs_generic_and_oracle(case, misalign, monkeypatch):
     """the fast kernels of the general path (real-to-real pencils and tiles, paired real columns,
