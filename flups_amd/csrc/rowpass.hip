@@ -408,6 +408,11 @@ __global__ void __launch_bounds__(TX * Q / 8) col_real(PassDesc P) {
             double* o = cdst + 2 * (long long)k * sc;
             *reinterpret_cast<double2*>(o) = double2{A.x, A.y};
             if (hasb) *reinterpret_cast<double2*>(o + 2) = double2{df.y, -df.x};   // (zk - zc) / (2i)
+            if ((P.post & POST_PERIOD) && k == 0) {   // node-centred periodic: X[0] again at n/2 + 1
+                double* o2 = cdst + 2 * (long long)(NH + 1) * sc;
+                *reinterpret_cast<double2*>(o2) = double2{A.x, A.y};
+                if (hasb) *reinterpret_cast<double2*>(o2 + 2) = double2{df.y, -df.x};
+            }
         }
     } else {
         // rows 0..n/2 of the two spectra into LDS: A at row k, B at row n/2 + 1 + k
@@ -447,6 +452,11 @@ __global__ void __launch_bounds__(TX * Q / 8) col_real(PassDesc P) {
                 p[0] = v[i].x;
                 if (hasb) p[1] = v[i].y;
             }
+        }
+        if ((P.post & POST_PERIOD) && j == 0 && M < P.dst_nd) {   // node-centred periodic: x[n] = x[0]
+            double* p = rdst + (long long)M * sr;
+            p[0] = v[0].x;
+            if (hasb) p[1] = v[0].y;
         }
     }
 }
@@ -996,13 +1006,18 @@ bool col_real_fits(const PassDesc& P) {
     // the real side may carry the two padding reals of the in-place r2c layout (n + 2 per
     // pencil): only rows below n are data, the rest is neither read nor written here. A
     // zero-padded (unbounded) direction reads src_nd < n rows and stores dst_nd < n back.
-    if (fwd && !(P.nf_in == 1 && P.nf_out == 2 && P.src_nf == 1 && P.dst_nf == 2 && P.pen_in >= n && P.pen_out == n / 2 + 1 &&
-                 P.src_nd >= 1 && P.src_nd <= P.pen_in && P.dst_nd == n / 2 + 1))
+    // a node-centred periodic direction carries one more complex value (forward: X[0] again at
+    // n/2 + 1) or real value (backward: x[n] = x[0])
+    const bool per = P.post == POST_PERIOD;
+    if (P.post != 0 && !(per && (fwd ? P.n_out_plan - 1 == n / 2 + 1 : true))) return false;
+    const int nc = n / 2 + 1 + (per ? 1 : 0);   // complex values of the spectral pencil
+    if (fwd && !(P.nf_in == 1 && P.nf_out == 2 && P.src_nf == 1 && P.dst_nf == 2 && P.pen_in >= n && P.pen_out == nc &&
+                 P.src_nd >= 1 && P.src_nd <= P.pen_in && P.dst_nd == nc))
         return false;
-    if (!fwd && !(P.nf_in == 2 && P.nf_out == 1 && P.src_nf == 2 && P.dst_nf == 1 && P.pen_in == n / 2 + 1 && P.pen_out >= n &&
-                  P.src_nd == n / 2 + 1 && P.dst_nd >= 1 && P.dst_nd <= P.pen_out))
+    if (!fwd && !(P.nf_in == 2 && P.nf_out == 1 && P.src_nf == 2 && P.dst_nf == 1 && P.pen_in == nc && P.pen_out >= n + (per ? 1 : 0) &&
+                  P.src_nd == nc && P.dst_nd >= 1 && P.dst_nd <= P.pen_out))
         return false;
-    return P.si == 0 && P.so == 0 && P.gshift == 0 && P.sshift == 0 && P.post == 0 && (P.n1 * P.n2) % P.na == 0;
+    return P.si == 0 && P.so == 0 && P.gshift == 0 && P.sshift == 0 && (P.n1 * P.n2) % P.na == 0;
 }
 
 void launch_col_real(const PassDesc& P, hipStream_t s) {

@@ -653,6 +653,9 @@ ROW_CASES = [
     ((65, 65, 17), [[3, 3], [3, 3], [4, 4]], 0, 2, 5),
     ((65, 33, 33), [[4, 4], [3, 3], [3, 3]], 0, 2, 5),
     ((65, 33, 1), [[3, 3], [3, 3], [9, 9]], 0, 0, 5),
+    # node-centred periodic r2c along y or z (paired real columns with the periodic copy)
+    ((17, 65, 33), [[0, 0], [3, 3], [3, 3]], 0, 2, 2),
+    ((33, 17, 65), [[1, 1], [4, 4], [3, 3]], 0, 0, 2),
 ]
 
 
@@ -667,7 +670,8 @@ ROW_CASES = [
                                                  "real-odd-3x32", "c2c-7x8", "c2c-unb-7x32", "node-ee-32",
                                                  "node-oo-col-32", "node-oo-ee-64", "node-eu-64", "node-uo-64",
                                                  "node-ee-3x16-oo-3x8", "rowc2c-128", "rowc2c-3x32", "rowc2c-5x16",
-                                                 "rowc2c-2d-7x16", "node-per-c2c-y", "node-per-c2c-z", "node-per-2d"])
+                                                 "rowc2c-2d-7x16", "node-per-c2c-y", "node-per-c2c-z", "node-per-2d",
+                                                 "node-per-real-y", "node-per-real-z"])
 @pytest.mark.parametrize("misalign", [False, True])
 def test_fast_pass_kernels_vs_generic_and_oracle(case, misalign, monkeypatch):
     """the fast kernels of the general path (real-to-real pencils and tiles, paired real columns,
