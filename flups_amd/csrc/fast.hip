@@ -205,6 +205,53 @@ __global__ void __launch_bounds__(TX * M / (8 * PF)) z_fused(ColArgs a) {
     }
 }
 
+// The general path's fused last direction for M = PF Q with PF = 5 or 7: one row per block (no
+// mirror pairing), TX columns staged in LDS, PF thread teams of Q/8 threads (fft_pq_team) for
+// the forward FFT, the multiply by normfact G and the inverse FFT; in place.
+template <int PF, int Q, int TX>
+__global__ void __launch_bounds__(TX * PF * Q / 8) z_team(ColArgs a) {
+    constexpr int M = PF * Q, T = Q / 8, TT = M / 8;
+    __shared__ C2<double> lds[M * TX];
+    const int c = threadIdx.x % TX, tt = threadIdx.x / TX;
+    const int k1 = tt / T, j = tt % T;
+    const int ntx = (a.nx + TX - 1) / TX;
+    const int blk = a.xcd ? xcd_group(blockIdx.x, gridDim.x) : blockIdx.x;
+    const int tile = blk % ntx, y = blk / ntx;
+    const bool live = tile * TX + c < a.nx;
+    const int cl = min(c, a.nx - 1 - tile * TX);   // lanes past nx load the last valid column
+    C2<double>* col = reinterpret_cast<C2<double>*>(a.dst) + (long long)blockIdx.y * a.comp_src +
+                      (long long)y * a.s_other_src + (long long)tile * TX;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const int r = tt + i * TT;
+        lds[r * TX + c] = r < a.nin ? col[(long long)r * a.s_col + cl] : C2<double>{0.0, 0.0};
+    }
+    __syncthreads();
+    const C2<double>* tw = reinterpret_cast<const C2<double>*>(a.tw);
+    const C2<double>* twm = reinterpret_cast<const C2<double>*>(a.twm);
+    auto load = [&](int m) -> C2<double> { return lds[m * TX + c]; };
+    C2<double> v[8];
+    fft_pq_team<double, PF, Q, true, TX>(v, lds + c, k1, j, load, tw, twm, 1);
+    const double* g = reinterpret_cast<const double*>(a.green) + ((long long)y * a.gntx + tile) * (M / 2 + 1) * TX;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const int kz = k1 + PF * (j + i * T);
+        const double gv = g[min(kz, M - kz) * TX + c];
+        v[i] = {v[i].x * gv, v[i].y * gv};
+    }
+    __syncthreads();   // the forward FFT's last LDS reads are done
+#pragma unroll
+    for (int i = 0; i < 8; ++i) lds[(k1 + PF * (j + i * T)) * TX + c] = v[i];
+    __syncthreads();
+    fft_pq_team<double, PF, Q, false, TX>(v, lds + c, k1, j, load, tw, twm, 1);
+    if (!live) return;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const int r = k1 + PF * (j + i * T);
+        if (r < a.nout) col[(long long)r * a.s_col + c] = v[i];
+    }
+}
+
 // Rotational (Biot-Savart) z pass, in place on the three components of S1:
 // forward FFT of omega_x, omega_y, omega_z along z, psi = normfact G omega, then
 // u = curl psi = i K x psi with the derivative symbol K of each direction (the
@@ -1062,8 +1109,9 @@ HybridZ* make_hybrid_z(SolverImpl& s) {
     }
     if (in.nf != 2 || sp.nf != 2 || in.n[0] != sp.n[0] || (nd == 3 && in.n[1] != sp.n[1])) return nullptr;
     const int M = f.n_in[0];
-    const bool p2 = pow2_in(M, 32, nd == 2 ? 2048 : 1024), p3 = M % 3 == 0 && pow2_in(M / 3, 8, 128);
-    if (!(p2 || p3) || sp.n[nd - 1] != M) return nullptr;
+    const bool p2 = pow2_in(M, 32, nd == 2 ? 2048 : 1024), p3 = M % 3 == 0 && pow2_in(M / 3, 8, 128),
+               p5 = M % 5 == 0 && pow2_in(M / 5, 8, 128), p7 = M % 7 == 0 && pow2_in(M / 7, 8, 128);
+    if (!(p2 || p3 || p5 || p7) || sp.n[nd - 1] != M) return nullptr;
     const int ny = nd == 3 ? in.n[1] : 1;
     if ((double)(M / 8) * in.n[0] * ny * 16.0 >= 4294967296.0) return nullptr;
     auto* h = new HybridZ;
@@ -1079,7 +1127,11 @@ HybridZ* make_hybrid_z(SolverImpl& s) {
     // half-width x tiles in 3-D; two columns per block in 2-D (few columns, cache-resident field)
     h->gtx = nd == 2 ? 2 : 4;
     h->twod = nd == 2;
-    h->pf = p2 ? 1 : 3;
+    h->pf = p2 ? 1 : p3 ? 3 : p5 ? 5 : 7;
+    if (h->pf >= 5) {   // thread teams, one row per block: no mirror pairing, tiles by length (LDS <= 48 KB)
+        h->pair = false;
+        h->gtx = M <= 192 ? 16 : M <= 384 ? 8 : 4;
+    }
     // 3 Q points: Q/8 threads per column, so wider tiles keep the blocks larger (LDS M gtx
     // complex values, 48 KB at most)
     if (h->pf == 3) h->gtx = nd == 2 ? (M <= 96 ? 16 : 8) : (M <= 48 ? 32 : M <= 192 ? 16 : 8);
@@ -1154,6 +1206,35 @@ void dispatch_z_pf3(int M, const ColArgs& a, int comps, hipStream_t st) {
     }
 }
 
+template <int PF, int Q>
+void launch_z_team(int tx, const ColArgs& a, int comps, hipStream_t st) {
+    const int ntx = (a.nx + tx - 1) / tx;
+    dim3 grid(ntx * a.nother, comps);
+    switch (tx) {
+        case 16:
+            if constexpr (PF * Q <= 192) hipLaunchKernelGGL((z_team<PF, Q, 16>), grid, dim3(16 * PF * Q / 8), 0, st, a);
+            else fail("column length %d with tiles of 16", PF * Q);
+            break;
+        case 8:
+            if constexpr (PF * Q <= 384) hipLaunchKernelGGL((z_team<PF, Q, 8>), grid, dim3(8 * PF * Q / 8), 0, st, a);
+            else fail("column length %d with tiles of 8", PF * Q);
+            break;
+        default: hipLaunchKernelGGL((z_team<PF, Q, 4>), grid, dim3(4 * PF * Q / 8), 0, st, a); break;
+    }
+    HIP_CHECK(hipGetLastError());
+}
+template <int PF>
+void dispatch_z_team(int M, int tx, const ColArgs& a, int comps, hipStream_t st) {
+    switch (M / PF) {
+        case 8: launch_z_team<PF, 8>(tx, a, comps, st); break;
+        case 16: launch_z_team<PF, 16>(tx, a, comps, st); break;
+        case 32: launch_z_team<PF, 32>(tx, a, comps, st); break;
+        case 64: launch_z_team<PF, 64>(tx, a, comps, st); break;
+        case 128: launch_z_team<PF, 128>(tx, a, comps, st); break;
+        default: fail("unsupported column length %d", M);
+    }
+}
+
 // in place on the stage array before the z transform: (nx, ny, nz) complex (real for an r2c
 // last direction), lda components
 void hybrid_z(const HybridZ* h, double* data, hipStream_t st) {
@@ -1212,6 +1293,11 @@ void hybrid_z(const HybridZ* h, double* data, hipStream_t st) {
     z.x0 = 0;
     z.gntx = (h->nx + h->gtx - 1) / h->gtx;
     z.twm = h->twm;
+    if (h->pf >= 5) {
+        if (h->pf == 5) dispatch_z_team<5>(h->M, h->gtx, z, h->lda, st);
+        else dispatch_z_team<7>(h->M, h->gtx, z, h->lda, st);
+        return;
+    }
     if (h->pf == 3) {
         switch (h->gtx) {
             case 32: dispatch_z_pf3<32>(h->M, z, h->lda, st); break;
