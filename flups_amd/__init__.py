@@ -16,8 +16,9 @@ from ctypes import POINTER, byref, c_bool, c_char_p, c_double, c_int, c_int64, c
 
 import numpy as np
 
-__all__ = ["BoundaryType", "GreenType", "SolverType", "DiffType", "CenterType", "Topology", "Solver", "DeviceArray",
-           "lib", "lib_path", "last_error", "FLUPS_ALIGNMENT", "FLUPS_FORWARD", "FLUPS_BACKWARD"]
+__all__ = ["BoundaryType", "GreenType", "SolverType", "DiffType", "CenterType", "Topology", "Solver", "Profiler",
+           "DeviceArray", "locID", "print_data", "lib", "lib_path", "last_error", "FLUPS_ALIGNMENT", "FLUPS_FORWARD",
+           "FLUPS_BACKWARD", "FLUPS_BACKWARD_DIFF"]
 
 FLUPS_FORWARD = -1
 FLUPS_BACKWARD = 1
@@ -181,6 +182,25 @@ def lib():
         L.flups_batch_last_ms.argtypes = [c_void_p]
         L.flups_batch_device_bytes.restype = c_size_t
         L.flups_batch_device_bytes.argtypes = [c_void_p]
+        L.flups_profiler_new.restype = c_void_p
+        L.flups_profiler_new.argtypes = []
+        L.flups_profiler_new_n.restype = c_void_p
+        L.flups_profiler_new_n.argtypes = [c_char_p]
+        L.flups_profiler_free.argtypes = [c_void_p]
+        L.flups_profiler_disp.argtypes = [c_void_p]
+        L.flups_profiler_get.restype = c_int
+        L.flups_profiler_get.argtypes = [c_void_p, c_int, c_char_p, c_int, POINTER(c_double), POINTER(c_double),
+                                         POINTER(c_long)]
+        L.flups_print_data.argtypes = [c_void_p, c_void_p]
+        L.flups_topo_cmpt_rank_fromid.argtypes = [c_void_p, c_int, c_int]
+        L.flups_topo_cmpt_rank_fromid.restype = c_int
+        L.flups_topo_cmpt_start_id_from_rank.argtypes = [c_void_p, c_int, c_int]
+        L.flups_topo_cmpt_start_id_from_rank.restype = c_int
+        L.flups_topo_get_comm.argtypes = [c_void_p]
+        L.flups_topo_get_comm.restype = c_int64
+        L.flups_topo_ranksplit.argtypes = [c_void_p, c_int, POINTER(c_int)]
+        L.flups_topo_rankindex.argtypes = [c_void_p, POINTER(c_int)]
+        L.flups_topo_rankindex.restype = c_int
         _lib = L
     return _lib
 
@@ -307,9 +327,9 @@ class Topology:
         _check(self._topo, "flups_topo_new")
 
     def __del__(self):
-        if getattr(self, "_topo", None):
+        if getattr(self, "_topo", None) and getattr(self, "_owned", True):
             lib().flups_topo_free(self._topo)
-            self._topo = None
+        self._topo = None
 
     @property
     def is_complex(self):
@@ -348,16 +368,38 @@ class Topology:
     def memsize(self):
         return lib().flups_topo_get_memsize(self._topo)
 
+    def cmpt_rank_from_id(self, global_id, direction):
+        """rank owning global index global_id along direction (always 0 on one GPU)"""
+        return lib().flups_topo_cmpt_rank_fromid(self._topo, int(global_id), int(direction))
+
+    def cmpt_start_id_from_rank(self, rank_id, direction):
+        return lib().flups_topo_cmpt_start_id_from_rank(self._topo, int(rank_id), int(direction))
+
+    def get_comm(self):
+        """the communicator handle given at construction (kept, never used on one GPU)"""
+        return lib().flups_topo_get_comm(self._topo)
+
+    def ranksplit(self, rank):
+        out = (c_int * 3)()
+        lib().flups_topo_ranksplit(self._topo, int(rank), out)
+        return np.array(out[:])
+
+    def rankindex(self, rankd):
+        return lib().flups_topo_rankindex(self._topo, (c_int * 3)(*[int(r) for r in rankd]))
+
     @property
     def _ptr(self):
         return self._topo
 
     @classmethod
-    def _own(cls, handle):
-        """wrap a topology handle returned by the library (freed with the wrapper)"""
+    def _own(cls, handle, solver=None):
+        """wrap a topology handle owned by a solver (the inner topologies): not freed by the wrapper,
+        which keeps the solver alive"""
         _check(handle, "inner topology")
         t = cls.__new__(cls)
         t._topo = handle
+        t._owned = False
+        t._solver_ref = solver
         return t
 
 
@@ -424,6 +466,41 @@ def _data_ptr(a):
     return a.ctypes.data
 
 
+class Profiler:
+    """Timers of the solvers created with it (python_api/pyflups/flups.py:185-218): init, setup and
+    every solve with its stages (HIP events on the solve stream)."""
+
+    def __init__(self, name="default"):
+        L = lib()
+        self._prof = L.flups_profiler_new() if name == "default" else L.flups_profiler_new_n(name.encode("utf-8"))
+        _check(self._prof, "flups_profiler_new")
+
+    def __del__(self):
+        if getattr(self, "_prof", None):
+            lib().flups_profiler_free(self._prof)
+            self._prof = None
+
+    def display(self):
+        """print the timer table (flups_profiler_disp)"""
+        lib().flups_profiler_disp(self._prof)
+
+    def timers(self):
+        """{name: (calls, host seconds, device milliseconds)} of every timer recorded so far"""
+        L = lib()
+        out = {}
+        n = L.flups_profiler_get(self._prof, 0, None, 0, None, None, None)
+        for i in range(max(n, 0)):
+            buf = ctypes.create_string_buffer(256)
+            hs, dm, calls = c_double(), c_double(), c_long()
+            L.flups_profiler_get(self._prof, i, buf, 256, byref(hs), byref(dm), byref(calls))
+            out[buf.value.decode()] = (calls.value, hs.value, dm.value)
+        return out
+
+    @property
+    def _ptr(self):
+        return self._prof
+
+
 class Solver:
     """Poisson solver (python_api/pyflups/flups.py:221-500) on one MI355X."""
 
@@ -442,8 +519,10 @@ class Solver:
         self._bc_c = (POINTER(c_int) * 6)(*[ctypes.cast(a, POINTER(c_int)) for a in self._bc_arrays])
         if center_type is None:
             center_type = [CenterType.CELL_CENTER] * 3
-        self._solver = Lb.flups_init(topology._ptr, self._bc_c, (c_double * 3)(*h), (c_double * 3)(*L), int(order_diff),
-                                     (c_int * 3)(*[int(c) for c in center_type]))
+        self._profiler = profiler   # kept alive as long as the solver records into it
+        self._solver = Lb.flups_init_timed(topology._ptr, self._bc_c, (c_double * 3)(*h), (c_double * 3)(*L),
+                                           int(order_diff), (c_int * 3)(*[int(c) for c in center_type]),
+                                           profiler._ptr if profiler is not None else None)
         _check(self._solver, "flups_init")
 
     def __del__(self):
@@ -508,13 +587,31 @@ class Solver:
         return DeviceView(p, self.get_alloc_size(), self)
 
     def inner_topo_physical(self):
-        return Topology._own(lib().flups_get_innerTopo_physical(self._solver))
+        return Topology._own(lib().flups_get_innerTopo_physical(self._solver), self)
 
     def inner_topo_spectral(self):
-        return Topology._own(lib().flups_get_innerTopo_spectral(self._solver))
+        return Topology._own(lib().flups_get_innerTopo_spectral(self._solver), self)
+
+    # pyflups names (python_api/pyflups/flups.py:439-470): the inner buffer lives in HBM, so it is
+    # returned as a device view (numpy() / upload() copy it) rather than a host numpy array
+    def get_inner_buffer(self):
+        return self.inner_buffer()
+
+    def get_inner_topo_physical(self):
+        return self.inner_topo_physical()
+
+    def get_inner_topo_spectral(self):
+        return self.inner_topo_spectral()
 
     def skip_first_switchtopo(self):
         lib().flups_skip_firstSwitchtopo(self._solver)
+
+    def switchtopo_info(self):
+        lib().flups_switchtopo_info(self._solver)
+
+    @staticmethod
+    def cleanup_backend():
+        lib().flups_cleanup_backend()
 
     def _adv(self, fn, *args):
         L = lib()
@@ -552,3 +649,19 @@ class Solver:
     @property
     def last_solve_ns(self):
         return lib().flups_get_last_solve_ns(self._solver)
+
+
+def locID(axsrc, i0, i1, i2, lia, axtrg, size, nf):
+    """memory index of (i0, i1, i2) given in the axsrc frame, component lia, in an array whose fastest
+    direction is axtrg (src/flups.h:109-119, python_api/pyflups/flups.py:534)"""
+    i = (int(i0), int(i1), int(i2))
+    d0 = (3 + axtrg - axsrc) % 3
+    d1, d2 = (d0 + 1) % 3, (d0 + 2) % 3
+    a0 = axtrg
+    a1, a2 = (a0 + 1) % 3, (a0 + 2) % 3
+    return i[d0] * nf + size[a0] * nf * (i[d1] + size[a1] * (i[d2] + lia * size[a2]))
+
+
+def print_data(topology, data):
+    """print data laid out by the topology (flups_print_data; host or device memory)"""
+    lib().flups_print_data(topology._ptr, _data_ptr(data))

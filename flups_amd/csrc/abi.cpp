@@ -17,11 +17,11 @@ struct Topology {
 };
 struct Solver {
     flups::SolverImpl* impl;
+    Topology* phys = nullptr;   // inner topologies handed out by flups_get_innerTopo_*: owned by
+    Topology* spec = nullptr;   // the solver, as in the reference (freed with it)
 };
 struct Profiler {
-    std::string name;
-    double total_ns = 0;
-    long calls = 0;
+    flups::Timers t;
 };
 
 namespace {
@@ -135,7 +135,7 @@ int flups_topo_rankindex(const FLUPS_Topology* topo, const int rankd[3]) {
 // ---- solver ----
 FLUPS_Solver* flups_init_timed(FLUPS_Topology* t, FLUPS_BoundaryType* bc[3][2], const double h[3], const double L[3],
                                const FLUPS_DiffType orderDiff, const FLUPS_CenterType center_type[3], FLUPS_Profiler* prof) {
-    (void)prof;
+    const auto t0 = std::chrono::steady_clock::now();
     return guard("flups_init", [&]() -> FLUPS_Solver* {
         FLUPS_REQUIRE(t != nullptr, "topology is NULL");
         FLUPS_REQUIRE(t->t.nf == 1, "the physical topology must be real");
@@ -149,6 +149,10 @@ FLUPS_Solver* flups_init_timed(FLUPS_Topology* t, FLUPS_BoundaryType* bc[3][2], 
         const int center[3] = {(int)center_type[0], (int)center_type[1], (int)center_type[2]};
         auto* S = new Solver;
         S->impl = new flups::SolverImpl(&t->t, flat.data(), h, L, (int)orderDiff, center);
+        if (prof) {
+            S->impl->set_profiler(&prof->t);
+            prof->t.add("init", std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(), 0.0);
+        }
         return S;
     });
 }
@@ -159,6 +163,8 @@ FLUPS_Solver* flups_init(FLUPS_Topology* t, FLUPS_BoundaryType* bc[3][2], const 
 void flups_cleanup_solver(FLUPS_Solver* s) {
     if (!s) return;
     guard("flups_cleanup", [&]() { delete s->impl; return 0; });
+    delete s->phys;
+    delete s->spec;
     delete s;
 }
 void flups_cleanup(FLUPS_Solver* s) { flups_cleanup_solver(s); }
@@ -219,20 +225,21 @@ double* flups_get_innerBuffer(FLUPS_Solver* s) {
         return s->impl->inner_buffer();
     });
 }
+// the returned topologies belong to the solver (src/flups.h:517-525): callers do not free them
 FLUPS_Topology* flups_get_innerTopo_physical(FLUPS_Solver* s) {
     return guard("flups_get_innerTopo_physical", [&]() -> FLUPS_Topology* {
         FLUPS_REQUIRE(s, "NULL solver");
-        auto* T = new Topology;
-        T->t = s->impl->inner_topo_physical();
-        return T;
+        if (!s->phys) s->phys = new Topology;
+        s->phys->t = s->impl->inner_topo_physical();
+        return s->phys;
     });
 }
 FLUPS_Topology* flups_get_innerTopo_spectral(FLUPS_Solver* s) {
     return guard("flups_get_innerTopo_spectral", [&]() -> FLUPS_Topology* {
         FLUPS_REQUIRE(s, "NULL solver");
-        auto* T = new Topology;
-        T->t = s->impl->inner_topo_spectral();
-        return T;
+        if (!s->spec) s->spec = new Topology;
+        s->spec->t = s->impl->inner_topo_spectral();
+        return s->spec;
     });
 }
 void flups_skip_firstSwitchtopo(FLUPS_Solver* s) {
@@ -335,12 +342,78 @@ int flups_pointer_is_device(const void* p) {
     return a.type == hipMemoryTypeDevice ? 1 : 0;
 }
 
-// ---- profiler handles ----
-FLUPS_Profiler* flups_profiler_new(void) { return new Profiler{"flups", 0, 0}; }
-FLUPS_Profiler* flups_profiler_new_n(const char name[]) { return new Profiler{name ? name : "flups", 0, 0}; }
+// ---- debug output (src/flups.h:634-638) ----
+// data in the topology's layout (host or device memory): one line per pencil of the fastest
+// direction, components one after the other (src/defines.hpp:204-240)
+void flups_print_data(const FLUPS_Topology* topo, double* data) {
+    guard("flups_print_data", [&]() {
+        FLUPS_REQUIRE(topo && data, "NULL topology or data");
+        const flups::Topo& t = topo->t;
+        std::vector<double> host(t.memsize());
+        HIP_CHECK(hipMemcpy(host.data(), data, host.size() * sizeof(double), hipMemcpyDefault));
+        const int a0 = t.axis, a1 = (a0 + 1) % 3, a2 = (a0 + 2) % 3;
+        std::printf("ax0 = %d -- lda = %d -- nf = %d - nmem = %d %d %d -- end = %d %d %d \n", a0, t.lda, t.nf, t.nmem[0],
+                    t.nmem[1], t.nmem[2], t.nloc[0], t.nloc[1], t.nloc[2]);
+        for (int c = 0; c < t.lda; ++c) {
+            std::printf("lia == %d \n", c);
+            for (int i2 = 0; i2 < t.nloc[a2]; ++i2)
+                for (int i1 = 0; i1 < t.nloc[a1]; ++i1) {
+                    if (i1 == 0) std::printf("\n");
+                    const double* row = host.data() + c * t.memdim() +
+                                        (size_t)t.nf * t.nmem[a0] * ((size_t)i1 + (size_t)t.nmem[a1] * i2);
+                    for (int i0 = 0; i0 < t.nloc[a0]; ++i0) {
+                        if (t.nf == 1) std::printf("%e \t ", row[i0]);
+                        else std::printf("(%e, %e) \t", row[2 * i0], row[2 * i0 + 1]);
+                    }
+                    std::printf("\n");
+                }
+        }
+        std::fflush(stdout);
+        return 0;
+    });
+}
+// HDF5 / XDMF dumps are not part of this library (no HDF5 dependency): the entry point exists
+// so that callers link, and it reports that nothing was written
+void flups_hdf5_dump(const FLUPS_Topology* topo, const char filename[], const double* data) {
+    (void)topo;
+    (void)data;
+    guard("flups_hdf5_dump", [&]() {
+        flups::fail("HDF5 output is not built into this library; %s was not written", filename ? filename : "(null)");
+        return 0;
+    });
+}
+
+// ---- profiler handles (src/flups.cpp:219-237): timers of the solvers made by flups_init_timed ----
+FLUPS_Profiler* flups_profiler_new(void) {
+    auto* p = new Profiler;
+    p->t.name = "default";
+    return p;
+}
+FLUPS_Profiler* flups_profiler_new_n(const char name[]) {
+    auto* p = new Profiler;
+    p->t.name = name ? name : "default";
+    return p;
+}
 void flups_profiler_free(FLUPS_Profiler* p) { delete p; }
 void flups_profiler_disp(FLUPS_Profiler* p) {
-    if (p) std::fprintf(stderr, "[flups] profiler %s\n", p->name.c_str());
+    if (p) p->t.display(stdout);
+}
+int flups_profiler_get(FLUPS_Profiler* p, int idx, char* name, int name_len, double* host_s, double* device_ms,
+                       long* calls) {
+    if (!p) return -1;
+    std::lock_guard<std::mutex> lk(p->t.mu);
+    const int n = (int)p->t.e.size();
+    if (idx >= 0 && idx < n) {
+        const auto& x = p->t.e[idx];
+        if (name && name_len > 0) {
+            std::strncpy(name, x.name.c_str(), name_len - 1);
+            name[name_len - 1] = 0;
+        }
+        if (host_s) *host_s = x.host_s;
+        if (device_ms) *device_ms = x.dev_ms;
+        if (calls) *calls = x.calls;
+    }
+    return n;
 }
 
 }  // extern "C"

@@ -261,8 +261,9 @@ __global__ void fill_2unb(double* g, FillGeom geo, GreenArgs A) {
         } else if (A.type == HEJ_0) {
             // src/green_kernels.hpp:416-428, the same expression at r = 0 and on k = 0, with
             // Ji0c(rho) from the table of the integer squared distances s = |i|^2 (isotropic
-            // grid: rho = pi sqrt(s) < 30 for s <= 91); elsewhere zero_ (0.5 |r|), which no
-            // spectral rewrite replaces for HEJ_0
+            // grid: rho = pi sqrt(s) < 30 for s <= 91); elsewhere zero_ = 0
+            // (src/green_kernels.hpp:430-432), which no spectral rewrite replaces for HEJ_0
+            // (src/Solver.cpp:1089-1101)
             if (r0 || k0) {
                 const double rho = r / A.length;
                 const int s = (A.hfact[0] != 0.0 ? is[0] * is[0] : 0) + (A.hfact[1] != 0.0 ? is[1] * is[1] : 0) +
@@ -270,7 +271,7 @@ __global__ void fill_2unb(double* g, FillGeom geo, GreenArgs A) {
                 v = (rho < 30.0 && s < A.ntable) ? (A.table[s] + log(2.0 * A.length) - kEuler) / (2.0 * kPi)
                                                  : log(r) / (2.0 * kPi);
             } else {
-                v = 0.5 * fabs(r);
+                v = 0.0;
             }
         } else {
             if (r0) {
@@ -279,7 +280,7 @@ __global__ void fill_2unb(double* g, FillGeom geo, GreenArgs A) {
                 const double rho2 = (r / A.length) * (r / A.length);
                 v = (log(r) - hej2_q(A.type, rho2) * exp(-0.5 * rho2) + 0.5 * expint_e1(0.5 * rho2)) / (2.0 * kPi);
             } else {
-                v = 0.5 * fabs(r);   // zero_ (src/green_kernels.hpp:430-434); the spectral rewrite replaces these planes
+                v = 0.0;   // zero_ (src/green_kernels.hpp:430-432); the spectral rewrite replaces these planes
             }
         }
         if (e == 0 && A.type == CHAT_2) v = 0.25 / (2.0 * kPi) * (kPi - 6.0 + 2.0 * log(0.5 * kPi * req * req));

@@ -9,6 +9,7 @@
 #include <dlfcn.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -79,6 +80,47 @@ std::string kernel_dir() {
         return (s == std::string::npos ? std::string(".") : p.substr(0, s)) + "/kernel";
     }
     return "kernel";
+}
+
+void Timers::add(const std::string& n, double host_s, double dev_ms, long calls) {
+    std::lock_guard<std::mutex> lk(mu);
+    for (auto& x : e)
+        if (x.name == n) {
+            x.host_s += host_s;
+            x.dev_ms += dev_ms;
+            x.calls += calls;
+            return;
+        }
+    Entry x;
+    x.name = n;
+    x.host_s = host_s;
+    x.dev_ms = dev_ms;
+    x.calls = calls;
+    e.push_back(x);
+}
+
+// one line per timer, in first-recorded order, indented by nesting depth; the share of each
+// stage is taken against the device time of its parent
+void Timers::display(FILE* f) {
+    std::lock_guard<std::mutex> lk(mu);
+    std::fprintf(f, "===== FLUPS profiler \"%s\" =====\n", name.c_str());
+    std::fprintf(f, "%-44s %8s %12s %12s %12s %8s\n", "timer", "calls", "host [s]", "device [ms]", "mean [ms]", "% parent");
+    for (const auto& x : e) {
+        const size_t cut = x.name.find_last_of('/');
+        const int depth = (int)std::count(x.name.begin(), x.name.end(), '/');
+        double parent = 0;
+        if (cut != std::string::npos)
+            for (const auto& y : e)
+                if (y.name == x.name.substr(0, cut)) parent = y.dev_ms;
+        const std::string label = std::string(2 * depth, ' ') + (cut == std::string::npos ? x.name : x.name.substr(cut + 1));
+        const double mean = x.calls ? (x.dev_ms > 0 ? x.dev_ms : 1e3 * x.host_s) / x.calls : 0.0;
+        if (parent > 0)
+            std::fprintf(f, "%-44s %8ld %12.6f %12.4f %12.5f %8.1f\n", label.c_str(), x.calls, x.host_s, x.dev_ms, mean,
+                         100.0 * x.dev_ms / parent);
+        else
+            std::fprintf(f, "%-44s %8ld %12.6f %12.4f %12.5f %8s\n", label.c_str(), x.calls, x.host_s, x.dev_ms, mean, "-");
+    }
+    std::fflush(f);
 }
 
 View dense_view(double* p, const Shape& sh) {
@@ -213,6 +255,7 @@ SolverImpl::~SolverImpl() {
     if (cap_) (void)hipStreamDestroy(cap_);
     if (ev0_) (void)hipEventDestroy(ev0_);
     if (ev1_) (void)hipEventDestroy(ev1_);
+    for (hipEvent_t e : pev_) (void)hipEventDestroy(e);
 }
 
 size_t SolverImpl::alloc_size() const {
@@ -625,11 +668,17 @@ void SolverImpl::compute_green() {
 
 void SolverImpl::setup() {
     FLUPS_REQUIRE(!setup_done_, "setup was already done");
+    const auto t_setup = std::chrono::steady_clock::now();
     if (const char* e = std::getenv("FLUPS_GRAPH")) use_graph_ = std::atoi(e) != 0;
     HIP_CHECK(hipEventCreate(&ev0_));
     HIP_CHECK(hipEventCreate(&ev1_));
     compute_green();
+    if (prof_) {
+        HIP_CHECK(hipStreamSynchronize(stream_));
+        prof_->add("setup/green", std::chrono::duration<double>(std::chrono::steady_clock::now() - t_setup).count(), 0.0);
+    }
     fast_ = make_fast_plan(*this);
+    if (fast_ && prof_) fast_profile(fast_, true);   // per-kernel events of the fused pipeline
     const char* at_env = std::getenv("FLUPS_AUTOTUNE");
     if (fast_ && (autotune_ || (at_env && std::atoi(at_env) != 0))) {
         // time the candidate schedules on scratch arrays of the caller's layout
@@ -663,6 +712,7 @@ void SolverImpl::setup() {
     }
     HIP_CHECK(hipStreamSynchronize(stream_));
     setup_done_ = true;
+    if (prof_) prof_->add("setup", std::chrono::duration<double>(std::chrono::steady_clock::now() - t_setup).count(), 0.0);
 }
 
 // ---------------------------------------------------------------------------
@@ -724,7 +774,9 @@ void SolverImpl::solve_generic(double* field, const double* rhs, int type) {
             const View dst = dense_view(dstage_[ip], stage_[ip]);
             for (int c = 0; c < lda_; ++c) run_pass(pass_desc(p, c, true, src, dst, hat_[ip], stage_[ip], p.fieldstart, 0));
         }
+        pmark("fftw forward");
         hybrid_z(hybrid_, dstage_[ndim_ - 2], stream_);
+        pmark("fftw + domagic (last direction fused)");
         for (int ip = ndim_ - 2; ip >= 0; --ip) {
             const DirPlan& p = bwd_[ip];
             const View src = dense_view(dstage_[ip], stage_[ip]);
@@ -732,11 +784,39 @@ void SolverImpl::solve_generic(double* field, const double* rhs, int type) {
             for (int c = 0; c < lda_; ++c)
                 run_pass(pass_desc(p, c, false, src, dst, stage_[ip], hat_[ip], 0, fwd_[ip].fieldstart));
         }
+        pmark("fftw backward");
         return;
     }
     forward_passes(user_view(const_cast<double*>(rhs)), dstage_[ndim_ - 1]);
+    pmark("fftw forward");
     multiply(dstage_[ndim_ - 1], type);
+    pmark("domagic");
     backward_passes(dstage_[ndim_ - 1], user_view(field), type != 0);
+    pmark("fftw backward");
+}
+
+// profiler stages of flups_solve: an event on the solve stream closes each stage; pmark is a
+// no-op without a profiler handle (and so under the graph captures of batches and servers)
+void SolverImpl::pmark(const char* label) {
+    if (!prof_) return;
+    if (pn_ == (int)pev_.size()) {
+        hipEvent_t e = nullptr;
+        HIP_CHECK(hipEventCreate(&e));
+        pev_.push_back(e);
+        plab_.emplace_back();
+    }
+    plab_[pn_] = label;
+    HIP_CHECK(hipEventRecord(pev_[pn_], stream_));
+    ++pn_;
+}
+
+void SolverImpl::pflush(const std::string& prefix) {
+    for (int i = 1; i < pn_; ++i) {
+        float ms = 0;
+        HIP_CHECK(hipEventElapsedTime(&ms, pev_[i - 1], pev_[i]));
+        prof_->add(prefix + plab_[i], 0.0, ms);
+    }
+    pn_ = 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -915,6 +995,9 @@ void SolverImpl::solve(double* field, const double* rhs, int type) {
     const size_t n = phys_.memsize();
     double* din = const_cast<double*>(rhs);
     double* dout = field;
+    const auto t0 = std::chrono::steady_clock::now();
+    pn_ = 0;
+    pmark("start");
     if (!fdev || !rdev) {
         if (staging_n_ < 2 * n) {
             if (dstaging_) HIP_CHECK(hipFree(dstaging_));
@@ -927,16 +1010,45 @@ void SolverImpl::solve(double* field, const double* rhs, int type) {
         }
         if (!fdev) dout = dstaging_ + n;
     }
+    if (din != rhs) pmark("copy rhs");
     HIP_CHECK(hipEventRecord(ev0_, stream_));
-    if (use_graph_ && !profiling_) launch_graph(dout, din, type);
+    if (use_graph_ && !profiling_ && !prof_) launch_graph(dout, din, type);
     else enqueue(dout, din, type);
     HIP_CHECK(hipEventRecord(ev1_, stream_));
+    const bool fused = fast_ && (type == 0 || lda_ == 3);
+    if (fused) pmark("fused pipeline (fftw + domagic)");
     if (!fdev) HIP_CHECK(hipMemcpyAsync(field, dout, n * sizeof(double), hipMemcpyDeviceToHost, stream_));
+    if (!fdev) pmark("copy field");
     HIP_CHECK(hipStreamSynchronize(stream_));
     float ms = 0;
     HIP_CHECK(hipEventElapsedTime(&ms, ev0_, ev1_));
     last_ns_ = ms * 1e6;
-    if (fast_ && (type == 0 || lda_ == 3)) fast_collect(fast_);
+    if (prof_) {
+        // the whole solve: device time from the first to the last mark, host wall-clock
+        float all = 0;
+        if (pn_ > 1) HIP_CHECK(hipEventElapsedTime(&all, pev_[0], pev_[pn_ - 1]));
+        prof_->add("solve", std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(), all);
+        pflush("solve/");
+    }
+    if (fused) {
+        double before[5] = {0, 0, 0, 0, 0};
+        const char* names[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+        if (prof_)
+            for (int i = 0; i < 5; ++i) {
+                long c = 0;
+                double b = 0;
+                fast_profile_get(fast_, i, &names[i], &before[i], &c, &b);
+            }
+        fast_collect(fast_);
+        if (prof_)
+            for (int i = 0; i < 5; ++i) {
+                long c = 0;
+                double b = 0, after = 0;
+                const char* nm = nullptr;
+                if (fast_profile_get(fast_, i, &nm, &after, &c, &b) && after > before[i])
+                    prof_->add(std::string("solve/fused pipeline (fftw + domagic)/") + nm, 0.0, after - before[i]);
+            }
+    }
 }
 
 // The solve's kernels are recorded once per (field, rhs, type) into a hipGraph and

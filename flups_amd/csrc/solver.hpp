@@ -2,8 +2,10 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <cstdio>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -21,6 +23,23 @@ struct Topo {
     size_t memsize() const { return memdim() * lda; }
     size_t locsize() const { return (size_t)nloc[0] * nloc[1] * nloc[2] * nf; }
     void cmpt_sizes();
+};
+
+// Named timers of a FLUPS_Profiler handle (the reference times its solve through H3LPR's
+// profiler: src/flups.cpp:110-114, 219-237, src/Solver.cpp:1218, 1286, 1411-1419, 1475).
+// Host wall-clock seconds and device milliseconds (HIP events on the solver's stream) are
+// accumulated per name; names nest with '/' ("solve/fftw fwd x").
+struct Timers {
+    struct Entry {
+        std::string name;
+        double host_s = 0, dev_ms = 0;
+        long calls = 0;
+    };
+    std::string name;
+    std::vector<Entry> e;
+    std::mutex mu;
+    void add(const std::string& n, double host_s, double dev_ms, long calls = 1);
+    void display(FILE* f);
 };
 
 struct FastPlan;   // fused unbounded pipeline, fast.hip
@@ -70,6 +89,11 @@ class SolverImpl {
     void profile(bool on);
     int profile_get(int i, const char** name, double* ms, long* calls, double* bytes) const;
 
+    // FLUPS_Profiler handle given to flups_init_timed (not owned): setup and every flups_solve
+    // record their stages into it
+    void set_profiler(Timers* t) { prof_ = t; }
+    Timers* profiler() const { return prof_; }
+
     // building blocks (also used by the fused pipeline)
     hipStream_t stream() const { return stream_; }
 
@@ -91,6 +115,8 @@ class SolverImpl {
     void warm_tables();
     void launch_graph(double* field, const double* rhs, int type);
     void check_type(int type) const;
+    void pmark(const char* label);          // profiler: event on the stream closing the stage `label`
+    void pflush(const std::string& prefix);  // after a sync: stage times into the profiler
 
     Topo phys_;
     int lda_ = 1, ndim_ = 3, odiff_ = NOD, gtype_ = CHAT_2;
@@ -119,6 +145,10 @@ class SolverImpl {
     FastPlan* fast_ = nullptr;
     HybridZ* hybrid_ = nullptr;
     hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
+    Timers* prof_ = nullptr;
+    std::vector<hipEvent_t> pev_;       // profiler stage events, reused
+    std::vector<std::string> plab_;
+    int pn_ = 0;
     double last_ns_ = 0;
     std::string kdir_;
     friend struct FastPlan;

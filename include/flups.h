@@ -145,11 +145,28 @@ void            flups_do_FFT(FLUPS_Solver* s, double* data, const int sign);    
 void            flups_do_mult(FLUPS_Solver* s, double* data, const FLUPS_SolverType type);      /* src/flups.h:560 */
 void            flups_switchtopo_info(FLUPS_Solver* s);                                          /* src/flups.h:578 */
 
-/* ---- profiler handles (src/flups.h:592-611): accepted, reports solve timings ---- */
-FLUPS_Profiler* flups_profiler_new(void);
-FLUPS_Profiler* flups_profiler_new_n(const char name[]);
-void            flups_profiler_free(FLUPS_Profiler* p);
-void            flups_profiler_disp(FLUPS_Profiler* p);
+/* ---- profiler handles (src/flups.h:592-611). A solver made by flups_init_timed records into
+   its handle: "init", "setup" and "setup/green" (host wall-clock), and for every flups_solve
+   "solve" (host wall-clock and device time) with its stages timed by HIP events on the solve
+   stream: "solve/copy rhs" and "solve/copy field" (host arrays), "solve/fftw forward",
+   "solve/domagic", "solve/fftw backward" (general path), "solve/fftw + domagic (last direction
+   fused)", or "solve/fused pipeline (fftw + domagic)" with one entry per kernel. The
+   pencil switches are part of the transform passes. flups_profiler_disp prints the table to
+   stdout. ---- */
+FLUPS_Profiler* flups_profiler_new(void);                                                         /* src/flups.h:592 */
+FLUPS_Profiler* flups_profiler_new_n(const char name[]);                                          /* src/flups.h:599 */
+void            flups_profiler_free(FLUPS_Profiler* p);                                           /* src/flups.h:605 */
+void            flups_profiler_disp(FLUPS_Profiler* p);                                           /* src/flups.h:611 */
+/* addition: timer idx of the handle (name, host seconds, device milliseconds, calls); returns the
+   number of timers, -1 for a NULL handle */
+int             flups_profiler_get(FLUPS_Profiler* p, int idx, char* name, int name_len, double* host_s,
+                                   double* device_ms, long* calls);
+
+/* ---- debug output (src/flups.h:634-638) ---- */
+/* prints data (host or device memory) laid out by topo, one pencil of the fastest direction per line */
+void flups_print_data(const FLUPS_Topology* topo, double* data);                                 /* src/flups.h:638 */
+/* HDF5 is not a dependency of this library: records an error and writes nothing */
+void flups_hdf5_dump(const FLUPS_Topology* topo, const char filename[], const double* data);     /* src/flups.h:634 */
 
 /* ---- additions of this library (no counterpart in the reference) ---- */
 const char* flups_get_last_error(void);        /* message of the last failed check, "" if none */

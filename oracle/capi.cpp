@@ -139,6 +139,9 @@ double oracle_bessel_k1(double x) { return bessel_k1(x); }
 double oracle_expint_e1(double x) { return expint_e1(x); }
 double oracle_sine_integral(double x) { return sine_integral(x); }
 double oracle_ji0c(double x) { return ji0c(x); }
+double oracle_hej_1unb(int type, double r, double k, double sig, int k0) {
+    return hej_1unb((GreenType)type, r, k, sig, k0 != 0);
+}
 double oracle_mehr_1d(int kind, int n, double k1, double k2) { return mehr_1d(kind, n, k1, k2); }
 
 }  // extern "C"

@@ -48,6 +48,8 @@ double expint_e1(double x);
 double lgf1d(int order, int n, double c);          // 1-D lattice Green's function
 double sine_integral(double x);                     // Si(x)
 double ji0c(double x);                              // int_0^x (1 - J0(u)) / u du, the reference's series
+// HEJ_2..10 with one unbounded direction: G(k, r) (k > 0) or the k = 0 form
+double hej_1unb(GreenType type, double r, double k, double sig, bool k0);
 double mehr_1d(int kind, int n, double k1, double k2);   // 1-D Mehrstellen LGF: 0 4L, 1 6L, 2 4F, 3 6F
 double lgf_symbol(int order, double kh);
 

@@ -148,3 +148,49 @@ def test_init_rejects_unpaired_periodic(lib):
     assert not s
     assert b"boundary" in lib.flups_get_last_error()
     lib.flups_topo_free(t)
+
+
+def test_profiler_handle_without_solver(lib, capfd):
+    """profiler handles (src/flups.h:592-611) exist without a GPU: empty, displayable, freeable"""
+    lib.flups_profiler_new_n.restype = ctypes.c_void_p
+    lib.flups_profiler_new_n.argtypes = [ctypes.c_char_p]
+    lib.flups_profiler_get.restype = ctypes.c_int
+    lib.flups_profiler_get.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_int, ctypes.c_void_p,
+                                       ctypes.c_void_p, ctypes.c_void_p]
+    lib.flups_profiler_disp.argtypes = [ctypes.c_void_p]
+    lib.flups_profiler_free.argtypes = [ctypes.c_void_p]
+    p = lib.flups_profiler_new_n(b"cpu")
+    assert p
+    assert lib.flups_profiler_get(p, 0, None, 0, None, None, None) == 0
+    lib.flups_profiler_disp(p)
+    assert 'FLUPS profiler "cpu"' in capfd.readouterr().out
+    lib.flups_profiler_free(p)
+    assert lib.flups_profiler_get(None, 0, None, 0, None, None, None) == -1
+
+
+def test_c_driver_is_built_against_the_header():
+    """tests/c_driver/solve_c: compiled against include/flups.h and MPICH's <mpi.h>, linked to this
+    libflups.so (a GPU test runs it)"""
+    import subprocess
+    exe = os.path.join(ROOT, "tests", "c_driver", "solve_c")
+    assert os.path.exists(exe), "run __graft_entry__.build()"
+    out = subprocess.run(["ldd", exe], capture_output=True, text=True).stdout
+    assert os.path.realpath(LIB) in {os.path.realpath(l.split("=>")[1].split("(")[0].strip())
+                                     for l in out.splitlines() if "libflups" in l}
+    assert "libmpi" in out
+
+
+def test_python_locid_matches_the_header(tmp_path):
+    """flups_amd.locID equals the header's inline flups_locID (src/flups.h:109-119) for every
+    source / target axis pair"""
+    import subprocess
+    import flups_amd as fl
+    src = tmp_path / "loc.c"
+    src.write_text('#include <stdio.h>\n#include "flups.h"\nint main(void){const int sz[3]={5,7,3};\n'
+                   'for(int as=0;as<3;++as)for(int at=0;at<3;++at)for(int nf=1;nf<3;++nf)\n'
+                   'printf("%zu\\n",flups_locID(as,1,2,2,1,at,sz,nf));return 0;}\n')
+    exe = tmp_path / "loc"
+    subprocess.run(["gcc", "-std=c99", "-I" + os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    got = [int(x) for x in subprocess.run([str(exe)], capture_output=True, text=True).stdout.split()]
+    want = [fl.locID(a, 1, 2, 2, 1, b, [5, 7, 3], nf) for a in range(3) for b in range(3) for nf in (1, 2)]
+    assert got == want

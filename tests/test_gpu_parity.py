@@ -120,9 +120,9 @@ def test_kernel_sweep(center):
     assert matched == len(KERN_BCS) * len(DEVICE_KERNELS) - 1
 
 
-# the reference's tests hold no case of these: HEJ_0 with two unbounded directions (Ji0c on the
-# k = 0 plane, 0.5 |r| elsewhere, no spectral rewrite) and the Mehrstellen kernels with one
-# unbounded direction (1-D lattice Green's functions); oracle parity at 1e-12
+# the reference's tests hold no case of these: HEJ_0 with two unbounded directions (Ji0c at r = 0
+# and on the k = 0 plane, zero_ = 0 elsewhere, no spectral rewrite) and the Mehrstellen kernels
+# with one unbounded direction (1-D lattice Green's functions); oracle parity at 1e-12
 TWO_ONE_CASES = [
     # (nglob, bc, center, kernel)
     ((32, 32, 16), [[4, 4], [4, 4], [3, 3]], 1, 7),
@@ -156,6 +156,57 @@ def test_hej0_two_unbounded_and_mehr_one_unbounded(case):
     ora = orc.solve(list(nglob), bc, 1, h, L, center, kernel, rhs)
     e2, ei = rel_err(f, ora)
     assert e2 <= TOL and ei <= TOL, (e2, ei)
+
+
+def test_hej0_two_unbounded_device_green_planes():
+    """the device's transformed HEJ_0 Green's function with x, y unbounded and z periodic, read back
+    through flups_do_mult on a spectral buffer of ones (data = normfact G): equal to the oracle's
+    G at 1e-12, every kz != 0 plane the constant volfact G(r = 0) = h^2 (log(2 h / pi) - gamma) /
+    (2 pi), as zero_ = 0 off r = 0 and k = 0 implies (src/green_kernels.hpp:430-432)"""
+    import ctypes
+    n, hh = 32, 1.0 / 32
+    nglob, bc, L = [n, n, 16], [[4, 4], [4, 4], [3, 3]], [1.0, 1.0, 0.5]
+    topo = fl.Topology(0, 1, nglob, [1, 1, 1], False)
+    s = fl.Solver(topo, bc, [hh] * 3, L)
+    s.set_green_type(7)
+    s.setup()
+    spec_t = s.inner_topo_spectral()
+    nl = [spec_t.get_nloc(d) for d in range(3)]
+    assert nl == [2 * n, 2 * n, 9] and spec_t.is_complex
+    inner = s.inner_buffer()
+    ones = np.zeros(inner.shape[0])
+    ones[0: 2 * int(np.prod(nl)): 2] = 1.0
+    inner.upload(ones)
+    s.do_mult(inner, fl.SolverType.STD)
+    got = inner.numpy()[: 2 * int(np.prod(nl))].reshape(nl[2], nl[1], nl[0], 2)
+    c_int, dp = ctypes.c_int, ctypes.POINTER(ctypes.c_double)
+    fb = orc.flat_bc(bc, 1)
+    ng = np.asarray(nglob, np.int32)
+    nf_, vf_ = np.zeros(1), np.zeros(1)
+    kf, ko, ss, order = np.zeros(3), np.zeros(3), np.zeros(3), np.zeros(3, np.int32)
+    orc.check(orc.lib().oracle_plan_info(orc._p(ng, c_int), orc._p(fb, c_int), 1, orc._p(np.array([hh] * 3)),
+                                         orc._p(np.array(L)), 1, orc._p(nf_), orc._p(vf_), orc._p(kf), orc._p(ko),
+                                         orc._p(ss), orc._p(order, c_int)))
+    g = np.zeros(2 * int(np.prod(nl)))
+    orc.check(orc.lib().oracle_green(orc._p(ng, c_int), orc._p(fb, c_int), orc._p(np.array([hh] * 3)),
+                                     orc._p(np.array(L)), 1, 7, 2.0, orc.KDIR, orc._p(g)))
+    want = nf_[0] * g.reshape(got.shape)
+    assert np.max(np.abs(got - want)) <= TOL * np.max(np.abs(want))
+    c = nf_[0] * hh * hh * (np.log(2 * hh / np.pi) - np.euler_gamma) / (2 * np.pi)
+    for kz in range(1, 9):
+        assert np.max(np.abs(got[kz, :, :, 0] - c)) <= TOL * abs(c), kz
+        assert np.max(np.abs(got[kz, :, :, 1])) <= TOL * abs(c), kz
+
+
+HEJ_ONE_UNB = [(bc6, k) for bc6 in [(4, 4, 3, 3, 3, 3), (0, 1, 3, 3, 4, 4)] for k in (3, 4, 5, 6)]
+
+
+@pytest.mark.parametrize("case", HEJ_ONE_UNB, ids=["%s-k%d" % ("".join(map(str, c[0])), c[1]) for c in HEJ_ONE_UNB])
+def test_hej_one_unbounded_vs_oracle(case):
+    """HEJ_4..10 with one unbounded direction (green_kernels.hpp:461-566) at N = 32: GPU against the
+    oracle at 1e-12 (HEJ_2 is in test_mixed_bc_lgf)"""
+    bc6, kernel = case
+    run_validation_case(1, bc6, kernel, 32)
 
 
 def test_vtube_rot():

@@ -236,6 +236,116 @@ def test_hej0_two_unbounded_2d_converges():
     assert e16[0] / e32[0] > 3.0 and e16[1] / e32[1] > 2.5
 
 
+def oracle_green(nglob, bc, h, L, center, kernel):
+    """the oracle's transformed Green's function, shaped (n2, n1, n0, nf) (dimension 0 fastest)"""
+    import ctypes
+    c_int = ctypes.c_int
+    ng = np.asarray(nglob, np.int32)
+    fb = orc.flat_bc(bc, 1)
+    no, nf = np.zeros(3, np.int32), np.zeros(1, np.int32)
+    hh, LL = np.asarray(h, float), np.asarray(L, float)
+    orc.check(orc.lib().oracle_spectral_size(orc._p(ng, c_int), orc._p(fb, c_int), 1, orc._p(hh), orc._p(LL), center,
+                                             orc._p(no, c_int), orc._p(nf, c_int)))
+    g = np.zeros(int(np.prod(no)) * int(nf[0]))
+    orc.check(orc.lib().oracle_green(orc._p(ng, c_int), orc._p(fb, c_int), orc._p(hh), orc._p(LL), center, kernel, 2.0,
+                                     orc.KDIR, orc._p(g)))
+    return g.reshape(no[2], no[1], no[0], nf[0])
+
+
+def ji0c_quad(x):
+    """int_0^x (1 - J0(u)) / u du by adaptive quadrature"""
+    import scipy.integrate as si
+    if x == 0.0:
+        return 0.0
+    v, _ = si.quad(lambda u: (1.0 - sp.j0(u)) / u if u > 1e-6 else 0.25 * u, 0.0, x, limit=200, epsabs=0, epsrel=1e-13)
+    return v
+
+
+def test_hej0_two_unbounded_green_planes():
+    """HEJ_0 with x, y unbounded and z periodic (cell 32 x 32 x 16): G is Ji0c's expression at r = 0
+    and on the kz = 0 plane and zero_ = 0 elsewhere, with no spectral rewrite
+    (src/green_kernels.hpp:416-432, src/green_functions.cpp:251-256, src/Solver.cpp:1089-1101).
+    After the two unbounded transforms every kz != 0 plane of G is therefore the constant
+    volfact G(r = 0) = h^2 (log(2 sigma) - gamma) / (2 pi), sigma = h / pi, and the kz = 0 plane is
+    h^2 times the 2-D DFT of the kernel on the doubled grid, evaluated here with Ji0c by
+    quadrature (log(r) / (2 pi) past rho = 30)."""
+    n, hh = 32, 1.0 / 32
+    G = oracle_green((n, n, 16), [[4, 4], [4, 4], [3, 3]], [hh] * 3, [1.0, 1.0, 0.5], 1, 7)
+    assert G.shape == (9, 2 * n, 2 * n, 2)
+    sig = hh / np.pi
+    c = hh * hh * (np.log(2 * sig) - np.euler_gamma) / (2 * np.pi)
+    for kz in range(1, 9):
+        assert np.max(np.abs(G[kz, :, :, 0] - c)) <= 1e-13 * abs(c), kz
+        assert np.max(np.abs(G[kz, :, :, 1])) <= 1e-15, kz
+    M = 2 * n
+    i = np.arange(M)
+    si_ = np.where(i <= n, i, -(M - i))
+    q = si_[:, None] ** 2 + si_[None, :] ** 2
+    vals = {}
+    for s2 in np.unique(q):
+        r = hh * np.sqrt(s2)
+        rho = r / sig
+        vals[s2] = (ji0c_quad(rho) + np.log(2 * sig) - np.euler_gamma) / (2 * np.pi) if rho < 30 else np.log(r) / (2 * np.pi)
+    g = np.vectorize(vals.get)(q)
+    F = np.fft.fft2(g) * hh * hh
+    assert np.max(np.abs(G[0, :, :, 0] - F.real)) <= 1e-10 * np.max(np.abs(F))
+    assert np.max(np.abs(G[0, :, :, 1])) <= 1e-12 * np.max(np.abs(F))
+
+
+def _hej_fourier(K, sig, n):
+    """Fourier-space HEJ_n kernel -P(x) exp(-x) / K^2, x = (K sigma)^2 / 2, P the Taylor
+    polynomial of exp of degree n/2 - 1 (the fully spectral kernels, green_kernels.hpp:681-717)"""
+    from math import factorial
+    x = 0.5 * (K * sig) ** 2
+    return -sum(x ** j / factorial(j) for j in range(n // 2)) * np.exp(-x) / (K * K)
+
+
+@pytest.mark.parametrize("kernel,n", [(2, 2), (3, 4), (4, 6), (5, 8), (6, 10)])
+def test_hej_one_unbounded_is_fourier_kernel(kernel, n):
+    """HEJ_2..10 with one unbounded direction (green_kernels.hpp:461-566) against the definition:
+    G(k, z) = 1/pi int_0^inf g(sqrt(k^2 + q^2)) cos(q z) dq of the Fourier-space kernel g, and the
+    k = 0 form = 1/pi int_0^inf g(q) (cos(q z) - 1) dq, both by quadrature. k sigma up to 2.5:
+    beyond ~5 the closed form's 1 - erf(.) loses digits (the reference's arithmetic, kept)."""
+    import ctypes
+    import scipy.integrate as si
+    L = orc.lib()
+    L.oracle_hej_1unb.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int]
+    L.oracle_hej_1unb.restype = ctypes.c_double
+    sig = 2.0 / 64
+    qmax = 40.0 / sig
+    zs = [0.0, 0.2 * sig, sig, 3 * sig, 10 * sig, 0.5]
+
+    def quad(k, z):
+        f = lambda q: _hej_fourier(np.sqrt(k * k + q * q), sig, n)
+        kw = dict(limit=400, epsabs=0, epsrel=1e-13)
+        v = si.quad(f, 0, qmax, **kw)[0] if z == 0 else si.quad(f, 0, qmax, weight="cos", wvar=z, **kw)[0]
+        return v / np.pi
+
+    for k in [0.3, 3.0, 20.0, 50.0, 80.0]:
+        scale = abs(quad(k, 0.0))
+        for z in zs:
+            got = L.oracle_hej_1unb(kernel, z, k, sig, 0)
+            assert abs(got - quad(k, z)) <= 1e-11 * scale, (k, z, got, quad(k, z))
+    for z in zs:
+        f = lambda q: _hej_fourier(q, sig, n) * (np.cos(q * z) - 1.0) if q > 0 else 0.5 * z * z
+        want = si.quad(f, 0, qmax, limit=2000, epsabs=1e-15, epsrel=1e-12,
+                       points=[2 * np.pi / max(z, 1e-9) * i for i in range(1, 8)])[0] / np.pi
+        got = L.oracle_hej_1unb(kernel, z, 0.0, sig, 1)
+        assert abs(got - want) <= 1e-11 * max(abs(want), sig), (z, got, want)
+
+
+@pytest.mark.parametrize("bc6", [(4, 4, 3, 3, 3, 3), (0, 1, 3, 3, 4, 4)])
+@pytest.mark.parametrize("kernel,order", [(2, 2), (3, 4), (4, 6), (5, 8), (6, 10)])
+def test_hej_one_unbounded_convergence_order(kernel, order, bc6):
+    """HEJ_n solves with one unbounded direction converge at order n against the manufactured
+    solution: the L2 error ratio between N = 64 and 128 (the orders still rise with N; at
+    128 the HEJ_10 error reaches 1e-9)"""
+    e64 = orc.validation_errors(1, bc6, kernel, 64)
+    e128 = orc.validation_errors(1, bc6, kernel, 128)
+    assert np.log2(e64[0] / e128[0]) > order - 1.25, (e64, e128)
+    assert np.log2(e64[1] / e128[1]) > order - 1.5, (e64, e128)
+
+
 @pytest.mark.parametrize("kernel,order", [(11, 2), (12, 2), (13, 4), (14, 6)])
 def test_mehr_one_unbounded_convergence_order(kernel, order):
     """Mehrstellen kernels with one unbounded direction (x unbounded, y and z periodic): the
