@@ -127,6 +127,7 @@ def lib():
         L.flups_set_precision.argtypes = [c_void_p, c_int]
         L.flups_setup.argtypes = [c_void_p, c_bool]
         L.flups_solve.argtypes = [c_void_p, c_void_p, c_void_p, c_int]
+        L.flups_solve_noalloc.argtypes = [c_void_p, c_void_p, c_void_p, c_int]
         L.flups_get_allocSize.argtypes = [c_void_p]
         L.flups_get_allocSize.restype = c_size_t
         L.flups_get_spectralInfo.argtypes = [c_void_p, POINTER(c_double), POINTER(c_double), POINTER(c_double)]

@@ -207,6 +207,13 @@ void flups_solve(FLUPS_Solver* s, double* field, double* rhs, const FLUPS_Solver
         return 0;
     });
 }
+void flups_solve_noalloc(FLUPS_Solver* s, double* field, double* rhs, const FLUPS_SolverType type) {
+    guard("flups_solve_noalloc", [&]() {
+        FLUPS_REQUIRE(s != nullptr, "solver is NULL");
+        s->impl->solve_noalloc(field, rhs, (int)type);
+        return 0;
+    });
+}
 size_t flups_get_allocSize(FLUPS_Solver* s) { return s ? s->impl->alloc_size() : 0; }
 void flups_get_spectralInfo(FLUPS_Solver* s, double kfact[3], double koffset[3], double symstart[3]) {
     if (s) s->impl->spectral_info(kfact, koffset, symstart);

@@ -864,8 +864,10 @@ void row_dispatch(bool fwd, int H, const RowArgs& a, int comps, hipStream_t st) 
     else row_launch<R, 0>(fwd, H, a, comps, st);
 }
 
+// s0 / s1: the stage arrays (the plan's own, or caller-provided ones of fast_stage_bytes)
 template <typename R>
-void run_fast(FastPlan* f, const double* rhs, double* field, const View& uin, const View& uout, hipStream_t st, int type = 0) {
+void run_fast(FastPlan* f, void* s0, void* s1, const double* rhs, double* field, const View& uin, const View& uout,
+              hipStream_t st, int type = 0) {
     const int N0 = f->n[0], N1 = f->n[1], N2 = f->n[2];
     const int M0 = f->M[0], M1 = f->M[1], M2 = f->M[2];
     const int H = M0 / 2;
@@ -877,7 +879,7 @@ void run_fast(FastPlan* f, const double* rhs, double* field, const View& uin, co
     RowArgs r{};
     r.rev = (f->order >> 3) & 1;
     r.src = rhs;
-    r.dst = f->s0;
+    r.dst = s0;
     r.tw = f->tw[0];
     r.tw2 = f->twx;
     r.s_row_src = uin.s[1];
@@ -901,8 +903,8 @@ void run_fast(FastPlan* f, const double* rhs, double* field, const View& uin, co
         ColArgs a{};
         a.rev = f->order & 1;
         a.xcd = (f->ysched >> 4) & 1;
-        a.src = reinterpret_cast<const char*>(f->s0) + es * x0;
-        a.dst = f->s1;
+        a.src = reinterpret_cast<const char*>(s0) + es * x0;
+        a.dst = s1;
         a.tw = f->tw[1];
         a.nx = w;
         a.ldx = (int)f->ldxs;
@@ -919,8 +921,8 @@ void run_fast(FastPlan* f, const double* rhs, double* field, const View& uin, co
         if (f->profile) HIP_CHECK(hipEventRecord(f->ev[2 + 3 * k], st));
         // 3. z fused: S1 in place, forward, * normfact G, inverse, keep N2
         ColArgs z{};
-        z.src = f->s1;
-        z.dst = f->s1;
+        z.src = s1;
+        z.dst = s1;
         z.green = f->green;
         z.tw = f->tw[2];
         z.nx = w;
@@ -958,8 +960,8 @@ void run_fast(FastPlan* f, const double* rhs, double* field, const View& uin, co
         // 4. y inverse: S1 -> columns [x0, x0 + w) of S0, keep N1
         ColArgs b = a;
         b.rev = (f->order >> 1) & 1;
-        b.src = f->s1;
-        b.dst = reinterpret_cast<char*>(f->s0) + es * x0;
+        b.src = s1;
+        b.dst = reinterpret_cast<char*>(s0) + es * x0;
         b.s_col = f->ldxs;
         b.s_col_dst = ldx;
         b.s_other_src = zs;
@@ -974,7 +976,7 @@ void run_fast(FastPlan* f, const double* rhs, double* field, const View& uin, co
     // 5. x c2r: S0 -> user (real), keep N0
     RowArgs q{};
     q.rev = (f->order >> 2) & 1;
-    q.src = f->s0;
+    q.src = s0;
     q.dst = field;
     q.tw = f->tw[0];
     q.tw2 = f->twx;
@@ -995,8 +997,26 @@ void run_fast(FastPlan* f, const double* rhs, double* field, const View& uin, co
 }  // namespace
 
 void fast_solve(FastPlan* f, double* field, const double* rhs, const View& uin, const View& uout, hipStream_t s, int type) {
-    if (f->fp32) run_fast<float>(f, rhs, field, uin, uout, s, type);
-    else run_fast<double>(f, rhs, field, uin, uout, s, type);
+    if (f->fp32) run_fast<float>(f, f->s0, f->s1, rhs, field, uin, uout, s, type);
+    else run_fast<double>(f, f->s0, f->s1, rhs, field, uin, uout, s, type);
+}
+
+// bytes of the two stage arrays S0 and S1 (each a whole number of 256-byte blocks)
+void fast_stage_bytes(const FastPlan* f, size_t* b0, size_t* b1) {
+    const size_t es = f->fp32 ? 2 * sizeof(float) : 2 * sizeof(double);
+    const size_t n0 = (size_t)f->ldx * f->n[1] * f->n[2] * f->lda;
+    const size_t n1 = (size_t)f->zs * f->n[2] * f->lda;
+    *b0 = (n0 * es + 255) / 256 * 256;
+    *b1 = (n1 * es + 255) / 256 * 256;
+}
+
+// the same solve on caller-provided stage arrays of fast_stage_bytes (no zero fill needed:
+// a pass reads only what the pass before it wrote, or padding columns whose values never
+// reach a stored element)
+void fast_solve_on(FastPlan* f, void* s0, void* s1, double* field, const double* rhs, const View& uin, const View& uout,
+                   hipStream_t s, int type) {
+    if (f->fp32) run_fast<float>(f, s0, s1, rhs, field, uin, uout, s, type);
+    else run_fast<double>(f, s0, s1, rhs, field, uin, uout, s, type);
 }
 
 size_t fast_device_bytes(const FastPlan* f) { return f->dev_bytes; }
@@ -1358,8 +1378,8 @@ void fast_autotune(FastPlan* f, double* out, const double* in, const View& uin, 
         }
     }
     auto run = [&]() {
-        if (f->fp32) run_fast<float>(f, in, out, uin, uout, st);
-        else run_fast<double>(f, in, out, uin, uout, st);
+        if (f->fp32) run_fast<float>(f, f->s0, f->s1, in, out, uin, uout, st);
+        else run_fast<double>(f, f->s0, f->s1, in, out, uin, uout, st);
     };
     const bool prof = f->profile;
     f->profile = false;

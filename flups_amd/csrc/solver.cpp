@@ -46,6 +46,11 @@ void fast_collect(FastPlan* f);
 void fast_profile(FastPlan* f, bool on);
 int fast_profile_get(FastPlan* f, int i, const char** name, double* ms, long* calls, double* bytes);
 size_t fast_device_bytes(const FastPlan* f);
+void fast_stage_bytes(const FastPlan* f, size_t* b0, size_t* b1);
+void fast_solve_on(FastPlan* f, void* s0, void* s1, double* field, const double* rhs, const View& uin, const View& uout,
+                   hipStream_t s, int type);
+void scratch_init();                           // scratch.hip
+double* scratch_device_ptr(size_t* bytes);
 HybridZ* make_hybrid_z(SolverImpl& s);
 void destroy_hybrid_z(HybridZ* h);
 size_t hybrid_device_bytes(const HybridZ* h);
@@ -277,6 +282,7 @@ void SolverImpl::spectral_info(double kf[3], double ko[3], double ss[3]) const {
 const double* SolverImpl::twiddles(int M) {
     auto it = tw_.find(M);
     if (it != tw_.end()) return it->second;
+    FLUPS_REQUIRE(!frozen_, "twiddle table of length %d missing in a solve that may not allocate", M);
     std::vector<double> t(2 * (size_t)M);
     for (int i = 0; i < M; ++i) {
         const long double a = -2.0L * 3.14159265358979323846264338327950288L * (long double)i / (long double)M;
@@ -294,6 +300,7 @@ const double* SolverImpl::r2r_table(int kind, int n) {
     const auto key = std::make_pair(kind, n);
     auto it = r2r_tab_.find(key);
     if (it != r2r_tab_.end()) return it->second;
+    FLUPS_REQUIRE(!frozen_, "real-to-real table (kind %d, n %d) missing in a solve that may not allocate", kind, n);
     const auto t = row_r2r_table(kind, n);
     double* d = nullptr;
     HIP_CHECK(hipMalloc(&d, t.size() * sizeof(double)));
@@ -710,6 +717,7 @@ void SolverImpl::setup() {
         warm_tables();
         warmed_ = true;
     }
+    scratch_init();   // this device's static scratch address, for flups_solve_noalloc
     HIP_CHECK(hipStreamSynchronize(stream_));
     setup_done_ = true;
     if (prof_) prof_->add("setup", std::chrono::duration<double>(std::chrono::steady_clock::now() - t_setup).count(), 0.0);
@@ -954,6 +962,64 @@ void SolverImpl::enqueue(double* field, const double* rhs, int type) {
         FLUPS_REQUIRE(dstage_[0] && dgreen_, "solver not prepared for solve type %d", type);
         solve_generic(field, rhs, type);
     }
+}
+
+// flups_solve_noalloc: the solve of `enqueue` with its stage arrays carved from the device's
+// static scratch instead of the solver's own; field and rhs are device pointers; nothing is
+// allocated, freed or synchronised (the tables were built at setup; a missing one is an
+// error, not an allocation)
+void SolverImpl::solve_noalloc(double* field, const double* rhs, int type) {
+    check_type(type);
+    FLUPS_REQUIRE(field && rhs, "field and rhs must not be NULL");
+    hipPointerAttribute_t at;
+    for (const void* p : {(const void*)field, (const void*)rhs}) {
+        const bool dev = hipPointerGetAttributes(&at, p) == hipSuccess && at.type == hipMemoryTypeDevice;
+        if (!dev) (void)hipGetLastError();
+        FLUPS_REQUIRE(dev, "flups_solve_noalloc takes device pointers only (host arrays would need staging buffers)");
+    }
+    size_t cap = 0;
+    double* base = scratch_device_ptr(&cap);
+    FLUPS_REQUIRE(base, "no static scratch on this device: flups_setup of this solver ran on another device");
+    const bool fused = fast_ && (type == 0 || lda_ == 3);
+    size_t need = 0;
+    size_t off[3] = {0, 0, 0};
+    if (fused) {
+        size_t b0 = 0, b1 = 0;
+        fast_stage_bytes(fast_, &b0, &b1);
+        off[1] = b0;
+        need = b0 + b1;
+    } else {
+        for (int ip = 0; ip < ndim_; ++ip) {
+            off[ip] = need;
+            need += (sizeof(double) * std::max(hat_[ip].count(), stage_[ip].count()) + 255) / 256 * 256;
+        }
+    }
+    FLUPS_REQUIRE(need <= cap, "this solve needs %zu bytes of stage arrays; the static scratch of flups_solve_noalloc holds %zu",
+                  need, cap);
+    char* sb = reinterpret_cast<char*>(base);
+    if (fused) {
+        fast_solve_on(fast_, sb + off[0], sb + off[1], field, rhs, user_view(const_cast<double*>(rhs)), user_view(field),
+                      stream_, type);
+        return;
+    }
+    FLUPS_REQUIRE(dgreen_, "solver not set up for solve type %d", type);
+    double* keep[3] = {dstage_[0], dstage_[1], dstage_[2]};
+    Timers* keep_prof = prof_;
+    for (int ip = 0; ip < 3; ++ip) dstage_[ip] = ip < ndim_ ? reinterpret_cast<double*>(sb + off[ip]) : nullptr;
+    prof_ = nullptr;   // profiler stages create events: not in a call that may not allocate
+    frozen_ = true;
+    auto restore = [&]() {
+        for (int ip = 0; ip < 3; ++ip) dstage_[ip] = keep[ip];
+        prof_ = keep_prof;
+        frozen_ = false;
+    };
+    try {
+        solve_generic(field, rhs, type);
+    } catch (...) {
+        restore();
+        throw;
+    }
+    restore();
 }
 
 size_t SolverImpl::device_bytes() const {

@@ -61,6 +61,7 @@ class SolverImpl {
     void solve(double* field, const double* rhs, int type);
     void prepare(int type);                                          // allocate what a solve of this type needs
     void enqueue(double* field, const double* rhs, int type);        // device pointers, no sync, no allocation
+    void solve_noalloc(double* field, const double* rhs, int type);  // enqueue on the static scratch's stage arrays
     size_t device_bytes() const;                                     // device memory held now
     size_t predicted_device_bytes(int type) const;                   // bound for setup + solve, before setup
     bool is_setup() const { return setup_done_; }
@@ -133,6 +134,7 @@ class SolverImpl {
     double* dstaging_ = nullptr;   // rhs/field copies for host pointers
     double* dinner_ = nullptr;     // inner buffer of the advanced interface
     std::vector<int> pass_kernels_;
+    bool frozen_ = false;   // a solve that may not allocate is running: missing tables are errors
     bool use_graph_ = false, profiling_ = false, warmed_ = false;   // FLUPS_GRAPH=1: replay solves as graphs
     hipStream_t cap_ = nullptr;    // capture stream of the solve graph
     hipGraphExec_t gexec_ = nullptr;

@@ -123,6 +123,15 @@ void flups_set_greenType(FLUPS_Solver* s, const FLUPS_GreenType type);          
 void flups_set_stream(FLUPS_Solver* s, void* stream);                                             /* src/flups.h:429 */
 void flups_setup(FLUPS_Solver* s, const bool changeComm);                                         /* src/flups.h:441 */
 void flups_solve(FLUPS_Solver* s, double* field, double* rhs, const FLUPS_SolverType type);        /* src/flups.h:452 */
+/* No counterpart in the reference: for callers that cannot allocate device memory (code under
+   HIP graph capture, tight serving loops). The solve of flups_solve for DEVICE field / rhs
+   pointers, with the solve's stage arrays taken from a static device array of 128 MiB per GPU
+   (held from library load to process end) instead of the solver's own: no hipMalloc, hipFree
+   or other allocation, no synchronisation. The work is queued on the solver's stream
+   (flups_set_stream); the result is complete once that stream is synchronised. A solve whose
+   stage arrays exceed 128 MiB (e.g. an all-unbounded FP64 problem above about 128^3) fails
+   with an error. All calls on one GPU share the array: calls that overlap on one GPU race. */
+void flups_solve_noalloc(FLUPS_Solver* s, double* field, double* rhs, const FLUPS_SolverType type);
 
 /* ---- advanced (src/flups.h:467-578) ---- */
 size_t flups_get_allocSize(FLUPS_Solver* s);                                                      /* src/flups.h:473 */
