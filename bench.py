@@ -41,19 +41,44 @@ def blobs(n, seed):
     return f[None]
 
 
-def cpu_baseline(n, kernel, reps, threads):
-    """oracle solver timed on the host: setup excluded, 1 warm-up + `reps` timed solves"""
+def host_cpus():
+    """the host's CPU topology (lscpu) and the threads the CPU baseline uses: the CPUs this process may run
+    on, capped by OMP_NUM_THREADS when the environment sets it"""
+    import subprocess
+    info = {}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            k, v = k.strip(), v.strip()
+            if k in ("Socket(s)", "Core(s) per socket", "Thread(s) per core", "Model name", "CPU(s)"):
+                info[k] = v
+    except Exception:
+        pass
+    avail = len(os.sched_getaffinity(0))
+    omp = os.environ.get("OMP_NUM_THREADS")
+    used = min(avail, int(omp)) if omp and omp.isdigit() and int(omp) > 0 else avail
+    return {"lscpu_sockets": info.get("Socket(s)"), "lscpu_cores_per_socket": info.get("Core(s) per socket"),
+            "lscpu_threads_per_core": info.get("Thread(s) per core"), "lscpu_cpus": info.get("CPU(s)"),
+            "model": info.get("Model name"), "affinity_cpus": avail, "threads_used": used}
+
+
+def cpu_baseline(n, kernel, budget_s, threads):
+    """oracle solver timed on the host: setup excluded, 1 warm-up, then timed solves for about `budget_s`
+    seconds (at least 3)"""
     from tests import oracle_lib as orc
-    os.environ.setdefault("OMP_NUM_THREADS", str(threads))
+    os.environ["OMP_NUM_THREADS"] = str(threads)   # read by the OpenMP runtime when the oracle loads
     rhs = blobs(n, 2024)
     field = np.zeros_like(rhs)
     fb = orc.flat_bc([[4, 4]] * 3, 1)
     ng = np.array([n] * 3, dtype=np.int32)
     sec = np.zeros(1)
-    orc.check(orc.lib().oracle_solve_timed(orc._p(ng, ctypes.c_int), orc._p(fb, ctypes.c_int), 1,
-                                           orc._p(np.array([1.0 / n] * 3)), orc._p(np.ones(3)), 1, 0, kernel, 2.0,
-                                           orc.KDIR, 0, orc._p(rhs), orc._p(field), 1, reps, orc._p(sec)))
-    return reps / sec[0], sec[0]
+    reps = np.zeros(1, dtype=np.int32)
+    orc.check(orc.lib().oracle_solve_timed_for(orc._p(ng, ctypes.c_int), orc._p(fb, ctypes.c_int), 1,
+                                               orc._p(np.array([1.0 / n] * 3)), orc._p(np.ones(3)), 1, 0, kernel, 2.0,
+                                               orc.KDIR, 0, orc._p(rhs), orc._p(field), 1, budget_s, 3, 1000,
+                                               orc._p(reps, ctypes.c_int), orc._p(sec)))
+    return int(reps[0]) / sec[0], int(reps[0]), sec[0]
 
 
 SERVE_SHAPES = [(64, 64, 64), (96, 128, 64), (128, 128, 128), (256, 256, 256), (1024, 1024, 1)]
@@ -62,9 +87,11 @@ SERVE_TEMPLATES = {"unbounded": [[4, 4], [4, 4], [4, 4]], "periodic-x": [[3, 3],
 SERVE_KERNELS = {"HEJ_2": 2, "HEJ_6": 4, "LGF_4": 8}
 
 
-def serve_mix(fl, nreq, rate, workers, seed=7):
-    """Poisson arrivals of mixed requests through the library's serving loop, against the plain path
-    (one request at a time, same arrivals). Solvers are warmed (set up once per request kind) first."""
+def serve_mix(fl, nreq, rates, workers, seed=7):
+    """Poisson arrivals of mixed requests through the library's serving loop (`workers` threads), against the
+    plain path (one request at a time), at each arrival rate in `rates`: the same request sequence and
+    arrival pattern scaled to the rate. Each server's solver cache is warmed (every request kind set up once)
+    before the first rate; solves/s, p50 and p99 latency (submit to completion) per rate."""
     import threading
     rng = np.random.default_rng(seed)
     kinds = {}
@@ -85,7 +112,7 @@ def serve_mix(fl, nreq, rate, workers, seed=7):
                 L[2] = h[2]
             kinds[key] = fl.ServeRequest.make(ng, bc, h, L, green=SERVE_KERNELS[kern], fp32=fp32)
         reqs.append(key)
-    arrivals = np.cumsum(rng.exponential(1.0 / rate, nreq))
+    gaps = rng.exponential(1.0, nreq)   # unit-rate Poisson gaps, scaled per rate
     # device buffers: one input per shape, a pool of outputs per shape
     shapes = sorted({k[0] for k in reqs})
     inputs, pools = {}, {}
@@ -97,10 +124,8 @@ def serve_mix(fl, nreq, rate, workers, seed=7):
         inputs[ng] = fl.DeviceArray.from_numpy(f)
         pools[ng] = [fl.DeviceArray(f.shape) for _ in range(workers + 2)]
 
-    def run(nw):
-        srv = fl.Server(160 << 30, workers=nw, solvers_per_key=1)
-        for key, r in kinds.items():   # warm the solver cache
-            srv.wait(srv.submit(r, inputs[key[0]], pools[key[0]][0]))
+    def run(srv, nw, rate):
+        arrivals = np.cumsum(gaps / rate)
         srv.reset_stats()
         free = {ng: list(range(len(pools[ng]))) for ng in shapes}
         cv = threading.Condition()
@@ -135,17 +160,22 @@ def serve_mix(fl, nreq, rate, workers, seed=7):
                 cv.notify_all()
         th.join()
         st = srv.stats()
-        srv.close()
         return {"p50_ms": round(st["p50_ms"], 3), "p99_ms": round(st["p99_ms"], 3), "mean_ms": round(st["mean_ms"], 3),
                 "solves_per_s": round(st["solves_per_s"], 1), "completed": int(st["completed"]),
                 "failed": int(st["failed"]), "workers": nw}
 
-    served = run(workers)
-    plain = run(1)
-    return {"requests": nreq, "rate_per_s": rate, "kinds": len(kinds),
+    results = {r: {"rate_per_s": r} for r in rates}
+    for label, nw in (("served", workers), ("plain_one_at_a_time", 1)):
+        srv = fl.Server(160 << 30, workers=nw, solvers_per_key=1)
+        for key, r in kinds.items():   # warm the solver cache
+            srv.wait(srv.submit(r, inputs[key[0]], pools[key[0]][0]))
+        for r in rates:
+            results[r][label] = run(srv, nw, r)
+        srv.close()
+    return {"requests": nreq, "kinds": len(kinds),
             "mix": "shapes 64^3, 96x128x64, 128^3, 256^3, 2-D 1024^2; BC templates unbounded, periodic-x, "
                    "symmetric-x/periodic-y, periodic; HEJ_2/HEJ_6/LGF_4; FP32 or FP64; Poisson arrivals",
-            "served": served, "plain_one_at_a_time": plain}
+            "sweep": [results[r] for r in rates]}
 
 
 def rhs_mixed(n, seed):
@@ -196,12 +226,17 @@ def run_config(fl, name, nglob, bc, center, kernel, lda, typ, odiff, rhs, h, L, 
     setup_s = time.time() - t0
     din = fl.DeviceArray.from_numpy(rhs)
     dout = fl.DeviceArray(rhs.shape)
+    stream = fl.Stream()   # solves queued back to back on a caller stream, as for the headline
+    s.set_stream(stream)
     for _ in range(warmup):
         s.solve(dout, din, typ)
+    stream.synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):
         s.solve(dout, din, typ)
+    stream.synchronize()
     el = (time.perf_counter() - t0) / steps
+    s.set_stream(None)
     by = s.solve_bytes(typ)
     out = {"workload": name, "solves_per_s": round(1.0 / el, 3), "ms_per_step": round(el * 1e3, 3),
            "algorithmic_GB": round(by / 1e9, 3), "achieved_GBs": round(by / el / 1e9, 1),
@@ -322,11 +357,13 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--kernel", type=int, default=2, help="Green's kernel (2 = HEJ_2)")
-    ap.add_argument("--cpu-reps", type=int, default=2, help="timed oracle solves for the CPU baseline (0 = skip)")
+    ap.add_argument("--cpu-seconds", type=float, default=20.0,
+                    help="time budget of the CPU baseline's timed solves (0 = skip)")
     ap.add_argument("--profile-steps", type=int, default=10)
     ap.add_argument("--fp32-steps", type=int, default=50, help="timed solves of the FP32 variant (0 = skip)")
     ap.add_argument("--serve-requests", type=int, default=300, help="requests of the served mix (0 = skip)")
-    ap.add_argument("--serve-rate", type=float, default=400.0, help="Poisson arrival rate of the served mix (1/s)")
+    ap.add_argument("--serve-rates", default="400,2000,5000,10000",
+                    help="Poisson arrival rates of the served mix (1/s, comma list)")
     ap.add_argument("--serve-workers", type=int, default=4)
     ap.add_argument("--configs", default="mixed,ring,batch",
                     help="other workloads to time (comma list of mixed, ring, batch; '' = none)")
@@ -357,17 +394,36 @@ def main():
         a, b = fl.DeviceArray((nb // 8,)), fl.DeviceArray((nb // 8,))
         fl.lib().flups_memcpy(b.ptr, a.ptr, nb)
         del a, b
+    # throughput: solves queued back to back on a caller stream (flups_set_stream: flups_solve on device
+    # arrays then returns once queued), the timed block bracketed by stream synchronisations
+    stream = fl.Stream()
+    s.set_stream(stream)
     for _ in range(args.warmup):
         s.solve(field, rhs, fl.SolverType.STD)
-    # each flups_solve returns with its stream synchronised: the timed block starts and ends idle
-    lat = []
+    stream.synchronize()
     t_start = time.perf_counter()
     for _ in range(args.steps):
+        s.solve(field, rhs, fl.SolverType.STD)
+    stream.synchronize()
+    elapsed = time.perf_counter() - t_start
+    solves_per_s = args.steps / elapsed
+    # latency: one solve at a time on the default stream (flups_solve returns with the result complete)
+    s.set_stream(None)
+    lat = []
+    for _ in range(max(args.steps, 20)):
         t1 = time.perf_counter()
         s.solve(field, rhs, fl.SolverType.STD)
         lat.append(time.perf_counter() - t1)
-    elapsed = time.perf_counter() - t_start
-    solves_per_s = args.steps / elapsed
+    sync_solves_per_s = len(lat) / sum(lat)
+    # host arrays: the same solve with the right-hand side and the field in host memory (PCIe copies inside)
+    h_rhs = blobs(n, 2024)
+    h_field = np.zeros_like(h_rhs)
+    s.solve(h_field, h_rhs, fl.SolverType.STD)
+    t0 = time.perf_counter()
+    for _ in range(5):
+        s.solve(h_field, h_rhs, fl.SolverType.STD)
+    host_solves_per_s = 5 / (time.perf_counter() - t0)
+    del h_rhs, h_field
 
     # dominant kernel: per-kernel HIP events on the solve stream over a profiled block
     L = fl.lib()
@@ -402,7 +458,10 @@ def main():
             traffic = json.load(open(tf)).get("bytes_per_launch")
         roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": PEAK_HBM / 1e9, "unit": "GB/s",
                 "frac": round(ach * 1e9 / PEAK_HBM, 4), "traffic": traffic, "kernel": dom["kernel"],
-                "algorithmic_bytes": dom["bytes"], "avg_us": round(dom["avg_us"], 2)}
+                "algorithmic_bytes": dom["bytes"], "avg_us": round(dom["avg_us"], 2),
+                # traffic is not measured in this run: it is the committed rocprofv3 figure of the same
+                # kernel (FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000 per launch, factors from a 1 GiB copy)
+                "traffic_source": os.path.relpath(tf, ROOT) if traffic is not None else None}
     total_bytes = sum(k["bytes"] for k in kernels) if kernels else None
 
     # FP32 transforms + multiply on the same workload (FP64 caller buffers), error against the FP64 path
@@ -414,12 +473,16 @@ def main():
         s32.set_precision(True)
         s32.setup()
         f32 = fl.DeviceArray(rhs.shape)
+        s32.set_stream(stream)
         for _ in range(args.warmup):
             s32.solve(f32, rhs, fl.SolverType.STD)
+        stream.synchronize()
         t0 = time.perf_counter()
         for _ in range(args.fp32_steps):
             s32.solve(f32, rhs, fl.SolverType.STD)
+        stream.synchronize()
         el32 = time.perf_counter() - t0
+        s32.set_stream(None)
         L.flups_profile_enable(s32._solver, 1)
         for _ in range(args.profile_steps):
             s32.solve(f32, rhs, fl.SolverType.STD)
@@ -446,16 +509,19 @@ def main():
                 "schedule": list(s32.schedule), "kernels": k32}
         del s32, f32
 
-    served = serve_mix(fl, args.serve_requests, args.serve_rate, args.serve_workers) if args.serve_requests > 0 else None
+    rates = [float(r) for r in args.serve_rates.split(",") if r]
+    served = serve_mix(fl, args.serve_requests, rates, args.serve_workers) if args.serve_requests > 0 and rates else None
     others = other_configs(fl, [c for c in args.configs.split(",") if c]) if args.configs else None
 
     cpu = None
-    if args.cpu_reps > 0:
-        threads = int(os.environ.get("OMP_NUM_THREADS", str(min(16, os.cpu_count() or 1))))
-        v, sec = cpu_baseline(n, args.kernel, args.cpu_reps, threads)
+    if args.cpu_seconds > 0:
+        topo_cpu = host_cpus()
+        threads = topo_cpu["threads_used"]
+        v, reps, sec = cpu_baseline(n, args.kernel, args.cpu_seconds, threads)
         cpu = {"value": round(v, 4), "unit": "solves/s", "cores": threads, "kind": "port",
-               "sample": "%d solves of the same %d^3 unbounded HEJ_2 workload after 1 warm-up, %.1f s" %
-                         (args.cpu_reps, n, sec)}
+               "sample": "%d timed solves of the same %d^3 unbounded HEJ_2 workload (setup excluded) after 1 warm-up, "
+                         "%.1f s, OpenMP over pencils with %d threads" % (reps, n, sec, threads),
+               "host": topo_cpu}
 
     lat_ms = np.array(lat) * 1e3
     out = {
@@ -467,7 +533,10 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "solve_roofline_frac": round(total_bytes * solves_per_s / PEAK_HBM, 4) if total_bytes else None,
-        "latency_ms": {"p50": round(float(np.percentile(lat_ms, 50)), 4), "p99": round(float(np.percentile(lat_ms, 99)), 4)},
+        "latency_ms": {"p50": round(float(np.percentile(lat_ms, 50)), 4), "p99": round(float(np.percentile(lat_ms, 99)), 4),
+                       "solves": len(lat)},
+        "sync_solves_per_s": round(sync_solves_per_s, 3),
+        "host_pointer_solves_per_s": round(host_solves_per_s, 3),
         "kernels": [{k: (round(v, 3) if isinstance(v, float) else v) for k, v in kk.items()} for kk in kernels],
         "setup_s": round(setup_s, 3),
         "schedule": list(s.schedule),

@@ -160,6 +160,10 @@ def lib():
         L.flups_memcpy.restype = c_int
         L.flups_pointer_is_device.argtypes = [c_void_p]
         L.flups_pointer_is_device.restype = c_int
+        L.flups_stream_create.restype = c_void_p
+        L.flups_stream_destroy.argtypes = [c_void_p]
+        L.flups_stream_synchronize.argtypes = [c_void_p]
+        L.flups_stream_synchronize.restype = c_int
         L.flups_server_new.restype = c_void_p
         L.flups_server_new.argtypes = [c_size_t, c_int, c_int]
         L.flups_server_free.argtypes = [c_void_p]
@@ -467,6 +471,22 @@ def _data_ptr(a):
     return a.ctypes.data
 
 
+class Stream:
+    """HIP stream of libflups' runtime (flups_stream_create), for Solver.set_stream."""
+
+    def __init__(self):
+        self.handle = lib().flups_stream_create()
+        _check(self.handle, "flups_stream_create")
+
+    def synchronize(self):
+        _check(lib().flups_stream_synchronize(self.handle), "flups_stream_synchronize")
+
+    def __del__(self):
+        if getattr(self, "handle", None):
+            lib().flups_stream_destroy(self.handle)
+            self.handle = None
+
+
 class Profiler:
     """Timers of the solvers created with it (python_api/pyflups/flups.py:185-218): init, setup and
     every solve with its stages (HIP events on the solve stream)."""
@@ -537,8 +557,13 @@ class Solver:
     def set_alpha(self, alpha):
         lib().flups_set_alpha(self._solver, float(alpha))
 
-    def set_stream(self, stream_handle):
-        lib().flups_set_stream(self._solver, stream_handle)
+    def set_stream(self, stream):
+        """run on `stream` (a Stream, a raw hipStream_t of the library's runtime, or None for the
+        default stream). With a stream, solve() on device arrays returns once the work is queued;
+        synchronise the stream before reading the result."""
+        handle = stream.handle if isinstance(stream, Stream) else stream
+        self._stream = stream   # kept alive while the solver uses it
+        lib().flups_set_stream(self._solver, handle)
 
     def set_autotune(self, on=True):
         lib().flups_clear_last_error()

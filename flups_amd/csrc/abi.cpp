@@ -180,7 +180,9 @@ void flups_set_greenType(FLUPS_Solver* s, const FLUPS_GreenType type) {
     });
 }
 void flups_set_stream(FLUPS_Solver* s, void* stream) {
-    if (s) s->impl->set_stream((hipStream_t)stream);
+    // a caller-given stream opts into asynchronous solves on device arrays; NULL restores the
+    // default stream and the synchronous return
+    if (s) s->impl->set_stream((hipStream_t)stream, stream != nullptr);
 }
 void flups_set_alpha(FLUPS_Solver* s, const double alpha) {
     if (s) s->impl->set_alpha(alpha);
@@ -337,6 +339,22 @@ void flups_device_free(void* p) {
 int flups_memcpy(void* dst, const void* src, size_t bytes) {
     return guard("flups_memcpy", [&]() {
         HIP_CHECK(hipMemcpy(dst, src, bytes, hipMemcpyDefault));
+        return 1;
+    });
+}
+void* flups_stream_create(void) {
+    return guard("flups_stream_create", [&]() -> void* {
+        hipStream_t st = nullptr;
+        HIP_CHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        return (void*)st;
+    });
+}
+void flups_stream_destroy(void* stream) {
+    if (stream) (void)hipStreamDestroy((hipStream_t)stream);
+}
+int flups_stream_synchronize(void* stream) {
+    return guard("flups_stream_synchronize", [&]() -> int {
+        HIP_CHECK(hipStreamSynchronize((hipStream_t)stream));
         return 1;
     });
 }

@@ -1432,6 +1432,7 @@ void fast_collect(FastPlan* f) {
     }
     ++f->calls;
 }
+bool fast_profiling(const FastPlan* f) { return f->profile; }
 void fast_profile(FastPlan* f, bool on) {
     f->profile = on;
     for (double& m : f->ms) m = 0;

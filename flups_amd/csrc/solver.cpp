@@ -43,6 +43,7 @@ void destroy_fast_plan(FastPlan* f);
 void fast_solve(FastPlan* f, double* field, const double* rhs, const View& uin, const View& uout, hipStream_t s,
                 int type);
 void fast_collect(FastPlan* f);
+bool fast_profiling(const FastPlan* f);
 void fast_profile(FastPlan* f, bool on);
 int fast_profile_get(FastPlan* f, int i, const char** name, double* ms, long* calls, double* bytes);
 size_t fast_device_bytes(const FastPlan* f);
@@ -1022,6 +1023,17 @@ void SolverImpl::solve_noalloc(double* field, const double* rhs, int type) {
     restore();
 }
 
+double SolverImpl::last_solve_ns() {
+    if (last_pending_) {   // an asynchronous solve: its end event, once the stream reached it
+        HIP_CHECK(hipEventSynchronize(ev1_));
+        float ms = 0;
+        HIP_CHECK(hipEventElapsedTime(&ms, ev0_, ev1_));
+        last_ns_ = ms * 1e6;
+        last_pending_ = false;
+    }
+    return last_ns_;
+}
+
 size_t SolverImpl::device_bytes() const {
     size_t b = 0;
     for (const auto& kv : tw_) b += 2 * sizeof(double) * (size_t)kv.first;
@@ -1085,10 +1097,16 @@ void SolverImpl::solve(double* field, const double* rhs, int type) {
     if (fused) pmark("fused pipeline (fftw + domagic)");
     if (!fdev) HIP_CHECK(hipMemcpyAsync(field, dout, n * sizeof(double), hipMemcpyDeviceToHost, stream_));
     if (!fdev) pmark("copy field");
+    if (async_ && fdev && rdev && !prof_ && !(fused && fast_profiling(fast_))) {
+        // the caller gave a stream (flups_set_stream): return with the solve queued on it
+        last_pending_ = true;
+        return;
+    }
     HIP_CHECK(hipStreamSynchronize(stream_));
     float ms = 0;
     HIP_CHECK(hipEventElapsedTime(&ms, ev0_, ev1_));
     last_ns_ = ms * 1e6;
+    last_pending_ = false;
     if (prof_) {
         // the whole solve: device time from the first to the last mark, host wall-clock
         float all = 0;

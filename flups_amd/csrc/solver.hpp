@@ -52,7 +52,12 @@ class SolverImpl {
     ~SolverImpl();
     void set_green(int t) { gtype_ = t; }
     void set_alpha(double a) { alpha_ = a; }
-    void set_stream(hipStream_t s) { stream_ = s; }
+    // async: flups_solve on device arrays returns once its work is queued on the stream (a
+    // caller-given stream); otherwise it returns with the result complete
+    void set_stream(hipStream_t s, bool async = false) {
+        stream_ = s;
+        async_ = async;
+    }
     void set_fp32(bool f) {
         FLUPS_REQUIRE(!setup_done_, "flups_set_precision must be called before flups_setup");
         fp32_ = f;
@@ -84,7 +89,7 @@ class SolverImpl {
     int path() const { return fast_ ? (fp32_ ? 2 : 1) : hybrid_ ? 3 : 0; }
     // kernel of each general-path pass (component 0): forward passes in order, then the backward ones
     const std::vector<int>& pass_kernels() const { return pass_kernels_; }
-    double last_solve_ns() const { return last_ns_; }
+    double last_solve_ns();   // waits for the last solve's end when it was queued asynchronously
     int ndim() const { return ndim_; }
     // per-kernel timing of the fused pipeline (HIP events on the solve stream)
     void profile(bool on);
@@ -152,6 +157,7 @@ class SolverImpl {
     std::vector<std::string> plab_;
     int pn_ = 0;
     double last_ns_ = 0;
+    bool async_ = false, last_pending_ = false;
     std::string kdir_;
     friend struct FastPlan;
     friend FastPlan* make_fast_plan(SolverImpl&);

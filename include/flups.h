@@ -10,7 +10,11 @@
  * field/rhs pointers passed to flups_solve may be device pointers (HBM
  * resident, the fast path) or host pointers (copied over PCIe around the
  * solve). Work runs on the stream given by flups_set_stream (default stream
- * otherwise); flups_solve returns with the result complete.
+ * otherwise). flups_solve returns with the result complete, except when the
+ * caller gave a stream with flups_set_stream and both arrays are device
+ * arrays (and no profiler records the solve): the solve is then queued on
+ * that stream and the call returns at once, as GPU libraries do on a
+ * caller's stream; the result is complete once the stream is synchronised.
  *
  * Errors: entry points keep the reference's void/pointer returns. On a failed
  * check the library records a message (flups_get_last_error), prints it to
@@ -119,7 +123,9 @@ void flups_cleanup_solver(FLUPS_Solver* s);                                     
 void flups_cleanup_all(FLUPS_Solver* s);                                                          /* src/flups.h:401 */
 void flups_cleanup_backend(void);                                                                 /* src/flups.h:410 */
 void flups_set_greenType(FLUPS_Solver* s, const FLUPS_GreenType type);                            /* src/flups.h:421 */
-/* stream = a hipStream_t; the reference's hook is a no-op (src/flups.cpp:147) */
+/* stream = a hipStream_t; the reference's hook is a no-op (src/flups.cpp:147). A non-NULL stream
+   makes flups_solve on device arrays asynchronous on it; NULL restores the default stream and the
+   synchronous return */
 void flups_set_stream(FLUPS_Solver* s, void* stream);                                             /* src/flups.h:429 */
 void flups_setup(FLUPS_Solver* s, const bool changeComm);                                         /* src/flups.h:441 */
 void flups_solve(FLUPS_Solver* s, double* field, double* rhs, const FLUPS_SolverType type);        /* src/flups.h:452 */
@@ -218,6 +224,11 @@ void*       flups_device_malloc(size_t bytes);
 void        flups_device_free(void* p);
 int         flups_memcpy(void* dst, const void* src, size_t bytes);   /* any direction; 1 on success */
 int         flups_pointer_is_device(const void* p);
+/* HIP streams of the library's runtime (for flups_set_stream from callers without their own):
+   create (default priority, non-blocking), destroy, wait for (1 on success) */
+void*       flups_stream_create(void);
+void        flups_stream_destroy(void* stream);
+int         flups_stream_synchronize(void* stream);
 
 /* ---- serving and batching (no counterpart in the reference: its solver serves one
    problem per process). A request describes a whole problem; caller arrays are dense

@@ -80,6 +80,33 @@ int oracle_solve_timed(const int* nglob, const int* bc, int lda, const double* h
     } catch (const std::exception& e) { return fail(e); }
 }
 
+// the same with a time budget: `warm` solves, then timed solves until `budget_s` seconds have
+// passed (at least min_reps, at most max_reps); *reps_done and *seconds report the timed part
+int oracle_solve_timed_for(const int* nglob, const int* bc, int lda, const double* h, const double* L, int center,
+                           int odiff, int kernel, double alpha, const char* kdir, int type, const double* rhs,
+                           double* field, int warm, double budget_s, int min_reps, int max_reps, int* reps_done,
+                           double* seconds) {
+    try {
+        Solver s(nglob, bc, lda, h, L, center == 0, (DiffType)odiff);
+        s.kernel_dir = kdir ? kdir : ".";
+        s.set_green((GreenType)kernel);
+        s.set_alpha(alpha);
+        s.setup();
+        for (int i = 0; i < warm; ++i) s.solve(field, rhs, (SolverType)type);
+        const auto t0 = std::chrono::steady_clock::now();
+        int r = 0;
+        double el = 0;
+        while (r < max_reps && (r < min_reps || el < budget_s)) {
+            s.solve(field, rhs, (SolverType)type);
+            ++r;
+            el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        }
+        *reps_done = r;
+        *seconds = el;
+        return 0;
+    } catch (const std::exception& e) { return fail(e); }
+}
+
 // spectral-space data after the forward transforms (for stage-wise comparisons)
 int oracle_spectral_size(const int* nglob, const int* bc, int lda, const double* h, const double* L, int center, int* n_out,
                          int* nf) {

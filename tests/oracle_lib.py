@@ -28,6 +28,8 @@ def lib():
         L.oracle_solve.argtypes = [ip, ip, c_int, dp, dp, c_int, c_int, c_int, c_double, c_char_p, c_int, dp, dp]
         L.oracle_solve_timed.argtypes = [ip, ip, c_int, dp, dp, c_int, c_int, c_int, c_double, c_char_p, c_int, dp, dp,
                                          c_int, c_int, dp]
+        L.oracle_solve_timed_for.argtypes = [ip, ip, c_int, dp, dp, c_int, c_int, c_int, c_double, c_char_p, c_int, dp,
+                                             dp, c_int, c_double, c_int, c_int, ip, dp]
         L.oracle_spectral_size.argtypes = [ip, ip, c_int, dp, dp, c_int, ip, ip]
         L.oracle_green.argtypes = [ip, ip, dp, dp, c_int, c_int, c_double, c_char_p, dp]
         L.oracle_plan_info.argtypes = [ip, ip, c_int, dp, dp, c_int, dp, dp, dp, dp, dp, ip]
