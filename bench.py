@@ -240,7 +240,8 @@ def run_config(fl, name, nglob, bc, center, kernel, lda, typ, odiff, rhs, h, L, 
     by = s.solve_bytes(typ)
     out = {"workload": name, "solves_per_s": round(1.0 / el, 3), "ms_per_step": round(el * 1e3, 3),
            "algorithmic_GB": round(by / 1e9, 3), "achieved_GBs": round(by / el / 1e9, 1),
-           "roofline_frac": round(by / el / PEAK_HBM, 4), "path": ["general", "fused-fp64", "fused-fp32", "general+fused-z"][s.path],
+           "roofline_frac": round(by / el / PEAK_HBM, 4), "path": ["general", "fused-fp64", "fused-fp32", "general+fused-z", "general-fp32",
+                    "general-fp32+fused-z"][s.path],
            "setup_s": round(setup_s, 2), "steps": steps}
     if ref is not None:
         a, b = dout.numpy(), ref.numpy()

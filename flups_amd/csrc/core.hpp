@@ -74,13 +74,14 @@ struct Shape {
 
 // a strided view of a 3-D (x components) array of doubles
 struct View {
-    double* p = nullptr;
+    void* p = nullptr;
+    int es = 8;                   // bytes of one real: 8 (double) or 4 (float, FP32 stage arrays)
     int n[3] = {1, 1, 1};
     int nf = 1;
-    long long s[3] = {0, 0, 0};   // stride in elements (an element is nf doubles)
-    long long sc = 0;             // component stride in doubles
+    long long s[3] = {0, 0, 0};   // stride in elements (an element is nf reals)
+    long long sc = 0;             // component stride in reals
 };
-View dense_view(double* p, const Shape& sh);
+View dense_view(void* p, const Shape& sh, int es = 8);
 
 // everything one launch of the generic pencil kernel needs (one component)
 struct PassDesc {
@@ -91,11 +92,12 @@ struct PassDesc {
     int gshift = 0, sshift = 0;
     int M = 1, nst = 0, rad[16] = {0};
     const double* tw = nullptr;     // exp(-2 pi i t / M), t < M, interleaved
-    const double* src = nullptr;
+    const void* src = nullptr;      // reals of the input array: double, or float with src_f32
     int src_nd = 1, src_nf = 1;
     long long src_s[3] = {0, 0, 0};
-    double* dst = nullptr;
+    void* dst = nullptr;            // reals of the output array: double, or float with dst_f32
     int dst_nd = 1, dst_nf = 1;
+    int src_f32 = 0, dst_f32 = 0;   // storage of the pass's input / output (arithmetic is FP64)
     long long dst_s[3] = {0, 0, 0};
     int n1 = 1, n2 = 1;     // extents of the two other dims ((d+1)%3, (d+2)%3)
     int ppb = 1;            // pencils per block
@@ -112,8 +114,9 @@ struct PassDesc {
 // col_rz, real columns along the last dimension, pairs of neighbours along x; row_rz, real rows
 // along x (x the last direction); nin real values per column / row, transform length M
 struct RzArgs {
-    double* d = nullptr;         // field before the transform, in place: (nx, ny, nin) reals, x fastest
-    const double* g = nullptr;   // Green's function on the spectral layout (nx, ny, M/2 + 1) complex
+    void* d = nullptr;           // field before the transform, in place: (nx, ny, nin) reals, x fastest
+    const void* g = nullptr;     // Green's function on the spectral layout (nx, ny, M/2 + 1) complex
+    int f32 = 0;                 // data and Green's function stored as float (FP64 arithmetic)
     const double* tw = nullptr;  // M-point twiddle table
     int nx = 1, ny = 1, nin = 1, M = 1, lda = 1;
     long long s_col = 1;         // reals between rows of a column (nx ny)

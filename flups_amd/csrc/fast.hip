@@ -34,6 +34,7 @@
 #include <map>
 #include <mutex>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "fft_core.hpp"
@@ -119,7 +120,9 @@ __global__ void __launch_bounds__(TX * M / 8) col_fft(ColArgs a) {
 // loaded and stored, the zero half is pruned from the first forward stage.
 // PF = 3: M = 3 Q (the general path's fused last direction), 8 PF values per thread through
 // column_fft_pq, no HALF pruning.
-template <typename R, int M, int TX, int SCHED, bool HALF, int PF = 1>
+// S: storage of the field and of G (R, or float with R = double: the general path's FP32 solves,
+// FP32 stage arrays with FP64 arithmetic)
+template <typename R, int M, int TX, int SCHED, bool HALF, int PF = 1, typename S = R>
 __global__ void __launch_bounds__(TX * M / (8 * PF)) z_fused(ColArgs a) {
     // SCHED 0: G loaded with the first set, second set loaded after the first is stored;
     //       1: both sets prefetched up front; 2: G loaded just before each multiply.
@@ -145,15 +148,15 @@ __global__ void __launch_bounds__(TX * M / (8 * PF)) z_fused(ColArgs a) {
     // uniform bases (scalar registers) + one 32-bit per-lane byte offset for every access;
     // lanes past nx load the last valid column and store nothing
     char* base = reinterpret_cast<char*>(a.dst) +
-                 (size_t)sizeof(C2<R>) * ((long long)blockIdx.y * a.comp_src + (long long)tile * TX);
-    char* b0 = base + (size_t)sizeof(C2<R>) * ((long long)yrow * a.s_other_src);
-    char* b1 = base + (size_t)sizeof(C2<R>) * ((long long)y1 * a.s_other_src);
-    const size_t rstride = (size_t)sizeof(C2<R>) * T * a.s_col;                      // i -> i + 1
-    const unsigned voff = (unsigned)(sizeof(C2<R>) * ((size_t)j * a.s_col + c));     // < 4 GiB: j < M/8
-    const unsigned loff = (unsigned)(sizeof(C2<R>) * ((size_t)j * a.s_col + min(c, a.nx - 1 - tile * TX)));
+                 (size_t)sizeof(C2<S>) * ((long long)blockIdx.y * a.comp_src + (long long)tile * TX);
+    char* b0 = base + (size_t)sizeof(C2<S>) * ((long long)yrow * a.s_other_src);
+    char* b1 = base + (size_t)sizeof(C2<S>) * ((long long)y1 * a.s_other_src);
+    const size_t rstride = (size_t)sizeof(C2<S>) * T * a.s_col;                      // i -> i + 1
+    const unsigned voff = (unsigned)(sizeof(C2<S>) * ((size_t)j * a.s_col + c));     // < 4 GiB: j < M/8
+    const unsigned loff = (unsigned)(sizeof(C2<S>) * ((size_t)j * a.s_col + min(c, a.nx - 1 - tile * TX)));
     constexpr int NL = HALF ? 4 : NV;   // rows loaded / stored per thread
     const char* g = reinterpret_cast<const char*>(a.green) +
-                    sizeof(R) * ((long long)y0 * a.gntx + a.x0 / TX + tile) * (M / 2 + 1) * TX;
+                    sizeof(S) * ((long long)y0 * a.gntx + a.x0 / TX + tile) * (M / 2 + 1) * TX;
     const C2<R>* tw = reinterpret_cast<const C2<R>*>(a.tw);
     const C2<R>* twm = reinterpret_cast<const C2<R>*>(a.twm);
     R gv[NV];
@@ -162,15 +165,15 @@ __global__ void __launch_bounds__(TX * M / (8 * PF)) z_fused(ColArgs a) {
 #pragma unroll
         for (int i = 0; i < NV; ++i) {
             const int kz = j + i * T;
-            gv[i] = *reinterpret_cast<const R*>(g + (unsigned)(sizeof(R) * (min(kz, M - kz) * TX + c)));
+            gv[i] = (R)*reinterpret_cast<const S*>(g + (unsigned)(sizeof(S) * (min(kz, M - kz) * TX + c)));
         }
     };
     auto load = [&](C2<R>* u, const char* b) {
 #pragma unroll
         for (int i = 0; i < NV; ++i) {
             const int row = j + i * T;
-            if (HALF) u[i] = i < 4 ? *reinterpret_cast<const C2<R>*>(b + i * rstride + loff) : C2<R>{0, 0};
-            else u[i] = row < a.nin ? *reinterpret_cast<const C2<R>*>(b + i * rstride + loff) : C2<R>{0, 0};
+            if (HALF) u[i] = i < 4 ? ld2<R, S>(b + i * rstride + loff) : C2<R>{0, 0};
+            else u[i] = row < a.nin ? ld2<R, S>(b + i * rstride + loff) : C2<R>{0, 0};
         }
     };
     if (SCHED != 2) load_g();
@@ -190,7 +193,7 @@ __global__ void __launch_bounds__(TX * M / (8 * PF)) z_fused(ColArgs a) {
 #pragma unroll
             for (int i = 0; i < NL; ++i) {
                 const int row = j + i * T;
-                if (HALF || row < a.nout) *reinterpret_cast<C2<R>*>(b + i * rstride + voff) = v[i];
+                if (HALF || row < a.nout) st2<R, S>(b + i * rstride + voff, v[i]);
             }
         }
         if (!pair) break;   // uniform over the block
@@ -208,7 +211,7 @@ __global__ void __launch_bounds__(TX * M / (8 * PF)) z_fused(ColArgs a) {
 // The general path's fused last direction for M = PF Q with PF = 5 or 7: one row per block (no
 // mirror pairing), TX columns staged in LDS, PF thread teams of Q/8 threads (fft_pq_team) for
 // the forward FFT, the multiply by normfact G and the inverse FFT; in place.
-template <int PF, int Q, int TX>
+template <int PF, int Q, int TX, typename S = double>
 __global__ void __launch_bounds__(TX * PF * Q / 8) z_team(ColArgs a) {
     constexpr int M = PF * Q, T = Q / 8, TT = M / 8;
     __shared__ C2<double> lds[M * TX];
@@ -219,12 +222,12 @@ __global__ void __launch_bounds__(TX * PF * Q / 8) z_team(ColArgs a) {
     const int tile = blk % ntx, y = blk / ntx;
     const bool live = tile * TX + c < a.nx;
     const int cl = min(c, a.nx - 1 - tile * TX);   // lanes past nx load the last valid column
-    C2<double>* col = reinterpret_cast<C2<double>*>(a.dst) + (long long)blockIdx.y * a.comp_src +
-                      (long long)y * a.s_other_src + (long long)tile * TX;
+    C2<S>* col = reinterpret_cast<C2<S>*>(a.dst) + (long long)blockIdx.y * a.comp_src + (long long)y * a.s_other_src +
+                 (long long)tile * TX;
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
         const int r = tt + i * TT;
-        lds[r * TX + c] = r < a.nin ? col[(long long)r * a.s_col + cl] : C2<double>{0.0, 0.0};
+        lds[r * TX + c] = r < a.nin ? c2d(col[(long long)r * a.s_col + cl]) : C2<double>{0.0, 0.0};
     }
     __syncthreads();
     const C2<double>* tw = reinterpret_cast<const C2<double>*>(a.tw);
@@ -232,11 +235,11 @@ __global__ void __launch_bounds__(TX * PF * Q / 8) z_team(ColArgs a) {
     auto load = [&](int m) -> C2<double> { return lds[m * TX + c]; };
     C2<double> v[8];
     fft_pq_team<double, PF, Q, true, TX>(v, lds + c, k1, j, load, tw, twm, 1);
-    const double* g = reinterpret_cast<const double*>(a.green) + ((long long)y * a.gntx + tile) * (M / 2 + 1) * TX;
+    const S* g = reinterpret_cast<const S*>(a.green) + ((long long)y * a.gntx + tile) * (M / 2 + 1) * TX;
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
         const int kz = k1 + PF * (j + i * T);
-        const double gv = g[min(kz, M - kz) * TX + c];
+        const double gv = (double)g[min(kz, M - kz) * TX + c];
         v[i] = {v[i].x * gv, v[i].y * gv};
     }
     __syncthreads();   // the forward FFT's last LDS reads are done
@@ -248,7 +251,7 @@ __global__ void __launch_bounds__(TX * PF * Q / 8) z_team(ColArgs a) {
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
         const int r = k1 + PF * (j + i * T);
-        if (r < a.nout) col[(long long)r * a.s_col + c] = v[i];
+        if (r < a.nout) col[(long long)r * a.s_col + c] = c2t<S>(v[i]);
     }
 }
 
@@ -761,21 +764,28 @@ void destroy_fast_plan(FastPlan* f) {
 
 namespace {
 
-template <typename R, int M, int TX, bool FWD, bool FUSED, bool HALF>
+// S: storage of the z_fused field and G (S = R but for the general path's FP32 solves)
+template <typename R, int M, int TX, bool FWD, bool FUSED, bool HALF, typename S = R>
 void launch_col_h(const ColArgs& a, int comps, int zsched, hipStream_t st) {
-    if (FUSED) {
+    if constexpr (FUSED && !std::is_same<R, S>::value) {
+        // the general path's FP32 fused last direction: full- or half-width tiles, schedule 0
+        const int tx = (zsched & 4) ? TX / 2 : TX;
+        dim3 grid(((a.nx + tx - 1) / tx) * (a.nopair == 1 ? a.nother : a.gmy / 2 + 1), comps);
+        if (zsched & 4) hipLaunchKernelGGL((z_fused<R, M, TX / 2, 0, HALF, 1, S>), grid, dim3(TX / 2 * M / 8), 0, st, a);
+        else hipLaunchKernelGGL((z_fused<R, M, TX, 0, HALF, 1, S>), grid, dim3(TX * M / 8), 0, st, a);
+    } else if (FUSED) {
         // z schedules: bit 0-1 = SCHED of z_fused, 4 = half-width tiles (TX/2 columns)
         const int tx = (zsched & 8) ? TX / 4 : (zsched & 4) ? TX / 2 : TX;
         dim3 grid(((a.nx + tx - 1) / tx) * (a.nopair == 1 ? a.nother : a.nopair == 2 ? 2 * (a.gmy / 2 + 1) : a.gmy / 2 + 1), comps);
         switch (zsched & 15) {
-            case 0: hipLaunchKernelGGL((z_fused<R, M, TX, 0, HALF>), grid, dim3(TX * M / 8), 0, st, a); break;
-            case 1: hipLaunchKernelGGL((z_fused<R, M, TX, 1, HALF>), grid, dim3(TX * M / 8), 0, st, a); break;
-            case 2: hipLaunchKernelGGL((z_fused<R, M, TX, 2, HALF>), grid, dim3(TX * M / 8), 0, st, a); break;
-            case 4: hipLaunchKernelGGL((z_fused<R, M, TX / 2, 0, HALF>), grid, dim3(TX / 2 * M / 8), 0, st, a); break;
-            case 5: hipLaunchKernelGGL((z_fused<R, M, TX / 2, 1, HALF>), grid, dim3(TX / 2 * M / 8), 0, st, a); break;
-            case 6: hipLaunchKernelGGL((z_fused<R, M, TX / 2, 2, HALF>), grid, dim3(TX / 2 * M / 8), 0, st, a); break;
+            case 0: hipLaunchKernelGGL((z_fused<R, M, TX, 0, HALF, 1, S>), grid, dim3(TX * M / 8), 0, st, a); break;
+            case 1: hipLaunchKernelGGL((z_fused<R, M, TX, 1, HALF, 1, S>), grid, dim3(TX * M / 8), 0, st, a); break;
+            case 2: hipLaunchKernelGGL((z_fused<R, M, TX, 2, HALF, 1, S>), grid, dim3(TX * M / 8), 0, st, a); break;
+            case 4: hipLaunchKernelGGL((z_fused<R, M, TX / 2, 0, HALF, 1, S>), grid, dim3(TX / 2 * M / 8), 0, st, a); break;
+            case 5: hipLaunchKernelGGL((z_fused<R, M, TX / 2, 1, HALF, 1, S>), grid, dim3(TX / 2 * M / 8), 0, st, a); break;
+            case 6: hipLaunchKernelGGL((z_fused<R, M, TX / 2, 2, HALF, 1, S>), grid, dim3(TX / 2 * M / 8), 0, st, a); break;
             case 8:
-                hipLaunchKernelGGL((z_fused<R, M, (TX >= 8 ? TX / 4 : 1), 0, HALF>), grid, dim3((TX >= 8 ? TX / 4 : 1) * M / 8), 0, st,
+                hipLaunchKernelGGL((z_fused<R, M, (TX >= 8 ? TX / 4 : 1), 0, HALF, 1, S>), grid, dim3((TX >= 8 ? TX / 4 : 1) * M / 8), 0, st,
                                    a);
                 break;
             default: fail("unknown z schedule %d", zsched);
@@ -792,23 +802,23 @@ void launch_col_h(const ColArgs& a, int comps, int zsched, hipStream_t st) {
 
 // HALF kernels when the column is zero-padded to twice its length (unbounded
 // directions): fused pass nin = nout = M/2, forward nin = M/2, inverse nout = M/2
-template <typename R, int M, int TX, bool FWD, bool FUSED>
+template <typename R, int M, int TX, bool FWD, bool FUSED, typename S = R>
 void launch_col(const ColArgs& a, int comps, int zsched, hipStream_t st) {
     const int h = M / 2;
     const bool half = FUSED ? (a.nin == h && a.nout == h) : FWD ? (a.nin == h && a.nout == M) : (a.nin == M && a.nout == h);
-    if (half) launch_col_h<R, M, TX, FWD, FUSED, true>(a, comps, zsched, st);
-    else launch_col_h<R, M, TX, FWD, FUSED, false>(a, comps, zsched, st);
+    if (half) launch_col_h<R, M, TX, FWD, FUSED, true, S>(a, comps, zsched, st);
+    else launch_col_h<R, M, TX, FWD, FUSED, false, S>(a, comps, zsched, st);
 }
 
-template <typename R, int TX, bool FWD, bool FUSED>
+template <typename R, int TX, bool FWD, bool FUSED, typename S = R>
 void col_dispatch(int M, const ColArgs& a, int comps, int zsched, hipStream_t st) {
     switch (M) {
-        case 32: launch_col<R, 32, TX, FWD, FUSED>(a, comps, zsched, st); break;
-        case 64: launch_col<R, 64, TX, FWD, FUSED>(a, comps, zsched, st); break;
-        case 128: launch_col<R, 128, TX, FWD, FUSED>(a, comps, zsched, st); break;
-        case 256: launch_col<R, 256, TX, FWD, FUSED>(a, comps, zsched, st); break;
-        case 512: launch_col<R, 512, TX, FWD, FUSED>(a, comps, zsched, st); break;
-        case 1024: launch_col<R, 1024, TX / 2, FWD, FUSED>(a, comps, zsched & ~12, st); break;   // already half width
+        case 32: launch_col<R, 32, TX, FWD, FUSED, S>(a, comps, zsched, st); break;
+        case 64: launch_col<R, 64, TX, FWD, FUSED, S>(a, comps, zsched, st); break;
+        case 128: launch_col<R, 128, TX, FWD, FUSED, S>(a, comps, zsched, st); break;
+        case 256: launch_col<R, 256, TX, FWD, FUSED, S>(a, comps, zsched, st); break;
+        case 512: launch_col<R, 512, TX, FWD, FUSED, S>(a, comps, zsched, st); break;
+        case 1024: launch_col<R, 1024, TX / 2, FWD, FUSED, S>(a, comps, zsched & ~12, st); break;   // already half width
         default: fail("unsupported column length %d", M);
     }
 }
@@ -1041,7 +1051,8 @@ struct HybridZ {
     bool xrow = false;       // r2c last direction along x (row_rz): real rows
     void* twn = nullptr;     // row_rz: n-point table
     double normfact = 1;
-    const double* g = nullptr;   // real: the solver's spectral Green's function (not owned)
+    const void* g = nullptr;     // real: the solver's spectral Green's function (not owned)
+    bool f32 = false;            // FP32 stage arrays and Green's function (FP64 arithmetic)
     int pf = 1;              // M = pf 2^k: 1, or 3 (mixed-radix column FFT, 8 pf values per thread)
     bool pair = false, twod = false;
     void* green = nullptr;   // normfact * Re G, [y'][tile][kz'][gtx]
@@ -1053,7 +1064,9 @@ struct HybridZ {
 
 HybridZ* make_hybrid_z(SolverImpl& s) {
     const int nd = s.ndim_;
-    if ((nd != 3 && nd != 2) || s.fp32_) return nullptr;
+    if (nd != 3 && nd != 2) return nullptr;
+    const bool f32 = s.gf32_;           // the general path's FP32 solves: float data and Green's function
+    const double es = f32 ? 4.0 : 8.0;   // bytes of one stored real
     if (getenv("FLUPS_FORCE_GENERIC")) return nullptr;
     const int ip = nd - 1;
     const DirPlan &f = s.fwd_[ip], &b = s.bwd_[ip];
@@ -1073,6 +1086,7 @@ HybridZ* make_hybrid_z(SolverImpl& s) {
         for (int d = 1; d < nd; ++d)
             if (in.n[d] != sp.n[d]) return nullptr;
         auto* h = new HybridZ;
+        h->f32 = f32;
         h->real = true;
         h->xrow = true;
         h->M = M;
@@ -1081,7 +1095,7 @@ HybridZ* make_hybrid_z(SolverImpl& s) {
         h->nz = 1;
         h->lda = s.lda_;
         h->normfact = s.normfact_;
-        h->g = s.dgreen_;
+        h->g = f32 ? (const void*)s.dgreen32_ : (const void*)s.dgreen_;
         int pf = 1, q = 0;
         split_pq(M / 2, pf, q);
         const auto t = twiddle_table(q), tn = twiddle_table(M);
@@ -1096,7 +1110,7 @@ HybridZ* make_hybrid_z(SolverImpl& s) {
             throw;
         }
         h->dev_bytes = (t.size() + tn.size()) * sizeof(double);
-        h->bytes = h->lda * 2.0 * (double)h->nx * h->ny * 8.0 + (double)h->ny * (M / 2 + 1) * 16.0;
+        h->bytes = h->lda * 2.0 * (double)h->nx * h->ny * es + (double)h->ny * (M / 2 + 1) * 2 * es;
         return h;
     }
     if (f.r2c) {
@@ -1106,6 +1120,7 @@ HybridZ* make_hybrid_z(SolverImpl& s) {
         if (in.nf != 1 || sp.nf != 2 || in.n[0] != sp.n[0] || (nd == 3 && in.n[1] != sp.n[1])) return nullptr;
         if (!pow2_in(M, 32, 1024) || sp.n[nd - 1] != M / 2 + 1 || in.n[nd - 1] > M || !s.dgreen_) return nullptr;
         auto* h = new HybridZ;
+        h->f32 = f32;
         h->real = true;
         h->M = M;
         h->nx = in.n[0];
@@ -1113,7 +1128,7 @@ HybridZ* make_hybrid_z(SolverImpl& s) {
         h->nz = in.n[nd - 1];
         h->lda = s.lda_;
         h->normfact = s.normfact_;
-        h->g = s.dgreen_;
+        h->g = f32 ? (const void*)s.dgreen32_ : (const void*)s.dgreen_;
         const auto t = twiddle_table(M);
         try {
             HIP_CHECK(hipMalloc(&h->tw, t.size() * sizeof(double)));
@@ -1124,7 +1139,7 @@ HybridZ* make_hybrid_z(SolverImpl& s) {
         }
         h->dev_bytes = t.size() * sizeof(double);
         // the field in and out, the Green's function's (M/2 + 1) rows of complex values
-        h->bytes = h->lda * 2.0 * (double)h->nx * h->ny * h->nz * 8.0 + (double)h->nx * h->ny * (M / 2 + 1) * 16.0;
+        h->bytes = h->lda * 2.0 * (double)h->nx * h->ny * h->nz * es + (double)h->nx * h->ny * (M / 2 + 1) * 2 * es;
         return h;
     }
     if (in.nf != 2 || sp.nf != 2 || in.n[0] != sp.n[0] || (nd == 3 && in.n[1] != sp.n[1])) return nullptr;
@@ -1135,6 +1150,7 @@ HybridZ* make_hybrid_z(SolverImpl& s) {
     const int ny = nd == 3 ? in.n[1] : 1;
     if ((double)(M / 8) * in.n[0] * ny * 16.0 >= 4294967296.0) return nullptr;
     auto* h = new HybridZ;
+    h->f32 = f32;
     h->M = M;
     h->nx = in.n[0];
     h->ny = ny;
@@ -1159,10 +1175,10 @@ HybridZ* make_hybrid_z(SolverImpl& s) {
     const int gy = h->pair ? h->ny / 2 + 1 : h->ny, gz = M / 2 + 1;
     const size_t gn = (size_t)gy * ntx * gz * h->gtx;
     try {
-        HIP_CHECK(hipMalloc(&h->green, gn * sizeof(double)));
-        h->dev_bytes += gn * sizeof(double);
-        hipLaunchKernelGGL(compress_green, dim3(4096), dim3(256), 0, s.stream_, s.dgreen_, (double*)h->green, (float*)nullptr,
-                           h->nx, h->ny, M, gy, gz, h->gtx, ntx, s.normfact_);
+        HIP_CHECK(hipMalloc(&h->green, gn * (size_t)es));
+        h->dev_bytes += gn * (size_t)es;
+        hipLaunchKernelGGL(compress_green, dim3(4096), dim3(256), 0, s.stream_, s.dgreen_, f32 ? nullptr : (double*)h->green,
+                           f32 ? (float*)h->green : nullptr, h->nx, h->ny, M, gy, gz, h->gtx, ntx, s.normfact_);
         HIP_CHECK(hipGetLastError());
         const auto t = twiddle_table(M / h->pf);
         HIP_CHECK(hipMalloc(&h->tw, t.size() * sizeof(double)));
@@ -1182,8 +1198,8 @@ HybridZ* make_hybrid_z(SolverImpl& s) {
         delete h;
         throw;
     }
-    const double cnt = (double)h->nx * h->ny * h->nz * 16.0;
-    h->bytes = h->lda * 2.0 * cnt + (double)gy * gz * h->nx * 8.0;
+    const double cnt = (double)h->nx * h->ny * h->nz * 2 * es;
+    h->bytes = h->lda * 2.0 * cnt + (double)gy * gz * h->nx * es;
     return h;
 }
 
@@ -1199,67 +1215,69 @@ size_t hybrid_device_bytes(const HybridZ* h) { return h->dev_bytes; }
 double hybrid_bytes(const HybridZ* h) { return h->bytes; }
 
 // M = 3 Q: TX columns per block (the compressed G's tile width), Q/8 threads per column
-template <int M, int TX>
+template <int M, int TX, typename S>
 void launch_z_pf3(const ColArgs& a, int comps, hipStream_t st) {
     dim3 grid(((a.nx + TX - 1) / TX) * (a.nopair == 1 ? a.nother : a.gmy / 2 + 1), comps);
-    hipLaunchKernelGGL((z_fused<double, M, TX, 0, false, 3>), grid, dim3(TX * M / 24), 0, st, a);
+    hipLaunchKernelGGL((z_fused<double, M, TX, 0, false, 3, S>), grid, dim3(TX * M / 24), 0, st, a);
     HIP_CHECK(hipGetLastError());
 }
-template <int TX>
+template <int TX, typename S>
 void dispatch_z_pf3(int M, const ColArgs& a, int comps, hipStream_t st) {
     switch (M) {
-        case 24: launch_z_pf3<24, TX>(a, comps, st); break;
-        case 48: launch_z_pf3<48, TX>(a, comps, st); break;
+        case 24: launch_z_pf3<24, TX, S>(a, comps, st); break;
+        case 48: launch_z_pf3<48, TX, S>(a, comps, st); break;
         case 96:
-            if constexpr (TX <= 16) launch_z_pf3<96, TX>(a, comps, st);
+            if constexpr (TX <= 16) launch_z_pf3<96, TX, S>(a, comps, st);
             else fail("column length %d with tiles of %d", M, TX);
             break;
         case 192:
-            if constexpr (TX <= 16) launch_z_pf3<192, TX>(a, comps, st);
+            if constexpr (TX <= 16) launch_z_pf3<192, TX, S>(a, comps, st);
             else fail("column length %d with tiles of %d", M, TX);
             break;
         case 384:
-            if constexpr (TX <= 8) launch_z_pf3<384, TX>(a, comps, st);
+            if constexpr (TX <= 8) launch_z_pf3<384, TX, S>(a, comps, st);
             else fail("column length %d with tiles of %d", M, TX);
             break;
         default: fail("unsupported column length %d", M);
     }
 }
 
-template <int PF, int Q>
+template <int PF, int Q, typename S>
 void launch_z_team(int tx, const ColArgs& a, int comps, hipStream_t st) {
     const int ntx = (a.nx + tx - 1) / tx;
     dim3 grid(ntx * a.nother, comps);
     switch (tx) {
         case 16:
-            if constexpr (PF * Q <= 192) hipLaunchKernelGGL((z_team<PF, Q, 16>), grid, dim3(16 * PF * Q / 8), 0, st, a);
+            if constexpr (PF * Q <= 192) hipLaunchKernelGGL((z_team<PF, Q, 16, S>), grid, dim3(16 * PF * Q / 8), 0, st, a);
             else fail("column length %d with tiles of 16", PF * Q);
             break;
         case 8:
-            if constexpr (PF * Q <= 384) hipLaunchKernelGGL((z_team<PF, Q, 8>), grid, dim3(8 * PF * Q / 8), 0, st, a);
+            if constexpr (PF * Q <= 384) hipLaunchKernelGGL((z_team<PF, Q, 8, S>), grid, dim3(8 * PF * Q / 8), 0, st, a);
             else fail("column length %d with tiles of 8", PF * Q);
             break;
-        default: hipLaunchKernelGGL((z_team<PF, Q, 4>), grid, dim3(4 * PF * Q / 8), 0, st, a); break;
+        default: hipLaunchKernelGGL((z_team<PF, Q, 4, S>), grid, dim3(4 * PF * Q / 8), 0, st, a); break;
     }
     HIP_CHECK(hipGetLastError());
 }
-template <int PF>
+template <int PF, typename S>
 void dispatch_z_team(int M, int tx, const ColArgs& a, int comps, hipStream_t st) {
     switch (M / PF) {
-        case 8: launch_z_team<PF, 8>(tx, a, comps, st); break;
-        case 16: launch_z_team<PF, 16>(tx, a, comps, st); break;
-        case 32: launch_z_team<PF, 32>(tx, a, comps, st); break;
-        case 64: launch_z_team<PF, 64>(tx, a, comps, st); break;
-        case 128: launch_z_team<PF, 128>(tx, a, comps, st); break;
+        case 8: launch_z_team<PF, 8, S>(tx, a, comps, st); break;
+        case 16: launch_z_team<PF, 16, S>(tx, a, comps, st); break;
+        case 32: launch_z_team<PF, 32, S>(tx, a, comps, st); break;
+        case 64: launch_z_team<PF, 64, S>(tx, a, comps, st); break;
+        case 128: launch_z_team<PF, 128, S>(tx, a, comps, st); break;
         default: fail("unsupported column length %d", M);
     }
 }
 
 // in place on the stage array before the z transform: (nx, ny, nz) complex (real for an r2c
-// last direction), lda components
-void hybrid_z(const HybridZ* h, double* data, hipStream_t st) {
+// last direction), lda components; S = the storage of the stage array (float for FP32 solves)
+template <typename S>
+void hybrid_z_t(const HybridZ* h, void* data, hipStream_t st) {
     if (h->xrow) {
         RzArgs A;
+        A.f32 = std::is_same<S, float>::value;
         A.d = data;
         A.g = h->g;
         A.tw = static_cast<const double*>(h->tw);
@@ -1275,6 +1293,7 @@ void hybrid_z(const HybridZ* h, double* data, hipStream_t st) {
     }
     if (h->real) {
         RzArgs A;
+        A.f32 = std::is_same<S, float>::value;
         A.d = data;
         A.g = h->g;
         A.tw = static_cast<const double*>(h->tw);
@@ -1314,33 +1333,37 @@ void hybrid_z(const HybridZ* h, double* data, hipStream_t st) {
     z.gntx = (h->nx + h->gtx - 1) / h->gtx;
     z.twm = h->twm;
     if (h->pf >= 5) {
-        if (h->pf == 5) dispatch_z_team<5>(h->M, h->gtx, z, h->lda, st);
-        else dispatch_z_team<7>(h->M, h->gtx, z, h->lda, st);
+        if (h->pf == 5) dispatch_z_team<5, S>(h->M, h->gtx, z, h->lda, st);
+        else dispatch_z_team<7, S>(h->M, h->gtx, z, h->lda, st);
         return;
     }
     if (h->pf == 3) {
         switch (h->gtx) {
-            case 32: dispatch_z_pf3<32>(h->M, z, h->lda, st); break;
-            case 16: dispatch_z_pf3<16>(h->M, z, h->lda, st); break;
-            default: dispatch_z_pf3<8>(h->M, z, h->lda, st); break;
+            case 32: dispatch_z_pf3<32, S>(h->M, z, h->lda, st); break;
+            case 16: dispatch_z_pf3<16, S>(h->M, z, h->lda, st); break;
+            default: dispatch_z_pf3<8, S>(h->M, z, h->lda, st); break;
         }
         return;
     }
     if (h->twod) {   // two columns per block (the tile width of the compressed G), XCD-grouped
         switch (h->M) {
-            case 32: launch_col<double, 32, 2, true, true>(z, h->lda, 16, st); break;
-            case 64: launch_col<double, 64, 2, true, true>(z, h->lda, 16, st); break;
-            case 128: launch_col<double, 128, 2, true, true>(z, h->lda, 16, st); break;
-            case 256: launch_col<double, 256, 2, true, true>(z, h->lda, 16, st); break;
-            case 512: launch_col<double, 512, 2, true, true>(z, h->lda, 16, st); break;
-            case 1024: launch_col<double, 1024, 2, true, true>(z, h->lda, 16, st); break;
-            case 2048: launch_col<double, 2048, 2, true, true>(z, h->lda, 16, st); break;
+            case 32: launch_col<double, 32, 2, true, true, S>(z, h->lda, 16, st); break;
+            case 64: launch_col<double, 64, 2, true, true, S>(z, h->lda, 16, st); break;
+            case 128: launch_col<double, 128, 2, true, true, S>(z, h->lda, 16, st); break;
+            case 256: launch_col<double, 256, 2, true, true, S>(z, h->lda, 16, st); break;
+            case 512: launch_col<double, 512, 2, true, true, S>(z, h->lda, 16, st); break;
+            case 1024: launch_col<double, 1024, 2, true, true, S>(z, h->lda, 16, st); break;
+            case 2048: launch_col<double, 2048, 2, true, true, S>(z, h->lda, 16, st); break;
             default: fail("unsupported column length %d", h->M);
         }
         return;
     }
     // half-width tiles (schedule bit 2), XCD-grouped; M = 1024 is already half width
-    col_dispatch<double, 8, true, true>(h->M, z, h->lda, 4 | 16, st);
+    col_dispatch<double, 8, true, true, S>(h->M, z, h->lda, 4 | 16, st);
+}
+void hybrid_z(const HybridZ* h, void* data, hipStream_t st) {
+    if (h->f32) hybrid_z_t<float>(h, data, st);
+    else hybrid_z_t<double>(h, data, st);
 }
 
 double fast_bytes(const FastPlan* f) { return f->bytes[0] + f->bytes[1] + f->bytes[2] + f->bytes[3] + f->bytes[4]; }

@@ -11,6 +11,35 @@ template <typename R>
 struct C2 {
     R x, y;
 };
+// storage conversions: complex values loaded from T and computed in double, rounded once on store
+template <typename T>
+__device__ __forceinline__ C2<double> c2d(C2<T> a) { return {(double)a.x, (double)a.y}; }
+template <typename T>
+__device__ __forceinline__ C2<T> c2t(C2<double> a) { return {(T)a.x, (T)a.y}; }
+// complex value stored as S at byte address p, computed as R
+template <typename R, typename S>
+__device__ __forceinline__ C2<R> ld2(const char* p) {
+    const C2<S> a = *reinterpret_cast<const C2<S>*>(p);
+    return {(R)a.x, (R)a.y};
+}
+template <typename R, typename S>
+__device__ __forceinline__ void st2(char* p, C2<R> v) {
+    *reinterpret_cast<C2<S>*>(p) = C2<S>{(S)v.x, (S)v.y};
+}
+// two consecutive reals of type T as one vector access (16 bytes for double, 8 for float)
+template <typename T>
+struct Vec2;
+template <>
+struct Vec2<double> {
+    using type = double2;
+};
+template <>
+struct Vec2<float> {
+    using type = float2;
+};
+template <typename T>
+using vec2_t = typename Vec2<T>::type;
+
 template <typename R>
 __device__ __forceinline__ C2<R> cadd(C2<R> a, C2<R> b) { return {a.x + b.x, a.y + b.y}; }
 template <typename R>

@@ -120,6 +120,9 @@ __device__ void stockham_stage(const cd* in, cd* out, int M, int p, int span, co
 
 }  // namespace
 
+// TI / TO: storage of the source / destination reals (double, or float for the stage arrays of
+// FP32 solves); the pencil buffer and the arithmetic are FP64
+template <typename TI, typename TO>
 __global__ void __launch_bounds__(256) generic_pass(PassDesc P) {
     extern __shared__ double smem[];
     const int len = P.pen_in * P.nf_in;   // doubles per pencil (same before and after)
@@ -148,8 +151,8 @@ __global__ void __launch_bounds__(256) generic_pass(PassDesc P) {
             double* o = pb + (size_t)pen * len + (size_t)k * nfi;
             if (ks >= 0 && ks < P.src_nd) {
                 const long long idx = (long long)ks * P.src_s[d] + (long long)ja * P.src_s[P.fa] + (long long)jb * P.src_s[P.fb];
-                const double* sp = P.src + idx * P.src_nf;
-                for (int c = 0; c < nfi; ++c) o[c] = sp[c];
+                const TI* sp = reinterpret_cast<const TI*>(P.src) + idx * P.src_nf;
+                for (int c = 0; c < nfi; ++c) o[c] = (double)sp[c];
             } else {
                 for (int c = 0; c < nfi; ++c) o[c] = 0.0;
             }
@@ -312,8 +315,8 @@ __global__ void __launch_bounds__(256) generic_pass(PassDesc P) {
             const int ja = g % P.na, jb = g / P.na;
             const double* v = pb + (size_t)pen * len + (size_t)kt * P.nf_out;
             const long long idx = (long long)k * P.dst_s[d] + (long long)ja * P.dst_s[P.fa] + (long long)jb * P.dst_s[P.fb];
-            double* dp = P.dst + idx * nfo;
-            for (int c = 0; c < nfo; ++c) dp[c] = v[c];
+            TO* dp = reinterpret_cast<TO*>(P.dst) + idx * nfo;
+            for (int c = 0; c < nfo; ++c) dp[c] = (TO)v[c];
         }
     }
 }
@@ -334,7 +337,10 @@ void launch_generic(const PassDesc& P, hipStream_t s) {
     // threads: enough for one element (or butterfly) each of the block's pencils, whole waves, at most 256
     const int work = P.ppb * std::max(P.M, P.pen_in);
     const int threads = std::min(256, std::max(64, (work + 63) / 64 * 64));
-    hipLaunchKernelGGL(generic_pass, dim3(blocks), dim3(threads), lds, s, P);
+    if (!P.src_f32 && !P.dst_f32) hipLaunchKernelGGL((generic_pass<double, double>), dim3(blocks), dim3(threads), lds, s, P);
+    else if (!P.src_f32) hipLaunchKernelGGL((generic_pass<double, float>), dim3(blocks), dim3(threads), lds, s, P);
+    else if (P.dst_f32) hipLaunchKernelGGL((generic_pass<float, float>), dim3(blocks), dim3(threads), lds, s, P);
+    else hipLaunchKernelGGL((generic_pass<float, double>), dim3(blocks), dim3(threads), lds, s, P);
     HIP_CHECK(hipGetLastError());
 }
 

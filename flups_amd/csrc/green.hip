@@ -605,22 +605,29 @@ __global__ void scale_kernel(double* g, size_t n, double f) {
 // the fully spectral zero mode after the 2-unbounded/1-spectral completion (green_functions.cpp:703-706)
 __global__ void zero_first(double* g) { g[0] = 0.0; }
 
-__global__ void mult_std(double* d, const double* g, size_t per, int nf, int lda, double normfact) {
+// S: storage of the data and the Green's function (double, or float for FP32 solves); FP64 arithmetic
+template <typename S>
+__global__ void mult_std(S* d, const S* g, size_t per, int nf, int lda, double normfact) {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < per; i += (size_t)gridDim.x * blockDim.x) {
         for (int c = 0; c < lda; ++c) {
-            double* x = d + c * per * nf;
+            S* x = d + c * per * nf;
             if (nf == 1) {
-                x[i] *= normfact * g[i];
+                x[i] = (S)((double)x[i] * normfact * (double)g[i]);
             } else {
                 const double a = x[2 * i], b = x[2 * i + 1], gr = g[2 * i], gi = g[2 * i + 1];
-                x[2 * i] = normfact * (a * gr - b * gi);
-                x[2 * i + 1] = normfact * (a * gi + b * gr);
+                x[2 * i] = (S)(normfact * (a * gr - b * gi));
+                x[2 * i + 1] = (S)(normfact * (a * gi + b * gr));
             }
         }
     }
 }
 
-__global__ void mult_rot(double* d, const double* g, RotArgs R) {
+__global__ void to_float(const double* a, float* b, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) b[i] = (float)a[i];
+}
+
+template <typename S>
+__global__ void mult_rot(S* d, const S* g, RotArgs R) {
     const size_t per = (size_t)R.n[0] * R.n[1] * R.n[2];
     for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < per; e += (size_t)gridDim.x * blockDim.x) {
         const int i0 = (int)(e % R.n[0]), i1 = (int)((e / R.n[0]) % R.n[1]), i2 = (int)(e / ((size_t)R.n[0] * R.n[1]));
@@ -648,20 +655,20 @@ __global__ void mult_rot(double* d, const double* g, RotArgs R) {
             }
         double fr[3], fi[3];
         for (int c = 0; c < 3; ++c) {
-            fr[c] = d[(c * per + e) * R.nf];
-            fi[c] = R.nf == 2 ? d[(c * per + e) * 2 + 1] : 0.0;
+            fr[c] = (double)d[(c * per + e) * R.nf];
+            fi[c] = R.nf == 2 ? (double)d[(c * per + e) * 2 + 1] : 0.0;
         }
         auto dr = [&](int c, int dd) { return fr[c] * Kr[dd][c] - fi[c] * Ki[dd][c]; };
         auto di = [&](int c, int dd) { return fr[c] * Ki[dd][c] + fi[c] * Kr[dd][c]; };
         const double rr[3] = {dr(2, 1) - dr(1, 2), dr(0, 2) - dr(2, 0), dr(1, 0) - dr(0, 1)};
         const double ri[3] = {di(2, 1) - di(1, 2), di(0, 2) - di(2, 0), di(1, 0) - di(0, 1)};
         if (R.nf == 1) {
-            for (int c = 0; c < 3; ++c) d[c * per + e] = R.normfact * rr[c] * g[e];
+            for (int c = 0; c < 3; ++c) d[c * per + e] = (S)(R.normfact * rr[c] * (double)g[e]);
         } else {
             const double gr = g[2 * e], gi = g[2 * e + 1];
             for (int c = 0; c < 3; ++c) {
-                d[(c * per + e) * 2] = R.normfact * (rr[c] * gr - ri[c] * gi);
-                d[(c * per + e) * 2 + 1] = R.normfact * (rr[c] * gi + ri[c] * gr);
+                d[(c * per + e) * 2] = (S)(R.normfact * (rr[c] * gr - ri[c] * gi));
+                d[(c * per + e) * 2 + 1] = (S)(R.normfact * (rr[c] * gi + ri[c] * gr));
             }
         }
     }
@@ -700,14 +707,27 @@ void launch_scale(double* g, size_t n, double f, hipStream_t s) {
     HIP_CHECK(hipGetLastError());
 }
 
-void launch_mult_std(double* d, const double* g, size_t per, int nf, int lda, double normfact, hipStream_t s) {
-    hipLaunchKernelGGL(mult_std, dim3(grid_for(per)), dim3(256), 0, s, d, g, per, nf, lda, normfact);
+// f32: the data and the Green's function are float arrays (FP32 solves), else double
+void launch_mult_std(void* d, const void* g, size_t per, int nf, int lda, double normfact, bool f32, hipStream_t s) {
+    if (f32)
+        hipLaunchKernelGGL((mult_std<float>), dim3(grid_for(per)), dim3(256), 0, s, (float*)d, (const float*)g, per, nf, lda,
+                           normfact);
+    else
+        hipLaunchKernelGGL((mult_std<double>), dim3(grid_for(per)), dim3(256), 0, s, (double*)d, (const double*)g, per, nf,
+                           lda, normfact);
     HIP_CHECK(hipGetLastError());
 }
 
-void launch_mult_rot(double* d, const double* g, const RotArgs& R, hipStream_t s) {
+void launch_mult_rot(void* d, const void* g, const RotArgs& R, bool f32, hipStream_t s) {
     const size_t per = (size_t)R.n[0] * R.n[1] * R.n[2];
-    hipLaunchKernelGGL(mult_rot, dim3(grid_for(per)), dim3(256), 0, s, d, g, R);
+    if (f32) hipLaunchKernelGGL((mult_rot<float>), dim3(grid_for(per)), dim3(256), 0, s, (float*)d, (const float*)g, R);
+    else hipLaunchKernelGGL((mult_rot<double>), dim3(grid_for(per)), dim3(256), 0, s, (double*)d, (const double*)g, R);
+    HIP_CHECK(hipGetLastError());
+}
+
+// b[i] = (float)a[i]: FP32 copies of FP64 Green's functions (rounded once)
+void launch_to_float(const double* a, float* b, size_t n, hipStream_t s) {
+    hipLaunchKernelGGL(to_float, dim3(grid_for(n)), dim3(256), 0, s, a, b, n);
     HIP_CHECK(hipGetLastError());
 }
 

@@ -32,7 +32,9 @@ struct RotArgs {
 
 void launch_green_fill(int nunb, double* g, const int n[3], int nf, int axis, const GreenArgs& A, const int* lo, hipStream_t s);
 void launch_scale(double* g, size_t n, double f, hipStream_t s);
-void launch_mult_std(double* d, const double* g, size_t per, int nf, int lda, double normfact, hipStream_t s);
-void launch_mult_rot(double* d, const double* g, const RotArgs& R, hipStream_t s);
+// f32: data and Green's function stored as float (FP32 solves of the general path), FP64 arithmetic
+void launch_mult_std(void* d, const void* g, size_t per, int nf, int lda, double normfact, bool f32, hipStream_t s);
+void launch_mult_rot(void* d, const void* g, const RotArgs& R, bool f32, hipStream_t s);
+void launch_to_float(const double* a, float* b, size_t n, hipStream_t s);
 
 }  // namespace flups

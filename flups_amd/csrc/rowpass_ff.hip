@@ -1,0 +1,10 @@
+// Fast pass kernels of the general path for float stage input and float output (FP32 solves; see
+// rowpass_kernels.hpp).
+#include "rowpass_kernels.hpp"
+
+namespace flups {
+
+FLUPS_ROWPASS_IO(rowpass_ff, float, float)
+FLUPS_RZ(rz_f, float)
+
+}  // namespace flups

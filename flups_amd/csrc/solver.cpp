@@ -56,7 +56,7 @@ HybridZ* make_hybrid_z(SolverImpl& s);
 void destroy_hybrid_z(HybridZ* h);
 size_t hybrid_device_bytes(const HybridZ* h);
 double hybrid_bytes(const HybridZ* h);
-void hybrid_z(const HybridZ* h, double* data, hipStream_t st);
+void hybrid_z(const HybridZ* h, void* data, hipStream_t st);
 void fast_schedule(const FastPlan* f, int out[3]);
 double fast_bytes(const FastPlan* f);
 void fast_autotune(FastPlan* f, double* out, const double* in, const View& uin, const View& uout, const double* dgreen,
@@ -129,9 +129,10 @@ void Timers::display(FILE* f) {
     std::fflush(f);
 }
 
-View dense_view(double* p, const Shape& sh) {
+View dense_view(void* p, const Shape& sh, int es) {
     View v;
     v.p = p;
+    v.es = es;
     for (int d = 0; d < 3; ++d) v.n[d] = sh.n[d];
     v.nf = sh.nf;
     v.s[0] = 1;
@@ -253,8 +254,9 @@ SolverImpl::~SolverImpl() {
     if (hybrid_) destroy_hybrid_z(hybrid_);
     for (auto& kv : tw_) (void)hipFree(kv.second);
     for (auto& kv : r2r_tab_) (void)hipFree(kv.second);
-    for (double* p : dstage_) if (p) (void)hipFree(p);
+    for (void* p : dstage_) if (p) (void)hipFree(p);
     if (dgreen_) (void)hipFree(dgreen_);
+    if (dgreen32_) (void)hipFree(dgreen32_);
     if (dstaging_) (void)hipFree(dstaging_);
     if (dinner_) (void)hipFree(dinner_);
     if (gexec_) (void)hipGraphExecDestroy(gexec_);
@@ -361,10 +363,12 @@ PassDesc SolverImpl::pass_desc(const DirPlan& p, int comp, bool forward, const V
         M = P.n;
     }
     plan_fft(P, M);
-    P.src = src.p + comp * src.sc;
+    P.src = static_cast<const char*>(src.p) + (size_t)src.es * comp * src.sc;
+    P.src_f32 = src.es == 4;
     P.src_nd = src.n[d];
     P.src_nf = src.nf;
-    P.dst = dst.p + comp * dst.sc;
+    P.dst = static_cast<char*>(dst.p) + (size_t)dst.es * comp * dst.sc;
+    P.dst_f32 = dst.es == 4;
     P.dst_nd = dst.n[d];
     P.dst_nf = dst.nf;
     for (int i = 0; i < 3; ++i) { P.src_s[i] = src.s[i]; P.dst_s[i] = dst.s[i]; }
@@ -389,7 +393,8 @@ PassDesc SolverImpl::pass_desc(const DirPlan& p, int comp, bool forward, const V
         const int fa = P.fa, fb = P.fb;
         const bool plain = P.gshift == 0 && P.sshift == 0 && P.src_nd == P.n && P.dst_nd == P.n && P.pen_in == P.n &&
                            P.si == 0 && P.so == 0 && P.post == 0 && P.kind != REDFT00 && P.kind != RODFT00;
-        P.vec = plain && ((uintptr_t)P.src % 16 == 0) && ((uintptr_t)P.dst % 16 == 0) && P.src_s[fa] % 2 == 0 &&
+        // pairs of reals as one vector access: 16-byte (double) or 8-byte (float) aligned
+        P.vec = plain && ((uintptr_t)P.src % (2 * src.es) == 0) && ((uintptr_t)P.dst % (2 * dst.es) == 0) && P.src_s[fa] % 2 == 0 &&
                 P.src_s[fb] % 2 == 0 && P.dst_s[fa] % 2 == 0 && P.dst_s[fb] % 2 == 0;
         return P;
     }
@@ -440,9 +445,10 @@ PassDesc SolverImpl::pass_desc(const DirPlan& p, int comp, bool forward, const V
         P.tw = twiddles(q);
         P.aux = twiddles(P.n);
         const bool fwd = P.op == OP_R2C;
-        const double* real = fwd ? P.src : P.dst;
+        const void* real = fwd ? P.src : P.dst;
         const long long* rs = fwd ? P.src_s : P.dst_s;
-        P.vec = ((uintptr_t)real % 16 == 0) && rs[P.d] % 2 == 0 && rs[P.fb] % 2 == 0 && P.na % 2 == 0;
+        const int res = fwd ? src.es : dst.es;
+        P.vec = ((uintptr_t)real % (2 * res) == 0) && rs[P.d] % 2 == 0 && rs[P.fb] % 2 == 0 && P.na % 2 == 0;
         return P;
     }
     P.alias = generic_alias(P) ? 1 : 0;
@@ -709,11 +715,26 @@ void SolverImpl::setup() {
         HIP_CHECK(hipFree(dgreen_));
         dgreen_ = nullptr;
     }
+    // FP32 requests the fused pipeline cannot run: the general path with FP32 stage arrays and an
+    // FP32 copy of the Green's function (FP64 arithmetic in the kernels); the FP64 function is
+    // released once the fused last direction has taken its compressed copy
+    gf32_ = fp32_ && !fast_;
+    ses_ = gf32_ ? 4 : 8;
+    if (gf32_) {
+        const size_t gn = gpost_[ndim_ - 1].count();
+        HIP_CHECK(hipMalloc(&dgreen32_, gn * sizeof(float)));
+        launch_to_float(dgreen_, dgreen32_, gn, stream_);
+    }
     if (!fast_) hybrid_ = make_hybrid_z(*this);
+    if (gf32_) {
+        HIP_CHECK(hipStreamSynchronize(stream_));
+        HIP_CHECK(hipFree(dgreen_));
+        dgreen_ = nullptr;
+    }
     if (!fast_) {
         for (int ip = 0; ip < ndim_; ++ip) {
             const size_t n = std::max(hat_[ip].count(), stage_[ip].count());
-            HIP_CHECK(hipMalloc(&dstage_[ip], n * sizeof(double)));
+            HIP_CHECK(hipMalloc(&dstage_[ip], n * ses_));
         }
         warm_tables();
         warmed_ = true;
@@ -726,20 +747,21 @@ void SolverImpl::setup() {
 
 // ---------------------------------------------------------------------------
 // forward transforms: user layout (uin) -> dense spectral array `spec`
-void SolverImpl::forward_passes(const View& uin, double* spec) {
+void SolverImpl::forward_passes(const View& uin, void* spec) {
     for (int ip = 0; ip < ndim_; ++ip) {
         const DirPlan& p = fwd_[ip];
-        const View src = (ip == 0) ? uin : dense_view(dstage_[ip - 1], stage_[ip - 1]);
-        const View dst = dense_view(ip == ndim_ - 1 ? spec : dstage_[ip], stage_[ip]);
+        const View src = (ip == 0) ? uin : dense_view(dstage_[ip - 1], stage_[ip - 1], ses_);
+        const View dst = dense_view(ip == ndim_ - 1 ? spec : dstage_[ip], stage_[ip], ses_);
         for (int c = 0; c < lda_; ++c) run_pass(pass_desc(p, c, true, src, dst, hat_[ip], stage_[ip], p.fieldstart, 0));
     }
 }
 
 // multiply the dense spectral array by normfact * G (STD) or take the curl of G * data (ROT)
-void SolverImpl::multiply(double* spec, int type) {
+void SolverImpl::multiply(void* spec, int type) {
     const Shape& sp = stage_[ndim_ - 1];
+    const void* g = gf32_ ? (const void*)dgreen32_ : (const void*)dgreen_;
     if (type == 0) {
-        launch_mult_std(spec, dgreen_, (size_t)sp.n[0] * sp.n[1] * sp.n[2], sp.nf, lda_, normfact_, stream_);
+        launch_mult_std(spec, g, (size_t)sp.n[0] * sp.n[1] * sp.n[2], sp.nf, lda_, normfact_, gf32_, stream_);
         return;
     }
     RotArgs R;
@@ -757,17 +779,17 @@ void SolverImpl::multiply(double* spec, int type) {
             R.kf[d][c][1] = ph == 0 ? kabs[d] : 0.0;
         }
     }
-    launch_mult_rot(spec, dgreen_, R, stream_);
+    launch_mult_rot(spec, g, R, gf32_, stream_);
 }
 
 // backward transforms: dense spectral array `spec` -> user layout (uout); `diff`
 // selects the plans of the differentiated field (after a ROT multiply)
-void SolverImpl::backward_passes(const double* spec, const View& uout, bool diff) {
+void SolverImpl::backward_passes(const void* spec, const View& uout, bool diff) {
     const DirPlan* back = diff ? bwdd_ : bwd_;
     for (int ip = ndim_ - 1; ip >= 0; --ip) {
         const DirPlan& p = back[ip];
-        const View src = dense_view(ip == ndim_ - 1 ? const_cast<double*>(spec) : dstage_[ip], stage_[ip]);
-        const View dst = (ip == 0) ? uout : dense_view(dstage_[ip - 1], stage_[ip - 1]);
+        const View src = dense_view(ip == ndim_ - 1 ? const_cast<void*>(spec) : dstage_[ip], stage_[ip], ses_);
+        const View dst = (ip == 0) ? uout : dense_view(dstage_[ip - 1], stage_[ip - 1], ses_);
         for (int c = 0; c < lda_; ++c)
             run_pass(pass_desc(p, c, false, src, dst, stage_[ip], hat_[ip], 0, fwd_[ip].fieldstart));
     }
@@ -779,8 +801,8 @@ void SolverImpl::solve_generic(double* field, const double* rhs, int type) {
         const View uin = user_view(const_cast<double*>(rhs)), uout = user_view(field);
         for (int ip = 0; ip < ndim_ - 1; ++ip) {
             const DirPlan& p = fwd_[ip];
-            const View src = (ip == 0) ? uin : dense_view(dstage_[ip - 1], stage_[ip - 1]);
-            const View dst = dense_view(dstage_[ip], stage_[ip]);
+            const View src = (ip == 0) ? uin : dense_view(dstage_[ip - 1], stage_[ip - 1], ses_);
+            const View dst = dense_view(dstage_[ip], stage_[ip], ses_);
             for (int c = 0; c < lda_; ++c) run_pass(pass_desc(p, c, true, src, dst, hat_[ip], stage_[ip], p.fieldstart, 0));
         }
         pmark("fftw forward");
@@ -788,8 +810,8 @@ void SolverImpl::solve_generic(double* field, const double* rhs, int type) {
         pmark("fftw + domagic (last direction fused)");
         for (int ip = ndim_ - 2; ip >= 0; --ip) {
             const DirPlan& p = bwd_[ip];
-            const View src = dense_view(dstage_[ip], stage_[ip]);
-            const View dst = (ip == 0) ? uout : dense_view(dstage_[ip - 1], stage_[ip - 1]);
+            const View src = dense_view(dstage_[ip], stage_[ip], ses_);
+            const View dst = (ip == 0) ? uout : dense_view(dstage_[ip - 1], stage_[ip - 1], ses_);
             for (int c = 0; c < lda_; ++c)
                 run_pass(pass_desc(p, c, false, src, dst, stage_[ip], hat_[ip], 0, fwd_[ip].fieldstart));
         }
@@ -834,10 +856,12 @@ void SolverImpl::pflush(const std::string& prefix) {
 // user topology; the spectral layout is the dense array of inner_topo_spectral.
 void SolverImpl::prepare_advanced() {
     FLUPS_REQUIRE(setup_done_, "flups_setup must be called first");
+    FLUPS_REQUIRE(!gf32_, "the advanced interface works on FP64 buffers: this solver was set up for FP32 "
+                          "(flups_set_precision(s, 1)) and keeps FP32 stage arrays");
     if (!dstage_[0]) {
         for (int ip = 0; ip < ndim_; ++ip) {
             const size_t m = std::max(hat_[ip].count(), stage_[ip].count());
-            HIP_CHECK(hipMalloc(&dstage_[ip], m * sizeof(double)));
+            HIP_CHECK(hipMalloc(&dstage_[ip], m * ses_));
         }
     }
     if (!dgreen_) compute_green();   // the fused pipeline released the full Green's function
@@ -854,6 +878,8 @@ static bool device_ptr(const void* p) {
 
 double* SolverImpl::inner_buffer() {
     FLUPS_REQUIRE(setup_done_, "flups_setup must be called first");
+    FLUPS_REQUIRE(!gf32_, "the advanced interface works on FP64 buffers: this solver was set up for FP32 "
+                          "(flups_set_precision(s, 1)) and keeps FP32 stage arrays");
     if (!dinner_) {
         HIP_CHECK(hipMalloc(&dinner_, alloc_size() * sizeof(double)));
         HIP_CHECK(hipMemset(dinner_, 0, alloc_size() * sizeof(double)));
@@ -921,7 +947,7 @@ void SolverImpl::prepare(int type) {
     if (!dstage_[0]) {
         for (int ip = 0; ip < ndim_; ++ip) {
             const size_t m = std::max(hat_[ip].count(), stage_[ip].count());
-            HIP_CHECK(hipMalloc(&dstage_[ip], m * sizeof(double)));
+            HIP_CHECK(hipMalloc(&dstage_[ip], m * ses_));
         }
     }
     if (!warmed_) {
@@ -934,11 +960,11 @@ void SolverImpl::prepare(int type) {
 // user layout for the first and last pass), so that the tables of the kernels they
 // select exist before a solve is queued (a solve under graph capture may not allocate)
 void SolverImpl::warm_tables() {
-    const View u = user_view(dstage_[0]);
+    const View u = user_view(static_cast<double*>(dstage_[0]));
     std::vector<int> fk(2 * ndim_, 0);
     for (int ip = 0; ip < ndim_; ++ip) {
-        const View prev = (ip == 0) ? u : dense_view(dstage_[ip - 1], stage_[ip - 1]);
-        const View cur = dense_view(dstage_[ip], stage_[ip]);
+        const View prev = (ip == 0) ? u : dense_view(dstage_[ip - 1], stage_[ip - 1], ses_);
+        const View cur = dense_view(dstage_[ip], stage_[ip], ses_);
         for (int c = 0; c < lda_; ++c) {
             const PassDesc f = pass_desc(fwd_[ip], c, true, prev, cur, hat_[ip], stage_[ip], fwd_[ip].fieldstart, 0);
             const PassDesc b = pass_desc(bwd_[ip], c, false, cur, prev, stage_[ip], hat_[ip], 0, fwd_[ip].fieldstart);
@@ -960,7 +986,7 @@ void SolverImpl::enqueue(double* field, const double* rhs, int type) {
     if (fast_ && (type == 0 || lda_ == 3))
         fast_solve(fast_, field, rhs, user_view(const_cast<double*>(rhs)), user_view(field), stream_, type);
     else {
-        FLUPS_REQUIRE(dstage_[0] && dgreen_, "solver not prepared for solve type %d", type);
+        FLUPS_REQUIRE(dstage_[0] && (dgreen_ || dgreen32_), "solver not prepared for solve type %d", type);
         solve_generic(field, rhs, type);
     }
 }
@@ -992,7 +1018,7 @@ void SolverImpl::solve_noalloc(double* field, const double* rhs, int type) {
     } else {
         for (int ip = 0; ip < ndim_; ++ip) {
             off[ip] = need;
-            need += (sizeof(double) * std::max(hat_[ip].count(), stage_[ip].count()) + 255) / 256 * 256;
+            need += ((size_t)ses_ * std::max(hat_[ip].count(), stage_[ip].count()) + 255) / 256 * 256;
         }
     }
     FLUPS_REQUIRE(need <= cap, "this solve needs %zu bytes of stage arrays; the static scratch of flups_solve_noalloc holds %zu",
@@ -1003,10 +1029,10 @@ void SolverImpl::solve_noalloc(double* field, const double* rhs, int type) {
                       stream_, type);
         return;
     }
-    FLUPS_REQUIRE(dgreen_, "solver not set up for solve type %d", type);
-    double* keep[3] = {dstage_[0], dstage_[1], dstage_[2]};
+    FLUPS_REQUIRE(dgreen_ || dgreen32_, "solver not set up for solve type %d", type);
+    void* keep[3] = {dstage_[0], dstage_[1], dstage_[2]};
     Timers* keep_prof = prof_;
-    for (int ip = 0; ip < 3; ++ip) dstage_[ip] = ip < ndim_ ? reinterpret_cast<double*>(sb + off[ip]) : nullptr;
+    for (int ip = 0; ip < 3; ++ip) dstage_[ip] = ip < ndim_ ? static_cast<void*>(sb + off[ip]) : nullptr;
     prof_ = nullptr;   // profiler stages create events: not in a call that may not allocate
     frozen_ = true;
     auto restore = [&]() {
@@ -1039,8 +1065,9 @@ size_t SolverImpl::device_bytes() const {
     for (const auto& kv : tw_) b += 2 * sizeof(double) * (size_t)kv.first;
     for (const auto& kv : r2r_tab_) b += 4 * sizeof(double) * (size_t)(kv.first.second / 2 + 1);
     for (int ip = 0; ip < 3; ++ip)
-        if (dstage_[ip]) b += sizeof(double) * std::max(hat_[ip].count(), stage_[ip].count());
+        if (dstage_[ip]) b += (size_t)ses_ * std::max(hat_[ip].count(), stage_[ip].count());
     if (dgreen_) b += sizeof(double) * gpost_[ndim_ - 1].count();
+    if (dgreen32_) b += sizeof(float) * gpost_[ndim_ - 1].count();
     b += sizeof(double) * staging_n_;
     if (fast_) b += fast_device_bytes(fast_);
     if (hybrid_) b += hybrid_device_bytes(hybrid_);

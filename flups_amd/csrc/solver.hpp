@@ -85,8 +85,9 @@ class SolverImpl {
     Topo inner_topo_spectral() const;
     size_t alloc_size() const;
     void spectral_info(double kf[3], double ko[3], double ss[3]) const;
-    // 0 general FP64, 1 fused FP64, 2 fused FP32, 3 general with the last direction fused with the multiply
-    int path() const { return fast_ ? (fp32_ ? 2 : 1) : hybrid_ ? 3 : 0; }
+    // 0 general FP64, 1 fused FP64, 2 fused FP32, 3 general with the last direction fused with the
+    // multiply (FP64), 4 general with FP32 stage arrays, 5 the same with the fused last direction
+    int path() const { return fast_ ? (fp32_ ? 2 : 1) : hybrid_ ? (gf32_ ? 5 : 3) : (gf32_ ? 4 : 0); }
     // kernel of each general-path pass (component 0): forward passes in order, then the backward ones
     const std::vector<int>& pass_kernels() const { return pass_kernels_; }
     double last_solve_ns();   // waits for the last solve's end when it was queued asynchronously
@@ -114,9 +115,9 @@ class SolverImpl {
                          const double symstart[3], double length, int nunb);
     View user_view(double* p) const;
     void solve_generic(double* field, const double* rhs, int type);
-    void forward_passes(const View& uin, double* spec);
-    void multiply(double* spec, int type);
-    void backward_passes(const double* spec, const View& uout, bool diff);
+    void forward_passes(const View& uin, void* spec);
+    void multiply(void* spec, int type);
+    void backward_passes(const void* spec, const View& uout, bool diff);
     void prepare_advanced();
     void warm_tables();
     void launch_graph(double* field, const double* rhs, int type);
@@ -134,8 +135,11 @@ class SolverImpl {
     std::map<int, double*> tw_;
     std::map<std::pair<int, int>, double*> r2r_tab_;   // (kind, n) -> pre / post twiddles of the row r2r kernel
     const double* r2r_table(int kind, int n);
-    double* dstage_[3] = {nullptr, nullptr, nullptr};
+    void* dstage_[3] = {nullptr, nullptr, nullptr};   // general path stage arrays: double, or float (gf32_)
     double* dgreen_ = nullptr;
+    float* dgreen32_ = nullptr;   // FP32 copy of the Green's function (gf32_)
+    bool gf32_ = false;           // FP32 request on the general path: FP32 stage arrays, FP64 arithmetic
+    int ses_ = 8;                 // bytes of one real in the stage arrays
     double* dstaging_ = nullptr;   // rhs/field copies for host pointers
     double* dinner_ = nullptr;     // inner buffer of the advanced interface
     std::vector<int> pass_kernels_;

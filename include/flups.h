@@ -187,13 +187,16 @@ void flups_hdf5_dump(const FLUPS_Topology* topo, const char filename[], const do
 const char* flups_get_last_error(void);        /* message of the last failed check, "" if none */
 void        flups_clear_last_error(void);
 void        flups_set_abort_on_error(int on);  /* 1: abort() on a failed check, as the reference */
-/* 0 = FP64 throughout (default), 1 = FP32 transforms and multiply with FP64-computed
-   Green's function and twiddles, rounded once; caller buffers stay double. Call
-   before flups_setup. Applies to the fused unbounded pipeline; the general path
-   stays FP64 (flups_get_path tells which one runs). */
+/* 0 = FP64 throughout (default), 1 = FP32: the fused unbounded pipeline runs its transforms
+   and multiply in FP32 arithmetic with FP64-computed Green's function and twiddles, rounded
+   once; every other boundary set runs the general path with FP32 stage arrays and an FP32
+   copy of the Green's function (half the HBM bytes) and FP64 arithmetic in the kernels. The
+   caller's arrays stay double. Call before flups_setup. The advanced interface
+   (flups_do_copy / do_FFT / do_mult) works on FP64 solvers only. */
 void        flups_set_precision(FLUPS_Solver* s, int fp32);
 /* 0 = general pencil path (FP64), 1 = fused pipeline in FP64, 2 = fused in FP32,
-   3 = general passes with the last direction fused with the multiply (FP64) */
+   3 = general passes with the last direction fused with the multiply (FP64),
+   4 = general path with FP32 stage arrays, 5 = the same with the fused last direction */
 int         flups_get_path(FLUPS_Solver* s);
 /* kernel of each pass of the general path after flups_setup (component 0; the forward
    passes in order, then the backward ones): 0 generic pencil kernel, 1 contiguous
