@@ -289,7 +289,7 @@ double rot_symbol(const RotK& k, int d, int idx) {
 
 template <typename R, int M, int TX>
 __device__ __forceinline__ void z_rot_body(const ColArgs& a, const RotK& rk, C2<R>* lds) {
-    constexpr int T = M / 8;
+    constexpr int NV = 8, T = M / NV;
     const int c = threadIdx.x % TX;
     const int j = threadIdx.x / TX;
     const int ntx = (a.nx + TX - 1) / TX;
@@ -318,25 +318,29 @@ __device__ __forceinline__ void z_rot_body(const ColArgs& a, const RotK& rk, C2<
         const int y = s == 0 ? y0 : y1;
         char* b = base + (size_t)sizeof(C2<R>) * ((long long)y * a.s_other_src);
         const R ky = sym[rk.off[1] + y];
-        C2<R> v[3][8];
+        C2<R> v[3][NV];
+        // the three components' loads all issued before the first FFT (its barriers would
+        // otherwise hold each component's loads back until the previous transform is done)
 #pragma unroll
-        for (int cc = 0; cc < 3; ++cc) {
+        for (int cc = 0; cc < 3; ++cc)
 #pragma unroll
-            for (int i = 0; i < 8; ++i) {
+            for (int i = 0; i < NV; ++i) {
                 const int row = j + i * T;
                 v[cc][i] = row < a.nin ? *reinterpret_cast<const C2<R>*>(b + cc * cstride + i * rstride + loff) : C2<R>{0, 0};
             }
+#pragma unroll
+        for (int cc = 0; cc < 3; ++cc) {
             column_fft<R, M, true, TX>(v[cc], lds + c, j, tw);
         }
         // G of this tile, loaded after the forward transforms (fewer registers live during them)
-        R gv[8];
+        R gv[NV];
 #pragma unroll
-        for (int i = 0; i < 8; ++i) {
+        for (int i = 0; i < NV; ++i) {
             const int kz = j + i * T;
             gv[i] = *reinterpret_cast<const R*>(g + (unsigned)(sizeof(R) * (min(kz, M - kz) * a.gtx + cg)));
         }
 #pragma unroll
-        for (int i = 0; i < 8; ++i) {
+        for (int i = 0; i < NV; ++i) {
             const R kz = sym[rk.off[2] + j + i * T];
             const C2<R> wx = {v[0][i].x * gv[i], v[0][i].y * gv[i]};
             const C2<R> wy = {v[1][i].x * gv[i], v[1][i].y * gv[i]};
@@ -354,7 +358,7 @@ __device__ __forceinline__ void z_rot_body(const ColArgs& a, const RotK& rk, C2<
             column_fft<R, M, false, TX>(v[cc], lds + c, j, tw);
             if (live) {
 #pragma unroll
-                for (int i = 0; i < 8; ++i) {
+                for (int i = 0; i < NV; ++i) {
                     const int row = j + i * T;
                     if (row < a.nout) *reinterpret_cast<C2<R>*>(b + cc * cstride + i * rstride + voff) = v[cc][i];
                 }
@@ -954,12 +958,16 @@ void run_fast(FastPlan* f, void* s0, void* s1, const double* rhs, double* field,
         z.x0 = x0;
         z.gntx = (f->nx + f->gtx - 1) / f->gtx;
         if (type == 1) {
-            // rotational pass: all three components per block, tile width of the Green's layout;
-            // M2 = 1024: half of it (256 threads per block for FP64, 512 for FP32), so that the
-            // three components' registers fit without spilling
+            // rotational pass: all three components per block, tile width of the Green's layout
             FLUPS_REQUIRE(f->lda == 3, "the rotational solve needs lda = 3");
             z.gtx = f->gtx;
-            if (M2 == 1024 && f->gtx == TX / 2) launch_rot<R, TX / 4>(M2, z, f->rot, st);
+            if (M2 == 1024 && f->gtx == TX / 2 && sizeof(R) == 8) {
+                // M2 = 1024, FP64: the Green's layout width, four columns of 64-byte rows per block
+                // of 512 threads (two columns of 32-byte rows: 28.5 instead of 27.8 ms at 513^3 x 3);
+                // FP32 keeps a quarter of the stage tile (eight columns would need 1024 threads of
+                // 157 registers)
+                launch_rot<R, TX / 2>(M2, z, f->rot, st);
+            } else if (M2 == 1024 && f->gtx == TX / 2) launch_rot<R, TX / 4>(M2, z, f->rot, st);
             else if (f->gtx == TX) launch_rot<R, TX>(M2, z, f->rot, st);
             else if (f->gtx == TX / 2) launch_rot<R, TX / 2>(M2, z, f->rot, st);
             else launch_rot<R, (TX >= 8 ? TX / 4 : 1)>(M2, z, f->rot, st);

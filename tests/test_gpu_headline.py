@@ -6,7 +6,10 @@
     1e-5;
   * cell 256^3, x EVEN-ODD, y periodic, z unbounded, LGF_4 (the mixed-boundary workload): the
     general path's 256-point row real-to-real and paired real-column kernels and its fused
-    512-point last direction, against the oracle at 1e-12.
+    512-point last direction, against the oracle at 1e-12;
+  * node-centred 33 x 33 x 513, all faces unbounded, HEJ_6, lda 3, ROT with the spectral curl: the
+    rotational z pass at the 1024-point length of the 513^3 ring workload (z_rot<double, 1024, 4>
+    and z_rot<float, 1024, 4>), FP64 at 1e-12 and FP32 at 1e-5 against the oracle.
 """
 import numpy as np
 import pytest
@@ -91,3 +94,26 @@ def test_mixed_256_even_odd_periodic_unbounded():
     ref = orc.solve([N] * 3, bc, 1, h, [1.0] * 3, 1, 8, rhs)
     e2, ei = rel_err(d_f.numpy(), ref)
     assert e2 <= 1e-12 and ei <= 1e-12, (e2, ei)
+
+
+@pytest.mark.parametrize("fp32", [False, True], ids=["fp64", "fp32"])
+def test_rot_1024_node(fp32):
+    nglob = (33, 33, 513)
+    h = [1.0 / 512] * 3
+    L = [h[0] * (n - 1) for n in nglob]
+    rng = np.random.default_rng(77)
+    rhs = rng.normal(size=(3, nglob[2], nglob[1], nglob[0]))
+    topo = fl.Topology(0, 3, list(nglob), [1, 1, 1], False)
+    s = fl.Solver(topo, [[4, 4]] * 3, h, L, order_diff=fl.DiffType.SPE, center_type=[fl.CenterType.NODE_CENTER] * 3)
+    s.set_green_type(fl.GreenType.HEJ_6)
+    if fp32:
+        s.set_precision(True)
+    s.setup()
+    assert s.path == (2 if fp32 else 1)
+    d_rhs = fl.DeviceArray.from_numpy(rhs)
+    d_f = fl.DeviceArray(rhs.shape)
+    s.solve(d_f, d_rhs, fl.SolverType.ROT)
+    ref = orc.solve(list(nglob), [[4, 4]] * 3, 3, h, L, 0, 4, rhs, typ=1, odiff=1)
+    e2, ei = rel_err(d_f.numpy(), ref)
+    tol = 1e-5 if fp32 else 1e-12
+    assert e2 <= tol and ei <= tol, (e2, ei)
