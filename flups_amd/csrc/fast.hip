@@ -1172,7 +1172,8 @@ HybridZ* make_hybrid_z(SolverImpl& s) {
     h->gtx = nd == 2 ? 2 : 4;
     h->twod = nd == 2;
     h->pf = p2 ? 1 : p3 ? 3 : p5 ? 5 : 7;
-    if (h->pf >= 5) {   // thread teams, one row per block: no mirror pairing, tiles by length (LDS <= 48 KB)
+    if (h->pf >= 5) {   // thread teams, one row per block: no mirror pairing, tiles by length (LDS M gtx complex:
+                        // 48 KB up to M = 384, then 4 columns: 28 / 40 / 56 KB at M = 448 / 640 / 896)
         h->pair = false;
         h->gtx = M <= 192 ? 16 : M <= 384 ? 8 : 4;
     }

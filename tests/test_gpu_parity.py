@@ -533,6 +533,9 @@ HYBRID_CASES = [
     ((48, 32, 80), [[0, 1], [3, 3], [3, 3]], 1, 0, 1),      # z PER 80
     ((32, 24, 112), [[4, 4], [4, 4], [4, 4]], 1, 2, 1),     # all UNB, z of 224
     ((32, 32, 40), [[0, 0], [3, 3], [4, 4]], 1, 2, 3),      # lda 3, z of 80
+    ((32, 24, 224), [[4, 4], [4, 4], [4, 4]], 1, 2, 1),     # z of 448 (7 x 64): four-column tiles
+    ((16, 16, 448), [[3, 3], [3, 3], [4, 4]], 1, 2, 1),     # z of 896 (7 x 128): four-column tiles, 56 KB of LDS
+    ((64, 80, 1), [[3, 3], [4, 4], [9, 9]], 1, 2, 1),       # 2-D, y of 160 (5 x 32)
     # r2c last direction after real-to-real ones: r2c, G and c2r of paired real columns in one launch
     ((64, 32, 32), [[0, 1], [0, 0], [4, 4]], 1, 8, 1),      # x EO, y EE, z UNB
     ((33, 48, 64), [[1, 1], [0, 1], [3, 3]], 1, 0, 1),      # odd x count (an unpaired column), z PER
@@ -550,7 +553,8 @@ HYBRID_CASES = [
 @pytest.mark.parametrize("case", HYBRID_CASES, ids=["mixed64", "semiunb-y", "node-lda3", "sym-per-per", "2d-unb-1024",
                                                     "2d-per-unb-2048", "2d-per-lda2", "2d-unb-lgf4", "pf3-per48", "pf3-unb-pairs",
                                                     "pf3-lda3-192", "pf3-2d-96", "pf5-per80", "pf7-unb224",
-                                                    "pf5-lda3", "rz-eo-ee-unb", "rz-odd-per",
+                                                    "pf5-lda3", "pf7-448-tx4", "pf7-896-tx4", "pf5-2d-160",
+                                                    "rz-eo-ee-unb", "rz-odd-per",
                                                     "rz-lda3", "rz-2d", "rz-node", "xrow-per", "xrow-unb-3x32",
                                                     "xrow-2d-lda2", "xrow-node"])
 def test_hybrid_last_direction_vs_general_and_oracle(case, monkeypatch):
