@@ -269,9 +269,11 @@ def blob_rhs(shape, seed):
     return out
 
 
-def batch_config(fl, nprob, seed, reps):
-    """independent problems solved by one hipGraph launch (flups_batch_run) against the plain path, the same
-    solves one after the other with a synchronisation each (flups_batch_run_sequential)"""
+def batch_config(fl, nprob, seed, reps, same=False):
+    """independent problems solved by one hipGraph launch of grouped kernels (flups_batch_run: one launch per
+    kernel, step and grid over the problems), by one hipGraph launch with a branch per problem
+    (flups_batch_run_branches), and by the plain path, the same solves one after the other with a
+    synchronisation each (flups_batch_run_sequential)"""
     rng = np.random.default_rng(seed)
     sizes = [32, 48, 64, 80, 96, 112, 128]
     tpls = {"unbounded": [[4, 4]] * 3, "periodic-x/unbounded-yz": [[3, 3], [4, 4], [4, 4]],
@@ -285,6 +287,8 @@ def batch_config(fl, nprob, seed, reps):
         tpl = names[int(rng.integers(len(names)))]
         lda = 3 if rng.random() < 0.25 else 1
         fp32 = bool(rng.random() < 0.5)
+        if same:   # one problem description, nprob right-hand sides
+            ng, tpl, lda, fp32 = [64, 48, 80], "periodic-x/unbounded-yz", 1, False
         typ = 1 if lda == 3 else 0   # three-component problems are Biot-Savart (ROT, spectral derivative)
         n = max(ng)
         h = [1.0 / n] * 3
@@ -305,6 +309,7 @@ def batch_config(fl, nprob, seed, reps):
     setup_s = time.time() - t0
     for _ in range(2):
         b.run()
+        b.run_branches()
         b.run_sequential()
     t0 = time.perf_counter()
     ev = []
@@ -312,13 +317,23 @@ def batch_config(fl, nprob, seed, reps):
         ev.append(b.run())
     graph = (time.perf_counter() - t0) / reps
     t0 = time.perf_counter()
+    evb = []
+    for _ in range(reps):
+        evb.append(b.run_branches())
+    branches = (time.perf_counter() - t0) / reps
+    t0 = time.perf_counter()
     for _ in range(reps):
         b.run_sequential()
     seq = (time.perf_counter() - t0) / reps
-    out = {"workload": "%d independent problems, shapes from {32..128}^3 with unequal axes, 4 BC templates, lda 1 or 3 "
-                       "(lda 3: ROT), FP32 or FP64, HEJ_2, Gaussian-blob right-hand sides" % nprob,
+    wl = ("%d problems of one description (64 x 48 x 80, periodic x, unbounded y and z, lda 1, FP64, HEJ_2), "
+          "Gaussian-blob right-hand sides" % nprob) if same else (
+        "%d independent problems, shapes from {32..128}^3 with unequal axes, 4 BC templates, lda 1 or 3 "
+        "(lda 3: ROT), FP32 or FP64, HEJ_2, Gaussian-blob right-hand sides" % nprob)
+    out = {"workload": wl,
            "problems": nprob, "mix": kinds, "graph_ms_per_batch": round(graph * 1e3, 3),
            "graph_device_ms": round(float(np.median(ev)), 3), "solves_per_s_graph": round(nprob / graph, 1),
+           "launches_grouped": b.launches(True), "launches_per_problem": b.launches(False),
+           "branches_ms_per_batch": round(branches * 1e3, 3), "branches_device_ms": round(float(np.median(evb)), 3),
            "sequential_ms_per_batch": round(seq * 1e3, 3), "solves_per_s_sequential": round(nprob / seq, 1),
            "speedup": round(seq / graph, 3), "setup_s": round(setup_s, 2), "reps": reps,
            "device_MB": round(b.device_bytes() / 1e6, 1)}
@@ -348,6 +363,7 @@ def other_configs(fl, which):
         del ref, d32, rhs
     if "batch" in which:
         out["batch"] = batch_config(fl, 64, 7, 10)
+        out["batch_same"] = batch_config(fl, 64, 7, 10, same=True)
     return out
 
 

@@ -181,6 +181,10 @@ def lib():
         L.flups_batch_free.argtypes = [c_void_p]
         L.flups_batch_run.restype = c_int
         L.flups_batch_run.argtypes = [c_void_p]
+        L.flups_batch_run_branches.restype = c_int
+        L.flups_batch_run_branches.argtypes = [c_void_p]
+        L.flups_batch_launches.restype = c_int
+        L.flups_batch_launches.argtypes = [c_void_p, c_int]
         L.flups_batch_run_sequential.restype = c_int
         L.flups_batch_run_sequential.argtypes = [c_void_p]
         L.flups_batch_last_ms.restype = c_double
@@ -247,6 +251,7 @@ class Server:
     """serving loop of the library: worker threads, solver cache keyed by the request, HBM-budget admission"""
 
     def __init__(self, hbm_budget_bytes, workers=4, solvers_per_key=1):
+        self._lib = lib()   # kept for the release at collection (also at interpreter exit)
         self._s = lib().flups_server_new(int(hbm_budget_bytes), int(workers), int(solvers_per_key))
         _check(self._s, "flups_server_new")
 
@@ -273,7 +278,7 @@ class Server:
 
     def close(self):
         if getattr(self, "_s", None):
-            lib().flups_server_free(self._s)
+            self._lib.flups_server_free(self._s)
             self._s = None
 
     def __del__(self):
@@ -281,9 +286,11 @@ class Server:
 
 
 class Batch:
-    """independent problems on device buffers, set up once, solved by one hipGraph launch"""
+    """independent problems on device buffers, set up once, solved by one hipGraph launch of grouped
+    kernels (flups_batch_run) or of one branch per problem (run_branches)"""
 
     def __init__(self, reqs, rhs, fields):
+        self._lib = lib()   # kept for the release at collection (also at interpreter exit)
         n = len(reqs)
         self._reqs = (ServeRequest * n)(*reqs)
         self._rhs = (c_void_p * n)(*[_data_ptr(a) for a in rhs])
@@ -292,8 +299,18 @@ class Batch:
         _check(self._b, "flups_batch_new")
 
     def run(self):
+        """grouped launches (one launch per kernel and step over the problems); device ms"""
         _check(lib().flups_batch_run(self._b), "flups_batch_run")
         return lib().flups_batch_last_ms(self._b)
+
+    def run_branches(self):
+        """one graph branch per problem with its own launches; device ms"""
+        _check(lib().flups_batch_run_branches(self._b), "flups_batch_run_branches")
+        return lib().flups_batch_last_ms(self._b)
+
+    def launches(self, grouped=True):
+        """kernel launches of one run (grouped) or of one run_branches"""
+        return lib().flups_batch_launches(self._b, 1 if grouped else 0)
 
     def run_sequential(self):
         _check(lib().flups_batch_run_sequential(self._b), "flups_batch_run_sequential")
@@ -303,7 +320,7 @@ class Batch:
 
     def close(self):
         if getattr(self, "_b", None):
-            lib().flups_batch_free(self._b)
+            self._lib.flups_batch_free(self._b)
             self._b = None
 
     def __del__(self):
@@ -324,6 +341,7 @@ class Topology:
 
     def __init__(self, axis, lda, nglob, nproc=(1, 1, 1), is_complex=False, axproc=None,
                  alignment=FLUPS_ALIGNMENT, comm=0):
+        self._lib = lib()   # kept for the release at collection (also at interpreter exit)
         L = lib()
         L.flups_clear_last_error()
         ax = (c_int * 3)(*axproc) if axproc is not None else None
@@ -333,7 +351,7 @@ class Topology:
 
     def __del__(self):
         if getattr(self, "_topo", None) and getattr(self, "_owned", True):
-            lib().flups_topo_free(self._topo)
+            self._lib.flups_topo_free(self._topo)
         self._topo = None
 
     @property
@@ -402,6 +420,7 @@ class Topology:
         which keeps the solver alive"""
         _check(handle, "inner topology")
         t = cls.__new__(cls)
+        t._lib = lib()
         t._topo = handle
         t._owned = False
         t._solver_ref = solver
@@ -436,6 +455,7 @@ class DeviceArray:
     """float64 array in HBM allocated by libflups (shape in C order, dimension 0 of FLUPS last)."""
 
     def __init__(self, shape):
+        self._lib = lib()   # kept for the release at collection (also at interpreter exit)
         self.shape = tuple(int(s) for s in shape)
         self.nbytes = int(np.prod(self.shape)) * 8
         self.ptr = lib().flups_device_malloc(self.nbytes)
@@ -458,7 +478,7 @@ class DeviceArray:
 
     def __del__(self):
         if getattr(self, "ptr", None):
-            lib().flups_device_free(self.ptr)
+            self._lib.flups_device_free(self.ptr)
             self.ptr = None
 
 
@@ -475,6 +495,7 @@ class Stream:
     """HIP stream of libflups' runtime (flups_stream_create), for Solver.set_stream."""
 
     def __init__(self):
+        self._lib = lib()   # kept for the release at collection (also at interpreter exit)
         self.handle = lib().flups_stream_create()
         _check(self.handle, "flups_stream_create")
 
@@ -483,7 +504,7 @@ class Stream:
 
     def __del__(self):
         if getattr(self, "handle", None):
-            lib().flups_stream_destroy(self.handle)
+            self._lib.flups_stream_destroy(self.handle)
             self.handle = None
 
 
@@ -492,13 +513,14 @@ class Profiler:
     every solve with its stages (HIP events on the solve stream)."""
 
     def __init__(self, name="default"):
+        self._lib = lib()   # kept for the release at collection (also at interpreter exit)
         L = lib()
         self._prof = L.flups_profiler_new() if name == "default" else L.flups_profiler_new_n(name.encode("utf-8"))
         _check(self._prof, "flups_profiler_new")
 
     def __del__(self):
         if getattr(self, "_prof", None):
-            lib().flups_profiler_free(self._prof)
+            self._lib.flups_profiler_free(self._prof)
             self._prof = None
 
     def display(self):
@@ -527,6 +549,7 @@ class Solver:
 
     def __init__(self, topology, boundary_conditions, h, L, order_diff=DiffType.NOD, center_type=None,
                  profiler=None):
+        self._lib = lib()   # kept for the release at collection (also at interpreter exit)
         Lb = lib()
         Lb.flups_clear_last_error()
         self._topology = topology
@@ -548,7 +571,7 @@ class Solver:
 
     def __del__(self):
         if getattr(self, "_solver", None):
-            lib().flups_cleanup_solver(self._solver)
+            self._lib.flups_cleanup_solver(self._solver)
             self._solver = None
 
     def set_green_type(self, green_type):

@@ -38,6 +38,7 @@
 #include <vector>
 
 #include "fft_core.hpp"
+#include "group.hpp"
 #include "solver.hpp"
 
 namespace flups {
@@ -75,39 +76,43 @@ struct ColArgs {
 // (inverse); nin / nout are then not consulted. Lanes past nx load a valid
 // column (the last one) and store nothing.
 template <typename R, int M, int TX, bool FWD, bool HALF>
-__global__ void __launch_bounds__(TX * M / 8) col_fft(ColArgs a) {
-    __shared__ C2<R> lds[M * TX];
-    constexpr int T = M / 8;
-    const int c = threadIdx.x % TX;
-    const int j = threadIdx.x / TX;
-    const int ntx = (a.nx + TX - 1) / TX;
-    int blk = a.xcd ? xcd_group(blockIdx.x, gridDim.x) : blockIdx.x;
-    if (a.rev) blk = gridDim.x - 1 - blk;
-    const int tile = blk % ntx;
-    const int other = blk / ntx;
-    const int x = tile * TX + c;
-    const bool live = x < a.nx;
-    const C2<R>* src = reinterpret_cast<const C2<R>*>(a.src) + blockIdx.y * a.comp_src + other * a.s_other_src + min(x, a.nx - 1);
-    C2<R>* dst = reinterpret_cast<C2<R>*>(a.dst) + blockIdx.y * a.comp_dst + other * a.s_other_dst + x;
-    const C2<R>* tw = reinterpret_cast<const C2<R>*>(a.tw);
-    constexpr bool HIN = HALF && FWD, HOUT = HALF && !FWD;
-    C2<R> v[8];
-#pragma unroll
-    for (int i = 0; i < 8; ++i) {
-        const int row = j + i * T;
-        if (HIN) v[i] = i < 4 ? src[row * a.s_col] : C2<R>{0, 0};
-        else if (HOUT) v[i] = src[row * a.s_col];
-        else v[i] = row < a.nin ? src[row * a.s_col] : C2<R>{0, 0};
-    }
-    column_fft<R, M, FWD, TX, HIN>(v, lds + c, j, tw);
-    if (live) {
-#pragma unroll
+struct col_fft {
+    using Args = ColArgs;
+    static constexpr int NT = TX * M / 8;
+    static __device__ __forceinline__ void run(const ColArgs& a, Blk bk) {
+        __shared__ C2<R> lds[M * TX];
+        constexpr int T = M / 8;
+        const int c = threadIdx.x % TX;
+        const int j = threadIdx.x / TX;
+        const int ntx = (a.nx + TX - 1) / TX;
+        int blk = a.xcd ? xcd_group(bk.x, bk.nx) : bk.x;
+        if (a.rev) blk = bk.nx - 1 - blk;
+        const int tile = blk % ntx;
+        const int other = blk / ntx;
+        const int x = tile * TX + c;
+        const bool live = x < a.nx;
+        const C2<R>* src = reinterpret_cast<const C2<R>*>(a.src) + bk.y * a.comp_src + other * a.s_other_src + min(x, a.nx - 1);
+        C2<R>* dst = reinterpret_cast<C2<R>*>(a.dst) + bk.y * a.comp_dst + other * a.s_other_dst + x;
+        const C2<R>* tw = reinterpret_cast<const C2<R>*>(a.tw);
+        constexpr bool HIN = HALF && FWD, HOUT = HALF && !FWD;
+        C2<R> v[8];
+    #pragma unroll
         for (int i = 0; i < 8; ++i) {
             const int row = j + i * T;
-            if (HOUT ? i < 4 : (HIN || row < a.nout)) dst[row * a.s_col_dst] = v[i];
+            if (HIN) v[i] = i < 4 ? src[row * a.s_col] : C2<R>{0, 0};
+            else if (HOUT) v[i] = src[row * a.s_col];
+            else v[i] = row < a.nin ? src[row * a.s_col] : C2<R>{0, 0};
+        }
+        column_fft<R, M, FWD, TX, HIN>(v, lds + c, j, tw);
+        if (live) {
+    #pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                const int row = j + i * T;
+                if (HOUT ? i < 4 : (HIN || row < a.nout)) dst[row * a.s_col_dst] = v[i];
+            }
         }
     }
-}
+};
 
 // Fused z pass, in place on S1: forward FFT along z, multiply by normfact * G,
 // inverse FFT, keep the first nin rows. G is real and even in ky and kz, so one
@@ -123,137 +128,146 @@ __global__ void __launch_bounds__(TX * M / 8) col_fft(ColArgs a) {
 // S: storage of the field and of G (R, or float with R = double: the general path's FP32 solves,
 // FP32 stage arrays with FP64 arithmetic)
 template <typename R, int M, int TX, int SCHED, bool HALF, int PF = 1, typename S = R>
-__global__ void __launch_bounds__(TX * M / (8 * PF)) z_fused(ColArgs a) {
-    // SCHED 0: G loaded with the first set, second set loaded after the first is stored;
-    //       1: both sets prefetched up front; 2: G loaded just before each multiply.
-    static_assert(PF == 1 || !HALF, "the zero-padded half is pruned for powers of two only");
-    __shared__ C2<R> lds[M * TX];
-    constexpr int Q = M / PF, T = Q / 8, NV = 8 * PF;
-    const int c = threadIdx.x % TX;
-    const int j = threadIdx.x / TX;
-    const int ntx = (a.nx + TX - 1) / TX;
-    const int blk = a.xcd ? xcd_group(blockIdx.x, gridDim.x) : blockIdx.x;
-    const int tile = blk % ntx;
-    int y0 = blk / ntx;                 // 0 .. M1/2 (every row with nopair = 1)
-    int yrow = y0;                      // the row of the first (or only) column set
-    if (a.nopair == 2) {                // the two rows of a pair in neighbouring blocks (same G slice)
-        const int r = y0;
-        y0 = r >> 1;
-        yrow = (r & 1) ? a.gmy - y0 : y0;
-        if ((r & 1) && (y0 == 0 || 2 * y0 == a.gmy)) return;   // no mirror row: the whole block leaves
-    }
-    const int y1 = a.gmy - y0;          // mirror row; none for y0 = 0 (M1) and y0 = M1/2 (itself)
-    const bool pair = a.nopair == 0 && y0 != 0 && y1 != y0;
-    const bool live = tile * TX + c < a.nx;
-    // uniform bases (scalar registers) + one 32-bit per-lane byte offset for every access;
-    // lanes past nx load the last valid column and store nothing
-    char* base = reinterpret_cast<char*>(a.dst) +
-                 (size_t)sizeof(C2<S>) * ((long long)blockIdx.y * a.comp_src + (long long)tile * TX);
-    char* b0 = base + (size_t)sizeof(C2<S>) * ((long long)yrow * a.s_other_src);
-    char* b1 = base + (size_t)sizeof(C2<S>) * ((long long)y1 * a.s_other_src);
-    const size_t rstride = (size_t)sizeof(C2<S>) * T * a.s_col;                      // i -> i + 1
-    const unsigned voff = (unsigned)(sizeof(C2<S>) * ((size_t)j * a.s_col + c));     // < 4 GiB: j < M/8
-    const unsigned loff = (unsigned)(sizeof(C2<S>) * ((size_t)j * a.s_col + min(c, a.nx - 1 - tile * TX)));
-    constexpr int NL = HALF ? 4 : NV;   // rows loaded / stored per thread
-    const char* g = reinterpret_cast<const char*>(a.green) +
-                    sizeof(S) * ((long long)y0 * a.gntx + a.x0 / TX + tile) * (M / 2 + 1) * TX;
-    const C2<R>* tw = reinterpret_cast<const C2<R>*>(a.tw);
-    const C2<R>* twm = reinterpret_cast<const C2<R>*>(a.twm);
-    R gv[NV];
-    C2<R> v[NV], w[NV];
-    auto load_g = [&]() {
-#pragma unroll
-        for (int i = 0; i < NV; ++i) {
-            const int kz = j + i * T;
-            gv[i] = (R)*reinterpret_cast<const S*>(g + (unsigned)(sizeof(S) * (min(kz, M - kz) * TX + c)));
+struct z_fused {
+    using Args = ColArgs;
+    static constexpr int NT = TX * M / (8 * PF);
+    static constexpr int kWaves = PF == 3 ? 2 : 0;   // the mixed-radix columns: at most 256 registers per lane
+    static __device__ __forceinline__ void run(const ColArgs& a, Blk bk) {
+        // SCHED 0: G loaded with the first set, second set loaded after the first is stored;
+        //       1: both sets prefetched up front; 2: G loaded just before each multiply.
+        static_assert(PF == 1 || !HALF, "the zero-padded half is pruned for powers of two only");
+        __shared__ C2<R> lds[M * TX];
+        constexpr int Q = M / PF, T = Q / 8, NV = 8 * PF;
+        const int c = threadIdx.x % TX;
+        const int j = threadIdx.x / TX;
+        const int ntx = (a.nx + TX - 1) / TX;
+        const int blk = a.xcd ? xcd_group(bk.x, bk.nx) : bk.x;
+        const int tile = blk % ntx;
+        int y0 = blk / ntx;                 // 0 .. M1/2 (every row with nopair = 1)
+        int yrow = y0;                      // the row of the first (or only) column set
+        if (a.nopair == 2) {                // the two rows of a pair in neighbouring blocks (same G slice)
+            const int r = y0;
+            y0 = r >> 1;
+            yrow = (r & 1) ? a.gmy - y0 : y0;
+            if ((r & 1) && (y0 == 0 || 2 * y0 == a.gmy)) return;   // no mirror row: the whole block leaves
         }
-    };
-    auto load = [&](C2<R>* u, const char* b) {
-#pragma unroll
-        for (int i = 0; i < NV; ++i) {
-            const int row = j + i * T;
-            if (HALF) u[i] = i < 4 ? ld2<R, S>(b + i * rstride + loff) : C2<R>{0, 0};
-            else u[i] = row < a.nin ? ld2<R, S>(b + i * rstride + loff) : C2<R>{0, 0};
-        }
-    };
-    if (SCHED != 2) load_g();
-    load(v, b0);
-    if (SCHED == 1 && pair) load(w, b1);
-    for (int s = 0; s < 2; ++s) {
-        column_fft_pq<R, PF, Q, true, TX, HALF>(v, lds + c, j, tw, twm, 1);
-        if (SCHED == 2) load_g();
-#pragma unroll
-        for (int i = 0; i < NV; ++i) {
-            v[i].x *= gv[i];
-            v[i].y *= gv[i];
-        }
-        column_fft_pq<R, PF, Q, false, TX>(v, lds + c, j, tw, twm, 1);   // syncs before its first LDS write
-        char* b = s == 0 ? b0 : b1;
-        if (live) {
-#pragma unroll
-            for (int i = 0; i < NL; ++i) {
+        const int y1 = a.gmy - y0;          // mirror row; none for y0 = 0 (M1) and y0 = M1/2 (itself)
+        const bool pair = a.nopair == 0 && y0 != 0 && y1 != y0;
+        const bool live = tile * TX + c < a.nx;
+        // uniform bases (scalar registers) + one 32-bit per-lane byte offset for every access;
+        // lanes past nx load the last valid column and store nothing
+        char* base = reinterpret_cast<char*>(a.dst) +
+                     (size_t)sizeof(C2<S>) * ((long long)bk.y * a.comp_src + (long long)tile * TX);
+        char* b0 = base + (size_t)sizeof(C2<S>) * ((long long)yrow * a.s_other_src);
+        char* b1 = base + (size_t)sizeof(C2<S>) * ((long long)y1 * a.s_other_src);
+        const size_t rstride = (size_t)sizeof(C2<S>) * T * a.s_col;                      // i -> i + 1
+        const unsigned voff = (unsigned)(sizeof(C2<S>) * ((size_t)j * a.s_col + c));     // < 4 GiB: j < M/8
+        const unsigned loff = (unsigned)(sizeof(C2<S>) * ((size_t)j * a.s_col + min(c, a.nx - 1 - tile * TX)));
+        constexpr int NL = HALF ? 4 : NV;   // rows loaded / stored per thread
+        const char* g = reinterpret_cast<const char*>(a.green) +
+                        sizeof(S) * ((long long)y0 * a.gntx + a.x0 / TX + tile) * (M / 2 + 1) * TX;
+        const C2<R>* tw = reinterpret_cast<const C2<R>*>(a.tw);
+        const C2<R>* twm = reinterpret_cast<const C2<R>*>(a.twm);
+        R gv[NV];
+        C2<R> v[NV], w[NV];
+        auto load_g = [&]() {
+    #pragma unroll
+            for (int i = 0; i < NV; ++i) {
+                const int kz = j + i * T;
+                gv[i] = (R)*reinterpret_cast<const S*>(g + (unsigned)(sizeof(S) * (min(kz, M - kz) * TX + c)));
+            }
+        };
+        auto load = [&](C2<R>* u, const char* b) {
+    #pragma unroll
+            for (int i = 0; i < NV; ++i) {
                 const int row = j + i * T;
-                if (HALF || row < a.nout) st2<R, S>(b + i * rstride + voff, v[i]);
+                if (HALF) u[i] = i < 4 ? ld2<R, S>(b + i * rstride + loff) : C2<R>{0, 0};
+                else u[i] = row < a.nin ? ld2<R, S>(b + i * rstride + loff) : C2<R>{0, 0};
             }
-        }
-        if (!pair) break;   // uniform over the block
-        if (s == 0) {
-            if (SCHED == 1) {
-#pragma unroll
-                for (int i = 0; i < NV; ++i) v[i] = w[i];
-            } else {
-                load(v, b1);
+        };
+        if (SCHED != 2) load_g();
+        load(v, b0);
+        if (SCHED == 1 && pair) load(w, b1);
+        for (int s = 0; s < 2; ++s) {
+            column_fft_pq<R, PF, Q, true, TX, HALF>(v, lds + c, j, tw, twm, 1);
+            if (SCHED == 2) load_g();
+    #pragma unroll
+            for (int i = 0; i < NV; ++i) {
+                v[i].x *= gv[i];
+                v[i].y *= gv[i];
+            }
+            column_fft_pq<R, PF, Q, false, TX>(v, lds + c, j, tw, twm, 1);   // syncs before its first LDS write
+            char* b = s == 0 ? b0 : b1;
+            if (live) {
+    #pragma unroll
+                for (int i = 0; i < NL; ++i) {
+                    const int row = j + i * T;
+                    if (HALF || row < a.nout) st2<R, S>(b + i * rstride + voff, v[i]);
+                }
+            }
+            if (!pair) break;   // uniform over the block
+            if (s == 0) {
+                if (SCHED == 1) {
+    #pragma unroll
+                    for (int i = 0; i < NV; ++i) v[i] = w[i];
+                } else {
+                    load(v, b1);
+                }
             }
         }
     }
-}
+};
 
 // The general path's fused last direction for M = PF Q with PF = 5 or 7: one row per block (no
 // mirror pairing), TX columns staged in LDS, PF thread teams of Q/8 threads (fft_pq_team) for
 // the forward FFT, the multiply by normfact G and the inverse FFT; in place.
 template <int PF, int Q, int TX, typename S = double>
-__global__ void __launch_bounds__(TX * PF * Q / 8) z_team(ColArgs a) {
-    constexpr int M = PF * Q, T = Q / 8, TT = M / 8;
-    __shared__ C2<double> lds[M * TX];
-    const int c = threadIdx.x % TX, tt = threadIdx.x / TX;
-    const int k1 = tt / T, j = tt % T;
-    const int ntx = (a.nx + TX - 1) / TX;
-    const int blk = a.xcd ? xcd_group(blockIdx.x, gridDim.x) : blockIdx.x;
-    const int tile = blk % ntx, y = blk / ntx;
-    const bool live = tile * TX + c < a.nx;
-    const int cl = min(c, a.nx - 1 - tile * TX);   // lanes past nx load the last valid column
-    C2<S>* col = reinterpret_cast<C2<S>*>(a.dst) + (long long)blockIdx.y * a.comp_src + (long long)y * a.s_other_src +
-                 (long long)tile * TX;
-#pragma unroll
-    for (int i = 0; i < 8; ++i) {
-        const int r = tt + i * TT;
-        lds[r * TX + c] = r < a.nin ? c2d(col[(long long)r * a.s_col + cl]) : C2<double>{0.0, 0.0};
+struct z_team {
+    using Args = ColArgs;
+    static constexpr int NT = TX * PF * Q / 8;
+    static __device__ __forceinline__ void run(const ColArgs& a, Blk bk) {
+        constexpr int M = PF * Q, T = Q / 8, TT = M / 8;
+        __shared__ C2<double> lds[M * TX];
+        const int c = threadIdx.x % TX, tt = threadIdx.x / TX;
+        const int k1 = tt / T, j = tt % T;
+        const int ntx = (a.nx + TX - 1) / TX;
+        const int blk = a.xcd ? xcd_group(bk.x, bk.nx) : bk.x;
+        const int tile = blk % ntx, y = blk / ntx;
+        const bool live = tile * TX + c < a.nx;
+        const int cl = min(c, a.nx - 1 - tile * TX);   // lanes past nx load the last valid column
+        C2<S>* col = reinterpret_cast<C2<S>*>(a.dst) + (long long)bk.y * a.comp_src + (long long)y * a.s_other_src +
+                     (long long)tile * TX;
+    #pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const int r = tt + i * TT;
+            lds[r * TX + c] = r < a.nin ? c2d(col[(long long)r * a.s_col + cl]) : C2<double>{0.0, 0.0};
+        }
+        __syncthreads();
+        const C2<double>* tw = reinterpret_cast<const C2<double>*>(a.tw);
+        const C2<double>* twm = reinterpret_cast<const C2<double>*>(a.twm);
+        auto load = [&](int m) -> C2<double> { return lds[m * TX + c]; };
+        C2<double> v[8];
+        fft_pq_team<double, PF, Q, true, TX>(v, lds + c, k1, j, load, tw, twm, 1);
+        const S* g = reinterpret_cast<const S*>(a.green) + ((long long)y * a.gntx + tile) * (M / 2 + 1) * TX;
+    #pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const int kz = k1 + PF * (j + i * T);
+            const double gv = (double)g[min(kz, M - kz) * TX + c];
+            v[i] = {v[i].x * gv, v[i].y * gv};
+        }
+        __syncthreads();   // the forward FFT's last LDS reads are done
+    #pragma unroll
+        for (int i = 0; i < 8; ++i) lds[(k1 + PF * (j + i * T)) * TX + c] = v[i];
+        __syncthreads();
+        fft_pq_team<double, PF, Q, false, TX>(v, lds + c, k1, j, load, tw, twm, 1);
+        if (!live) return;
+    #pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const int r = k1 + PF * (j + i * T);
+            if (r < a.nout) col[(long long)r * a.s_col + c] = c2t<S>(v[i]);
+        }
     }
-    __syncthreads();
-    const C2<double>* tw = reinterpret_cast<const C2<double>*>(a.tw);
-    const C2<double>* twm = reinterpret_cast<const C2<double>*>(a.twm);
-    auto load = [&](int m) -> C2<double> { return lds[m * TX + c]; };
-    C2<double> v[8];
-    fft_pq_team<double, PF, Q, true, TX>(v, lds + c, k1, j, load, tw, twm, 1);
-    const S* g = reinterpret_cast<const S*>(a.green) + ((long long)y * a.gntx + tile) * (M / 2 + 1) * TX;
-#pragma unroll
-    for (int i = 0; i < 8; ++i) {
-        const int kz = k1 + PF * (j + i * T);
-        const double gv = (double)g[min(kz, M - kz) * TX + c];
-        v[i] = {v[i].x * gv, v[i].y * gv};
-    }
-    __syncthreads();   // the forward FFT's last LDS reads are done
-#pragma unroll
-    for (int i = 0; i < 8; ++i) lds[(k1 + PF * (j + i * T)) * TX + c] = v[i];
-    __syncthreads();
-    fft_pq_team<double, PF, Q, false, TX>(v, lds + c, k1, j, load, tw, twm, 1);
-    if (!live) return;
-#pragma unroll
-    for (int i = 0; i < 8; ++i) {
-        const int r = k1 + PF * (j + i * T);
-        if (r < a.nout) col[(long long)r * a.s_col + c] = c2t<S>(v[i]);
-    }
-}
+};
 
 // Rotational (Biot-Savart) z pass, in place on the three components of S1:
 // forward FFT of omega_x, omega_y, omega_z along z, psi = normfact G omega, then
@@ -288,12 +302,12 @@ double rot_symbol(const RotK& k, int d, int idx) {
 }
 
 template <typename R, int M, int TX>
-__device__ __forceinline__ void z_rot_body(const ColArgs& a, const RotK& rk, C2<R>* lds) {
+__device__ __forceinline__ void z_rot_body(const ColArgs& a, const RotK& rk, C2<R>* lds, Blk bk) {
     constexpr int NV = 8, T = M / NV;
     const int c = threadIdx.x % TX;
     const int j = threadIdx.x / TX;
     const int ntx = (a.nx + TX - 1) / TX;
-    const int blk = a.xcd ? xcd_group(blockIdx.x, gridDim.x) : blockIdx.x;
+    const int blk = a.xcd ? xcd_group(bk.x, bk.nx) : bk.x;
     const int tile = blk % ntx;
     const int y0 = blk / ntx;
     const int y1 = a.gmy - y0;
@@ -371,11 +385,20 @@ __device__ __forceinline__ void z_rot_body(const ColArgs& a, const RotK& rk, C2<
 // at most 256 registers per lane (two waves per SIMD): the three components need 250-285
 // (FP64), and the few that spill cost less than halving the waves (513^3 x 3 solve: 34.0 ->
 // 29.8 ms)
+struct ZRotArgs {
+    ColArgs a;
+    RotK rk;
+};
 template <typename R, int M, int TX>
-__global__ void __launch_bounds__(TX * M / 8) __attribute__((amdgpu_waves_per_eu(2))) z_rot(ColArgs a, RotK rk) {
-    __shared__ C2<R> lds[M * TX];
-    z_rot_body<R, M, TX>(a, rk, lds);
-}
+struct z_rot {
+    using Args = ZRotArgs;
+    static constexpr int NT = TX * M / 8;
+    static constexpr int kWaves = 2;
+    static __device__ __forceinline__ void run(const ZRotArgs& z, Blk bk) {
+        __shared__ C2<R> lds[M * TX];
+        z_rot_body<R, M, TX>(z.a, z.rk, lds, bk);
+    }
+};
 
 // ---------------------------------------------------------------------------
 // x rows: r2c of the zero-padded real row (length M = 2*H) through the packed
@@ -401,107 +424,115 @@ struct RowArgs {
 // packed input is zero and pruned from the first stage), 16-byte loads; 2: nvalid = 2H,
 // 16-byte loads. Modes 1 and 2 need 16-byte aligned user rows.
 template <typename R, int H, int ROWS, int MODE>
-__global__ void __launch_bounds__(ROWS * H / 8) x_r2c(RowArgs a) {
-    constexpr int HP = H + H / 8;     // padded row buffer
-    __shared__ C2<R> lds[HP * ROWS];
-    constexpr int T = H / 8;
-    const int rl = threadIdx.x / T;   // row within the block
-    const int j = threadIdx.x % T;
-    const int blk = a.rev ? gridDim.x - 1 - blockIdx.x : blockIdx.x;
-    const int row = blk * ROWS + rl;
-    const bool live = row < a.nrows;
-    const int y = live ? row % a.n1 : 0, z = live ? row / a.n1 : 0;
-    const double* src = reinterpret_cast<const double*>(a.src) + blockIdx.y * a.comp_src + y * a.s_row_src + z * a.s_z_src;
-    C2<R>* dst = reinterpret_cast<C2<R>*>(a.dst) + blockIdx.y * a.comp_dst + y * a.s_row_dst + z * a.s_z_dst;
-    C2<R>* col = lds + rl * HP;   // row's buffer
-    C2<R> v[8];
-#pragma unroll
-    for (int i = 0; i < 8; ++i) {
-        const int n = j + i * T;
-        const int e0 = 2 * n, e1 = 2 * n + 1;
-        if (MODE == 0) {
-            R re = 0, im = 0;
-            if (live && e0 < a.nvalid) re = (R)src[e0];
-            if (live && e1 < a.nvalid) im = (R)src[e1];
-            v[i] = {re, im};
-        } else if (MODE == 2 || i < 4) {
-            // dead rows (past nrows) read row 0 of the plane and store nothing
-            const double2 p = *reinterpret_cast<const double2*>(src + e0);
-            v[i] = {(R)p.x, (R)p.y};
-        } else {
-            v[i] = {0, 0};
-        }
-    }
-    column_fft<R, H, true, 0, MODE == 1>(v, col, j, reinterpret_cast<const C2<R>*>(a.tw));
-    __syncthreads();
-#pragma unroll
-    for (int i = 0; i < 8; ++i) col[lslot<0>(j + i * T)] = v[i];
-    __syncthreads();
-    const C2<R>* tw2 = reinterpret_cast<const C2<R>*>(a.tw2);
-    if (live) {
-#pragma unroll
+struct x_r2c {
+    using Args = RowArgs;
+    static constexpr int NT = ROWS * H / 8;
+    static __device__ __forceinline__ void run(const RowArgs& a, Blk bk) {
+        constexpr int HP = H + H / 8;     // padded row buffer
+        __shared__ C2<R> lds[HP * ROWS];
+        constexpr int T = H / 8;
+        const int rl = threadIdx.x / T;   // row within the block
+        const int j = threadIdx.x % T;
+        const int blk = a.rev ? bk.nx - 1 - bk.x : bk.x;
+        const int row = blk * ROWS + rl;
+        const bool live = row < a.nrows;
+        const int y = live ? row % a.n1 : 0, z = live ? row / a.n1 : 0;
+        const double* src = reinterpret_cast<const double*>(a.src) + bk.y * a.comp_src + y * a.s_row_src + z * a.s_z_src;
+        C2<R>* dst = reinterpret_cast<C2<R>*>(a.dst) + bk.y * a.comp_dst + y * a.s_row_dst + z * a.s_z_dst;
+        C2<R>* col = lds + rl * HP;   // row's buffer
+        C2<R> v[8];
+    #pragma unroll
         for (int i = 0; i < 8; ++i) {
-            const int k = j + i * T;
-            const C2<R> zk = col[lslot<0>(k)], zc = cconj(col[lslot<0>((H - k) & (H - 1))]);
-            const C2<R> e = {(zk.x + zc.x) * (R)0.5, (zk.y + zc.y) * (R)0.5};
-            const C2<R> df = {(zk.x - zc.x) * (R)0.5, (zk.y - zc.y) * (R)0.5};
-            const C2<R> o = {df.y, -df.x};
-            dst[k] = cadd(e, cmul(tw2[k], o));
+            const int n = j + i * T;
+            const int e0 = 2 * n, e1 = 2 * n + 1;
+            if (MODE == 0) {
+                R re = 0, im = 0;
+                if (live && e0 < a.nvalid) re = (R)src[e0];
+                if (live && e1 < a.nvalid) im = (R)src[e1];
+                v[i] = {re, im};
+            } else if (MODE == 2 || i < 4) {
+                // dead rows (past nrows) read row 0 of the plane and store nothing
+                const double2 p = *reinterpret_cast<const double2*>(src + e0);
+                v[i] = {(R)p.x, (R)p.y};
+            } else {
+                v[i] = {0, 0};
+            }
         }
-        if (j == 0) {   // Nyquist: Re Z0 - Im Z0
-            const C2<R> z0 = col[0];
-            dst[H] = {z0.x - z0.y, (R)0};
+        column_fft<R, H, true, 0, MODE == 1>(v, col, j, reinterpret_cast<const C2<R>*>(a.tw));
+        __syncthreads();
+    #pragma unroll
+        for (int i = 0; i < 8; ++i) col[lslot<0>(j + i * T)] = v[i];
+        __syncthreads();
+        const C2<R>* tw2 = reinterpret_cast<const C2<R>*>(a.tw2);
+        if (live) {
+    #pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                const int k = j + i * T;
+                const C2<R> zk = col[lslot<0>(k)], zc = cconj(col[lslot<0>((H - k) & (H - 1))]);
+                const C2<R> e = {(zk.x + zc.x) * (R)0.5, (zk.y + zc.y) * (R)0.5};
+                const C2<R> df = {(zk.x - zc.x) * (R)0.5, (zk.y - zc.y) * (R)0.5};
+                const C2<R> o = {df.y, -df.x};
+                dst[k] = cadd(e, cmul(tw2[k], o));
+            }
+            if (j == 0) {   // Nyquist: Re Z0 - Im Z0
+                const C2<R> z0 = col[0];
+                dst[H] = {z0.x - z0.y, (R)0};
+            }
         }
     }
-}
+};
 
 // x rows inverse: Z[k] = (X[k] + conj X[H-k]) + i exp(+2 pi i k/M) (X[k] - conj X[H-k]), z = IFFT_H(Z),
 // x[2n] = Re z[n], x[2n+1] = Im z[n]; the imaginary parts of X[0] and X[H] are ignored.
 // MODE as for x_r2c: 1 stores the first H reals of the row (the upper half of the
 // inverse is not computed), 2 all 2H; both with 16-byte stores to aligned user rows.
 template <typename R, int H, int ROWS, int MODE>
-__global__ void __launch_bounds__(ROWS * H / 8) x_c2r(RowArgs a) {
-    constexpr int HP = H + H / 8;
-    __shared__ C2<R> lds[HP * ROWS];
-    constexpr int T = H / 8;
-    const int rl = threadIdx.x / T;
-    const int j = threadIdx.x % T;
-    const int blk = a.rev ? gridDim.x - 1 - blockIdx.x : blockIdx.x;
-    const int row = blk * ROWS + rl;
-    const bool live = row < a.nrows;
-    const int y = live ? row % a.n1 : 0, z = live ? row / a.n1 : 0;
-    const C2<R>* src = reinterpret_cast<const C2<R>*>(a.src) + blockIdx.y * a.comp_src + y * a.s_row_src + z * a.s_z_src;
-    double* dst = reinterpret_cast<double*>(a.dst) + blockIdx.y * a.comp_dst + y * a.s_row_dst + z * a.s_z_dst;
-    const C2<R>* tw2 = reinterpret_cast<const C2<R>*>(a.tw2);
-    C2<R> v[8];
-#pragma unroll
-    for (int i = 0; i < 8; ++i) {
-        const int k = j + i * T;
-        C2<R> xa = {0, 0}, xb = {0, 0};
-        if (live) {
-            xa = src[k];
-            xb = src[H - k];
-        }
-        if (k == 0) { xa.y = 0; xb.y = 0; }
-        const C2<R> bc = cconj(xb);
-        const C2<R> e = cadd(xa, bc), f = csub(xa, bc);
-        const C2<R> t = cmul(cconj(tw2[k]), f);
-        v[i] = {e.x - t.y, e.y + t.x};
-    }
-    column_fft<R, H, false, 0>(v, lds + rl * HP, j, reinterpret_cast<const C2<R>*>(a.tw));
-    if (live) {
-#pragma unroll
+struct x_c2r {
+    using Args = RowArgs;
+    static constexpr int NT = ROWS * H / 8;
+    static __device__ __forceinline__ void run(const RowArgs& a, Blk bk) {
+        constexpr int HP = H + H / 8;
+        __shared__ C2<R> lds[HP * ROWS];
+        constexpr int T = H / 8;
+        const int rl = threadIdx.x / T;
+        const int j = threadIdx.x % T;
+        const int blk = a.rev ? bk.nx - 1 - bk.x : bk.x;
+        const int row = blk * ROWS + rl;
+        const bool live = row < a.nrows;
+        const int y = live ? row % a.n1 : 0, z = live ? row / a.n1 : 0;
+        const C2<R>* src = reinterpret_cast<const C2<R>*>(a.src) + bk.y * a.comp_src + y * a.s_row_src + z * a.s_z_src;
+        double* dst = reinterpret_cast<double*>(a.dst) + bk.y * a.comp_dst + y * a.s_row_dst + z * a.s_z_dst;
+        const C2<R>* tw2 = reinterpret_cast<const C2<R>*>(a.tw2);
+        C2<R> v[8];
+    #pragma unroll
         for (int i = 0; i < 8; ++i) {
-            const int n = j + i * T;
-            if (MODE == 0) {
-                if (2 * n < a.nvalid) dst[2 * n] = (double)v[i].x;
-                if (2 * n + 1 < a.nvalid) dst[2 * n + 1] = (double)v[i].y;
-            } else if (MODE == 2 || i < 4) {
-                *reinterpret_cast<double2*>(dst + 2 * n) = double2{(double)v[i].x, (double)v[i].y};
+            const int k = j + i * T;
+            C2<R> xa = {0, 0}, xb = {0, 0};
+            if (live) {
+                xa = src[k];
+                xb = src[H - k];
+            }
+            if (k == 0) { xa.y = 0; xb.y = 0; }
+            const C2<R> bc = cconj(xb);
+            const C2<R> e = cadd(xa, bc), f = csub(xa, bc);
+            const C2<R> t = cmul(cconj(tw2[k]), f);
+            v[i] = {e.x - t.y, e.y + t.x};
+        }
+        column_fft<R, H, false, 0>(v, lds + rl * HP, j, reinterpret_cast<const C2<R>*>(a.tw));
+        if (live) {
+    #pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                const int n = j + i * T;
+                if (MODE == 0) {
+                    if (2 * n < a.nvalid) dst[2 * n] = (double)v[i].x;
+                    if (2 * n + 1 < a.nvalid) dst[2 * n + 1] = (double)v[i].y;
+                } else if (MODE == 2 || i < 4) {
+                    *reinterpret_cast<double2*>(dst + 2 * n) = double2{(double)v[i].x, (double)v[i].y};
+                }
             }
         }
     }
-}
+};
 
 // normfact * Re G of the general path's complex (nx, n1, n2) Green's function,
 // one octant, laid out [ky'][tile][kz'][tx] with tiles of tx x columns (zero past nx)
@@ -775,31 +806,30 @@ void launch_col_h(const ColArgs& a, int comps, int zsched, hipStream_t st) {
         // the general path's FP32 fused last direction: full- or half-width tiles, schedule 0
         const int tx = (zsched & 4) ? TX / 2 : TX;
         dim3 grid(((a.nx + tx - 1) / tx) * (a.nopair == 1 ? a.nother : a.gmy / 2 + 1), comps);
-        if (zsched & 4) hipLaunchKernelGGL((z_fused<R, M, TX / 2, 0, HALF, 1, S>), grid, dim3(TX / 2 * M / 8), 0, st, a);
-        else hipLaunchKernelGGL((z_fused<R, M, TX, 0, HALF, 1, S>), grid, dim3(TX * M / 8), 0, st, a);
+        if (zsched & 4) launch_k<z_fused<R, M, TX / 2, 0, HALF, 1, S>>(grid, dim3(TX / 2 * M / 8), 0, st, a);
+        else launch_k<z_fused<R, M, TX, 0, HALF, 1, S>>(grid, dim3(TX * M / 8), 0, st, a);
     } else if (FUSED) {
         // z schedules: bit 0-1 = SCHED of z_fused, 4 = half-width tiles (TX/2 columns)
         const int tx = (zsched & 8) ? TX / 4 : (zsched & 4) ? TX / 2 : TX;
         dim3 grid(((a.nx + tx - 1) / tx) * (a.nopair == 1 ? a.nother : a.nopair == 2 ? 2 * (a.gmy / 2 + 1) : a.gmy / 2 + 1), comps);
         switch (zsched & 15) {
-            case 0: hipLaunchKernelGGL((z_fused<R, M, TX, 0, HALF, 1, S>), grid, dim3(TX * M / 8), 0, st, a); break;
-            case 1: hipLaunchKernelGGL((z_fused<R, M, TX, 1, HALF, 1, S>), grid, dim3(TX * M / 8), 0, st, a); break;
-            case 2: hipLaunchKernelGGL((z_fused<R, M, TX, 2, HALF, 1, S>), grid, dim3(TX * M / 8), 0, st, a); break;
-            case 4: hipLaunchKernelGGL((z_fused<R, M, TX / 2, 0, HALF, 1, S>), grid, dim3(TX / 2 * M / 8), 0, st, a); break;
-            case 5: hipLaunchKernelGGL((z_fused<R, M, TX / 2, 1, HALF, 1, S>), grid, dim3(TX / 2 * M / 8), 0, st, a); break;
-            case 6: hipLaunchKernelGGL((z_fused<R, M, TX / 2, 2, HALF, 1, S>), grid, dim3(TX / 2 * M / 8), 0, st, a); break;
+            case 0: launch_k<z_fused<R, M, TX, 0, HALF, 1, S>>(grid, dim3(TX * M / 8), 0, st, a); break;
+            case 1: launch_k<z_fused<R, M, TX, 1, HALF, 1, S>>(grid, dim3(TX * M / 8), 0, st, a); break;
+            case 2: launch_k<z_fused<R, M, TX, 2, HALF, 1, S>>(grid, dim3(TX * M / 8), 0, st, a); break;
+            case 4: launch_k<z_fused<R, M, TX / 2, 0, HALF, 1, S>>(grid, dim3(TX / 2 * M / 8), 0, st, a); break;
+            case 5: launch_k<z_fused<R, M, TX / 2, 1, HALF, 1, S>>(grid, dim3(TX / 2 * M / 8), 0, st, a); break;
+            case 6: launch_k<z_fused<R, M, TX / 2, 2, HALF, 1, S>>(grid, dim3(TX / 2 * M / 8), 0, st, a); break;
             case 8:
-                hipLaunchKernelGGL((z_fused<R, M, (TX >= 8 ? TX / 4 : 1), 0, HALF, 1, S>), grid, dim3((TX >= 8 ? TX / 4 : 1) * M / 8), 0, st,
-                                   a);
+                launch_k<z_fused<R, M, (TX >= 8 ? TX / 4 : 1), 0, HALF, 1, S>>(grid, dim3((TX >= 8 ? TX / 4 : 1) * M / 8), 0, st, a);
                 break;
             default: fail("unknown z schedule %d", zsched);
         }
     } else if (zsched & 4) {   // y schedule: half-width tiles
         dim3 grid(((a.nx + TX / 2 - 1) / (TX / 2)) * a.nother, comps);
-        hipLaunchKernelGGL((col_fft<R, M, TX / 2, FWD, HALF>), grid, dim3(TX / 2 * M / 8), 0, st, a);
+        launch_k<col_fft<R, M, TX / 2, FWD, HALF>>(grid, dim3(TX / 2 * M / 8), 0, st, a);
     } else {
         dim3 grid(((a.nx + TX - 1) / TX) * a.nother, comps);
-        hipLaunchKernelGGL((col_fft<R, M, TX, FWD, HALF>), grid, dim3(TX * M / 8), 0, st, a);
+        launch_k<col_fft<R, M, TX, FWD, HALF>>(grid, dim3(TX * M / 8), 0, st, a);
     }
     HIP_CHECK(hipGetLastError());
 }
@@ -831,12 +861,12 @@ template <typename R, int TX>
 void launch_rot(int M, const ColArgs& a, const RotK& rk, hipStream_t st) {
     dim3 grid(((a.nx + TX - 1) / TX) * (a.gmy / 2 + 1), 1);
     switch (M) {
-        case 32: hipLaunchKernelGGL((z_rot<R, 32, TX>), grid, dim3(TX * 4), 0, st, a, rk); break;
-        case 64: hipLaunchKernelGGL((z_rot<R, 64, TX>), grid, dim3(TX * 8), 0, st, a, rk); break;
-        case 128: hipLaunchKernelGGL((z_rot<R, 128, TX>), grid, dim3(TX * 16), 0, st, a, rk); break;
-        case 256: hipLaunchKernelGGL((z_rot<R, 256, TX>), grid, dim3(TX * 32), 0, st, a, rk); break;
-        case 512: hipLaunchKernelGGL((z_rot<R, 512, TX>), grid, dim3(TX * 64), 0, st, a, rk); break;
-        case 1024: hipLaunchKernelGGL((z_rot<R, 1024, TX>), grid, dim3(TX * 128), 0, st, a, rk); break;
+        case 32: launch_k<z_rot<R, 32, TX>>(grid, dim3(TX * 4), 0, st, {a, rk}); break;
+        case 64: launch_k<z_rot<R, 64, TX>>(grid, dim3(TX * 8), 0, st, {a, rk}); break;
+        case 128: launch_k<z_rot<R, 128, TX>>(grid, dim3(TX * 16), 0, st, {a, rk}); break;
+        case 256: launch_k<z_rot<R, 256, TX>>(grid, dim3(TX * 32), 0, st, {a, rk}); break;
+        case 512: launch_k<z_rot<R, 512, TX>>(grid, dim3(TX * 64), 0, st, {a, rk}); break;
+        case 1024: launch_k<z_rot<R, 1024, TX>>(grid, dim3(TX * 128), 0, st, {a, rk}); break;
         default: fail("unsupported column length %d", M);
     }
     HIP_CHECK(hipGetLastError());
@@ -847,8 +877,8 @@ void row_launch(bool fwd, int H, const RowArgs& a, int comps, hipStream_t st) {
 #define FLUPS_ROW(HH, RR)                                                                                   \
     case HH: {                                                                                              \
         dim3 grid((a.nrows + RR - 1) / RR, comps);                                                          \
-        if (fwd) hipLaunchKernelGGL((x_r2c<R, HH, RR, MODE>), grid, dim3(RR * HH / 8), 0, st, a);           \
-        else hipLaunchKernelGGL((x_c2r<R, HH, RR, MODE>), grid, dim3(RR * HH / 8), 0, st, a);               \
+        if (fwd) launch_k<x_r2c<R, HH, RR, MODE>>(grid, dim3(RR * HH / 8), 0, st, a);                       \
+        else launch_k<x_c2r<R, HH, RR, MODE>>(grid, dim3(RR * HH / 8), 0, st, a);                           \
         break;                                                                                              \
     }
     switch (H) {
@@ -1227,7 +1257,7 @@ double hybrid_bytes(const HybridZ* h) { return h->bytes; }
 template <int M, int TX, typename S>
 void launch_z_pf3(const ColArgs& a, int comps, hipStream_t st) {
     dim3 grid(((a.nx + TX - 1) / TX) * (a.nopair == 1 ? a.nother : a.gmy / 2 + 1), comps);
-    hipLaunchKernelGGL((z_fused<double, M, TX, 0, false, 3, S>), grid, dim3(TX * M / 24), 0, st, a);
+    launch_k<z_fused<double, M, TX, 0, false, 3, S>>(grid, dim3(TX * M / 24), 0, st, a);
     HIP_CHECK(hipGetLastError());
 }
 template <int TX, typename S>
@@ -1257,14 +1287,14 @@ void launch_z_team(int tx, const ColArgs& a, int comps, hipStream_t st) {
     dim3 grid(ntx * a.nother, comps);
     switch (tx) {
         case 16:
-            if constexpr (PF * Q <= 192) hipLaunchKernelGGL((z_team<PF, Q, 16, S>), grid, dim3(16 * PF * Q / 8), 0, st, a);
+            if constexpr (PF * Q <= 192) launch_k<z_team<PF, Q, 16, S>>(grid, dim3(16 * PF * Q / 8), 0, st, a);
             else fail("column length %d with tiles of 16", PF * Q);
             break;
         case 8:
-            if constexpr (PF * Q <= 384) hipLaunchKernelGGL((z_team<PF, Q, 8, S>), grid, dim3(8 * PF * Q / 8), 0, st, a);
+            if constexpr (PF * Q <= 384) launch_k<z_team<PF, Q, 8, S>>(grid, dim3(8 * PF * Q / 8), 0, st, a);
             else fail("column length %d with tiles of 8", PF * Q);
             break;
-        default: hipLaunchKernelGGL((z_team<PF, Q, 4, S>), grid, dim3(4 * PF * Q / 8), 0, st, a); break;
+        default: launch_k<z_team<PF, Q, 4, S>>(grid, dim3(4 * PF * Q / 8), 0, st, a); break;
     }
     HIP_CHECK(hipGetLastError());
 }

@@ -18,6 +18,7 @@
 #include <algorithm>
 
 #include "core.hpp"
+#include "group.hpp"
 
 namespace flups {
 
@@ -123,203 +124,208 @@ __device__ void stockham_stage(const cd* in, cd* out, int M, int p, int span, co
 // TI / TO: storage of the source / destination reals (double, or float for the stage arrays of
 // FP32 solves); the pencil buffer and the arithmetic are FP64
 template <typename TI, typename TO>
-__global__ void __launch_bounds__(256) generic_pass(PassDesc P) {
-    extern __shared__ double smem[];
-    const int len = P.pen_in * P.nf_in;   // doubles per pencil (same before and after)
-    const int npen_total = P.n1 * P.n2;
-    const int pen0 = blockIdx.x * P.ppb;
-    const int npen = min(P.ppb, npen_total - pen0);
-    if (npen <= 0) return;
-    double* pb = smem;                                       // pencils, ppb * len doubles
-    cd* A = reinterpret_cast<cd*>(smem + (size_t)P.ppb * len + ((P.ppb * len) & 1));
-    cd* B = P.alias ? reinterpret_cast<cd*>(smem) : A + (size_t)P.ppb * P.M;   // free while the FFT runs
-    const int d = P.d;
+struct generic_pass {
+    using Args = PassDesc;
+    static constexpr int NT = 256;
+    static constexpr int kWaves = 2;   // at most 256 registers per lane: two waves per SIMD
+    static __device__ __forceinline__ void run(const PassDesc& P, Blk blk) {
+        extern __shared__ double smem[];
+        const int len = P.pen_in * P.nf_in;   // doubles per pencil (same before and after)
+        const int npen_total = P.n1 * P.n2;
+        const int pen0 = blk.x * P.ppb;
+        const int npen = min(P.ppb, npen_total - pen0);
+        if (npen <= 0) return;
+        double* pb = smem;                                       // pencils, ppb * len doubles
+        cd* A = reinterpret_cast<cd*>(smem + (size_t)P.ppb * len + ((P.ppb * len) & 1));
+        cd* B = P.alias ? reinterpret_cast<cd*>(smem) : A + (size_t)P.ppb * P.M;   // free while the FFT runs
+        const int d = P.d;
 
-    // ---- gather (pencil switch: shift, zero padding) ----
-    // Neighbouring threads take neighbouring elements of a pencil when the pencil
-    // is contiguous in the source, neighbouring pencils (adjacent along fa) otherwise.
-    {
-        const bool efast = P.src_s[d] == 1;
-        const int nfi = P.nf_in;
-        const int per = P.pen_in;   // elements per pencil
-        for (int i = threadIdx.x; i < npen * per; i += blockDim.x) {
-            const int pen = efast ? i / per : i % npen;
-            const int k = efast ? i - pen * per : i / npen;
-            const int g = pen0 + pen;
-            const int ja = g % P.na, jb = g / P.na;
-            const int ks = k - P.gshift;
-            double* o = pb + (size_t)pen * len + (size_t)k * nfi;
-            if (ks >= 0 && ks < P.src_nd) {
-                const long long idx = (long long)ks * P.src_s[d] + (long long)ja * P.src_s[P.fa] + (long long)jb * P.src_s[P.fb];
-                const TI* sp = reinterpret_cast<const TI*>(P.src) + idx * P.src_nf;
-                for (int c = 0; c < nfi; ++c) o[c] = (double)sp[c];
-            } else {
-                for (int c = 0; c < nfi; ++c) o[c] = 0.0;
+        // ---- gather (pencil switch: shift, zero padding) ----
+        // Neighbouring threads take neighbouring elements of a pencil when the pencil
+        // is contiguous in the source, neighbouring pencils (adjacent along fa) otherwise.
+        {
+            const bool efast = P.src_s[d] == 1;
+            const int nfi = P.nf_in;
+            const int per = P.pen_in;   // elements per pencil
+            for (int i = threadIdx.x; i < npen * per; i += blockDim.x) {
+                const int pen = efast ? i / per : i % npen;
+                const int k = efast ? i - pen * per : i / npen;
+                const int g = pen0 + pen;
+                const int ja = g % P.na, jb = g / P.na;
+                const int ks = k - P.gshift;
+                double* o = pb + (size_t)pen * len + (size_t)k * nfi;
+                if (ks >= 0 && ks < P.src_nd) {
+                    const long long idx = (long long)ks * P.src_s[d] + (long long)ja * P.src_s[P.fa] + (long long)jb * P.src_s[P.fb];
+                    const TI* sp = reinterpret_cast<const TI*>(P.src) + idx * P.src_nf;
+                    for (int c = 0; c < nfi; ++c) o[c] = (double)sp[c];
+                } else {
+                    for (int c = 0; c < nfi; ++c) o[c] = 0.0;
+                }
             }
         }
-    }
-    __syncthreads();
-
-    // ---- pre-map: pencil -> complex sequence of length M ----
-    const int n = P.n, M = P.M, st = P.r2r_stride;
-    int kind = P.kind;
-    // DST kinds are computed through the matching DCT
-    const bool dst = (kind >= RODFT10);
-    const int dct = dst ? kind - 4 : kind;   // RODFT10 -> REDFT10, RODFT01 -> REDFT01, RODFT11 -> REDFT11
-    for (int i = threadIdx.x; i < npen * M; i += blockDim.x) {
-        const int pen = i / M, j = i - pen * M;
-        const double* x = pb + (size_t)pen * len;
-        cd v = {0, 0};
-        if (P.op == OP_C2C) {
-            v = mk(x[2 * (P.si + j)], x[2 * (P.si + j) + 1]);
-        } else if (P.op == OP_R2C) {
-            if ((n & 1) == 0) v = mk(x[P.si + 2 * j], x[P.si + 2 * j + 1]);
-            else v = mk(x[P.si + j], 0.0);
-        } else if (P.op == OP_C2R) {
-            if ((n & 1) == 0) {
-                const int h = n / 2;
-                cd a = mk(x[2 * (P.si + j)], x[2 * (P.si + j) + 1]);
-                cd b = mk(x[2 * (P.si + h - j)], x[2 * (P.si + h - j) + 1]);
-                if (j == 0) { a.y = 0.0; b.y = 0.0; }   // DC and Nyquist are real
-                const cd bc = conj(b);
-                const cd e = add(a, bc), f = sub(a, bc);
-                const cd w = conj(expmpi(2.0 * j / n));          // exp(+2 pi i j/n)
-                const cd t = mul(w, f);
-                v = add(e, mk(-t.y, t.x));                          // e + i * w * f
-            } else {
-                if (j == 0) v = mk(x[2 * P.si], 0.0);
-                else if (j <= n / 2) v = mk(x[2 * (P.si + j)], x[2 * (P.si + j) + 1]);
-                else v = conj(mk(x[2 * (P.si + n - j)], x[2 * (P.si + n - j) + 1]));
-            }
-        } else if (P.op == OP_R2R) {
-            // input element jj of the (possibly reflected) real sequence
-            auto xin = [&](int jj) -> double {
-                if (kind == RODFT10) return (jj & 1) ? -x[P.si + st * jj] : x[P.si + st * jj];
-                if (kind == RODFT01 || kind == RODFT11) return x[P.si + st * (n - 1 - jj)];
-                return x[P.si + st * jj];
-            };
-            if (kind == REDFT00) {
-                const int jj = (j < n) ? j : M - j;
-                v = mk(xin(jj), 0.0);
-            } else if (kind == RODFT00) {
-                if (j >= 1 && j <= n) v = mk(xin(j - 1), 0.0);
-                else if (j >= n + 2) v = mk(-xin(M - 1 - j), 0.0);
-            } else if (dct == REDFT10) {
-                v = mk(xin(j < n ? j : M - 1 - j), 0.0);
-            } else if (dct == REDFT01) {
-                if (j < n) v = scal(conj(expmpi(0.5 * j / n)), (j == 0 ? 1.0 : 2.0) * xin(j));
-            } else {   // REDFT11
-                if (j < n) v = scal(expmpi(0.5 * j / n), xin(j));
-            }
-        }
-        A[(size_t)pen * M + j] = v;
-    }
-    __syncthreads();
-
-    // ---- FFT ----
-    int sgn = P.sign;
-    if (P.op == OP_R2C) sgn = FWD;
-    if (P.op == OP_C2R) sgn = BWD;
-    if (P.op == OP_R2R) sgn = (dct == REDFT01) ? BWD : FWD;
-    cd* in = A;
-    cd* out = B;
-    int span = 1;
-    for (int s = 0; s < P.nst; ++s) {
-        stockham_stage(in, out, M, P.rad[s], span, P.tw, sgn, npen);
-        span *= P.rad[s];
         __syncthreads();
-        cd* t = in; in = out; out = t;
-    }
-    if (P.alias && in != A) {   // the result sits in the pencil buffer, which the post-map writes: move it
-        for (int i = threadIdx.x; i < npen * M; i += blockDim.x) A[i] = in[i];
+
+        // ---- pre-map: pencil -> complex sequence of length M ----
+        const int n = P.n, M = P.M, st = P.r2r_stride;
+        int kind = P.kind;
+        // DST kinds are computed through the matching DCT
+        const bool dst = (kind >= RODFT10);
+        const int dct = dst ? kind - 4 : kind;   // RODFT10 -> REDFT10, RODFT01 -> REDFT01, RODFT11 -> REDFT11
+        for (int i = threadIdx.x; i < npen * M; i += blockDim.x) {
+            const int pen = i / M, j = i - pen * M;
+            const double* x = pb + (size_t)pen * len;
+            cd v = {0, 0};
+            if (P.op == OP_C2C) {
+                v = mk(x[2 * (P.si + j)], x[2 * (P.si + j) + 1]);
+            } else if (P.op == OP_R2C) {
+                if ((n & 1) == 0) v = mk(x[P.si + 2 * j], x[P.si + 2 * j + 1]);
+                else v = mk(x[P.si + j], 0.0);
+            } else if (P.op == OP_C2R) {
+                if ((n & 1) == 0) {
+                    const int h = n / 2;
+                    cd a = mk(x[2 * (P.si + j)], x[2 * (P.si + j) + 1]);
+                    cd b = mk(x[2 * (P.si + h - j)], x[2 * (P.si + h - j) + 1]);
+                    if (j == 0) { a.y = 0.0; b.y = 0.0; }   // DC and Nyquist are real
+                    const cd bc = conj(b);
+                    const cd e = add(a, bc), f = sub(a, bc);
+                    const cd w = conj(expmpi(2.0 * j / n));          // exp(+2 pi i j/n)
+                    const cd t = mul(w, f);
+                    v = add(e, mk(-t.y, t.x));                          // e + i * w * f
+                } else {
+                    if (j == 0) v = mk(x[2 * P.si], 0.0);
+                    else if (j <= n / 2) v = mk(x[2 * (P.si + j)], x[2 * (P.si + j) + 1]);
+                    else v = conj(mk(x[2 * (P.si + n - j)], x[2 * (P.si + n - j) + 1]));
+                }
+            } else if (P.op == OP_R2R) {
+                // input element jj of the (possibly reflected) real sequence
+                auto xin = [&](int jj) -> double {
+                    if (kind == RODFT10) return (jj & 1) ? -x[P.si + st * jj] : x[P.si + st * jj];
+                    if (kind == RODFT01 || kind == RODFT11) return x[P.si + st * (n - 1 - jj)];
+                    return x[P.si + st * jj];
+                };
+                if (kind == REDFT00) {
+                    const int jj = (j < n) ? j : M - j;
+                    v = mk(xin(jj), 0.0);
+                } else if (kind == RODFT00) {
+                    if (j >= 1 && j <= n) v = mk(xin(j - 1), 0.0);
+                    else if (j >= n + 2) v = mk(-xin(M - 1 - j), 0.0);
+                } else if (dct == REDFT10) {
+                    v = mk(xin(j < n ? j : M - 1 - j), 0.0);
+                } else if (dct == REDFT01) {
+                    if (j < n) v = scal(conj(expmpi(0.5 * j / n)), (j == 0 ? 1.0 : 2.0) * xin(j));
+                } else {   // REDFT11
+                    if (j < n) v = scal(expmpi(0.5 * j / n), xin(j));
+                }
+            }
+            A[(size_t)pen * M + j] = v;
+        }
         __syncthreads();
-        in = A;
-    }
-    const cd* R = in;
 
-    // ---- post-map: sequence -> pencil ----
-    int nwrite = 0;
-    if (P.op == OP_C2C) nwrite = n;
-    else if (P.op == OP_R2C) nwrite = n / 2 + 1;
-    else if (P.op == OP_C2R) nwrite = ((n & 1) == 0) ? n / 2 : n;
-    else if (P.op == OP_R2R) nwrite = n;
-    for (int i = threadIdx.x; i < npen * nwrite; i += blockDim.x) {
-        const int pen = i / nwrite, k = i - pen * nwrite;
-        double* x = pb + (size_t)pen * len;
-        const cd* r = R + (size_t)pen * M;
-        if (P.op == OP_C2C) {
-            x[2 * (P.so + k)] = r[k].x;
-            x[2 * (P.so + k) + 1] = r[k].y;
-        } else if (P.op == OP_R2C) {
-            cd X;
-            if ((n & 1) == 0) {
-                const int h = n / 2;
-                const cd zk = r[k % h], zc = conj(r[(h - k) % h]);
-                const cd e = scal(add(zk, zc), 0.5);
-                const cd df = scal(sub(zk, zc), 0.5);
-                const cd o = mk(df.y, -df.x);                       // (zk - zc) / (2 i)
-                X = add(e, mul(expmpi(2.0 * k / n), o));
+        // ---- FFT ----
+        int sgn = P.sign;
+        if (P.op == OP_R2C) sgn = FWD;
+        if (P.op == OP_C2R) sgn = BWD;
+        if (P.op == OP_R2R) sgn = (dct == REDFT01) ? BWD : FWD;
+        cd* in = A;
+        cd* out = B;
+        int span = 1;
+        for (int s = 0; s < P.nst; ++s) {
+            stockham_stage(in, out, M, P.rad[s], span, P.tw, sgn, npen);
+            span *= P.rad[s];
+            __syncthreads();
+            cd* t = in; in = out; out = t;
+        }
+        if (P.alias && in != A) {   // the result sits in the pencil buffer, which the post-map writes: move it
+            for (int i = threadIdx.x; i < npen * M; i += blockDim.x) A[i] = in[i];
+            __syncthreads();
+            in = A;
+        }
+        const cd* R = in;
+
+        // ---- post-map: sequence -> pencil ----
+        int nwrite = 0;
+        if (P.op == OP_C2C) nwrite = n;
+        else if (P.op == OP_R2C) nwrite = n / 2 + 1;
+        else if (P.op == OP_C2R) nwrite = ((n & 1) == 0) ? n / 2 : n;
+        else if (P.op == OP_R2R) nwrite = n;
+        for (int i = threadIdx.x; i < npen * nwrite; i += blockDim.x) {
+            const int pen = i / nwrite, k = i - pen * nwrite;
+            double* x = pb + (size_t)pen * len;
+            const cd* r = R + (size_t)pen * M;
+            if (P.op == OP_C2C) {
+                x[2 * (P.so + k)] = r[k].x;
+                x[2 * (P.so + k) + 1] = r[k].y;
+            } else if (P.op == OP_R2C) {
+                cd X;
+                if ((n & 1) == 0) {
+                    const int h = n / 2;
+                    const cd zk = r[k % h], zc = conj(r[(h - k) % h]);
+                    const cd e = scal(add(zk, zc), 0.5);
+                    const cd df = scal(sub(zk, zc), 0.5);
+                    const cd o = mk(df.y, -df.x);                       // (zk - zc) / (2 i)
+                    X = add(e, mul(expmpi(2.0 * k / n), o));
+                } else {
+                    X = r[k];
+                }
+                x[2 * (P.so + k)] = X.x;
+                x[2 * (P.so + k) + 1] = X.y;
+            } else if (P.op == OP_C2R) {
+                if ((n & 1) == 0) {
+                    x[P.so + 2 * k] = r[k].x;
+                    x[P.so + 2 * k + 1] = r[k].y;
+                } else {
+                    x[P.so + k] = r[k].x;
+                }
             } else {
-                X = r[k];
+                double y;
+                int kk = k;
+                if (kind == RODFT10) kk = n - 1 - k;   // DST-II(x)[k] = DCT-II((-1)^j x)[n-1-k]
+                if (kind == REDFT00) y = r[kk].x;
+                else if (kind == RODFT00) y = -r[kk + 1].y;
+                else if (dct == REDFT10) y = mul(expmpi(0.5 * kk / n), r[kk]).x;
+                else if (dct == REDFT01) y = r[kk].x;
+                else y = 2.0 * mul(expmpi((2.0 * kk + 1.0) / (4.0 * n)), r[kk]).x;
+                if ((kind == RODFT01 || kind == RODFT11) && (k & 1)) y = -y;   // (-1)^k reflection
+                x[P.so + st * k] = y;
             }
-            x[2 * (P.so + k)] = X.x;
-            x[2 * (P.so + k) + 1] = X.y;
-        } else if (P.op == OP_C2R) {
-            if ((n & 1) == 0) {
-                x[P.so + 2 * k] = r[k].x;
-                x[P.so + 2 * k + 1] = r[k].y;
-            } else {
-                x[P.so + k] = r[k].x;
+        }
+        __syncthreads();
+
+        // ---- post-correction ----
+        if (threadIdx.x < npen) {
+            double* x = pb + (size_t)threadIdx.x * len;
+            const int pp = P.post;
+            const bool f = pp & POST_FIRST, l = pp & POST_LAST;
+            if (f) x[0] = 0.0;
+            if (l) x[P.pen_out - 1] = 0.0;
+            if (!f && !l && (pp & POST_PERIOD)) {
+                const int nf = (P.sign == FWD) ? P.n_out_plan - 1 : P.n;
+                if (P.op == OP_C2R) x[nf] = x[0];
+                else { x[2 * nf] = x[0]; x[2 * nf + 1] = x[1]; }
             }
-        } else {
-            double y;
-            int kk = k;
-            if (kind == RODFT10) kk = n - 1 - k;   // DST-II(x)[k] = DCT-II((-1)^j x)[n-1-k]
-            if (kind == REDFT00) y = r[kk].x;
-            else if (kind == RODFT00) y = -r[kk + 1].y;
-            else if (dct == REDFT10) y = mul(expmpi(0.5 * kk / n), r[kk]).x;
-            else if (dct == REDFT01) y = r[kk].x;
-            else y = 2.0 * mul(expmpi((2.0 * kk + 1.0) / (4.0 * n)), r[kk]).x;
-            if ((kind == RODFT01 || kind == RODFT11) && (k & 1)) y = -y;   // (-1)^k reflection
-            x[P.so + st * k] = y;
         }
-    }
-    __syncthreads();
+        __syncthreads();
 
-    // ---- post-correction ----
-    if (threadIdx.x < npen) {
-        double* x = pb + (size_t)threadIdx.x * len;
-        const int pp = P.post;
-        const bool f = pp & POST_FIRST, l = pp & POST_LAST;
-        if (f) x[0] = 0.0;
-        if (l) x[P.pen_out - 1] = 0.0;
-        if (!f && !l && (pp & POST_PERIOD)) {
-            const int nf = (P.sign == FWD) ? P.n_out_plan - 1 : P.n;
-            if (P.op == OP_C2R) x[nf] = x[0];
-            else { x[2 * nf] = x[0]; x[2 * nf + 1] = x[1]; }
+        // ---- scatter (truncation on the way back) ----
+        {
+            const bool efast = P.dst_s[d] == 1;
+            const int nfo = P.dst_nf;
+            const int per = P.dst_nd;
+            for (int i = threadIdx.x; i < npen * per; i += blockDim.x) {
+                const int pen = efast ? i / per : i % npen;
+                const int k = efast ? i - pen * per : i / npen;
+                const int kt = k + P.sshift;
+                if (kt >= P.pen_out) continue;
+                const int g = pen0 + pen;
+                const int ja = g % P.na, jb = g / P.na;
+                const double* v = pb + (size_t)pen * len + (size_t)kt * P.nf_out;
+                const long long idx = (long long)k * P.dst_s[d] + (long long)ja * P.dst_s[P.fa] + (long long)jb * P.dst_s[P.fb];
+                TO* dp = reinterpret_cast<TO*>(P.dst) + idx * nfo;
+                for (int c = 0; c < nfo; ++c) dp[c] = (TO)v[c];
+            }
         }
     }
-    __syncthreads();
-
-    // ---- scatter (truncation on the way back) ----
-    {
-        const bool efast = P.dst_s[d] == 1;
-        const int nfo = P.dst_nf;
-        const int per = P.dst_nd;
-        for (int i = threadIdx.x; i < npen * per; i += blockDim.x) {
-            const int pen = efast ? i / per : i % npen;
-            const int k = efast ? i - pen * per : i / npen;
-            const int kt = k + P.sshift;
-            if (kt >= P.pen_out) continue;
-            const int g = pen0 + pen;
-            const int ja = g % P.na, jb = g / P.na;
-            const double* v = pb + (size_t)pen * len + (size_t)kt * P.nf_out;
-            const long long idx = (long long)k * P.dst_s[d] + (long long)ja * P.dst_s[P.fa] + (long long)jb * P.dst_s[P.fb];
-            TO* dp = reinterpret_cast<TO*>(P.dst) + idx * nfo;
-            for (int c = 0; c < nfo; ++c) dp[c] = (TO)v[c];
-        }
-    }
-}
+};
 
 // B aliases the pencil buffer when that is at least as large (c2c and r2c/c2r pencils)
 bool generic_alias(const PassDesc& P) { return (size_t)P.pen_in * P.nf_in >= 2 * (size_t)P.M; }
@@ -337,10 +343,10 @@ void launch_generic(const PassDesc& P, hipStream_t s) {
     // threads: enough for one element (or butterfly) each of the block's pencils, whole waves, at most 256
     const int work = P.ppb * std::max(P.M, P.pen_in);
     const int threads = std::min(256, std::max(64, (work + 63) / 64 * 64));
-    if (!P.src_f32 && !P.dst_f32) hipLaunchKernelGGL((generic_pass<double, double>), dim3(blocks), dim3(threads), lds, s, P);
-    else if (!P.src_f32) hipLaunchKernelGGL((generic_pass<double, float>), dim3(blocks), dim3(threads), lds, s, P);
-    else if (P.dst_f32) hipLaunchKernelGGL((generic_pass<float, float>), dim3(blocks), dim3(threads), lds, s, P);
-    else hipLaunchKernelGGL((generic_pass<float, double>), dim3(blocks), dim3(threads), lds, s, P);
+    if (!P.src_f32 && !P.dst_f32) launch_k<generic_pass<double, double>>(dim3(blocks), dim3(threads), lds, s, P);
+    else if (!P.src_f32) launch_k<generic_pass<double, float>>(dim3(blocks), dim3(threads), lds, s, P);
+    else if (P.dst_f32) launch_k<generic_pass<float, float>>(dim3(blocks), dim3(threads), lds, s, P);
+    else launch_k<generic_pass<float, double>>(dim3(blocks), dim3(threads), lds, s, P);
     HIP_CHECK(hipGetLastError());
 }
 

@@ -17,8 +17,11 @@
 // src/dothemagic_rot.ipp:102-349.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "core.hpp"
 #include "green.hpp"
+#include "group.hpp"
 
 namespace flups {
 
@@ -607,72 +610,129 @@ __global__ void zero_first(double* g) { g[0] = 0.0; }
 
 // S: storage of the data and the Green's function (double, or float for FP32 solves); FP64 arithmetic
 template <typename S>
-__global__ void mult_std(S* d, const S* g, size_t per, int nf, int lda, double normfact) {
-    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < per; i += (size_t)gridDim.x * blockDim.x) {
-        for (int c = 0; c < lda; ++c) {
-            S* x = d + c * per * nf;
-            if (nf == 1) {
-                x[i] = (S)((double)x[i] * normfact * (double)g[i]);
-            } else {
-                const double a = x[2 * i], b = x[2 * i + 1], gr = g[2 * i], gi = g[2 * i + 1];
-                x[2 * i] = (S)(normfact * (a * gr - b * gi));
-                x[2 * i + 1] = (S)(normfact * (a * gi + b * gr));
+struct MultArgs {
+    S* d;
+    const S* g;
+    size_t per;
+    int nf, lda;
+    double normfact;
+};
+template <typename S>
+struct mult_std {
+    using Args = MultArgs<S>;
+    static constexpr int NT = 256;
+    static __device__ __forceinline__ void run(const Args& a, Blk blk) {
+        const size_t per = a.per;
+        const int nf = a.nf;
+        for (size_t i = blk.x * (size_t)NT + threadIdx.x; i < per; i += (size_t)blk.nx * NT) {
+            for (int c = 0; c < a.lda; ++c) {
+                S* x = a.d + c * per * nf;
+                if (nf == 1) {
+                    x[i] = (S)((double)x[i] * a.normfact * (double)a.g[i]);
+                } else {
+                    const double p = x[2 * i], q = x[2 * i + 1], gr = a.g[2 * i], gi = a.g[2 * i + 1];
+                    x[2 * i] = (S)(a.normfact * (p * gr - q * gi));
+                    x[2 * i + 1] = (S)(a.normfact * (p * gi + q * gr));
+                }
             }
         }
     }
-}
+};
 
 __global__ void to_float(const double* a, float* b, size_t n) {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) b[i] = (float)a[i];
 }
 
 template <typename S>
-__global__ void mult_rot(S* d, const S* g, RotArgs R) {
-    const size_t per = (size_t)R.n[0] * R.n[1] * R.n[2];
-    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < per; e += (size_t)gridDim.x * blockDim.x) {
-        const int i0 = (int)(e % R.n[0]), i1 = (int)((e / R.n[0]) % R.n[1]), i2 = (int)(e / ((size_t)R.n[0] * R.n[1]));
-        const int is[3] = {symid(i0, R.symstart[0]), symid(i1, R.symstart[1]), symid(i2, R.symstart[2])};
-        double Kr[3][3], Ki[3][3];
-        for (int dd = 0; dd < 3; ++dd)
-            for (int c = 0; c < 3; ++c) {
-                const double base = is[dd] + R.koffset[dd];
-                const double hh = R.h[dd];
-                auto sym = [&](double k) -> double {
-                    switch (R.order) {
-                        case SPE: return k;
-                        case FD2: return sin(k * hh) / hh;
-                        case FD4: return (4.0 / 3.0) * sin(k * hh) / hh - (1.0 / 6.0) * sin(2.0 * k * hh) / hh;
-                        default: return 1.5 * sin(k * hh) / hh - 0.3 * sin(2.0 * k * hh) / hh + (1.0 / 30.0) * sin(3.0 * k * hh) / hh;
+struct RotMultArgs {
+    S* d;
+    const S* g;
+    RotArgs R;
+};
+template <typename S>
+struct mult_rot {
+    using Args = RotMultArgs<S>;
+    static constexpr int NT = 256;
+    // derivative symbol along one axis at k h: k itself (spectral), the modified wavenumber of
+    // the centred differences of order 2, 4, 6 otherwise. The difference symbols are one
+    // out-of-line function: inlined, their sines (18 per point) made the kernel tens of thousands
+    // of instructions long, and the spectral path paid for it in instruction-cache misses.
+    static __device__ __noinline__ double sym_fd(int order, double k, double hh) {
+        switch (order) {
+            case FD2: return sin(k * hh) / hh;
+            case FD4: return (4.0 / 3.0) * sin(k * hh) / hh - (1.0 / 6.0) * sin(2.0 * k * hh) / hh;
+            default: return 1.5 * sin(k * hh) / hh - 0.3 * sin(2.0 * k * hh) / hh + (1.0 / 30.0) * sin(3.0 * k * hh) / hh;
+        }
+    }
+    // I: index type, 32-bit below 2^31 points per component (the divisions by the extents are
+    // then 32-bit ones), 64-bit above; FD: centred-difference symbols (else spectral, k itself)
+    template <typename I, bool FD>
+    static __device__ __forceinline__ void body(const Args& a, Blk blk) {
+        auto sym = [](int order, double k, double hh) -> double { return FD ? sym_fd(order, k, hh) : k; };
+        S* d = a.d;
+        const S* g = a.g;
+        const RotArgs& R = a.R;
+        const I n0 = (I)R.n[0], n1 = (I)R.n[1];
+        const I per = n0 * n1 * (I)R.n[2];
+        for (I e = (I)blk.x * NT + threadIdx.x; e < per; e += (I)blk.nx * NT) {
+            const I r = e / n0;
+            const int i0 = (int)(e - r * n0), i1 = (int)(r % n1), i2 = (int)(r / n1);
+            const int is[3] = {symid(i0, R.symstart[0]), symid(i1, R.symstart[1]), symid(i2, R.symstart[2])};
+            double Kr[3][3], Ki[3][3];
+            for (int dd = 0; dd < 3; ++dd)
+                for (int c = 0; c < 3; ++c) {
+                    const double base = is[dd] + R.koffset[dd];
+                    if (R.nf == 1) {
+                        Kr[dd][c] = sym(R.order, base * (R.kf[dd][c][0] + R.kf[dd][c][1]), R.h[dd]);
+                        Ki[dd][c] = 0.0;
+                    } else {
+                        Kr[dd][c] = sym(R.order, base * R.kf[dd][c][0], R.h[dd]);
+                        Ki[dd][c] = sym(R.order, base * R.kf[dd][c][1], R.h[dd]);
                     }
-                };
-                if (R.nf == 1) {
-                    Kr[dd][c] = sym(base * (R.kf[dd][c][0] + R.kf[dd][c][1]));
-                    Ki[dd][c] = 0.0;
+                }
+            // complex values as one 8- or 16-byte load each
+            using V = typename std::conditional<std::is_same<S, float>::value, float2, double2>::type;
+            double fr[3], fi[3];
+            for (int c = 0; c < 3; ++c) {
+                if (R.nf == 2) {
+                    const V v = reinterpret_cast<const V*>(d)[c * per + e];
+                    fr[c] = (double)v.x;
+                    fi[c] = (double)v.y;
                 } else {
-                    Kr[dd][c] = sym(base * R.kf[dd][c][0]);
-                    Ki[dd][c] = sym(base * R.kf[dd][c][1]);
+                    fr[c] = (double)d[c * per + e];
+                    fi[c] = 0.0;
                 }
             }
-        double fr[3], fi[3];
-        for (int c = 0; c < 3; ++c) {
-            fr[c] = (double)d[(c * per + e) * R.nf];
-            fi[c] = R.nf == 2 ? (double)d[(c * per + e) * 2 + 1] : 0.0;
-        }
-        auto dr = [&](int c, int dd) { return fr[c] * Kr[dd][c] - fi[c] * Ki[dd][c]; };
-        auto di = [&](int c, int dd) { return fr[c] * Ki[dd][c] + fi[c] * Kr[dd][c]; };
-        const double rr[3] = {dr(2, 1) - dr(1, 2), dr(0, 2) - dr(2, 0), dr(1, 0) - dr(0, 1)};
-        const double ri[3] = {di(2, 1) - di(1, 2), di(0, 2) - di(2, 0), di(1, 0) - di(0, 1)};
-        if (R.nf == 1) {
-            for (int c = 0; c < 3; ++c) d[c * per + e] = (S)(R.normfact * rr[c] * (double)g[e]);
-        } else {
-            const double gr = g[2 * e], gi = g[2 * e + 1];
-            for (int c = 0; c < 3; ++c) {
-                d[(c * per + e) * 2] = (S)(R.normfact * (rr[c] * gr - ri[c] * gi));
-                d[(c * per + e) * 2 + 1] = (S)(R.normfact * (rr[c] * gi + ri[c] * gr));
+            auto dr = [&](int c, int dd) { return fr[c] * Kr[dd][c] - fi[c] * Ki[dd][c]; };
+            auto di = [&](int c, int dd) { return fr[c] * Ki[dd][c] + fi[c] * Kr[dd][c]; };
+            const double rr[3] = {dr(2, 1) - dr(1, 2), dr(0, 2) - dr(2, 0), dr(1, 0) - dr(0, 1)};
+            const double ri[3] = {di(2, 1) - di(1, 2), di(0, 2) - di(2, 0), di(1, 0) - di(0, 1)};
+            if (R.nf == 1) {
+                for (int c = 0; c < 3; ++c) d[c * per + e] = (S)(R.normfact * rr[c] * (double)g[e]);
+            } else {
+                const V gv = reinterpret_cast<const V*>(g)[e];
+                const double gr = gv.x, gi = gv.y;
+                for (int c = 0; c < 3; ++c) {
+                    V o;
+                    o.x = (S)(R.normfact * (rr[c] * gr - ri[c] * gi));
+                    o.y = (S)(R.normfact * (rr[c] * gi + ri[c] * gr));
+                    reinterpret_cast<V*>(d)[c * per + e] = o;
+                }
             }
         }
     }
-}
+    static __device__ __forceinline__ void run(const Args& a, Blk blk) {
+        const size_t per = (size_t)a.R.n[0] * a.R.n[1] * a.R.n[2];
+        const bool fd = a.R.order != SPE;
+        if (3 * per * a.R.nf < (1u << 31)) {
+            if (fd) body<unsigned, true>(a, blk);
+            else body<unsigned, false>(a, blk);
+        } else {
+            if (fd) body<size_t, true>(a, blk);
+            else body<size_t, false>(a, blk);
+        }
+    }
+};
 
 unsigned grid_for(size_t n) {
     size_t b = (n + 255) / 256;
@@ -709,19 +769,15 @@ void launch_scale(double* g, size_t n, double f, hipStream_t s) {
 
 // f32: the data and the Green's function are float arrays (FP32 solves), else double
 void launch_mult_std(void* d, const void* g, size_t per, int nf, int lda, double normfact, bool f32, hipStream_t s) {
-    if (f32)
-        hipLaunchKernelGGL((mult_std<float>), dim3(grid_for(per)), dim3(256), 0, s, (float*)d, (const float*)g, per, nf, lda,
-                           normfact);
-    else
-        hipLaunchKernelGGL((mult_std<double>), dim3(grid_for(per)), dim3(256), 0, s, (double*)d, (const double*)g, per, nf,
-                           lda, normfact);
+    if (f32) launch_k<mult_std<float>>(dim3(grid_for(per)), dim3(256), 0, s, {(float*)d, (const float*)g, per, nf, lda, normfact});
+    else launch_k<mult_std<double>>(dim3(grid_for(per)), dim3(256), 0, s, {(double*)d, (const double*)g, per, nf, lda, normfact});
     HIP_CHECK(hipGetLastError());
 }
 
 void launch_mult_rot(void* d, const void* g, const RotArgs& R, bool f32, hipStream_t s) {
     const size_t per = (size_t)R.n[0] * R.n[1] * R.n[2];
-    if (f32) hipLaunchKernelGGL((mult_rot<float>), dim3(grid_for(per)), dim3(256), 0, s, (float*)d, (const float*)g, R);
-    else hipLaunchKernelGGL((mult_rot<double>), dim3(grid_for(per)), dim3(256), 0, s, (double*)d, (const double*)g, R);
+    if (f32) launch_k<mult_rot<float>>(dim3(grid_for(per)), dim3(256), 0, s, {(float*)d, (const float*)g, R});
+    else launch_k<mult_rot<double>>(dim3(grid_for(per)), dim3(256), 0, s, {(double*)d, (const double*)g, R});
     HIP_CHECK(hipGetLastError());
 }
 

@@ -71,6 +71,7 @@
 
 #include "core.hpp"
 #include "fft_core.hpp"
+#include "group.hpp"
 
 namespace flups {
 
@@ -86,391 +87,403 @@ namespace {
 // H = PF Q: PF = 1 (a power of two) or 3, 5, 7 (PF teams of Q/8 threads, fft_pq_team);
 // H/8 threads per pencil either way
 template <int PF, int Q, int R, int KIND, bool COL, typename TI, typename TO>
-__global__ void __launch_bounds__(R * PF * Q / 8) pen_r2r(PassDesc P) {
-    constexpr int H = PF * Q, T = Q / 8, TT = H / 8, N = 2 * H, HP = H + H / 8;
-    constexpr int S = COL ? R : 1;   // LDS stride between consecutive elements of a pencil
-    __shared__ C2<double> lds[COL ? (H + 1) * R : HP * R];
-    constexpr int DCT = KIND >= RODFT10 ? KIND - 4 : KIND;
-    // DCT-I / DST-I (node-centred symmetric directions): the even / odd extension of length 2H
-    // through the packed real FFT; n = H + 1 (DCT-I) or H - 1 (DST-I) points in and out
-    constexpr bool E00 = KIND == REDFT00 || KIND == RODFT00;
-    constexpr int NT = KIND == REDFT00 ? H + 1 : KIND == RODFT00 ? H - 1 : N;   // transform outputs
-    constexpr bool ALT = KIND == RODFT10;                       // input (-1)^n x, output reversed
-    constexpr bool REV = KIND == RODFT01 || KIND == RODFT11;    // input reversed, output (-1)^k
-    int rl, tt, ja, jb;
-    bool live;
-    if (COL) {
-        rl = threadIdx.x % R;
-        tt = threadIdx.x / R;
-        const int ntile = (P.na + R - 1) / R;
-        const int tile = blockIdx.x % ntile;
-        jb = blockIdx.x / ntile;
-        ja = tile * R + rl;
-        live = ja < P.na;
-        if (!live) ja = P.na - 1;   // dead pencils read the last one and store nothing
-    } else {
-        rl = threadIdx.x / TT;
-        tt = threadIdx.x % TT;
-        const int g = blockIdx.x * R + rl;
-        live = g < P.n1 * P.n2;
-        const int gg = live ? g : 0;   // dead rows read pencil 0 and store nothing
-        ja = gg % P.na;
-        jb = gg / P.na;
-    }
-    const int k1 = tt / T, j = tt % T;   // team and thread in the team
-    const TI* src = reinterpret_cast<const TI*>(P.src) + (long long)ja * P.src_s[P.fa] + (long long)jb * P.src_s[P.fb];
-    TO* dst = reinterpret_cast<TO*>(P.dst) + (long long)ja * P.dst_s[P.fa] + (long long)jb * P.dst_s[P.fb];
-    const long long sd = P.src_s[P.d], dd = P.dst_s[P.d];
-    C2<double>* row = COL ? lds + rl : lds + rl * HP;                                    // complex view
-    double* rdb = COL ? reinterpret_cast<double*>(lds) + rl : reinterpret_cast<double*>(row);   // real view
-    auto RD = [&](int k) -> double& { return rdb[k * S]; };
-    auto RW = [&](int k) -> C2<double>& { return row[k * S]; };
-    const C2<double>* A = reinterpret_cast<const C2<double>*>(P.aux);
-    const C2<double>* B = A + (H + 1);
-    const C2<double>* tw = reinterpret_cast<const C2<double>*>(P.tw);   // Q-point table
-    const C2<double>* twh = B + (H + 1);                                 // H-point table (PF > 1)
-    const int pin = P.pen_in;   // pencil buffer length (the same after the transform), at most N + 2
-
-    // 1. gather the pencil into LDS (field shift, zero padding)
-    if (!COL && P.vec) {
-#pragma unroll
-        for (int t = 0; t < 8; ++t) {
-            const int q = tt + t * TT;
-            const vec2_t<TI> p = reinterpret_cast<const vec2_t<TI>*>(src)[q];
-            RD(2 * q) = (double)p.x;
-            RD(2 * q + 1) = (double)p.y;
+struct pen_r2r {
+    using Args = PassDesc;
+    static constexpr int NT = R * PF * Q / 8;
+    static __device__ __forceinline__ void run(const PassDesc& P, Blk blk) {
+        constexpr int H = PF * Q, T = Q / 8, TT = H / 8, N = 2 * H, HP = H + H / 8;
+        constexpr int S = COL ? R : 1;   // LDS stride between consecutive elements of a pencil
+        __shared__ C2<double> lds[COL ? (H + 1) * R : HP * R];
+        constexpr int DCT = KIND >= RODFT10 ? KIND - 4 : KIND;
+        // DCT-I / DST-I (node-centred symmetric directions): the even / odd extension of length 2H
+        // through the packed real FFT; n = H + 1 (DCT-I) or H - 1 (DST-I) points in and out
+        constexpr bool E00 = KIND == REDFT00 || KIND == RODFT00;
+        constexpr int NT = KIND == REDFT00 ? H + 1 : KIND == RODFT00 ? H - 1 : N;   // transform outputs
+        constexpr bool ALT = KIND == RODFT10;                       // input (-1)^n x, output reversed
+        constexpr bool REV = KIND == RODFT01 || KIND == RODFT11;    // input reversed, output (-1)^k
+        int rl, tt, ja, jb;
+        bool live;
+        if (COL) {
+            rl = threadIdx.x % R;
+            tt = threadIdx.x / R;
+            const int ntile = (P.na + R - 1) / R;
+            const int tile = blk.x % ntile;
+            jb = blk.x / ntile;
+            ja = tile * R + rl;
+            live = ja < P.na;
+            if (!live) ja = P.na - 1;   // dead pencils read the last one and store nothing
+        } else {
+            rl = threadIdx.x / TT;
+            tt = threadIdx.x % TT;
+            const int g = blk.x * R + rl;
+            live = g < P.n1 * P.n2;
+            const int gg = live ? g : 0;   // dead rows read pencil 0 and store nothing
+            ja = gg % P.na;
+            jb = gg / P.na;
         }
-    } else {
-        for (int k = tt; k < pin; k += TT) {
-            const int ks = k - P.gshift;
-            RD(k) = (ks >= 0 && ks < P.src_nd) ? (double)src[(long long)ks * sd] : 0.0;
-        }
-    }
-    __syncthreads();
-    const int si = P.si;
-    auto xin = [&](int q) -> double {
-        const double v = RD(si + (REV ? N - 1 - q : q));
-        return (ALT && (q & 1)) ? -v : v;
-    };
+        const int k1 = tt / T, j = tt % T;   // team and thread in the team
+        const TI* src = reinterpret_cast<const TI*>(P.src) + (long long)ja * P.src_s[P.fa] + (long long)jb * P.src_s[P.fb];
+        TO* dst = reinterpret_cast<TO*>(P.dst) + (long long)ja * P.dst_s[P.fa] + (long long)jb * P.dst_s[P.fb];
+        const long long sd = P.src_s[P.d], dd = P.dst_s[P.d];
+        C2<double>* row = COL ? lds + rl : lds + rl * HP;                                    // complex view
+        double* rdb = COL ? reinterpret_cast<double*>(lds) + rl : reinterpret_cast<double*>(row);   // real view
+        auto RD = [&](int k) -> double& { return rdb[k * S]; };
+        auto RW = [&](int k) -> C2<double>& { return row[k * S]; };
+        const C2<double>* A = reinterpret_cast<const C2<double>*>(P.aux);
+        const C2<double>* B = A + (H + 1);
+        const C2<double>* tw = reinterpret_cast<const C2<double>*>(P.tw);   // Q-point table
+        const C2<double>* twh = B + (H + 1);                                 // H-point table (PF > 1)
+        const int pin = P.pen_in;   // pencil buffer length (the same after the transform), at most N + 2
 
-    // 2. pre-map to the complex sequence of length H: element m
-    auto load = [&](int m) -> C2<double> {
-        if (KIND == REDFT00) {   // e[q] = x[q] (q <= H), x[2H - q] (q > H)
-            auto e = [&](int q) { return xin(q <= H ? q : N - q); };
-            return {e(2 * m), e(2 * m + 1)};
-        } else if (KIND == RODFT00) {   // o[0] = o[H] = 0, o[q] = x[q-1] (q < H), -x[2H-q-1] (q > H)
-            auto o = [&](int q) { return (q == 0 || q == H) ? 0.0 : q < H ? xin(q - 1) : -xin(N - q - 1); };
-            return {o(2 * m), o(2 * m + 1)};
+        // 1. gather the pencil into LDS (field shift, zero padding)
+        if (!COL && P.vec) {
+    #pragma unroll
+            for (int t = 0; t < 8; ++t) {
+                const int q = tt + t * TT;
+                const vec2_t<TI> p = reinterpret_cast<const vec2_t<TI>*>(src)[q];
+                RD(2 * q) = (double)p.x;
+                RD(2 * q + 1) = (double)p.y;
+            }
+        } else {
+            for (int k = tt; k < pin; k += TT) {
+                const int ks = k - P.gshift;
+                RD(k) = (ks >= 0 && ks < P.src_nd) ? (double)src[(long long)ks * sd] : 0.0;
+            }
+        }
+        __syncthreads();
+        const int si = P.si;
+        auto xin = [&](int q) -> double {
+            const double v = RD(si + (REV ? N - 1 - q : q));
+            return (ALT && (q & 1)) ? -v : v;
+        };
+
+        // 2. pre-map to the complex sequence of length H: element m
+        auto load = [&](int m) -> C2<double> {
+            if (KIND == REDFT00) {   // e[q] = x[q] (q <= H), x[2H - q] (q > H)
+                auto e = [&](int q) { return xin(q <= H ? q : N - q); };
+                return {e(2 * m), e(2 * m + 1)};
+            } else if (KIND == RODFT00) {   // o[0] = o[H] = 0, o[q] = x[q-1] (q < H), -x[2H-q-1] (q > H)
+                auto o = [&](int q) { return (q == 0 || q == H) ? 0.0 : q < H ? xin(q - 1) : -xin(N - q - 1); };
+                return {o(2 * m), o(2 * m + 1)};
+            } else if (DCT == REDFT11) {
+                return cmul(C2<double>{xin(2 * m), xin(N - 1 - 2 * m)}, A[m]);
+            } else if (DCT == REDFT10) {
+                auto u = [&](int q) { return q < H ? xin(2 * q) : xin(2 * (N - 1 - q) + 1); };
+                return {u(2 * m), u(2 * m + 1)};
+            } else {   // REDFT01
+                auto U = [&](int k) {
+                    const double a = k < N ? xin(k) : 0.0, b = (N - k) < N ? xin(N - k) : 0.0;
+                    return cmul(A[k], C2<double>{a, -b});
+                };
+                const C2<double> uk = U(m), uc = cconj(U(H - m));
+                const C2<double> e = cadd(uk, uc), f = csub(uk, uc);
+                const C2<double> t = cmul(B[m], f);
+                return {e.x - t.y, e.y + t.x};   // e + i t
+            }
+        };
+
+        // 3. FFT of length H (forward for DCT-II / IV, inverse for DCT-III): v[i] = Z[k1 + PF (j + i T)]
+        C2<double> v[8];
+        fft_pq_team<double, PF, Q, DCT != REDFT01, COL ? R : 0>(v, row, k1, j, load, tw, twh, 1);
+        __syncthreads();   // every thread is done with the FFT's LDS before the buffer is reused
+
+        // 4. post-map into the LDS pencil
+        if (E00) {
+            // X[k] = (Z[k] + conj Z[H-k])/2 + e^{-i pi k/H} (Z[k] - conj Z[H-k])/(2i), the real DFT of the
+            // extension; DCT-I y[k] = Re X[k] (k <= H), DST-I y[k] = -Im X[k+1] (k < H - 1)
+    #pragma unroll
+            for (int i = 0; i < 8; ++i) RW(k1 + PF * (j + i * T)) = v[i];
+            __syncthreads();
+            double XH = 0.0;
+    #pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                const int k = tt + i * TT;
+                const C2<double> zk = RW(k), zc = cconj(RW(k == 0 ? 0 : H - k));
+                const C2<double> e = {0.5 * (zk.x + zc.x), 0.5 * (zk.y + zc.y)};
+                const C2<double> df = {0.5 * (zk.x - zc.x), 0.5 * (zk.y - zc.y)};
+                v[i] = cadd(e, cmul(A[k], C2<double>{df.y, -df.x}));
+            }
+            if (tt == 0) {
+                const C2<double> z0 = RW(0);
+                XH = z0.x - z0.y;
+            }
+            __syncthreads();
+    #pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                const int k = tt + i * TT;
+                if (KIND == REDFT00) RD(k) = v[i].x;
+                else if (k >= 1) RD(k - 1) = -v[i].y;
+            }
+            if (KIND == REDFT00 && tt == 0) RD(H) = XH;
         } else if (DCT == REDFT11) {
-            return cmul(C2<double>{xin(2 * m), xin(N - 1 - 2 * m)}, A[m]);
+    #pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                const int k = k1 + PF * (j + i * T);
+                const C2<double> w = cmul(v[i], B[k]);
+                RD(2 * k) = 2.0 * w.x;
+                RD(N - 1 - 2 * k) = -2.0 * w.y;
+            }
         } else if (DCT == REDFT10) {
-            auto u = [&](int q) { return q < H ? xin(2 * q) : xin(2 * (N - 1 - q) + 1); };
-            return {u(2 * m), u(2 * m + 1)};
-        } else {   // REDFT01
-            auto U = [&](int k) {
-                const double a = k < N ? xin(k) : 0.0, b = (N - k) < N ? xin(N - k) : 0.0;
-                return cmul(A[k], C2<double>{a, -b});
-            };
-            const C2<double> uk = U(m), uc = cconj(U(H - m));
-            const C2<double> e = cadd(uk, uc), f = csub(uk, uc);
-            const C2<double> t = cmul(B[m], f);
-            return {e.x - t.y, e.y + t.x};   // e + i t
+    #pragma unroll
+            for (int i = 0; i < 8; ++i) RW(k1 + PF * (j + i * T)) = v[i];
+            __syncthreads();
+            C2<double> U[8], UH = {0, 0};
+    #pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                const int k = tt + i * TT;
+                const C2<double> zk = RW(k), zc = cconj(RW(k == 0 ? 0 : H - k));
+                const C2<double> e = {0.5 * (zk.x + zc.x), 0.5 * (zk.y + zc.y)};
+                const C2<double> df = {0.5 * (zk.x - zc.x), 0.5 * (zk.y - zc.y)};
+                U[i] = cadd(e, cmul(A[k], C2<double>{df.y, -df.x}));   // e + A[k] (zk - zc) / (2i)
+            }
+            if (tt == 0) {   // U[H] = Re Z[0] - Im Z[0]
+                const C2<double> z0 = RW(0);
+                UH = {z0.x - z0.y, 0.0};
+            }
+            __syncthreads();
+    #pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                const int k = tt + i * TT;
+                const C2<double> c = cmul(B[k], U[i]);
+                RD(k) = 2.0 * c.x;
+                if (k > 0) RD(N - k) = -2.0 * c.y;
+            }
+            if (tt == 0) RD(H) = 2.0 * cmul(B[H], UH).x;
+        } else {   // REDFT01: v[i] = z[m], u[2m] = Re, u[2m+1] = Im
+    #pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                const int m = k1 + PF * (j + i * T);
+                const int q0 = 2 * m, q1 = 2 * m + 1;
+                RD(q0 < H ? 2 * q0 : 2 * (N - 1 - q0) + 1) = v[i].x;
+                RD(q1 < H ? 2 * q1 : 2 * (N - 1 - q1) + 1) = v[i].y;
+            }
         }
-    };
+        __syncthreads();
 
-    // 3. FFT of length H (forward for DCT-II / IV, inverse for DCT-III): v[i] = Z[k1 + PF (j + i T)]
-    C2<double> v[8];
-    fft_pq_team<double, PF, Q, DCT != REDFT01, COL ? R : 0>(v, row, k1, j, load, tw, twh, 1);
-    __syncthreads();   // every thread is done with the FFT's LDS before the buffer is reused
-
-    // 4. post-map into the LDS pencil
-    if (E00) {
-        // X[k] = (Z[k] + conj Z[H-k])/2 + e^{-i pi k/H} (Z[k] - conj Z[H-k])/(2i), the real DFT of the
-        // extension; DCT-I y[k] = Re X[k] (k <= H), DST-I y[k] = -Im X[k+1] (k < H - 1)
-#pragma unroll
-        for (int i = 0; i < 8; ++i) RW(k1 + PF * (j + i * T)) = v[i];
-        __syncthreads();
-        double XH = 0.0;
-#pragma unroll
-        for (int i = 0; i < 8; ++i) {
-            const int k = tt + i * TT;
-            const C2<double> zk = RW(k), zc = cconj(RW(k == 0 ? 0 : H - k));
-            const C2<double> e = {0.5 * (zk.x + zc.x), 0.5 * (zk.y + zc.y)};
-            const C2<double> df = {0.5 * (zk.x - zc.x), 0.5 * (zk.y - zc.y)};
-            v[i] = cadd(e, cmul(A[k], C2<double>{df.y, -df.x}));
-        }
-        if (tt == 0) {
-            const C2<double> z0 = RW(0);
-            XH = z0.x - z0.y;
-        }
-        __syncthreads();
-#pragma unroll
-        for (int i = 0; i < 8; ++i) {
-            const int k = tt + i * TT;
-            if (KIND == REDFT00) RD(k) = v[i].x;
-            else if (k >= 1) RD(k - 1) = -v[i].y;
-        }
-        if (KIND == REDFT00 && tt == 0) RD(H) = XH;
-    } else if (DCT == REDFT11) {
-#pragma unroll
-        for (int i = 0; i < 8; ++i) {
-            const int k = k1 + PF * (j + i * T);
-            const C2<double> w = cmul(v[i], B[k]);
-            RD(2 * k) = 2.0 * w.x;
-            RD(N - 1 - 2 * k) = -2.0 * w.y;
-        }
-    } else if (DCT == REDFT10) {
-#pragma unroll
-        for (int i = 0; i < 8; ++i) RW(k1 + PF * (j + i * T)) = v[i];
-        __syncthreads();
-        C2<double> U[8], UH = {0, 0};
-#pragma unroll
-        for (int i = 0; i < 8; ++i) {
-            const int k = tt + i * TT;
-            const C2<double> zk = RW(k), zc = cconj(RW(k == 0 ? 0 : H - k));
-            const C2<double> e = {0.5 * (zk.x + zc.x), 0.5 * (zk.y + zc.y)};
-            const C2<double> df = {0.5 * (zk.x - zc.x), 0.5 * (zk.y - zc.y)};
-            U[i] = cadd(e, cmul(A[k], C2<double>{df.y, -df.x}));   // e + A[k] (zk - zc) / (2i)
-        }
-        if (tt == 0) {   // U[H] = Re Z[0] - Im Z[0]
-            const C2<double> z0 = RW(0);
-            UH = {z0.x - z0.y, 0.0};
-        }
-        __syncthreads();
-#pragma unroll
-        for (int i = 0; i < 8; ++i) {
-            const int k = tt + i * TT;
-            const C2<double> c = cmul(B[k], U[i]);
-            RD(k) = 2.0 * c.x;
-            if (k > 0) RD(N - k) = -2.0 * c.y;
-        }
-        if (tt == 0) RD(H) = 2.0 * cmul(B[H], UH).x;
-    } else {   // REDFT01: v[i] = z[m], u[2m] = Re, u[2m+1] = Im
-#pragma unroll
-        for (int i = 0; i < 8; ++i) {
-            const int m = k1 + PF * (j + i * T);
-            const int q0 = 2 * m, q1 = 2 * m + 1;
-            RD(q0 < H ? 2 * q0 : 2 * (N - 1 - q0) + 1) = v[i].x;
-            RD(q1 < H ? 2 * q1 : 2 * (N - 1 - q1) + 1) = v[i].y;
+        // 5. scatter: output pencil position p holds transform value p - so (DST reflections
+        // applied), the first / last point zeroed by the plan's post-correction
+        if (!live) return;
+        const int so = P.so, post = P.post;
+        auto val = [&](int p) -> double {
+            if (((post & POST_FIRST) && p == 0) || ((post & POST_LAST) && p == pin - 1)) return 0.0;
+            const int k = p - so;
+            if (k < 0 || k >= NT) return 0.0;
+            const double y = RD(ALT ? N - 1 - k : k);
+            return (REV && (k & 1)) ? -y : y;
+        };
+        if (!COL && P.vec) {
+    #pragma unroll
+            for (int t = 0; t < 8; ++t) {
+                const int q = tt + t * TT;
+                reinterpret_cast<vec2_t<TO>*>(dst)[q] = vec2_t<TO>{(TO)val(2 * q), (TO)val(2 * q + 1)};
+            }
+        } else {
+            for (int k = tt; k < P.dst_nd; k += TT) {
+                const int p = k + P.sshift;
+                if (p < pin) dst[(long long)k * dd] = (TO)val(p);
+            }
         }
     }
-    __syncthreads();
-
-    // 5. scatter: output pencil position p holds transform value p - so (DST reflections
-    // applied), the first / last point zeroed by the plan's post-correction
-    if (!live) return;
-    const int so = P.so, post = P.post;
-    auto val = [&](int p) -> double {
-        if (((post & POST_FIRST) && p == 0) || ((post & POST_LAST) && p == pin - 1)) return 0.0;
-        const int k = p - so;
-        if (k < 0 || k >= NT) return 0.0;
-        const double y = RD(ALT ? N - 1 - k : k);
-        return (REV && (k & 1)) ? -y : y;
-    };
-    if (!COL && P.vec) {
-#pragma unroll
-        for (int t = 0; t < 8; ++t) {
-            const int q = tt + t * TT;
-            reinterpret_cast<vec2_t<TO>*>(dst)[q] = vec2_t<TO>{(TO)val(2 * q), (TO)val(2 * q + 1)};
-        }
-    } else {
-        for (int k = tt; k < P.dst_nd; k += TT) {
-            const int p = k + P.sshift;
-            if (p < pin) dst[(long long)k * dd] = (TO)val(p);
-        }
-    }
-}
+};
 
 // H = PF Q: PF = 1 (a power of two) or 3, 5, 7 (PF teams of Q/8 threads, fft_pq_team);
 // H/8 threads per row either way
 template <int PF, int Q, int ROWS, bool FWD, typename TI, typename TO>
-__global__ void __launch_bounds__(ROWS * PF * Q / 8) row_rc(PassDesc P) {
-    constexpr int H = PF * Q, T = Q / 8, TT = H / 8, HP = H + H / 8;
-    __shared__ C2<double> lds[HP * ROWS];
-    const int rl = threadIdx.x / TT, tt = threadIdx.x % TT;
-    const int k1 = tt / T, j = tt % T;   // team and thread in the team
-    const int g = blockIdx.x * ROWS + rl;
-    const bool live = g < P.n1 * P.n2;
-    const int gg = live ? g : 0;   // dead rows read pencil 0 and store nothing
-    const int ja = gg % P.na, jb = gg / P.na;
-    const TI* src = reinterpret_cast<const TI*>(P.src) + (long long)ja * P.src_s[P.fa] * P.src_nf + (long long)jb * P.src_s[P.fb] * P.src_nf;
-    TO* dst = reinterpret_cast<TO*>(P.dst) + (long long)ja * P.dst_s[P.fa] * P.dst_nf + (long long)jb * P.dst_s[P.fb] * P.dst_nf;
-    C2<double>* row = lds + rl * HP;
-    double* rd = reinterpret_cast<double*>(row);
-    const C2<double>* tw = reinterpret_cast<const C2<double>*>(P.tw);    // Q-point table
-    const C2<double>* twn = reinterpret_cast<const C2<double>*>(P.aux);  // n-point table
-    // 1. gather: the pencil's doubles (reals for r2c, interleaved complex for c2r)
-    const int nfi = P.nf_in, len = P.pen_in * nfi;
-    for (int q = tt; q < len; q += TT) {
-        const int k = q / nfi, ks = k - P.gshift;
-        rd[q] = (ks >= 0 && ks < P.src_nd) ? (double)src[(long long)ks * nfi + (q - k * nfi)] : 0.0;
-    }
-    __syncthreads();
-    // element m of the packed complex sequence: z[m] = x[2m] + i x[2m+1] (r2c), or built from
-    // X[m] and X[H-m] (c2r)
-    auto load = [&](int m) -> C2<double> {
-        if (FWD) return {rd[2 * m], rd[2 * m + 1]};
-        C2<double> a = {rd[2 * m], rd[2 * m + 1]}, b = {rd[2 * (H - m)], rd[2 * (H - m) + 1]};
-        if (m == 0) {   // DC and Nyquist are real
-            a.y = 0.0;
-            b.y = 0.0;
-        }
-        const C2<double> bc = cconj(b);
-        const C2<double> e = cadd(a, bc), f = csub(a, bc);
-        const C2<double> t = cmul(cconj(twn[m]), f);   // e^{+2 pi i m/n} (a - conj b)
-        return {e.x - t.y, e.y + t.x};
-    };
-    C2<double> v[8];
-    fft_pq_team<double, PF, Q, FWD, 0>(v, row, k1, j, load, tw, twn, 2);   // w_H^e = twn[2 e]
-    __syncthreads();
-    const int nfo = P.nf_out, plen = P.pen_out;
-    // v[i] = Z[k1 + PF (j + i T)]
-#pragma unroll
-    for (int i = 0; i < 8; ++i) row[k1 + PF * (j + i * T)] = v[i];
-    if (FWD) {
-        // X[k], k <= H, into the row as complex values
-        __syncthreads();
-        C2<double> XH = {0, 0};
-#pragma unroll
-        for (int i = 0; i < 8; ++i) {
-            const int k = tt + i * TT;
-            const C2<double> zk = row[k], zc = cconj(row[k == 0 ? 0 : H - k]);
-            const C2<double> e = {0.5 * (zk.x + zc.x), 0.5 * (zk.y + zc.y)};
-            const C2<double> df = {0.5 * (zk.x - zc.x), 0.5 * (zk.y - zc.y)};
-            v[i] = cadd(e, cmul(twn[k], C2<double>{df.y, -df.x}));
-        }
-        if (tt == 0) {
-            const C2<double> z0 = row[0];
-            XH = {z0.x - z0.y, 0.0};
+struct row_rc {
+    using Args = PassDesc;
+    static constexpr int NT = ROWS * PF * Q / 8;
+    static __device__ __forceinline__ void run(const PassDesc& P, Blk blk) {
+        constexpr int H = PF * Q, T = Q / 8, TT = H / 8, HP = H + H / 8;
+        __shared__ C2<double> lds[HP * ROWS];
+        const int rl = threadIdx.x / TT, tt = threadIdx.x % TT;
+        const int k1 = tt / T, j = tt % T;   // team and thread in the team
+        const int g = blk.x * ROWS + rl;
+        const bool live = g < P.n1 * P.n2;
+        const int gg = live ? g : 0;   // dead rows read pencil 0 and store nothing
+        const int ja = gg % P.na, jb = gg / P.na;
+        const TI* src = reinterpret_cast<const TI*>(P.src) + (long long)ja * P.src_s[P.fa] * P.src_nf + (long long)jb * P.src_s[P.fb] * P.src_nf;
+        TO* dst = reinterpret_cast<TO*>(P.dst) + (long long)ja * P.dst_s[P.fa] * P.dst_nf + (long long)jb * P.dst_s[P.fb] * P.dst_nf;
+        C2<double>* row = lds + rl * HP;
+        double* rd = reinterpret_cast<double*>(row);
+        const C2<double>* tw = reinterpret_cast<const C2<double>*>(P.tw);    // Q-point table
+        const C2<double>* twn = reinterpret_cast<const C2<double>*>(P.aux);  // n-point table
+        // 1. gather: the pencil's doubles (reals for r2c, interleaved complex for c2r)
+        const int nfi = P.nf_in, len = P.pen_in * nfi;
+        for (int q = tt; q < len; q += TT) {
+            const int k = q / nfi, ks = k - P.gshift;
+            rd[q] = (ks >= 0 && ks < P.src_nd) ? (double)src[(long long)ks * nfi + (q - k * nfi)] : 0.0;
         }
         __syncthreads();
-#pragma unroll
-        for (int i = 0; i < 8; ++i) row[tt + i * TT] = v[i];
-        if (tt == 0) row[H] = XH;
-        // positions past H + 1 complex values keep nothing (zero)
-        for (int q = H + 1 + tt; q < plen; q += TT) row[q] = {0.0, 0.0};
-        // periodic node-centred direction: the first value again at the end (n_out - 1)
-        if ((P.post & POST_PERIOD) && tt == 0) row[P.n_out_plan - 1] = v[0];
-    } else {
-        // x[2m] = Re z[m], x[2m+1] = Im z[m] (written above)
-        for (int q = 2 * H + tt; q < plen; q += TT) rd[q] = 0.0;
-        if ((P.post & POST_PERIOD) && tt == 0) rd[2 * H] = v[0].x;   // x[n] = x[0]
+        // element m of the packed complex sequence: z[m] = x[2m] + i x[2m+1] (r2c), or built from
+        // X[m] and X[H-m] (c2r)
+        auto load = [&](int m) -> C2<double> {
+            if (FWD) return {rd[2 * m], rd[2 * m + 1]};
+            C2<double> a = {rd[2 * m], rd[2 * m + 1]}, b = {rd[2 * (H - m)], rd[2 * (H - m) + 1]};
+            if (m == 0) {   // DC and Nyquist are real
+                a.y = 0.0;
+                b.y = 0.0;
+            }
+            const C2<double> bc = cconj(b);
+            const C2<double> e = cadd(a, bc), f = csub(a, bc);
+            const C2<double> t = cmul(cconj(twn[m]), f);   // e^{+2 pi i m/n} (a - conj b)
+            return {e.x - t.y, e.y + t.x};
+        };
+        C2<double> v[8];
+        fft_pq_team<double, PF, Q, FWD, 0>(v, row, k1, j, load, tw, twn, 2);   // w_H^e = twn[2 e]
+        __syncthreads();
+        const int nfo = P.nf_out, plen = P.pen_out;
+        // v[i] = Z[k1 + PF (j + i T)]
+    #pragma unroll
+        for (int i = 0; i < 8; ++i) row[k1 + PF * (j + i * T)] = v[i];
+        if (FWD) {
+            // X[k], k <= H, into the row as complex values
+            __syncthreads();
+            C2<double> XH = {0, 0};
+    #pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                const int k = tt + i * TT;
+                const C2<double> zk = row[k], zc = cconj(row[k == 0 ? 0 : H - k]);
+                const C2<double> e = {0.5 * (zk.x + zc.x), 0.5 * (zk.y + zc.y)};
+                const C2<double> df = {0.5 * (zk.x - zc.x), 0.5 * (zk.y - zc.y)};
+                v[i] = cadd(e, cmul(twn[k], C2<double>{df.y, -df.x}));
+            }
+            if (tt == 0) {
+                const C2<double> z0 = row[0];
+                XH = {z0.x - z0.y, 0.0};
+            }
+            __syncthreads();
+    #pragma unroll
+            for (int i = 0; i < 8; ++i) row[tt + i * TT] = v[i];
+            if (tt == 0) row[H] = XH;
+            // positions past H + 1 complex values keep nothing (zero)
+            for (int q = H + 1 + tt; q < plen; q += TT) row[q] = {0.0, 0.0};
+            // periodic node-centred direction: the first value again at the end (n_out - 1)
+            if ((P.post & POST_PERIOD) && tt == 0) row[P.n_out_plan - 1] = v[0];
+        } else {
+            // x[2m] = Re z[m], x[2m+1] = Im z[m] (written above)
+            for (int q = 2 * H + tt; q < plen; q += TT) rd[q] = 0.0;
+            if ((P.post & POST_PERIOD) && tt == 0) rd[2 * H] = v[0].x;   // x[n] = x[0]
+        }
+        __syncthreads();
+        if (!live) return;
+        // 2. scatter (truncation), element k of the destination = pencil position k + sshift
+        const int dn = P.dst_nd;
+        for (int q = tt; q < dn * nfo; q += TT) {
+            const int k = q / nfo, p = k + P.sshift;
+            if (p < plen) dst[(long long)k * nfo + (q - k * nfo)] = (TO)rd[p * nfo + (q - k * nfo)];
+        }
     }
-    __syncthreads();
-    if (!live) return;
-    // 2. scatter (truncation), element k of the destination = pencil position k + sshift
-    const int dn = P.dst_nd;
-    for (int q = tt; q < dn * nfo; q += TT) {
-        const int k = q / nfo, p = k + P.sshift;
-        if (p < plen) dst[(long long)k * nfo + (q - k * nfo)] = (TO)rd[p * nfo + (q - k * nfo)];
-    }
-}
+};
 
 // M = PF Q: PF = 1 (a power of two) or 3, 5 (8 PF values per thread, column_fft_pq)
 template <int PF, int Q, int TX, bool FWD, typename TI, typename TO>
-__global__ void __launch_bounds__(TX * Q / 8) col_real(PassDesc P) {
-    constexpr int M = PF * Q, T = Q / 8, NH = M / 2, NV = 8 * PF;
-    __shared__ C2<double> lds[(M + 2) * TX];
-    const int c = threadIdx.x % TX, j = threadIdx.x / TX;
-    const int npair = (P.na + 1) / 2;
-    const int ntile = (npair + TX - 1) / TX;
-    const int tile = blockIdx.x % ntile, jb = blockIdx.x / ntile;
-    const int pr = tile * TX + c;
-    const bool la = 2 * pr < P.na, lb = 2 * pr + 1 < P.na;
-    const int xa = la ? 2 * pr : 2 * (npair - 1);   // dead pairs read the last pair and store nothing
-    const int d = P.d;
-    const long long sr = FWD ? P.src_s[d] : P.dst_s[d];       // row stride of the real array
-    const long long sc = FWD ? P.dst_s[d] : P.src_s[d];       // row stride of the complex array
-    const TI* rsrc = reinterpret_cast<const TI*>(P.src) + (long long)jb * P.src_s[P.fb] + xa;          // FWD: real source
-    TO* rdst = reinterpret_cast<TO*>(P.dst) + (long long)jb * P.dst_s[P.fb] + xa;                    // !FWD: real destination
-    const TI* csrc = reinterpret_cast<const TI*>(P.src) + 2 * ((long long)jb * P.src_s[P.fb] + xa);   // !FWD: complex source
-    TO* cdst = reinterpret_cast<TO*>(P.dst) + 2 * ((long long)jb * P.dst_s[P.fb] + xa);             // FWD: complex destination
-    const bool hasb = lb;   // a real partner column exists
-    const C2<double>* tw = reinterpret_cast<const C2<double>*>(P.tw);    // Q-point table
-    const C2<double>* twm = reinterpret_cast<const C2<double>*>(P.aux);  // M-point table (PF > 1)
-    C2<double> v[NV];
-    if (FWD) {
-#pragma unroll
-        for (int i = 0; i < NV; ++i) {
-            const int row = j + i * T;
-            const TI* p = rsrc + (long long)row * sr;
-            if (row >= P.src_nd) {   // zero padding of an unbounded direction
-                v[i] = {0.0, 0.0};
-            } else if (P.vec) {
-                const vec2_t<TI> q = *reinterpret_cast<const vec2_t<TI>*>(p);
-                v[i] = {(double)q.x, (double)q.y};
-            } else {
-                v[i] = {(double)p[0], hasb ? (double)p[1] : 0.0};
+struct col_real {
+    using Args = PassDesc;
+    static constexpr int NT = TX * Q / 8;
+    static __device__ __forceinline__ void run(const PassDesc& P, Blk blk) {
+        constexpr int M = PF * Q, T = Q / 8, NH = M / 2, NV = 8 * PF;
+        __shared__ C2<double> lds[(M + 2) * TX];
+        const int c = threadIdx.x % TX, j = threadIdx.x / TX;
+        const int npair = (P.na + 1) / 2;
+        const int ntile = (npair + TX - 1) / TX;
+        const int tile = blk.x % ntile, jb = blk.x / ntile;
+        const int pr = tile * TX + c;
+        const bool la = 2 * pr < P.na, lb = 2 * pr + 1 < P.na;
+        const int xa = la ? 2 * pr : 2 * (npair - 1);   // dead pairs read the last pair and store nothing
+        const int d = P.d;
+        const long long sr = FWD ? P.src_s[d] : P.dst_s[d];       // row stride of the real array
+        const long long sc = FWD ? P.dst_s[d] : P.src_s[d];       // row stride of the complex array
+        const TI* rsrc = reinterpret_cast<const TI*>(P.src) + (long long)jb * P.src_s[P.fb] + xa;          // FWD: real source
+        TO* rdst = reinterpret_cast<TO*>(P.dst) + (long long)jb * P.dst_s[P.fb] + xa;                    // !FWD: real destination
+        const TI* csrc = reinterpret_cast<const TI*>(P.src) + 2 * ((long long)jb * P.src_s[P.fb] + xa);   // !FWD: complex source
+        TO* cdst = reinterpret_cast<TO*>(P.dst) + 2 * ((long long)jb * P.dst_s[P.fb] + xa);             // FWD: complex destination
+        const bool hasb = lb;   // a real partner column exists
+        const C2<double>* tw = reinterpret_cast<const C2<double>*>(P.tw);    // Q-point table
+        const C2<double>* twm = reinterpret_cast<const C2<double>*>(P.aux);  // M-point table (PF > 1)
+        C2<double> v[NV];
+        if (FWD) {
+    #pragma unroll
+            for (int i = 0; i < NV; ++i) {
+                const int row = j + i * T;
+                const TI* p = rsrc + (long long)row * sr;
+                if (row >= P.src_nd) {   // zero padding of an unbounded direction
+                    v[i] = {0.0, 0.0};
+                } else if (P.vec) {
+                    const vec2_t<TI> q = *reinterpret_cast<const vec2_t<TI>*>(p);
+                    v[i] = {(double)q.x, (double)q.y};
+                } else {
+                    v[i] = {(double)p[0], hasb ? (double)p[1] : 0.0};
+                }
             }
-        }
-        column_fft_pq<double, PF, Q, true, TX>(v, lds + c, j, tw, twm, 1);
-        __syncthreads();   // the FFT's last LDS reads are done
-#pragma unroll
-        for (int i = 0; i < NV; ++i) lds[(j + i * T) * TX + c] = v[i];
-        __syncthreads();
-        if (!la) return;
-#pragma unroll
-        for (int i = 0; i <= NV / 2; ++i) {
-            const int k = j + i * T;
-            if (i == NV / 2 && j != 0) break;   // k = n/2 once
-            const C2<double> zk = lds[k * TX + c], zc = cconj(lds[(k == 0 ? 0 : M - k) * TX + c]);
-            const C2<double> A = {0.5 * (zk.x + zc.x), 0.5 * (zk.y + zc.y)};
-            const C2<double> df = {0.5 * (zk.x - zc.x), 0.5 * (zk.y - zc.y)};
-            TO* o = cdst + 2 * (long long)k * sc;
-            *reinterpret_cast<vec2_t<TO>*>(o) = vec2_t<TO>{(TO)A.x, (TO)A.y};
-            if (hasb) *reinterpret_cast<vec2_t<TO>*>(o + 2) = vec2_t<TO>{(TO)df.y, (TO)-df.x};   // (zk - zc) / (2i)
-            if ((P.post & POST_PERIOD) && k == 0) {   // node-centred periodic: X[0] again at n/2 + 1
-                TO* o2 = cdst + 2 * (long long)(NH + 1) * sc;
-                *reinterpret_cast<vec2_t<TO>*>(o2) = vec2_t<TO>{(TO)A.x, (TO)A.y};
-                if (hasb) *reinterpret_cast<vec2_t<TO>*>(o2 + 2) = vec2_t<TO>{(TO)df.y, (TO)-df.x};
+            column_fft_pq<double, PF, Q, true, TX>(v, lds + c, j, tw, twm, 1);
+            __syncthreads();   // the FFT's last LDS reads are done
+    #pragma unroll
+            for (int i = 0; i < NV; ++i) lds[(j + i * T) * TX + c] = v[i];
+            __syncthreads();
+            if (!la) return;
+    #pragma unroll
+            for (int i = 0; i <= NV / 2; ++i) {
+                const int k = j + i * T;
+                if (i == NV / 2 && j != 0) break;   // k = n/2 once
+                const C2<double> zk = lds[k * TX + c], zc = cconj(lds[(k == 0 ? 0 : M - k) * TX + c]);
+                const C2<double> A = {0.5 * (zk.x + zc.x), 0.5 * (zk.y + zc.y)};
+                const C2<double> df = {0.5 * (zk.x - zc.x), 0.5 * (zk.y - zc.y)};
+                TO* o = cdst + 2 * (long long)k * sc;
+                *reinterpret_cast<vec2_t<TO>*>(o) = vec2_t<TO>{(TO)A.x, (TO)A.y};
+                if (hasb) *reinterpret_cast<vec2_t<TO>*>(o + 2) = vec2_t<TO>{(TO)df.y, (TO)-df.x};   // (zk - zc) / (2i)
+                if ((P.post & POST_PERIOD) && k == 0) {   // node-centred periodic: X[0] again at n/2 + 1
+                    TO* o2 = cdst + 2 * (long long)(NH + 1) * sc;
+                    *reinterpret_cast<vec2_t<TO>*>(o2) = vec2_t<TO>{(TO)A.x, (TO)A.y};
+                    if (hasb) *reinterpret_cast<vec2_t<TO>*>(o2 + 2) = vec2_t<TO>{(TO)df.y, (TO)-df.x};
+                }
             }
-        }
-    } else {
-        // rows 0..n/2 of the two spectra into LDS: A at row k, B at row n/2 + 1 + k
-#pragma unroll
-        for (int i = 0; i <= NV / 2; ++i) {
-            const int k = j + i * T;
-            if (i == NV / 2 && j != 0) break;
-            const TI* p = csrc + 2 * (long long)k * sc;
-            const vec2_t<TI> qa = *reinterpret_cast<const vec2_t<TI>*>(p);
-            const vec2_t<TI> qb = hasb ? *reinterpret_cast<const vec2_t<TI>*>(p + 2) : vec2_t<TI>{0, 0};
-            C2<double> A = {(double)qa.x, (double)qa.y}, B = {(double)qb.x, (double)qb.y};
-            if (k == 0 || k == NH) {   // DC and Nyquist are real
-                A.y = 0.0;
-                B.y = 0.0;
+        } else {
+            // rows 0..n/2 of the two spectra into LDS: A at row k, B at row n/2 + 1 + k
+    #pragma unroll
+            for (int i = 0; i <= NV / 2; ++i) {
+                const int k = j + i * T;
+                if (i == NV / 2 && j != 0) break;
+                const TI* p = csrc + 2 * (long long)k * sc;
+                const vec2_t<TI> qa = *reinterpret_cast<const vec2_t<TI>*>(p);
+                const vec2_t<TI> qb = hasb ? *reinterpret_cast<const vec2_t<TI>*>(p + 2) : vec2_t<TI>{0, 0};
+                C2<double> A = {(double)qa.x, (double)qa.y}, B = {(double)qb.x, (double)qb.y};
+                if (k == 0 || k == NH) {   // DC and Nyquist are real
+                    A.y = 0.0;
+                    B.y = 0.0;
+                }
+                lds[k * TX + c] = A;
+                lds[(NH + 1 + k) * TX + c] = B;
             }
-            lds[k * TX + c] = A;
-            lds[(NH + 1 + k) * TX + c] = B;
-        }
-        __syncthreads();
-#pragma unroll
-        for (int i = 0; i < NV; ++i) {
-            const int k = j + i * T;
-            const int m = k <= NH ? k : M - k;
-            const C2<double> A = lds[m * TX + c], B = lds[(NH + 1 + m) * TX + c];
-            v[i] = k <= NH ? C2<double>{A.x - B.y, A.y + B.x} : C2<double>{A.x + B.y, B.x - A.y};
-        }
-        column_fft_pq<double, PF, Q, false, TX>(v, lds + c, j, tw, twm, 1);   // syncs before its first LDS write
-        if (!la) return;
-#pragma unroll
-        for (int i = 0; i < NV; ++i) {
-            const int row = j + i * T;
-            if (row >= P.dst_nd) continue;   // truncation back to the field's extent
-            TO* p = rdst + (long long)row * sr;
-            if (P.vec) {
-                *reinterpret_cast<vec2_t<TO>*>(p) = vec2_t<TO>{(TO)v[i].x, (TO)v[i].y};
-            } else {
-                p[0] = (TO)v[i].x;
-                if (hasb) p[1] = (TO)v[i].y;
+            __syncthreads();
+    #pragma unroll
+            for (int i = 0; i < NV; ++i) {
+                const int k = j + i * T;
+                const int m = k <= NH ? k : M - k;
+                const C2<double> A = lds[m * TX + c], B = lds[(NH + 1 + m) * TX + c];
+                v[i] = k <= NH ? C2<double>{A.x - B.y, A.y + B.x} : C2<double>{A.x + B.y, B.x - A.y};
             }
-        }
-        if ((P.post & POST_PERIOD) && j == 0 && M < P.dst_nd) {   // node-centred periodic: x[n] = x[0]
-            TO* p = rdst + (long long)M * sr;
-            p[0] = (TO)v[0].x;
-            if (hasb) p[1] = (TO)v[0].y;
+            column_fft_pq<double, PF, Q, false, TX>(v, lds + c, j, tw, twm, 1);   // syncs before its first LDS write
+            if (!la) return;
+    #pragma unroll
+            for (int i = 0; i < NV; ++i) {
+                const int row = j + i * T;
+                if (row >= P.dst_nd) continue;   // truncation back to the field's extent
+                TO* p = rdst + (long long)row * sr;
+                if (P.vec) {
+                    *reinterpret_cast<vec2_t<TO>*>(p) = vec2_t<TO>{(TO)v[i].x, (TO)v[i].y};
+                } else {
+                    p[0] = (TO)v[i].x;
+                    if (hasb) p[1] = (TO)v[i].y;
+                }
+            }
+            if ((P.post & POST_PERIOD) && j == 0 && M < P.dst_nd) {   // node-centred periodic: x[n] = x[0]
+                TO* p = rdst + (long long)M * sr;
+                p[0] = (TO)v[0].x;
+                if (hasb) p[1] = (TO)v[0].y;
+            }
         }
     }
-}
+};
 
 // The general path's last direction when it is an r2c one (symmetric x and y, unbounded or
 // periodic z): forward r2c, normfact G, inverse c2r in one launch, in place. Two neighbouring
@@ -479,123 +492,131 @@ __global__ void __launch_bounds__(TX * Q / 8) col_real(PassDesc P) {
 // recombine to Z'[k] = A' + i B' = (Ga + Gb)/2 Z[k] + (Ga - Gb)/2 conj Z[M-k] (G real and even
 // in kz), and IFFT(Z') = a' + i b'. Rows from nin on are zero (padding) and are not stored.
 template <int M, int TX, typename S>
-__global__ void __launch_bounds__(TX * M / 8) col_rz(RzArgs A) {
-    constexpr int T = M / 8;
-    __shared__ C2<double> lds[M * TX];
-    const int c = threadIdx.x % TX, j = threadIdx.x / TX;
-    const int npair = (A.nx + 1) / 2;
-    const int ntile = (npair + TX - 1) / TX;
-    const int tile = blockIdx.x % ntile, y = blockIdx.x / ntile;
-    const int pr = tile * TX + c;
-    const bool la = 2 * pr < A.nx, lb = 2 * pr + 1 < A.nx;
-    const int xa = la ? 2 * pr : 2 * (npair - 1);   // dead pairs read the last pair and store nothing
-    const bool hb = 2 * pr + 1 < A.nx || (!la && xa + 1 < A.nx);   // partner column present (for the loads)
-    S* base = reinterpret_cast<S*>(A.d) + (long long)blockIdx.y * A.comp + (long long)y * A.nx + xa;
-    const C2<double>* tw = reinterpret_cast<const C2<double>*>(A.tw);
-    C2<double> v[8];
-#pragma unroll
-    for (int i = 0; i < 8; ++i) {
-        const int row = j + i * T;
-        const S* p = base + (long long)row * A.s_col;
-        v[i] = row < A.nin ? C2<double>{(double)p[0], hb ? (double)p[1] : 0.0} : C2<double>{0.0, 0.0};
+struct col_rz {
+    using Args = RzArgs;
+    static constexpr int NT = TX * M / 8;
+    static __device__ __forceinline__ void run(const RzArgs& A, Blk blk) {
+        constexpr int T = M / 8;
+        __shared__ C2<double> lds[M * TX];
+        const int c = threadIdx.x % TX, j = threadIdx.x / TX;
+        const int npair = (A.nx + 1) / 2;
+        const int ntile = (npair + TX - 1) / TX;
+        const int tile = blk.x % ntile, y = blk.x / ntile;
+        const int pr = tile * TX + c;
+        const bool la = 2 * pr < A.nx, lb = 2 * pr + 1 < A.nx;
+        const int xa = la ? 2 * pr : 2 * (npair - 1);   // dead pairs read the last pair and store nothing
+        const bool hb = 2 * pr + 1 < A.nx || (!la && xa + 1 < A.nx);   // partner column present (for the loads)
+        S* base = reinterpret_cast<S*>(A.d) + (long long)blk.y * A.comp + (long long)y * A.nx + xa;
+        const C2<double>* tw = reinterpret_cast<const C2<double>*>(A.tw);
+        C2<double> v[8];
+    #pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const int row = j + i * T;
+            const S* p = base + (long long)row * A.s_col;
+            v[i] = row < A.nin ? C2<double>{(double)p[0], hb ? (double)p[1] : 0.0} : C2<double>{0.0, 0.0};
+        }
+        column_fft<double, M, true, TX>(v, lds + c, j, tw);
+        __syncthreads();   // the FFT's last LDS reads are done
+    #pragma unroll
+        for (int i = 0; i < 8; ++i) lds[(j + i * T) * TX + c] = v[i];
+        __syncthreads();
+        const S* g = reinterpret_cast<const S*>(A.g) + 2 * ((long long)y * A.nx + xa);
+        const long long gs = 2LL * A.nx * A.ny;   // doubles between kz rows of G
+    #pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const int k = j + i * T;
+            const C2<double> zc = cconj(lds[(k == 0 ? 0 : M - k) * TX + c]);
+            const int kz = k <= M / 2 ? k : M - k;
+            const double ga = (double)g[kz * gs], gb = hb ? (double)g[kz * gs + 2] : ga;
+            const double p = 0.5 * (ga + gb) * A.normfact, q = 0.5 * (ga - gb) * A.normfact;
+            v[i] = {p * v[i].x + q * zc.x, p * v[i].y + q * zc.y};
+        }
+        column_fft<double, M, false, TX>(v, lds + c, j, tw);   // syncs before its first LDS write
+        if (!la) return;
+    #pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const int row = j + i * T;
+            if (row >= A.nin) continue;
+            S* p = base + (long long)row * A.s_col;
+            p[0] = (S)v[i].x;
+            if (lb) p[1] = (S)v[i].y;
+        }
     }
-    column_fft<double, M, true, TX>(v, lds + c, j, tw);
-    __syncthreads();   // the FFT's last LDS reads are done
-#pragma unroll
-    for (int i = 0; i < 8; ++i) lds[(j + i * T) * TX + c] = v[i];
-    __syncthreads();
-    const S* g = reinterpret_cast<const S*>(A.g) + 2 * ((long long)y * A.nx + xa);
-    const long long gs = 2LL * A.nx * A.ny;   // doubles between kz rows of G
-#pragma unroll
-    for (int i = 0; i < 8; ++i) {
-        const int k = j + i * T;
-        const C2<double> zc = cconj(lds[(k == 0 ? 0 : M - k) * TX + c]);
-        const int kz = k <= M / 2 ? k : M - k;
-        const double ga = (double)g[kz * gs], gb = hb ? (double)g[kz * gs + 2] : ga;
-        const double p = 0.5 * (ga + gb) * A.normfact, q = 0.5 * (ga - gb) * A.normfact;
-        v[i] = {p * v[i].x + q * zc.x, p * v[i].y + q * zc.y};
-    }
-    column_fft<double, M, false, TX>(v, lds + c, j, tw);   // syncs before its first LDS write
-    if (!la) return;
-#pragma unroll
-    for (int i = 0; i < 8; ++i) {
-        const int row = j + i * T;
-        if (row >= A.nin) continue;
-        S* p = base + (long long)row * A.s_col;
-        p[0] = (S)v[i].x;
-        if (lb) p[1] = (S)v[i].y;
-    }
-}
+};
 
 // The general path's last direction when it is an r2c along x (every other direction real-
 // to-real): r2c of each row (length n = 2H, rows zero-padded from nin), X[k] normfact G[k]
 // (G real, one row of the spectral layout per field row), c2r, the first nin values stored,
 // all in LDS, in place. PF teams of Q/8 threads as row_rc.
 template <int PF, int Q, int ROWS, typename S>
-__global__ void __launch_bounds__(ROWS * PF * Q / 8) row_rz(RzArgs A) {
-    constexpr int H = PF * Q, T = Q / 8, TT = H / 8, HP = H + H / 8;
-    __shared__ C2<double> lds[HP * ROWS];
-    const int rl = threadIdx.x / TT, tt = threadIdx.x % TT;
-    const int k1 = tt / T, j = tt % T;
-    const int g = blockIdx.x * ROWS + rl;
-    const bool live = g < A.rows;
-    const int r = live ? g : 0;   // dead rows read row 0 and store nothing
-    S* d = reinterpret_cast<S*>(A.d) + (long long)blockIdx.y * A.comp + (long long)r * A.nin;
-    const S* gr = reinterpret_cast<const S*>(A.g) + 2LL * r * (H + 1);   // real parts at even offsets
-    C2<double>* row = lds + rl * HP;
-    double* rd = reinterpret_cast<double*>(row);
-    const C2<double>* tw = reinterpret_cast<const C2<double>*>(A.tw);    // Q-point table
-    const C2<double>* twn = reinterpret_cast<const C2<double>*>(A.twn);  // n-point table
-    for (int q = tt; q < 2 * H; q += TT) rd[q] = q < A.nin ? (double)d[q] : 0.0;
-    __syncthreads();
-    C2<double> v[8];
-    auto loadf = [&](int m) -> C2<double> { return {rd[2 * m], rd[2 * m + 1]}; };
-    fft_pq_team<double, PF, Q, true, 0>(v, row, k1, j, loadf, tw, twn, 2);
-    __syncthreads();
-#pragma unroll
-    for (int i = 0; i < 8; ++i) row[k1 + PF * (j + i * T)] = v[i];
-    __syncthreads();
-    // X[k] = (Z[k] + conj Z[H-k])/2 + e^{-2 pi i k/n} (Z[k] - conj Z[H-k])/(2i), times normfact G[k]
-    C2<double> XH = {0, 0};
-#pragma unroll
-    for (int i = 0; i < 8; ++i) {
-        const int k = tt + i * TT;
-        const C2<double> zk = row[k], zc = cconj(row[k == 0 ? 0 : H - k]);
-        const C2<double> e = {0.5 * (zk.x + zc.x), 0.5 * (zk.y + zc.y)};
-        const C2<double> df = {0.5 * (zk.x - zc.x), 0.5 * (zk.y - zc.y)};
-        const C2<double> X = cadd(e, cmul(twn[k], C2<double>{df.y, -df.x}));
-        const double gk = A.normfact * (double)gr[2 * k];
-        v[i] = {X.x * gk, X.y * gk};
-    }
-    if (tt == 0) {
-        const C2<double> z0 = row[0];
-        XH = {(z0.x - z0.y) * A.normfact * (double)gr[2 * H], 0.0};
-    }
-    __syncthreads();
-#pragma unroll
-    for (int i = 0; i < 8; ++i) row[tt + i * TT] = v[i];
-    if (tt == 0) row[H] = XH;
-    __syncthreads();
-    // c2r: Z[m] = (X[m] + conj X[H-m]) + i e^{2 pi i m/n} (X[m] - conj X[H-m]), z = IFFT_H(Z)
-    auto loadb = [&](int m) -> C2<double> {
-        C2<double> a = {rd[2 * m], rd[2 * m + 1]}, b = {rd[2 * (H - m)], rd[2 * (H - m) + 1]};
-        if (m == 0) {   // DC and Nyquist are real
-            a.y = 0.0;
-            b.y = 0.0;
+struct row_rz {
+    using Args = RzArgs;
+    static constexpr int NT = ROWS * PF * Q / 8;
+    static __device__ __forceinline__ void run(const RzArgs& A, Blk blk) {
+        constexpr int H = PF * Q, T = Q / 8, TT = H / 8, HP = H + H / 8;
+        __shared__ C2<double> lds[HP * ROWS];
+        const int rl = threadIdx.x / TT, tt = threadIdx.x % TT;
+        const int k1 = tt / T, j = tt % T;
+        const int g = blk.x * ROWS + rl;
+        const bool live = g < A.rows;
+        const int r = live ? g : 0;   // dead rows read row 0 and store nothing
+        S* d = reinterpret_cast<S*>(A.d) + (long long)blk.y * A.comp + (long long)r * A.nin;
+        const S* gr = reinterpret_cast<const S*>(A.g) + 2LL * r * (H + 1);   // real parts at even offsets
+        C2<double>* row = lds + rl * HP;
+        double* rd = reinterpret_cast<double*>(row);
+        const C2<double>* tw = reinterpret_cast<const C2<double>*>(A.tw);    // Q-point table
+        const C2<double>* twn = reinterpret_cast<const C2<double>*>(A.twn);  // n-point table
+        for (int q = tt; q < 2 * H; q += TT) rd[q] = q < A.nin ? (double)d[q] : 0.0;
+        __syncthreads();
+        C2<double> v[8];
+        auto loadf = [&](int m) -> C2<double> { return {rd[2 * m], rd[2 * m + 1]}; };
+        fft_pq_team<double, PF, Q, true, 0>(v, row, k1, j, loadf, tw, twn, 2);
+        __syncthreads();
+    #pragma unroll
+        for (int i = 0; i < 8; ++i) row[k1 + PF * (j + i * T)] = v[i];
+        __syncthreads();
+        // X[k] = (Z[k] + conj Z[H-k])/2 + e^{-2 pi i k/n} (Z[k] - conj Z[H-k])/(2i), times normfact G[k]
+        C2<double> XH = {0, 0};
+    #pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const int k = tt + i * TT;
+            const C2<double> zk = row[k], zc = cconj(row[k == 0 ? 0 : H - k]);
+            const C2<double> e = {0.5 * (zk.x + zc.x), 0.5 * (zk.y + zc.y)};
+            const C2<double> df = {0.5 * (zk.x - zc.x), 0.5 * (zk.y - zc.y)};
+            const C2<double> X = cadd(e, cmul(twn[k], C2<double>{df.y, -df.x}));
+            const double gk = A.normfact * (double)gr[2 * k];
+            v[i] = {X.x * gk, X.y * gk};
         }
-        const C2<double> bc = cconj(b);
-        const C2<double> e = cadd(a, bc), f = csub(a, bc);
-        const C2<double> t = cmul(cconj(twn[m]), f);
-        return {e.x - t.y, e.y + t.x};
-    };
-    fft_pq_team<double, PF, Q, false, 0>(v, row, k1, j, loadb, tw, twn, 2);
-    __syncthreads();
-#pragma unroll
-    for (int i = 0; i < 8; ++i) row[k1 + PF * (j + i * T)] = v[i];   // x[2m] = Re z[m], x[2m+1] = Im z[m]
-    __syncthreads();
-    if (!live) return;
-    for (int q = tt; q < A.nin; q += TT) d[q] = (S)rd[q];
-}
+        if (tt == 0) {
+            const C2<double> z0 = row[0];
+            XH = {(z0.x - z0.y) * A.normfact * (double)gr[2 * H], 0.0};
+        }
+        __syncthreads();
+    #pragma unroll
+        for (int i = 0; i < 8; ++i) row[tt + i * TT] = v[i];
+        if (tt == 0) row[H] = XH;
+        __syncthreads();
+        // c2r: Z[m] = (X[m] + conj X[H-m]) + i e^{2 pi i m/n} (X[m] - conj X[H-m]), z = IFFT_H(Z)
+        auto loadb = [&](int m) -> C2<double> {
+            C2<double> a = {rd[2 * m], rd[2 * m + 1]}, b = {rd[2 * (H - m)], rd[2 * (H - m) + 1]};
+            if (m == 0) {   // DC and Nyquist are real
+                a.y = 0.0;
+                b.y = 0.0;
+            }
+            const C2<double> bc = cconj(b);
+            const C2<double> e = cadd(a, bc), f = csub(a, bc);
+            const C2<double> t = cmul(cconj(twn[m]), f);
+            return {e.x - t.y, e.y + t.x};
+        };
+        fft_pq_team<double, PF, Q, false, 0>(v, row, k1, j, loadb, tw, twn, 2);
+        __syncthreads();
+    #pragma unroll
+        for (int i = 0; i < 8; ++i) row[k1 + PF * (j + i * T)] = v[i];   // x[2m] = Re z[m], x[2m+1] = Im z[m]
+        __syncthreads();
+        if (!live) return;
+        for (int q = tt; q < A.nin; q += TT) d[q] = (S)rd[q];
+    }
+};
 
 // Strided complex pencils, TX neighbours along the unit-stride dimension fa per block,
 // interleaved in LDS; the gather zero-pads (rows below gshift or from src_nd on read as
@@ -604,118 +625,130 @@ __global__ void __launch_bounds__(ROWS * PF * Q / 8) row_rz(RzArgs A) {
 // first stage skips them; HOUT: only rows below M/2 are stored (sshift 0, dst_nd <= M/2).
 // M = PF Q: PF = 1 (a power of two) or 3 (mixed radix, 8 PF values per thread, column_fft_pq)
 template <int PF, int Q, int TX, bool FWD, bool HIN, bool HOUT, typename TI, typename TO>
-__global__ void __launch_bounds__(TX * Q / 8) col_c2c(PassDesc P) {
-    constexpr int M = PF * Q, T = Q / 8, NV = 8 * PF;
-    __shared__ C2<double> lds[M * TX];
-    const int c = threadIdx.x % TX, j = threadIdx.x / TX;
-    const int ntile = (P.na + TX - 1) / TX;
-    const int tile = blockIdx.x % ntile, jb = blockIdx.x / ntile;
-    int ja = tile * TX + c;
-    const bool live = ja < P.na;
-    if (!live) ja = P.na - 1;   // dead columns read the last one and store nothing
-    const C2<TI>* src = reinterpret_cast<const C2<TI>*>(P.src) + (long long)ja * P.src_s[P.fa] + (long long)jb * P.src_s[P.fb];
-    C2<TO>* dst = reinterpret_cast<C2<TO>*>(P.dst) + (long long)ja * P.dst_s[P.fa] + (long long)jb * P.dst_s[P.fb];
-    const long long sd = P.src_s[P.d], dd = P.dst_s[P.d];
-    C2<double> v[NV];
-#pragma unroll
-    for (int i = 0; i < NV; ++i) {
-        if (HIN && i >= NV / 2) {
-            v[i] = {0.0, 0.0};
-            continue;
+struct col_c2c {
+    using Args = PassDesc;
+    static constexpr int NT = TX * Q / 8;
+    static __device__ __forceinline__ void run(const PassDesc& P, Blk blk) {
+        constexpr int M = PF * Q, T = Q / 8, NV = 8 * PF;
+        __shared__ C2<double> lds[M * TX];
+        const int c = threadIdx.x % TX, j = threadIdx.x / TX;
+        const int ntile = (P.na + TX - 1) / TX;
+        const int tile = blk.x % ntile, jb = blk.x / ntile;
+        int ja = tile * TX + c;
+        const bool live = ja < P.na;
+        if (!live) ja = P.na - 1;   // dead columns read the last one and store nothing
+        const C2<TI>* src = reinterpret_cast<const C2<TI>*>(P.src) + (long long)ja * P.src_s[P.fa] + (long long)jb * P.src_s[P.fb];
+        C2<TO>* dst = reinterpret_cast<C2<TO>*>(P.dst) + (long long)ja * P.dst_s[P.fa] + (long long)jb * P.dst_s[P.fb];
+        const long long sd = P.src_s[P.d], dd = P.dst_s[P.d];
+        C2<double> v[NV];
+    #pragma unroll
+        for (int i = 0; i < NV; ++i) {
+            if (HIN && i >= NV / 2) {
+                v[i] = {0.0, 0.0};
+                continue;
+            }
+            const int ks = j + i * T - P.gshift;
+            v[i] = (ks >= 0 && ks < P.src_nd) ? c2d(src[ks * sd]) : C2<double>{0.0, 0.0};
         }
-        const int ks = j + i * T - P.gshift;
-        v[i] = (ks >= 0 && ks < P.src_nd) ? c2d(src[ks * sd]) : C2<double>{0.0, 0.0};
+        column_fft_pq<double, PF, Q, FWD, TX, HIN && PF == 1>(v, lds + c, j, reinterpret_cast<const C2<double>*>(P.tw),
+                                                             reinterpret_cast<const C2<double>*>(P.aux), 1);
+        if (!live) return;
+    #pragma unroll
+        for (int i = 0; i < (HOUT ? NV / 2 : NV); ++i) {
+            const int k = j + i * T - P.sshift;
+            if (k >= 0 && k < P.dst_nd) dst[k * dd] = c2t<TO>(v[i]);
+        }
+        if ((P.post & POST_PERIOD) && j == 0) {   // node-centred periodic: X[0] again at position n
+            const int k = M - P.sshift;
+            if (k >= 0 && k < P.dst_nd) dst[k * dd] = c2t<TO>(v[0]);
+        }
     }
-    column_fft_pq<double, PF, Q, FWD, TX, HIN && PF == 1>(v, lds + c, j, reinterpret_cast<const C2<double>*>(P.tw),
-                                                         reinterpret_cast<const C2<double>*>(P.aux), 1);
-    if (!live) return;
-#pragma unroll
-    for (int i = 0; i < (HOUT ? NV / 2 : NV); ++i) {
-        const int k = j + i * T - P.sshift;
-        if (k >= 0 && k < P.dst_nd) dst[k * dd] = c2t<TO>(v[i]);
-    }
-    if ((P.post & POST_PERIOD) && j == 0) {   // node-centred periodic: X[0] again at position n
-        const int k = M - P.sshift;
-        if (k >= 0 && k < P.dst_nd) dst[k * dd] = c2t<TO>(v[0]);
-    }
-}
+};
 
 // Contiguous complex pencils of M = PF Q points (a c2c direction along x, after an r2c one),
 // ROWS pencils per block, M/8 threads per pencil in PF teams (fft_pq_team); each pencil is
 // staged in LDS so that HBM is read and written in whole rows, with the zero padding of the
 // gather and the truncation of the scatter.
 template <int PF, int Q, int ROWS, bool FWD, typename TI, typename TO>
-__global__ void __launch_bounds__(ROWS * PF * Q / 8) row_c2c(PassDesc P) {
-    constexpr int M = PF * Q, T = Q / 8, TT = M / 8, HP = M + M / 8;
-    __shared__ C2<double> lds[HP * ROWS];
-    const int rl = threadIdx.x / TT, tt = threadIdx.x % TT;
-    const int k1 = tt / T, j = tt % T;
-    const int g = blockIdx.x * ROWS + rl;
-    const bool live = g < P.n1 * P.n2;
-    const int gg = live ? g : 0;   // dead rows read pencil 0 and store nothing
-    const int ja = gg % P.na, jb = gg / P.na;
-    const C2<TI>* src = reinterpret_cast<const C2<TI>*>(P.src) + (long long)ja * P.src_s[P.fa] + (long long)jb * P.src_s[P.fb];
-    C2<TO>* dst = reinterpret_cast<C2<TO>*>(P.dst) + (long long)ja * P.dst_s[P.fa] + (long long)jb * P.dst_s[P.fb];
-    C2<double>* row = lds + rl * HP;
-    for (int k = tt; k < M; k += TT) {
-        const int ks = k - P.gshift;
-        row[k] = (ks >= 0 && ks < P.src_nd) ? c2d(src[ks]) : C2<double>{0.0, 0.0};
+struct row_c2c {
+    using Args = PassDesc;
+    static constexpr int NT = ROWS * PF * Q / 8;
+    static __device__ __forceinline__ void run(const PassDesc& P, Blk blk) {
+        constexpr int M = PF * Q, T = Q / 8, TT = M / 8, HP = M + M / 8;
+        __shared__ C2<double> lds[HP * ROWS];
+        const int rl = threadIdx.x / TT, tt = threadIdx.x % TT;
+        const int k1 = tt / T, j = tt % T;
+        const int g = blk.x * ROWS + rl;
+        const bool live = g < P.n1 * P.n2;
+        const int gg = live ? g : 0;   // dead rows read pencil 0 and store nothing
+        const int ja = gg % P.na, jb = gg / P.na;
+        const C2<TI>* src = reinterpret_cast<const C2<TI>*>(P.src) + (long long)ja * P.src_s[P.fa] + (long long)jb * P.src_s[P.fb];
+        C2<TO>* dst = reinterpret_cast<C2<TO>*>(P.dst) + (long long)ja * P.dst_s[P.fa] + (long long)jb * P.dst_s[P.fb];
+        C2<double>* row = lds + rl * HP;
+        for (int k = tt; k < M; k += TT) {
+            const int ks = k - P.gshift;
+            row[k] = (ks >= 0 && ks < P.src_nd) ? c2d(src[ks]) : C2<double>{0.0, 0.0};
+        }
+        __syncthreads();
+        auto load = [&](int m) -> C2<double> { return row[m]; };
+        C2<double> v[8];
+        fft_pq_team<double, PF, Q, FWD, 0>(v, row, k1, j, load, reinterpret_cast<const C2<double>*>(P.tw),
+                                            reinterpret_cast<const C2<double>*>(P.aux), 1);
+        __syncthreads();
+    #pragma unroll
+        for (int i = 0; i < 8; ++i) row[k1 + PF * (j + i * T)] = v[i];
+        if ((P.post & POST_PERIOD) && k1 == 0 && j == 0) row[M] = v[0];   // node-centred periodic copy
+        __syncthreads();
+        if (!live) return;
+        for (int k = tt; k < P.dst_nd; k += TT) {
+            const int p = k + P.sshift;
+            if (p < P.pen_out) dst[k] = c2t<TO>(row[p]);
+        }
     }
-    __syncthreads();
-    auto load = [&](int m) -> C2<double> { return row[m]; };
-    C2<double> v[8];
-    fft_pq_team<double, PF, Q, FWD, 0>(v, row, k1, j, load, reinterpret_cast<const C2<double>*>(P.tw),
-                                        reinterpret_cast<const C2<double>*>(P.aux), 1);
-    __syncthreads();
-#pragma unroll
-    for (int i = 0; i < 8; ++i) row[k1 + PF * (j + i * T)] = v[i];
-    if ((P.post & POST_PERIOD) && k1 == 0 && j == 0) row[M] = v[0];   // node-centred periodic copy
-    __syncthreads();
-    if (!live) return;
-    for (int k = tt; k < P.dst_nd; k += TT) {
-        const int p = k + P.sshift;
-        if (p < P.pen_out) dst[k] = c2t<TO>(row[p]);
-    }
-}
+};
 
 // Strided complex pencils of M = PF Q points through thread teams (fft_pq_team): the TX
 // columns are staged in LDS (rows read as contiguous runs of TX values), M/8 threads per
 // column form the PF twiddled inputs and run the Q-point FFTs. Same plan semantics as col_c2c.
 template <int PF, int Q, int TX, bool FWD, typename TI, typename TO>
-__global__ void __launch_bounds__(TX * PF * Q / 8) col_c2c_team(PassDesc P) {
-    constexpr int M = PF * Q, T = Q / 8, TT = M / 8;
-    __shared__ C2<double> lds[M * TX];
-    const int c = threadIdx.x % TX, tt = threadIdx.x / TX;
-    const int k1 = tt / T, j = tt % T;
-    const int ntile = (P.na + TX - 1) / TX;
-    const int tile = blockIdx.x % ntile, jb = blockIdx.x / ntile;
-    int ja = tile * TX + c;
-    const bool live = ja < P.na;
-    if (!live) ja = P.na - 1;   // dead columns read the last one and store nothing
-    const C2<TI>* src = reinterpret_cast<const C2<TI>*>(P.src) + (long long)ja * P.src_s[P.fa] + (long long)jb * P.src_s[P.fb];
-    C2<TO>* dst = reinterpret_cast<C2<TO>*>(P.dst) + (long long)ja * P.dst_s[P.fa] + (long long)jb * P.dst_s[P.fb];
-    const long long sd = P.src_s[P.d], dd = P.dst_s[P.d];
-#pragma unroll
-    for (int i = 0; i < 8; ++i) {
-        const int r = tt + i * TT, ks = r - P.gshift;
-        lds[r * TX + c] = (ks >= 0 && ks < P.src_nd) ? c2d(src[ks * sd]) : C2<double>{0.0, 0.0};
+struct col_c2c_team {
+    using Args = PassDesc;
+    static constexpr int NT = TX * PF * Q / 8;
+    static __device__ __forceinline__ void run(const PassDesc& P, Blk blk) {
+        constexpr int M = PF * Q, T = Q / 8, TT = M / 8;
+        __shared__ C2<double> lds[M * TX];
+        const int c = threadIdx.x % TX, tt = threadIdx.x / TX;
+        const int k1 = tt / T, j = tt % T;
+        const int ntile = (P.na + TX - 1) / TX;
+        const int tile = blk.x % ntile, jb = blk.x / ntile;
+        int ja = tile * TX + c;
+        const bool live = ja < P.na;
+        if (!live) ja = P.na - 1;   // dead columns read the last one and store nothing
+        const C2<TI>* src = reinterpret_cast<const C2<TI>*>(P.src) + (long long)ja * P.src_s[P.fa] + (long long)jb * P.src_s[P.fb];
+        C2<TO>* dst = reinterpret_cast<C2<TO>*>(P.dst) + (long long)ja * P.dst_s[P.fa] + (long long)jb * P.dst_s[P.fb];
+        const long long sd = P.src_s[P.d], dd = P.dst_s[P.d];
+    #pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const int r = tt + i * TT, ks = r - P.gshift;
+            lds[r * TX + c] = (ks >= 0 && ks < P.src_nd) ? c2d(src[ks * sd]) : C2<double>{0.0, 0.0};
+        }
+        __syncthreads();
+        auto load = [&](int m) -> C2<double> { return lds[m * TX + c]; };
+        C2<double> v[8];
+        fft_pq_team<double, PF, Q, FWD, TX>(v, lds + c, k1, j, load, reinterpret_cast<const C2<double>*>(P.tw),
+                                             reinterpret_cast<const C2<double>*>(P.aux), 1);
+        if (!live) return;
+    #pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const int k = k1 + PF * (j + i * T) - P.sshift;
+            if (k >= 0 && k < P.dst_nd) dst[k * dd] = c2t<TO>(v[i]);
+        }
+        if ((P.post & POST_PERIOD) && k1 == 0 && j == 0) {   // node-centred periodic: X[0] again at position n
+            const int k = M - P.sshift;
+            if (k >= 0 && k < P.dst_nd) dst[k * dd] = c2t<TO>(v[0]);
+        }
     }
-    __syncthreads();
-    auto load = [&](int m) -> C2<double> { return lds[m * TX + c]; };
-    C2<double> v[8];
-    fft_pq_team<double, PF, Q, FWD, TX>(v, lds + c, k1, j, load, reinterpret_cast<const C2<double>*>(P.tw),
-                                         reinterpret_cast<const C2<double>*>(P.aux), 1);
-    if (!live) return;
-#pragma unroll
-    for (int i = 0; i < 8; ++i) {
-        const int k = k1 + PF * (j + i * T) - P.sshift;
-        if (k >= 0 && k < P.dst_nd) dst[k * dd] = c2t<TO>(v[i]);
-    }
-    if ((P.post & POST_PERIOD) && k1 == 0 && j == 0) {   // node-centred periodic: X[0] again at position n
-        const int k = M - P.sshift;
-        if (k >= 0 && k < P.dst_nd) dst[k * dd] = c2t<TO>(v[0]);
-    }
-}
+};
 
 template <int PF, int Q, int R, bool COL, typename TI, typename TO>
 void launch_pen_h(const PassDesc& P, hipStream_t s) {
@@ -724,14 +757,14 @@ void launch_pen_h(const PassDesc& P, hipStream_t s) {
     dim3 grid(nb);
     dim3 block(R * PF * Q / 8);
     switch (P.kind) {
-        case REDFT10: hipLaunchKernelGGL((pen_r2r<PF, Q, R, REDFT10, COL, TI, TO>), grid, block, 0, s, P); break;
-        case REDFT01: hipLaunchKernelGGL((pen_r2r<PF, Q, R, REDFT01, COL, TI, TO>), grid, block, 0, s, P); break;
-        case REDFT11: hipLaunchKernelGGL((pen_r2r<PF, Q, R, REDFT11, COL, TI, TO>), grid, block, 0, s, P); break;
-        case RODFT10: hipLaunchKernelGGL((pen_r2r<PF, Q, R, RODFT10, COL, TI, TO>), grid, block, 0, s, P); break;
-        case RODFT01: hipLaunchKernelGGL((pen_r2r<PF, Q, R, RODFT01, COL, TI, TO>), grid, block, 0, s, P); break;
-        case RODFT11: hipLaunchKernelGGL((pen_r2r<PF, Q, R, RODFT11, COL, TI, TO>), grid, block, 0, s, P); break;
-        case REDFT00: hipLaunchKernelGGL((pen_r2r<PF, Q, R, REDFT00, COL, TI, TO>), grid, block, 0, s, P); break;
-        case RODFT00: hipLaunchKernelGGL((pen_r2r<PF, Q, R, RODFT00, COL, TI, TO>), grid, block, 0, s, P); break;
+        case REDFT10: launch_k<pen_r2r<PF, Q, R, REDFT10, COL, TI, TO>>(grid, block, 0, s, P); break;
+        case REDFT01: launch_k<pen_r2r<PF, Q, R, REDFT01, COL, TI, TO>>(grid, block, 0, s, P); break;
+        case REDFT11: launch_k<pen_r2r<PF, Q, R, REDFT11, COL, TI, TO>>(grid, block, 0, s, P); break;
+        case RODFT10: launch_k<pen_r2r<PF, Q, R, RODFT10, COL, TI, TO>>(grid, block, 0, s, P); break;
+        case RODFT01: launch_k<pen_r2r<PF, Q, R, RODFT01, COL, TI, TO>>(grid, block, 0, s, P); break;
+        case RODFT11: launch_k<pen_r2r<PF, Q, R, RODFT11, COL, TI, TO>>(grid, block, 0, s, P); break;
+        case REDFT00: launch_k<pen_r2r<PF, Q, R, REDFT00, COL, TI, TO>>(grid, block, 0, s, P); break;
+        case RODFT00: launch_k<pen_r2r<PF, Q, R, RODFT00, COL, TI, TO>>(grid, block, 0, s, P); break;
         default: fail("r2r pencil kernel: kind %d", P.kind);
     }
 }
@@ -754,8 +787,8 @@ template <int PF, int Q, int ROWS, typename TI, typename TO>
 void launch_row_rc_h(const PassDesc& P, hipStream_t s) {
     dim3 grid((P.n1 * P.n2 + ROWS - 1) / ROWS);
     dim3 block(ROWS * PF * Q / 8);
-    if (P.op == OP_R2C) hipLaunchKernelGGL((row_rc<PF, Q, ROWS, true, TI, TO>), grid, block, 0, s, P);
-    else hipLaunchKernelGGL((row_rc<PF, Q, ROWS, false, TI, TO>), grid, block, 0, s, P);
+    if (P.op == OP_R2C) launch_k<row_rc<PF, Q, ROWS, true, TI, TO>>(grid, block, 0, s, P);
+    else launch_k<row_rc<PF, Q, ROWS, false, TI, TO>>(grid, block, 0, s, P);
 }
 // H = PF Q, 3 <= PF <= 7: about 256 threads (2048 / H rows) and 36 KB of LDS per block
 template <int PF, typename TI, typename TO>
@@ -776,8 +809,8 @@ void launch_col_real_m(const PassDesc& P, hipStream_t s) {
     const int ntile = (npair + TX - 1) / TX;
     const int nb = P.n1 * P.n2 / P.na;
     dim3 grid(ntile * nb);
-    if (P.op == OP_R2C) hipLaunchKernelGGL((col_real<PF, Q, TX, true, TI, TO>), grid, dim3(TX * Q / 8), 0, s, P);
-    else hipLaunchKernelGGL((col_real<PF, Q, TX, false, TI, TO>), grid, dim3(TX * Q / 8), 0, s, P);
+    if (P.op == OP_R2C) launch_k<col_real<PF, Q, TX, true, TI, TO>>(grid, dim3(TX * Q / 8), 0, s, P);
+    else launch_k<col_real<PF, Q, TX, false, TI, TO>>(grid, dim3(TX * Q / 8), 0, s, P);
 }
 // mixed radix: TX Q = 1024 (PF 3) or 512 (PF 5), LDS about 48 / 40 KB
 template <int PF, typename TI, typename TO>
@@ -801,11 +834,11 @@ void launch_col_c2c_m(const PassDesc& P, hipStream_t s) {
     dim3 block(TX * Q / 8);
     const bool hin = P.gshift == 0 && P.src_nd <= M / 2, hout = P.sshift == 0 && P.dst_nd <= M / 2;
     if (P.sign == FWD) {
-        if (hin) hipLaunchKernelGGL((col_c2c<PF, Q, TX, true, true, false, TI, TO>), grid, block, 0, s, P);
-        else hipLaunchKernelGGL((col_c2c<PF, Q, TX, true, false, false, TI, TO>), grid, block, 0, s, P);
+        if (hin) launch_k<col_c2c<PF, Q, TX, true, true, false, TI, TO>>(grid, block, 0, s, P);
+        else launch_k<col_c2c<PF, Q, TX, true, false, false, TI, TO>>(grid, block, 0, s, P);
     } else {
-        if (hout) hipLaunchKernelGGL((col_c2c<PF, Q, TX, false, false, true, TI, TO>), grid, block, 0, s, P);
-        else hipLaunchKernelGGL((col_c2c<PF, Q, TX, false, false, false, TI, TO>), grid, block, 0, s, P);
+        if (hout) launch_k<col_c2c<PF, Q, TX, false, false, true, TI, TO>>(grid, block, 0, s, P);
+        else launch_k<col_c2c<PF, Q, TX, false, false, false, TI, TO>>(grid, block, 0, s, P);
     }
 }
 template <int PF, int Q, int TX, typename TI, typename TO>
@@ -813,16 +846,16 @@ void launch_col_c2c_t(const PassDesc& P, hipStream_t s) {
     const int ntile = (P.na + TX - 1) / TX;
     dim3 grid(ntile * (P.n1 * P.n2 / P.na));
     dim3 block(TX * PF * Q / 8);
-    if (P.sign == FWD) hipLaunchKernelGGL((col_c2c_team<PF, Q, TX, true, TI, TO>), grid, block, 0, s, P);
-    else hipLaunchKernelGGL((col_c2c_team<PF, Q, TX, false, TI, TO>), grid, block, 0, s, P);
+    if (P.sign == FWD) launch_k<col_c2c_team<PF, Q, TX, true, TI, TO>>(grid, block, 0, s, P);
+    else launch_k<col_c2c_team<PF, Q, TX, false, TI, TO>>(grid, block, 0, s, P);
 }
 template <int PF, int Q, typename TI, typename TO>
 void launch_row_c2c_h(const PassDesc& P, hipStream_t s) {
     constexpr int ROWS = 2048 / (PF * Q) > 0 ? 2048 / (PF * Q) : 1;   // about 256 threads, 36 KB of LDS
     dim3 grid((P.n1 * P.n2 + ROWS - 1) / ROWS);
     dim3 block(ROWS * PF * Q / 8);
-    if (P.sign == FWD) hipLaunchKernelGGL((row_c2c<PF, Q, ROWS, true, TI, TO>), grid, block, 0, s, P);
-    else hipLaunchKernelGGL((row_c2c<PF, Q, ROWS, false, TI, TO>), grid, block, 0, s, P);
+    if (P.sign == FWD) launch_k<row_c2c<PF, Q, ROWS, true, TI, TO>>(grid, block, 0, s, P);
+    else launch_k<row_c2c<PF, Q, ROWS, false, TI, TO>>(grid, block, 0, s, P);
 }
 template <int PF, typename TI, typename TO>
 void launch_row_c2c_pf(int q, const PassDesc& P, hipStream_t s) {
@@ -865,13 +898,13 @@ void launch_col_c2c_pf(int q, const PassDesc& P, hipStream_t s) {
 template <int M, int TX, typename S>
 void launch_col_rz_m(const RzArgs& A, hipStream_t s) {
     const int ntile = ((A.nx + 1) / 2 + TX - 1) / TX;
-    hipLaunchKernelGGL((col_rz<M, TX, S>), dim3(ntile * A.ny, A.lda), dim3(TX * M / 8), 0, s, A);
+    launch_k<col_rz<M, TX, S>>(dim3(ntile * A.ny, A.lda), dim3(TX * M / 8), 0, s, A);
 }
 
 template <int PF, int Q, typename S>
 void launch_row_rz_h(const RzArgs& A, hipStream_t s) {
     constexpr int ROWS = 2048 / (PF * Q) > 0 ? 2048 / (PF * Q) : 1;   // about 256 threads, 36 KB of LDS
-    hipLaunchKernelGGL((row_rz<PF, Q, ROWS, S>), dim3((A.rows + ROWS - 1) / ROWS, A.lda), dim3(ROWS * PF * Q / 8), 0, s, A);
+    launch_k<row_rz<PF, Q, ROWS, S>>(dim3((A.rows + ROWS - 1) / ROWS, A.lda), dim3(ROWS * PF * Q / 8), 0, s, A);
 }
 
 template <int PF, typename S>

@@ -11,9 +11,14 @@
 // requests may still run on solvers that are already cached (no new memory),
 // so a large request is not starved by small ones.
 //
-// Batch: a fixed list of problems, set up once; their solves are captured
-// into one hipGraph (one independent branch per problem) and replayed with a
-// single launch.
+// Batch: a fixed list of problems, set up once. Each problem's solve is recorded as a list of
+// kernel launches (group.hpp); launches of the same kernel at the same step of different
+// problems become one grouped launch whose blocks are mapped to the problems through an offset
+// table, so a batch of 64 problems of 5-9 kernels each runs as a few hundred launches of
+// blocks from many problems instead of one launch per problem and step. The grouped launches
+// form one hipGraph (each depends on the launches of the previous step of its problems) that
+// is replayed with a single launch. The previous form, one graph branch per problem, is kept
+// for comparison (run_branches).
 #define FLUPS_BUILDING_LIBRARY 1
 #include <algorithm>
 #include <chrono>
@@ -29,9 +34,15 @@
 #include <vector>
 
 #include "../../include/flups.h"
+#include "group.hpp"
 #include "solver.hpp"
 
 namespace flups {
+
+LaunchSink*& launch_sink() {
+    thread_local LaunchSink* sink = nullptr;
+    return sink;
+}
 
 namespace {
 
@@ -408,8 +419,12 @@ class Batch {
         }
         HIP_CHECK(hipStreamEndCapture(main_, &graph_));
         HIP_CHECK(hipGraphInstantiate(&exec_, graph_, nullptr, nullptr, 0));
+        build_grouped();
     }
     ~Batch() {
+        if (gexec_) (void)hipGraphExecDestroy(gexec_);
+        if (ggraph_) (void)hipGraphDestroy(ggraph_);
+        if (dtab_) (void)hipFree(dtab_);
         if (exec_) (void)hipGraphExecDestroy(exec_);
         if (graph_) (void)hipGraphDestroy(graph_);
         for (auto& it : items_) {
@@ -422,15 +437,17 @@ class Batch {
         if (t1_) (void)hipEventDestroy(t1_);
         if (main_) (void)hipStreamDestroy(main_);
     }
-    void run() {
+    // grouped launches (the default) or one graph branch per problem
+    void run(bool grouped = true) {
         HIP_CHECK(hipEventRecord(t0_, main_));
-        HIP_CHECK(hipGraphLaunch(exec_, main_));
+        HIP_CHECK(hipGraphLaunch(grouped ? gexec_ : exec_, main_));
         HIP_CHECK(hipEventRecord(t1_, main_));
         HIP_CHECK(hipStreamSynchronize(main_));
         float ms = 0;
         HIP_CHECK(hipEventElapsedTime(&ms, t0_, t1_));
         last_ms_ = ms;
     }
+    int launches(bool grouped) const { return grouped ? nlaunch_grouped_ : nlaunch_solo_; }
     // the same problems one after the other on the main stream, without the graph
     void run_sequential() {
         for (auto& it : items_) {
@@ -454,6 +471,189 @@ class Batch {
         }
         return a.type == hipMemoryTypeDevice;
     }
+
+    // record every problem's solve, group the launches by (step, kernel, block size, dynamic
+    // LDS), and build the graph of grouped launches
+    void build_grouped() {
+        const int np = (int)items_.size();
+        std::vector<std::vector<LaunchRec>> recs(np);
+        for (int p = 0; p < np; ++p) {
+            LaunchSink sink;
+            launch_sink() = &sink;
+            try {
+                items_[p].solver->enqueue(items_[p].field, items_[p].rhs, items_[p].type);
+            } catch (...) {
+                launch_sink() = nullptr;
+                throw;
+            }
+            launch_sink() = nullptr;
+            FLUPS_REQUIRE(sink.same == 0, "unbalanced launch stages while recording batch problem %d", p);
+            recs[p] = std::move(sink.recs);
+            nlaunch_solo_ += (int)recs[p].size();
+        }
+        // groups: key -> (problem, record) pairs, larger grids first within a group
+        // clusters: the problems sorted by algorithmic bytes and cut into `ncl` runs of about equal
+        // size; launches are grouped within a cluster only, so that a grouped launch holds problems
+        // of about the same work and one slow problem does not hold back the steps of small ones
+        std::vector<int> order(np), cl(np, 0);
+        for (int p = 0; p < np; ++p) order[p] = p;
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+            return items_[a].solver->algorithmic_bytes(items_[a].type) > items_[b].solver->algorithmic_bytes(items_[b].type);
+        });
+        const char* re = std::getenv("FLUPS_BATCH_RATIO");
+        const double ratio = std::max(1.0, re ? std::atof(re) : kRatio);
+        const char* ce = std::getenv("FLUPS_BATCH_CLUSTERS");
+        const int ncl = std::max(1, std::min(np, ce ? std::atoi(ce) : kClusters));
+        std::vector<int> rank(np);
+        for (int i = 0; i < np; ++i) {
+            cl[order[i]] = (int)((long long)i * ncl / np);
+            rank[order[i]] = i;
+        }
+        struct Key {
+            int cluster;
+            int stage;
+            const void* fn;
+            int threads;
+            size_t smem, argsz;
+            bool operator<(const Key& o) const {
+                if (stage != o.stage) return stage < o.stage;
+                if (cluster != o.cluster) return cluster < o.cluster;
+                if (fn != o.fn) return fn < o.fn;
+                if (threads != o.threads) return threads < o.threads;
+                if (smem != o.smem) return smem < o.smem;
+                return argsz < o.argsz;
+            }
+        };
+        std::map<Key, std::vector<std::pair<int, int>>> groups;
+        for (int p = 0; p < np; ++p)
+            for (int r = 0; r < (int)recs[p].size(); ++r) {
+                const LaunchRec& L = recs[p][r];
+                FLUPS_REQUIRE(L.gx >= 1 && L.gy >= 1 && L.threads >= 1, "batch problem %d: empty launch recorded", p);
+                groups[Key{cl[p], L.stage, L.grouped, L.threads, L.smem, L.args.size()}].push_back({p, r});
+            }
+        struct Chunk {
+            std::vector<std::pair<int, int>> ent;
+            size_t tab = 0;   // byte offset of its Args table
+            int stride = 0;
+        };
+        std::vector<Chunk> chunks;
+        size_t tab_bytes = 0;
+        for (auto& kv : groups) {
+            auto& v = kv.second;
+            std::stable_sort(v.begin(), v.end(), [&](const std::pair<int, int>& a, const std::pair<int, int>& b) {
+                const LaunchRec& x = recs[a.first][a.second];
+                const LaunchRec& y = recs[b.first][b.second];
+                return (long long)x.gx * x.gy > (long long)y.gx * y.gy;
+            });
+            // a chunk holds launches whose grids are within a factor `ratio` of its largest one
+            auto blocks = [&](const std::pair<int, int>& e) {
+                const LaunchRec& x = recs[e.first][e.second];
+                return (long long)x.gx * x.gy;
+            };
+            for (size_t i = 0; i < v.size();) {
+                size_t j = i + 1;
+                while (j < v.size() && j - i < (size_t)kGroupMax && (double)blocks(v[j]) * ratio >= (double)blocks(v[i])) ++j;
+                Chunk c;
+                c.ent.assign(v.begin() + i, v.begin() + j);
+                i = j;
+                if (c.ent.size() > 1) {
+                    c.stride = (int)((kv.first.argsz + 15) / 16 * 16);
+                    c.tab = tab_bytes;
+                    tab_bytes += (size_t)c.stride * c.ent.size();
+                }
+                chunks.push_back(std::move(c));
+            }
+        }
+        // within a step, the launches of the larger problems first
+        std::stable_sort(chunks.begin(), chunks.end(), [&](const Chunk& a, const Chunk& b) {
+            const int sa = recs[a.ent[0].first][a.ent[0].second].stage, sb = recs[b.ent[0].first][b.ent[0].second].stage;
+            if (sa != sb) return sa < sb;
+            return rank[a.ent[0].first] < rank[b.ent[0].first];
+        });
+        std::vector<char> host(std::max<size_t>(tab_bytes, 16), 0);
+        for (auto& c : chunks)
+            for (size_t e = 0; c.ent.size() > 1 && e < c.ent.size(); ++e) {
+                const LaunchRec& L = recs[c.ent[e].first][c.ent[e].second];
+                std::memcpy(host.data() + c.tab + e * c.stride, L.args.data(), L.args.size());
+            }
+        HIP_CHECK(hipMalloc(&dtab_, host.size()));
+        HIP_CHECK(hipMemcpy(dtab_, host.data(), host.size(), hipMemcpyHostToDevice));
+
+        // graph: chunks in step order; each depends on the chunks that hold the previous
+        // step of any of its problems
+        HIP_CHECK(hipGraphCreate(&ggraph_, 0));
+        std::map<std::pair<int, int>, std::vector<hipGraphNode_t>> at;   // (problem, step) -> nodes
+        std::vector<std::vector<int>> steps(np);                          // steps of each problem, ascending
+        for (int p = 0; p < np; ++p) {
+            for (const LaunchRec& L : recs[p])
+                if (steps[p].empty() || steps[p].back() != L.stage) steps[p].push_back(L.stage);
+        }
+        auto prev_step = [&](int p, int st) {
+            int best = -1;
+            for (int x : steps[p])
+                if (x < st) best = x;
+            return best;
+        };
+        const char* je = std::getenv("FLUPS_BATCH_JOIN");
+        const bool join = je && std::atoi(je) != 0;
+        std::vector<hipGraphNode_t> prev, cur;
+        int jstage = -1;
+        for (auto& c : chunks) {
+            const LaunchRec& L0 = recs[c.ent[0].first][c.ent[0].second];
+            std::vector<hipGraphNode_t> deps;
+            for (auto& e : c.ent) {
+                const int ps = prev_step(e.first, L0.stage);
+                if (ps < 0) continue;
+                for (hipGraphNode_t n : at[{e.first, ps}])
+                    if (std::find(deps.begin(), deps.end(), n) == deps.end()) deps.push_back(n);
+            }
+            hipKernelNodeParams kp{};
+            GroupArgs g{};
+            void* args[1];
+            kp.blockDim = dim3(L0.threads);
+            kp.sharedMemBytes = (unsigned)L0.smem;
+            if (c.ent.size() == 1) {
+                kp.func = const_cast<void*>(L0.solo);
+                kp.gridDim = dim3(L0.gx, L0.gy);
+                args[0] = const_cast<char*>(L0.args.data());
+            } else {
+                g.tab = static_cast<const char*>(dtab_) + c.tab;
+                g.stride = c.stride;
+                g.n = (int)c.ent.size();
+                long long tot = 0;
+                for (int e = 0; e < g.n; ++e) {
+                    const LaunchRec& L = recs[c.ent[e].first][c.ent[e].second];
+                    g.start[e] = (int)tot;
+                    g.gx[e] = L.gx;
+                    tot += (long long)L.gx * L.gy;
+                }
+                FLUPS_REQUIRE(tot < (1LL << 31), "grouped launch of %lld blocks", tot);
+                g.start[g.n] = (int)tot;
+                kp.func = const_cast<void*>(L0.grouped);
+                kp.gridDim = dim3((unsigned)tot);
+                args[0] = &g;
+            }
+            kp.kernelParams = args;
+            kp.extra = nullptr;
+            if (join) {   // every launch of a step after every launch of the step before
+                if (L0.stage != jstage) {
+                    hipGraphNode_t j;
+                    HIP_CHECK(hipGraphAddEmptyNode(&j, ggraph_, cur.data(), cur.size()));
+                    prev = {j};
+                    cur.clear();
+                    jstage = L0.stage;
+                }
+                deps = prev;
+            }
+            hipGraphNode_t node;
+            HIP_CHECK(hipGraphAddKernelNode(&node, ggraph_, deps.data(), deps.size(), &kp));
+            cur.push_back(node);
+            ++nlaunch_grouped_;
+            for (auto& e : c.ent) at[{e.first, L0.stage}].push_back(node);
+        }
+        HIP_CHECK(hipGraphInstantiate(&gexec_, ggraph_, nullptr, nullptr, 0));
+    }
+   private:
     struct Item {
         std::unique_ptr<SolverImpl> solver;
         hipStream_t stream = nullptr;
@@ -465,8 +665,16 @@ class Batch {
     std::vector<Item> items_;
     hipStream_t main_ = nullptr;
     hipEvent_t fork_ = nullptr, t0_ = nullptr, t1_ = nullptr;
-    hipGraph_t graph_ = nullptr;
-    hipGraphExec_t exec_ = nullptr;
+    hipGraph_t graph_ = nullptr, ggraph_ = nullptr;
+    hipGraphExec_t exec_ = nullptr, gexec_ = nullptr;
+    // grouping policy, measured on the 64-problem mixes of the batch benchmark: launches are grouped
+    // across all problems (one cluster) but only with launches of the same grid (ratio 1); wider
+    // groups hold back their small problems behind the large ones (3.0 ms for the random mix with
+    // equal grids only, 3.6 ms when any grids are grouped)
+    static constexpr int kClusters = 1;
+    static constexpr double kRatio = 1.0;
+    void* dtab_ = nullptr;   // Args tables of the grouped launches
+    int nlaunch_solo_ = 0, nlaunch_grouped_ = 0;
     double last_ms_ = 0;
 };
 
@@ -559,6 +767,16 @@ int flups_batch_run(FLUPS_Batch* b) {
         reinterpret_cast<FLUPS_BatchS*>(b)->b.run();
         return 1;
     });
+}
+int flups_batch_run_branches(FLUPS_Batch* b) {
+    return sguard("flups_batch_run_branches", [&]() -> int {
+        FLUPS_REQUIRE(b, "NULL batch");
+        reinterpret_cast<FLUPS_BatchS*>(b)->b.run(false);
+        return 1;
+    });
+}
+int flups_batch_launches(FLUPS_Batch* b, int grouped) {
+    return b ? reinterpret_cast<FLUPS_BatchS*>(b)->b.launches(grouped != 0) : 0;
 }
 int flups_batch_run_sequential(FLUPS_Batch* b) {
     return sguard("flups_batch_run_sequential", [&]() -> int {

@@ -6,6 +6,8 @@
 //                (src/Solver.cpp:1207-1272, 1402-1542)
 #include "solver.hpp"
 
+#include "group.hpp"
+
 #include <dlfcn.h>
 
 #include <algorithm>
@@ -752,6 +754,7 @@ void SolverImpl::forward_passes(const View& uin, void* spec) {
         const DirPlan& p = fwd_[ip];
         const View src = (ip == 0) ? uin : dense_view(dstage_[ip - 1], stage_[ip - 1], ses_);
         const View dst = dense_view(ip == ndim_ - 1 ? spec : dstage_[ip], stage_[ip], ses_);
+        SameStage same;   // the components' launches are independent
         for (int c = 0; c < lda_; ++c) run_pass(pass_desc(p, c, true, src, dst, hat_[ip], stage_[ip], p.fieldstart, 0));
     }
 }
@@ -790,6 +793,7 @@ void SolverImpl::backward_passes(const void* spec, const View& uout, bool diff) 
         const DirPlan& p = back[ip];
         const View src = dense_view(ip == ndim_ - 1 ? const_cast<void*>(spec) : dstage_[ip], stage_[ip], ses_);
         const View dst = (ip == 0) ? uout : dense_view(dstage_[ip - 1], stage_[ip - 1], ses_);
+        SameStage same;
         for (int c = 0; c < lda_; ++c)
             run_pass(pass_desc(p, c, false, src, dst, stage_[ip], hat_[ip], 0, fwd_[ip].fieldstart));
     }
@@ -803,6 +807,7 @@ void SolverImpl::solve_generic(double* field, const double* rhs, int type) {
             const DirPlan& p = fwd_[ip];
             const View src = (ip == 0) ? uin : dense_view(dstage_[ip - 1], stage_[ip - 1], ses_);
             const View dst = dense_view(dstage_[ip], stage_[ip], ses_);
+            SameStage same;
             for (int c = 0; c < lda_; ++c) run_pass(pass_desc(p, c, true, src, dst, hat_[ip], stage_[ip], p.fieldstart, 0));
         }
         pmark("fftw forward");
@@ -812,6 +817,7 @@ void SolverImpl::solve_generic(double* field, const double* rhs, int type) {
             const DirPlan& p = bwd_[ip];
             const View src = dense_view(dstage_[ip], stage_[ip], ses_);
             const View dst = (ip == 0) ? uout : dense_view(dstage_[ip - 1], stage_[ip - 1], ses_);
+            SameStage same;
             for (int c = 0; c < lda_; ++c)
                 run_pass(pass_desc(p, c, false, src, dst, stage_[ip], hat_[ip], 0, fwd_[ip].fieldstart));
         }

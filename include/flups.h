@@ -269,12 +269,17 @@ void          flups_server_reset_stats(FLUPS_Server* s);
 size_t        flups_server_request_bytes(const FLUPS_ServeRequest* req);
 
 /* batch: n independent problems (device buffers), set up once, solved by one hipGraph
-   launch (one branch per problem) */
+   launch. flups_batch_run: launches of the same kernel, step and grid in several problems
+   run as one grouped launch (blocks mapped to problems through an offset table; up to 32
+   launches per group); every problem's result is bit for bit its result solved alone.
+   flups_batch_run_branches: the same graph with one branch per problem and its own launches. */
 FLUPS_Batch*  flups_batch_new(int n, const FLUPS_ServeRequest* reqs, const double* const* rhs, double* const* field);
 void          flups_batch_free(FLUPS_Batch* b);
 int           flups_batch_run(FLUPS_Batch* b);              /* 1 on success; waits for completion */
+int           flups_batch_run_branches(FLUPS_Batch* b);     /* 1 on success; waits for completion */
 int           flups_batch_run_sequential(FLUPS_Batch* b);   /* same problems one by one, no graph */
-double        flups_batch_last_ms(FLUPS_Batch* b);          /* device time of the last flups_batch_run */
+double        flups_batch_last_ms(FLUPS_Batch* b);          /* device time of the last run / run_branches */
+int           flups_batch_launches(FLUPS_Batch* b, int grouped);   /* kernel launches of one run (grouped 1) or run_branches (0) */
 size_t        flups_batch_device_bytes(FLUPS_Batch* b);
 
 #ifdef __cplusplus

@@ -144,21 +144,82 @@ def test_server_rejects_oversized_request():
     srv.close()
 
 
+def rot_problems():
+    """Biot-Savart (ROT) problems with three components: the fused pipeline's rotational z pass (all
+    unbounded) and the general path's passes with the rotational multiply (spectral and centred-difference
+    derivative symbols, FP64 and FP32 storage)"""
+    P = []
+    spec = [((32, 40, 48), "unb", 2, False, 1), ((48, 32, 40), "perx", 2, False, 1), ((40, 48, 32), "symx-pery", 2, True, 2),
+            ((32, 32, 48), "per", 4, False, 4), ((48, 40, 32), "unb", 8, True, 1)]
+    for k, (ng, tpl, green, fp32, odiff) in enumerate(spec):
+        n = max(ng)
+        h = [1.0 / n] * 3
+        L = [h[0] * m for m in ng]
+        bc = TEMPLATES[tpl]
+        r = fl.ServeRequest.make(ng, bc, h, L, green=green, lda=3, fp32=fp32, solver_type=1, order_diff=odiff)
+        rhs = blob_field(r.shape(), 300 + k)
+        if tpl == "per":
+            rhs -= rhs.mean(axis=(1, 2, 3), keepdims=True)
+        if tpl == "symx-pery":
+            rhs -= rhs.mean(axis=2, keepdims=True)
+        P.append((r, rhs, ng, bc, h, L, green, fp32, odiff))
+    return P
+
+
+def solve_alone_rot(r, rhs, ng, bc, h, L, green, fp32, odiff):
+    topo = fl.Topology(0, 3, list(ng), [1, 1, 1], False, alignment=8)
+    s = fl.Solver(topo, bc, h, L, order_diff=odiff)
+    s.set_green_type(green)
+    if fp32:
+        s.set_precision(True)
+    s.setup()
+    f = np.zeros_like(rhs)
+    s.solve(f, np.ascontiguousarray(rhs), 1)
+    return f
+
+
+def test_rot_problems_alone_vs_oracle():
+    for (r, rhs, ng, bc, h, L, green, fp32, odiff) in rot_problems():
+        fa = solve_alone_rot(r, rhs, ng, bc, h, L, green, fp32, odiff)
+        fo = orc.solve(list(ng), bc, 3, h, L, 1, green, rhs, typ=1, odiff=odiff)
+        e2, ei = rel(fa, fo)
+        tol = 1e-5 if fp32 else 1e-12
+        assert e2 <= tol and ei <= tol, (ng, bc, green, fp32, odiff, e2, ei)
+
+
 def test_batch_graph_replays_match_eager():
+    """grouped launches, one branch per problem and one problem after the other all give each problem's
+    result solved alone, bit for bit, over replays with eager work between them"""
     P = problems()
-    alone = [solve_alone(*p) for p in P]
-    dev_in = [fl.DeviceArray.from_numpy(p[1]) for p in P]
-    dev_out = [fl.DeviceArray(p[1].shape) for p in P]
-    b = fl.Batch([p[0] for p in P], dev_in, dev_out)
-    for rep in range(3):
-        for d in dev_out:   # eager work between replays: clear the outputs through the library
+    # copies of two problems with other right-hand sides: launches of equal grids are grouped across problems
+    for k, seed in ((0, 500), (5, 501)):
+        q = list(P[k])
+        q[1] = blob_field(q[0].shape(), seed)
+        P.append(tuple(q))
+    R = rot_problems()
+    alone = [solve_alone(*p) for p in P] + [solve_alone_rot(*p) for p in R]
+    reqs = [p[0] for p in P] + [p[0] for p in R]
+    rhs = [p[1] for p in P] + [p[1] for p in R]
+    dev_in = [fl.DeviceArray.from_numpy(x) for x in rhs]
+    dev_out = [fl.DeviceArray(x.shape) for x in rhs]
+    b = fl.Batch(reqs, dev_in, dev_out)
+    ng, nb = b.launches(True), b.launches(False)
+    assert 0 < ng < nb, (ng, nb)
+
+    def clear():   # eager work between replays: clear the outputs through the library
+        for d in dev_out:
             zero = fl.DeviceArray.from_numpy(np.zeros(d.shape))
             assert fl.lib().flups_memcpy(d.ptr, zero.ptr, d.nbytes) == 1
             del zero
-        ms = b.run()
-        assert ms > 0
-        for i, fa in enumerate(alone):
-            assert np.array_equal(dev_out[i].numpy(), fa), ("graph replay", rep, i)
+
+    for mode in ("grouped", "branches", "grouped"):
+        for rep in range(2):
+            clear()
+            ms = b.run() if mode == "grouped" else b.run_branches()
+            assert ms > 0
+            for i, fa in enumerate(alone):
+                assert np.array_equal(dev_out[i].numpy(), fa), (mode, rep, i)
+    clear()
     b.run_sequential()
     for i, fa in enumerate(alone):
         assert np.array_equal(dev_out[i].numpy(), fa), ("sequential", i)
