@@ -158,7 +158,33 @@ struct pen_r2r {
         };
 
         // 2. pre-map to the complex sequence of length H: element m
+        auto z01 = [&](int m) -> C2<double> {   // REDFT01
+            auto U = [&](int k) {
+                const double a = k < N ? xin(k) : 0.0, b = (N - k) < N ? xin(N - k) : 0.0;
+                return cmul(A[k], C2<double>{a, -b});
+            };
+            const C2<double> uk = U(m), uc = cconj(U(H - m));
+            const C2<double> e = cadd(uk, uc), f = csub(uk, uc);
+            const C2<double> t = cmul(B[m], f);
+            return {e.x - t.y, e.y + t.x};   // e + i t
+        };
+        // mixed radix (PF > 1) with one thread per team (Q = 8) reads every input PF times in one
+        // unrolled loop: the REDFT01 sequence is built in LDS first (all of it read into registers,
+        // then written over the pencil), so that the transform's loads are plain LDS reads;
+        // inlined there, the construction held 256 registers and more live (one wave per SIMD).
+        // With larger teams the inlined form needs no more registers than the staged one.
+        constexpr bool PRE = DCT == REDFT01 && PF > 1 && Q == 8;
+        if constexpr (PRE) {
+            C2<double> zt[8];
+    #pragma unroll
+            for (int i = 0; i < 8; ++i) zt[i] = z01(tt + i * TT);
+            __syncthreads();
+    #pragma unroll
+            for (int i = 0; i < 8; ++i) RW(tt + i * TT) = zt[i];
+            __syncthreads();
+        }
         auto load = [&](int m) -> C2<double> {
+            if (PRE) return RW(m);
             if (KIND == REDFT00) {   // e[q] = x[q] (q <= H), x[2H - q] (q > H)
                 auto e = [&](int q) { return xin(q <= H ? q : N - q); };
                 return {e(2 * m), e(2 * m + 1)};
@@ -171,14 +197,7 @@ struct pen_r2r {
                 auto u = [&](int q) { return q < H ? xin(2 * q) : xin(2 * (N - 1 - q) + 1); };
                 return {u(2 * m), u(2 * m + 1)};
             } else {   // REDFT01
-                auto U = [&](int k) {
-                    const double a = k < N ? xin(k) : 0.0, b = (N - k) < N ? xin(N - k) : 0.0;
-                    return cmul(A[k], C2<double>{a, -b});
-                };
-                const C2<double> uk = U(m), uc = cconj(U(H - m));
-                const C2<double> e = cadd(uk, uc), f = csub(uk, uc);
-                const C2<double> t = cmul(B[m], f);
-                return {e.x - t.y, e.y + t.x};   // e + i t
+                return z01(m);
             }
         };
 
@@ -316,8 +335,7 @@ struct row_rc {
         __syncthreads();
         // element m of the packed complex sequence: z[m] = x[2m] + i x[2m+1] (r2c), or built from
         // X[m] and X[H-m] (c2r)
-        auto load = [&](int m) -> C2<double> {
-            if (FWD) return {rd[2 * m], rd[2 * m + 1]};
+        auto c2r_z = [&](int m) -> C2<double> {
             C2<double> a = {rd[2 * m], rd[2 * m + 1]}, b = {rd[2 * (H - m)], rd[2 * (H - m) + 1]};
             if (m == 0) {   // DC and Nyquist are real
                 a.y = 0.0;
@@ -327,6 +345,23 @@ struct row_rc {
             const C2<double> e = cadd(a, bc), f = csub(a, bc);
             const C2<double> t = cmul(cconj(twn[m]), f);   // e^{+2 pi i m/n} (a - conj b)
             return {e.x - t.y, e.y + t.x};
+        };
+        // mixed radix (PF > 1) reads every input PF times: build z in the row first, the pairs
+        // (m, H - m) by one thread, so that the transform's loads are plain LDS reads (inlined into
+        // its PF-fold unrolled loads, the construction held several hundred registers live)
+        constexpr bool PRE = !FWD && PF > 1;
+        if constexpr (PRE) {
+            for (int m = tt; m <= H / 2; m += TT) {
+                const C2<double> zm = c2r_z(m);
+                const C2<double> zh = (m > 0 && 2 * m != H) ? c2r_z(H - m) : C2<double>{0, 0};
+                row[m] = zm;
+                if (m > 0 && 2 * m != H) row[H - m] = zh;
+            }
+            __syncthreads();
+        }
+        auto load = [&](int m) -> C2<double> {
+            if (FWD || PRE) return {rd[2 * m], rd[2 * m + 1]};
+            return c2r_z(m);
         };
         C2<double> v[8];
         fft_pq_team<double, PF, Q, FWD, 0>(v, row, k1, j, load, tw, twn, 2);   // w_H^e = twn[2 e]
@@ -597,7 +632,7 @@ struct row_rz {
         if (tt == 0) row[H] = XH;
         __syncthreads();
         // c2r: Z[m] = (X[m] + conj X[H-m]) + i e^{2 pi i m/n} (X[m] - conj X[H-m]), z = IFFT_H(Z)
-        auto loadb = [&](int m) -> C2<double> {
+        auto c2r_z = [&](int m) -> C2<double> {
             C2<double> a = {rd[2 * m], rd[2 * m + 1]}, b = {rd[2 * (H - m)], rd[2 * (H - m) + 1]};
             if (m == 0) {   // DC and Nyquist are real
                 a.y = 0.0;
@@ -607,6 +642,21 @@ struct row_rz {
             const C2<double> e = cadd(a, bc), f = csub(a, bc);
             const C2<double> t = cmul(cconj(twn[m]), f);
             return {e.x - t.y, e.y + t.x};
+        };
+        // mixed radix: Z built in the row first (as in row_rc's c2r), the transform reads it plainly
+        constexpr bool PRE = PF > 1;
+        if constexpr (PRE) {
+            for (int m = tt; m <= H / 2; m += TT) {
+                const C2<double> zm = c2r_z(m);
+                const C2<double> zh = (m > 0 && 2 * m != H) ? c2r_z(H - m) : C2<double>{0, 0};
+                row[m] = zm;
+                if (m > 0 && 2 * m != H) row[H - m] = zh;
+            }
+            __syncthreads();
+        }
+        auto loadb = [&](int m) -> C2<double> {
+            if (PRE) return {rd[2 * m], rd[2 * m + 1]};
+            return c2r_z(m);
         };
         fft_pq_team<double, PF, Q, false, 0>(v, row, k1, j, loadb, tw, twn, 2);
         __syncthreads();

@@ -547,6 +547,8 @@ HYBRID_CASES = [
     ((48, 32, 16), [[4, 4], [0, 0], [1, 1]], 1, 2, 1),      # x UNB, rows of 96 (3 x 32)
     ((80, 32, 1), [[3, 3], [0, 1], [9, 9]], 1, 0, 2),       # 2-D, lda 2, rows of 80 (5 x 16)
     ((33, 17, 17), [[4, 4], [0, 0], [1, 1]], 0, 2, 1),      # node-centred x UNB
+    ((80, 16, 16), [[3, 3], [1, 1], [0, 0]], 1, 2, 1),      # x PER, rows of 80 (5 x 8): c2r built in LDS first
+    ((112, 16, 16), [[3, 3], [0, 0], [1, 1]], 1, 0, 1),     # x PER, rows of 112 (7 x 8)
 ]
 
 
@@ -556,7 +558,7 @@ HYBRID_CASES = [
                                                     "pf5-lda3", "pf7-448-tx4", "pf7-896-tx4", "pf5-2d-160",
                                                     "rz-eo-ee-unb", "rz-odd-per",
                                                     "rz-lda3", "rz-2d", "rz-node", "xrow-per", "xrow-unb-3x32",
-                                                    "xrow-2d-lda2", "xrow-node"])
+                                                    "xrow-2d-lda2", "xrow-node", "xrow-per-5x8", "xrow-per-7x8"])
 def test_hybrid_last_direction_vs_general_and_oracle(case, monkeypatch):
     """general passes + the last direction fused with the multiply, against the all-general path and the oracle"""
     nglob, bc, center, kernel, lda = case
@@ -716,6 +718,11 @@ ROW_CASES = [
     # node-centred periodic r2c along y or z (paired real columns with the periodic copy)
     ((17, 65, 33), [[0, 0], [3, 3], [3, 3]], 0, 2, 2),
     ((33, 17, 65), [[1, 1], [4, 4], [3, 3]], 0, 0, 2),
+    # mixed radix with one thread per team (Q = 8): the c2r and REDFT01 sequences built in LDS first
+    ((80, 16, 16), [[3, 3], [0, 0], [4, 4]], 1, 2, 3),
+    ((112, 16, 16), [[3, 3], [1, 1], [4, 4]], 1, 0, 3),
+    ((112, 16, 16), [[0, 0], [3, 3], [4, 4]], 1, 2, 1),
+    ((16, 80, 16), [[3, 3], [0, 0], [4, 4]], 1, 2, 4),
 ]
 
 
@@ -731,7 +738,8 @@ ROW_CASES = [
                                                  "node-oo-col-32", "node-oo-ee-64", "node-eu-64", "node-uo-64",
                                                  "node-ee-3x16-oo-3x8", "rowc2c-128", "rowc2c-3x32", "rowc2c-5x16",
                                                  "rowc2c-2d-7x16", "node-per-c2c-y", "node-per-c2c-z", "node-per-2d",
-                                                 "node-per-real-y", "node-per-real-z"])
+                                                 "node-per-real-y", "node-per-real-z", "rc-5x8", "rc-7x8",
+                                                 "r2r-ee-7x8", "col-ee-5x8"])
 @pytest.mark.parametrize("misalign", [False, True])
 def test_fast_pass_kernels_vs_generic_and_oracle(case, misalign, monkeypatch):
     """the fast kernels of the general path (real-to-real pencils and tiles, paired real columns,
