@@ -237,11 +237,16 @@ struct z_team {
         const int cl = min(c, a.nx - 1 - tile * TX);   // lanes past nx load the last valid column
         C2<S>* col = reinterpret_cast<C2<S>*>(a.dst) + (long long)bk.y * a.comp_src + (long long)y * a.s_other_src +
                      (long long)tile * TX;
+        // all eight loads (clamped rows, zero selected afterwards) before the LDS writes
+        C2<double> u[8];
     #pragma unroll
         for (int i = 0; i < 8; ++i) {
             const int r = tt + i * TT;
-            lds[r * TX + c] = r < a.nin ? c2d(col[(long long)r * a.s_col + cl]) : C2<double>{0.0, 0.0};
+            const C2<double> x = c2d(col[(long long)min(r, a.nin - 1) * a.s_col + cl]);
+            u[i] = r < a.nin ? x : C2<double>{0.0, 0.0};
         }
+    #pragma unroll
+        for (int i = 0; i < 8; ++i) lds[(tt + i * TT) * TX + c] = u[i];
         __syncthreads();
         const C2<double>* tw = reinterpret_cast<const C2<double>*>(a.tw);
         const C2<double>* twm = reinterpret_cast<const C2<double>*>(a.twm);

@@ -101,6 +101,24 @@ __device__ __forceinline__ C2<R> twl(const C2<R>* tw, int e) {
     return FWD ? w : cconj(w);
 }
 
+// Gather of n values by the threads t0, t0 + step, t0 + 2 step, ...: element q is get(q) and
+// is handed to put(q, value) (an LDS write). get must load from a valid address for every
+// q < n (clamp the index, select zero afterwards) so that its load is unconditional. The U
+// loads of a thread's next U elements are all issued before the first of their LDS writes:
+// a load behind a branch followed by its LDS write would wait for that one load, leaving a
+// single load of the thread in flight.
+template <int U, typename Get, typename Put>
+__device__ __forceinline__ void staged_gather(int t0, int step, int n, Get&& get, Put&& put) {
+    for (int q0 = t0; q0 < n; q0 += U * step) {
+        decltype(get(0)) v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) v[u] = get(min(q0 + u * step, n - 1));
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (q0 + u * step < n) put(q0 + u * step, v[u]);
+    }
+}
+
 // LDS slot of column element e: e*ES for interleaved columns (ES columns side by
 // side), or e + e/8 for a single contiguous column (ES = 0): the pad slot every 8
 // elements keeps the stride-8 writes of the first stage on distinct banks.

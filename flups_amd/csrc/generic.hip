@@ -18,6 +18,7 @@
 #include <algorithm>
 
 #include "core.hpp"
+#include "fft_core.hpp"
 #include "group.hpp"
 
 namespace flups {
@@ -147,21 +148,31 @@ struct generic_pass {
             const bool efast = P.src_s[d] == 1;
             const int nfi = P.nf_in;
             const int per = P.pen_in;   // elements per pencil
-            for (int i = threadIdx.x; i < npen * per; i += blockDim.x) {
-                const int pen = efast ? i / per : i % npen;
-                const int k = efast ? i - pen * per : i / npen;
+            // element i: pencil and position; nf_in is 1 (real) or 2 (complex)
+            auto at = [&](int i, int& pen, int& k) {
+                pen = efast ? i / per : i % npen;
+                k = efast ? i - pen * per : i / npen;
+            };
+            const int last = max(P.src_nd - 1, 0);
+            staged_gather<4>(threadIdx.x, blockDim.x, npen * per, [&](int i) -> cd {
+                int pen, k;
+                at(i, pen, k);
                 const int g = pen0 + pen;
                 const int ja = g % P.na, jb = g / P.na;
                 const int ks = k - P.gshift;
+                const long long idx = (long long)min(max(ks, 0), last) * P.src_s[d] + (long long)ja * P.src_s[P.fa] +
+                                      (long long)jb * P.src_s[P.fb];
+                const TI* sp = reinterpret_cast<const TI*>(P.src) + idx * P.src_nf;
+                const double x0 = (double)sp[0], x1 = nfi > 1 ? (double)sp[1] : 0.0;
+                const bool in = ks >= 0 && ks < P.src_nd;
+                return cd{in ? x0 : 0.0, in ? x1 : 0.0};
+            }, [&](int i, cd x) {
+                int pen, k;
+                at(i, pen, k);
                 double* o = pb + (size_t)pen * len + (size_t)k * nfi;
-                if (ks >= 0 && ks < P.src_nd) {
-                    const long long idx = (long long)ks * P.src_s[d] + (long long)ja * P.src_s[P.fa] + (long long)jb * P.src_s[P.fb];
-                    const TI* sp = reinterpret_cast<const TI*>(P.src) + idx * P.src_nf;
-                    for (int c = 0; c < nfi; ++c) o[c] = (double)sp[c];
-                } else {
-                    for (int c = 0; c < nfi; ++c) o[c] = 0.0;
-                }
-            }
+                o[0] = x.x;
+                if (nfi > 1) o[1] = x.y;
+            });
         }
         __syncthreads();
 

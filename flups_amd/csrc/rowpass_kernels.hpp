@@ -145,10 +145,12 @@ struct pen_r2r {
                 RD(2 * q + 1) = (double)p.y;
             }
         } else {
-            for (int k = tt; k < pin; k += TT) {
+            const int last = max(P.src_nd - 1, 0);
+            staged_gather<8>(tt, TT, pin, [&](int k) {
                 const int ks = k - P.gshift;
-                RD(k) = (ks >= 0 && ks < P.src_nd) ? (double)src[(long long)ks * sd] : 0.0;
-            }
+                const double x = (double)src[(long long)min(max(ks, 0), last) * sd];
+                return (ks >= 0 && ks < P.src_nd) ? x : 0.0;
+            }, [&](int k, double x) { RD(k) = x; });
         }
         __syncthreads();
         const int si = P.si;
@@ -328,10 +330,12 @@ struct row_rc {
         const C2<double>* twn = reinterpret_cast<const C2<double>*>(P.aux);  // n-point table
         // 1. gather: the pencil's doubles (reals for r2c, interleaved complex for c2r)
         const int nfi = P.nf_in, len = P.pen_in * nfi;
-        for (int q = tt; q < len; q += TT) {
+        const int last = max(P.src_nd - 1, 0);
+        staged_gather<8>(tt, TT, len, [&](int q) {
             const int k = q / nfi, ks = k - P.gshift;
-            rd[q] = (ks >= 0 && ks < P.src_nd) ? (double)src[(long long)ks * nfi + (q - k * nfi)] : 0.0;
-        }
+            const double x = (double)src[(long long)min(max(ks, 0), last) * nfi + (q - k * nfi)];
+            return (ks >= 0 && ks < P.src_nd) ? x : 0.0;
+        }, [&](int q, double x) { rd[q] = x; });
         __syncthreads();
         // element m of the packed complex sequence: z[m] = x[2m] + i x[2m+1] (r2c), or built from
         // X[m] and X[H-m] (c2r)
@@ -601,7 +605,10 @@ struct row_rz {
         double* rd = reinterpret_cast<double*>(row);
         const C2<double>* tw = reinterpret_cast<const C2<double>*>(A.tw);    // Q-point table
         const C2<double>* twn = reinterpret_cast<const C2<double>*>(A.twn);  // n-point table
-        for (int q = tt; q < 2 * H; q += TT) rd[q] = q < A.nin ? (double)d[q] : 0.0;
+        staged_gather<8>(tt, TT, 2 * H, [&](int q) {
+            const double x = (double)d[min(q, A.nin - 1)];
+            return q < A.nin ? x : 0.0;
+        }, [&](int q, double x) { rd[q] = x; });
         __syncthreads();
         C2<double> v[8];
         auto loadf = [&](int m) -> C2<double> { return {rd[2 * m], rd[2 * m + 1]}; };
@@ -697,8 +704,9 @@ struct col_c2c {
                 v[i] = {0.0, 0.0};
                 continue;
             }
-            const int ks = j + i * T - P.gshift;
-            v[i] = (ks >= 0 && ks < P.src_nd) ? c2d(src[ks * sd]) : C2<double>{0.0, 0.0};
+            const int ks = j + i * T - P.gshift;   // clamped address, zero selected: no branch
+            const C2<double> x = c2d(src[min(max(ks, 0), max(P.src_nd - 1, 0)) * sd]);
+            v[i] = (ks >= 0 && ks < P.src_nd) ? x : C2<double>{0.0, 0.0};
         }
         column_fft_pq<double, PF, Q, FWD, TX, HIN && PF == 1>(v, lds + c, j, reinterpret_cast<const C2<double>*>(P.tw),
                                                              reinterpret_cast<const C2<double>*>(P.aux), 1);
@@ -735,10 +743,12 @@ struct row_c2c {
         const C2<TI>* src = reinterpret_cast<const C2<TI>*>(P.src) + (long long)ja * P.src_s[P.fa] + (long long)jb * P.src_s[P.fb];
         C2<TO>* dst = reinterpret_cast<C2<TO>*>(P.dst) + (long long)ja * P.dst_s[P.fa] + (long long)jb * P.dst_s[P.fb];
         C2<double>* row = lds + rl * HP;
-        for (int k = tt; k < M; k += TT) {
+        const int last = max(P.src_nd - 1, 0);
+        staged_gather<8>(tt, TT, M, [&](int k) {
             const int ks = k - P.gshift;
-            row[k] = (ks >= 0 && ks < P.src_nd) ? c2d(src[ks]) : C2<double>{0.0, 0.0};
-        }
+            const C2<double> x = c2d(src[min(max(ks, 0), last)]);
+            return (ks >= 0 && ks < P.src_nd) ? x : C2<double>{0.0, 0.0};
+        }, [&](int k, C2<double> x) { row[k] = x; });
         __syncthreads();
         auto load = [&](int m) -> C2<double> { return row[m]; };
         C2<double> v[8];
@@ -777,11 +787,17 @@ struct col_c2c_team {
         const C2<TI>* src = reinterpret_cast<const C2<TI>*>(P.src) + (long long)ja * P.src_s[P.fa] + (long long)jb * P.src_s[P.fb];
         C2<TO>* dst = reinterpret_cast<C2<TO>*>(P.dst) + (long long)ja * P.dst_s[P.fa] + (long long)jb * P.dst_s[P.fb];
         const long long sd = P.src_s[P.d], dd = P.dst_s[P.d];
+        // all eight loads (clamped addresses, zero selected afterwards) before the LDS writes
+        const int last = max(P.src_nd - 1, 0);
+        C2<double> u[8];
     #pragma unroll
         for (int i = 0; i < 8; ++i) {
-            const int r = tt + i * TT, ks = r - P.gshift;
-            lds[r * TX + c] = (ks >= 0 && ks < P.src_nd) ? c2d(src[ks * sd]) : C2<double>{0.0, 0.0};
+            const int ks = tt + i * TT - P.gshift;
+            const C2<double> x = c2d(src[min(max(ks, 0), last) * sd]);
+            u[i] = (ks >= 0 && ks < P.src_nd) ? x : C2<double>{0.0, 0.0};
         }
+    #pragma unroll
+        for (int i = 0; i < 8; ++i) lds[(tt + i * TT) * TX + c] = u[i];
         __syncthreads();
         auto load = [&](int m) -> C2<double> { return lds[m * TX + c]; };
         C2<double> v[8];
