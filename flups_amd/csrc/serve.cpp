@@ -594,10 +594,6 @@ class Batch {
                 if (x < st) best = x;
             return best;
         };
-        const char* je = std::getenv("FLUPS_BATCH_JOIN");
-        const bool join = je && std::atoi(je) != 0;
-        std::vector<hipGraphNode_t> prev, cur;
-        int jstage = -1;
         for (auto& c : chunks) {
             const LaunchRec& L0 = recs[c.ent[0].first][c.ent[0].second];
             std::vector<hipGraphNode_t> deps;
@@ -635,19 +631,8 @@ class Batch {
             }
             kp.kernelParams = args;
             kp.extra = nullptr;
-            if (join) {   // every launch of a step after every launch of the step before
-                if (L0.stage != jstage) {
-                    hipGraphNode_t j;
-                    HIP_CHECK(hipGraphAddEmptyNode(&j, ggraph_, cur.data(), cur.size()));
-                    prev = {j};
-                    cur.clear();
-                    jstage = L0.stage;
-                }
-                deps = prev;
-            }
             hipGraphNode_t node;
             HIP_CHECK(hipGraphAddKernelNode(&node, ggraph_, deps.data(), deps.size(), &kp));
-            cur.push_back(node);
             ++nlaunch_grouped_;
             for (auto& e : c.ent) at[{e.first, L0.stage}].push_back(node);
         }
