@@ -31,6 +31,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
+#include <cstdlib>
 #include <map>
 #include <mutex>
 #include <string>
@@ -69,6 +70,7 @@ struct ColArgs {
     int gntx;            // z_fused / z_rot: x tiles of one ky' row of the compressed G
     int gtx;             // z_rot: tile width of the compressed G (a multiple of the kernel's TX)
     const void* twm;     // z_fused with PF > 1: M-point twiddle table (tw is then the M/PF-point one)
+    int crot;            // col_fft: 1 = component c is read from slot (c + 1) mod 3 (after z_rot)
 };
 
 // TX columns per block, M/8 threads per column; block = TX * M/8 threads.
@@ -91,7 +93,8 @@ struct col_fft {
         const int other = blk / ntx;
         const int x = tile * TX + c;
         const bool live = x < a.nx;
-        const C2<R>* src = reinterpret_cast<const C2<R>*>(a.src) + bk.y * a.comp_src + other * a.s_other_src + min(x, a.nx - 1);
+        const int cin = a.crot ? (bk.y + 1) % 3 : bk.y;   // source slot of this component
+        const C2<R>* src = reinterpret_cast<const C2<R>*>(a.src) + cin * a.comp_src + other * a.s_other_src + min(x, a.nx - 1);
         C2<R>* dst = reinterpret_cast<C2<R>*>(a.dst) + bk.y * a.comp_dst + other * a.s_other_dst + x;
         const C2<R>* tw = reinterpret_cast<const C2<R>*>(a.tw);
         constexpr bool HIN = HALF && FWD, HOUT = HALF && !FWD;
@@ -306,8 +309,10 @@ double rot_symbol(const RotK& k, int d, int idx) {
     }
 }
 
+// FP64: the three components' sets of NV values live together (all loads issued before the
+// first transform); in place, u_c back in slot c
 template <typename R, int M, int TX>
-__device__ __forceinline__ void z_rot_body(const ColArgs& a, const RotK& rk, C2<R>* lds, Blk bk) {
+__device__ __forceinline__ void z_rot_body3(const ColArgs& a, const RotK& rk, C2<R>* lds, Blk bk) {
     constexpr int NV = 8, T = M / NV;
     const int c = threadIdx.x % TX;
     const int j = threadIdx.x / TX;
@@ -385,6 +390,113 @@ __device__ __forceinline__ void z_rot_body(const ColArgs& a, const RotK& rk, C2<
         }
         if (!pair) break;   // uniform over the block
     }
+}
+
+// FP32: two sets of NV values live at a time (see the comment inside); u_c lands in slot
+// (c + 1) mod 3, and the inverse y pass reads it there (ColArgs::crot). For FP64 this order
+// (one component read and transformed twice) is slower than the three-set one: 32.5 against
+// 27.6 ms at 513^3 x 3, the fewer registers still leave one block of 512 threads per CU; FP32:
+// 17.6 against 18.7 ms (no spills at 128 registers, where the three-set order spilled 20)
+template <typename R, int M, int TX>
+__device__ __forceinline__ void z_rot_body2(const ColArgs& a, const RotK& rk, C2<R>* lds, Blk bk) {
+    constexpr int NV = 8, T = M / NV;
+    const int c = threadIdx.x % TX;
+    const int j = threadIdx.x / TX;
+    const int ntx = (a.nx + TX - 1) / TX;
+    const int blk = a.xcd ? xcd_group(bk.x, bk.nx) : bk.x;
+    const int tile = blk % ntx;
+    const int y0 = blk / ntx;
+    const int y1 = a.gmy - y0;
+    const bool pair = y0 != 0 && y1 != y0;
+    const int x = tile * TX + c;
+    const bool live = x < a.nx;
+    char* base = reinterpret_cast<char*>(a.dst) + (size_t)sizeof(C2<R>) * ((long long)tile * TX);
+    const size_t cstride = (size_t)sizeof(C2<R>) * a.comp_src;
+    const size_t rstride = (size_t)sizeof(C2<R>) * T * a.s_col;
+    const unsigned voff = (unsigned)(sizeof(C2<R>) * ((size_t)j * a.s_col + c));
+    // lanes past nx load the last valid column and store nothing
+    const unsigned loff = (unsigned)(sizeof(C2<R>) * ((size_t)j * a.s_col + min(c, a.nx - 1 - tile * TX)));
+    // the block's TX columns lie in one tile of the G layout (gtx a multiple of TX)
+    const int xg = a.x0 + tile * TX;
+    const int cg = xg % a.gtx + c;
+    const char* g = reinterpret_cast<const char*>(a.green) +
+                    sizeof(R) * ((long long)y0 * a.gntx + xg / a.gtx) * (M / 2 + 1) * a.gtx;
+    const C2<R>* tw = reinterpret_cast<const C2<R>*>(a.tw);
+    const R* sym = reinterpret_cast<const R*>(rk.sym);
+    const R kx = sym[rk.off[0] + a.x0 + min(x, a.nx - 1)];
+    for (int s = 0; s < 2; ++s) {
+        const int y = s == 0 ? y0 : y1;
+        char* b = base + (size_t)sizeof(C2<R>) * ((long long)y * a.s_other_src);
+        const R ky = sym[rk.off[1] + y];
+        // Two sets of NV values live at a time (u = i K x psi, psi = G w). The pass is in place and
+        // every w_c enters two components of u, so the components are written to the slot of an
+        // input this block already holds as psi: u_z to slot x, u_y to slot z, u_x to slot y (w_y is
+        // read and transformed twice); the inverse y pass reads component c from slot (c + 1) mod 3.
+        C2<R> p[NV], q[NV];
+        auto load = [&](C2<R>* u, int cc) {
+#pragma unroll
+            for (int i = 0; i < NV; ++i) {
+                const int row = j + i * T;
+                u[i] = row < a.nin ? *reinterpret_cast<const C2<R>*>(b + cc * cstride + i * rstride + loff) : C2<R>{0, 0};
+            }
+        };
+        auto store = [&](int cc, const C2<R>* u) {
+            if (!live) return;
+#pragma unroll
+            for (int i = 0; i < NV; ++i) {
+                const int row = j + i * T;
+                if (row < a.nout) *reinterpret_cast<C2<R>*>(b + cc * cstride + i * rstride + voff) = u[i];
+            }
+        };
+        auto psi = [&](C2<R>* u) {   // forward transform, times normfact G (re-read for each use: fewer live registers)
+            column_fft<R, M, true, TX>(u, lds + c, j, tw);
+#pragma unroll
+            for (int i = 0; i < NV; ++i) {
+                const int kz = j + i * T;
+                const R gk = *reinterpret_cast<const R*>(g + (unsigned)(sizeof(R) * (min(kz, M - kz) * a.gtx + cg)));
+                u[i] = {u[i].x * gk, u[i].y * gk};
+            }
+        };
+        auto kz_of = [&](int i) { return sym[rk.off[2] + j + i * T]; };
+        load(p, 0);
+        load(q, 1);
+        psi(p);   // psi_x
+        psi(q);   // psi_y
+#pragma unroll
+        for (int i = 0; i < NV; ++i) {   // u_z = i (kx psi_y - ky psi_x)
+            const C2<R> cz = {kx * q[i].x - ky * p[i].x, kx * q[i].y - ky * p[i].y};
+            q[i] = {-cz.y, cz.x};
+        }
+        column_fft<R, M, false, TX>(q, lds + c, j, tw);
+        store(0, q);
+        load(q, 2);
+        psi(q);   // psi_z
+#pragma unroll
+        for (int i = 0; i < NV; ++i) {   // u_y = i (kz psi_x - kx psi_z)
+            const R kz = kz_of(i);
+            const C2<R> cy = {kz * p[i].x - kx * q[i].x, kz * p[i].y - kx * q[i].y};
+            p[i] = {-cy.y, cy.x};
+        }
+        column_fft<R, M, false, TX>(p, lds + c, j, tw);
+        store(2, p);
+        load(p, 1);
+        psi(p);   // psi_y again
+#pragma unroll
+        for (int i = 0; i < NV; ++i) {   // u_x = i (ky psi_z - kz psi_y)
+            const R kz = kz_of(i);
+            const C2<R> cx = {ky * q[i].x - kz * p[i].x, ky * q[i].y - kz * p[i].y};
+            p[i] = {-cx.y, cx.x};
+        }
+        column_fft<R, M, false, TX>(p, lds + c, j, tw);
+        store(1, p);
+        if (!pair) break;   // uniform over the block
+    }
+}
+
+template <typename R, int M, int TX>
+__device__ __forceinline__ void z_rot_body(const ColArgs& a, const RotK& rk, C2<R>* lds, Blk bk) {
+    if constexpr (sizeof(R) == 4) z_rot_body2<R, M, TX>(a, rk, lds, bk);
+    else z_rot_body3<R, M, TX>(a, rk, lds, bk);
 }
 
 // at most 256 registers per lane (two waves per SIMD): the three components need 250-285
@@ -1013,6 +1125,7 @@ void run_fast(FastPlan* f, void* s0, void* s1, const double* rhs, double* field,
         // 4. y inverse: S1 -> columns [x0, x0 + w) of S0, keep N1
         ColArgs b = a;
         b.rev = (f->order >> 1) & 1;
+        b.crot = (type == 1 && sizeof(R) == 4) ? 1 : 0;   // FP32 z_rot leaves u_c in slot (c + 1) mod 3
         b.src = s1;
         b.dst = reinterpret_cast<char*>(s0) + es * x0;
         b.s_col = f->ldxs;
