@@ -197,6 +197,10 @@ def test_batch_graph_replays_match_eager():
         q[1] = blob_field(q[0].shape(), seed)
         P.append(tuple(q))
     R = rot_problems()
+    # a copy of the FP32 rotational problem of the fused path (grouped rotational z passes)
+    q = list(R[4])
+    q[1] = blob_field(q[0].shape(), 502)
+    R.append(tuple(q))
     alone = [solve_alone(*p) for p in P] + [solve_alone_rot(*p) for p in R]
     reqs = [p[0] for p in P] + [p[0] for p in R]
     rhs = [p[1] for p in P] + [p[1] for p in R]
