@@ -196,6 +196,15 @@ def test_batch_graph_replays_match_eager():
         q = list(P[k])
         q[1] = blob_field(q[0].shape(), seed)
         P.append(tuple(q))
+    # two FP32 problems of the general path (FP32 stage arrays; symmetric x, periodic y, unbounded z), grouped
+    ng, bc = (32, 48, 40), TEMPLATES["symx-pery"]
+    h = [1.0 / 48] * 3
+    L = [h[0] * m for m in ng]
+    for seed in (503, 504):
+        r = fl.ServeRequest.make(ng, bc, h, L, green=2, lda=1, fp32=True)
+        rhs = blob_field(r.shape(), seed)
+        rhs -= rhs.mean(axis=2, keepdims=True)
+        P.append((r, rhs, ng, bc, h, L, 2, 1, True))
     R = rot_problems()
     # a copy of the FP32 rotational problem of the fused path (grouped rotational z passes)
     q = list(R[4])
