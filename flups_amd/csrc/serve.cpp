@@ -377,13 +377,25 @@ class Server {
 class Batch {
    public:
     Batch(int n, const FLUPS_ServeRequest* reqs, const double* const* rhs, double* const* field) {
+        try {
+            init(n, reqs, rhs, field);
+        } catch (...) {   // a failed set-up releases what it made
+            release();
+            throw;
+        }
+    }
+    ~Batch() { release(); }
+
+   private:
+    void init(int n, const FLUPS_ServeRequest* reqs, const double* const* rhs, double* const* field) {
         FLUPS_REQUIRE(n >= 1, "a batch needs at least one problem");
         HIP_CHECK(hipStreamCreate(&main_));
         for (int i = 0; i < n; ++i) {
             validate(reqs[i]);
             FLUPS_REQUIRE(rhs[i] && field[i], "batch buffers must not be NULL");
             FLUPS_REQUIRE(is_device(rhs[i]) && is_device(field[i]), "batch problem %d: buffers must be device memory", i);
-            Item it;
+            items_.emplace_back();   // owned by the batch from here on (released on failure)
+            Item& it = items_.back();
             it.solver = make_solver(reqs[i]);
             HIP_CHECK(hipStreamCreate(&it.stream));
             it.solver->set_stream(it.stream);
@@ -393,7 +405,6 @@ class Batch {
             it.rhs = rhs[i];
             it.field = field[i];
             it.type = reqs[i].type;
-            items_.push_back(std::move(it));
         }
         HIP_CHECK(hipEventCreateWithFlags(&fork_, hipEventDisableTiming));
         HIP_CHECK(hipEventCreate(&t0_));
@@ -421,7 +432,14 @@ class Batch {
         HIP_CHECK(hipGraphInstantiate(&exec_, graph_, nullptr, nullptr, 0));
         build_grouped();
     }
-    ~Batch() {
+    void release() {
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        if (main_ && hipStreamIsCapturing(main_, &cs) == hipSuccess && cs == hipStreamCaptureStatusActive) {
+            hipGraph_t g = nullptr;   // a set-up that failed during the capture
+            (void)hipStreamEndCapture(main_, &g);
+            if (g) (void)hipGraphDestroy(g);
+        }
+        (void)hipGetLastError();
         if (gexec_) (void)hipGraphExecDestroy(gexec_);
         if (ggraph_) (void)hipGraphDestroy(ggraph_);
         if (dtab_) (void)hipFree(dtab_);
@@ -436,7 +454,15 @@ class Batch {
         if (t0_) (void)hipEventDestroy(t0_);
         if (t1_) (void)hipEventDestroy(t1_);
         if (main_) (void)hipStreamDestroy(main_);
+        gexec_ = exec_ = nullptr;
+        ggraph_ = graph_ = nullptr;
+        dtab_ = nullptr;
+        fork_ = t0_ = t1_ = nullptr;
+        main_ = nullptr;
+        items_.clear();
     }
+
+   public:
     // grouped launches (the default) or one graph branch per problem
     void run(bool grouped = true) {
         HIP_CHECK(hipEventRecord(t0_, main_));

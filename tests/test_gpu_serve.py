@@ -237,3 +237,20 @@ def test_batch_graph_replays_match_eager():
     for i, fa in enumerate(alone):
         assert np.array_equal(dev_out[i].numpy(), fa), ("sequential", i)
     b.close()
+
+
+def test_batch_rejects_host_buffers_and_recovers():
+    """a batch whose second problem has host buffers fails at set-up (the first problem's solver, stream and
+    event are released) and a valid batch built afterwards runs"""
+    P = problems()[:2]
+    dev_in = [fl.DeviceArray.from_numpy(P[0][1])]
+    dev_out = [fl.DeviceArray(P[0][1].shape)]
+    host_in, host_out = np.ascontiguousarray(P[1][1]), np.zeros_like(P[1][1])
+    fl.lib().flups_clear_last_error()
+    with pytest.raises(RuntimeError):
+        fl.Batch([P[0][0], P[1][0]], dev_in + [host_in], dev_out + [host_out])
+    assert "device memory" in fl.last_error()
+    b = fl.Batch([P[0][0]], dev_in, dev_out)
+    b.run()
+    assert np.array_equal(dev_out[0].numpy(), solve_alone(*P[0]))
+    b.close()
