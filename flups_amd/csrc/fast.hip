@@ -46,6 +46,29 @@ namespace flups {
 
 namespace {
 
+// Non-temporal accesses (the nt bit: the lines are not kept in the Infinity Cache) for data
+// that no later pass of the solve reads from the cache, when the arrays are too large for the
+// 256 MB cache anyway (the launch's `nt` flag, set for stage arrays of 128 MiB or more; smaller
+// problems keep every access cached, the next pass then reads what fits from the cache):
+//  * the caller's rows, read once by x_r2c and written once by x_c2r;
+//  * the stores of x_r2c into S0 and of y_fwd into S1: kept lines are written back as dirty
+//    victims while the next pass streams; stored non-temporally, y_fwd takes 143 instead of
+//    160 us and z_fused 328 instead of 343 us (FP32: 169 instead of 182 us) at 256^3.
+// The z pass's and y_inv's stores stay cached: y_inv and x_c2r read them back first (no gain
+// measured from non-temporal stores there).
+typedef double d2nt __attribute__((ext_vector_type(2)));
+typedef float f2nt __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ d2nt ld_nt(const double* p) { return __builtin_nontemporal_load(reinterpret_cast<const d2nt*>(p)); }
+__device__ __forceinline__ void st_nt(double* p, double a, double b) {
+    __builtin_nontemporal_store(d2nt{a, b}, reinterpret_cast<d2nt*>(p));
+}
+__device__ __forceinline__ void st_nt(C2<double>* p, C2<double> v) {
+    __builtin_nontemporal_store(d2nt{v.x, v.y}, reinterpret_cast<d2nt*>(p));
+}
+__device__ __forceinline__ void st_nt(C2<float>* p, C2<float> v) {
+    __builtin_nontemporal_store(f2nt{v.x, v.y}, reinterpret_cast<f2nt*>(p));
+}
+
 // ---------------------------------------------------------------------------
 // column passes along y or z of the stage arrays
 struct ColArgs {
@@ -71,6 +94,7 @@ struct ColArgs {
     int gtx;             // z_rot: tile width of the compressed G (a multiple of the kernel's TX)
     const void* twm;     // z_fused with PF > 1: M-point twiddle table (tw is then the M/PF-point one)
     int crot;            // col_fft: 1 = component c is read from slot (c + 1) mod 3 (after z_rot)
+    int nt;              // col_fft forward: 1 = non-temporal stores
 };
 
 // TX columns per block, M/8 threads per column; block = TX * M/8 threads.
@@ -111,7 +135,10 @@ struct col_fft {
     #pragma unroll
             for (int i = 0; i < 8; ++i) {
                 const int row = j + i * T;
-                if (HOUT ? i < 4 : (HIN || row < a.nout)) dst[row * a.s_col_dst] = v[i];
+                if (HOUT ? i < 4 : (HIN || row < a.nout)) {
+                    if (FWD && a.nt) st_nt(dst + row * a.s_col_dst, v[i]);   // y_fwd into a large S1
+                    else dst[row * a.s_col_dst] = v[i];
+                }
             }
         }
     }
@@ -535,6 +562,7 @@ struct RowArgs {
     int nx;              // complex outputs per row (H + 1)
     long long comp_src, comp_dst;
     int rev;             // 1: blocks walk the rows from the last one (reads what the previous pass wrote last)
+    int nt;              // 1: the caller's rows and the stores into S0 (x_r2c) are non-temporal
 };
 
 // MODE 0: any nvalid, scalar loads; 1: nvalid = H (zero-padded row: the upper half of the
@@ -569,8 +597,13 @@ struct x_r2c {
                 v[i] = {re, im};
             } else if (MODE == 2 || i < 4) {
                 // dead rows (past nrows) read row 0 of the plane and store nothing
-                const double2 p = *reinterpret_cast<const double2*>(src + e0);
-                v[i] = {(R)p.x, (R)p.y};
+                if (a.nt) {
+                    const d2nt p = ld_nt(src + e0);
+                    v[i] = {(R)p.x, (R)p.y};
+                } else {
+                    const double2 p = *reinterpret_cast<const double2*>(src + e0);
+                    v[i] = {(R)p.x, (R)p.y};
+                }
             } else {
                 v[i] = {0, 0};
             }
@@ -589,7 +622,9 @@ struct x_r2c {
                 const C2<R> e = {(zk.x + zc.x) * (R)0.5, (zk.y + zc.y) * (R)0.5};
                 const C2<R> df = {(zk.x - zc.x) * (R)0.5, (zk.y - zc.y) * (R)0.5};
                 const C2<R> o = {df.y, -df.x};
-                dst[k] = cadd(e, cmul(tw2[k], o));
+                const C2<R> r = cadd(e, cmul(tw2[k], o));
+                if (a.nt) st_nt(dst + k, r);
+                else dst[k] = r;
             }
             if (j == 0) {   // Nyquist: Re Z0 - Im Z0
                 const C2<R> z0 = col[0];
@@ -603,12 +638,14 @@ struct x_r2c {
 // x[2n] = Re z[n], x[2n+1] = Im z[n]; the imaginary parts of X[0] and X[H] are ignored.
 // MODE as for x_r2c: 1 stores the first H reals of the row (the upper half of the
 // inverse is not computed), 2 all 2H; both with 16-byte stores to aligned user rows.
+// Each thread loads its X[k] once; X[H - k] comes from the thread holding it through LDS
+// (loading both from memory doubled the load instructions: 256^3 FP64 73 -> 69 us).
 template <typename R, int H, int ROWS, int MODE>
 struct x_c2r {
     using Args = RowArgs;
     static constexpr int NT = ROWS * H / 8;
     static __device__ __forceinline__ void run(const RowArgs& a, Blk bk) {
-        constexpr int HP = H + H / 8;
+        constexpr int HP = H + H / 8 + 8;   // slots lslot(0 .. H): X[H] beside the FFT's column
         __shared__ C2<R> lds[HP * ROWS];
         constexpr int T = H / 8;
         const int rl = threadIdx.x / T;
@@ -620,22 +657,27 @@ struct x_c2r {
         const C2<R>* src = reinterpret_cast<const C2<R>*>(a.src) + bk.y * a.comp_src + y * a.s_row_src + z * a.s_z_src;
         double* dst = reinterpret_cast<double*>(a.dst) + bk.y * a.comp_dst + y * a.s_row_dst + z * a.s_z_dst;
         const C2<R>* tw2 = reinterpret_cast<const C2<R>*>(a.tw2);
-        C2<R> v[8];
+        C2<R>* col = lds + rl * HP;
+        C2<R> v[8], xs[8];
+    #pragma unroll
+        for (int i = 0; i < 8; ++i) xs[i] = live ? src[j + i * T] : C2<R>{0, 0};
+        const C2<R> xh = live ? src[H] : C2<R>{0, 0};
+    #pragma unroll
+        for (int i = 0; i < 8; ++i) col[lslot<0>(j + i * T)] = xs[i];
+        if (j == 0) col[lslot<0>(H)] = xh;
+        __syncthreads();
     #pragma unroll
         for (int i = 0; i < 8; ++i) {
             const int k = j + i * T;
-            C2<R> xa = {0, 0}, xb = {0, 0};
-            if (live) {
-                xa = src[k];
-                xb = src[H - k];
-            }
+            C2<R> xa = xs[i], xb = col[lslot<0>(H - k)];
             if (k == 0) { xa.y = 0; xb.y = 0; }
             const C2<R> bc = cconj(xb);
             const C2<R> e = cadd(xa, bc), f = csub(xa, bc);
             const C2<R> t = cmul(cconj(tw2[k]), f);
             v[i] = {e.x - t.y, e.y + t.x};
         }
-        column_fft<R, H, false, 0>(v, lds + rl * HP, j, reinterpret_cast<const C2<R>*>(a.tw));
+        __syncthreads();   // the pairs are read before the FFT writes the column
+        column_fft<R, H, false, 0>(v, col, j, reinterpret_cast<const C2<R>*>(a.tw));
         if (live) {
     #pragma unroll
             for (int i = 0; i < 8; ++i) {
@@ -644,7 +686,8 @@ struct x_c2r {
                     if (2 * n < a.nvalid) dst[2 * n] = (double)v[i].x;
                     if (2 * n + 1 < a.nvalid) dst[2 * n + 1] = (double)v[i].y;
                 } else if (MODE == 2 || i < 4) {
-                    *reinterpret_cast<double2*>(dst + 2 * n) = double2{(double)v[i].x, (double)v[i].y};
+                    if (a.nt) st_nt(dst + 2 * n, (double)v[i].x, (double)v[i].y);
+                    else *reinterpret_cast<double2*>(dst + 2 * n) = double2{(double)v[i].x, (double)v[i].y};
                 }
             }
         }
@@ -1036,9 +1079,14 @@ void run_fast(FastPlan* f, void* s0, void* s1, const double* rhs, double* field,
     const long long zs = f->zs;                                // z stride of S1
     const long long c0 = ldx * N1 * N2, c1 = zs * N2;          // component strides (complex)
     constexpr int TX = sizeof(R) == 8 ? 8 : 16;
+    // non-temporal accesses where the stage arrays exceed half the 256 MB Infinity Cache
+    constexpr size_t kNtBytes = (size_t)128 << 20;
+    const bool nt0 = (size_t)c0 * f->lda * sizeof(C2<R>) >= kNtBytes;
+    const bool nt1 = (size_t)zs * N2 * f->lda * sizeof(C2<R>) >= kNtBytes;
     // 1. x r2c: user (real) -> S0
     RowArgs r{};
     r.rev = (f->order >> 3) & 1;
+    r.nt = nt0;
     r.src = rhs;
     r.dst = s0;
     r.tw = f->tw[0];
@@ -1063,6 +1111,7 @@ void run_fast(FastPlan* f, void* s0, void* s1, const double* rhs, double* field,
         // 2. y forward: columns [x0, x0 + w) of S0 (ldx, N1, N2) -> S1 (ldxs, M1, N2)
         ColArgs a{};
         a.rev = f->order & 1;
+        a.nt = nt1;
         a.xcd = (f->ysched >> 4) & 1;
         a.src = reinterpret_cast<const char*>(s0) + es * x0;
         a.dst = s1;
@@ -1142,6 +1191,7 @@ void run_fast(FastPlan* f, void* s0, void* s1, const double* rhs, double* field,
     // 5. x c2r: S0 -> user (real), keep N0
     RowArgs q{};
     q.rev = (f->order >> 2) & 1;
+    q.nt = nt0;
     q.src = s0;
     q.dst = field;
     q.tw = f->tw[0];
