@@ -735,7 +735,7 @@ struct FastPlan {
     bool fp32;
     int zsched = 20;         // z pass schedule (see launch_col): half-width tiles, XCD-grouped
     int ysched = 0;          // y passes: bit 2 half-width tiles, bit 4 XCD-grouped
-    int order = 4;           // traversal: bit 0 y_fwd, 1 y_inv, 2 x_c2r, 3 x_r2c walk their batch backwards
+    int order = 12;          // traversal: bit 0 y_fwd, 1 y_inv, 2 x_c2r, 3 x_r2c walk their batch backwards
     int gtx = 0;             // x tile width of the compressed Green's layout
     RotK rot{};              // wave-number data of the rotational pass
     void* s0 = nullptr;     // (ldx, N1, N2) complex, FP64 or FP32
@@ -1620,7 +1620,7 @@ void fast_autotune(FastPlan* f, double* out, const double* in, const View& uin, 
     std::array<int, 3> pick = {f->zsched, f->ysched, f->order};
     for (int z : {20, 16, 4, 0})
         for (int y : {0, 16})
-            for (int o : {4, 0}) {
+            for (int o : {12, 4}) {
                 f->zsched = z;
                 f->ysched = y;
                 f->order = o;
