@@ -161,12 +161,20 @@ template <typename R, int M, int TX, int SCHED, bool HALF, int PF = 1, typename 
 struct z_fused {
     using Args = ColArgs;
     static constexpr int NT = TX * M / (8 * PF);
-    static constexpr int kWaves = PF == 3 ? 2 : 0;   // the mixed-radix columns: at most 256 registers per lane
+    // the mixed-radix columns: at most 256 registers per lane; the zero-padded FP32 columns up to
+    // 512 with the LDS twiddles (68 registers uncapped): 64, eight waves per SIMD, at most 16 bytes
+    // spilled (256^3: 2278 against 2254 solves/s uncapped); the other FP32 variants would spill
+    // up to 80 bytes under the cap and are left uncapped
+    static constexpr int kWaves = PF == 3 ? 2 : (sizeof(R) == 4 && PF == 1 && HALF && SCHED != 1 && M <= 512 ? 8 : 0);
     static __device__ __forceinline__ void run(const ColArgs& a, Blk bk) {
         // SCHED 0: G loaded with the first set, second set loaded after the first is stored;
         //       1: both sets prefetched up front; 2: G loaded just before each multiply.
         static_assert(PF == 1 || !HALF, "the zero-padded half is pruned for powers of two only");
         __shared__ C2<R> lds[M * TX];
+        // FP32 (bound by its VALU instructions): the twiddle table copied to LDS, seven LDS reads
+        // per radix-8 stage instead of three loads and four complex products
+        constexpr bool LTW = sizeof(R) == 4 && PF == 1;
+        __shared__ C2<R> ltw[LTW ? M : 1];
         constexpr int Q = M / PF, T = Q / 8, NV = 8 * PF;
         const int c = threadIdx.x % TX;
         const int j = threadIdx.x / TX;
@@ -218,15 +226,20 @@ struct z_fused {
         if (SCHED != 2) load_g();
         load(v, b0);
         if (SCHED == 1 && pair) load(w, b1);
+        if constexpr (LTW) {   // after the data loads are issued: the first FFT stage waits for them anyway
+            for (int i = threadIdx.x; i < M; i += NT) ltw[i] = tw[i];
+            __syncthreads();
+            tw = ltw;
+        }
         for (int s = 0; s < 2; ++s) {
-            column_fft_pq<R, PF, Q, true, TX, HALF>(v, lds + c, j, tw, twm, 1);
+            column_fft_pq<R, PF, Q, true, TX, HALF, LTW>(v, lds + c, j, tw, twm, 1);
             if (SCHED == 2) load_g();
     #pragma unroll
             for (int i = 0; i < NV; ++i) {
                 v[i].x *= gv[i];
                 v[i].y *= gv[i];
             }
-            column_fft_pq<R, PF, Q, false, TX>(v, lds + c, j, tw, twm, 1);   // syncs before its first LDS write
+            column_fft_pq<R, PF, Q, false, TX, false, LTW>(v, lds + c, j, tw, twm, 1);   // syncs before its first LDS write
             char* b = s == 0 ? b0 : b1;
             if (live) {
     #pragma unroll

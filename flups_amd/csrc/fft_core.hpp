@@ -130,7 +130,10 @@ __device__ __forceinline__ int lslot(int e) { return ES > 0 ? e * ES : e + (e >>
 // column (element e at lds[lslot<ES>(e)]); the whole block must call this together.
 // HIN: the upper half of the input is zero (v[4..7] not read). Outputs the caller
 // never uses (the upper half of a truncated inverse) are removed by the compiler.
-template <typename R, int M, bool FWD, int ES, bool HIN = false>
+// LTW: tw is a copy of the table in LDS, and each radix-8 stage reads its seven
+// twiddles w^q from it (seven LDS reads instead of three table loads and four
+// complex products: fewer VALU instructions for kernels bound by them).
+template <typename R, int M, bool FWD, int ES, bool HIN = false, bool LTW = false>
 __device__ __forceinline__ void column_fft(C2<R>* v, C2<R>* lds, int j, const C2<R>* tw) {
     constexpr int T = M / 8;
     // table entries w, w^2, w^4 of the next radix-8 stage, loaded one stage ahead
@@ -153,13 +156,17 @@ __device__ __forceinline__ void column_fft(C2<R>* v, C2<R>* lds, int j, const C2
             const int k = j % span;
             C2<R> n1 = {1, 0}, n2 = {1, 0}, n4 = {1, 0};
             const int nspan = span * 8;
-            if (nspan * 8 <= M) {
+            if (!LTW && nspan * 8 <= M) {
                 const int e = (j % nspan) * (M / (nspan * 8));
                 n1 = twl<R, FWD>(tw, e);
                 n2 = twl<R, FWD>(tw, 2 * e);
                 n4 = twl<R, FWD>(tw, 4 * e);
             }
-            if (span > 1) {
+            if (span > 1 && LTW) {
+                const int e = k * (M / (span * 8));   // q e < M
+#pragma unroll
+                for (int q = 1; q < 8; ++q) v[q] = cmul(v[q], twl<R, FWD>(tw, q * e));
+            } else if (span > 1) {
                 // w^q, q = 1..7, from w, w^2, w^4 and four products
                 C2<R> tw8[8];
                 tw8[1] = r1;
@@ -218,10 +225,11 @@ __device__ __forceinline__ void column_fft(C2<R>* v, C2<R>* lds, int j, const C2
 // Q-point column FFTs over n2 one after the other on the same LDS, and an LDS exchange back
 // to the natural order. twq: Q-point table; twh[e * hs] = exp(-2 pi i e / H). The whole block
 // calls this together; it ends with LDS reads (sync before the next LDS write).
-template <typename R, int P, int Q, bool FWD, int ES, bool HIN = false>
+template <typename R, int P, int Q, bool FWD, int ES, bool HIN = false, bool LTW = false>
 __device__ __forceinline__ void column_fft_pq(C2<R>* v, C2<R>* lds, int j, const C2<R>* twq, const C2<R>* twh, int hs) {
+    static_assert(P == 1 || !LTW, "LDS twiddles for powers of two only");
     if constexpr (P == 1) {
-        column_fft<R, Q, FWD, ES, HIN>(v, lds, j, twq);
+        column_fft<R, Q, FWD, ES, HIN, LTW>(v, lds, j, twq);
     } else {
         constexpr int T = Q / 8;
         C2<R> wp[P];   // w_P^m = w_H^(m Q)
