@@ -749,6 +749,7 @@ struct FastPlan {
     int zsched = 20;         // z pass schedule (see launch_col): half-width tiles, XCD-grouped
     int ysched = 0;          // y passes: bit 2 half-width tiles, bit 4 XCD-grouped
     int order = 12;          // traversal: bit 0 y_fwd, 1 y_inv, 2 x_c2r, 3 x_r2c walk their batch backwards
+    size_t nt_bytes = (size_t)128 << 20;   // stage arrays from this size on: non-temporal accesses (FLUPS_NT_MB)
     int gtx = 0;             // x tile width of the compressed Green's layout
     RotK rot{};              // wave-number data of the rotational pass
     void* s0 = nullptr;     // (ldx, N1, N2) complex, FP64 or FP32
@@ -847,6 +848,7 @@ FastPlan* make_fast_plan(SolverImpl& s) {
     if (const char* e = getenv("FLUPS_ZSCHED")) f->zsched = atoi(e);
     if (const char* e = getenv("FLUPS_ORDER")) f->order = atoi(e) & 15;
     if (const char* e = getenv("FLUPS_YSCHED")) f->ysched = atoi(e) & 20;
+    if (const char* e = getenv("FLUPS_NT_MB")) f->nt_bytes = (size_t)(atof(e) * (1 << 20));
     if (f->zsched < 0 || f->zsched > 63 || (f->zsched & 3) == 3 || ((f->zsched & 8) && (f->zsched & 7))) {
         delete f;
         fail("FLUPS_ZSCHED: bits 0-1 schedule (0, 1, 2), bit 2 half-width tiles, bit 3 quarter-width tiles (schedule 0), "
@@ -1093,9 +1095,8 @@ void run_fast(FastPlan* f, void* s0, void* s1, const double* rhs, double* field,
     const long long c0 = ldx * N1 * N2, c1 = zs * N2;          // component strides (complex)
     constexpr int TX = sizeof(R) == 8 ? 8 : 16;
     // non-temporal accesses where the stage arrays exceed half the 256 MB Infinity Cache
-    constexpr size_t kNtBytes = (size_t)128 << 20;
-    const bool nt0 = (size_t)c0 * f->lda * sizeof(C2<R>) >= kNtBytes;
-    const bool nt1 = (size_t)zs * N2 * f->lda * sizeof(C2<R>) >= kNtBytes;
+    const bool nt0 = (size_t)c0 * f->lda * sizeof(C2<R>) >= f->nt_bytes;
+    const bool nt1 = (size_t)zs * N2 * f->lda * sizeof(C2<R>) >= f->nt_bytes;
     // 1. x r2c: user (real) -> S0
     RowArgs r{};
     r.rev = (f->order >> 3) & 1;
