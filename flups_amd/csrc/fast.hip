@@ -438,7 +438,7 @@ __device__ __forceinline__ void z_rot_body3(const ColArgs& a, const RotK& rk, C2
 // 27.6 ms at 513^3 x 3, the fewer registers still leave one block of 512 threads per CU; FP32:
 // 17.6 against 18.7 ms (no spills at 128 registers, where the three-set order spilled 20)
 template <typename R, int M, int TX>
-__device__ __forceinline__ void z_rot_body2(const ColArgs& a, const RotK& rk, C2<R>* lds, Blk bk) {
+__device__ __forceinline__ void z_rot_body2(const ColArgs& a, const RotK& rk, C2<R>* lds, C2<R>* ltw, Blk bk) {
     constexpr int NV = 8, T = M / NV;
     const int c = threadIdx.x % TX;
     const int j = threadIdx.x / TX;
@@ -461,9 +461,13 @@ __device__ __forceinline__ void z_rot_body2(const ColArgs& a, const RotK& rk, C2
     const int cg = xg % a.gtx + c;
     const char* g = reinterpret_cast<const char*>(a.green) +
                     sizeof(R) * ((long long)y0 * a.gntx + xg / a.gtx) * (M / 2 + 1) * a.gtx;
-    const C2<R>* tw = reinterpret_cast<const C2<R>*>(a.tw);
     const R* sym = reinterpret_cast<const R*>(rk.sym);
     const R kx = sym[rk.off[0] + a.x0 + min(x, a.nx - 1)];
+    // the twiddle table copied to LDS: seven LDS reads per radix-8 stage instead of three loads and
+    // four complex products (the FP32 pass runs seven column FFTs per column set)
+    for (int i = threadIdx.x; i < M; i += TX * M / 8) ltw[i] = reinterpret_cast<const C2<R>*>(a.tw)[i];
+    __syncthreads();
+    const C2<R>* tw = ltw;
     for (int s = 0; s < 2; ++s) {
         const int y = s == 0 ? y0 : y1;
         char* b = base + (size_t)sizeof(C2<R>) * ((long long)y * a.s_other_src);
@@ -489,7 +493,7 @@ __device__ __forceinline__ void z_rot_body2(const ColArgs& a, const RotK& rk, C2
             }
         };
         auto psi = [&](C2<R>* u) {   // forward transform, times normfact G (re-read for each use: fewer live registers)
-            column_fft<R, M, true, TX>(u, lds + c, j, tw);
+            column_fft<R, M, true, TX, false, true>(u, lds + c, j, tw);
 #pragma unroll
             for (int i = 0; i < NV; ++i) {
                 const int kz = j + i * T;
@@ -507,7 +511,7 @@ __device__ __forceinline__ void z_rot_body2(const ColArgs& a, const RotK& rk, C2
             const C2<R> cz = {kx * q[i].x - ky * p[i].x, kx * q[i].y - ky * p[i].y};
             q[i] = {-cz.y, cz.x};
         }
-        column_fft<R, M, false, TX>(q, lds + c, j, tw);
+        column_fft<R, M, false, TX, false, true>(q, lds + c, j, tw);
         store(0, q);
         load(q, 2);
         psi(q);   // psi_z
@@ -517,7 +521,7 @@ __device__ __forceinline__ void z_rot_body2(const ColArgs& a, const RotK& rk, C2
             const C2<R> cy = {kz * p[i].x - kx * q[i].x, kz * p[i].y - kx * q[i].y};
             p[i] = {-cy.y, cy.x};
         }
-        column_fft<R, M, false, TX>(p, lds + c, j, tw);
+        column_fft<R, M, false, TX, false, true>(p, lds + c, j, tw);
         store(2, p);
         load(p, 1);
         psi(p);   // psi_y again
@@ -527,15 +531,15 @@ __device__ __forceinline__ void z_rot_body2(const ColArgs& a, const RotK& rk, C2
             const C2<R> cx = {ky * q[i].x - kz * p[i].x, ky * q[i].y - kz * p[i].y};
             p[i] = {-cx.y, cx.x};
         }
-        column_fft<R, M, false, TX>(p, lds + c, j, tw);
+        column_fft<R, M, false, TX, false, true>(p, lds + c, j, tw);
         store(1, p);
         if (!pair) break;   // uniform over the block
     }
 }
 
 template <typename R, int M, int TX>
-__device__ __forceinline__ void z_rot_body(const ColArgs& a, const RotK& rk, C2<R>* lds, Blk bk) {
-    if constexpr (sizeof(R) == 4) z_rot_body2<R, M, TX>(a, rk, lds, bk);
+__device__ __forceinline__ void z_rot_body(const ColArgs& a, const RotK& rk, C2<R>* lds, C2<R>* ltw, Blk bk) {
+    if constexpr (sizeof(R) == 4) z_rot_body2<R, M, TX>(a, rk, lds, ltw, bk);
     else z_rot_body3<R, M, TX>(a, rk, lds, bk);
 }
 
@@ -553,7 +557,8 @@ struct z_rot {
     static constexpr int kWaves = 2;
     static __device__ __forceinline__ void run(const ZRotArgs& z, Blk bk) {
         __shared__ C2<R> lds[M * TX];
-        z_rot_body<R, M, TX>(z.a, z.rk, lds, bk);
+        __shared__ C2<R> ltw[sizeof(R) == 4 ? M : 1];   // FP32: the twiddle table
+        z_rot_body<R, M, TX>(z.a, z.rk, lds, ltw, bk);
     }
 };
 
