@@ -107,6 +107,7 @@ struct PassDesc {
     int fastk = 0;          // fast kernels of rowpass.hip instead of the generic one: 1 row r2r, 2 paired real columns,
                             // 3 row r2c / c2r, 4 column r2r, 5 column c2c, 6 row c2c
     int vec = 0;            // fast kernels: pencils start 16-byte aligned in source and destination
+    int nt = 0;             // row r2r kernel: its vector accesses are non-temporal (both arrays 128 MiB or more)
     const double* aux = nullptr;   // fast kernels: pre / post twiddle table
 };
 

@@ -39,6 +39,26 @@ struct Vec2<float> {
 };
 template <typename T>
 using vec2_t = typename Vec2<T>::type;
+// non-temporal vector accesses (lines not kept in the Infinity Cache): for arrays read or
+// written once per solve that are too large for the cache anyway
+__device__ __forceinline__ double2 ld_nt2(const double2* p) {
+    typedef double v2 __attribute__((ext_vector_type(2)));
+    const v2 a = __builtin_nontemporal_load(reinterpret_cast<const v2*>(p));
+    return double2{a.x, a.y};
+}
+__device__ __forceinline__ float2 ld_nt2(const float2* p) {
+    typedef float v2 __attribute__((ext_vector_type(2)));
+    const v2 a = __builtin_nontemporal_load(reinterpret_cast<const v2*>(p));
+    return float2{a.x, a.y};
+}
+__device__ __forceinline__ void st_nt2(double2* p, double2 v) {
+    typedef double v2 __attribute__((ext_vector_type(2)));
+    __builtin_nontemporal_store(v2{v.x, v.y}, reinterpret_cast<v2*>(p));
+}
+__device__ __forceinline__ void st_nt2(float2* p, float2 v) {
+    typedef float v2 __attribute__((ext_vector_type(2)));
+    __builtin_nontemporal_store(v2{v.x, v.y}, reinterpret_cast<v2*>(p));
+}
 
 template <typename R>
 __device__ __forceinline__ C2<R> cadd(C2<R> a, C2<R> b) { return {a.x + b.x, a.y + b.y}; }

@@ -140,7 +140,8 @@ struct pen_r2r {
     #pragma unroll
             for (int t = 0; t < 8; ++t) {
                 const int q = tt + t * TT;
-                const vec2_t<TI> p = reinterpret_cast<const vec2_t<TI>*>(src)[q];
+                const vec2_t<TI>* a = reinterpret_cast<const vec2_t<TI>*>(src) + q;
+                const vec2_t<TI> p = P.nt ? ld_nt2(a) : *a;
                 RD(2 * q) = (double)p.x;
                 RD(2 * q + 1) = (double)p.y;
             }
@@ -296,7 +297,9 @@ struct pen_r2r {
     #pragma unroll
             for (int t = 0; t < 8; ++t) {
                 const int q = tt + t * TT;
-                reinterpret_cast<vec2_t<TO>*>(dst)[q] = vec2_t<TO>{(TO)val(2 * q), (TO)val(2 * q + 1)};
+                const vec2_t<TO> o{(TO)val(2 * q), (TO)val(2 * q + 1)};
+                if (P.nt) st_nt2(reinterpret_cast<vec2_t<TO>*>(dst) + q, o);
+                else reinterpret_cast<vec2_t<TO>*>(dst)[q] = o;
             }
         } else {
             for (int k = tt; k < P.dst_nd; k += TT) {

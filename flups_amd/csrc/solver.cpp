@@ -328,6 +328,14 @@ void SolverImpl::plan_fft(PassDesc& P, int M) {
     P.tw = twiddles(M);
 }
 
+// Arrays larger than half the 256 MB Infinity Cache are read or written with non-temporal
+// accesses where that was measured to pay: the row r2r pass between the caller's rows and a
+// stage array (256^3, x EVEN-ODD / y PER / z UNB: 2828 -> 2849 solves/s). Non-temporal stores of
+// the forward paired-column pass into the spectral array the fused z pass reads next measured
+// slower (2800 -> 2694-2728) and are not used.
+static constexpr double kNtBytes = 134217728.0;
+static double view_bytes(const View& v) { return (double)v.es * v.nf * v.n[0] * v.n[1] * v.n[2]; }
+
 PassDesc SolverImpl::pass_desc(const DirPlan& p, int comp, bool forward, const View& src, const View& dst, const Shape& tin,
                                const Shape& tout, int gshift, int sshift) {
     PassDesc P;
@@ -398,6 +406,7 @@ PassDesc SolverImpl::pass_desc(const DirPlan& p, int comp, bool forward, const V
         // pairs of reals as one vector access: 16-byte (double) or 8-byte (float) aligned
         P.vec = plain && ((uintptr_t)P.src % (2 * src.es) == 0) && ((uintptr_t)P.dst % (2 * dst.es) == 0) && P.src_s[fa] % 2 == 0 &&
                 P.src_s[fb] % 2 == 0 && P.dst_s[fa] % 2 == 0 && P.dst_s[fb] % 2 == 0;
+        P.nt = P.vec && view_bytes(src) >= kNtBytes && view_bytes(dst) >= kNtBytes;
         return P;
     }
     // strided real-to-real pencils, neighbours along a unit-stride dimension: the same transforms
