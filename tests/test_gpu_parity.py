@@ -425,6 +425,36 @@ def test_schedules_are_bitwise_equal_and_autotune(monkeypatch, nglob):
     assert e2 <= TOL and ei <= TOL
 
 
+@pytest.mark.parametrize("fp32", [False, True], ids=["fp64", "fp32"])
+def test_nontemporal_accesses_are_bitwise_neutral(monkeypatch, fp32):
+    """the fused pipeline's non-temporal accesses (stage arrays from FLUPS_NT_MB MiB on) change the
+    cache policy only: with every array non-temporal (threshold 0) the result is bitwise the one
+    with none (a threshold above the problem), on aligned user rows (16-byte row modes)"""
+    nglob = (64, 32, 128)
+    rng = np.random.default_rng(11)
+    rhs = rng.normal(size=(1, nglob[2], nglob[1], nglob[0]))
+    h = [1.0 / max(nglob)] * 3
+    L = [h[0] * n for n in nglob]
+    out = {}
+    for mb in ("0", "100000"):
+        monkeypatch.setenv("FLUPS_NT_MB", mb)
+        topo = fl.Topology(0, 1, list(nglob), [1, 1, 1], False)
+        s = fl.Solver(topo, [[4, 4]] * 3, h, L)
+        s.set_green_type(2)
+        if fp32:
+            s.set_precision(1)
+        s.setup()
+        assert s.path == (2 if fp32 else 1)
+        f = np.zeros_like(rhs)
+        s.solve(f, rhs)
+        out[mb] = f
+    assert np.array_equal(out["0"], out["100000"])
+    f_ora = orc.solve(list(nglob), [[4, 4]] * 3, 1, h, L, 1, 2, rhs)
+    e2, ei = rel_err(out["0"], f_ora)
+    tol = FP32_TOL if fp32 else TOL
+    assert e2 <= tol and ei <= tol, (e2, ei)
+
+
 @pytest.mark.parametrize("case", [((64, 32, 32), 1, 1), ((64, 32, 64), 1, 2), ((65, 33, 33), 0, 4), ((64, 64, 32), 1, 6)],
                          ids=["64x32x32-spe", "64x32x64-fd2", "node65x33x33-fd4", "64x64x32-fd6"])
 def test_fused_rot_vs_oracle(case):
