@@ -214,6 +214,9 @@ def rhs_ring(n, seed, h):
     return out
 
 
+SETTLE_MS = 150.0   # untimed back-to-back work before every timed block (--settle-ms; see settle())
+
+
 def run_config(fl, name, nglob, bc, center, kernel, lda, typ, odiff, rhs, h, L, steps, warmup, fp32=False, ref=None):
     """time one workload; with `ref` (the FP64 result on the device) also its relative L2 / Linf error against it"""
     topo = fl.Topology(0, lda, list(nglob), [1, 1, 1], False)
@@ -231,6 +234,7 @@ def run_config(fl, name, nglob, bc, center, kernel, lda, typ, odiff, rhs, h, L, 
     for _ in range(warmup):
         s.solve(dout, din, typ)
     stream.synchronize()
+    nsettle = settle(lambda: s.solve(dout, din, typ), stream, SETTLE_MS)
     t0 = time.perf_counter()
     for _ in range(steps):
         s.solve(dout, din, typ)
@@ -242,7 +246,7 @@ def run_config(fl, name, nglob, bc, center, kernel, lda, typ, odiff, rhs, h, L, 
            "algorithmic_GB": round(by / 1e9, 3), "achieved_GBs": round(by / el / 1e9, 1),
            "roofline_frac": round(by / el / PEAK_HBM, 4), "path": ["general", "fused-fp64", "fused-fp32", "general+fused-z", "general-fp32",
                     "general-fp32+fused-z"][s.path],
-           "setup_s": round(setup_s, 2), "steps": steps}
+           "setup_s": round(setup_s, 2), "steps": steps, "settle_solves": nsettle}
     if ref is not None:
         a, b = dout.numpy(), ref.numpy()
         out["rel_l2_vs_fp64"] = float(np.sqrt(np.sum((a - b) ** 2) / np.sum(b * b)))
@@ -330,11 +334,19 @@ def batch_config(fl, nprob, seed, reps, same=False):
         b.run()
         b.run_branches()
         b.run_sequential()
+
+    def settle_batch(fn):   # as settle(): untimed back-to-back batches for at least SETTLE_MS
+        t = time.perf_counter()
+        while (time.perf_counter() - t) * 1e3 < SETTLE_MS:
+            fn()
+
+    settle_batch(b.run)
     t0 = time.perf_counter()
     ev = []
     for _ in range(reps):
         ev.append(b.run())
     graph = (time.perf_counter() - t0) / reps
+    settle_batch(b.run_branches)
     t0 = time.perf_counter()
     evb = []
     for _ in range(reps):
@@ -408,6 +420,8 @@ def main():
     ap.add_argument("--copy-calib-mb", type=int, default=0,
                     help="also run one plain device-to-device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
+    global SETTLE_MS
+    SETTLE_MS = args.settle_ms
     if args.gpus != 1:
         print("bench.py runs on one GPU only (this build has no multi-GPU path); got --gpus %d" % args.gpus,
               file=sys.stderr)
